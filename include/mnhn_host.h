@@ -1,0 +1,107 @@
+/*
+ * mnhn_host.h -- host-side C API of the MI355X DBSCAN path.
+ *
+ * The types and entry points keep the reference's names, layouts and
+ * argument meaning so that a caller of the reference's GPU build can switch
+ * by replacing `opencl_stuff` with `mnhn_gpu_stuff`:
+ *   dataset / dataset_from_fasta          dataset.h:27-35, :74; dataset.c:264-344
+ *   cluster / split_set / connections     cluster.h:10-31
+ *   store_split_set / read_split_set      cluster.h:140,146; cluster_io.c:595-669
+ *   create_cluster_files                  cluster.h:67; cluster_io.c:672-720
+ *   data_not_in_clusters                  cluster_io.c:564-593
+ *   generate_split_set_relation           cluster.h:131; cluster_io.c:407-445
+ *   intersection_of_clusters              cluster_io.c:447-501
+ *   opencl_initialization/destroy       -> mnhn_gpu_initialization/destroy
+ *                                          (dbscan.h:32-37; dbscan.c:97-298)
+ *   dbscan_SW_GPU                         dbscan.h:36-37; dbscan.c:950-1047
+ *   adaptive_dbscan                       dbscan.h:38-48; dbscan.c:1117-1380
+ * The Smith-Waterman distances all come from the gfx950 engine
+ * (mnhn_sw.h); there is no CPU distance path in this library.
+ */
+#ifndef MNHN_HOST_H
+#define MNHN_HOST_H
+
+#include <stddef.h>
+#include <stdio.h>
+
+#include "mnhn_sw.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* dataset.h:27-35 (same field order) */
+typedef struct {
+  char **sequences;
+  size_t *sequence_lengths;
+  void **binary_sequences;
+  int n_values;
+  float **values;
+  int n_dimensions;
+  size_t max_sequence_length;
+} dataset;
+
+/* cluster.h:10-23 */
+typedef struct {
+  int *members;
+  int n_members;
+  int id;
+} cluster;
+
+typedef struct {
+  int n_clusters;
+  cluster *clusters;
+} split_set;
+
+/* cluster.h:27-31 */
+typedef struct {
+  int *connections;
+  int n_connections;
+} cluster_connections;
+
+/* Device context of a dataset: the engine plus the cached epsilon-neighbour
+ * lists of the dataset.  Passed by value like the reference's opencl_stuff;
+ * copies share the same engine and cache. */
+typedef struct mnhn_eps_cache mnhn_eps_cache;
+typedef struct {
+  mnhn_sw_ctx *sw;
+  mnhn_eps_cache *cache;
+  int num_devs;
+} mnhn_gpu_stuff;
+
+/* ---- dataset ---- */
+dataset dataset_from_fasta(FILE *in);
+void free_sequences_from_dataset(dataset ds);
+
+/* ---- split sets / clusters ---- */
+void store_split_set(char *filename, split_set s);
+split_set read_split_set(char *filename);
+void create_single_cluster_file(char *filename, cluster cl, dataset ds);
+void create_cluster_files(char *prefix, split_set s, dataset ds);
+cluster data_not_in_clusters(split_set s, dataset ds);
+cluster intersection_of_clusters(cluster a, cluster b);
+cluster_connections *generate_split_set_relation(split_set ancient, split_set new_set);
+void free_split_set_and_associated_clusters(split_set s);
+
+/* ---- device context (replaces the OpenCL set-up) ----
+ * On failure (no GPU, HIP error) these print the reason and _exit(1), the
+ * reference's error convention (dbscan.c:149-152). */
+mnhn_gpu_stuff mnhn_gpu_initialization(dataset ds);
+void mnhn_gpu_destroy(mnhn_gpu_stuff g);
+
+/* epsilon-neighbour lists are computed once up to eps + lookahead and reused
+ * by every later pass with a smaller or equal epsilon. */
+void mnhn_gpu_set_lookahead(mnhn_gpu_stuff g, float lookahead);
+
+/* ---- clustering ---- */
+split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g);
+
+void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
+                     dataset ds, float epsilon_start, float epsilon_inc, int minpts,
+                     char *split_files_prefix, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MNHN_HOST_H */

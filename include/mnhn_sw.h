@@ -1,0 +1,127 @@
+/*
+ * mnhn_sw.h -- C ABI of the MI355X (gfx950) Smith-Waterman distance engine.
+ *
+ * Plain C types only (pointers, sizes, int return codes); every entry point is
+ * extern "C" and never throws.  Output buffers are owned by the caller unless
+ * stated otherwise.  One context is used by one host thread at a time.
+ *
+ * Distance definition (identical to the reference's score(),
+ * smith-waterman.c:16-81): with match +4, mismatch -4 and linear gap 3,
+ *   d(A, B) = 4*max(la, lb) - max_{i in [1,la], j in [1,lb]} H(i, j),
+ * where H is the local-alignment matrix comparing A[i] with B[j], index 0 of
+ * each sequence is never compared, and A[la] / B[lb] are string terminators
+ * that match each other and nothing else.
+ */
+#ifndef MNHN_SW_H
+#define MNHN_SW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define MNHN_OK 0
+#define MNHN_ERR_ARG 1       /* invalid argument (NULL, index out of range, ...) */
+#define MNHN_ERR_DEVICE 2    /* HIP runtime / kernel launch error */
+#define MNHN_ERR_NOMEM 3     /* device or host allocation failed */
+#define MNHN_ERR_NODEVICE 4  /* no usable GPU */
+#define MNHN_ERR_RANGE 5     /* a score would not fit the 16-bit lane format */
+
+/* Scoring parameters.  The reference hard-codes match 4 / mismatch -4 / gap 3
+ * (smith-waterman.c:4-14); NULL selects those.  Requires match > 0,
+ * mismatch <= 0, gap >= 0. */
+typedef struct {
+  int32_t match;
+  int32_t mismatch;
+  int32_t gap;
+} mnhn_sw_params;
+
+/* Opaque engine context: sequences uploaded once to device memory (2-bit
+ * packed when the residues at index >= 1 use at most four distinct bytes,
+ * 16-bit codes otherwise), sorted by length into groups of 128 targets. */
+typedef struct mnhn_sw_ctx mnhn_sw_ctx;
+
+/* epsilon-neighbour lists in CSR form (library-owned; free with
+ * mnhn_csr_free).  Row i lists every j != i with d(i, j) <= eps_max, in
+ * ascending j, with its distance. */
+typedef struct {
+  int32_t n_rows;
+  int32_t eps_max;
+  int64_t nnz;
+  int64_t *row_ptr; /* [n_rows + 1] */
+  int32_t *col;     /* [nnz] */
+  int16_t *dist;    /* [nnz] */
+} mnhn_csr;
+
+/* Timing of the last engine call (HIP events on the engine's stream). */
+typedef struct {
+  double kernel_ms;   /* Smith-Waterman kernel(s) only */
+  double total_ms;    /* kernel + epsilon-list compaction/sort on device */
+  double cells;       /* algorithmic cells: sum of la*lb over computed pairs */
+  double pairs;       /* pairs computed */
+  int32_t lane_bits;  /* 16 (packed) or 32 */
+  int32_t strip;      /* register strip width used */
+  int32_t waves;      /* persistent waves launched */
+  int32_t items;      /* work items in the queue */
+} mnhn_sw_timing;
+
+/* Upload n sequences (bytes as read from FASTA; not necessarily
+ * NUL-terminated) and build the context.  Replaces the OpenCL set-up of the
+ * reference's GPU build: opencl_initialization(dataset), dbscan.c:97-284. */
+int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
+                   const mnhn_sw_params *params, mnhn_sw_ctx **out);
+
+/* Release device memory and the context.  Replaces opencl_destroy,
+ * dbscan.c:286-298. */
+void mnhn_sw_destroy(mnhn_sw_ctx *ctx);
+
+/* dist_out[j] = d(seq[j], seq[query]) for all j in [0, n).  One launch; the
+ * drop-in for the per-query kernel launch + readback of the reference's GPU
+ * region_query (dbscan.c:471-513, smith-waterman-gpu.cl:13-98). */
+int mnhn_sw_one_vs_all(mnhn_sw_ctx *ctx, int query, int32_t *dist_out);
+
+/* dist_out[k] = d(seq[ij[2k]], seq[ij[2k+1]]) for k in [0, m).  Replaces the
+ * loops of score() calls in the reference's CPU region_query
+ * (dbscan.c:546-555) and other batch callers of score() (smith-waterman.h:1). */
+int mnhn_sw_pairs(mnhn_sw_ctx *ctx, const int32_t *ij, int64_t m,
+                  int32_t *dist_out);
+
+/* Full n x n distance matrix (row-major, host buffer of n*n int32).  Every
+ * unordered pair is computed once and mirrored; the diagonal is 0.  For
+ * moderate n (tests, small data sets). */
+int mnhn_sw_matrix(mnhn_sw_ctx *ctx, int32_t *dist_out);
+
+/* All-pairs epsilon-neighbour lists with d <= eps_max, computed in one
+ * persistent launch over unique pairs and compacted on the device; no n x n
+ * matrix is stored.  Feeds every region_query of a DBSCAN pass (and of an
+ * adaptive epsilon sweep up to eps_max) -- the reference recomputes n
+ * distances per region_query (dbscan.c:471-561). */
+int mnhn_sw_eps_csr(mnhn_sw_ctx *ctx, int32_t eps_max, mnhn_csr *out);
+
+/* Same computation as mnhn_sw_eps_csr but keeps the lists on the device and
+ * returns only their count (for timing the device path alone). */
+int mnhn_sw_eps_count(mnhn_sw_ctx *ctx, int32_t eps_max, int64_t *nnz_out);
+
+void mnhn_csr_free(mnhn_csr *csr);
+
+/* Number of sequences, alphabet mode (4 = 2-bit residues, 0 = 16-bit codes). */
+int mnhn_sw_size(const mnhn_sw_ctx *ctx);
+int mnhn_sw_alphabet(const mnhn_sw_ctx *ctx);
+
+/* Timing of the last one_vs_all / pairs / matrix / eps call. */
+int mnhn_sw_last_timing(const mnhn_sw_ctx *ctx, mnhn_sw_timing *out);
+
+/* Force the register-strip width (16 or 32) and waves per CU used by later
+ * calls; 0 restores the tuned defaults. */
+int mnhn_sw_set_tuning(mnhn_sw_ctx *ctx, int strip, int waves_per_cu);
+
+const char *mnhn_sw_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MNHN_SW_H */

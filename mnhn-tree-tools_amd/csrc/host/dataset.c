@@ -1,0 +1,98 @@
+/*
+ * dataset.c -- FASTA reader with the reference's parsing rules
+ * (dataset_from_fasta, dataset.c:264-344 of the reference):
+ *   - a line whose first byte is '>' opens a new record (its text is ignored);
+ *   - any other line contributes its first whitespace-delimited token,
+ *     upper-cased (toupper) and appended to the current record;
+ *   - a line without a token appends the previous token again, because the
+ *     reference's sscanf("%s") leaves its line buffer unchanged;
+ *   - every sequence is NUL-terminated; non-ACGT bytes are kept as they are.
+ * Unlike the reference, tokens are not limited to 1,999 bytes, and text
+ * before the first header is ignored (the reference writes it out of bounds).
+ */
+#define _GNU_SOURCE
+#include <ctype.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mnhn_host.h"
+
+static void *xrealloc(void *p, size_t n) {
+  void *q = realloc(p, n ? n : 1);
+  if (!q) {
+    printf("out of memory reading FASTA\n");
+    exit(1);
+  }
+  return q;
+}
+
+dataset dataset_from_fasta(FILE *in) {
+  dataset ds;
+  memset(&ds, 0, sizeof(ds));
+  char *line = NULL;
+  size_t cap = 0;
+  ssize_t got;
+  int records = 0;
+
+  rewind(in);
+  while ((got = getline(&line, &cap, in)) != -1)
+    if (line[0] == '>') records++;
+  rewind(in);
+
+  ds.n_values = records;
+  ds.sequences = (char **)xrealloc(NULL, sizeof(char *) * (size_t)records);
+  ds.sequence_lengths = (size_t *)xrealloc(NULL, sizeof(size_t) * (size_t)records);
+  size_t *caps = (size_t *)xrealloc(NULL, sizeof(size_t) * (size_t)records);
+
+  char *tok = NULL; /* last token seen (sscanf's line buffer) */
+  size_t tok_len = 0, tok_cap = 0;
+  int cur = -1;
+  while ((got = getline(&line, &cap, in)) != -1) {
+    if (line[0] == '>') {
+      cur++;
+      caps[cur] = 1024;
+      ds.sequences[cur] = (char *)xrealloc(NULL, caps[cur]);
+      ds.sequences[cur][0] = 0;
+      ds.sequence_lengths[cur] = 0;
+      continue;
+    }
+    /* first token of the line, as sscanf("%s") would read it */
+    size_t k = 0;
+    while (k < (size_t)got && line[k] != 0 && isspace((unsigned char)line[k])) k++;
+    if (k < (size_t)got && line[k] != 0) {
+      size_t e = k;
+      while (e < (size_t)got && line[e] != 0 && !isspace((unsigned char)line[e])) e++;
+      tok_len = e - k;
+      if (tok_len + 1 > tok_cap) {
+        tok_cap = 2 * (tok_len + 1);
+        tok = (char *)xrealloc(tok, tok_cap);
+      }
+      for (size_t t = 0; t < tok_len; t++) tok[t] = (char)toupper((unsigned char)line[k + t]);
+    }
+    if (cur < 0 || tok_len == 0) continue;
+    size_t L = ds.sequence_lengths[cur];
+    if (L + tok_len + 1 > caps[cur]) {
+      while (L + tok_len + 1 > caps[cur]) caps[cur] *= 2;
+      ds.sequences[cur] = (char *)xrealloc(ds.sequences[cur], caps[cur]);
+    }
+    memcpy(ds.sequences[cur] + L, tok, tok_len);
+    ds.sequences[cur][L + tok_len] = 0;
+    ds.sequence_lengths[cur] = L + tok_len;
+  }
+  for (int i = 0; i < records; i++) {
+    ds.sequences[i] = (char *)xrealloc(ds.sequences[i], ds.sequence_lengths[i] + 1);
+    if (ds.max_sequence_length < ds.sequence_lengths[i])
+      ds.max_sequence_length = ds.sequence_lengths[i];
+  }
+  free(caps);
+  free(tok);
+  free(line);
+  return ds;
+}
+
+void free_sequences_from_dataset(dataset ds) {
+  for (int i = 0; i < ds.n_values; i++) free(ds.sequences[i]);
+  free(ds.sequence_lengths);
+  free(ds.sequences);
+}

@@ -1,0 +1,313 @@
+/*
+ * dbscan.c -- DBSCAN and the adaptive epsilon sweep on top of the gfx950
+ * Smith-Waterman engine.
+ *
+ * Semantics follow the reference's dbscan.c:
+ *   region_query (:471-561): ascending list of every j with (float)d <= eps,
+ *     the point itself included (d(p, p) = 0);
+ *   dbscan_SW_GPU (:950-1047): points in index order; an unvisited point seeds
+ *     a cluster when minpts < |N(p)| (strict); cluster ids are 0, 1, ... in seed
+ *     order; eps == -1 becomes 0.015 and minpts == -1 becomes 5 with the same
+ *     warnings on stdout;
+ *   expand_cluster (:783-948): breadth-first queue that starts with N(p) in
+ *     ascending order; an unvisited queue element q is marked visited and, when
+ *     |N(q)| >= minpts, the members of N(q) not yet in the union of merged
+ *     neighbourhoods are appended in ascending order; every queue element not
+ *     yet in a cluster joins this one, in queue order.  (The union is kept as a
+ *     membership stamp instead of the reference's sorted-merge arrays; the
+ *     queue order is the same.)
+ *   adaptive_dbscan (:1117-1380) with one scanning thread: the stdout report,
+ *     the halving retries, float32 epsilon accumulation, the strictly
+ *     decreasing layer filter, the relation side effect and %04d split files.
+ *
+ * Where the distances come from: every region query of a pass reads the
+ * epsilon-neighbour lists (CSR) that the engine computes once for all unique
+ * pairs up to eps + lookahead (mnhn_sw_eps_csr); later passes with a smaller
+ * or equal epsilon reuse them.  With MNHN_REGION_QUERY=launch each region
+ * query instead launches the one-vs-all kernel, as the reference's GPU build
+ * launches its kernel once per query.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "mnhn_host.h"
+
+struct mnhn_eps_cache {
+  mnhn_csr csr;
+  int have;
+  float lookahead;
+  int per_query_launch;
+  int32_t *dist_row; /* per-query launch buffer [n] */
+};
+
+static void die_engine(const char *what, int rc) {
+  printf("%s failed: %s\n", what, mnhn_sw_strerror(rc));
+  fflush(stdout);
+  _exit(1);
+}
+
+mnhn_gpu_stuff mnhn_gpu_initialization(dataset ds) {
+  mnhn_gpu_stuff g;
+  memset(&g, 0, sizeof(g));
+  int rc = mnhn_sw_create((const char *const *)ds.sequences, ds.sequence_lengths, ds.n_values,
+                          NULL, &g.sw);
+  if (rc) die_engine("mnhn_sw_create", rc);
+  g.cache = (mnhn_eps_cache *)calloc(1, sizeof(mnhn_eps_cache));
+  const char *mode = getenv("MNHN_REGION_QUERY");
+  g.cache->per_query_launch = mode && strcmp(mode, "launch") == 0;
+  g.num_devs = 1;
+  return g;
+}
+
+void mnhn_gpu_destroy(mnhn_gpu_stuff g) {
+  if (g.cache) {
+    if (g.cache->have) mnhn_csr_free(&g.cache->csr);
+    free(g.cache->dist_row);
+    free(g.cache);
+  }
+  mnhn_sw_destroy(g.sw);
+}
+
+void mnhn_gpu_set_lookahead(mnhn_gpu_stuff g, float lookahead) {
+  if (g.cache) g.cache->lookahead = lookahead > 0 ? lookahead : 0;
+}
+
+/* Largest integer distance d with (float)d <= eps (distances are >= 0 and far
+ * below 2^24, so the float compare is exact); -1 when no distance qualifies. */
+static int32_t eps_floor(float eps) {
+  if (!(eps >= 0)) return -1;
+  if (eps >= 2.0e9f) return INT32_MAX;
+  return (int32_t)floorf(eps);
+}
+
+static void ensure_lists(mnhn_gpu_stuff g, float eps) {
+  mnhn_eps_cache *c = g.cache;
+  const int32_t need = eps_floor(eps);
+  if (c->have && c->csr.eps_max >= need) return;
+  if (c->have) mnhn_csr_free(&c->csr);
+  c->have = 0;
+  double want = (double)need + (double)c->lookahead;
+  if (need < 0) want = -1;
+  int32_t eps_max = want > 2.0e9 ? INT32_MAX : (int32_t)floor(want);
+  int rc = mnhn_sw_eps_csr(g.sw, eps_max, &c->csr);
+  if (rc) die_engine("mnhn_sw_eps_csr", rc);
+  c->have = 1;
+}
+
+/* N(p) for threshold eps into out[] (ascending, p included); returns |N(p)|. */
+static int region_query(mnhn_gpu_stuff g, int n, int p, float eps, int *out) {
+  mnhn_eps_cache *c = g.cache;
+  int k = 0;
+  if (c->per_query_launch) {
+    int rc = mnhn_sw_one_vs_all(g.sw, p, c->dist_row);
+    if (rc) die_engine("mnhn_sw_one_vs_all", rc);
+    for (int j = 0; j < n; j++)
+      if ((float)c->dist_row[j] <= eps) out[k++] = j;
+    return k;
+  }
+  const int64_t b = c->csr.row_ptr[p], e = c->csr.row_ptr[p + 1];
+  const int self = (float)0 <= eps;
+  int placed = !self;
+  for (int64_t t = b; t < e; t++) {
+    const int j = c->csr.col[t];
+    if (!placed && j > p) {
+      out[k++] = p;
+      placed = 1;
+    }
+    if ((float)c->csr.dist[t] <= eps) out[k++] = j;
+  }
+  if (!placed) out[k++] = p;
+  return k;
+}
+
+split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
+  const int n = ds.n_values;
+  if (epsilon == -1) {
+    printf("Warning @dbscan: epsilon value is: %f\n", epsilon);
+    epsilon = 0.015f;
+    printf("Warning @dbscan: epsilon was changed to: %f\n", epsilon);
+  }
+  if (minpts == -1) {
+    printf("Warning @dbscan: minpts value is: %i\n", minpts);
+    minpts = 5;
+    printf("Warning @dbscan: minpts was changed to: %i\n", minpts);
+  }
+  split_set ret;
+  ret.n_clusters = 0;
+  ret.clusters = (cluster *)malloc(sizeof(cluster) * (size_t)(n ? n : 1));
+  if (n == 0) return ret;
+
+  if (g.cache->per_query_launch) {
+    if (!g.cache->dist_row) g.cache->dist_row = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  } else {
+    ensure_lists(g, epsilon);
+  }
+
+  unsigned char *visited = (unsigned char *)calloc((size_t)n, 1);
+  unsigned char *member = (unsigned char *)calloc((size_t)n, 1);
+  int *stamp = (int *)calloc((size_t)n, sizeof(int)); /* union membership, per cluster */
+  int *nb = (int *)malloc(sizeof(int) * (size_t)n);
+  int *queue = (int *)malloc(sizeof(int) * (size_t)n);
+
+  for (int i = 0; i < n; i++) {
+    if (visited[i]) continue;
+    visited[i] = 1;
+    int cnt = region_query(g, n, i, epsilon, nb);
+    if (!(minpts < cnt)) continue;
+    /* expand_cluster */
+    const int id = ret.n_clusters;
+    const int mark = id + 1;
+    cluster *cl = &ret.clusters[id];
+    cl->id = id;
+    cl->members = (int *)malloc(sizeof(int) * (size_t)n);
+    cl->n_members = 0;
+    cl->members[cl->n_members++] = i;
+    member[i] = 1;
+    int qn = 0;
+    for (int t = 0; t < cnt; t++) {
+      queue[qn++] = nb[t];
+      stamp[nb[t]] = mark;
+    }
+    for (int qi = 0; qi < qn; qi++) {
+      const int q = queue[qi];
+      if (!visited[q]) {
+        visited[q] = 1;
+        int cq = region_query(g, n, q, epsilon, nb);
+        if (cq >= minpts) {
+          for (int t = 0; t < cq; t++) {
+            const int x = nb[t];
+            if (stamp[x] != mark) {
+              stamp[x] = mark;
+              queue[qn++] = x;
+            }
+          }
+        }
+      }
+      if (!member[q]) {
+        cl->members[cl->n_members++] = q;
+        member[q] = 1;
+      }
+    }
+    cl->members = (int *)realloc(cl->members, sizeof(int) * (size_t)(cl->n_members ? cl->n_members : 1));
+    ret.n_clusters++;
+  }
+  free(visited);
+  free(member);
+  free(stamp);
+  free(nb);
+  free(queue);
+  return ret;
+}
+
+void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
+                     dataset ds, float epsilon_start, float epsilon_inc, int minpts,
+                     char *split_files_prefix, int n_threads) {
+  (void)n_threads; /* epsilon values are scanned in order by one thread */
+  mnhn_gpu_stuff g = mnhn_gpu_initialization(ds);
+  /* neighbour lists are computed up to 8 increments beyond the epsilon asked
+   * for, so the sweep's next passes reuse them */
+  mnhn_gpu_set_lookahead(g, 8.0f * (epsilon_inc > 0 ? epsilon_inc : 0));
+
+  printf("Performing adaptive clustering with parameters: \n"
+         "minpts: %i\n"
+         "epsilon_start: %f\n"
+         "epsilon_inc: %f\n",
+         minpts, epsilon_start, epsilon_inc);
+
+  int cap = 16;
+  split_set *sets = (split_set *)malloc(sizeof(split_set) * (size_t)cap);
+  int nsets = 0;
+  split_set s0 = dbscanner(ds, epsilon_start, minpts, g);
+  printf("Initial set obtained with %i clusters\n", s0.n_clusters);
+  int tries = 0;
+  while (s0.n_clusters == 1) {
+    if (tries == 20 || epsilon_start == 0) {
+      printf("Error did not find a sufficient starting position in 20 tries \n");
+      fflush(stdout);
+      _exit(1);
+    }
+    epsilon_start = epsilon_start / 2;
+    printf("Trying new starting point \n");
+    free_split_set_and_associated_clusters(s0);
+    s0 = dbscanner(ds, epsilon_start, minpts, g);
+    tries++;
+  }
+  printf("Sarting with %i clusters\n", s0.n_clusters);
+  cluster nc = data_not_in_clusters(s0, ds);
+  printf("Coverage at initial point: %f\n", (float)1.f - (float)nc.n_members / (float)ds.n_values);
+  free(nc.members);
+  if (s0.n_clusters == 1) {
+    printf("All clusters fusioned in one step, decrease epsilon increment\n");
+    fflush(stdout);
+    _exit(1);
+  }
+  sets[nsets++] = s0;
+
+  /* the scanning thread of dbscan.c:1061-1115, run inline */
+  float eps_now = epsilon_start + epsilon_inc;
+  for (;;) {
+    float eps = eps_now;
+    eps_now = eps_now + epsilon_inc;
+    split_set s = dbscanner(ds, eps, minpts, g);
+    if (nsets == cap) {
+      cap *= 2;
+      sets = (split_set *)realloc(sets, sizeof(split_set) * (size_t)cap);
+    }
+    sets[nsets++] = s;
+    if (s.n_clusters == 1) break;
+  }
+  const int stop = nsets - 1;
+
+  split_set *kept = (split_set *)malloc(sizeof(split_set) * (size_t)nsets);
+  cluster_connections **conn = (cluster_connections **)malloc(sizeof(*conn) * (size_t)nsets);
+  unsigned char *is_kept = (unsigned char *)calloc((size_t)nsets, 1);
+  int count = 0;
+  kept[0] = sets[0];
+  is_kept[0] = 1;
+  for (int i = 1; i <= stop; i++) {
+    split_set ns = sets[i];
+    if (ns.n_clusters < kept[count].n_clusters) {
+      cluster nc2 = data_not_in_clusters(ns, ds);
+      printf("Layer %i has %i clusters\n", count + 1, ns.n_clusters);
+      printf("  Coverage at layer %i: %f\n", count + 1,
+             1.f - (float)nc2.n_members / (float)ds.n_values);
+      free(nc2.members);
+      printf("  Epsilon at layer %i: %f\n", count + 1, epsilon_start + (i + 1) * (float)epsilon_inc);
+      conn[count] = generate_split_set_relation(kept[count], ns);
+      count++;
+      kept[count] = ns;
+      is_kept[i] = 1;
+    }
+  }
+  mnhn_gpu_destroy(g);
+
+  printf("Connections found: \n");
+  size_t plen = strlen(split_files_prefix);
+  char *fname = (char *)malloc(plen + 16);
+  for (int i = 0; i < count + 1; i++) {
+    snprintf(fname, plen + 16, "%s%04d", split_files_prefix, i);
+    store_split_set(fname, kept[i]);
+  }
+  free(fname);
+  for (int i = count; i > 0; i--) {
+    printf("Layer %i:\n", i);
+    for (int j = 0; j < kept[i].n_clusters; j++) {
+      printf("  Cluster %i connected to: \n  ", j);
+      for (int k = 0; k < conn[i - 1][j].n_connections; k++) printf("%i ", conn[i - 1][j].connections[k]);
+      printf("\n");
+    }
+  }
+  for (int i = count; i > 0; i--) {
+    for (int j = 0; j < kept[i].n_clusters; j++) free(conn[i - 1][j].connections);
+    free(conn[i - 1]);
+  }
+  for (int i = 0; i < nsets; i++) free_split_set_and_associated_clusters(sets[i]);
+  free(is_kept);
+  free(conn);
+  free(kept);
+  free(sets);
+}

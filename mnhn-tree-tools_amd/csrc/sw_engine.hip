@@ -1,0 +1,1029 @@
+// sw_engine.hip -- gfx950 Smith-Waterman distance engine behind include/mnhn_sw.h.
+//
+// Computes the reference's distance (smith-waterman.c:16-81)
+//     d(A, B) = 4*max(la, lb) - max H(i, j),  i in [1, la], j in [1, lb]
+// with match +4 / mismatch -4 / linear gap 3, index 0 never compared and the
+// string terminators A[la], B[lb] matching only each other.
+//
+// Kernel design (inter-sequence parallelism, one pair per 16-bit lane half):
+//  * A work item is (target group of 128 sequences, range of queries).  Each
+//    wave64 lane holds two targets, one per 16-bit half of a 32-bit VGPR, and
+//    the query is wave-uniform, so its residues come through scalar loads.
+//  * Scores are kept biased by -32768 in signed 16-bit halves.  Saturating
+//    v_pk_add_i16 then applies the local-alignment floor max(0, .) for free:
+//        D = sat(Hdiag + s), E = sat(max(Hleft, Hup) - gap), H = max(D, E)
+//    plus best = max(best, H): five packed VALU ops per two cells.
+//  * The target dimension is cut into register strips of S columns.  For
+//    each strip the lane builds a query profile P[c][jj] = s(c, target[jj])
+//    for the four residue codes c; the wave-uniform query code selects which
+//    P is used (scalar branch), so no per-cell lookup is needed.  The strip's
+//    right-most H column is handed to the next strip through a per-wave
+//    global workspace laid out [row][lane] (coalesced 256 B per row).
+//  * The terminators need no DP row or column: every cell in the terminator
+//    row/column is bounded by its neighbours minus 3 except the corner
+//    (la, lb), whose value is H(la-1, lb-1) + 4.  So the DP runs over the
+//    residues only (rows 1..la-1, columns 1..lb-1) and adds the corner term
+//    in closed form.  Columns past a target's end are padding that mismatches
+//    every query residue; such cells never exceed their neighbours, so they
+//    cannot change the maximum.
+//  * Persistent grid: waves pull work items from an atomic queue (items are
+//    sorted by cost, longest first).
+//  * epsilon mode emits (row, col, d) keys for d <= eps_max for both
+//    directions of each unordered pair; the keys are radix-sorted on the
+//    device into ascending-column CSR rows.
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "mnhn_sw.h"
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kGroup = 128;          // targets per work item (2 per lane)
+constexpr int kBlock = 256;          // threads per workgroup (4 waves)
+constexpr short kFloor = -32768;     // biased representation of H = 0
+constexpr int kBias = 32768;
+
+enum { MODE_DIST = 0, MODE_EPS = 1 };
+
+struct KArgs {
+  const uint32_t *words;   // packed residues of all sequences
+  const uint32_t *woff;    // first word of each sequence
+  const int32_t *len;      // sequence lengths
+  const int32_t *rank;     // position of each sequence in length order
+  const int32_t *gtgt;     // [groups][128] target ids, -1 = empty slot
+  const int32_t *gmax;     // [groups] longest target of the group
+  const int4 *items;       // (group, q_begin, q_end, out_base)
+  const int32_t *qids;     // query ids referenced by item ranges
+  int *counter;            // work-queue head
+  uint32_t *bound;         // per-wave strip-boundary workspace
+  int32_t *dist_out;       // MODE_DIST output
+  unsigned long long *keys;      // MODE_EPS output
+  unsigned long long *nkeys;     // MODE_EPS key counter
+  unsigned long long key_cap;
+  int n_items;
+  int bound_stride;        // uint32 words per wave
+  int mode;
+  int triangle;            // only pairs with rank[q] < rank[t]
+  int eps_max;
+  int match, mismatch, gap;
+};
+
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) {
+  return __builtin_bit_cast(s16x2, v);
+}
+__device__ __forceinline__ uint32_t as_u32(s16x2 v) {
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+// One DP row over a register strip, query residue profile P.  The diagonal
+// term of column jj+1 is formed from the old H[jj] before H[jj] is replaced,
+// so every new H value can take its predecessor's register (no copies).
+template <int S>
+__device__ __forceinline__ void dp_row(s16x2 (&H)[S], const s16x2 (&P)[S],
+                                       s16x2 left, s16x2 diag, s16x2 gapv,
+                                       s16x2 &best) {
+  s16x2 dnext = __builtin_elementwise_add_sat(diag, P[0]);
+#pragma unroll
+  for (int jj = 0; jj < S; ++jj) {
+    const s16x2 D = dnext;
+    if (jj + 1 < S) dnext = __builtin_elementwise_add_sat(H[jj], P[jj + 1]);
+    const s16x2 M = __builtin_elementwise_max(left, H[jj]);
+    const s16x2 E = __builtin_elementwise_add_sat(M, gapv);
+    const s16x2 h = __builtin_elementwise_max(D, E);
+    best = __builtin_elementwise_max(best, h);
+    H[jj] = h;
+    left = h;
+  }
+}
+
+// Generic alphabet: substitution score computed from 16-bit codes per cell.
+template <int S>
+__device__ __forceinline__ s16x2 code_score(uint32_t tc, uint32_t qq, s16x2 pm, s16x2 dmm) {
+  const u16x2 x = __builtin_bit_cast(u16x2, tc ^ qq);
+  const u16x2 m01 = __builtin_elementwise_min(x, (u16x2){1, 1});
+  return pm + __builtin_bit_cast(s16x2, m01) * dmm;
+}
+
+template <int S>
+__device__ __forceinline__ void dp_row_codes(s16x2 (&H)[S], const uint32_t (&TC)[S],
+                                             uint32_t qq, s16x2 pm, s16x2 dmm,
+                                             s16x2 left, s16x2 diag, s16x2 gapv,
+                                             s16x2 &best) {
+  s16x2 dnext = __builtin_elementwise_add_sat(diag, code_score<S>(TC[0], qq, pm, dmm));
+#pragma unroll
+  for (int jj = 0; jj < S; ++jj) {
+    const s16x2 D = dnext;
+    if (jj + 1 < S)
+      dnext = __builtin_elementwise_add_sat(H[jj], code_score<S>(TC[jj + 1], qq, pm, dmm));
+    const s16x2 M = __builtin_elementwise_max(left, H[jj]);
+    const s16x2 E = __builtin_elementwise_add_sat(M, gapv);
+    const s16x2 h = __builtin_elementwise_max(D, E);
+    best = __builtin_elementwise_max(best, h);
+    H[jj] = h;
+    left = h;
+  }
+}
+
+template <int S, bool kAcgt>
+__global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+  // the packed residues are read-only for the whole launch: reading them
+  // through the constant address space lets uniform loads use s_load
+  const __attribute__((address_space(4))) uint32_t *cwords =
+      (const __attribute__((address_space(4))) uint32_t *)a.words;
+  uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+
+  const s16x2 floor2 = {kFloor, kFloor};
+  const s16x2 gapv = {(short)-a.gap, (short)-a.gap};
+  const short smat = (short)a.match, smis = (short)a.mismatch;
+  const s16x2 pm = {smat, smat};
+  const s16x2 dmm = {(short)(smis - smat), (short)(smis - smat)};
+
+  for (;;) {
+    int item_idx = 0;
+    if (lane == 0) item_idx = atomicAdd(a.counter, 1);
+    item_idx = __builtin_amdgcn_readfirstlane(item_idx);
+    if (item_idx >= a.n_items) break;
+    const int4 it = a.items[item_idx];
+    const int g = __builtin_amdgcn_readfirstlane(it.x);
+    const int qb = __builtin_amdgcn_readfirstlane(it.y);
+    const int qe = __builtin_amdgcn_readfirstlane(it.z);
+    const int ob = __builtin_amdgcn_readfirstlane(it.w);
+
+    const int t_lo = a.gtgt[g * kGroup + lane];
+    const int t_hi = a.gtgt[g * kGroup + kWave + lane];
+    const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
+    const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
+    const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
+    const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
+    const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
+    const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
+    const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
+    // inner columns are 1..lb-1
+    const int nstrips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
+
+    for (int qi = qb; qi < qe; ++qi) {
+      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
+      const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
+      const int nrows = la > 1 ? la - 1 : 0;
+      s16x2 best = floor2;
+      s16x2 corner = floor2;
+
+      for (int s = 0; s < nstrips && nrows > 0; ++s) {
+        const int j0 = 1 + s * S;  // DP column of strip slot 0
+        s16x2 H[S];
+#pragma unroll
+        for (int jj = 0; jj < S; ++jj) H[jj] = floor2;
+
+        // ---- target residues of this strip (residue index j0-1 .. j0+S-2)
+        constexpr int kPerWord = kAcgt ? 16 : 2;
+        constexpr int kWords = S / kPerWord;
+        uint32_t wl[kWords], wh[kWords];
+        const uint32_t wbase = (uint32_t)(s * S) / kPerWord;
+#pragma unroll
+        for (int w = 0; w < kWords; ++w) {
+          wl[w] = a.words[wo_lo + wbase + w];
+          wh[w] = a.words[wo_hi + wbase + w];
+        }
+        // strip-local index of the last inner column of each target
+        const int cl_lo = lb_lo - 1 - j0;  // corner column slot, may be out of range
+        const int cl_hi = lb_hi - 1 - j0;
+
+        if constexpr (kAcgt) {
+          s16x2 P0[S], P1[S], P2[S], P3[S];
+#pragma unroll
+          for (int jj = 0; jj < S; ++jj) {
+            const int sh = 2 * (jj % 16);
+            // residue valid while column j0+jj <= lb-1, i.e. jj <= cl
+            uint32_t c_lo = jj <= cl_lo ? (wl[jj / 16] >> sh) & 3u : 4u;
+            uint32_t c_hi = jj <= cl_hi ? (wh[jj / 16] >> sh) & 3u : 4u;
+            P0[jj] = (s16x2){c_lo == 0u ? smat : smis, c_hi == 0u ? smat : smis};
+            P1[jj] = (s16x2){c_lo == 1u ? smat : smis, c_hi == 1u ? smat : smis};
+            P2[jj] = (s16x2){c_lo == 2u ? smat : smis, c_hi == 2u ? smat : smis};
+            P3[jj] = (s16x2){c_lo == 3u ? smat : smis, c_hi == 3u ? smat : smis};
+          }
+          s16x2 diag_b = floor2;
+          uint32_t next_b = as_u32(floor2);
+          if (s > 0) next_b = bnd[kWave + (uint32_t)lane];
+          // query residues: 16 per word, fetched one word ahead
+          uint32_t qword = cwords[qwo];
+          uint32_t qnext = cwords[qwo + 1];
+          for (int i = 1; i <= nrows; ++i) {
+            const s16x2 left_b = as_s16x2(next_b);
+            if (s > 0 && i < nrows) next_b = bnd[(uint32_t)(i + 1) * kWave + (uint32_t)lane];
+            const int r = i - 1;  // residue index of the query row
+            if ((r & 15) == 0 && r > 0) {
+              qword = qnext;
+              qnext = cwords[qwo + (uint32_t)(r >> 4) + 1];
+            }
+            const uint32_t code = (qword >> (2 * (r & 15))) & 3u;
+            // each case ends in a distinct asm marker so the four row bodies
+            // are not merged into one body fed by register copies
+            switch (code) {
+              case 0: dp_row<S>(H, P0, left_b, diag_b, gapv, best); asm volatile("; row c0"); break;
+              case 1: dp_row<S>(H, P1, left_b, diag_b, gapv, best); asm volatile("; row c1"); break;
+              case 2: dp_row<S>(H, P2, left_b, diag_b, gapv, best); asm volatile("; row c2"); break;
+              default: dp_row<S>(H, P3, left_b, diag_b, gapv, best); asm volatile("; row c3"); break;
+            }
+            if (s + 1 < nstrips) bnd[(uint32_t)i * kWave + (uint32_t)lane] = as_u32(H[S - 1]);
+            diag_b = left_b;
+          }
+        } else {
+          uint32_t TC[S];
+#pragma unroll
+          for (int jj = 0; jj < S; ++jj) {
+            const int sh = 16 * (jj & 1);
+            uint32_t c_lo = jj <= cl_lo ? (wl[jj / 2] >> sh) & 0xffffu : 0xffffu;
+            uint32_t c_hi = jj <= cl_hi ? (wh[jj / 2] >> sh) & 0xffffu : 0xffffu;
+            TC[jj] = c_lo | (c_hi << 16);
+          }
+          s16x2 diag_b = floor2;
+          uint32_t next_b = as_u32(floor2);
+          if (s > 0) next_b = bnd[kWave + (uint32_t)lane];
+          uint32_t qword = cwords[qwo];
+          uint32_t qnext = cwords[qwo + 1];
+          for (int i = 1; i <= nrows; ++i) {
+            const s16x2 left_b = as_s16x2(next_b);
+            if (s > 0 && i < nrows) next_b = bnd[(uint32_t)(i + 1) * kWave + (uint32_t)lane];
+            const int r = i - 1;
+            if ((r & 1) == 0 && r > 0) {
+              qword = qnext;
+              qnext = cwords[qwo + (uint32_t)(r >> 1) + 1];
+            }
+            const uint32_t code = (qword >> (16 * (r & 1))) & 0xffffu;
+            const uint32_t qq = code | (code << 16);
+            dp_row_codes<S>(H, TC, qq, pm, dmm, left_b, diag_b, gapv, best);
+            if (s + 1 < nstrips) bnd[(uint32_t)i * kWave + (uint32_t)lane] = as_u32(H[S - 1]);
+            diag_b = left_b;
+          }
+        }
+        // corner H(la-1, lb-1) of each half, if its column is in this strip
+        uint32_t cu = as_u32(corner);
+#pragma unroll
+        for (int jj = 0; jj < S; ++jj) {
+          const uint32_t hv = as_u32(H[jj]);
+          if (jj == cl_lo) cu = (cu & 0xffff0000u) | (hv & 0x0000ffffu);
+          if (jj == cl_hi) cu = (cu & 0x0000ffffu) | (hv & 0xffff0000u);
+        }
+        corner = as_s16x2(cu);
+      }
+
+      // ---- distances of the two targets
+      const int best_lo = (int)best.x + kBias, best_hi = (int)best.y + kBias;
+      const int cor_lo = (int)corner.x + kBias, cor_hi = (int)corner.y + kBias;
+      int tot_lo = 0, tot_hi = 0;
+      if (la >= 1 && lb_lo >= 1) tot_lo = max(best_lo, cor_lo + a.match);
+      if (la >= 1 && lb_hi >= 1) tot_hi = max(best_hi, cor_hi + a.match);
+      const int d_lo = a.match * max(la, lb_lo) - tot_lo;
+      const int d_hi = a.match * max(la, lb_hi) - tot_hi;
+
+      const int rq = a.triangle ? a.rank[q] : 0;
+      const bool ok_lo = t_lo >= 0 && (!a.triangle || rq < rk_lo);
+      const bool ok_hi = t_hi >= 0 && (!a.triangle || rq < rk_hi);
+      if (a.mode == MODE_DIST) {
+        int32_t *o = a.dist_out + (size_t)ob + (size_t)(qi - qb) * kGroup;
+        if (ok_lo) o[lane] = d_lo;
+        if (ok_hi) o[kWave + lane] = d_hi;
+      } else {
+        const bool h_lo = ok_lo && d_lo <= a.eps_max;
+        const bool h_hi = ok_hi && d_hi <= a.eps_max;
+        const unsigned long long b_lo = __ballot(h_lo);
+        const unsigned long long b_hi = __ballot(h_hi);
+        const int total = 2 * (__popcll(b_lo) + __popcll(b_hi));
+        if (total > 0) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(a.nkeys, (unsigned long long)total);
+          base = __builtin_amdgcn_readfirstlane((unsigned)base) |
+                 ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
+          const unsigned mlo = __builtin_amdgcn_mbcnt_hi((unsigned)(b_lo >> 32),
+                                __builtin_amdgcn_mbcnt_lo((unsigned)b_lo, 0u));
+          const unsigned mhi = __builtin_amdgcn_mbcnt_hi((unsigned)(b_hi >> 32),
+                                __builtin_amdgcn_mbcnt_lo((unsigned)b_hi, 0u));
+          unsigned long long pos = base + 2ull * (mlo + mhi);
+          const unsigned long long uq = (unsigned long long)q;
+          if (h_lo) {
+            const unsigned long long ut = (unsigned long long)t_lo, ud = (unsigned)d_lo;
+            if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
+            if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
+            pos += 2;
+          }
+          if (h_hi) {
+            const unsigned long long ut = (unsigned long long)t_hi, ud = (unsigned)d_hi;
+            if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
+            if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Decode sorted keys into CSR columns / distances and count rows.
+__global__ void decode_keys_kernel(const unsigned long long *keys, long long n,
+                                   int32_t *col, int16_t *dist, int32_t *row_cnt) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long key = keys[k];
+  col[k] = (int32_t)((key >> 16) & 0xffffffull);
+  dist[k] = (int16_t)(key & 0xffffull);
+  atomicAdd(&row_cnt[(int)(key >> 40)], 1);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+
+struct mnhn_sw_ctx {
+  int n = 0;
+  bool acgt = true;
+  int match = 4, mismatch = -4, gap = 3;
+  std::vector<int32_t> len, order, rank;
+  std::vector<uint32_t> woff;
+  int groups = 0;
+  std::vector<int32_t> gtgt, gmax;
+  int32_t maxlen = 0;
+  int device = 0, num_cu = 0;
+  int strip = 0, waves_per_cu = 0;  // 0 = tuned default
+  uint32_t *d_words = nullptr, *d_woff = nullptr;
+  int32_t *d_len = nullptr, *d_rank = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr;
+  int *d_counter = nullptr;
+  unsigned long long *d_nkeys = nullptr;
+  uint32_t *d_bound = nullptr;
+  size_t bound_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  mnhn_sw_timing last{};
+  // device-resident epsilon lists of the last mnhn_sw_eps_count call
+  unsigned long long *d_keys = nullptr;
+  size_t keys_cap = 0;
+};
+
+namespace {
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t e_ = (expr);                            \
+    if (e_ != hipSuccess) return MNHN_ERR_DEVICE;      \
+  } while (0)
+
+template <typename T>
+int dev_alloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return MNHN_ERR_NOMEM;
+  }
+  return MNHN_OK;
+}
+
+template <typename T>
+void dev_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct Work {
+  std::vector<int4> items;
+  std::vector<int32_t> qids;
+  std::vector<int32_t> gtgt, gmax;  // empty = use the context's length groups
+  double cells = 0, pairs = 0;
+  int max_q_len = 0;
+};
+
+int pick_strip(const mnhn_sw_ctx *c) {
+  if (c->strip == 16 || c->strip == 32) return c->strip;
+  const char *e = getenv("MNHN_SW_STRIP");
+  if (e) {
+    int v = atoi(e);
+    if (v == 16 || v == 32) return v;
+  }
+  return 32;
+}
+
+template <int S, bool A>
+void *kernel_ptr() {
+  return reinterpret_cast<void *>(&sw_tiles_kernel<S, A>);
+}
+
+int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
+                 int32_t *d_dist, unsigned long long *d_keys,
+                 unsigned long long key_cap) {
+  if (w.items.empty()) return MNHN_OK;
+  int rc;
+  int4 *d_items = nullptr;
+  int32_t *d_qids = nullptr, *d_gt = nullptr, *d_gm = nullptr;
+  if ((rc = dev_alloc(&d_items, w.items.size()))) return rc;
+  if ((rc = dev_alloc(&d_qids, w.qids.size()))) { dev_free(d_items); return rc; }
+  const bool own_groups = !w.gtgt.empty();
+  if (own_groups) {
+    if ((rc = dev_alloc(&d_gt, w.gtgt.size())) || (rc = dev_alloc(&d_gm, w.gmax.size()))) {
+      dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm);
+      return rc;
+    }
+  }
+  auto cleanup = [&]() {
+    dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm);
+  };
+  hipError_t e = hipSuccess;
+  e = hipMemcpyAsync(d_items, w.items.data(), w.items.size() * sizeof(int4),
+                     hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_qids, w.qids.data(), w.qids.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && own_groups) {
+    e = hipMemcpyAsync(d_gt, w.gtgt.data(), w.gtgt.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_gm, w.gmax.data(), w.gmax.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice, c->stream);
+  }
+  if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(int), c->stream);
+  if (e != hipSuccess) { cleanup(); return MNHN_ERR_DEVICE; }
+
+  const int S = pick_strip(c);
+  int wpc = c->waves_per_cu;
+  if (wpc <= 0) {
+    const char *ev = getenv("MNHN_SW_WAVES_PER_CU");
+    wpc = ev ? atoi(ev) : 0;
+  }
+  void *kp = S == 16 ? (c->acgt ? kernel_ptr<16, true>() : kernel_ptr<16, false>())
+                     : (c->acgt ? kernel_ptr<32, true>() : kernel_ptr<32, false>());
+  if (wpc <= 0) {
+    int blocks_per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) !=
+            hipSuccess || blocks_per_cu <= 0)
+      blocks_per_cu = 1;
+    wpc = blocks_per_cu * (kBlock / kWave);
+  }
+  int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
+  const int max_useful = (int)((w.items.size() + 3) / 4);
+  if (blocks > max_useful) blocks = std::max(1, max_useful);
+  const int waves = blocks * (kBlock / kWave);
+
+  // strip-boundary workspace: (max query length + 1) rows x 64 lanes per wave
+  const int stride = (w.max_q_len + 2) * kWave;
+  const size_t need = (size_t)waves * stride * sizeof(uint32_t);
+  if (need > c->bound_bytes) {
+    dev_free(c->d_bound);
+    c->bound_bytes = 0;
+    if (hipMalloc((void **)&c->d_bound, need) != hipSuccess) {
+      c->d_bound = nullptr;
+      cleanup();
+      return MNHN_ERR_NOMEM;
+    }
+    c->bound_bytes = need;
+  }
+
+  KArgs a{};
+  a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
+  a.gtgt = own_groups ? d_gt : c->d_gtgt;
+  a.gmax = own_groups ? d_gm : c->d_gmax;
+  a.items = d_items; a.qids = d_qids; a.counter = c->d_counter;
+  a.bound = c->d_bound; a.dist_out = d_dist; a.keys = d_keys; a.nkeys = c->d_nkeys;
+  a.key_cap = key_cap; a.n_items = (int)w.items.size(); a.bound_stride = stride;
+  a.mode = mode; a.triangle = triangle; a.eps_max = eps_max;
+  a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
+
+  (void)hipEventRecord(c->ev[0], c->stream);
+  if (S == 16) {
+    if (c->acgt) hipLaunchKernelGGL((sw_tiles_kernel<16, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL((sw_tiles_kernel<16, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+  } else {
+    if (c->acgt) hipLaunchKernelGGL((sw_tiles_kernel<32, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL((sw_tiles_kernel<32, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+  }
+  e = hipGetLastError();
+  (void)hipEventRecord(c->ev[1], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+  float ms = 0.f;
+  if (e == hipSuccess) (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+  cleanup();
+  if (e != hipSuccess) return MNHN_ERR_DEVICE;
+  c->last.kernel_ms = ms;
+  c->last.total_ms = ms;
+  c->last.cells = w.cells;
+  c->last.pairs = w.pairs;
+  c->last.lane_bits = 16;
+  c->last.strip = S;
+  c->last.waves = waves;
+  c->last.items = (int)w.items.size();
+  return MNHN_OK;
+}
+
+// Split the query ranges [0, nq_g) of each group into items of similar cost.
+void add_group_items(const mnhn_sw_ctx *c, Work &w, int g, int q_begin, int q_end,
+                     double target_cost, int S) {
+  const int gm = c->gmax[g];
+  const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
+  int qb = q_begin;
+  double acc = 0;
+  for (int qi = q_begin; qi < q_end; ++qi) {
+    const int la = c->len[w.qids[qi]];
+    acc += (double)std::max(la - 1, 0) * strips * S + 64.0;
+    if (acc >= target_cost || qi + 1 == q_end) {
+      w.items.push_back(make_int4(g, qb, qi + 1, 0));
+      qb = qi + 1;
+      acc = 0;
+    }
+  }
+}
+
+double item_cost(const mnhn_sw_ctx *c, const Work &w, const int4 &it, int S) {
+  const int gm = w.gmax.empty() ? c->gmax[it.x] : w.gmax[it.x];
+  const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
+  double acc = 0;
+  for (int qi = it.y; qi < it.z; ++qi)
+    acc += (double)std::max(c->len[w.qids[qi]] - 1, 0) * strips * S + 64.0;
+  return acc;
+}
+
+void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
+  std::vector<double> cost(w.items.size());
+  for (size_t k = 0; k < w.items.size(); ++k) cost[k] = item_cost(c, w, w.items[k], S);
+  std::vector<size_t> idx(w.items.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  std::vector<int4> sorted;
+  sorted.reserve(idx.size());
+  for (size_t k : idx) sorted.push_back(w.items[k]);
+  w.items.swap(sorted);
+  if (dense_out) {
+    int off = 0;
+    for (auto &it : w.items) {
+      it.w = off;
+      off += (it.z - it.y) * kGroup;
+    }
+  }
+  int mq = 0;
+  for (int32_t q : w.qids) mq = std::max(mq, c->len[q]);
+  w.max_q_len = mq;
+}
+
+// All unique pairs: group g holds sequences at length-order positions
+// [128g, 128g+128); its queries are the positions before its last member.
+Work triangle_work(const mnhn_sw_ctx *c, int S) {
+  Work w;
+  w.qids = c->order;
+  double total = 0;
+  for (int g = 0; g < c->groups; ++g) {
+    const int last = std::min((g + 1) * kGroup, c->n) - 1;
+    const int gm = c->gmax[g];
+    const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
+    for (int p = 0; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
+  }
+  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 64.0), 2.0e5);
+  for (int g = 0; g < c->groups; ++g) {
+    const int last = std::min((g + 1) * kGroup, c->n) - 1;
+    if (last <= 0) continue;
+    add_group_items(c, w, g, 0, last, target, S);
+  }
+  // algorithmic cells over unique pairs
+  double sumL = 0, sumL2 = 0;
+  for (int i = 0; i < c->n; ++i) {
+    sumL += c->len[i];
+    sumL2 += (double)c->len[i] * c->len[i];
+  }
+  w.cells = 0.5 * (sumL * sumL - sumL2);
+  w.pairs = 0.5 * (double)c->n * (c->n - 1);
+  return w;
+}
+
+int check_range(const mnhn_sw_ctx *c) {
+  // biased 16-bit halves hold H in [0, 65535]; H <= match * min(la, lb)
+  if ((long long)c->match * (c->maxlen + 1) > 65535) return MNHN_ERR_RANGE;
+  return MNHN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mnhn_sw_strerror(int code) {
+  switch (code) {
+    case MNHN_OK: return "ok";
+    case MNHN_ERR_ARG: return "invalid argument";
+    case MNHN_ERR_DEVICE: return "HIP runtime error";
+    case MNHN_ERR_NOMEM: return "out of memory";
+    case MNHN_ERR_NODEVICE: return "no usable GPU";
+    case MNHN_ERR_RANGE: return "score range exceeds the 16-bit lane format";
+    default: return "unknown error";
+  }
+}
+
+int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
+                   const mnhn_sw_params *params, mnhn_sw_ctx **out) {
+  if (!out || n < 0 || (n > 0 && (!seqs || !lens))) return MNHN_ERR_ARG;
+  *out = nullptr;
+  if (n >= (1 << 24)) return MNHN_ERR_ARG;  // key format: 24-bit ids
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MNHN_ERR_NODEVICE;
+  mnhn_sw_ctx *c = new (std::nothrow) mnhn_sw_ctx();
+  if (!c) return MNHN_ERR_NOMEM;
+  if (params) {
+    if (params->match <= 0 || params->mismatch > 0 || params->gap < 0 ||
+        params->match > 1000 || params->mismatch < -1000 || params->gap > 1000) {
+      delete c;
+      return MNHN_ERR_ARG;
+    }
+    c->match = params->match;
+    c->mismatch = params->mismatch;
+    c->gap = params->gap;
+  }
+  c->n = n;
+  c->len.resize(n);
+  for (int i = 0; i < n; ++i) {
+    if (lens[i] > (size_t)INT32_MAX / 2) { delete c; return MNHN_ERR_ARG; }
+    c->len[i] = (int32_t)lens[i];
+    c->maxlen = std::max(c->maxlen, c->len[i]);
+  }
+  // alphabet of the compared residues (index >= 1)
+  bool seen[256] = {false};
+  int distinct = 0;
+  for (int i = 0; i < n; ++i)
+    for (int32_t k = 1; k < c->len[i]; ++k) {
+      unsigned char b = (unsigned char)seqs[i][k];
+      if (!seen[b]) { seen[b] = true; ++distinct; }
+    }
+  c->acgt = distinct <= 4;
+  uint8_t code_of[256];
+  memset(code_of, 0, sizeof(code_of));
+  {
+    int next = 0;
+    for (int b = 0; b < 256; ++b)
+      if (seen[b]) code_of[b] = (uint8_t)(next++);
+  }
+  // pack residues 1..L-1; each sequence padded with one spare word so that
+  // strip loads past the end stay inside the buffer
+  const int per_word = c->acgt ? 16 : 2;
+  c->woff.resize(n);
+  size_t total_words = 0;
+  for (int i = 0; i < n; ++i) {
+    c->woff[i] = (uint32_t)total_words;
+    const size_t res = c->len[i] > 1 ? (size_t)c->len[i] - 1 : 0;
+    total_words += (res + per_word - 1) / per_word;
+  }
+  // strip loads of a short target run up to the longest length of its group
+  total_words += (size_t)(c->maxlen + per_word) / per_word + 64;
+  if (total_words >= 0xffffffffull) { delete c; return MNHN_ERR_ARG; }
+  std::vector<uint32_t> words(total_words, 0u);
+  for (int i = 0; i < n; ++i) {
+    uint32_t *w = words.data() + c->woff[i];
+    for (int32_t k = 1; k < c->len[i]; ++k) {
+      const unsigned char b = (unsigned char)seqs[i][k];
+      const size_t r = (size_t)k - 1;
+      if (c->acgt) w[r / 16] |= (uint32_t)code_of[b] << (2 * (r % 16));
+      else w[r / 2] |= (uint32_t)b << (16 * (r % 2));
+    }
+  }
+  // length order (longest first) and target groups of 128
+  c->order.resize(n);
+  std::iota(c->order.begin(), c->order.end(), 0);
+  std::stable_sort(c->order.begin(), c->order.end(),
+                   [&](int x, int y) { return c->len[x] > c->len[y]; });
+  c->rank.resize(n);
+  for (int p = 0; p < n; ++p) c->rank[c->order[p]] = p;
+  c->groups = (n + kGroup - 1) / kGroup;
+  c->gtgt.assign((size_t)c->groups * kGroup, -1);
+  c->gmax.assign(c->groups, 0);
+  for (int p = 0; p < n; ++p) {
+    const int g = p / kGroup, t = p % kGroup;
+    // slot t: lane t for t < 64 (low half), lane t-64 (high half) otherwise
+    c->gtgt[(size_t)g * kGroup + t] = c->order[p];
+    c->gmax[g] = std::max(c->gmax[g], c->len[c->order[p]]);
+  }
+
+  int rc = MNHN_OK;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+    delete c;
+    return MNHN_ERR_DEVICE;
+  }
+  c->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MNHN_ERR_DEVICE;
+  }
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) { mnhn_sw_destroy(c); return MNHN_ERR_DEVICE; }
+  if ((rc = dev_alloc(&c->d_words, words.size())) || (rc = dev_alloc(&c->d_woff, (size_t)n)) ||
+      (rc = dev_alloc(&c->d_len, (size_t)n)) || (rc = dev_alloc(&c->d_rank, (size_t)n)) ||
+      (rc = dev_alloc(&c->d_gtgt, c->gtgt.size())) || (rc = dev_alloc(&c->d_gmax, c->gmax.size())) ||
+      (rc = dev_alloc(&c->d_counter, 1)) || (rc = dev_alloc(&c->d_nkeys, 1))) {
+    mnhn_sw_destroy(c);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(c->d_words, words.data(), words.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n) e = hipMemcpy(c->d_woff, c->woff.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n) e = hipMemcpy(c->d_len, c->len.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n) e = hipMemcpy(c->d_rank, c->rank.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->groups)
+    e = hipMemcpy(c->d_gtgt, c->gtgt.data(), c->gtgt.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->groups)
+    e = hipMemcpy(c->d_gmax, c->gmax.data(), c->gmax.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { mnhn_sw_destroy(c); return MNHN_ERR_DEVICE; }
+  *out = c;
+  return MNHN_OK;
+}
+
+void mnhn_sw_destroy(mnhn_sw_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dev_free(c->d_words); dev_free(c->d_woff); dev_free(c->d_len); dev_free(c->d_rank);
+  dev_free(c->d_gtgt); dev_free(c->d_gmax); dev_free(c->d_counter); dev_free(c->d_nkeys);
+  dev_free(c->d_bound); dev_free(c->d_keys);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mnhn_sw_size(const mnhn_sw_ctx *c) { return c ? c->n : -1; }
+int mnhn_sw_alphabet(const mnhn_sw_ctx *c) { return c ? (c->acgt ? 4 : 0) : -1; }
+
+int mnhn_sw_last_timing(const mnhn_sw_ctx *c, mnhn_sw_timing *out) {
+  if (!c || !out) return MNHN_ERR_ARG;
+  *out = c->last;
+  return MNHN_OK;
+}
+
+int mnhn_sw_set_tuning(mnhn_sw_ctx *c, int strip, int waves_per_cu) {
+  if (!c || (strip != 0 && strip != 16 && strip != 32) || waves_per_cu < 0) return MNHN_ERR_ARG;
+  c->strip = strip;
+  c->waves_per_cu = waves_per_cu;
+  return MNHN_OK;
+}
+
+int mnhn_sw_one_vs_all(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
+  if (!c || !dist_out || query < 0 || query >= c->n) return MNHN_ERR_ARG;
+  int rc = check_range(c);
+  if (rc) return rc;
+  const int S = pick_strip(c);
+  Work w;
+  w.qids.push_back(query);
+  for (int g = 0; g < c->groups; ++g) w.items.push_back(make_int4(g, 0, 1, 0));
+  for (int j = 0; j < c->n; ++j) w.cells += (double)c->len[query] * c->len[j];
+  w.pairs = c->n;
+  finalize_items(c, w, S, true);
+  const size_t out_n = w.items.size() * kGroup;
+  int32_t *d_out = nullptr;
+  if ((rc = dev_alloc(&d_out, out_n))) return rc;
+  rc = launch_tiles(c, w, MODE_DIST, 0, 0, d_out, nullptr, 0);
+  if (rc == MNHN_OK) {
+    std::vector<int32_t> h(out_n);
+    if (hipMemcpy(h.data(), d_out, out_n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
+    else
+      for (const auto &it : w.items)
+        for (int t = 0; t < kGroup; ++t) {
+          const int id = c->gtgt[(size_t)it.x * kGroup + t];
+          if (id >= 0) dist_out[id] = h[(size_t)it.w + t];
+        }
+  }
+  dev_free(d_out);
+  return rc;
+}
+
+int mnhn_sw_pairs(mnhn_sw_ctx *c, const int32_t *ij, int64_t m, int32_t *dist_out) {
+  if (!c || m < 0 || (m > 0 && (!ij || !dist_out))) return MNHN_ERR_ARG;
+  for (int64_t k = 0; k < 2 * m; ++k)
+    if (ij[k] < 0 || ij[k] >= c->n) return MNHN_ERR_ARG;
+  if (m == 0) return MNHN_OK;
+  int rc = check_range(c);
+  if (rc) return rc;
+  const int S = pick_strip(c);
+  // group pairs by first index; the first index is the wave-uniform query
+  std::vector<int64_t> idx(m);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
+    if (ij[2 * x] != ij[2 * y]) return ij[2 * x] < ij[2 * y];
+    return c->len[ij[2 * x + 1]] > c->len[ij[2 * y + 1]];
+  });
+  Work w;
+  std::vector<int64_t> slot_pair;  // pair index of each (group, slot)
+  for (int64_t k = 0; k < m;) {
+    const int q = ij[2 * idx[k]];
+    int64_t e = k;
+    while (e < m && ij[2 * idx[e]] == q) ++e;
+    w.qids.push_back(q);
+    const int qi = (int)w.qids.size() - 1;
+    for (int64_t s = k; s < e; s += kGroup) {
+      const int g = (int)w.gmax.size();
+      int gm = 0;
+      for (int t = 0; t < kGroup; ++t) {
+        const int64_t p = s + t;
+        const int id = p < e ? ij[2 * idx[p] + 1] : -1;
+        w.gtgt.push_back(id);
+        slot_pair.push_back(p < e ? idx[p] : -1);
+        if (id >= 0) gm = std::max(gm, c->len[id]);
+      }
+      w.gmax.push_back(gm);
+      w.items.push_back(make_int4(g, qi, qi + 1, g * kGroup));
+    }
+    k = e;
+  }
+  for (int64_t k = 0; k < m; ++k) w.cells += (double)c->len[ij[2 * k]] * c->len[ij[2 * k + 1]];
+  w.pairs = (double)m;
+  finalize_items(c, w, S, false);
+  const size_t out_n = w.gmax.size() * kGroup;
+  int32_t *d_out = nullptr;
+  if ((rc = dev_alloc(&d_out, out_n))) return rc;
+  rc = launch_tiles(c, w, MODE_DIST, 0, 0, d_out, nullptr, 0);
+  if (rc == MNHN_OK) {
+    std::vector<int32_t> h(out_n);
+    if (hipMemcpy(h.data(), d_out, out_n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
+    else
+      for (size_t s = 0; s < out_n; ++s)
+        if (slot_pair[s] >= 0) dist_out[slot_pair[s]] = h[s];
+  }
+  dev_free(d_out);
+  return rc;
+}
+
+int mnhn_sw_matrix(mnhn_sw_ctx *c, int32_t *dist_out) {
+  if (!c || (c->n > 0 && !dist_out)) return MNHN_ERR_ARG;
+  const int n = c->n;
+  if (n == 0) return MNHN_OK;
+  int rc = check_range(c);
+  if (rc) return rc;
+  const int S = pick_strip(c);
+  Work w = triangle_work(c, S);
+  finalize_items(c, w, S, true);
+  size_t out_n = 0;
+  for (const auto &it : w.items) out_n += (size_t)(it.z - it.y) * kGroup;
+  int32_t *d_out = nullptr;
+  if ((rc = dev_alloc(&d_out, out_n))) return rc;
+  rc = launch_tiles(c, w, MODE_DIST, 1, 0, d_out, nullptr, 0);
+  if (rc == MNHN_OK) {
+    std::vector<int32_t> h(out_n);
+    if (hipMemcpy(h.data(), d_out, out_n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = MNHN_ERR_DEVICE;
+    } else {
+      for (int i = 0; i < n; ++i) dist_out[(size_t)i * n + i] = 0;
+      for (const auto &it : w.items)
+        for (int qi = it.y; qi < it.z; ++qi) {
+          const int q = w.qids[qi];
+          for (int t = 0; t < kGroup; ++t) {
+            const int id = c->gtgt[(size_t)it.x * kGroup + t];
+            if (id < 0 || c->rank[q] >= c->rank[id]) continue;
+            const int32_t d = h[(size_t)it.w + (size_t)(qi - it.y) * kGroup + t];
+            dist_out[(size_t)q * n + id] = d;
+            dist_out[(size_t)id * n + q] = d;
+          }
+        }
+    }
+  }
+  dev_free(d_out);
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Run the all-pairs epsilon pass; on return the context's d_keys holds nkeys
+// unsorted keys (two per unordered pair with d <= eps_max).
+int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
+  int rc = check_range(c);
+  if (rc) return rc;
+  const int S = pick_strip(c);
+  Work w = triangle_work(c, S);
+  finalize_items(c, w, S, false);
+  const unsigned long long all = (unsigned long long)c->n * (unsigned long long)(c->n > 0 ? c->n - 1 : 0);
+  unsigned long long cap = c->keys_cap;
+  if (cap == 0) {
+    const char *e = getenv("MNHN_SW_KEY_BUDGET_GB");
+    const double gb = e ? atof(e) : 32.0;
+    const unsigned long long budget = (unsigned long long)(gb * 1e9 / 8.0);
+    cap = std::max<unsigned long long>(std::min(all, budget), 1024ull);
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (cap > c->keys_cap) {
+      dev_free(c->d_keys);
+      c->keys_cap = 0;
+      if (hipMalloc((void **)&c->d_keys, cap * sizeof(unsigned long long)) != hipSuccess) {
+        c->d_keys = nullptr;
+        return MNHN_ERR_NOMEM;
+      }
+      c->keys_cap = cap;
+    }
+    HIP_TRY(hipMemsetAsync(c->d_nkeys, 0, sizeof(unsigned long long), c->stream));
+    rc = launch_tiles(c, w, MODE_EPS, 1, eps_max, nullptr, c->d_keys, c->keys_cap);
+    if (rc) return rc;
+    unsigned long long nk = 0;
+    HIP_TRY(hipMemcpy(&nk, c->d_nkeys, sizeof(nk), hipMemcpyDeviceToHost));
+    if (nk <= c->keys_cap) {
+      *nkeys_out = nk;
+      return MNHN_OK;
+    }
+    cap = nk;  // overflowed: grow to the exact size and recompute once
+  }
+  return MNHN_ERR_NOMEM;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mnhn_sw_eps_count(mnhn_sw_ctx *c, int32_t eps_max, int64_t *nnz_out) {
+  if (!c || !nnz_out) return MNHN_ERR_ARG;
+  unsigned long long nk = 0;
+  int rc = eps_pass(c, eps_max, &nk);
+  if (rc == MNHN_OK) *nnz_out = (int64_t)nk;
+  return rc;
+}
+
+int mnhn_sw_eps_csr(mnhn_sw_ctx *c, int32_t eps_max, mnhn_csr *out) {
+  if (!c || !out) return MNHN_ERR_ARG;
+  memset(out, 0, sizeof(*out));
+  const int n = c->n;
+  out->n_rows = n;
+  out->eps_max = eps_max;
+  out->row_ptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  if (!out->row_ptr) return MNHN_ERR_NOMEM;
+  if (n == 0) return MNHN_OK;
+  unsigned long long nk = 0;
+  int rc = eps_pass(c, eps_max, &nk);
+  if (rc) { mnhn_csr_free(out); return rc; }
+  const double kernel_ms = c->last.kernel_ms;
+  (void)hipEventRecord(c->ev[2], c->stream);
+  // sort keys by (row, col): bits 16..63
+  unsigned long long *d_sorted = nullptr;
+  int32_t *d_col = nullptr, *d_cnt = nullptr;
+  int16_t *d_dist = nullptr;
+  void *d_tmp = nullptr;
+  size_t tmp_bytes = 0;
+  auto cleanup = [&]() {
+    dev_free(d_sorted); dev_free(d_col); dev_free(d_cnt); dev_free(d_dist);
+    if (d_tmp) (void)hipFree(d_tmp);
+    d_tmp = nullptr;
+  };
+  if ((rc = dev_alloc(&d_sorted, nk)) || (rc = dev_alloc(&d_col, nk)) ||
+      (rc = dev_alloc(&d_dist, nk)) || (rc = dev_alloc(&d_cnt, (size_t)n))) {
+    cleanup(); mnhn_csr_free(out); return rc;
+  }
+  hipError_t e = rocprim::radix_sort_keys(nullptr, tmp_bytes, c->d_keys, d_sorted, (size_t)nk,
+                                          16, 64, c->stream);
+  if (e == hipSuccess) e = hipMalloc(&d_tmp, std::max<size_t>(tmp_bytes, 1));
+  if (e == hipSuccess)
+    e = rocprim::radix_sort_keys(d_tmp, tmp_bytes, c->d_keys, d_sorted, (size_t)nk, 16, 64,
+                                 c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, (size_t)n * sizeof(int32_t), c->stream);
+  if (e == hipSuccess && nk > 0) {
+    const int tb = 256;
+    const long long blocks = ((long long)nk + tb - 1) / tb;
+    hipLaunchKernelGGL(decode_keys_kernel, dim3((unsigned)blocks), dim3(tb), 0, c->stream,
+                       d_sorted, (long long)nk, d_col, d_dist, d_cnt);
+    e = hipGetLastError();
+  }
+  (void)hipEventRecord(c->ev[3], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[3]);
+  float post_ms = 0.f;
+  if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
+  std::vector<int32_t> cnt(n);
+  if (e == hipSuccess) {
+    out->col = (int32_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
+    out->dist = (int16_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
+    if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
+    e = hipMemcpy(out->col, d_col, nk * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out->dist, d_dist, nk * sizeof(int16_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(cnt.data(), d_cnt, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  }
+  cleanup();
+  if (e != hipSuccess) { mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+  for (int i = 0; i < n; ++i) out->row_ptr[i + 1] = out->row_ptr[i] + cnt[i];
+  out->nnz = (int64_t)nk;
+  c->last.kernel_ms = kernel_ms;
+  c->last.total_ms = kernel_ms + post_ms;
+  return MNHN_OK;
+}
+
+void mnhn_csr_free(mnhn_csr *csr) {
+  if (!csr) return;
+  free(csr->row_ptr);
+  free(csr->col);
+  free(csr->dist);
+  csr->row_ptr = nullptr;
+  csr->col = nullptr;
+  csr->dist = nullptr;
+  csr->nnz = 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,357 @@
+"""CPU restatement of the reference's DBSCAN / adaptive-clustering path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, and only as the checker.  The shipped library
+never imports it.
+
+Every function names the reference lines it restates (paths are relative to
+the reference repository root).  Distances come either from the C restatement
+of smith-waterman.c (oracle/liboracle_sw.so, via ``sw_matrix``) or from a
+caller-supplied matrix, so the clustering logic here can be checked against
+golden files written by the compiled reference and then used to check the
+product's clustering on inputs too large for the reference to run.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def _lib():
+    """Load oracle/liboracle_sw.so (built by ``make -C oracle oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle_sw.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle oracle`")
+        lib = ctypes.CDLL(path)
+        lib.oracle_sw_distance.restype = ctypes.c_uint
+        lib.oracle_sw_distance.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.c_size_t, ctypes.c_size_t]
+        lib.oracle_sw_pairs.restype = ctypes.c_int
+        lib.oracle_sw_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.c_void_p, ctypes.c_int]
+        lib.oracle_sw_matrix.restype = ctypes.c_int
+        lib.oracle_sw_matrix.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_int]
+        lib.oracle_cpu_allpairs.restype = ctypes.c_double
+        lib.oracle_cpu_allpairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_uint64)]
+        _LIB = lib
+    return _LIB
+
+
+class _SeqArgs:
+    """Keeps C views (char**, size_t*) of a list of byte strings alive."""
+
+    def __init__(self, seqs: Sequence[bytes]):
+        self.bufs = [ctypes.create_string_buffer(s, len(s) + 1) for s in seqs]
+        self.ptrs = (ctypes.c_char_p * len(seqs))(
+            *[ctypes.cast(b, ctypes.c_char_p) for b in self.bufs])
+        self.lens = (ctypes.c_size_t * len(seqs))(*[len(s) for s in seqs])
+
+
+def sw_distance(a: bytes, b: bytes) -> int:
+    """smith-waterman.c:16-81 -- one pair."""
+    return int(_lib().oracle_sw_distance(a, b, len(a), len(b)))
+
+
+def sw_pairs(seqs: Sequence[bytes], ij: np.ndarray, nthreads: int = 8) -> np.ndarray:
+    """d(seqs[i], seqs[j]) for each row (i, j) of ij."""
+    ij = np.ascontiguousarray(ij, dtype=np.int32).reshape(-1, 2)
+    out = np.zeros(len(ij), dtype=np.int32)
+    sa = _SeqArgs(seqs)
+    _lib().oracle_sw_pairs(sa.ptrs, sa.lens, ij.ctypes.data, len(ij),
+                           out.ctypes.data, nthreads)
+    return out
+
+
+def sw_matrix(seqs: Sequence[bytes], nthreads: int = 8) -> np.ndarray:
+    """Full N x N distance matrix (unique pairs computed once, mirrored)."""
+    n = len(seqs)
+    out = np.zeros((n, n), dtype=np.int32)
+    sa = _SeqArgs(seqs)
+    _lib().oracle_sw_matrix(sa.ptrs, sa.lens, n, out.ctypes.data, nthreads)
+    return out
+
+
+def cpu_allpairs(seqs: Sequence[bytes], nthreads: int, ordered: bool):
+    """Time the restated SW over all pairs: returns (seconds, checksum)."""
+    sa = _SeqArgs(seqs)
+    cs = ctypes.c_uint64(0)
+    t = _lib().oracle_cpu_allpairs(sa.ptrs, sa.lens, len(seqs), nthreads,
+                                   1 if ordered else 0, None, ctypes.byref(cs))
+    return float(t), int(cs.value)
+
+
+# --------------------------------------------------------------------------
+# FASTA reader: dataset.c:264-344
+# --------------------------------------------------------------------------
+
+def read_fasta(path: str) -> List[bytes]:
+    """dataset_from_fasta: a '>' line opens a record; any other line adds its
+    first whitespace-delimited token (<2000 chars), uppercased.  A line with
+    no token re-adds the previous token, because sscanf leaves the line
+    buffer unchanged (dataset.c:316)."""
+    seqs: List[bytearray] = []
+    last = b""
+    with open(path, "rb") as f:
+        for line in f:
+            if line[:1] == b">":
+                seqs.append(bytearray())
+                continue
+            toks = line.split()
+            if toks:
+                last = toks[0].upper()
+            if not seqs:
+                continue  # text before the first header is undefined there
+            seqs[-1] += last
+    return [bytes(s) for s in seqs]
+
+
+# --------------------------------------------------------------------------
+# DBSCAN: dbscan.c:783-1047
+# --------------------------------------------------------------------------
+
+@dataclass
+class Cluster:
+    members: List[int]
+    id: int
+
+
+@dataclass
+class SplitSet:
+    clusters: List[Cluster] = field(default_factory=list)
+
+    @property
+    def n_clusters(self) -> int:
+        return len(self.clusters)
+
+
+def region_query_from_matrix(dist: np.ndarray) -> Callable[[int, float], List[int]]:
+    """region_query (dbscan.c:525-561): ascending j with (float)d <= eps."""
+    df = dist.astype(np.float32)
+
+    def rq(point: int, eps: float) -> List[int]:
+        return np.nonzero(df[point] <= np.float32(eps))[0].tolist()
+
+    return rq
+
+
+def dbscan(n: int, eps: float, minpts: int,
+           region_query: Callable[[int, float], List[int]],
+           out_lines: Optional[List[str]] = None) -> SplitSet:
+    """dbscan_SW (dbscan.c:950-1047) with expand_cluster (dbscan.c:783-948).
+
+    The queue append of expand_cluster (dbscan.c:892-912) is restated with its
+    intended meaning: the members of N(q) not yet in the sorted union, in
+    ascending order."""
+    eps = float(np.float32(eps))
+    if eps == -1:  # dbscan.c:980-984
+        if out_lines is not None:
+            out_lines.append("Warning @dbscan: epsilon value is: %f\n" % eps)
+        eps = float(np.float32(0.015))
+        if out_lines is not None:
+            out_lines.append("Warning @dbscan: epsilon was changed to: %f\n" % eps)
+    if minpts == -1:  # dbscan.c:985-989
+        if out_lines is not None:
+            out_lines.append("Warning @dbscan: minpts value is: %i\n" % minpts)
+        minpts = 5
+        if out_lines is not None:
+            out_lines.append("Warning @dbscan: minpts was changed to: %i\n" % minpts)
+    visited = np.zeros(n, dtype=bool)
+    member = np.zeros(n, dtype=bool)
+    clusters: List[Cluster] = []
+    for i in range(n):
+        if visited[i]:
+            continue
+        visited[i] = True
+        nb = region_query(i, eps)
+        if minpts < len(nb):  # strict seed test, dbscan.c:1011
+            cl = Cluster(members=[i], id=len(clusters))
+            member[i] = True
+            union = sorted(nb)
+            in_union = set(union)
+            queue = list(nb)
+            k = 0
+            while k < len(queue):
+                q = queue[k]
+                if not visited[q]:
+                    visited[q] = True
+                    nbq = region_query(q, eps)
+                    if len(nbq) >= minpts:  # expansion test, dbscan.c:847
+                        for x in nbq:
+                            if x not in in_union:
+                                in_union.add(x)
+                                queue.append(x)
+                if not member[q]:  # dbscan.c:934-941
+                    cl.members.append(q)
+                    member[q] = True
+                k += 1
+            clusters.append(cl)
+    return SplitSet(clusters)
+
+
+# --------------------------------------------------------------------------
+# split-set relations / coverage / formats: cluster_io.c
+# --------------------------------------------------------------------------
+
+def data_not_in_clusters(s: SplitSet, n: int) -> int:
+    """cluster_io.c:564-593 -- count of points in no cluster."""
+    covered = np.zeros(n, dtype=bool)
+    for c in s.clusters:
+        covered[c.members] = True
+    return int(n - covered.sum())
+
+
+def intersection_count(a: Cluster, b: Cluster) -> int:
+    """intersection_of_clusters (cluster_io.c:447-501), including its side
+    effect: the larger cluster (a on ties) is sorted in place when it has
+    more than 20 members (cluster_io.c:467-469)."""
+    if len(a.members) >= len(b.members):
+        big, small = a, b
+    else:
+        big, small = b, a
+    if len(big.members) > 20:
+        big.members.sort()
+    bs = set(big.members)
+    return sum(1 for x in small.members if x in bs)
+
+
+def split_set_relation(ancient: SplitSet, new: SplitSet) -> List[List[int]]:
+    """generate_split_set_relation (cluster_io.c:407-445): new cluster i is
+    connected to ancient cluster j when |A_j & N_i| > 0.8 |A_j|."""
+    conns = []
+    for nc in new.clusters:
+        row = []
+        for j, ac in enumerate(ancient.clusters):
+            inter = intersection_count(ac, nc)
+            if float(inter) > float(len(ac.members)) * 0.8:
+                row.append(j)
+        conns.append(row)
+    return conns
+
+
+def split_set_bytes(s: SplitSet) -> bytes:
+    """store_split_set (cluster_io.c:638-669) byte layout, little endian."""
+    out = bytearray(b"SPLIT_SET")
+    out += struct.pack("<i", s.n_clusters)
+    for c in s.clusters:
+        out += struct.pack("<ii", len(c.members), c.id)
+        out += np.asarray(c.members, dtype="<i4").tobytes()
+    return bytes(out)
+
+
+def cluster_file_text(cl: Cluster, seqs: Sequence[bytes]) -> bytes:
+    """create_single_cluster_file (cluster_io.c:672-699): '>sequence_<idx>'
+    header and a newline before every 50th residue after the first."""
+    out = bytearray()
+    for m in cl.members:
+        out += b">sequence_%d\n" % m
+        s = seqs[m]
+        for k in range(len(s)):
+            if k != 0 and k % 50 == 0:
+                out += b"\n"
+            out += s[k:k + 1]
+        out += b"\n"
+    return bytes(out)
+
+
+def cluster_dbscan_sw(seqs: Sequence[bytes], eps: float, minpts: int,
+                      dist: Optional[np.ndarray] = None):
+    """cluster_dbscan_SW.c:25-71: returns (stdout text, {suffix: bytes})."""
+    if dist is None:
+        dist = sw_matrix(seqs)
+    lines: List[str] = []
+    s = dbscan(len(seqs), eps, minpts, region_query_from_matrix(dist), lines)
+    lines.append("%u clusters obtained \n" % s.n_clusters)
+    files = {}
+    if s.n_clusters < 500:
+        for i, c in enumerate(s.clusters):
+            files["-%03d" % i] = cluster_file_text(c, seqs)
+    return "".join(lines), files, s
+
+
+def adaptive_clustering(seqs: Sequence[bytes], eps_start: float, eps_inc: float,
+                        minpts: int, dist: Optional[np.ndarray] = None,
+                        max_passes: int = 100000):
+    """adaptive_dbscan (dbscan.c:1117-1380) with one worker thread, and the
+    main() prints of adaptive_clustering.c.  Returns (stdout text,
+    {suffix: split-set bytes}, kept layers) or raises on the reference's
+    _exit(1) paths."""
+    if dist is None:
+        dist = sw_matrix(seqs)
+    n = len(seqs)
+    rq = region_query_from_matrix(dist)
+    f32 = np.float32
+    eps_start = f32(eps_start)
+    eps_inc = f32(eps_inc)
+    out: List[str] = ["Dataset read!\n"]
+    out.append("Performing adaptive clustering with parameters: \n"
+               "minpts: %i\nepsilon_start: %f\nepsilon_inc: %f\n"
+               % (minpts, float(eps_start), float(eps_inc)))
+    s0 = dbscan(n, float(eps_start), minpts, rq, out)
+    out.append("Initial set obtained with %i clusters\n" % s0.n_clusters)
+    tries = 0
+    while s0.n_clusters == 1:
+        if tries == 20 or eps_start == 0:
+            out.append("Error did not find a sufficient starting position in 20 tries \n")
+            raise RuntimeError("".join(out))
+        eps_start = f32(eps_start / f32(2))
+        out.append("Trying new starting point \n")
+        s0 = dbscan(n, float(eps_start), minpts, rq, out)
+        tries += 1
+    out.append("Sarting with %i clusters\n" % s0.n_clusters)
+    nc = data_not_in_clusters(s0, n)
+    out.append("Coverage at initial point: %f\n"
+               % float(f32(1) - f32(nc) / f32(n)))
+    # worker loop (dbscan.c:1061-1115): eps_now starts at eps_start + inc
+    sets = [s0]
+    eps_now = f32(eps_start + eps_inc)
+    passes = 0
+    while True:
+        eps = eps_now
+        eps_now = f32(eps_now + eps_inc)
+        s = dbscan(n, float(eps), minpts, rq, out)
+        sets.append(s)
+        passes += 1
+        if s.n_clusters == 1:
+            break
+        if passes >= max_passes:
+            raise RuntimeError("no single-cluster pass within max_passes")
+    stop = len(sets) - 1
+    kept = [s0]
+    conns = []
+    for i in range(1, stop + 1):
+        ns = sets[i]
+        if ns.n_clusters < kept[-1].n_clusters:
+            count = len(kept)
+            nc = data_not_in_clusters(ns, n)
+            out.append("Layer %i has %i clusters\n" % (count, ns.n_clusters))
+            out.append("  Coverage at layer %i: %f\n"
+                       % (count, float(f32(1) - f32(nc) / f32(n))))
+            out.append("  Epsilon at layer %i: %f\n"
+                       % (count, float(f32(eps_start + f32(i + 1) * eps_inc))))
+            conns.append(split_set_relation(kept[-1], ns))
+            kept.append(ns)
+    out.append("Connections found: \n")
+    files = {"%04d" % i: split_set_bytes(s) for i, s in enumerate(kept)}
+    for i in range(len(kept) - 1, 0, -1):
+        out.append("Layer %i:\n" % i)
+        for j in range(kept[i].n_clusters):
+            out.append("  Cluster %i connected to: \n  " % j)
+            out.append("".join("%i " % c for c in conns[i - 1][j]))
+            out.append("\n")
+    return "".join(out), files, kept

@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in a container where the reference sources exist (default
+/root/reference); it builds the reference's own C path with
+``make -C oracle ref`` (outputs in oracle/_ref/, never committed) and records:
+
+  pairs_golden.json       -- score() of the reference (smith-waterman.c:16-81)
+                             on random and adversarial pairs
+  bruteforce_ac6.json     -- score() on every pair of {A,C} strings of length
+                             0..6
+  inputs/*.fasta          -- the FASTA inputs of the CLI cases (the reference's
+                             own test2/3/4.fasta fixtures, a 300-record slice
+                             of its test-x.fasta, and seeded synthetic sets)
+  cli/<case>/             -- stdout and every output file of the reference's
+                             cluster_dbscan_SW / adaptive_clustering_SW run on
+                             that input (cmd.json names the arguments)
+
+The fixtures are data (inputs and expected outputs) only.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import random
+import shutil
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "mnhn-tree-tools_amd"))
+import synth  # noqa: E402
+
+
+def build_ref(ref_dir: str) -> None:
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"),
+                    f"REF={ref_dir}", "ref"], check=True)
+
+
+def ref_score_fn():
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_sw.so"))
+    lib.score.restype = ctypes.c_uint
+    lib.score.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                          ctypes.c_size_t, ctypes.c_void_p]
+    return lambda a, b: int(lib.score(a, b, len(a), len(b), None))
+
+
+def rand_seq(rng: random.Random, alphabet: str, lo: int, hi: int) -> bytes:
+    return "".join(rng.choice(alphabet) for _ in range(rng.randint(lo, hi))).encode()
+
+
+def make_pairs(score) -> None:
+    rng = random.Random(20261203)
+    pairs = []
+    # random pairs over several alphabets / length ranges
+    for alpha, lo, hi, n in [("ACGT", 0, 48, 1200), ("AC", 0, 24, 300),
+                             ("ACGTN-", 0, 48, 300), ("ACGT", 150, 320, 60),
+                             ("ACGTN", 280, 310, 20), ("ACGT", 590, 610, 8)]:
+        for _ in range(n):
+            pairs.append((rand_seq(rng, alpha, lo, hi), rand_seq(rng, alpha, lo, hi)))
+    # related pairs (mutated copies) where the local alignment is long
+    srng = synth._rng(7)
+    for L in [1, 2, 3, 16, 63, 64, 65, 100, 200, 299, 300, 301, 511, 512, 513, 600, 1000, 1500]:
+        a = synth.random_codes(srng, L)
+        b = synth.mutate(srng, a, 0.05, 0.01)
+        pairs.append((synth.to_bytes(a), synth.to_bytes(b)))
+        pairs.append((synth.to_bytes(a), synth.to_bytes(a)))
+    # adversarial: index-0 differences, terminator effects, repeats
+    adv = [(b"", b""), (b"", b"ACGT"), (b"A", b"C"), (b"A", b"A"), (b"AAAA", b"CCCC"),
+           (b"ACGT", b"TCGT"), (b"ACGT", b"ACG"), (b"N", b"N"), (b"NNNN", b"NNNN"),
+           (b"ACGTACGTACGT", b"TACGTACGTACG"), (b"A" * 300, b"A" * 299),
+           (b"AC" * 150, b"CA" * 150), (b"ACG" * 100, b"ACG" * 57),
+           (b"-" * 10, b"-" * 12), (b"acgt", b"ACGT")]
+    pairs.extend(adv)
+    out = [[a.decode(), b.decode(), score(a, b)] for a, b in pairs]
+    with open(os.path.join(HERE, "pairs_golden.json"), "w") as f:
+        json.dump({"source": "reference score(), smith-waterman.c:16-81",
+                   "pairs": out}, f, separators=(",", ":"))
+    print("pairs:", len(out))
+
+
+def make_bruteforce(score) -> None:
+    strs = [b""]
+    for L in range(1, 7):
+        for k in range(1 << L):
+            strs.append(bytes(b"AC"[(k >> i) & 1] for i in range(L)))
+    mat = [[score(a, b) for b in strs] for a in strs]
+    with open(os.path.join(HERE, "bruteforce_ac6.json"), "w") as f:
+        json.dump({"source": "reference score(), smith-waterman.c:16-81",
+                   "strings": [s.decode() for s in strs], "dist": mat}, f,
+                  separators=(",", ":"))
+    print("bruteforce strings:", len(strs))
+
+
+def slice_fasta(src: str, dst: str, n_records: int) -> None:
+    with open(src, "rb") as f, open(dst, "wb") as g:
+        count = 0
+        for line in f:
+            if line.startswith(b">"):
+                count += 1
+                if count > n_records:
+                    break
+            g.write(line)
+
+
+def stride_fasta(src: str, dst: str, stride: int, n_records: int) -> None:
+    """Every stride-th record of src (records 0, stride, 2*stride, ...)."""
+    records, cur = [], None
+    with open(src, "rb") as f:
+        for line in f:
+            if line.startswith(b">"):
+                cur = [line]
+                records.append(cur)
+            elif cur is not None:
+                cur.append(line)
+    with open(dst, "wb") as g:
+        for rec in records[::stride][:n_records]:
+            g.write(b"".join(rec))
+
+
+def make_inputs(ref_dir: str) -> dict:
+    d = os.path.join(HERE, "inputs")
+    os.makedirs(d, exist_ok=True)
+    for name in ["test2.fasta", "test3.fasta", "test4.fasta"]:
+        shutil.copyfile(os.path.join(ref_dir, name), os.path.join(d, name))
+    slice_fasta(os.path.join(ref_dir, "test-x.fasta"), os.path.join(d, "testx_300.fasta"), 300)
+    stride_fasta(os.path.join(ref_dir, "test-x.fasta"), os.path.join(d, "testx_s33.fasta"), 33, 300)
+    synth.write_fasta(os.path.join(d, "c1_small.fasta"),
+                      synth.config1(seed=11, n_anc=8, per_anc=40))
+    synth.write_fasta(os.path.join(d, "c2_small.fasta"),
+                      synth.config2(seed=12, n_roots=8, n_children=4, n_leaves=8))
+    synth.write_fasta(os.path.join(d, "c1_full.fasta"), synth.config1(seed=1))
+    return {n: os.path.join(d, n) for n in os.listdir(d)}
+
+
+# (case, program, input, args after the FASTA path)
+CASES = [
+    ("dbscan_test2", "cluster_dbscan_SW", "test2.fasta", ["16", "1"]),
+    ("dbscan_test3", "cluster_dbscan_SW", "test3.fasta", ["12", "1"]),
+    ("dbscan_test4", "cluster_dbscan_SW", "test4.fasta", ["10", "1"]),
+    ("dbscan_testx300_e40", "cluster_dbscan_SW", "testx_300.fasta", ["40", "5"]),
+    ("dbscan_testx300_e25", "cluster_dbscan_SW", "testx_300.fasta", ["25", "3"]),
+    ("dbscan_c1small_e400", "cluster_dbscan_SW", "c1_small.fasta", ["400", "5"]),
+    ("dbscan_c1small_e160", "cluster_dbscan_SW", "c1_small.fasta", ["160", "5"]),
+    ("dbscan_c2small_e300", "cluster_dbscan_SW", "c2_small.fasta", ["300", "5"]),
+    ("dbscan_c2small_defaults", "cluster_dbscan_SW", "c2_small.fasta", ["-1", "-1"]),
+    ("adaptive_test3", "adaptive_clustering_SW", "test3.fasta", ["4", "4", "1", "PREFIX", "1"]),
+    ("dbscan_testxs33_e8", "cluster_dbscan_SW", "testx_s33.fasta", ["8", "5"]),
+    ("adaptive_testxs33", "adaptive_clustering_SW", "testx_s33.fasta", ["2", "2", "5", "PREFIX", "1"]),
+    ("adaptive_c1small_halving", "adaptive_clustering_SW", "c1_small.fasta", ["1000", "50", "5", "PREFIX", "1"]),
+    ("adaptive_c1small", "adaptive_clustering_SW", "c1_small.fasta", ["100", "60", "5", "PREFIX", "1"]),
+    ("adaptive_c2small", "adaptive_clustering_SW", "c2_small.fasta", ["150", "300.5", "4", "PREFIX", "1"]),
+    ("dbscan_c1full_e400", "cluster_dbscan_SW", "c1_full.fasta", ["400", "5"]),
+]
+
+
+def run_case(case, inputs) -> str:
+    name, prog, inp, args = case
+    exe = os.path.join(ROOT, "oracle", "_ref", prog)
+    outdir = os.path.join(HERE, "cli", name)
+    if os.path.exists(os.path.join(outdir, "cmd.json")):
+        return name + " (kept)"
+    with tempfile.TemporaryDirectory() as tmp:
+        fasta = os.path.join(tmp, "in.fasta")
+        shutil.copyfile(inputs[inp], fasta)
+        if prog == "cluster_dbscan_SW":
+            argv = [exe, fasta] + args + ["out"]
+        else:
+            argv = [exe, fasta] + [a if a != "PREFIX" else "out" for a in args]
+        r = subprocess.run(argv, cwd=tmp, capture_output=True)
+        if os.path.exists(outdir):
+            shutil.rmtree(outdir)
+        os.makedirs(outdir)
+        files = sorted(f for f in os.listdir(tmp) if f.startswith("out"))
+        for f in files:
+            shutil.copyfile(os.path.join(tmp, f), os.path.join(outdir, f))
+        with open(os.path.join(outdir, "stdout.txt"), "wb") as g:
+            g.write(r.stdout)
+        with open(os.path.join(outdir, "cmd.json"), "w") as g:
+            json.dump({"program": prog, "input": inp, "args": args,
+                       "prefix": "out", "returncode": r.returncode,
+                       "files": files}, g, indent=1)
+    return name
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--jobs", type=int, default=7)
+    ap.add_argument("--skip-full", action="store_true",
+                    help="skip the 2,000-sequence case (~12 min single-thread)")
+    a = ap.parse_args()
+    build_ref(a.ref)
+    score = ref_score_fn()
+    make_pairs(score)
+    make_bruteforce(score)
+    inputs = make_inputs(a.ref)
+    cases = [c for c in CASES if not (a.skip_full and "full" in c[0])]
+    # longest first
+    cases.sort(key=lambda c: ("full" not in c[0], "adaptive" not in c[0]))
+    with ThreadPoolExecutor(a.jobs) as ex:
+        for name in ex.map(lambda c: run_case(c, inputs), cases):
+            print("done", name, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -29,10 +29,11 @@
 //  * Persistent grid: waves pull work items from an atomic queue (items are
 //    sorted by cost, longest first).
 //  * epsilon mode emits (row, col, d) keys for d <= eps_max for both
-//    directions of each unordered pair; the keys are radix-sorted on the
-//    device into ascending-column CSR rows.
+//    directions of each unordered pair.  A counting pass, an exclusive scan
+//    and a scatter group them by row; each row is then put in ascending
+//    column order by ranking its columns (comparison ranks for short rows,
+//    a column bitmap + popcount prefix in LDS for long ones).
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -333,15 +334,91 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
   }
 }
 
-// Decode sorted keys into CSR columns / distances and count rows.
-__global__ void decode_keys_kernel(const unsigned long long *keys, long long n,
-                                   int32_t *col, int16_t *dist, int32_t *row_cnt) {
-  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// ---- epsilon-list compaction: keys -> CSR rows in ascending column order ----
+
+__global__ void count_rows_kernel(const unsigned long long *keys, long long n, int32_t *cnt) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  atomicAdd(&cnt[(int)(keys[k] >> 40)], 1);
+}
+
+__global__ void scatter_rows_kernel(const unsigned long long *keys, long long n,
+                                    const long long *row_ptr, int32_t *cursor,
+                                    int32_t *seg_col, int16_t *seg_d) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const unsigned long long key = keys[k];
-  col[k] = (int32_t)((key >> 16) & 0xffffffull);
-  dist[k] = (int16_t)(key & 0xffffull);
-  atomicAdd(&row_cnt[(int)(key >> 40)], 1);
+  const int row = (int)(key >> 40);
+  const long long pos = row_ptr[row] + atomicAdd(&cursor[row], 1);
+  seg_col[pos] = (int32_t)((key >> 16) & 0xffffffull);
+  seg_d[pos] = (int16_t)(key & 0xffffull);
+}
+
+constexpr int kSortThreads = 256;
+constexpr int kShortRow = 256;
+
+// One workgroup per row: columns are distinct, so each entry's position in
+// the sorted row is the number of smaller columns in the row.
+__global__ __launch_bounds__(kSortThreads) void sort_rows_kernel(
+    const long long *row_ptr, const int32_t *seg_col, const int16_t *seg_d,
+    int32_t *out_col, int16_t *out_d, int n_cols) {
+  extern __shared__ uint32_t lds[];
+  const int row = blockIdx.x;
+  const long long b = row_ptr[row];
+  const int len = (int)(row_ptr[row + 1] - b);
+  const int tid = threadIdx.x;
+  if (len == 0) return;
+  if (len <= kShortRow) {
+    for (int t = tid; t < len; t += kSortThreads) lds[t] = (uint32_t)seg_col[b + t];
+    __syncthreads();
+    for (int t = tid; t < len; t += kSortThreads) {
+      const uint32_t c = lds[t];
+      int r = 0;
+      for (int m = 0; m < len; ++m) r += lds[m] < c ? 1 : 0;
+      out_col[b + r] = (int32_t)c;
+      out_d[b + r] = seg_d[b + t];
+    }
+    return;
+  }
+  const int W = (n_cols + 31) / 32;
+  uint32_t *bits = lds;
+  uint32_t *prefix = lds + W;
+  uint32_t *tot = lds + 2 * W;
+  for (int w = tid; w < W; w += kSortThreads) bits[w] = 0u;
+  __syncthreads();
+  for (int t = tid; t < len; t += kSortThreads) {
+    const uint32_t c = (uint32_t)seg_col[b + t];
+    atomicOr(&bits[c >> 5], 1u << (c & 31));
+  }
+  __syncthreads();
+  const int chunk = (W + kSortThreads - 1) / kSortThreads;
+  const int w0 = tid * chunk, w1 = min(W, w0 + chunk);
+  uint32_t sum = 0;
+  for (int w = w0; w < w1; ++w) sum += __popc(bits[w]);
+  tot[tid] = sum;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int t = 0; t < kSortThreads; ++t) {
+      const uint32_t v = tot[t];
+      tot[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  uint32_t run = tot[tid];
+  for (int w = w0; w < w1; ++w) {
+    prefix[w] = run;
+    run += __popc(bits[w]);
+  }
+  __syncthreads();
+  for (int t = tid; t < len; t += kSortThreads) {
+    const uint32_t c = (uint32_t)seg_col[b + t];
+    const uint32_t w = c >> 5;
+    const uint32_t r = prefix[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
+    out_col[b + r] = (int32_t)c;
+    out_d[b + r] = seg_d[b + t];
+  }
 }
 
 }  // namespace
@@ -963,52 +1040,67 @@ int mnhn_sw_eps_csr(mnhn_sw_ctx *c, int32_t eps_max, mnhn_csr *out) {
   int rc = eps_pass(c, eps_max, &nk);
   if (rc) { mnhn_csr_free(out); return rc; }
   const double kernel_ms = c->last.kernel_ms;
-  (void)hipEventRecord(c->ev[2], c->stream);
-  // sort keys by (row, col): bits 16..63
-  unsigned long long *d_sorted = nullptr;
-  int32_t *d_col = nullptr, *d_cnt = nullptr;
-  int16_t *d_dist = nullptr;
-  void *d_tmp = nullptr;
-  size_t tmp_bytes = 0;
+  // LDS of the long-row path: column bitmap + per-word prefix + thread totals
+  const size_t lds_bytes = std::max<size_t>(
+      (size_t)(2 * ((n + 31) / 32) + kSortThreads) * sizeof(uint32_t), kShortRow * sizeof(uint32_t));
+  if (lds_bytes > 160 * 1024) { mnhn_csr_free(out); return MNHN_ERR_RANGE; }
+  int32_t *d_cnt = nullptr, *d_seg_col = nullptr, *d_col = nullptr;
+  int16_t *d_seg_d = nullptr, *d_dist = nullptr;
+  long long *d_rp = nullptr;
   auto cleanup = [&]() {
-    dev_free(d_sorted); dev_free(d_col); dev_free(d_cnt); dev_free(d_dist);
-    if (d_tmp) (void)hipFree(d_tmp);
-    d_tmp = nullptr;
+    dev_free(d_cnt); dev_free(d_seg_col); dev_free(d_col); dev_free(d_seg_d); dev_free(d_dist);
+    dev_free(d_rp);
   };
-  if ((rc = dev_alloc(&d_sorted, nk)) || (rc = dev_alloc(&d_col, nk)) ||
-      (rc = dev_alloc(&d_dist, nk)) || (rc = dev_alloc(&d_cnt, (size_t)n))) {
+  if ((rc = dev_alloc(&d_cnt, (size_t)n)) || (rc = dev_alloc(&d_rp, (size_t)n + 1)) ||
+      (rc = dev_alloc(&d_seg_col, nk)) || (rc = dev_alloc(&d_seg_d, nk)) ||
+      (rc = dev_alloc(&d_col, nk)) || (rc = dev_alloc(&d_dist, nk))) {
     cleanup(); mnhn_csr_free(out); return rc;
   }
-  hipError_t e = rocprim::radix_sort_keys(nullptr, tmp_bytes, c->d_keys, d_sorted, (size_t)nk,
-                                          16, 64, c->stream);
-  if (e == hipSuccess) e = hipMalloc(&d_tmp, std::max<size_t>(tmp_bytes, 1));
+  const int tb = 256;
+  const unsigned kblocks = (unsigned)std::max<long long>(((long long)nk + tb - 1) / tb, 1);
+  std::vector<int32_t> cnt(n);
+  (void)hipEventRecord(c->ev[2], c->stream);
+  hipError_t e = hipMemsetAsync(d_cnt, 0, (size_t)n * sizeof(int32_t), c->stream);
+  if (e == hipSuccess && nk > 0) {
+    hipLaunchKernelGGL(count_rows_kernel, dim3(kblocks), dim3(tb), 0, c->stream,
+                       c->d_keys, (long long)nk, d_cnt);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess)
-    e = rocprim::radix_sort_keys(d_tmp, tmp_bytes, c->d_keys, d_sorted, (size_t)nk, 16, 64,
-                                 c->stream);
+    e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) {
+    for (int i = 0; i < n; ++i) out->row_ptr[i + 1] = out->row_ptr[i] + cnt[i];
+    e = hipMemcpyAsync(d_rp, out->row_ptr, ((size_t)n + 1) * sizeof(long long),
+                       hipMemcpyHostToDevice, c->stream);
+  }
   if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, (size_t)n * sizeof(int32_t), c->stream);
   if (e == hipSuccess && nk > 0) {
-    const int tb = 256;
-    const long long blocks = ((long long)nk + tb - 1) / tb;
-    hipLaunchKernelGGL(decode_keys_kernel, dim3((unsigned)blocks), dim3(tb), 0, c->stream,
-                       d_sorted, (long long)nk, d_col, d_dist, d_cnt);
+    hipLaunchKernelGGL(scatter_rows_kernel, dim3(kblocks), dim3(tb), 0, c->stream,
+                       c->d_keys, (long long)nk, d_rp, d_cnt, d_seg_col, d_seg_d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && lds_bytes > 64 * 1024)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e == hipSuccess && nk > 0) {
+    hipLaunchKernelGGL(sort_rows_kernel, dim3((unsigned)n), dim3(kSortThreads), lds_bytes,
+                       c->stream, d_rp, d_seg_col, d_seg_d, d_col, d_dist, n);
     e = hipGetLastError();
   }
   (void)hipEventRecord(c->ev[3], c->stream);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[3]);
   float post_ms = 0.f;
   if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
-  std::vector<int32_t> cnt(n);
   if (e == hipSuccess) {
     out->col = (int32_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
     out->dist = (int16_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
     if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
     e = hipMemcpy(out->col, d_col, nk * sizeof(int32_t), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(out->dist, d_dist, nk * sizeof(int16_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(cnt.data(), d_cnt, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
   }
   cleanup();
   if (e != hipSuccess) { mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
-  for (int i = 0; i < n; ++i) out->row_ptr[i + 1] = out->row_ptr[i] + cnt[i];
   out->nnz = (int64_t)nk;
   c->last.kernel_ms = kernel_ms;
   c->last.total_ms = kernel_ms + post_ms;

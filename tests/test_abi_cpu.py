@@ -1,0 +1,187 @@
+"""CPU tests of the C ABI library and its host code (no GPU calls):
+symbols, FASTA parsing, split-set and cluster-file formats, relations."""
+import ctypes
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            prev = text[:m.start()].rstrip()
+            if name in ("if", "sizeof", "defined") or prev.endswith("typedef"):
+                continue
+            if re.search(r"[\w\*]\s*$", prev) and not prev.endswith("#define"):
+                names.add(name)
+    names.discard("dbscanner")  # function-pointer parameter name
+    return names
+
+
+def test_headers_declare_the_exported_list(pkg):
+    assert _declared_functions() == set(pkg.ABI_SYMBOLS) | set(pkg.HOST_SYMBOLS)
+
+
+def test_library_loads_and_exports_every_symbol(pkg):
+    lib = pkg.load_library()
+    for name in pkg.ABI_SYMBOLS + pkg.HOST_SYMBOLS:
+        assert hasattr(lib, name), name
+    assert lib.mnhn_sw_strerror(0) == b"ok"
+    assert lib.mnhn_sw_strerror(5).startswith(b"score range")
+
+
+def test_create_rejects_bad_arguments(pkg):
+    lib = pkg.load_library()
+    out = ctypes.c_void_p()
+    assert lib.mnhn_sw_create(None, None, -1, None, ctypes.byref(out)) == 1
+    assert lib.mnhn_sw_create(None, None, 0, None, None) == 1
+
+
+# ---- host code through ctypes ------------------------------------------------
+
+class Dataset(ctypes.Structure):
+    _fields_ = [("sequences", ctypes.POINTER(ctypes.c_char_p)),
+                ("sequence_lengths", ctypes.POINTER(ctypes.c_size_t)),
+                ("binary_sequences", ctypes.c_void_p), ("n_values", ctypes.c_int),
+                ("values", ctypes.c_void_p), ("n_dimensions", ctypes.c_int),
+                ("max_sequence_length", ctypes.c_size_t)]
+
+
+class Cluster(ctypes.Structure):
+    _fields_ = [("members", ctypes.POINTER(ctypes.c_int)), ("n_members", ctypes.c_int),
+                ("id", ctypes.c_int)]
+
+
+class SplitSet(ctypes.Structure):
+    _fields_ = [("n_clusters", ctypes.c_int), ("clusters", ctypes.POINTER(Cluster))]
+
+
+class Conn(ctypes.Structure):
+    _fields_ = [("connections", ctypes.POINTER(ctypes.c_int)), ("n_connections", ctypes.c_int)]
+
+
+@pytest.fixture(scope="module")
+def host(pkg):
+    lib = pkg.load_library()
+    libc = ctypes.CDLL(None)
+    libc.fopen.restype = ctypes.c_void_p
+    libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    libc.fclose.argtypes = [ctypes.c_void_p]
+    lib.dataset_from_fasta.restype = Dataset
+    lib.dataset_from_fasta.argtypes = [ctypes.c_void_p]
+    lib.free_sequences_from_dataset.argtypes = [Dataset]
+    lib.store_split_set.argtypes = [ctypes.c_char_p, SplitSet]
+    lib.read_split_set.restype = SplitSet
+    lib.read_split_set.argtypes = [ctypes.c_char_p]
+    lib.create_cluster_files.argtypes = [ctypes.c_char_p, SplitSet, Dataset]
+    lib.generate_split_set_relation.restype = ctypes.POINTER(Conn)
+    lib.generate_split_set_relation.argtypes = [SplitSet, SplitSet]
+    return lib, libc
+
+
+def _read_c(host, path):
+    lib, libc = host
+    f = libc.fopen(path.encode(), b"r")
+    ds = lib.dataset_from_fasta(f)
+    libc.fclose(f)
+    seqs = [ds.sequences[i][:ds.sequence_lengths[i]] for i in range(ds.n_values)]
+    assert ds.max_sequence_length == max([len(s) for s in seqs], default=0)
+    lib.free_sequences_from_dataset(ds)
+    return seqs
+
+
+def test_fasta_reader_matches_oracle_on_inputs(host, oracle):
+    d = os.path.join(GOLDEN, "inputs")
+    for fn in sorted(os.listdir(d)):
+        assert _read_c(host, os.path.join(d, fn)) == oracle.read_fasta(os.path.join(d, fn)), fn
+
+
+def test_fasta_reader_edge_cases(host, oracle, tmp_path):
+    p = tmp_path / "edge.fasta"
+    p.write_bytes(b">a\nacgt  xx\n\n  tt\n>b\n>c\n\t\nGgNn-\n>d no newline at end\nAC")
+    got = _read_c(host, str(p))
+    assert got == oracle.read_fasta(str(p))
+    # blank / whitespace-only lines repeat the previous token (sscanf quirk)
+    assert got == [b"ACGTACGTTT", b"", b"TTGGNN-", b"AC"]
+
+
+def _mk_split(clusters):
+    arr = (Cluster * max(len(clusters), 1))()
+    keep = []
+    for k, (members, cid) in enumerate(clusters):
+        m = (ctypes.c_int * max(len(members), 1))(*members)
+        keep.append(m)
+        arr[k] = Cluster(ctypes.cast(m, ctypes.POINTER(ctypes.c_int)), len(members), cid)
+    s = SplitSet(len(clusters), ctypes.cast(arr, ctypes.POINTER(Cluster)))
+    return s, (arr, keep)
+
+
+def test_split_set_bytes_and_roundtrip(host, oracle, tmp_path):
+    lib, _ = host
+    cl = [([3, 1, 2], 0), ([], 1), (list(range(40, 10, -1)), 7)]
+    s, keep = _mk_split(cl)
+    path = str(tmp_path / "ss0000").encode()
+    lib.store_split_set(path, s)
+    data = open(path, "rb").read()
+    want = oracle.split_set_bytes(oracle.SplitSet([oracle.Cluster(m, i) for m, i in cl]))
+    assert data == want
+    assert data[:9] == b"SPLIT_SET" and struct.unpack("<i", data[9:13])[0] == 3
+    r = lib.read_split_set(path)
+    assert r.n_clusters == 3
+    assert [r.clusters[k].members[i] for k in range(3) for i in range(r.clusters[k].n_members)] == \
+        [x for m, _ in cl for x in m]
+    # no O_TRUNC: a longer pre-existing file keeps its tail
+    longer = tmp_path / "ss0001"
+    longer.write_bytes(b"Z" * (len(want) + 5))
+    lib.store_split_set(str(longer).encode(), s)
+    assert longer.read_bytes() == want + b"Z" * 5
+
+
+def test_cluster_files_match_oracle(host, oracle, tmp_path):
+    lib, libc = host
+    path = os.path.join(GOLDEN, "inputs", "c2_small.fasta")
+    f = libc.fopen(path.encode(), b"r")
+    ds = lib.dataset_from_fasta(f)
+    libc.fclose(f)
+    seqs = oracle.read_fasta(path)
+    cl = [([5, 0, 17, 100], 0), ([250, 3], 1)]
+    s, keep = _mk_split(cl)
+    prefix = str(tmp_path / "out").encode()
+    lib.create_cluster_files(prefix, s, ds)
+    for k, (m, i) in enumerate(cl):
+        got = open(str(tmp_path / ("out-%03d" % k)), "rb").read()
+        assert got == oracle.cluster_file_text(oracle.Cluster(m, i), seqs)
+    lib.free_sequences_from_dataset(ds)
+
+
+def test_relation_and_sort_side_effect(host, oracle):
+    lib, _ = host
+    rng = np.random.default_rng(1)
+    old = [rng.permutation(100)[:k].tolist() for k in (30, 15, 25)]
+    new = [old[0] + old[2][:10], old[1][:5]]
+    so, ko = _mk_split([(m, i) for i, m in enumerate(old)])
+    sn, kn = _mk_split([(m, i) for i, m in enumerate(new)])
+    conn = lib.generate_split_set_relation(so, sn)
+    got = [[conn[i].connections[k] for k in range(conn[i].n_connections)] for i in range(2)]
+    O = [oracle.Cluster(list(m), i) for i, m in enumerate(old)]
+    N = [oracle.Cluster(list(m), i) for i, m in enumerate(new)]
+    want = oracle.split_set_relation(oracle.SplitSet(O), oracle.SplitSet(N))
+    assert got == want
+    # member order after the call (in-place sorts of clusters > 20 members)
+    for k in range(3):
+        assert [so.clusters[k].members[i] for i in range(so.clusters[k].n_members)] == O[k].members
+    for k in range(2):
+        assert [sn.clusters[k].members[i] for i in range(sn.clusters[k].n_members)] == N[k].members

@@ -1,0 +1,67 @@
+"""GPU integration tests: the drop-in CLIs produce byte-identical stdout and
+output files to the compiled reference on every golden case."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, PKG_DIR, cli_cases, read_case_files
+
+pytestmark = pytest.mark.gpu
+
+CASES = cli_cases()
+
+
+def _run(name, meta, tmp_path, env_extra=None):
+    stdout, files = read_case_files(name, meta)
+    exe = os.path.join(PKG_DIR, "bin", meta["program"])
+    assert os.path.exists(exe), f"{exe} not built"
+    fasta = tmp_path / "in.fasta"
+    shutil.copyfile(os.path.join(GOLDEN, "inputs", meta["input"]), fasta)
+    if meta["program"] == "cluster_dbscan_SW":
+        argv = [exe, str(fasta)] + meta["args"] + [meta["prefix"]]
+    else:
+        argv = [exe, str(fasta)] + [a if a != "PREFIX" else meta["prefix"] for a in meta["args"]]
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    print(f"cli case {name}: {' '.join(argv[:1])}", flush=True)
+    r = subprocess.run(argv, cwd=tmp_path, capture_output=True, env=env, timeout=300)
+    assert r.returncode == meta["returncode"], r.stderr.decode()[-2000:]
+    assert r.stdout == stdout
+    got = sorted(f for f in os.listdir(tmp_path) if f.startswith(meta["prefix"]))
+    assert got == sorted(files)
+    for fn, data in files.items():
+        assert (tmp_path / fn).read_bytes() == data, fn
+
+
+@pytest.mark.parametrize("name,meta", CASES, ids=[c[0] for c in CASES])
+def test_cli_matches_reference(name, meta, tmp_path):
+    _run(name, meta, tmp_path)
+
+
+@pytest.mark.parametrize("name", ["dbscan_test2", "dbscan_c1small_e400", "adaptive_testxs33"])
+def test_cli_per_query_launch_mode(name, tmp_path):
+    """The reference-shaped flow: one one-vs-all launch per region query."""
+    meta = dict(cli_cases())[name]
+    _run(name, meta, tmp_path, {"MNHN_REGION_QUERY": "launch"})
+
+
+def test_split_set_no_truncate(tmp_path):
+    """A pre-existing longer split-set file keeps its tail bytes, as the
+    reference's open(O_WRONLY | O_CREAT) leaves it."""
+    name = "adaptive_c1small"
+    meta = dict(cli_cases())[name]
+    stdout, files = read_case_files(name, meta)
+    first = sorted(files)[0]
+    junk = b"X" * (len(files[first]) + 37)
+    (tmp_path / first).write_bytes(junk)
+    exe = os.path.join(PKG_DIR, "bin", meta["program"])
+    fasta = tmp_path / "in.fasta"
+    shutil.copyfile(os.path.join(GOLDEN, "inputs", meta["input"]), fasta)
+    argv = [exe, str(fasta)] + [a if a != "PREFIX" else meta["prefix"] for a in meta["args"]]
+    r = subprocess.run(argv, cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0
+    data = (tmp_path / first).read_bytes()
+    assert data[:len(files[first])] == files[first]
+    assert data[len(files[first]):] == junk[len(files[first]):]

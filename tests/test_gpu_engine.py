@@ -1,0 +1,178 @@
+"""GPU parity tests: the gfx950 engine (through its C ABI) against the
+reference's golden vectors and the oracle, bit-exact."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, os.path.join(os.path.dirname(GOLDEN), "..", "mnhn-tree-tools_amd"))
+import synth  # noqa: E402
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rand_set(rng, n, lo, hi, alphabet=b"ACGT"):
+    alpha = np.frombuffer(alphabet, np.uint8)
+    return [alpha[rng.integers(0, len(alpha), rng.integers(lo, hi + 1))].tobytes()
+            for _ in range(n)]
+
+
+@pytest.mark.parametrize("strip", [16, 32])
+def test_golden_pairs_acgt(pkg, golden_pairs, strip):
+    pairs = [(a.encode(), b.encode(), d) for a, b, d in golden_pairs
+             if set(a + b) <= set("ACGT")]
+    seqs = [p[0] for p in pairs] + [p[1] for p in pairs]
+    _say(f"golden ACGT pairs: {len(pairs)} strip {strip}")
+    with pkg.Engine(seqs) as eng:
+        assert eng.alphabet == 4
+        eng.set_tuning(strip=strip)
+        ij = np.array([[k, k + len(pairs)] for k in range(len(pairs))], np.int32)
+        got = eng.pairs(ij)
+        got_rev = eng.pairs(ij[:, ::-1])
+    want = np.array([p[2] for p in pairs])
+    assert np.array_equal(got, want)
+    assert np.array_equal(got_rev, want)
+
+
+@pytest.mark.parametrize("strip", [16, 32])
+def test_golden_pairs_all_alphabets(pkg, golden_pairs, strip):
+    pairs = [(a.encode(), b.encode(), d) for a, b, d in golden_pairs]
+    seqs = [p[0] for p in pairs] + [p[1] for p in pairs]
+    _say(f"golden pairs (generic alphabet): {len(pairs)} strip {strip}")
+    with pkg.Engine(seqs) as eng:
+        assert eng.alphabet == 0
+        eng.set_tuning(strip=strip)
+        ij = np.array([[k, k + len(pairs)] for k in range(len(pairs))], np.int32)
+        got = eng.pairs(ij)
+    assert np.array_equal(got, np.array([p[2] for p in pairs]))
+
+
+def test_bruteforce_ac6_matrix(pkg, golden_bruteforce):
+    strs = [s.encode() for s in golden_bruteforce["strings"]]
+    want = np.array(golden_bruteforce["dist"], dtype=np.int32)
+    _say(f"bruteforce matrix n={len(strs)}")
+    with pkg.Engine(strs) as eng:
+        got = eng.matrix()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("alphabet", [b"ACGT", b"ACGTN"])
+@pytest.mark.parametrize("strip", [16, 32])
+def test_matrix_vs_oracle_mixed_lengths(pkg, oracle, alphabet, strip):
+    rng = np.random.default_rng(7 + strip)
+    seqs = _rand_set(rng, 260, 0, 700, alphabet)
+    # related sequences so that long local alignments occur
+    base = _rand_set(rng, 4, 300, 700, b"ACGT")
+    for b in base:
+        for _ in range(10):
+            seqs.append(synth.to_bytes(synth.mutate(
+                rng, np.frombuffer(b, np.uint8).copy() % 4, 0.04, 0.01)))
+    _say(f"matrix vs oracle n={len(seqs)} alphabet={alphabet} strip={strip}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    with pkg.Engine(seqs) as eng:
+        eng.set_tuning(strip=strip)
+        got = eng.matrix()
+        row = eng.one_vs_all(17)
+        row2 = eng.one_vs_all(len(seqs) - 1)
+    assert np.array_equal(got, want)
+    assert np.array_equal(row, want[17])
+    assert np.array_equal(row2, want[-1])
+
+
+def test_pairs_random_order_and_duplicates(pkg, oracle):
+    rng = np.random.default_rng(11)
+    seqs = _rand_set(rng, 500, 1, 400)
+    ij = rng.integers(0, len(seqs), size=(3000, 2)).astype(np.int32)
+    ij[:50, 1] = ij[:50, 0]  # self pairs
+    _say("pairs: 3000 random pairs")
+    with pkg.Engine(seqs) as eng:
+        got = eng.pairs(ij)
+    want = oracle.sw_pairs(seqs, ij, nthreads=16)
+    assert np.array_equal(got, want)
+    assert np.all(got[:50] == 0)
+
+
+def _csr_to_dense(rp, col, dist, n):
+    m = np.full((n, n), -1, dtype=np.int32)
+    for i in range(n):
+        m[i, col[rp[i]:rp[i + 1]]] = dist[rp[i]:rp[i + 1]]
+    return m
+
+
+@pytest.mark.parametrize("eps", [0, 150, 400, 100000])
+def test_eps_csr_vs_oracle(pkg, oracle, eps):
+    seqs = synth.config1(seed=21, n_anc=6, per_anc=50)
+    _say(f"eps csr n={len(seqs)} eps={eps}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    with pkg.Engine(seqs) as eng:
+        rp, col, dist = eng.eps_csr(eps)
+    n = len(seqs)
+    assert rp[0] == 0 and rp[-1] == len(col)
+    for i in range(n):
+        c = col[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(c) > 0)          # ascending, no duplicates
+        assert not np.any(c == i)              # self not stored
+    dense = _csr_to_dense(rp, col, dist, n)
+    expect = np.where(want <= eps, want, -1)
+    np.fill_diagonal(expect, -1)
+    assert np.array_equal(dense, expect)
+
+
+def test_config1_full_matrix_and_eps(pkg, oracle):
+    seqs = synth.config1(seed=1)
+    n = len(seqs)
+    _say(f"config1 full n={n}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    with pkg.Engine(seqs) as eng:
+        got = eng.matrix()
+        rp, col, dist = eng.eps_csr(400)
+        t = eng.last_timing()
+    assert np.array_equal(got, want)
+    dense = _csr_to_dense(rp, col, dist, n)
+    expect = np.where(want <= 400, want, -1)
+    np.fill_diagonal(expect, -1)
+    assert np.array_equal(dense, expect)
+    _say(f"config1 eps kernel {t['kernel_ms']:.2f} ms, {t['cells'] / t['kernel_ms'] / 1e6:.1f} GCUPS")
+
+
+def test_config2_full_size_properties(pkg, oracle):
+    """Full 20,000-sequence uneven-length set: size-independent properties
+    plus oracle checks on sampled pairs (neighbours and non-neighbours)."""
+    seqs = synth.config2()
+    n = len(seqs)
+    eps = 600
+    _say(f"config2 full n={n}")
+    with pkg.Engine(seqs) as eng:
+        rp, col, dist = eng.eps_csr(eps)
+        t = eng.last_timing()
+        _say(f"config2 eps kernel {t['kernel_ms']:.1f} ms, {t['cells'] / t['kernel_ms'] / 1e6:.1f} GCUPS, nnz={len(col)}")
+        rng = np.random.default_rng(3)
+        ij = rng.integers(0, n, size=(4000, 2)).astype(np.int32)
+        d_pairs = eng.pairs(ij)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert np.all(dist <= eps) and np.all(dist >= 0)
+    assert not np.any(rows == col)
+    # symmetric: (i, j, d) present iff (j, i, d) present
+    key = rows.astype(np.int64) * n + col
+    key_t = col.astype(np.int64) * n + rows
+    o1, o2 = np.argsort(key), np.argsort(key_t)
+    assert np.array_equal(key[o1], key_t[o2])
+    assert np.array_equal(dist[o1], dist[o2])
+    # sampled CSR entries are exact
+    pick = rng.choice(len(col), size=min(3000, len(col)), replace=False)
+    ij2 = np.stack([rows[pick], col[pick]], 1).astype(np.int32)
+    assert np.array_equal(dist[pick], oracle.sw_pairs(seqs, ij2, nthreads=16))
+    # random pairs: exact, and present in the CSR iff d <= eps
+    want = oracle.sw_pairs(seqs, ij, nthreads=16)
+    assert np.array_equal(d_pairs, want)
+    keyset = set(key.tolist())
+    for (i, j), d in zip(ij.tolist(), want.tolist()):
+        if i != j:
+            assert ((i * n + j) in keyset) == (d <= eps)

@@ -63,6 +63,7 @@ typedef struct {
   double cells;       /* algorithmic cells: sum of la*lb over computed pairs */
   double pairs;       /* pairs computed */
   int32_t lane_bits;  /* 16 (packed) or 32 */
+  int32_t kernel;     /* 1 = register-profile kernel, 2 = LDS-profile kernel */
   int32_t strip;      /* register strip width used */
   int32_t waves;      /* persistent waves launched */
   int32_t items;      /* work items in the queue */
@@ -114,9 +115,11 @@ int mnhn_sw_alphabet(const mnhn_sw_ctx *ctx);
 /* Timing of the last one_vs_all / pairs / matrix / eps call. */
 int mnhn_sw_last_timing(const mnhn_sw_ctx *ctx, mnhn_sw_timing *out);
 
-/* Force the register-strip width (16 or 32) and waves per CU used by later
- * calls; 0 restores the tuned defaults. */
-int mnhn_sw_set_tuning(mnhn_sw_ctx *ctx, int strip, int waves_per_cu);
+/* Force the kernel variant (1 = register profile, any alphabet; 2 = LDS
+ * profile, <= 4 residue codes), register-strip width (16, 32 or 64; 64 only
+ * for variant 2) and waves per CU used by later calls; 0 restores the tuned
+ * defaults. */
+int mnhn_sw_set_tuning(mnhn_sw_ctx *ctx, int kernel, int strip, int waves_per_cu);
 
 const char *mnhn_sw_strerror(int code);
 

@@ -63,7 +63,8 @@ class Csr(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("cells", ctypes.c_double), ("pairs", ctypes.c_double),
-                ("lane_bits", ctypes.c_int32), ("strip", ctypes.c_int32),
+                ("lane_bits", ctypes.c_int32), ("kernel", ctypes.c_int32),
+                ("strip", ctypes.c_int32),
                 ("waves", ctypes.c_int32), ("items", ctypes.c_int32)]
 
 
@@ -93,7 +94,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_sw_size.argtypes = [vp]
     lib.mnhn_sw_alphabet.argtypes = [vp]
     lib.mnhn_sw_last_timing.argtypes = [vp, ctypes.POINTER(Timing)]
-    lib.mnhn_sw_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.mnhn_sw_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.mnhn_sw_strerror.argtypes = [ctypes.c_int]
     lib.mnhn_sw_strerror.restype = ctypes.c_char_p
     _LIB = lib
@@ -145,8 +146,9 @@ class Engine:
     def alphabet(self) -> int:
         return int(self.lib.mnhn_sw_alphabet(self.ctx))
 
-    def set_tuning(self, strip: int = 0, waves_per_cu: int = 0) -> None:
-        _check(self.lib.mnhn_sw_set_tuning(self.ctx, strip, waves_per_cu), "set_tuning")
+    def set_tuning(self, kernel: int = 0, strip: int = 0, waves_per_cu: int = 0) -> None:
+        """kernel: 0 auto, 1 register profile, 2 LDS profile; strip 16/32/64."""
+        _check(self.lib.mnhn_sw_set_tuning(self.ctx, kernel, strip, waves_per_cu), "set_tuning")
 
     def one_vs_all(self, query: int) -> np.ndarray:
         out = np.empty(self.n, dtype=np.int32)

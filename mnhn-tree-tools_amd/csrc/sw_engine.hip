@@ -66,6 +66,7 @@ struct KArgs {
   const int32_t *rank;     // position of each sequence in length order
   const int32_t *gtgt;     // [groups][128] target ids, -1 = empty slot
   const int32_t *gmax;     // [groups] longest target of the group
+  const int32_t *gmin;     // [groups] shortest non-empty target of the group
   const int4 *items;       // (group, q_begin, q_end, out_base)
   const int32_t *qids;     // query ids referenced by item ranges
   int *counter;            // work-queue head
@@ -138,10 +139,61 @@ __device__ __forceinline__ void dp_row_codes(s16x2 (&H)[S], const uint32_t (&TC)
   }
 }
 
+// Distances of the lane's two targets against query q and their output:
+// MODE_DIST writes them, MODE_EPS emits keys for d <= eps_max.
+__device__ __forceinline__ void finish_query(const KArgs &a, int lane, int q, int qrel, int ob,
+                                             int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
+                                             int rk_lo, int rk_hi, s16x2 best, s16x2 corner) {
+  const int best_lo = (int)best.x + kBias, best_hi = (int)best.y + kBias;
+  const int cor_lo = (int)corner.x + kBias, cor_hi = (int)corner.y + kBias;
+  int tot_lo = 0, tot_hi = 0;
+  if (la >= 1 && lb_lo >= 1) tot_lo = max(best_lo, cor_lo + a.match);
+  if (la >= 1 && lb_hi >= 1) tot_hi = max(best_hi, cor_hi + a.match);
+  const int d_lo = a.match * max(la, lb_lo) - tot_lo;
+  const int d_hi = a.match * max(la, lb_hi) - tot_hi;
+
+  const int rq = a.triangle ? a.rank[q] : 0;
+  const bool ok_lo = t_lo >= 0 && (!a.triangle || rq < rk_lo);
+  const bool ok_hi = t_hi >= 0 && (!a.triangle || rq < rk_hi);
+  if (a.mode == MODE_DIST) {
+    int32_t *o = a.dist_out + (size_t)ob + (size_t)qrel * kGroup;
+    if (ok_lo) o[lane] = d_lo;
+    if (ok_hi) o[kWave + lane] = d_hi;
+    return;
+  }
+  const bool h_lo = ok_lo && d_lo <= a.eps_max;
+  const bool h_hi = ok_hi && d_hi <= a.eps_max;
+  const unsigned long long b_lo = __ballot(h_lo);
+  const unsigned long long b_hi = __ballot(h_hi);
+  const int total = 2 * (__popcll(b_lo) + __popcll(b_hi));
+  if (total == 0) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(a.nkeys, (unsigned long long)total);
+  base = __builtin_amdgcn_readfirstlane((unsigned)base) |
+         ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
+  const unsigned mlo = __builtin_amdgcn_mbcnt_hi((unsigned)(b_lo >> 32),
+                        __builtin_amdgcn_mbcnt_lo((unsigned)b_lo, 0u));
+  const unsigned mhi = __builtin_amdgcn_mbcnt_hi((unsigned)(b_hi >> 32),
+                        __builtin_amdgcn_mbcnt_lo((unsigned)b_hi, 0u));
+  unsigned long long pos = base + 2ull * (mlo + mhi);
+  const unsigned long long uq = (unsigned long long)q;
+  if (h_lo) {
+    const unsigned long long ut = (unsigned long long)t_lo, ud = (unsigned)d_lo;
+    if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
+    if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
+    pos += 2;
+  }
+  if (h_hi) {
+    const unsigned long long ut = (unsigned long long)t_hi, ud = (unsigned)d_hi;
+    if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
+    if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
+  }
+}
+
 template <int S, bool kAcgt>
 __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6));
   // the packed residues are read-only for the whole launch: reading them
   // through the constant address space lets uniform loads use s_load
   const __attribute__((address_space(4))) uint32_t *cwords =
@@ -284,52 +336,159 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
         corner = as_s16x2(cu);
       }
 
-      // ---- distances of the two targets
-      const int best_lo = (int)best.x + kBias, best_hi = (int)best.y + kBias;
-      const int cor_lo = (int)corner.x + kBias, cor_hi = (int)corner.y + kBias;
-      int tot_lo = 0, tot_hi = 0;
-      if (la >= 1 && lb_lo >= 1) tot_lo = max(best_lo, cor_lo + a.match);
-      if (la >= 1 && lb_hi >= 1) tot_hi = max(best_hi, cor_hi + a.match);
-      const int d_lo = a.match * max(la, lb_lo) - tot_lo;
-      const int d_hi = a.match * max(la, lb_hi) - tot_hi;
+      finish_query(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+                   best, corner);
+    }
+  }
+}
 
-      const int rq = a.triangle ? a.rank[q] : 0;
-      const bool ok_lo = t_lo >= 0 && (!a.triangle || rq < rk_lo);
-      const bool ok_hi = t_hi >= 0 && (!a.triangle || rq < rk_hi);
-      if (a.mode == MODE_DIST) {
-        int32_t *o = a.dist_out + (size_t)ob + (size_t)(qi - qb) * kGroup;
-        if (ok_lo) o[lane] = d_lo;
-        if (ok_hi) o[kWave + lane] = d_hi;
-      } else {
-        const bool h_lo = ok_lo && d_lo <= a.eps_max;
-        const bool h_hi = ok_hi && d_hi <= a.eps_max;
-        const unsigned long long b_lo = __ballot(h_lo);
-        const unsigned long long b_hi = __ballot(h_hi);
-        const int total = 2 * (__popcll(b_lo) + __popcll(b_hi));
-        if (total > 0) {
-          unsigned long long base = 0;
-          if (lane == 0) base = atomicAdd(a.nkeys, (unsigned long long)total);
-          base = __builtin_amdgcn_readfirstlane((unsigned)base) |
-                 ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
-          const unsigned mlo = __builtin_amdgcn_mbcnt_hi((unsigned)(b_lo >> 32),
-                                __builtin_amdgcn_mbcnt_lo((unsigned)b_lo, 0u));
-          const unsigned mhi = __builtin_amdgcn_mbcnt_hi((unsigned)(b_hi >> 32),
-                                __builtin_amdgcn_mbcnt_lo((unsigned)b_hi, 0u));
-          unsigned long long pos = base + 2ull * (mlo + mhi);
-          const unsigned long long uq = (unsigned long long)q;
-          if (h_lo) {
-            const unsigned long long ut = (unsigned long long)t_lo, ud = (unsigned)d_lo;
-            if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
-            if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
-            pos += 2;
+// Query-profile-in-LDS kernel (residues in at most four codes).  The
+// wave-uniform query is the register strip (S columns, right-aligned so its
+// last column is always strip slot S-1), and each lane streams its two targets
+// as DP rows.  Per strip the wave writes a table of packed substitution scores
+// into its own LDS slice: one row per (code_lo, code_hi) pair of the lane's two
+// target residues (codes A, C, G, T and 4 = past the target's end), S entries
+// per row.  A DP row then reads its S scores with S/4 ds_read_b128 -- no
+// per-row selection, no register copies.  Columns left of the query's first
+// residue ("pre-padding" of the first strip) hold -32768, which keeps their
+// cells at the floor.
+template <int S>
+__global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
+  constexpr int kPairs = 25;
+  constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ __attribute__((aligned(16))) uint32_t tab_all[kWaves * kPairs * RS];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = blockIdx.x * kWaves + wv;
+  uint32_t *tab = tab_all + wv * kPairs * RS;
+  uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+
+  const s16x2 floor2 = {kFloor, kFloor};
+  const s16x2 gapv = {(short)-a.gap, (short)-a.gap};
+  const uint32_t smat = (uint32_t)(uint16_t)(short)a.match;
+  const uint32_t smis = (uint32_t)(uint16_t)(short)a.mismatch;
+
+  for (;;) {
+    int item_idx = 0;
+    if (lane == 0) item_idx = atomicAdd(a.counter, 1);
+    item_idx = __builtin_amdgcn_readfirstlane(item_idx);
+    if (item_idx >= a.n_items) break;
+    const int4 it = a.items[item_idx];
+    const int g = __builtin_amdgcn_readfirstlane(it.x);
+    const int qb = __builtin_amdgcn_readfirstlane(it.y);
+    const int qe = __builtin_amdgcn_readfirstlane(it.z);
+    const int ob = __builtin_amdgcn_readfirstlane(it.w);
+
+    const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
+    const int gmn = __builtin_amdgcn_readfirstlane(a.gmin[g]);
+    const int t_lo = a.gtgt[g * kGroup + lane];
+    const int t_hi = a.gtgt[g * kGroup + kWave + lane];
+    const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
+    const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
+    // empty slots are treated as full length (their results are discarded)
+    const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
+    const int rows_hi = t_hi >= 0 ? lb_hi - 1 : gm;
+    const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
+    const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
+    const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
+    const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
+    const int nrows = gm > 1 ? gm - 1 : 0;     // DP rows = inner residues of the longest target
+    const int nfast = gmn > 1 ? gmn - 1 : 0;   // rows inside every target of the group
+
+    for (int qi = qb; qi < qe; ++qi) {
+      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
+      const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
+      const int ncols = la > 1 ? la - 1 : 0;
+      const int nstrips = (ncols + S - 1) / S;
+      s16x2 best = floor2;
+      s16x2 corner = floor2;
+
+      for (int s = 0; s < nstrips && nrows > 0; ++s) {
+        const int j0 = ncols - (nstrips - s) * S + 1;  // query column of slot 0
+        // ---- score table of this strip
+        __builtin_amdgcn_wave_barrier();
+        for (int e = lane; e < kPairs * S; e += kWave) {
+          const int p = e / S, jj = e - p * S;
+          const int k = j0 + jj;
+          uint32_t v = 0x80008000u;
+          if (k >= 1) {
+            const uint32_t r = (uint32_t)(k - 1);
+            const uint32_t qc = (a.words[qwo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            const uint32_t clo = (uint32_t)(p / 5), chi = (uint32_t)(p - 5 * (p / 5));
+            v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
           }
-          if (h_hi) {
-            const unsigned long long ut = (unsigned long long)t_hi, ud = (unsigned)d_hi;
-            if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
-            if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
+          tab[p * RS + jj] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+
+        s16x2 H[S];
+#pragma unroll
+        for (int jj = 0; jj < S; ++jj) H[jj] = floor2;
+        s16x2 diag_b = floor2;
+        // target residues: 16 rows per word; the next block's words are
+        // fetched at the start of each block
+        uint32_t w_lo = a.words[wo_lo], w_hi = a.words[wo_hi];
+        // one DP row; returns the row's left-boundary value for the next row's diagonal
+        auto do_row = [&](int j, uint32_t left_u) {
+          const uint32_t sh = 2u * ((uint32_t)(j - 1) & 15u);
+          uint32_t c_lo = (w_lo >> sh) & 3u;
+          uint32_t c_hi = (w_hi >> sh) & 3u;
+          if (j > nfast) {
+            if (j > rows_lo) c_lo = 4u;
+            if (j > rows_hi) c_hi = 4u;
           }
+          const uint4 *prow = reinterpret_cast<const uint4 *>(tab + (c_lo * 5u + c_hi) * RS);
+          s16x2 P[S];
+#pragma unroll
+          for (int w = 0; w < S / 4; ++w) {
+            const uint4 v = prow[w];
+            P[4 * w + 0] = as_s16x2(v.x);
+            P[4 * w + 1] = as_s16x2(v.y);
+            P[4 * w + 2] = as_s16x2(v.z);
+            P[4 * w + 3] = as_s16x2(v.w);
+          }
+          const s16x2 left_b = as_s16x2(left_u);
+          dp_row<S>(H, P, left_b, diag_b, gapv, best);
+          bnd[(uint32_t)j * kWave + (uint32_t)lane] = as_u32(H[S - 1]);
+          diag_b = left_b;
+        };
+        // left-boundary values: even rows in bA, odd rows in bB, each loaded two
+        // rows ahead, so a row never waits on the store issued just before it
+        uint32_t bA = as_u32(floor2), bB = as_u32(floor2);
+        if (s > 0) {
+          bA = bnd[kWave + (uint32_t)lane];
+          if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
+        }
+        for (int blk = 0; blk * 16 < nrows; ++blk) {
+          const uint32_t nw_lo = a.words[wo_lo + (uint32_t)blk + 1u];
+          const uint32_t nw_hi = a.words[wo_hi + (uint32_t)blk + 1u];
+          const int jend = min(nrows, blk * 16 + 16);
+          int j = blk * 16 + 1;
+          for (; j + 1 <= jend; j += 2) {
+            do_row(j, bA);
+            if (s > 0 && j + 2 <= nrows) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
+            do_row(j + 1, bB);
+            if (s > 0 && j + 3 <= nrows) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
+          }
+          if (j <= jend) {  // odd row count: the last row of the strip
+            do_row(j, bA);
+          }
+          w_lo = nw_lo;
+          w_hi = nw_hi;
         }
       }
+      if (nstrips > 0 && nrows > 0) {
+        // corner H(la-1, lb-1): the last strip's right-most column at row lb-1
+        uint32_t cu = as_u32(floor2);
+        if (lb_lo >= 2) cu = (cu & 0xffff0000u) | (bnd[(uint32_t)(lb_lo - 1) * kWave + lane] & 0xffffu);
+        if (lb_hi >= 2) cu = (cu & 0x0000ffffu) | (bnd[(uint32_t)(lb_hi - 1) * kWave + lane] & 0xffff0000u);
+        corner = as_s16x2(cu);
+      }
+      finish_query(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+                   best, corner);
     }
   }
 }
@@ -434,12 +593,13 @@ struct mnhn_sw_ctx {
   std::vector<int32_t> len, order, rank;
   std::vector<uint32_t> woff;
   int groups = 0;
-  std::vector<int32_t> gtgt, gmax;
+  std::vector<int32_t> gtgt, gmax, gmin;
   int32_t maxlen = 0;
   int device = 0, num_cu = 0;
-  int strip = 0, waves_per_cu = 0;  // 0 = tuned default
+  int kernel = 0, strip = 0, waves_per_cu = 0;  // 0 = tuned default
   uint32_t *d_words = nullptr, *d_woff = nullptr;
-  int32_t *d_len = nullptr, *d_rank = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr;
+  int32_t *d_len = nullptr, *d_rank = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr,
+          *d_gmin = nullptr;
   int *d_counter = nullptr;
   unsigned long long *d_nkeys = nullptr;
   uint32_t *d_bound = nullptr;
@@ -480,24 +640,48 @@ void dev_free(T *&p) {
 struct Work {
   std::vector<int4> items;
   std::vector<int32_t> qids;
-  std::vector<int32_t> gtgt, gmax;  // empty = use the context's length groups
+  std::vector<int32_t> gtgt, gmax, gmin;  // empty = use the context's length groups
   double cells = 0, pairs = 0;
   int max_q_len = 0;
 };
 
-int pick_strip(const mnhn_sw_ctx *c) {
-  if (c->strip == 16 || c->strip == 32) return c->strip;
-  const char *e = getenv("MNHN_SW_STRIP");
-  if (e) {
-    int v = atoi(e);
-    if (v == 16 || v == 32) return v;
+// Kernel variants: the LDS query-profile kernel (residues in <= 4 codes) and
+// the register-profile kernel (any alphabet).
+enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2 };
+
+int pick_kernel(const mnhn_sw_ctx *c) {
+  int k = c->kernel;
+  if (k == KERNEL_AUTO) {
+    const char *e = getenv("MNHN_SW_KERNEL");
+    if (e && strcmp(e, "reg") == 0) k = KERNEL_REG;
+    if (e && strcmp(e, "lds") == 0) k = KERNEL_LDS;
   }
-  return 32;
+  if (k == KERNEL_AUTO) k = KERNEL_LDS;
+  if (!c->acgt) k = KERNEL_REG;  // the LDS table needs <= 4 residue codes
+  return k;
 }
 
-template <int S, bool A>
-void *kernel_ptr() {
-  return reinterpret_cast<void *>(&sw_tiles_kernel<S, A>);
+int pick_strip(const mnhn_sw_ctx *c) {
+  const int k = pick_kernel(c);
+  int v = c->strip;
+  if (v == 0) {
+    const char *e = getenv("MNHN_SW_STRIP");
+    if (e) v = atoi(e);
+  }
+  if (k == KERNEL_LDS) return (v == 16 || v == 32 || v == 64) ? v : 32;
+  return (v == 16 || v == 32) ? v : 32;
+}
+
+void *kernel_ptr(int k, int S, bool acgt) {
+  if (k == KERNEL_LDS) {
+    if (S == 16) return reinterpret_cast<void *>(&sw_lds_kernel<16>);
+    if (S == 64) return reinterpret_cast<void *>(&sw_lds_kernel<64>);
+    return reinterpret_cast<void *>(&sw_lds_kernel<32>);
+  }
+  if (S == 16) return acgt ? reinterpret_cast<void *>(&sw_tiles_kernel<16, true>)
+                           : reinterpret_cast<void *>(&sw_tiles_kernel<16, false>);
+  return acgt ? reinterpret_cast<void *>(&sw_tiles_kernel<32, true>)
+              : reinterpret_cast<void *>(&sw_tiles_kernel<32, false>);
 }
 
 int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
@@ -506,18 +690,19 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   if (w.items.empty()) return MNHN_OK;
   int rc;
   int4 *d_items = nullptr;
-  int32_t *d_qids = nullptr, *d_gt = nullptr, *d_gm = nullptr;
+  int32_t *d_qids = nullptr, *d_gt = nullptr, *d_gm = nullptr, *d_gn = nullptr;
   if ((rc = dev_alloc(&d_items, w.items.size()))) return rc;
   if ((rc = dev_alloc(&d_qids, w.qids.size()))) { dev_free(d_items); return rc; }
   const bool own_groups = !w.gtgt.empty();
   if (own_groups) {
-    if ((rc = dev_alloc(&d_gt, w.gtgt.size())) || (rc = dev_alloc(&d_gm, w.gmax.size()))) {
-      dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm);
+    if ((rc = dev_alloc(&d_gt, w.gtgt.size())) || (rc = dev_alloc(&d_gm, w.gmax.size())) ||
+        (rc = dev_alloc(&d_gn, w.gmin.size()))) {
+      dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm); dev_free(d_gn);
       return rc;
     }
   }
   auto cleanup = [&]() {
-    dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm);
+    dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm); dev_free(d_gn);
   };
   hipError_t e = hipSuccess;
   e = hipMemcpyAsync(d_items, w.items.data(), w.items.size() * sizeof(int4),
@@ -531,18 +716,24 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_gm, w.gmax.data(), w.gmax.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_gn, w.gmin.data(), w.gmin.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice, c->stream);
   }
   if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(int), c->stream);
   if (e != hipSuccess) { cleanup(); return MNHN_ERR_DEVICE; }
 
+  const int K = pick_kernel(c);
   const int S = pick_strip(c);
   int wpc = c->waves_per_cu;
   if (wpc <= 0) {
     const char *ev = getenv("MNHN_SW_WAVES_PER_CU");
     wpc = ev ? atoi(ev) : 0;
   }
-  void *kp = S == 16 ? (c->acgt ? kernel_ptr<16, true>() : kernel_ptr<16, false>())
-                     : (c->acgt ? kernel_ptr<32, true>() : kernel_ptr<32, false>());
+  void *kp = kernel_ptr(K, S, c->acgt);
+  // measured on MI355X (config-2 workload): the LDS kernel at S = 32 runs
+  // best with 3 of its 4 possible waves per SIMD resident
+  if (wpc <= 0 && K == KERNEL_LDS && S == 32) wpc = 12;
   if (wpc <= 0) {
     int blocks_per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) !=
@@ -555,8 +746,10 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   if (blocks > max_useful) blocks = std::max(1, max_useful);
   const int waves = blocks * (kBlock / kWave);
 
-  // strip-boundary workspace: (max query length + 1) rows x 64 lanes per wave
-  const int stride = (w.max_q_len + 2) * kWave;
+  // strip-boundary workspace, [row][lane] per wave.  Rows are query residues
+  // in the register-profile kernel and target residues in the LDS kernel, so
+  // it is sized for the longest sequence of the context.
+  const int stride = (std::max(w.max_q_len, c->maxlen) + 2) * kWave;
   const size_t need = (size_t)waves * stride * sizeof(uint32_t);
   if (need > c->bound_bytes) {
     dev_free(c->d_bound);
@@ -573,6 +766,7 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
   a.gtgt = own_groups ? d_gt : c->d_gtgt;
   a.gmax = own_groups ? d_gm : c->d_gmax;
+  a.gmin = own_groups ? d_gn : c->d_gmin;
   a.items = d_items; a.qids = d_qids; a.counter = c->d_counter;
   a.bound = c->d_bound; a.dist_out = d_dist; a.keys = d_keys; a.nkeys = c->d_nkeys;
   a.key_cap = key_cap; a.n_items = (int)w.items.size(); a.bound_stride = stride;
@@ -580,14 +774,11 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
 
   (void)hipEventRecord(c->ev[0], c->stream);
-  if (S == 16) {
-    if (c->acgt) hipLaunchKernelGGL((sw_tiles_kernel<16, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
-    else hipLaunchKernelGGL((sw_tiles_kernel<16, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
-  } else {
-    if (c->acgt) hipLaunchKernelGGL((sw_tiles_kernel<32, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
-    else hipLaunchKernelGGL((sw_tiles_kernel<32, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+  {
+    void *args[] = {&a};
+    e = hipLaunchKernel(kp, dim3(blocks), dim3(kBlock), args, 0, c->stream);
   }
-  e = hipGetLastError();
+  if (e == hipSuccess) e = hipGetLastError();
   (void)hipEventRecord(c->ev[1], c->stream);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
   float ms = 0.f;
@@ -600,6 +791,7 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   c->last.pairs = w.pairs;
   c->last.lane_bits = 16;
   c->last.strip = S;
+  c->last.kernel = K;
   c->last.waves = waves;
   c->last.items = (int)w.items.size();
   return MNHN_OK;
@@ -780,11 +972,13 @@ int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
   c->groups = (n + kGroup - 1) / kGroup;
   c->gtgt.assign((size_t)c->groups * kGroup, -1);
   c->gmax.assign(c->groups, 0);
+  c->gmin.assign(c->groups, INT32_MAX);
   for (int p = 0; p < n; ++p) {
     const int g = p / kGroup, t = p % kGroup;
     // slot t: lane t for t < 64 (low half), lane t-64 (high half) otherwise
     c->gtgt[(size_t)g * kGroup + t] = c->order[p];
     c->gmax[g] = std::max(c->gmax[g], c->len[c->order[p]]);
+    c->gmin[g] = std::min(c->gmin[g], c->len[c->order[p]]);
   }
 
   int rc = MNHN_OK;
@@ -804,6 +998,7 @@ int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
   if ((rc = dev_alloc(&c->d_words, words.size())) || (rc = dev_alloc(&c->d_woff, (size_t)n)) ||
       (rc = dev_alloc(&c->d_len, (size_t)n)) || (rc = dev_alloc(&c->d_rank, (size_t)n)) ||
       (rc = dev_alloc(&c->d_gtgt, c->gtgt.size())) || (rc = dev_alloc(&c->d_gmax, c->gmax.size())) ||
+      (rc = dev_alloc(&c->d_gmin, c->gmin.size())) ||
       (rc = dev_alloc(&c->d_counter, 1)) || (rc = dev_alloc(&c->d_nkeys, 1))) {
     mnhn_sw_destroy(c);
     return rc;
@@ -816,6 +1011,8 @@ int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
     e = hipMemcpy(c->d_gtgt, c->gtgt.data(), c->gtgt.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess && c->groups)
     e = hipMemcpy(c->d_gmax, c->gmax.data(), c->gmax.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->groups)
+    e = hipMemcpy(c->d_gmin, c->gmin.data(), c->gmin.size() * 4, hipMemcpyHostToDevice);
   if (e != hipSuccess) { mnhn_sw_destroy(c); return MNHN_ERR_DEVICE; }
   *out = c;
   return MNHN_OK;
@@ -825,7 +1022,8 @@ void mnhn_sw_destroy(mnhn_sw_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dev_free(c->d_words); dev_free(c->d_woff); dev_free(c->d_len); dev_free(c->d_rank);
-  dev_free(c->d_gtgt); dev_free(c->d_gmax); dev_free(c->d_counter); dev_free(c->d_nkeys);
+  dev_free(c->d_gtgt); dev_free(c->d_gmax); dev_free(c->d_gmin); dev_free(c->d_counter);
+  dev_free(c->d_nkeys);
   dev_free(c->d_bound); dev_free(c->d_keys);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -842,8 +1040,11 @@ int mnhn_sw_last_timing(const mnhn_sw_ctx *c, mnhn_sw_timing *out) {
   return MNHN_OK;
 }
 
-int mnhn_sw_set_tuning(mnhn_sw_ctx *c, int strip, int waves_per_cu) {
-  if (!c || (strip != 0 && strip != 16 && strip != 32) || waves_per_cu < 0) return MNHN_ERR_ARG;
+int mnhn_sw_set_tuning(mnhn_sw_ctx *c, int kernel, int strip, int waves_per_cu) {
+  if (!c || kernel < 0 || kernel > 2 || (strip != 0 && strip != 16 && strip != 32 && strip != 64) ||
+      waves_per_cu < 0)
+    return MNHN_ERR_ARG;
+  c->kernel = kernel;
   c->strip = strip;
   c->waves_per_cu = waves_per_cu;
   return MNHN_OK;
@@ -903,15 +1104,19 @@ int mnhn_sw_pairs(mnhn_sw_ctx *c, const int32_t *ij, int64_t m, int32_t *dist_ou
     const int qi = (int)w.qids.size() - 1;
     for (int64_t s = k; s < e; s += kGroup) {
       const int g = (int)w.gmax.size();
-      int gm = 0;
+      int gm = 0, gn = INT32_MAX;
       for (int t = 0; t < kGroup; ++t) {
         const int64_t p = s + t;
         const int id = p < e ? ij[2 * idx[p] + 1] : -1;
         w.gtgt.push_back(id);
         slot_pair.push_back(p < e ? idx[p] : -1);
-        if (id >= 0) gm = std::max(gm, c->len[id]);
+        if (id >= 0) {
+          gm = std::max(gm, c->len[id]);
+          gn = std::min(gn, c->len[id]);
+        }
       }
       w.gmax.push_back(gm);
+      w.gmin.push_back(gn);
       w.items.push_back(make_int4(g, qi, qi + 1, g * kGroup));
     }
     k = e;

@@ -42,6 +42,13 @@ PKG_DIR = os.path.join(ROOT, "mnhn-tree-tools_amd")
 sys.path.insert(0, PKG_DIR)
 import synth  # noqa: E402
 
+WORKLOADS = {
+    "c2": "uneven-length all-pairs SW: 20,000 seqs, 3-level tree, lognormal lengths "
+          "median 300 in [100,600]",
+    "c3": "amplicon-scale all-pairs SW: 105,000 seqs (500 ancestors x 200 at 2-10% "
+          "mutation + 5% noise), lengths uniform in [150,300]",
+}
+
 # MI355X: 256 CUs x 64 lanes x 2 (packed int16) x 2.4 GHz
 PEAK_INT16_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 OPS_PER_CELL = 6
@@ -124,7 +131,23 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3"],
+                    help="c2: 20,000 uneven-length (headline); c3: 105,000 amplicon-scale")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (torch) 3 times, "
+                         "to calibrate FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
               file=sys.stderr)
@@ -132,7 +155,7 @@ def main():
 
     pkg = load_pkg()
     print("generating workload ...", file=sys.stderr, flush=True)
-    seqs = synth.config2()
+    seqs = synth.config2() if args.config == "c2" else synth.config3()
     if args.n:
         seqs = seqs[:args.n]
     L = np.array([len(s) for s in seqs], dtype=np.float64)
@@ -140,6 +163,10 @@ def main():
     pairs = len(seqs) * (len(seqs) - 1) // 2
 
     eng = pkg.Engine(seqs)
+    tuning = None
+    if not args.no_autotune:
+        tuning = eng.autotune(use_cache=True)
+        print(f"autotune: {tuning}", file=sys.stderr, flush=True)
     print(f"n={len(seqs)} pairs={pairs} cells={cells:.3e}; warmup {args.warmup}", file=sys.stderr,
           flush=True)
     for _ in range(args.warmup):
@@ -165,11 +192,12 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_s * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16",
         "data": "synthetic",
-        "config": {"workload": "uneven-length all-pairs SW: 20,000 seqs, 3-level tree, "
-                               "lognormal lengths median 300 in [100,600]",
+        "config": {"workload": WORKLOADS[args.config],
                    "n_seqs": len(seqs), "unique_pairs": pairs, "cells": cells,
                    "eps_max": args.eps_max, "eps_list_entries": nnz,
-                   "strip": t["strip"], "waves": t["waves"], "items": t["items"]},
+                   "kernel": {1: "register-profile", 2: "lds-profile"}.get(t["kernel"]),
+                   "strip": t["strip"], "waves": t["waves"], "items": t["items"],
+                   "autotune": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s (int16, 6 ops/cell)", "frac": achieved / PEAK_INT16_TOPS,
                      "traffic": None, "kernel": "sw_tiles_kernel",
@@ -179,7 +207,7 @@ def main():
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(seqs, args.cpu_sample)
-    if not args.no_dbscan:
+    if not args.no_dbscan and args.config == "c2":
         print("adaptive DBSCAN run ...", file=sys.stderr, flush=True)
         out["dbscan"] = run_dbscan(seqs, 100, 300, 5)
     print(json.dumps(out), flush=True)

@@ -63,7 +63,8 @@ typedef struct {
   double cells;       /* algorithmic cells: sum of la*lb over computed pairs */
   double pairs;       /* pairs computed */
   int32_t lane_bits;  /* 16 (packed) or 32 */
-  int32_t kernel;     /* 1 = register-profile kernel, 2 = LDS-profile kernel */
+  int32_t kernel;     /* 1 = register profile, 2 = LDS profile (int16),
+                         3 = LDS profile, exact fp16 where it applies */
   int32_t strip;      /* register strip width used */
   int32_t waves;      /* persistent waves launched */
   int32_t items;      /* work items in the queue */
@@ -116,10 +117,30 @@ int mnhn_sw_alphabet(const mnhn_sw_ctx *ctx);
 int mnhn_sw_last_timing(const mnhn_sw_ctx *ctx, mnhn_sw_timing *out);
 
 /* Force the kernel variant (1 = register profile, any alphabet; 2 = LDS
- * profile, <= 4 residue codes), register-strip width (16, 32 or 64; 64 only
- * for variant 2) and waves per CU used by later calls; 0 restores the tuned
- * defaults. */
+ * profile in int16, <= 4 residue codes; 3 = as 2 but pairs with
+ * match * min(la, lb) <= 2048 run in exact fp16), register-strip width (16,
+ * 32 or 64; 64 only for variants 2 and 3) and waves per CU used by later
+ * calls; 0 restores the tuned defaults. */
 int mnhn_sw_set_tuning(mnhn_sw_ctx *ctx, int kernel, int strip, int waves_per_cu);
+
+/* Settings chosen by mnhn_sw_autotune. */
+typedef struct {
+  int32_t kernel;
+  int32_t strip;
+  int32_t waves_per_cu;
+  double gcups;        /* measured on the timing sample */
+  int32_t from_cache;  /* 1 if read from the cache without timing */
+} mnhn_sw_tuning;
+
+/* Autotuner: times each candidate kernel variant / strip width / waves-per-CU
+ * setting on a strided sample of this context's all-pairs work and keeps the
+ * fastest for later calls.  Choices are cached as JSON lines in
+ * $MNHN_SW_TUNE_CACHE (default $HOME/.cache/mnhn_sw_tune.jsonl), keyed by GPU
+ * architecture, CU count, alphabet mode, length profile and scores; with
+ * use_cache != 0 a cached choice is applied without timing.  No counterpart
+ * in the reference (whose OpenCL launch lets the runtime pick the local size,
+ * dbscan.c:497-501). */
+int mnhn_sw_autotune(mnhn_sw_ctx *ctx, int use_cache, mnhn_sw_tuning *out);
 
 const char *mnhn_sw_strerror(int code);
 

@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "mnhn_sw_create", "mnhn_sw_destroy", "mnhn_sw_one_vs_all", "mnhn_sw_pairs",
     "mnhn_sw_matrix", "mnhn_sw_eps_csr", "mnhn_sw_eps_count", "mnhn_csr_free",
     "mnhn_sw_size", "mnhn_sw_alphabet", "mnhn_sw_last_timing",
-    "mnhn_sw_set_tuning", "mnhn_sw_strerror",
+    "mnhn_sw_set_tuning", "mnhn_sw_autotune", "mnhn_sw_strerror",
 ]
 HOST_SYMBOLS = [
     "dataset_from_fasta", "free_sequences_from_dataset", "store_split_set",
@@ -68,6 +68,12 @@ class Timing(ctypes.Structure):
                 ("waves", ctypes.c_int32), ("items", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("strip", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32), ("gcups", ctypes.c_double),
+                ("from_cache", ctypes.c_int32)]
+
+
 _LIB = None
 
 
@@ -95,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_sw_alphabet.argtypes = [vp]
     lib.mnhn_sw_last_timing.argtypes = [vp, ctypes.POINTER(Timing)]
     lib.mnhn_sw_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.mnhn_sw_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Tuning)]
     lib.mnhn_sw_strerror.argtypes = [ctypes.c_int]
     lib.mnhn_sw_strerror.restype = ctypes.c_char_p
     _LIB = lib
@@ -149,6 +156,14 @@ class Engine:
     def set_tuning(self, kernel: int = 0, strip: int = 0, waves_per_cu: int = 0) -> None:
         """kernel: 0 auto, 1 register profile, 2 LDS profile; strip 16/32/64."""
         _check(self.lib.mnhn_sw_set_tuning(self.ctx, kernel, strip, waves_per_cu), "set_tuning")
+
+    def autotune(self, use_cache: bool = True) -> dict:
+        """Time the candidate kernel settings on a sample of this set's
+        all-pairs work (or reuse the cached choice) and keep the fastest."""
+        t = Tuning()
+        _check(self.lib.mnhn_sw_autotune(self.ctx, 1 if use_cache else 0, ctypes.byref(t)),
+               "mnhn_sw_autotune")
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
     def one_vs_all(self, query: int) -> np.ndarray:
         out = np.empty(self.n, dtype=np.int32)

@@ -36,6 +36,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <string>
 #include <chrono>
 #include <cstdint>
 #include <cstdlib>
@@ -43,6 +46,8 @@
 #include <new>
 #include <numeric>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "mnhn_sw.h"
 
@@ -141,11 +146,10 @@ __device__ __forceinline__ void dp_row_codes(s16x2 (&H)[S], const uint32_t (&TC)
 
 // Distances of the lane's two targets against query q and their output:
 // MODE_DIST writes them, MODE_EPS emits keys for d <= eps_max.
-__device__ __forceinline__ void finish_query(const KArgs &a, int lane, int q, int qrel, int ob,
-                                             int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
-                                             int rk_lo, int rk_hi, s16x2 best, s16x2 corner) {
-  const int best_lo = (int)best.x + kBias, best_hi = (int)best.y + kBias;
-  const int cor_lo = (int)corner.x + kBias, cor_hi = (int)corner.y + kBias;
+__device__ __forceinline__ void finish_query_int(const KArgs &a, int lane, int q, int qrel, int ob,
+                                                 int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
+                                                 int rk_lo, int rk_hi, int best_lo, int best_hi,
+                                                 int cor_lo, int cor_hi) {
   int tot_lo = 0, tot_hi = 0;
   if (la >= 1 && lb_lo >= 1) tot_lo = max(best_lo, cor_lo + a.match);
   if (la >= 1 && lb_hi >= 1) tot_hi = max(best_hi, cor_hi + a.match);
@@ -188,6 +192,15 @@ __device__ __forceinline__ void finish_query(const KArgs &a, int lane, int q, in
     if (pos < a.key_cap) a.keys[pos] = (uq << 40) | (ut << 16) | ud;
     if (pos + 1 < a.key_cap) a.keys[pos + 1] = (ut << 40) | (uq << 16) | ud;
   }
+}
+
+// int16 kernels: best / corner are biased by -32768
+__device__ __forceinline__ void finish_query(const KArgs &a, int lane, int q, int qrel, int ob,
+                                             int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
+                                             int rk_lo, int rk_hi, s16x2 best, s16x2 corner) {
+  finish_query_int(a, lane, q, qrel, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+                   (int)best.x + kBias, (int)best.y + kBias, (int)corner.x + kBias,
+                   (int)corner.y + kBias);
 }
 
 template <int S, bool kAcgt>
@@ -493,6 +506,190 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Exact fp16 form of the LDS kernel, for pairs with match * min(la, lb) <= 2048.
+// Values are held as k / 2048 in packed fp16, which is exact for integers k of
+// magnitude <= 2048; every inner cell has H <= match * (min(la, lb) - 1), so no
+// value leaves that range.  Each column keeps G = H - gap instead of H, which
+// turns the cell into
+//     D = clamp01(G_diag + (s + gap))          v_pk_add_f16 ... clamp
+//     H = max3(D, G_left, G_up)                v_pk_maximum3_f16
+//     G = H - gap                              v_pk_add_f16
+// and best = max3(best, H_even, H_odd) every two columns: 3.5 packed VALU ops
+// per two cells instead of 5.  (clamp01 is the local-alignment floor; its upper
+// bound 1.0 = 2048 is never exceeded.)
+// ---------------------------------------------------------------------------
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+constexpr float kF16Scale = 1.0f / 2048.0f;
+
+__device__ __forceinline__ f16x2 as_f16x2(uint32_t v) { return __builtin_bit_cast(f16x2, v); }
+__device__ __forceinline__ uint32_t f16_u32(f16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ f16x2 clamp01(f16x2 x) {
+  return __builtin_elementwise_min(__builtin_elementwise_max(x, (f16x2){0, 0}), (f16x2){1, 1});
+}
+
+template <int S>
+__device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f16x2 gleft,
+                                           f16x2 gdiag, f16x2 gapv, f16x2 &best) {
+  f16x2 dnext = clamp01(gdiag + P[0]);
+  f16x2 hprev = best;
+#pragma unroll
+  for (int jj = 0; jj < S; ++jj) {
+    const f16x2 D = dnext;
+    if (jj + 1 < S) dnext = clamp01(G[jj] + P[jj + 1]);
+    const f16x2 H = __builtin_elementwise_maximum(__builtin_elementwise_maximum(D, gleft), G[jj]);
+    const f16x2 g = H + gapv;
+    G[jj] = g;
+    gleft = g;
+    if (jj & 1) best = __builtin_elementwise_maximum(__builtin_elementwise_maximum(best, hprev), H);
+    else hprev = H;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
+  constexpr int kPairs = 25;
+  constexpr int RS = S + 4;
+  constexpr int kWaves = kBlock / kWave;
+  static_assert(S % 2 == 0, "pairs of columns");
+  __shared__ __attribute__((aligned(16))) uint32_t tab_all[kWaves * kPairs * RS];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = blockIdx.x * kWaves + wv;
+  uint32_t *tab = tab_all + wv * kPairs * RS;
+  uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+
+  const _Float16 gap_h = (_Float16)((float)a.gap * kF16Scale);
+  const f16x2 gapv = {(_Float16)-gap_h, (_Float16)-gap_h};
+  const f16x2 g0 = gapv;  // G of an H = 0 cell
+  const f16x2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
+  // substitution scores plus the gap (G stores H - gap), scaled
+  const uint32_t smat = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)((float)(a.match + a.gap) * kF16Scale));
+  const uint32_t smis = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)((float)(a.mismatch + a.gap) * kF16Scale));
+  const uint32_t spre = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)-60000.0f);
+
+  for (;;) {
+    int item_idx = 0;
+    if (lane == 0) item_idx = atomicAdd(a.counter, 1);
+    item_idx = __builtin_amdgcn_readfirstlane(item_idx);
+    if (item_idx >= a.n_items) break;
+    const int4 it = a.items[item_idx];
+    const int g = __builtin_amdgcn_readfirstlane(it.x);
+    const int qb = __builtin_amdgcn_readfirstlane(it.y);
+    const int qe = __builtin_amdgcn_readfirstlane(it.z);
+    const int ob = __builtin_amdgcn_readfirstlane(it.w);
+
+    const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
+    const int gmn = __builtin_amdgcn_readfirstlane(a.gmin[g]);
+    const int t_lo = a.gtgt[g * kGroup + lane];
+    const int t_hi = a.gtgt[g * kGroup + kWave + lane];
+    const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
+    const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
+    const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
+    const int rows_hi = t_hi >= 0 ? lb_hi - 1 : gm;
+    const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
+    const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
+    const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
+    const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
+    const int nrows = gm > 1 ? gm - 1 : 0;
+    const int nfast = gmn > 1 ? gmn - 1 : 0;
+
+    for (int qi = qb; qi < qe; ++qi) {
+      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
+      const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
+      const int ncols = la > 1 ? la - 1 : 0;
+      const int nstrips = (ncols + S - 1) / S;
+      f16x2 best = zero2;
+
+      for (int s = 0; s < nstrips && nrows > 0; ++s) {
+        const int j0 = ncols - (nstrips - s) * S + 1;
+        __builtin_amdgcn_wave_barrier();
+        for (int e = lane; e < kPairs * S; e += kWave) {
+          const int p = e / S, jj = e - p * S;
+          const int k = j0 + jj;
+          uint32_t v = spre | (spre << 16);
+          if (k >= 1) {
+            const uint32_t r = (uint32_t)(k - 1);
+            const uint32_t qc = (a.words[qwo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            const uint32_t clo = (uint32_t)(p / 5), chi = (uint32_t)(p - 5 * (p / 5));
+            v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
+          }
+          tab[p * RS + jj] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+
+        f16x2 G[S];
+#pragma unroll
+        for (int jj = 0; jj < S; ++jj) G[jj] = g0;
+        f16x2 gdiag = g0;
+        uint32_t w_lo = a.words[wo_lo], w_hi = a.words[wo_hi];
+        auto do_row = [&](int j, uint32_t left_u, bool pad) {
+          const uint32_t sh = 2u * ((uint32_t)(j - 1) & 15u);
+          uint32_t c_lo = (w_lo >> sh) & 3u;
+          uint32_t c_hi = (w_hi >> sh) & 3u;
+          if (pad) {
+            if (j > rows_lo) c_lo = 4u;
+            if (j > rows_hi) c_hi = 4u;
+          }
+          const uint4 *prow = reinterpret_cast<const uint4 *>(tab + (c_lo * 5u + c_hi) * RS);
+          f16x2 P[S];
+#pragma unroll
+          for (int w = 0; w < S / 4; ++w) {
+            const uint4 v = prow[w];
+            P[4 * w + 0] = as_f16x2(v.x);
+            P[4 * w + 1] = as_f16x2(v.y);
+            P[4 * w + 2] = as_f16x2(v.z);
+            P[4 * w + 3] = as_f16x2(v.w);
+          }
+          const f16x2 gleft = as_f16x2(left_u);
+          dp_row_f16<S>(G, P, gleft, gdiag, gapv, best);
+          bnd[(uint32_t)j * kWave + (uint32_t)lane] = f16_u32(G[S - 1]);
+          gdiag = gleft;
+        };
+        uint32_t bA = f16_u32(g0), bB = f16_u32(g0);
+        if (s > 0) {
+          bA = bnd[kWave + (uint32_t)lane];
+          if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
+        }
+        for (int blk = 0; blk * 16 < nrows; ++blk) {
+          const uint32_t nw_lo = a.words[wo_lo + (uint32_t)blk + 1u];
+          const uint32_t nw_hi = a.words[wo_hi + (uint32_t)blk + 1u];
+          const int jend = min(nrows, blk * 16 + 16);
+          int j = blk * 16 + 1;
+          for (; j + 1 <= jend; j += 2) {
+            do_row(j, bA, j > nfast);
+            if (s > 0 && j + 2 <= nrows) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
+            do_row(j + 1, bB, j + 1 > nfast);
+            if (s > 0 && j + 3 <= nrows) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
+          }
+          if (j <= jend) do_row(j, bA, true);
+          w_lo = nw_lo;
+          w_hi = nw_hi;
+        }
+      }
+      // back to integers: H = 2048 * value; corner H = G + gap at row lb-1
+      const float sc = 2048.0f;
+      int best_lo = (int)__builtin_rintf((float)best.x * sc);
+      int best_hi = (int)__builtin_rintf((float)best.y * sc);
+      int cor_lo = 0, cor_hi = 0;
+      if (nstrips > 0 && nrows > 0) {
+        if (lb_lo >= 2) {
+          const f16x2 gv = as_f16x2(bnd[(uint32_t)(lb_lo - 1) * kWave + lane]);
+          cor_lo = (int)__builtin_rintf((float)gv.x * sc) + a.gap;
+        }
+        if (lb_hi >= 2) {
+          const f16x2 gv = as_f16x2(bnd[(uint32_t)(lb_hi - 1) * kWave + lane]);
+          cor_hi = (int)__builtin_rintf((float)gv.y * sc) + a.gap;
+        }
+      }
+      finish_query_int(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+                       best_lo, best_hi, cor_lo, cor_hi);
+    }
+  }
+}
+
 // ---- epsilon-list compaction: keys -> CSR rows in ascending column order ----
 
 __global__ void count_rows_kernel(const unsigned long long *keys, long long n, int32_t *cnt) {
@@ -647,7 +844,7 @@ struct Work {
 
 // Kernel variants: the LDS query-profile kernel (residues in <= 4 codes) and
 // the register-profile kernel (any alphabet).
-enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2 };
+enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3 };
 
 int pick_kernel(const mnhn_sw_ctx *c) {
   int k = c->kernel;
@@ -655,8 +852,9 @@ int pick_kernel(const mnhn_sw_ctx *c) {
     const char *e = getenv("MNHN_SW_KERNEL");
     if (e && strcmp(e, "reg") == 0) k = KERNEL_REG;
     if (e && strcmp(e, "lds") == 0) k = KERNEL_LDS;
+    if (e && strcmp(e, "f16") == 0) k = KERNEL_LDS_F16;
   }
-  if (k == KERNEL_AUTO) k = KERNEL_LDS;
+  if (k == KERNEL_AUTO) k = KERNEL_LDS_F16;
   if (!c->acgt) k = KERNEL_REG;  // the LDS table needs <= 4 residue codes
   return k;
 }
@@ -668,11 +866,16 @@ int pick_strip(const mnhn_sw_ctx *c) {
     const char *e = getenv("MNHN_SW_STRIP");
     if (e) v = atoi(e);
   }
-  if (k == KERNEL_LDS) return (v == 16 || v == 32 || v == 64) ? v : 32;
+  if (k == KERNEL_LDS || k == KERNEL_LDS_F16) return (v == 16 || v == 32 || v == 64) ? v : 32;
   return (v == 16 || v == 32) ? v : 32;
 }
 
 void *kernel_ptr(int k, int S, bool acgt) {
+  if (k == KERNEL_LDS_F16) {
+    if (S == 16) return reinterpret_cast<void *>(&sw_lds_f16_kernel<16>);
+    if (S == 64) return reinterpret_cast<void *>(&sw_lds_f16_kernel<64>);
+    return reinterpret_cast<void *>(&sw_lds_f16_kernel<32>);
+  }
   if (k == KERNEL_LDS) {
     if (S == 16) return reinterpret_cast<void *>(&sw_lds_kernel<16>);
     if (S == 64) return reinterpret_cast<void *>(&sw_lds_kernel<64>);
@@ -720,65 +923,94 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
       e = hipMemcpyAsync(d_gn, w.gmin.data(), w.gmin.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, c->stream);
   }
-  if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(int), c->stream);
   if (e != hipSuccess) { cleanup(); return MNHN_ERR_DEVICE; }
 
   const int K = pick_kernel(c);
   const int S = pick_strip(c);
-  int wpc = c->waves_per_cu;
-  if (wpc <= 0) {
+  int wpc_req = c->waves_per_cu;
+  if (wpc_req <= 0) {
     const char *ev = getenv("MNHN_SW_WAVES_PER_CU");
-    wpc = ev ? atoi(ev) : 0;
+    wpc_req = ev ? atoi(ev) : 0;
   }
-  void *kp = kernel_ptr(K, S, c->acgt);
-  // measured on MI355X (config-2 workload): the LDS kernel at S = 32 runs
-  // best with 3 of its 4 possible waves per SIMD resident
-  if (wpc <= 0 && K == KERNEL_LDS && S == 32) wpc = 12;
-  if (wpc <= 0) {
-    int blocks_per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) !=
-            hipSuccess || blocks_per_cu <= 0)
-      blocks_per_cu = 1;
-    wpc = blocks_per_cu * (kBlock / kWave);
+  // passes: (kernel, items).  The fp16 variant takes the items whose pairs
+  // all satisfy match * min(la, lb) <= 2048; the rest run the int16 LDS kernel.
+  std::vector<std::pair<int, std::vector<int4>>> passes;
+  if (K == KERNEL_LDS_F16) {
+    std::vector<int4> f16, i16;
+    for (const int4 &it : w.items) {
+      const int gm = own_groups ? w.gmax[it.x] : c->gmax[it.x];
+      int maxla = 0;
+      for (int qi = it.y; qi < it.z; ++qi) maxla = std::max(maxla, c->len[w.qids[qi]]);
+      const bool ok = (long long)c->match * gm <= 2048 || (long long)c->match * maxla <= 2048;
+      (ok ? f16 : i16).push_back(it);
+    }
+    if (!f16.empty()) passes.push_back({KERNEL_LDS_F16, std::move(f16)});
+    if (!i16.empty()) passes.push_back({KERNEL_LDS, std::move(i16)});
+  } else {
+    passes.push_back({K, w.items});
   }
-  int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
-  const int max_useful = (int)((w.items.size() + 3) / 4);
-  if (blocks > max_useful) blocks = std::max(1, max_useful);
-  const int waves = blocks * (kBlock / kWave);
 
   // strip-boundary workspace, [row][lane] per wave.  Rows are query residues
-  // in the register-profile kernel and target residues in the LDS kernel, so
+  // in the register-profile kernel and target residues in the LDS kernels, so
   // it is sized for the longest sequence of the context.
   const int stride = (std::max(w.max_q_len, c->maxlen) + 2) * kWave;
-  const size_t need = (size_t)waves * stride * sizeof(uint32_t);
-  if (need > c->bound_bytes) {
-    dev_free(c->d_bound);
-    c->bound_bytes = 0;
-    if (hipMalloc((void **)&c->d_bound, need) != hipSuccess) {
-      c->d_bound = nullptr;
-      cleanup();
-      return MNHN_ERR_NOMEM;
-    }
-    c->bound_bytes = need;
-  }
-
-  KArgs a{};
-  a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
-  a.gtgt = own_groups ? d_gt : c->d_gtgt;
-  a.gmax = own_groups ? d_gm : c->d_gmax;
-  a.gmin = own_groups ? d_gn : c->d_gmin;
-  a.items = d_items; a.qids = d_qids; a.counter = c->d_counter;
-  a.bound = c->d_bound; a.dist_out = d_dist; a.keys = d_keys; a.nkeys = c->d_nkeys;
-  a.key_cap = key_cap; a.n_items = (int)w.items.size(); a.bound_stride = stride;
-  a.mode = mode; a.triangle = triangle; a.eps_max = eps_max;
-  a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
-
+  int waves_used = 0;
   (void)hipEventRecord(c->ev[0], c->stream);
-  {
-    void *args[] = {&a};
-    e = hipLaunchKernel(kp, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+  for (auto &pass : passes) {
+    const int PK = pass.first;
+    const std::vector<int4> &items = pass.second;
+    void *kp = kernel_ptr(PK, S, c->acgt);
+    int wpc = wpc_req;
+    // measured on MI355X (config-2 workload): the LDS kernels at S = 32 run
+    // best with 3 of their 4 possible waves per SIMD resident
+    if (wpc <= 0 && (PK == KERNEL_LDS || PK == KERNEL_LDS_F16) && S == 32) wpc = 12;
+    if (wpc <= 0) {
+      int blocks_per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) !=
+              hipSuccess || blocks_per_cu <= 0)
+        blocks_per_cu = 1;
+      wpc = blocks_per_cu * (kBlock / kWave);
+    }
+    int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
+    const int max_useful = (int)((items.size() + 3) / 4);
+    if (blocks > max_useful) blocks = std::max(1, max_useful);
+    const int waves = blocks * (kBlock / kWave);
+    waves_used = std::max(waves_used, waves);
+    const size_t need = (size_t)waves * stride * sizeof(uint32_t);
+    if (need > c->bound_bytes) {
+      e = hipStreamSynchronize(c->stream);  // an earlier pass may still use it
+      dev_free(c->d_bound);
+      c->bound_bytes = 0;
+      if (e != hipSuccess || hipMalloc((void **)&c->d_bound, need) != hipSuccess) {
+        c->d_bound = nullptr;
+        cleanup();
+        return MNHN_ERR_NOMEM;
+      }
+      c->bound_bytes = need;
+    }
+    e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                       c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(int), c->stream);
+    KArgs a{};
+    a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
+    a.gtgt = own_groups ? d_gt : c->d_gtgt;
+    a.gmax = own_groups ? d_gm : c->d_gmax;
+    a.gmin = own_groups ? d_gn : c->d_gmin;
+    a.items = d_items; a.qids = d_qids; a.counter = c->d_counter;
+    a.bound = c->d_bound; a.dist_out = d_dist; a.keys = d_keys; a.nkeys = c->d_nkeys;
+    a.key_cap = key_cap; a.n_items = (int)items.size(); a.bound_stride = stride;
+    a.mode = mode; a.triangle = triangle; a.eps_max = eps_max;
+    a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
+    if (e == hipSuccess) {
+      void *args[] = {&a};
+      e = hipLaunchKernel(kp, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) break;
+    // the next pass reuses d_items: wait until this one has read them
+    if (passes.size() > 1) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) break;
   }
-  if (e == hipSuccess) e = hipGetLastError();
   (void)hipEventRecord(c->ev[1], c->stream);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
   float ms = 0.f;
@@ -792,7 +1024,7 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   c->last.lane_bits = 16;
   c->last.strip = S;
   c->last.kernel = K;
-  c->last.waves = waves;
+  c->last.waves = waves_used;
   c->last.items = (int)w.items.size();
   return MNHN_OK;
 }
@@ -858,7 +1090,8 @@ Work triangle_work(const mnhn_sw_ctx *c, int S) {
     const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
     for (int p = 0; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
   }
-  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 64.0), 2.0e5);
+  // ~256 items per CU: enough for the longest-first queue to balance the tail
+  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 256.0), 2.0e5);
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     if (last <= 0) continue;
@@ -1041,7 +1274,7 @@ int mnhn_sw_last_timing(const mnhn_sw_ctx *c, mnhn_sw_timing *out) {
 }
 
 int mnhn_sw_set_tuning(mnhn_sw_ctx *c, int kernel, int strip, int waves_per_cu) {
-  if (!c || kernel < 0 || kernel > 2 || (strip != 0 && strip != 16 && strip != 32 && strip != 64) ||
+  if (!c || kernel < 0 || kernel > 3 || (strip != 0 && strip != 16 && strip != 32 && strip != 64) ||
       waves_per_cu < 0)
     return MNHN_ERR_ARG;
   c->kernel = kernel;
@@ -1324,3 +1557,164 @@ void mnhn_csr_free(mnhn_csr *csr) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// autotuner: times every candidate (kernel variant, strip width, waves per
+// CU) on a strided sample of this context's all-pairs work items and keeps
+// the fastest; choices are cached as JSON lines keyed by device and length
+// profile.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Cand {
+  int kernel, strip, wpc;
+};
+
+std::string tune_cache_path() {
+  const char *e = getenv("MNHN_SW_TUNE_CACHE");
+  if (e && *e) return e;
+  const char *home = getenv("HOME");
+  return std::string(home && *home ? home : "/tmp") + "/.cache/mnhn_sw_tune.jsonl";
+}
+
+std::string tune_key(const mnhn_sw_ctx *c) {
+  hipDeviceProp_t prop;
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) arch = prop.gcnArchName;
+  std::vector<int32_t> L(c->len);
+  std::sort(L.begin(), L.end());
+  const int med = L.empty() ? 0 : L[L.size() / 2];
+  const int p90 = L.empty() ? 0 : L[(L.size() * 9) / 10];
+  // lengths bucketed on a quarter-octave scale
+  auto bucket = [](int x) { return x <= 1 ? 0 : (int)std::lround(4.0 * std::log2((double)x)); };
+  char buf[256];
+  snprintf(buf, sizeof(buf), "%s|cu%d|%s|m%d|p%d|g%d|mm%d", arch.c_str(), c->num_cu,
+           c->acgt ? "acgt" : "codes", bucket(med), bucket(p90), c->gap, c->match - c->mismatch);
+  return buf;
+}
+
+bool read_tune_cache(const std::string &key, Cand *out, double *gcups) {
+  FILE *f = fopen(tune_cache_path().c_str(), "r");
+  if (!f) return false;
+  char line[1024];
+  bool found = false;
+  const std::string needle = "\"key\": \"" + key + "\"";
+  while (fgets(line, sizeof(line), f)) {
+    if (!strstr(line, needle.c_str())) continue;
+    Cand t{0, 0, 0};
+    double g = 0;
+    const char *k = strstr(line, "\"kernel\": ");
+    const char *st = strstr(line, "\"strip\": ");
+    const char *w = strstr(line, "\"waves_per_cu\": ");
+    const char *gc = strstr(line, "\"gcups\": ");
+    if (k && st && w && gc && sscanf(k + 10, "%d", &t.kernel) == 1 && sscanf(st + 9, "%d", &t.strip) == 1 &&
+        sscanf(w + 16, "%d", &t.wpc) == 1 && sscanf(gc + 9, "%lf", &g) == 1) {
+      *out = t;  // the last matching line wins
+      *gcups = g;
+      found = true;
+    }
+  }
+  fclose(f);
+  return found;
+}
+
+void write_tune_cache(const std::string &key, const Cand &t, double gcups) {
+  const std::string path = tune_cache_path();
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos && slash > 0) {
+    const std::string dir = path.substr(0, slash);
+    (void)mkdir(dir.c_str(), 0755);
+  }
+  FILE *f = fopen(path.c_str(), "a");
+  if (!f) return;
+  fprintf(f, "{\"key\": \"%s\", \"kernel\": %d, \"strip\": %d, \"waves_per_cu\": %d, \"gcups\": %.1f}\n",
+          key.c_str(), t.kernel, t.strip, t.wpc, gcups);
+  fclose(f);
+}
+
+}  // namespace
+
+extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *out) {
+  if (!c) return MNHN_ERR_ARG;
+  int rc = check_range(c);
+  if (rc) return rc;
+  const std::string key = tune_key(c);
+  Cand best{0, 0, 0};
+  double best_gcups = 0;
+  if (use_cache && read_tune_cache(key, &best, &best_gcups)) {
+    c->kernel = best.kernel;
+    c->strip = best.strip;
+    c->waves_per_cu = best.wpc;
+    if (out) *out = mnhn_sw_tuning{best.kernel, best.strip, best.wpc, best_gcups, 1};
+    return MNHN_OK;
+  }
+  std::vector<Cand> cands;
+  if (c->acgt) {
+    for (int w : {8, 12, 16}) cands.push_back({KERNEL_LDS_F16, 32, w});
+    for (int w : {8, 16, 24}) cands.push_back({KERNEL_LDS_F16, 16, w});
+    for (int w : {8, 12}) cands.push_back({KERNEL_LDS_F16, 64, w});
+    for (int w : {8, 16, 24}) cands.push_back({KERNEL_LDS, 16, w});
+    for (int w : {8, 12, 16}) cands.push_back({KERNEL_LDS, 32, w});
+    for (int w : {8, 12}) cands.push_back({KERNEL_LDS, 64, w});
+  }
+  for (int w : {4, 8}) cands.push_back({KERNEL_REG, 32, w});
+  for (int w : {8, 16}) cands.push_back({KERNEL_REG, 16, w});
+  // sample of the all-pairs work: single-query items (one query against one
+  // target group) strided over all (group, query) pairs, about 8 per resident
+  // wave at the largest candidate occupancy -- a few hundred milliseconds of
+  // work per candidate whatever the data-set size
+  std::vector<double> gsum(c->groups, 0.0);
+  double units = 0;
+  for (int g = 0; g < c->groups; ++g) {
+    for (int t = 0; t < kGroup; ++t) {
+      const int id = c->gtgt[(size_t)g * kGroup + t];
+      if (id >= 0) gsum[g] += c->len[id];
+    }
+    units += std::max(std::min((g + 1) * kGroup, c->n) - 1, 0);
+  }
+  const double want = (double)std::max(c->num_cu, 1) * 24 * 8;
+  const double stride = std::max(1.0, units / want);
+  Work sample;
+  sample.qids = c->order;
+  double next = 0, u = 0;
+  for (int g = 0; g < c->groups; ++g) {
+    const int last = std::min((g + 1) * kGroup, c->n) - 1;
+    for (int p = 0; p < last; ++p, u += 1) {
+      if (u < next) continue;
+      next += stride;
+      sample.items.push_back(make_int4(g, p, p + 1, 0));
+      sample.cells += (double)c->len[c->order[p]] * gsum[g];
+    }
+  }
+  finalize_items(c, sample, 32, false);
+  const int saved_k = c->kernel, saved_s = c->strip, saved_w = c->waves_per_cu;
+  for (const Cand &cd : cands) {
+    c->kernel = cd.kernel;
+    c->strip = cd.strip;
+    c->waves_per_cu = cd.wpc;
+    double ms = 1e30;
+    for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
+      rc = launch_tiles(c, sample, MODE_EPS, 1, -1, nullptr, nullptr, 0);
+      if (rc) break;
+      ms = std::min(ms, c->last.kernel_ms);
+    }
+    if (rc) break;
+    const double g = sample.cells / std::max(ms, 1e-6) / 1e6;
+    if (g > best_gcups) {
+      best_gcups = g;
+      best = cd;
+    }
+  }
+  if (rc || best.kernel == 0) {
+    c->kernel = saved_k;
+    c->strip = saved_s;
+    c->waves_per_cu = saved_w;
+    return rc ? rc : MNHN_ERR_DEVICE;
+  }
+  c->kernel = best.kernel;
+  c->strip = best.strip;
+  c->waves_per_cu = best.wpc;
+  write_tune_cache(key, best, best_gcups);
+  if (out) *out = mnhn_sw_tuning{best.kernel, best.strip, best.wpc, best_gcups, 0};
+  return MNHN_OK;
+}

@@ -176,3 +176,37 @@ def test_config2_full_size_properties(pkg, oracle):
     for (i, j), d in zip(ij.tolist(), want.tolist()):
         if i != j:
             assert ((i * n + j) in keyset) == (d <= eps)
+
+
+def test_autotune_choice_is_exact_and_cached(pkg, oracle, tmp_path, monkeypatch):
+    monkeypatch.setenv("MNHN_SW_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
+    seqs = synth.config1(seed=31, n_anc=8, per_anc=60)
+    _say(f"autotune n={len(seqs)}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    with pkg.Engine(seqs) as eng:
+        t1 = eng.autotune(use_cache=True)
+        assert t1["from_cache"] == 0 and t1["kernel"] in (1, 2, 3) and t1["gcups"] > 0
+        got = eng.matrix()
+        assert eng.last_timing()["strip"] == t1["strip"]
+    assert np.array_equal(got, want)
+    with pkg.Engine(seqs) as eng:
+        t2 = eng.autotune(use_cache=True)
+        assert t2["from_cache"] == 1
+        assert (t2["kernel"], t2["strip"], t2["waves_per_cu"]) == (t1["kernel"], t1["strip"], t1["waves_per_cu"])
+        assert np.array_equal(eng.matrix(), want)
+
+
+@pytest.mark.parametrize("kernel,strip,wpc", [(1, 16, 0), (1, 32, 4), (2, 16, 24), (2, 32, 12), (2, 64, 8), (3, 16, 16), (3, 32, 12), (3, 64, 8)])
+def test_every_tuning_is_exact(pkg, oracle, kernel, strip, wpc):
+    rng = np.random.default_rng(kernel * 100 + strip)
+    seqs = _rand_set(rng, 200, 0, 520)
+    _say(f"tuning kernel={kernel} strip={strip} wpc={wpc}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    with pkg.Engine(seqs) as eng:
+        eng.set_tuning(kernel, strip, wpc)
+        got = eng.matrix()
+        row = eng.one_vs_all(5)
+        t = eng.last_timing()
+    assert (t["kernel"], t["strip"]) == (kernel, strip)
+    assert np.array_equal(got, want)
+    assert np.array_equal(row, want[5])

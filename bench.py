@@ -104,6 +104,21 @@ def cpu_baseline(seqs, sample_n, seed=7):
                       f"host: {cpu_model()}, nproc {os.cpu_count()}"}
 
 
+def profiled_traffic(config):
+    """HBM bytes per step of the SW kernels, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    plain copy kernel); None if there is no summary for it."""
+    path = os.path.join(ROOT, "profiles", {"c2": "sw_lds_f16_kernel_config2_20k.md"}.get(config, "-"))
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_step:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_dbscan(seqs, eps_start, eps_inc, minpts):
     exe = os.path.join(PKG_DIR, "bin", "adaptive_clustering_SW")
     with tempfile.TemporaryDirectory() as tmp:
@@ -182,6 +197,14 @@ def main():
         print(f"step {k}: {times[-1] * 1e3:.1f} ms (kernel {kms[-1]:.1f} ms), "
               f"{cells / times[-1] / 1e9:.1f} GCUPS, nnz {nnz}", file=sys.stderr, flush=True)
     t = eng.last_timing()
+    # the same step with the int16 form of the LDS kernel only (integer VALU path)
+    eng.set_tuning(2, 32, 16)
+    t0 = time.perf_counter()
+    eng.eps_count(args.eps_max)
+    int16_s = time.perf_counter() - t0
+    int16_kernel_ms = eng.last_timing()["kernel_ms"]
+    print(f"int16-only step: {int16_s * 1e3:.1f} ms, {cells / int16_s / 1e9:.1f} GCUPS",
+          file=sys.stderr, flush=True)
     eng.close()
     step_s = float(np.mean(times))
     kernel_s = float(np.mean(kms)) / 1e3
@@ -195,14 +218,20 @@ def main():
         "config": {"workload": WORKLOADS[args.config],
                    "n_seqs": len(seqs), "unique_pairs": pairs, "cells": cells,
                    "eps_max": args.eps_max, "eps_list_entries": nnz,
-                   "kernel": {1: "register-profile", 2: "lds-profile"}.get(t["kernel"]),
+                   "kernel": {1: "register-profile int16", 2: "lds-profile int16",
+                              3: "lds-profile exact fp16 (+int16 pass)"}.get(t["kernel"]),
                    "strip": t["strip"], "waves": t["waves"], "items": t["items"],
                    "autotune": tuning},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT16_TOPS,
-                     "unit": "TOP/s (int16, 6 ops/cell)", "frac": achieved / PEAK_INT16_TOPS,
-                     "traffic": None, "kernel": "sw_tiles_kernel",
+                     "unit": "TOP/s (packed 16-bit VALU; 6 algorithmic ops per cell)",
+                     "frac": achieved / PEAK_INT16_TOPS,
+                     "traffic": profiled_traffic(args.config),
+                     "kernel": "sw_lds_f16_kernel + sw_lds_kernel" if t["kernel"] == 3
+                               else {1: "sw_tiles_kernel", 2: "sw_lds_kernel"}.get(t["kernel"]),
                      "kernel_ms": kernel_s * 1e3,
-                     "kernel_gcups": cells / kernel_s / 1e9},
+                     "kernel_gcups": cells / kernel_s / 1e9,
+                     "int16_only_gcups": cells / int16_s / 1e9,
+                     "int16_only_kernel_ms": int16_kernel_ms},
     }
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)

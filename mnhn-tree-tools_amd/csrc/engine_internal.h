@@ -1,0 +1,93 @@
+// engine_internal.h -- shared internals of the gfx950 engine library (not
+// part of the C ABI): the context layout, device-memory helpers and the entry
+// points the learned-proposer code reuses.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "mnhn_sw.h"
+
+struct mnhn_sw_ctx {
+  int n = 0;
+  bool acgt = true;
+  int match = 4, mismatch = -4, gap = 3;
+  std::vector<int32_t> len, order, rank;
+  std::vector<uint32_t> woff;
+  int groups = 0;
+  std::vector<int32_t> gtgt, gmax, gmin;
+  int32_t maxlen = 0;
+  int device = 0, num_cu = 0;
+  int kernel = 0, strip = 0, waves_per_cu = 0;  // 0 = tuned default
+  uint32_t *d_words = nullptr, *d_woff = nullptr;
+  int32_t *d_len = nullptr, *d_rank = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr,
+          *d_gmin = nullptr;
+  int *d_counter = nullptr;
+  unsigned long long *d_nkeys = nullptr;
+  uint32_t *d_bound = nullptr;
+  size_t bound_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  mnhn_sw_timing last{};
+  // device-resident epsilon lists of the last mnhn_sw_eps_count call
+  unsigned long long *d_keys = nullptr;
+  size_t keys_cap = 0;
+};
+
+namespace mnhn_detail {
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t e_ = (expr);                            \
+    if (e_ != hipSuccess) return MNHN_ERR_DEVICE;      \
+  } while (0)
+
+template <typename T>
+int dev_alloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return MNHN_ERR_NOMEM;
+  }
+  return MNHN_OK;
+}
+
+template <typename T>
+void dev_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct Work {
+  std::vector<int4> items;
+  std::vector<int32_t> qids;
+  std::vector<int32_t> gtgt, gmax, gmin;  // empty = use the context's length groups
+  double cells = 0, pairs = 0;
+  int max_q_len = 0;
+  // device-resident work (used instead of the vectors when dev is set)
+  bool dev = false;
+  const int4 *d_items = nullptr;
+  int n_items = 0;
+  const int32_t *d_qids = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr, *d_gmin = nullptr;
+  int dev_kernel = 0;  // kernel variant for device work (KERNEL_LDS or KERNEL_REG)
+};
+
+
+// Launch the all-pairs / pairs / one-vs-all kernels over a work list (host
+// item list, or device-resident items and target groups when w.dev is set).
+int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
+                 int32_t *d_dist, unsigned long long *d_keys, unsigned long long key_cap);
+
+// Group the ctx's epsilon keys (c->d_keys, nk of them) into CSR rows.
+int csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out);
+
+// Make sure c->d_keys holds at least cap keys.
+int ensure_keys(mnhn_sw_ctx *c, unsigned long long cap);
+
+enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3 };
+enum { MODE_DIST = 0, MODE_EPS = 1 };
+constexpr int kGroupSize = 128;
+
+}  // namespace mnhn_detail

@@ -50,6 +50,9 @@
 #include <sys/stat.h>
 
 #include "mnhn_sw.h"
+#include "engine_internal.h"
+
+using namespace mnhn_detail;
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
@@ -62,7 +65,6 @@ constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr short kFloor = -32768;     // biased representation of H = 0
 constexpr int kBias = 32768;
 
-enum { MODE_DIST = 0, MODE_EPS = 1 };
 
 struct KArgs {
   const uint32_t *words;   // packed residues of all sequences
@@ -783,68 +785,10 @@ __global__ __launch_bounds__(kSortThreads) void sort_rows_kernel(
 // host side
 // ----------------------------------------------------------------------------
 
-struct mnhn_sw_ctx {
-  int n = 0;
-  bool acgt = true;
-  int match = 4, mismatch = -4, gap = 3;
-  std::vector<int32_t> len, order, rank;
-  std::vector<uint32_t> woff;
-  int groups = 0;
-  std::vector<int32_t> gtgt, gmax, gmin;
-  int32_t maxlen = 0;
-  int device = 0, num_cu = 0;
-  int kernel = 0, strip = 0, waves_per_cu = 0;  // 0 = tuned default
-  uint32_t *d_words = nullptr, *d_woff = nullptr;
-  int32_t *d_len = nullptr, *d_rank = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr,
-          *d_gmin = nullptr;
-  int *d_counter = nullptr;
-  unsigned long long *d_nkeys = nullptr;
-  uint32_t *d_bound = nullptr;
-  size_t bound_bytes = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  mnhn_sw_timing last{};
-  // device-resident epsilon lists of the last mnhn_sw_eps_count call
-  unsigned long long *d_keys = nullptr;
-  size_t keys_cap = 0;
-};
-
 namespace {
-
-#define HIP_TRY(expr)                                  \
-  do {                                                 \
-    hipError_t e_ = (expr);                            \
-    if (e_ != hipSuccess) return MNHN_ERR_DEVICE;      \
-  } while (0)
-
-template <typename T>
-int dev_alloc(T **p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
-    *p = nullptr;
-    return MNHN_ERR_NOMEM;
-  }
-  return MNHN_OK;
-}
-
-template <typename T>
-void dev_free(T *&p) {
-  if (p) (void)hipFree(p);
-  p = nullptr;
-}
-
-struct Work {
-  std::vector<int4> items;
-  std::vector<int32_t> qids;
-  std::vector<int32_t> gtgt, gmax, gmin;  // empty = use the context's length groups
-  double cells = 0, pairs = 0;
-  int max_q_len = 0;
-};
 
 // Kernel variants: the LDS query-profile kernel (residues in <= 4 codes) and
 // the register-profile kernel (any alphabet).
-enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3 };
 
 int pick_kernel(const mnhn_sw_ctx *c) {
   int k = c->kernel;
@@ -887,45 +831,53 @@ void *kernel_ptr(int k, int S, bool acgt) {
               : reinterpret_cast<void *>(&sw_tiles_kernel<32, false>);
 }
 
-int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
+}  // namespace
+
+int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
                  int32_t *d_dist, unsigned long long *d_keys,
                  unsigned long long key_cap) {
-  if (w.items.empty()) return MNHN_OK;
+  const size_t n_items_total = w.dev ? (size_t)w.n_items : w.items.size();
+  if (n_items_total == 0) return MNHN_OK;
   int rc;
   int4 *d_items = nullptr;
   int32_t *d_qids = nullptr, *d_gt = nullptr, *d_gm = nullptr, *d_gn = nullptr;
-  if ((rc = dev_alloc(&d_items, w.items.size()))) return rc;
-  if ((rc = dev_alloc(&d_qids, w.qids.size()))) { dev_free(d_items); return rc; }
-  const bool own_groups = !w.gtgt.empty();
-  if (own_groups) {
-    if ((rc = dev_alloc(&d_gt, w.gtgt.size())) || (rc = dev_alloc(&d_gm, w.gmax.size())) ||
-        (rc = dev_alloc(&d_gn, w.gmin.size()))) {
-      dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm); dev_free(d_gn);
-      return rc;
+  const bool own_groups = w.dev || !w.gtgt.empty();
+  if (!w.dev) {
+    if ((rc = dev_alloc(&d_items, w.items.size()))) return rc;
+    if ((rc = dev_alloc(&d_qids, w.qids.size()))) { dev_free(d_items); return rc; }
+    if (own_groups) {
+      if ((rc = dev_alloc(&d_gt, w.gtgt.size())) || (rc = dev_alloc(&d_gm, w.gmax.size())) ||
+          (rc = dev_alloc(&d_gn, w.gmin.size()))) {
+        dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm); dev_free(d_gn);
+        return rc;
+      }
     }
   }
   auto cleanup = [&]() {
     dev_free(d_items); dev_free(d_qids); dev_free(d_gt); dev_free(d_gm); dev_free(d_gn);
   };
   hipError_t e = hipSuccess;
-  e = hipMemcpyAsync(d_items, w.items.data(), w.items.size() * sizeof(int4),
-                     hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess)
+  if (!w.dev) {
     e = hipMemcpyAsync(d_qids, w.qids.data(), w.qids.size() * sizeof(int32_t),
                        hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess && own_groups) {
-    e = hipMemcpyAsync(d_gt, w.gtgt.data(), w.gtgt.size() * sizeof(int32_t),
-                       hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_gm, w.gmax.data(), w.gmax.size() * sizeof(int32_t),
+    if (e == hipSuccess && own_groups) {
+      e = hipMemcpyAsync(d_gt, w.gtgt.data(), w.gtgt.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_gn, w.gmin.data(), w.gmin.size() * sizeof(int32_t),
-                         hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(d_gm, w.gmax.data(), w.gmax.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(d_gn, w.gmin.data(), w.gmin.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c->stream);
+    }
   }
   if (e != hipSuccess) { cleanup(); return MNHN_ERR_DEVICE; }
+  const int32_t *q_ptr = w.dev ? w.d_qids : d_qids;
+  const int32_t *gt_ptr = w.dev ? w.d_gtgt : (own_groups ? d_gt : c->d_gtgt);
+  const int32_t *gm_ptr = w.dev ? w.d_gmax : (own_groups ? d_gm : c->d_gmax);
+  const int32_t *gn_ptr = w.dev ? w.d_gmin : (own_groups ? d_gn : c->d_gmin);
 
-  const int K = pick_kernel(c);
+  const int K = w.dev ? (w.dev_kernel ? w.dev_kernel : KERNEL_LDS) : pick_kernel(c);
   const int S = pick_strip(c);
   int wpc_req = c->waves_per_cu;
   if (wpc_req <= 0) {
@@ -935,7 +887,9 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   // passes: (kernel, items).  The fp16 variant takes the items whose pairs
   // all satisfy match * min(la, lb) <= 2048; the rest run the int16 LDS kernel.
   std::vector<std::pair<int, std::vector<int4>>> passes;
-  if (K == KERNEL_LDS_F16) {
+  if (w.dev) {
+    passes.push_back({K == KERNEL_LDS_F16 ? KERNEL_LDS : K, {}});
+  } else if (K == KERNEL_LDS_F16) {
     std::vector<int4> f16, i16;
     for (const int4 &it : w.items) {
       const int gm = own_groups ? w.gmax[it.x] : c->gmax[it.x];
@@ -959,6 +913,7 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   for (auto &pass : passes) {
     const int PK = pass.first;
     const std::vector<int4> &items = pass.second;
+    const size_t n_items = w.dev ? (size_t)w.n_items : items.size();
     void *kp = kernel_ptr(PK, S, c->acgt);
     int wpc = wpc_req;
     // measured on MI355X (config-2 workload): the LDS kernels at S = 32 run
@@ -972,7 +927,7 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
       wpc = blocks_per_cu * (kBlock / kWave);
     }
     int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
-    const int max_useful = (int)((items.size() + 3) / 4);
+    const int max_useful = (int)((n_items + 3) / 4);
     if (blocks > max_useful) blocks = std::max(1, max_useful);
     const int waves = blocks * (kBlock / kWave);
     waves_used = std::max(waves_used, waves);
@@ -988,17 +943,18 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
       }
       c->bound_bytes = need;
     }
-    e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
-                       c->stream);
+    if (!w.dev)
+      e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                         c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(int), c->stream);
     KArgs a{};
     a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
-    a.gtgt = own_groups ? d_gt : c->d_gtgt;
-    a.gmax = own_groups ? d_gm : c->d_gmax;
-    a.gmin = own_groups ? d_gn : c->d_gmin;
-    a.items = d_items; a.qids = d_qids; a.counter = c->d_counter;
+    a.gtgt = gt_ptr;
+    a.gmax = gm_ptr;
+    a.gmin = gn_ptr;
+    a.items = w.dev ? w.d_items : d_items; a.qids = q_ptr; a.counter = c->d_counter;
     a.bound = c->d_bound; a.dist_out = d_dist; a.keys = d_keys; a.nkeys = c->d_nkeys;
-    a.key_cap = key_cap; a.n_items = (int)items.size(); a.bound_stride = stride;
+    a.key_cap = key_cap; a.n_items = (int)n_items; a.bound_stride = stride;
     a.mode = mode; a.triangle = triangle; a.eps_max = eps_max;
     a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
     if (e == hipSuccess) {
@@ -1025,9 +981,11 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
   c->last.strip = S;
   c->last.kernel = K;
   c->last.waves = waves_used;
-  c->last.items = (int)w.items.size();
+  c->last.items = (int)n_items_total;
   return MNHN_OK;
 }
+
+namespace {
 
 // Split the query ranges [0, nq_g) of each group into items of similar cost.
 void add_group_items(const mnhn_sw_ctx *c, Work &w, int g, int q_begin, int q_end,
@@ -1430,15 +1388,7 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
     cap = std::max<unsigned long long>(std::min(all, budget), 1024ull);
   }
   for (int attempt = 0; attempt < 2; ++attempt) {
-    if (cap > c->keys_cap) {
-      dev_free(c->d_keys);
-      c->keys_cap = 0;
-      if (hipMalloc((void **)&c->d_keys, cap * sizeof(unsigned long long)) != hipSuccess) {
-        c->d_keys = nullptr;
-        return MNHN_ERR_NOMEM;
-      }
-      c->keys_cap = cap;
-    }
+    if ((rc = ensure_keys(c, cap))) return rc;
     HIP_TRY(hipMemsetAsync(c->d_nkeys, 0, sizeof(unsigned long long), c->stream));
     rc = launch_tiles(c, w, MODE_EPS, 1, eps_max, nullptr, c->d_keys, c->keys_cap);
     if (rc) return rc;
@@ -1477,6 +1427,28 @@ int mnhn_sw_eps_csr(mnhn_sw_ctx *c, int32_t eps_max, mnhn_csr *out) {
   unsigned long long nk = 0;
   int rc = eps_pass(c, eps_max, &nk);
   if (rc) { mnhn_csr_free(out); return rc; }
+  return csr_from_keys(c, nk, out);
+}
+
+}  // extern "C"
+
+int mnhn_detail::ensure_keys(mnhn_sw_ctx *c, unsigned long long cap) {
+  if (cap <= c->keys_cap && c->d_keys) return MNHN_OK;
+  dev_free(c->d_keys);
+  c->keys_cap = 0;
+  if (hipMalloc((void **)&c->d_keys, std::max<unsigned long long>(cap, 1) * sizeof(unsigned long long)) !=
+      hipSuccess) {
+    c->d_keys = nullptr;
+    return MNHN_ERR_NOMEM;
+  }
+  c->keys_cap = cap;
+  return MNHN_OK;
+}
+
+// out->row_ptr must be allocated ([n+1], zeroed) and out->n_rows set.
+int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
+  const int n = c->n;
+  int rc = MNHN_OK;
   const double kernel_ms = c->last.kernel_ms;
   // LDS of the long-row path: column bitmap + per-word prefix + thread totals
   const size_t lds_bytes = std::max<size_t>(
@@ -1544,6 +1516,8 @@ int mnhn_sw_eps_csr(mnhn_sw_ctx *c, int32_t eps_max, mnhn_csr *out) {
   c->last.total_ms = kernel_ms + post_ms;
   return MNHN_OK;
 }
+
+extern "C" {
 
 void mnhn_csr_free(mnhn_csr *csr) {
   if (!csr) return;

@@ -33,6 +33,12 @@ ABI_SYMBOLS = [
     "mnhn_sw_size", "mnhn_sw_alphabet", "mnhn_sw_last_timing",
     "mnhn_sw_set_tuning", "mnhn_sw_autotune", "mnhn_sw_strerror",
 ]
+PROPOSER_SYMBOLS = [
+    "mnhn_kmer_profiles", "mnhn_encoder_create", "mnhn_encoder_destroy",
+    "mnhn_encoder_train", "mnhn_encoder_embed", "mnhn_encoder_set_margin",
+    "mnhn_encoder_margin", "mnhn_encoder_save", "mnhn_encoder_load",
+    "mnhn_prefilter_eps_csr",
+]
 HOST_SYMBOLS = [
     "dataset_from_fasta", "free_sequences_from_dataset", "store_split_set",
     "read_split_set", "create_single_cluster_file", "create_cluster_files",
@@ -74,6 +80,24 @@ class Tuning(ctypes.Structure):
                 ("from_cache", ctypes.c_int32)]
 
 
+class TrainParams(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
+                ("eps_train", ctypes.c_int32), ("neighbor_weight", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+class TrainStats(ctypes.Structure):
+    _fields_ = [("first_loss", ctypes.c_double), ("final_loss", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("steps", ctypes.c_int64)]
+
+
+class PrefilterStats(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int64), ("neighbours", ctypes.c_int64),
+                ("profile_ms", ctypes.c_double), ("embed_ms", ctypes.c_double),
+                ("propose_ms", ctypes.c_double), ("verify_ms", ctypes.c_double),
+                ("csr_ms", ctypes.c_double)]
+
+
 _LIB = None
 
 
@@ -104,6 +128,20 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_sw_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Tuning)]
     lib.mnhn_sw_strerror.argtypes = [ctypes.c_int]
     lib.mnhn_sw_strerror.restype = ctypes.c_char_p
+    lib.mnhn_kmer_profiles.argtypes = [vp, vp]
+    lib.mnhn_encoder_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(vp)]
+    lib.mnhn_encoder_destroy.argtypes = [vp]
+    lib.mnhn_encoder_destroy.restype = None
+    lib.mnhn_encoder_train.argtypes = [vp, vp, vp, vp, i64, ctypes.POINTER(TrainParams),
+                                       ctypes.POINTER(TrainStats)]
+    lib.mnhn_encoder_embed.argtypes = [vp, vp, vp]
+    lib.mnhn_encoder_set_margin.argtypes = [vp, ctypes.c_float]
+    lib.mnhn_encoder_margin.argtypes = [vp]
+    lib.mnhn_encoder_margin.restype = ctypes.c_float
+    lib.mnhn_encoder_save.argtypes = [vp, ctypes.c_char_p]
+    lib.mnhn_encoder_load.argtypes = [vp, ctypes.c_char_p]
+    lib.mnhn_prefilter_eps_csr.argtypes = [vp, vp, i32, ctypes.POINTER(Csr),
+                                           ctypes.POINTER(PrefilterStats)]
     _LIB = lib
     return lib
 
@@ -209,3 +247,105 @@ class Engine:
         t = Timing()
         _check(self.lib.mnhn_sw_last_timing(self.ctx, ctypes.byref(t)), "last_timing")
         return {f: getattr(t, f) for f, _ in Timing._fields_}
+
+    def kmer_profiles(self) -> np.ndarray:
+        """5-mer profiles (n x 1024 fp32, L1-normalised) computed on the GPU."""
+        out = np.empty((self.n, 1024), dtype=np.float32)
+        _check(self.lib.mnhn_kmer_profiles(self.ctx, out.ctypes.data), "mnhn_kmer_profiles")
+        return out
+
+
+def _csr_to_numpy(lib, csr, n):
+    try:
+        nnz = int(csr.nnz)
+        rp = np.ctypeslib.as_array(csr.row_ptr, shape=(n + 1,)).copy()
+        col = (np.ctypeslib.as_array(csr.col, shape=(nnz,)).copy() if nnz
+               else np.zeros(0, np.int32))
+        dist = (np.ctypeslib.as_array(csr.dist, shape=(nnz,)).copy() if nnz
+                else np.zeros(0, np.int16))
+    finally:
+        lib.mnhn_csr_free(ctypes.byref(csr))
+    return rp, col, dist
+
+
+class Encoder:
+    """Learned candidate proposer (mnhn_encoder): 5-mer profile -> 128-d MLP."""
+
+    def __init__(self, seed: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self.lib.mnhn_encoder_create(ctypes.c_uint64(seed), ctypes.byref(h)),
+               "mnhn_encoder_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mnhn_encoder_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def train(self, eng: Engine, ij: np.ndarray, dist: np.ndarray, epochs: int = 4,
+              batch: int = 4096, lr: float = 1e-3, eps_train: int = 400,
+              neighbor_weight: float = 4.0, seed: int = 0) -> dict:
+        ij = np.ascontiguousarray(ij, dtype=np.int32).reshape(-1, 2)
+        dist = np.ascontiguousarray(dist, dtype=np.int32)
+        p = TrainParams(epochs, batch, lr, eps_train, neighbor_weight, seed)
+        st = TrainStats()
+        _check(self.lib.mnhn_encoder_train(self.h, eng.ctx, ij.ctypes.data, dist.ctypes.data,
+                                           len(ij), ctypes.byref(p), ctypes.byref(st)),
+               "mnhn_encoder_train")
+        return {f: getattr(st, f) for f, _ in TrainStats._fields_}
+
+    def embed(self, eng: Engine) -> np.ndarray:
+        out = np.empty((eng.n, 128), dtype=np.float32)
+        _check(self.lib.mnhn_encoder_embed(self.h, eng.ctx, out.ctypes.data), "mnhn_encoder_embed")
+        return out
+
+    @property
+    def margin(self) -> float:
+        return float(self.lib.mnhn_encoder_margin(self.h))
+
+    @margin.setter
+    def margin(self, v: float) -> None:
+        _check(self.lib.mnhn_encoder_set_margin(self.h, float(v)), "mnhn_encoder_set_margin")
+
+    def save(self, path: str) -> None:
+        _check(self.lib.mnhn_encoder_save(self.h, path.encode()), "mnhn_encoder_save")
+
+    def load(self, path: str) -> None:
+        _check(self.lib.mnhn_encoder_load(self.h, path.encode()), "mnhn_encoder_load")
+
+    def prefilter_csr(self, eng: Engine, eps_max: int):
+        """(row_ptr, col, dist, stats): exact distances of the proposed pairs with d <= eps_max."""
+        csr = Csr()
+        st = PrefilterStats()
+        _check(self.lib.mnhn_prefilter_eps_csr(self.h, eng.ctx, int(eps_max), ctypes.byref(csr),
+                                               ctypes.byref(st)), "mnhn_prefilter_eps_csr")
+        rp, col, dist = _csr_to_numpy(self.lib, csr, eng.n)
+        return rp, col, dist, {f: getattr(st, f) for f, _ in PrefilterStats._fields_}
+
+
+def load_encoder_weights(path: str):
+    """Read a file written by Encoder.save: (margin, [(Wt [out][in], b [out]) x 3])."""
+    with open(path, "rb") as f:
+        if f.read(8) != b"MNHNENC1":
+            raise MnhnError(f"{path}: not an encoder file")
+        margin = float(np.frombuffer(f.read(4), "<f4")[0])
+        layers = []
+        for _ in range(3):
+            din, dout = np.frombuffer(f.read(8), "<i4")
+            w = np.frombuffer(f.read(4 * din * dout), "<f4").reshape(dout, din)
+            b = np.frombuffer(f.read(4 * dout), "<f4")
+            layers.append((w, b))
+    return margin, layers

@@ -1,0 +1,919 @@
+// proposer.hip -- learned candidate proposer (include/mnhn_proposer.h).
+//
+// Pipeline, all on the device:
+//   5-mer profiles (kmer_profile_kernel)          1024-d, L1-normalised, bf16
+//   encoder MLP 1024 -> 512 -> 256 -> 128           gemm_nt_kernel (bf16 MFMA
+//                                                   16x16x32, fp32 accumulate)
+//                                                   with fused bias + GELU
+//   training: pair loss, backward GEMMs, Adam       fp32 master weights
+//   pairwise embedding distance + candidates      pair_candidates_kernel
+//                                                   (bf16 MFMA, fused radius
+//                                                   test and emission)
+//   exact verification                            the engine's int16 SW kernel
+//                                                   over candidate groups,
+//                                                   epsilon keys -> CSR
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <random>
+#include <vector>
+
+#include "engine_internal.h"
+#include "mnhn_proposer.h"
+
+using namespace mnhn_detail;
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int D0 = MNHN_KMER_DIM, D1 = 512, D2 = 256, D3 = MNHN_EMBED_DIM;
+constexpr float kProfileScale = 64.0f;  // profile entries scaled from 1/L to O(0.1)
+
+// ------------------------------------------------------------------ profiles
+// One workgroup per sequence (grid-stride): 5-mer counts of the stored
+// residues (sequence positions 1 .. L-1) in an LDS histogram, then L1-normalised.
+__global__ __launch_bounds__(256) void kmer_profile_kernel(const uint32_t *__restrict__ words,
+                                                           const uint32_t *__restrict__ woff,
+                                                           const int32_t *__restrict__ len, int n,
+                                                           bf16 *__restrict__ X, float *__restrict__ Xf) {
+  __shared__ int hist[D0];
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    for (int c = threadIdx.x; c < D0; c += 256) hist[c] = 0;
+    __syncthreads();
+    const int R = len[s] > 1 ? len[s] - 1 : 0;
+    const int nk = R >= 5 ? R - 4 : 0;
+    const uint32_t wo = woff[s];
+    for (int p = threadIdx.x; p < nk; p += 256) {
+      uint32_t code = 0;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const uint32_t r = (uint32_t)(p + i);
+        code = (code << 2) | ((words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u);
+      }
+      atomicAdd(&hist[code], 1);
+    }
+    __syncthreads();
+    const float inv = nk > 0 ? kProfileScale / (float)nk : 0.0f;
+    for (int c = threadIdx.x; c < D0; c += 256) {
+      const float v = (float)hist[c] * inv;
+      X[(size_t)s * D0 + c] = (bf16)v;
+      if (Xf) Xf[(size_t)s * D0 + c] = v / kProfileScale;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------- GEMM
+// C[m][n] = sum_k A[m][k] * B[n][k]  (both operands k-contiguous), bf16 in,
+// fp32 accumulate, 64 x 64 block tile, 4 waves of 32 x 32, MFMA 16x16x32.
+// Requires K, lda, ldb multiples of 8 (16-byte rows).
+enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_OUT = 2, EPI_GELU_BWD = 3 };
+
+struct Epi {
+  float *c32 = nullptr;       // fp32 output (F32: C, BIAS_GELU: pre-activation, BIAS_OUT: value)
+  int ldc = 0;
+  bf16 *c16 = nullptr;        // bf16 output (BIAS_GELU: gelu(pre), BIAS_OUT: value, GELU_BWD: dpre)
+  int ldc16 = 0;
+  const float *bias = nullptr;
+  const float *pre = nullptr; // GELU_BWD: saved pre-activation
+  int ldp = 0;
+};
+
+__device__ __forceinline__ float gelu_f(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118f));
+  const float pdf = 0.39894228040f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16 *__restrict__ A, int lda,
+                                                      const bf16 *__restrict__ B, int ldb, int M,
+                                                      int N, int K, Epi ep) {
+  constexpr int BM = 64, BN = 64, BK = 32, LK = BK + 8;
+  __shared__ __attribute__((aligned(16))) bf16 sA[BM * LK];
+  __shared__ __attribute__((aligned(16))) bf16 sB[BN * LK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int lr = tid >> 2, lk = (tid & 3) * 8;
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    bf16x8 va, vb;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) { va[t] = (bf16)0.0f; vb[t] = (bf16)0.0f; }
+    if (m0 + lr < M && k0 + lk < K) va = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + lr) * lda + k0 + lk);
+    if (n0 + lr < N && k0 + lk < K) vb = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + lr) * ldb + k0 + lk);
+    __syncthreads();
+    *reinterpret_cast<bf16x8 *>(sA + lr * LK + lk) = va;
+    *reinterpret_cast<bf16x8 *>(sB + lr * LK + lk) = vb;
+    __syncthreads();
+    bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(sA + (32 * wm + 16 * i + fr) * LK + fk);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(sB + (32 * wn + 16 * j + fr) * LK + fk);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 32 * wm + 16 * i + 4 * (lane >> 4) + r;
+        const int nn = n0 + 32 * wn + 16 * j + (lane & 15);
+        if (m >= M || nn >= N) continue;
+        const float v = acc[i][j][r];
+        if constexpr (EPI == EPI_F32) {
+          ep.c32[(size_t)m * ep.ldc + nn] = v;
+        } else if constexpr (EPI == EPI_BIAS_GELU) {
+          const float pre = v + ep.bias[nn];
+          if (ep.c32) ep.c32[(size_t)m * ep.ldc + nn] = pre;
+          ep.c16[(size_t)m * ep.ldc16 + nn] = (bf16)gelu_f(pre);
+        } else if constexpr (EPI == EPI_BIAS_OUT) {
+          const float o = v + ep.bias[nn];
+          if (ep.c32) ep.c32[(size_t)m * ep.ldc + nn] = o;
+          if (ep.c16) ep.c16[(size_t)m * ep.ldc16 + nn] = (bf16)o;
+        } else {
+          ep.c16[(size_t)m * ep.ldc16 + nn] = (bf16)(v * gelu_grad(ep.pre[(size_t)m * ep.ldp + nn]));
+        }
+      }
+}
+
+// bf16 [R][C] -> [C][R]
+__global__ void transpose_bf16_kernel(const bf16 *__restrict__ in, int R, int C, bf16 *__restrict__ out) {
+  __shared__ bf16 t[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int r = r0 + k, c = c0 + threadIdx.x;
+    if (r < R && c < C) t[k][threadIdx.x] = in[(size_t)r * C + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int c = c0 + k, r = r0 + threadIdx.x;
+    if (r < R && c < C) out[(size_t)c * R + r] = t[threadIdx.x][k];
+  }
+}
+
+__global__ void gather_rows_kernel(const bf16 *__restrict__ X, const int32_t *__restrict__ idx, int rows,
+                                   bf16 *__restrict__ out) {
+  const int r = blockIdx.x;
+  if (r >= rows) return;
+  const bf16x8 *src = reinterpret_cast<const bf16x8 *>(X + (size_t)idx[r] * D0);
+  bf16x8 *dst = reinterpret_cast<bf16x8 *>(out + (size_t)r * D0);
+  for (int t = threadIdx.x; t < D0 / 8; t += blockDim.x) dst[t] = src[t];
+}
+
+// One wave per pair: ||e_a - e_b|| regressed onto the normalised distance,
+// plus a hinge keeping neighbours inside their radius.
+__global__ __launch_bounds__(256) void pair_loss_kernel(const float *__restrict__ emb,
+                                                        const float *__restrict__ target,
+                                                        const float *__restrict__ radius, int npairs,
+                                                        float nbw, float inv_batch,
+                                                        bf16 *__restrict__ demb, float *__restrict__ loss) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= npairs) return;
+  const float *ea = emb + (size_t)(2 * k) * D3, *eb = emb + (size_t)(2 * k + 1) * D3;
+  const float e0 = ea[lane] - eb[lane], e1 = ea[lane + 64] - eb[lane + 64];
+  float ss = e0 * e0 + e1 * e1;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+  const float dist = sqrtf(ss + 1e-12f);
+  const float diff = dist - target[k];
+  float L = diff * diff, g = 2.0f * diff;
+  const float rad = radius[k];
+  if (rad >= 0.0f && dist > rad) {
+    const float h = dist - rad;
+    L += nbw * h * h;
+    g += 2.0f * nbw * h;
+  }
+  const float s = g * inv_batch / dist;
+  bf16 *da = demb + (size_t)(2 * k) * D3, *db = demb + (size_t)(2 * k + 1) * D3;
+  da[lane] = (bf16)(s * e0);
+  da[lane + 64] = (bf16)(s * e1);
+  db[lane] = (bf16)(-s * e0);
+  db[lane + 64] = (bf16)(-s * e1);
+  if (lane == 0) atomicAdd(loss, L);
+}
+
+// db[c] = sum_r dY[r][c]
+__global__ void col_sum_kernel(const bf16 *__restrict__ dY, int R, int C, float *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int r = 0; r < R; ++r) s += (float)dY[(size_t)r * C + c];
+  out[c] = s;
+}
+
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, int n, float lr, float b1, float b2, float c1,
+                            float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// fp32 master Wt [out][in] -> bf16 Wt [out][in] and bf16 W [in][out]
+__global__ void refresh_weights_kernel(const float *__restrict__ wt, int out_d, int in_d,
+                                       bf16 *__restrict__ wt16, bf16 *__restrict__ w16) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= out_d * in_d) return;
+  const int o = i / in_d, c = i - o * in_d;
+  const bf16 b = (bf16)wt[i];
+  wt16[i] = b;
+  w16[(size_t)c * out_d + o] = b;
+}
+
+__global__ void row_norm_kernel(const bf16 *__restrict__ E, int n, float *__restrict__ norm) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float s = 0.f;
+  for (int k = 0; k < D3; ++k) {
+    const float x = (float)E[(size_t)r * D3 + k];
+    s += x * x;
+  }
+  norm[r] = s;
+}
+
+// -------------------------------------------------------- pairwise proposals
+// Upper-triangle 128 x 128 tiles of E E^T (K = 128), 4 waves of 64 x 64.
+// Pair (i, j), i < j, is proposed when
+//   |e_i|^2 + |e_j|^2 - 2 e_i.e_j <= (eps / (4 max(l_i, l_j)) + margin)^2
+// and 4 |l_i - l_j| <= eps.
+__global__ __launch_bounds__(256) void pair_candidates_kernel(
+    const bf16 *__restrict__ E, const float *__restrict__ norm, const int32_t *__restrict__ len, int n,
+    int tiles, float eps, float margin, unsigned long long *__restrict__ out,
+    unsigned long long *__restrict__ count, unsigned long long cap) {
+  constexpr int BT = 128, KH = 64, LK = KH + 8;
+  __shared__ __attribute__((aligned(16))) bf16 sA[BT * LK];
+  __shared__ __attribute__((aligned(16))) bf16 sB[BT * LK];
+  // block -> (ti, tj), ti <= tj, row-major over the upper triangle: row t
+  // starts at offset(t) = t*tiles - t(t-1)/2
+  long long b = blockIdx.x;
+  int ti;
+  {
+    auto offset = [tiles](long long t) { return t * (long long)tiles - t * (t - 1) / 2; };
+    const double T2 = 2.0 * (double)tiles + 1.0;
+    long long t = (long long)floor((T2 - sqrt(fmax(T2 * T2 - 8.0 * (double)b, 0.0))) * 0.5);
+    if (t < 0) t = 0;
+    if (t > tiles - 1) t = tiles - 1;
+    while (t + 1 < tiles && offset(t + 1) <= b) ++t;
+    while (t > 0 && offset(t) > b) --t;
+    ti = (int)t;
+    b -= offset(t);
+  }
+  const int tj = ti + (int)b;
+  const int i0 = ti * BT, j0 = tj * BT;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int kh = 0; kh < D3; kh += KH) {
+    __syncthreads();
+    // 128 rows x 64 k = 1024 chunks of 8 per tile; 4 per thread
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ch = tid + 256 * t;
+      const int r = ch >> 3, k = (ch & 7) * 8;
+      bf16x8 va, vb;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) { va[q] = (bf16)0.0f; vb[q] = (bf16)0.0f; }
+      if (i0 + r < n) va = *reinterpret_cast<const bf16x8 *>(E + (size_t)(i0 + r) * D3 + kh + k);
+      if (j0 + r < n) vb = *reinterpret_cast<const bf16x8 *>(E + (size_t)(j0 + r) * D3 + kh + k);
+      *reinterpret_cast<bf16x8 *>(sA + r * LK + k) = va;
+      *reinterpret_cast<bf16x8 *>(sB + r * LK + k) = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KH; ks += 32) {
+      bf16x8 af[4], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(sA + (64 * wm + 16 * i + fr) * LK + ks + fk);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(sB + (64 * wn + 16 * j + fr) * LK + ks + fk);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // epilogue: radius test and wave-aggregated emission
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+      const bool vi = gi < n;
+      const float ni = vi ? norm[gi] : 0.f;
+      const int li = vi ? len[gi] : 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+        bool hit = false;
+        if (vi && gj < n && gi < gj) {
+          const int lj = len[gj];
+          const int lm = max(max(li, lj), 1);
+          const float rad = eps / (4.0f * (float)lm) + margin;
+          const float d2 = ni + norm[gj] - 2.0f * acc[i][j][r];
+          hit = d2 <= rad * rad && 4.0f * (float)abs(li - lj) <= eps;
+        }
+        const unsigned long long bal = __ballot(hit);
+        if (bal == 0ull) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(bal));
+        base = __builtin_amdgcn_readfirstlane((unsigned)base) |
+               ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
+        const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+        const unsigned long long pos = base + below;
+        if (hit && pos < cap) out[pos] = ((unsigned long long)gi << 32) | (unsigned)gj;
+      }
+    }
+}
+
+// candidate keys (i << 32 | j) -> per-row counts
+__global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long long m, int32_t *__restrict__ cnt) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) atomicAdd(&cnt[(int)(c[k] >> 32)], 1);
+}
+
+// scatter candidates into their row's target groups of 128
+__global__ void cand_groups_kernel(const unsigned long long *__restrict__ c, long long m,
+                                   const int32_t *__restrict__ gbase, int32_t *__restrict__ cursor,
+                                   const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
+                                   int32_t *__restrict__ gmax, int32_t *__restrict__ gmin) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int i = (int)(c[k] >> 32), j = (int)(c[k] & 0xffffffffull);
+  const int o = atomicAdd(&cursor[i], 1);
+  const int g = gbase[i] + o / kGroupSize;
+  gtgt[(size_t)g * kGroupSize + (o % kGroupSize)] = j;
+  atomicMax(&gmax[g], len[j]);
+  atomicMin(&gmin[g], len[j]);
+}
+
+__global__ void cand_items_kernel(const int32_t *__restrict__ gbase, int n, int4 *__restrict__ items) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int g = gbase[i]; g < gbase[i + 1]; ++g) items[g] = make_int4(g, i, i + 1, 0);
+}
+
+__global__ void iota_kernel(int32_t *a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+__global__ void fill_i32_kernel(int32_t *a, long long n, int32_t v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+unsigned blocks_for(long long n, int tb) { return (unsigned)std::max<long long>((n + tb - 1) / tb, 1); }
+
+struct Layer {
+  int in = 0, out = 0;
+  float *wt = nullptr, *b = nullptr;        // fp32 master, Wt [out][in]
+  float *gw = nullptr, *gb = nullptr;       // gradients
+  float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;  // Adam moments
+  bf16 *wt16 = nullptr, *w16 = nullptr;     // bf16 Wt [out][in] and W [in][out]
+};
+
+}  // namespace
+
+struct mnhn_encoder {
+  Layer L[3];
+  float margin = 0.02f;
+  int64_t adam_t = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+int gemm(hipStream_t st, int epi, const bf16 *A, int lda, const bf16 *B, int ldb, int M, int N, int K,
+         const Epi &ep) {
+  if (K % 8 || lda % 8 || ldb % 8) return MNHN_ERR_ARG;
+  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+  switch (epi) {
+    case EPI_F32: hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(256), 0, st, A, lda, B, ldb, M, N, K, ep); break;
+    case EPI_BIAS_GELU: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BIAS_GELU>, grid, dim3(256), 0, st, A, lda, B, ldb, M, N, K, ep); break;
+    case EPI_BIAS_OUT: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BIAS_OUT>, grid, dim3(256), 0, st, A, lda, B, ldb, M, N, K, ep); break;
+    default: hipLaunchKernelGGL(gemm_nt_kernel<EPI_GELU_BWD>, grid, dim3(256), 0, st, A, lda, B, ldb, M, N, K, ep); break;
+  }
+  return hipGetLastError() == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
+}
+
+int transpose(hipStream_t st, const bf16 *in, int R, int C, bf16 *out) {
+  hipLaunchKernelGGL(transpose_bf16_kernel, dim3((unsigned)((C + 31) / 32), (unsigned)((R + 31) / 32)),
+                     dim3(32, 8), 0, st, in, R, C, out);
+  return hipGetLastError() == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
+}
+
+void free_layer(Layer &l) {
+  dev_free(l.wt); dev_free(l.b); dev_free(l.gw); dev_free(l.gb); dev_free(l.mw); dev_free(l.vw);
+  dev_free(l.mb); dev_free(l.vb); dev_free(l.wt16); dev_free(l.w16);
+}
+
+int refresh(mnhn_encoder *enc) {
+  for (Layer &l : enc->L) {
+    const int n = l.in * l.out;
+    hipLaunchKernelGGL(refresh_weights_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, enc->stream,
+                       l.wt, l.out, l.in, l.wt16, l.w16);
+  }
+  return hipGetLastError() == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
+}
+
+// bf16 profiles of every sequence of ctx
+int profiles_dev(mnhn_sw_ctx *c, bf16 **X, float *Xf_host) {
+  int rc;
+  if ((rc = dev_alloc(X, (size_t)std::max(c->n, 1) * D0))) return rc;
+  float *d_f = nullptr;
+  if (Xf_host && (rc = dev_alloc(&d_f, (size_t)std::max(c->n, 1) * D0))) { dev_free(*X); return rc; }
+  if (c->n > 0)
+    hipLaunchKernelGGL(kmer_profile_kernel, dim3((unsigned)std::min(c->n, 16384)), dim3(256), 0, c->stream,
+                       c->d_words, c->d_woff, c->d_len, c->n, *X, d_f);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && Xf_host && c->n > 0)
+    e = hipMemcpyAsync(Xf_host, d_f, (size_t)c->n * D0 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dev_free(d_f);
+  if (e != hipSuccess) { dev_free(*X); return MNHN_ERR_DEVICE; }
+  return MNHN_OK;
+}
+
+// Forward pass of `rows` profile rows; writes fp32 / bf16 embeddings and, when
+// keep != nullptr, the activations needed by the backward pass.
+struct Acts {
+  float *pre1 = nullptr, *pre2 = nullptr, *emb = nullptr;
+  bf16 *a1 = nullptr, *a2 = nullptr, *emb16 = nullptr;
+};
+
+int forward(mnhn_encoder *enc, hipStream_t st, const bf16 *X, int rows, const Acts &a) {
+  int rc;
+  Epi e1; e1.c32 = a.pre1; e1.ldc = D1; e1.c16 = a.a1; e1.ldc16 = D1; e1.bias = enc->L[0].b;
+  if ((rc = gemm(st, EPI_BIAS_GELU, X, D0, enc->L[0].wt16, D0, rows, D1, D0, e1))) return rc;
+  Epi e2; e2.c32 = a.pre2; e2.ldc = D2; e2.c16 = a.a2; e2.ldc16 = D2; e2.bias = enc->L[1].b;
+  if ((rc = gemm(st, EPI_BIAS_GELU, a.a1, D1, enc->L[1].wt16, D1, rows, D2, D1, e2))) return rc;
+  Epi e3; e3.c32 = a.emb; e3.ldc = D3; e3.c16 = a.emb16; e3.ldc16 = D3; e3.bias = enc->L[2].b;
+  return gemm(st, EPI_BIAS_OUT, a.a2, D2, enc->L[2].wt16, D2, rows, D3, D2, e3);
+}
+
+// bf16 embeddings of every sequence (n x 128)
+int embed_dev(mnhn_encoder *enc, mnhn_sw_ctx *c, const bf16 *X, bf16 *E16, float *E32) {
+  const int chunk = 32768;
+  Acts a;
+  int rc = MNHN_OK;
+  if ((rc = dev_alloc(&a.a1, (size_t)chunk * D1)) || (rc = dev_alloc(&a.a2, (size_t)chunk * D2))) {
+    dev_free(a.a1); dev_free(a.a2);
+    return rc;
+  }
+  for (int r0 = 0; r0 < c->n && rc == MNHN_OK; r0 += chunk) {
+    const int rows = std::min(chunk, c->n - r0);
+    Acts b = a;
+    b.emb16 = E16 + (size_t)r0 * D3;
+    b.emb = E32 ? E32 + (size_t)r0 * D3 : nullptr;
+    rc = forward(enc, c->stream, X + (size_t)r0 * D0, rows, b);
+  }
+  if (rc == MNHN_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = MNHN_ERR_DEVICE;
+  dev_free(a.a1); dev_free(a.a2);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mnhn_kmer_profiles(mnhn_sw_ctx *c, float *out) {
+  if (!c || (!out && c->n > 0)) return MNHN_ERR_ARG;
+  if (!c->acgt) return MNHN_ERR_ARG;
+  bf16 *X = nullptr;
+  int rc = profiles_dev(c, &X, out);
+  dev_free(X);
+  return rc;
+}
+
+int mnhn_encoder_create(uint64_t seed, mnhn_encoder **out) {
+  if (!out) return MNHN_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MNHN_ERR_NODEVICE;
+  mnhn_encoder *enc = new (std::nothrow) mnhn_encoder();
+  if (!enc) return MNHN_ERR_NOMEM;
+  const int dims[4] = {D0, D1, D2, D3};
+  std::mt19937_64 rng(seed);
+  int rc = MNHN_OK;
+  if (hipGetDevice(&enc->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&enc->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&enc->ev[0]) != hipSuccess || hipEventCreate(&enc->ev[1]) != hipSuccess)
+    rc = MNHN_ERR_DEVICE;
+  for (int l = 0; l < 3 && rc == MNHN_OK; ++l) {
+    Layer &L = enc->L[l];
+    L.in = dims[l];
+    L.out = dims[l + 1];
+    const size_t nw = (size_t)L.in * L.out;
+    if ((rc = dev_alloc(&L.wt, nw)) || (rc = dev_alloc(&L.b, L.out)) || (rc = dev_alloc(&L.gw, nw)) ||
+        (rc = dev_alloc(&L.gb, L.out)) || (rc = dev_alloc(&L.mw, nw)) || (rc = dev_alloc(&L.vw, nw)) ||
+        (rc = dev_alloc(&L.mb, L.out)) || (rc = dev_alloc(&L.vb, L.out)) ||
+        (rc = dev_alloc(&L.wt16, nw)) || (rc = dev_alloc(&L.w16, nw)))
+      break;
+    // He-uniform for the GELU layers, Glorot-uniform for the output layer
+    const double bound = l < 2 ? std::sqrt(6.0 / L.in) : std::sqrt(6.0 / (L.in + L.out));
+    std::uniform_real_distribution<double> U(-bound, bound);
+    std::vector<float> w(nw);
+    for (auto &x : w) x = (float)U(rng);
+    std::vector<float> z(std::max(nw, (size_t)L.out), 0.f);
+    if (hipMemcpy(L.wt, w.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L.b, z.data(), (size_t)L.out * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L.mw, z.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L.vw, z.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L.mb, z.data(), (size_t)L.out * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L.vb, z.data(), (size_t)L.out * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = MNHN_ERR_DEVICE;
+  }
+  if (rc == MNHN_OK) rc = refresh(enc);
+  if (rc == MNHN_OK && hipStreamSynchronize(enc->stream) != hipSuccess) rc = MNHN_ERR_DEVICE;
+  if (rc) {
+    mnhn_encoder_destroy(enc);
+    return rc;
+  }
+  *out = enc;
+  return MNHN_OK;
+}
+
+void mnhn_encoder_destroy(mnhn_encoder *enc) {
+  if (!enc) return;
+  if (enc->stream) (void)hipStreamSynchronize(enc->stream);
+  for (Layer &l : enc->L) free_layer(l);
+  for (auto &e : enc->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (enc->stream) (void)hipStreamDestroy(enc->stream);
+  delete enc;
+}
+
+int mnhn_encoder_set_margin(mnhn_encoder *enc, float margin) {
+  if (!enc || !(margin >= 0.f)) return MNHN_ERR_ARG;
+  enc->margin = margin;
+  return MNHN_OK;
+}
+
+float mnhn_encoder_margin(const mnhn_encoder *enc) { return enc ? enc->margin : -1.f; }
+
+int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, const int32_t *dist,
+                       int64_t m, const mnhn_train_params *p, mnhn_train_stats *stats) {
+  if (!enc || !c || !p || m <= 0 || !ij || !dist || p->batch < 32 || p->batch % 32 || p->epochs < 1 ||
+      !(p->lr > 0.f))
+    return MNHN_ERR_ARG;
+  if (!c->acgt) return MNHN_ERR_ARG;
+  for (int64_t k = 0; k < 2 * m; ++k)
+    if (ij[k] < 0 || ij[k] >= c->n) return MNHN_ERR_ARG;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t st = c->stream;
+  int rc;
+  bf16 *X = nullptr;
+  if ((rc = profiles_dev(c, &X, nullptr))) return rc;
+  const int B = p->batch, R = 2 * B;
+  Acts a;
+  bf16 *Xb = nullptr, *XbT = nullptr, *demb = nullptr, *dembT = nullptr, *a1T = nullptr, *a2T = nullptr,
+       *dp1 = nullptr, *dp1T = nullptr, *dp2 = nullptr, *dp2T = nullptr;
+  int32_t *d_idx = nullptr;
+  float *d_tgt = nullptr, *d_rad = nullptr, *d_loss = nullptr;
+  auto cleanup = [&]() {
+    dev_free(X); dev_free(a.pre1); dev_free(a.pre2); dev_free(a.emb); dev_free(a.a1); dev_free(a.a2);
+    dev_free(Xb); dev_free(XbT); dev_free(demb); dev_free(dembT); dev_free(a1T); dev_free(a2T);
+    dev_free(dp1); dev_free(dp1T); dev_free(dp2); dev_free(dp2T); dev_free(d_idx); dev_free(d_tgt);
+    dev_free(d_rad); dev_free(d_loss);
+  };
+  if ((rc = dev_alloc(&a.pre1, (size_t)R * D1)) || (rc = dev_alloc(&a.pre2, (size_t)R * D2)) ||
+      (rc = dev_alloc(&a.emb, (size_t)R * D3)) || (rc = dev_alloc(&a.a1, (size_t)R * D1)) ||
+      (rc = dev_alloc(&a.a2, (size_t)R * D2)) || (rc = dev_alloc(&Xb, (size_t)R * D0)) ||
+      (rc = dev_alloc(&XbT, (size_t)R * D0)) || (rc = dev_alloc(&demb, (size_t)R * D3)) ||
+      (rc = dev_alloc(&dembT, (size_t)R * D3)) || (rc = dev_alloc(&a1T, (size_t)R * D1)) ||
+      (rc = dev_alloc(&a2T, (size_t)R * D2)) || (rc = dev_alloc(&dp1, (size_t)R * D1)) ||
+      (rc = dev_alloc(&dp1T, (size_t)R * D1)) || (rc = dev_alloc(&dp2, (size_t)R * D2)) ||
+      (rc = dev_alloc(&dp2T, (size_t)R * D2)) || (rc = dev_alloc(&d_idx, (size_t)R)) ||
+      (rc = dev_alloc(&d_tgt, (size_t)B)) || (rc = dev_alloc(&d_rad, (size_t)B)) ||
+      (rc = dev_alloc(&d_loss, 1))) {
+    cleanup();
+    return rc;
+  }
+  std::vector<int64_t> order(m);
+  for (int64_t k = 0; k < m; ++k) order[k] = k;
+  std::mt19937_64 rng(p->seed);
+  std::vector<int32_t> h_idx(R);
+  std::vector<float> h_tgt(B), h_rad(B);
+  double first_loss = 0, last_loss = 0;
+  int64_t steps = 0;
+  const int64_t steps_per_epoch = (m + B - 1) / B;
+  for (int ep = 0; ep < p->epochs && rc == MNHN_OK; ++ep) {
+    std::shuffle(order.begin(), order.end(), rng);
+    double ep_loss = 0;
+    int64_t ep_pairs = 0;
+    for (int64_t s = 0; s < steps_per_epoch && rc == MNHN_OK; ++s) {
+      // a batch always holds B pairs (the last one wraps around the epoch)
+      for (int k = 0; k < B; ++k) {
+        const int64_t pk = order[(s * B + k) % m];
+        const int ia = ij[2 * pk], ib = ij[2 * pk + 1];
+        h_idx[2 * k] = ia;
+        h_idx[2 * k + 1] = ib;
+        const double L4 = 4.0 * std::max(std::max(c->len[ia], c->len[ib]), 1);
+        h_tgt[k] = (float)(dist[pk] / L4);
+        h_rad[k] = dist[pk] <= p->eps_train ? (float)(p->eps_train / L4) : -1.f;
+      }
+      hipError_t e = hipMemcpyAsync(d_idx, h_idx.data(), R * 4, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_tgt, h_tgt.data(), B * 4, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_rad, h_rad.data(), B * 4, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = hipMemsetAsync(d_loss, 0, 4, st);
+      if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+      hipLaunchKernelGGL(gather_rows_kernel, dim3(R), dim3(128), 0, st, X, d_idx, R, Xb);
+      Acts b = a;
+      b.emb16 = nullptr;
+      if ((rc = forward(enc, st, Xb, R, b))) break;
+      hipLaunchKernelGGL(pair_loss_kernel, dim3(blocks_for(B, 4)), dim3(256), 0, st, a.emb, d_tgt, d_rad, B,
+                         p->neighbor_weight, 1.0f / (float)B, demb, d_loss);
+      // layer 3: dW3 = dEmb^T a2, db3 = colsum dEmb, dpre2 = (dEmb W3) * gelu'(pre2)
+      if ((rc = transpose(st, demb, R, D3, dembT)) || (rc = transpose(st, a.a2, R, D2, a2T))) break;
+      Epi g3; g3.c32 = enc->L[2].gw; g3.ldc = D2;
+      if ((rc = gemm(st, EPI_F32, dembT, R, a2T, R, D3, D2, R, g3))) break;
+      hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(D3, 128)), dim3(128), 0, st, demb, R, D3, enc->L[2].gb);
+      Epi d2; d2.c16 = dp2; d2.ldc16 = D2; d2.pre = a.pre2; d2.ldp = D2;
+      if ((rc = gemm(st, EPI_GELU_BWD, demb, D3, enc->L[2].w16, D3, R, D2, D3, d2))) break;
+      // layer 2
+      if ((rc = transpose(st, dp2, R, D2, dp2T)) || (rc = transpose(st, a.a1, R, D1, a1T))) break;
+      Epi g2; g2.c32 = enc->L[1].gw; g2.ldc = D1;
+      if ((rc = gemm(st, EPI_F32, dp2T, R, a1T, R, D2, D1, R, g2))) break;
+      hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(D2, 128)), dim3(128), 0, st, dp2, R, D2, enc->L[1].gb);
+      Epi d1; d1.c16 = dp1; d1.ldc16 = D1; d1.pre = a.pre1; d1.ldp = D1;
+      if ((rc = gemm(st, EPI_GELU_BWD, dp2, D2, enc->L[1].w16, D2, R, D1, D2, d1))) break;
+      // layer 1
+      if ((rc = transpose(st, dp1, R, D1, dp1T)) || (rc = transpose(st, Xb, R, D0, XbT))) break;
+      Epi g1; g1.c32 = enc->L[0].gw; g1.ldc = D0;
+      if ((rc = gemm(st, EPI_F32, dp1T, R, XbT, R, D1, D0, R, g1))) break;
+      hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(D1, 128)), dim3(128), 0, st, dp1, R, D1, enc->L[0].gb);
+      // Adam
+      enc->adam_t++;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - (float)std::pow(b1, (double)enc->adam_t), c2 = 1.f - (float)std::pow(b2, (double)enc->adam_t);
+      for (Layer &L : enc->L) {
+        const int nw = L.in * L.out;
+        hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(nw, 256)), dim3(256), 0, st, L.wt, L.gw, L.mw, L.vw, nw,
+                           p->lr, b1, b2, c1, c2);
+        hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(L.out, 256)), dim3(256), 0, st, L.b, L.gb, L.mb, L.vb,
+                           L.out, p->lr, b1, b2, c1, c2);
+        hipLaunchKernelGGL(refresh_weights_kernel, dim3(blocks_for(nw, 256)), dim3(256), 0, st, L.wt, L.out,
+                           L.in, L.wt16, L.w16);
+      }
+      float h_loss = 0;
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(&h_loss, d_loss, 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+      ep_loss += h_loss;
+      ep_pairs += B;
+      ++steps;
+    }
+    const double mean = ep_pairs ? ep_loss / ep_pairs : 0;
+    if (ep == 0) first_loss = mean;
+    last_loss = mean;
+  }
+  cleanup();
+  if (stats) {
+    stats->first_loss = first_loss;
+    stats->final_loss = last_loss;
+    stats->steps = steps;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return rc;
+}
+
+int mnhn_encoder_embed(mnhn_encoder *enc, mnhn_sw_ctx *c, float *out) {
+  if (!enc || !c || (!out && c->n > 0)) return MNHN_ERR_ARG;
+  if (!c->acgt) return MNHN_ERR_ARG;
+  if (c->n == 0) return MNHN_OK;
+  bf16 *X = nullptr, *E16 = nullptr;
+  float *E32 = nullptr;
+  int rc = profiles_dev(c, &X, nullptr);
+  if (rc) return rc;
+  if ((rc = dev_alloc(&E16, (size_t)c->n * D3)) || (rc = dev_alloc(&E32, (size_t)c->n * D3))) {
+    dev_free(X); dev_free(E16); dev_free(E32);
+    return rc;
+  }
+  rc = embed_dev(enc, c, X, E16, E32);
+  if (rc == MNHN_OK && hipMemcpy(out, E32, (size_t)c->n * D3 * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = MNHN_ERR_DEVICE;
+  dev_free(X); dev_free(E16); dev_free(E32);
+  return rc;
+}
+
+static const char kEncMagic[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '1'};
+
+int mnhn_encoder_save(const mnhn_encoder *enc, const char *path) {
+  if (!enc || !path) return MNHN_ERR_ARG;
+  FILE *f = fopen(path, "wb");
+  if (!f) return MNHN_ERR_ARG;
+  int rc = MNHN_OK;
+  fwrite(kEncMagic, 1, 8, f);
+  fwrite(&enc->margin, 4, 1, f);
+  for (const Layer &L : enc->L) {
+    std::vector<float> w((size_t)L.in * L.out), b(L.out);
+    if (hipMemcpy(w.data(), L.wt, w.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b.data(), L.b, b.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = MNHN_ERR_DEVICE;
+      break;
+    }
+    const int32_t dims[2] = {L.in, L.out};
+    fwrite(dims, 4, 2, f);
+    fwrite(w.data(), 4, w.size(), f);
+    fwrite(b.data(), 4, b.size(), f);
+  }
+  if (fclose(f) != 0 && rc == MNHN_OK) rc = MNHN_ERR_ARG;
+  return rc;
+}
+
+int mnhn_encoder_load(mnhn_encoder *enc, const char *path) {
+  if (!enc || !path) return MNHN_ERR_ARG;
+  FILE *f = fopen(path, "rb");
+  if (!f) return MNHN_ERR_ARG;
+  char magic[8];
+  float margin = 0;
+  int rc = MNHN_OK;
+  if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kEncMagic, 8) != 0 || fread(&margin, 4, 1, f) != 1) rc = MNHN_ERR_ARG;
+  for (Layer &L : enc->L) {
+    if (rc) break;
+    int32_t dims[2];
+    if (fread(dims, 4, 2, f) != 2 || dims[0] != L.in || dims[1] != L.out) { rc = MNHN_ERR_ARG; break; }
+    std::vector<float> w((size_t)L.in * L.out), b(L.out);
+    if (fread(w.data(), 4, w.size(), f) != w.size() || fread(b.data(), 4, b.size(), f) != b.size()) { rc = MNHN_ERR_ARG; break; }
+    if (hipMemcpy(L.wt, w.data(), w.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L.b, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+  }
+  fclose(f);
+  if (rc == MNHN_OK) {
+    enc->margin = margin;
+    rc = refresh(enc);
+    if (rc == MNHN_OK && hipStreamSynchronize(enc->stream) != hipSuccess) rc = MNHN_ERR_DEVICE;
+  }
+  return rc;
+}
+
+int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, mnhn_csr *out,
+                           mnhn_prefilter_stats *stats) {
+  if (!enc || !c || !out) return MNHN_ERR_ARG;
+  if (!c->acgt) return MNHN_ERR_ARG;
+  memset(out, 0, sizeof(*out));
+  mnhn_prefilter_stats stt{};
+  const int n = c->n;
+  out->n_rows = n;
+  out->eps_max = eps_max;
+  out->row_ptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  if (!out->row_ptr) return MNHN_ERR_NOMEM;
+  if (n == 0) return MNHN_OK;
+  hipStream_t st = c->stream;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  auto phase_ms = [&]() {
+    float ms = 0.f;
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventRecord(e0, st);
+    return (double)ms;
+  };
+  bf16 *X = nullptr, *E16 = nullptr;
+  float *norm = nullptr;
+  unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
+  int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
+          *gmin = nullptr, *qids = nullptr;
+  int4 *items = nullptr;
+  auto cleanup = [&]() {
+    dev_free(X); dev_free(E16); dev_free(norm); dev_free(cand); dev_free(d_cnt64); dev_free(rowcnt);
+    dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax); dev_free(gmin); dev_free(qids);
+    dev_free(items);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  };
+  int rc = MNHN_OK;
+  (void)hipEventRecord(e0, st);
+  if ((rc = profiles_dev(c, &X, nullptr))) { cleanup(); mnhn_csr_free(out); return rc; }
+  stt.profile_ms = phase_ms();
+  if ((rc = dev_alloc(&E16, (size_t)n * D3)) || (rc = dev_alloc(&norm, (size_t)n)) ||
+      (rc = embed_dev(enc, c, X, E16, nullptr))) {
+    cleanup(); mnhn_csr_free(out); return rc;
+  }
+  dev_free(X);
+  hipLaunchKernelGGL(row_norm_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, E16, n, norm);
+  stt.embed_ms = phase_ms();
+  // ---- proposals
+  const int tiles = (n + 127) / 128;
+  const long long nblk = (long long)tiles * (tiles + 1) / 2;
+  if (nblk > 0x7fffffffLL) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_RANGE; }
+  unsigned long long cap = std::max<unsigned long long>(1ull << 20, 64ull * (unsigned long long)n), m = 0;
+  if ((rc = dev_alloc(&d_cnt64, 1))) { cleanup(); mnhn_csr_free(out); return rc; }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    dev_free(cand);
+    if ((rc = dev_alloc(&cand, cap))) break;
+    hipError_t e = hipMemsetAsync(d_cnt64, 0, 8, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(pair_candidates_kernel, dim3((unsigned)nblk), dim3(256), 0, st, E16, norm, c->d_len, n,
+                         tiles, (float)eps_max, enc->margin, cand, d_cnt64, cap);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&m, d_cnt64, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+    if (m <= cap) break;
+    cap = m;  // overflowed: exact size, recompute once
+  }
+  if (rc || m > cap) { cleanup(); mnhn_csr_free(out); return rc ? rc : MNHN_ERR_NOMEM; }
+  stt.candidates = (int64_t)m;
+  stt.propose_ms = phase_ms();
+  // ---- exact verification: candidates grouped per row into targets of 128
+  unsigned long long nk = 0;
+  if (m > 0) {
+    std::vector<int32_t> cnt(n), gb(n + 1, 0);
+    if ((rc = dev_alloc(&rowcnt, (size_t)n)) || (rc = dev_alloc(&gbase, (size_t)n + 1)) ||
+        (rc = dev_alloc(&cursor, (size_t)n)) || (rc = dev_alloc(&qids, (size_t)n))) {
+      cleanup(); mnhn_csr_free(out); return rc;
+    }
+    hipError_t e = hipMemsetAsync(rowcnt, 0, (size_t)n * 4, st);
+    hipLaunchKernelGGL(cand_count_kernel, dim3(blocks_for((long long)m, 256)), dim3(256), 0, st, cand, (long long)m, rowcnt);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+    for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + (cnt[i] + kGroupSize - 1) / kGroupSize;
+    const int G = gb[n];
+    if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
+        (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G))) {
+      cleanup(); mnhn_csr_free(out); return rc;
+    }
+    e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, (size_t)n * 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(gmax, 0, (size_t)G * 4, st);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
+                       (long long)G * kGroupSize, -1);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(G, 256)), dim3(256), 0, st, gmin, (long long)G, 0x7fffffff);
+    hipLaunchKernelGGL(cand_groups_kernel, dim3(blocks_for((long long)m, 256)), dim3(256), 0, st, cand,
+                       (long long)m, gbase, cursor, c->d_len, gtgt, gmax, gmin);
+    hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, n, items);
+    hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+    dev_free(cand);
+    Work w;
+    w.dev = true;
+    w.d_items = items;
+    w.n_items = G;
+    w.d_qids = qids;
+    w.d_gtgt = gtgt;
+    w.d_gmax = gmax;
+    w.d_gmin = gmin;
+    w.dev_kernel = KERNEL_LDS;
+    w.max_q_len = c->maxlen;
+    w.pairs = (double)m;
+    const unsigned long long kcap = 2ull * m;
+    if ((rc = ensure_keys(c, kcap))) { cleanup(); mnhn_csr_free(out); return rc; }
+    if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+    rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, kcap);
+    if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
+    if (rc) { cleanup(); mnhn_csr_free(out); return rc; }
+  }
+  stt.verify_ms = phase_ms();
+  stt.neighbours = (int64_t)(nk / 2);
+  rc = csr_from_keys(c, nk, out);
+  stt.csr_ms = phase_ms();
+  cleanup();
+  if (stats) *stats = stt;
+  return rc;
+}
+
+}  // extern "C"

@@ -32,12 +32,13 @@ def _declared_functions():
 
 
 def test_headers_declare_the_exported_list(pkg):
-    assert _declared_functions() == set(pkg.ABI_SYMBOLS) | set(pkg.HOST_SYMBOLS)
+    assert _declared_functions() == (set(pkg.ABI_SYMBOLS) | set(pkg.HOST_SYMBOLS)
+                                     | set(pkg.PROPOSER_SYMBOLS))
 
 
 def test_library_loads_and_exports_every_symbol(pkg):
     lib = pkg.load_library()
-    for name in pkg.ABI_SYMBOLS + pkg.HOST_SYMBOLS:
+    for name in pkg.ABI_SYMBOLS + pkg.HOST_SYMBOLS + pkg.PROPOSER_SYMBOLS:
         assert hasattr(lib, name), name
     assert lib.mnhn_sw_strerror(0) == b"ok"
     assert lib.mnhn_sw_strerror(5).startswith(b"score range")

@@ -1,0 +1,142 @@
+"""GPU tests of the learned candidate proposer: profiles and embeddings against
+numpy references of the same arithmetic, training progress, and the
+prefilter's epsilon lists against the exact all-pairs lists."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, os.path.join(os.path.dirname(GOLDEN), "..", "mnhn-tree-tools_amd"))
+import synth  # noqa: E402
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _kmer_ref(seqs):
+    lut = np.full(256, 0, np.int64)
+    for i, ch in enumerate(b"ACGT"):
+        lut[ch] = i
+    out = np.zeros((len(seqs), 1024), np.float64)
+    for r, s in enumerate(seqs):
+        c = lut[np.frombuffer(s[1:], np.uint8)]
+        if len(c) < 5:
+            continue
+        code = np.zeros(len(c) - 4, np.int64)
+        for i in range(5):
+            code = code * 4 + c[i:len(c) - 4 + i]
+        out[r] = np.bincount(code, minlength=1024) / len(code)
+    return out
+
+
+def _bf16(x):
+    """Round float32 values to bf16 (nearest even), returned as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+    r = ((a + 0x7FFF + ((a >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def _gelu(x):
+    from scipy.special import erf
+    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+
+
+def test_kmer_profiles_match_numpy(pkg):
+    seqs = synth.config2(seed=41, n_roots=4, n_children=3, n_leaves=5) + [b"A", b"ACGTAC", b""]
+    _say(f"kmer profiles n={len(seqs)}")
+    with pkg.Engine(seqs) as eng:
+        got = eng.kmer_profiles()
+    want = _kmer_ref(seqs)
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-7)
+
+
+def test_embedding_matches_numpy_forward(pkg, tmp_path):
+    seqs = synth.config1(seed=42, n_anc=4, per_anc=30)
+    _say(f"embedding n={len(seqs)}")
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=7) as enc:
+        got = enc.embed(eng)
+        enc.save(str(tmp_path / "enc.bin"))
+    _, layers = pkg.load_encoder_weights(str(tmp_path / "enc.bin"))
+    x = _bf16(_kmer_ref(seqs).astype(np.float32) * 64.0)
+    (w1, b1), (w2, b2), (w3, b3) = layers
+    a1 = _bf16(_gelu(x.astype(np.float64) @ _bf16(w1).T.astype(np.float64) + b1))
+    a2 = _bf16(_gelu(a1.astype(np.float64) @ _bf16(w2).T.astype(np.float64) + b2))
+    e = a2.astype(np.float64) @ _bf16(w3).T.astype(np.float64) + b3
+    scale = np.abs(e).max()
+    assert scale > 0
+    np.testing.assert_allclose(got, e, rtol=0, atol=2e-2 * scale)
+    # layout errors would decorrelate the embeddings entirely
+    assert np.corrcoef(got.ravel(), e.ravel())[0, 1] > 0.999
+
+
+def test_training_reduces_loss(pkg):
+    seqs = synth.config1(seed=43, n_anc=10, per_anc=40, length=160)
+    rng = np.random.default_rng(0)
+    n = len(seqs)
+    ij = rng.integers(0, n, size=(20000, 2)).astype(np.int32)
+    same = rng.integers(0, 10, size=10000) * 40
+    ij[:10000, 0] = same + rng.integers(0, 40, 10000)
+    ij[:10000, 1] = same + rng.integers(0, 40, 10000)
+    _say(f"training on {len(ij)} pairs")
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=1) as enc:
+        d = eng.pairs(ij)
+        st = enc.train(eng, ij, d, epochs=8, batch=1024, lr=2e-3, eps_train=300)
+        emb = enc.embed(eng)
+    _say(f"loss {st['first_loss']:.4f} -> {st['final_loss']:.4f} in {st['seconds']:.2f} s")
+    assert st["steps"] == 8 * 20
+    assert st["final_loss"] < 0.5 * st["first_loss"]
+    # within-family embedding distances are smaller than across families
+    fam = np.arange(n) // 40
+    a, b = ij[:, 0], ij[:, 1]
+    de = np.linalg.norm(emb[a] - emb[b], axis=1)
+    assert de[fam[a] == fam[b]].mean() < 0.5 * de[fam[a] != fam[b]].mean()
+
+
+def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg):
+    """A huge margin proposes every pair that passes the exact length bound,
+    so the verified lists must equal the exact all-pairs lists."""
+    seqs = synth.config2(seed=44, n_roots=6, n_children=4, n_leaves=10)
+    eps = 350
+    _say(f"prefilter (unbounded radius) n={len(seqs)}")
+    L = np.array([len(s) for s in seqs])
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=3) as enc:
+        enc.margin = 1e4
+        rp, col, dist, st = enc.prefilter_csr(eng, eps)
+        erp, ecol, edist = eng.eps_csr(eps)
+    n = len(seqs)
+    iu = np.triu_indices(n, 1)
+    want_cands = int((4 * np.abs(L[iu[0]] - L[iu[1]]) <= eps).sum())
+    assert st["candidates"] == want_cands
+    assert np.array_equal(rp, erp) and np.array_equal(col, ecol) and np.array_equal(dist, edist)
+    assert st["neighbours"] * 2 == len(col)
+
+
+def test_prefilter_trained_recall(pkg, oracle):
+    seqs = synth.config3(seed=45, n_anc=20, per_anc=40, noise_frac=0.05)
+    n = len(seqs)
+    eps = 300
+    rng = np.random.default_rng(2)
+    ij = rng.integers(0, n, size=(30000, 2)).astype(np.int32)
+    anc = np.minimum(np.arange(n) // 40, 20)
+    within = rng.integers(0, 20, 15000) * 40
+    ij[:15000, 0] = within + rng.integers(0, 40, 15000)
+    ij[:15000, 1] = within + rng.integers(0, 40, 15000)
+    _say(f"prefilter recall n={n}")
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=5) as enc:
+        d = eng.pairs(ij)
+        enc.train(eng, ij, d, epochs=10, batch=1024, lr=2e-3, eps_train=eps)
+        enc.margin = 0.05
+        rp, col, dist, st = enc.prefilter_csr(eng, eps)
+        erp, ecol, edist = eng.eps_csr(eps)
+    got = set(zip(np.repeat(np.arange(n), np.diff(rp)).tolist(), col.tolist()))
+    want = set(zip(np.repeat(np.arange(n), np.diff(erp)).tolist(), ecol.tolist()))
+    assert got <= want  # only exact neighbours are ever reported
+    recall = len(got) / max(len(want), 1)
+    _say(f"recall {recall:.5f}, candidates {st['candidates']}, neighbours {st['neighbours']}")
+    assert recall > 0.99

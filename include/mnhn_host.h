@@ -96,6 +96,12 @@ void mnhn_gpu_set_lookahead(mnhn_gpu_stuff g, float lookahead);
 /* ---- clustering ---- */
 split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g);
 
+/* One DBSCAN pass (same semantics as dbscan_SW_GPU, without the -1 default
+ * warnings) over precomputed epsilon lists, e.g. from mnhn_sw_eps_csr or the
+ * learned prefilter; requires eps <= csr->eps_max.  No counterpart in the
+ * reference, whose passes always recompute distances. */
+split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts);
+
 void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
                      dataset ds, float epsilon_start, float epsilon_inc, int minpts,
                      char *split_files_prefix, int n_threads);

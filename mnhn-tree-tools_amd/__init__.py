@@ -45,7 +45,7 @@ HOST_SYMBOLS = [
     "data_not_in_clusters", "intersection_of_clusters",
     "generate_split_set_relation", "free_split_set_and_associated_clusters",
     "mnhn_gpu_initialization", "mnhn_gpu_destroy", "mnhn_gpu_set_lookahead",
-    "dbscan_SW_GPU", "adaptive_dbscan",
+    "dbscan_SW_GPU", "mnhn_dbscan_csr", "adaptive_dbscan",
 ]
 
 
@@ -349,3 +349,40 @@ def load_encoder_weights(path: str):
             b = np.frombuffer(f.read(4 * dout), "<f4")
             layers.append((w, b))
     return margin, layers
+
+
+class _Cluster(ctypes.Structure):
+    _fields_ = [("members", ctypes.POINTER(ctypes.c_int)), ("n_members", ctypes.c_int),
+                ("id", ctypes.c_int)]
+
+
+class _SplitSet(ctypes.Structure):
+    _fields_ = [("n_clusters", ctypes.c_int), ("clusters", ctypes.POINTER(_Cluster))]
+
+
+def dbscan_csr(rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int, eps: float,
+               minpts: int):
+    """One DBSCAN pass (the library's C host code) over epsilon lists; returns
+    the clusters as lists of member indices in the reference's order."""
+    lib = load_library()
+    lib.mnhn_dbscan_csr.restype = _SplitSet
+    lib.mnhn_dbscan_csr.argtypes = [ctypes.POINTER(Csr), ctypes.c_float, ctypes.c_int]
+    lib.free_split_set_and_associated_clusters.argtypes = [_SplitSet]
+    lib.free_split_set_and_associated_clusters.restype = None
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    dist = np.ascontiguousarray(dist, dtype=np.int16)
+    csr = Csr(len(rp) - 1, int(eps_max), len(col),
+              rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+              col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+    s = lib.mnhn_dbscan_csr(ctypes.byref(csr), float(eps), int(minpts))
+    try:
+        out = []
+        for k in range(s.n_clusters):
+            c = s.clusters[k]
+            out.append(np.ctypeslib.as_array(c.members, shape=(c.n_members,)).tolist()
+                       if c.n_members else [])
+    finally:
+        lib.free_split_set_and_associated_clusters(s)
+    return out

@@ -98,6 +98,24 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
   c->have = 1;
 }
 
+/* N(p) from epsilon lists (ascending, p included); returns |N(p)|. */
+static int region_query_csr(const mnhn_csr *csr, int p, float eps, int *out) {
+  int k = 0;
+  const int64_t b = csr->row_ptr[p], e = csr->row_ptr[p + 1];
+  const int self = (float)0 <= eps;
+  int placed = !self;
+  for (int64_t t = b; t < e; t++) {
+    const int j = csr->col[t];
+    if (!placed && j > p) {
+      out[k++] = p;
+      placed = 1;
+    }
+    if ((float)csr->dist[t] <= eps) out[k++] = j;
+  }
+  if (!placed) out[k++] = p;
+  return k;
+}
+
 /* N(p) for threshold eps into out[] (ascending, p included); returns |N(p)|. */
 static int region_query(mnhn_gpu_stuff g, int n, int p, float eps, int *out) {
   mnhn_eps_cache *c = g.cache;
@@ -109,43 +127,25 @@ static int region_query(mnhn_gpu_stuff g, int n, int p, float eps, int *out) {
       if ((float)c->dist_row[j] <= eps) out[k++] = j;
     return k;
   }
-  const int64_t b = c->csr.row_ptr[p], e = c->csr.row_ptr[p + 1];
-  const int self = (float)0 <= eps;
-  int placed = !self;
-  for (int64_t t = b; t < e; t++) {
-    const int j = c->csr.col[t];
-    if (!placed && j > p) {
-      out[k++] = p;
-      placed = 1;
-    }
-    if ((float)c->csr.dist[t] <= eps) out[k++] = j;
-  }
-  if (!placed) out[k++] = p;
-  return k;
+  return region_query_csr(&c->csr, p, eps, out);
 }
 
-split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
-  const int n = ds.n_values;
-  if (epsilon == -1) {
-    printf("Warning @dbscan: epsilon value is: %f\n", epsilon);
-    epsilon = 0.015f;
-    printf("Warning @dbscan: epsilon was changed to: %f\n", epsilon);
-  }
-  if (minpts == -1) {
-    printf("Warning @dbscan: minpts value is: %i\n", minpts);
-    minpts = 5;
-    printf("Warning @dbscan: minpts was changed to: %i\n", minpts);
-  }
+typedef int (*region_fn)(void *src, int n, int p, float eps, int *out);
+
+static int rq_gpu(void *src, int n, int p, float eps, int *out) {
+  return region_query(*(mnhn_gpu_stuff *)src, n, p, eps, out);
+}
+
+static int rq_csr(void *src, int n, int p, float eps, int *out) {
+  (void)n;
+  return region_query_csr((const mnhn_csr *)src, p, eps, out);
+}
+
+static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, void *src) {
   split_set ret;
   ret.n_clusters = 0;
   ret.clusters = (cluster *)malloc(sizeof(cluster) * (size_t)(n ? n : 1));
   if (n == 0) return ret;
-
-  if (g.cache->per_query_launch) {
-    if (!g.cache->dist_row) g.cache->dist_row = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-  } else {
-    ensure_lists(g, epsilon);
-  }
 
   unsigned char *visited = (unsigned char *)calloc((size_t)n, 1);
   unsigned char *member = (unsigned char *)calloc((size_t)n, 1);
@@ -156,7 +156,7 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
   for (int i = 0; i < n; i++) {
     if (visited[i]) continue;
     visited[i] = 1;
-    int cnt = region_query(g, n, i, epsilon, nb);
+    int cnt = rq(src, n, i, epsilon, nb);
     if (!(minpts < cnt)) continue;
     /* expand_cluster */
     const int id = ret.n_clusters;
@@ -176,7 +176,7 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
       const int q = queue[qi];
       if (!visited[q]) {
         visited[q] = 1;
-        int cq = region_query(g, n, q, epsilon, nb);
+        int cq = rq(src, n, q, epsilon, nb);
         if (cq >= minpts) {
           for (int t = 0; t < cq; t++) {
             const int x = nb[t];
@@ -201,6 +201,32 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
   free(nb);
   free(queue);
   return ret;
+}
+
+split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
+  const int n = ds.n_values;
+  if (epsilon == -1) {
+    printf("Warning @dbscan: epsilon value is: %f\n", epsilon);
+    epsilon = 0.015f;
+    printf("Warning @dbscan: epsilon was changed to: %f\n", epsilon);
+  }
+  if (minpts == -1) {
+    printf("Warning @dbscan: minpts value is: %i\n", minpts);
+    minpts = 5;
+    printf("Warning @dbscan: minpts was changed to: %i\n", minpts);
+  }
+  if (n > 0) {
+    if (g.cache->per_query_launch) {
+      if (!g.cache->dist_row) g.cache->dist_row = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    } else {
+      ensure_lists(g, epsilon);
+    }
+  }
+  return dbscan_core(n, epsilon, minpts, rq_gpu, &g);
+}
+
+split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts) {
+  return dbscan_core(csr ? csr->n_rows : 0, epsilon, minpts, rq_csr, (void *)csr);
 }
 
 void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),

@@ -112,6 +112,70 @@ def config3(seed: int = 3, n_anc: int = 500, per_anc: int = 200,
     return out
 
 
+def pareto_lengths(rng: np.random.Generator, n: int, alpha: float = 1.5, xm: float = 80.0,
+                   hi: int = 1500) -> np.ndarray:
+    """Pareto(alpha, x_m) lengths clipped to [x_m, hi]."""
+    return np.minimum(np.rint(xm * (1.0 + rng.pareto(alpha, size=n))), hi).astype(np.int64)
+
+
+def config5(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 8,
+            per_child: int = 5, tree_frac: float = 0.70, repeat_frac: float = 0.20,
+            n_monomer_families: int = 200, monomer: int = 171) -> List[bytes]:
+    """End-to-end learned-prefilter workload (heavy-tailed lengths 80-1500):
+      70% from a 4-level tree with 2,000 leaf clusters: 50 roots (Pareto
+         lengths, alpha 1.5, x_m 80, clipped to 1,500) -> 8 children at 8% ->
+         5 grandchildren at 4% (= 2,000 leaf-cluster ancestors) -> members at 2%;
+      20% tandem repeats: 2-8 copies of one of 200 random 171-bp monomers, every
+         copy at 3% divergence;
+      10% uniform random sequences with Pareto lengths.
+    (mutation: sub:indel 4:1)."""
+    rng = _rng(seed)
+    n_tree = int(round(tree_frac * n))
+    n_rep = int(round(repeat_frac * n))
+    n_noise = n - n_tree - n_rep
+    leaves = []
+    for L in pareto_lengths(rng, n_roots):
+        root = random_codes(rng, L)
+        for _ in range(per_root):
+            child = mutate(rng, root, 0.8 * 0.08, 0.2 * 0.08)
+            for _ in range(per_child):
+                leaves.append(mutate(rng, child, 0.8 * 0.04, 0.2 * 0.04))
+    out = []
+    for k in range(n_tree):
+        anc = leaves[k % len(leaves)]
+        out.append(to_bytes(mutate(rng, anc, 0.8 * 0.02, 0.2 * 0.02)))
+    monomers = [random_codes(rng, monomer) for _ in range(n_monomer_families)]
+    for k in range(n_rep):
+        mono = monomers[k % n_monomer_families]
+        copies = int(rng.integers(2, 9))
+        out.append(to_bytes(np.concatenate(
+            [mutate(rng, mono, 0.8 * 0.03, 0.2 * 0.03) for _ in range(copies)])))
+    for L in pareto_lengths(rng, n_noise):
+        out.append(to_bytes(random_codes(rng, L)))
+    return out
+
+
+def labelled_pairs(rng: np.random.Generator, n: int, group: np.ndarray, m: int,
+                   within_frac: float = 0.5) -> np.ndarray:
+    """m index pairs: within_frac of them inside the same group label, the
+    rest uniform random (no self pairs)."""
+    m_in = int(round(within_frac * m))
+    order = np.argsort(group, kind="stable")
+    g_sorted = group[order]
+    starts = np.searchsorted(g_sorted, g_sorted, side="left")
+    ends = np.searchsorted(g_sorted, g_sorted, side="right")
+    a = rng.integers(0, n, size=m_in)
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n)
+    pa = inv[a]
+    lo, hi = starts[pa], ends[pa]
+    b = order[lo + (rng.random(m_in) * (hi - lo)).astype(np.int64)]
+    rnd = rng.integers(0, n, size=(m - m_in, 2))
+    ij = np.concatenate([np.stack([a, b], 1), rnd]).astype(np.int32)
+    ij = ij[ij[:, 0] != ij[:, 1]]
+    return ij
+
+
 def write_fasta(path: str, seqs: Sequence[bytes], width: int = 50) -> None:
     """'>sequence_<i>' headers with lines wrapped at `width` columns."""
     with open(path, "wb") as f:

@@ -186,3 +186,22 @@ def test_relation_and_sort_side_effect(host, oracle):
         assert [so.clusters[k].members[i] for i in range(so.clusters[k].n_members)] == O[k].members
     for k in range(2):
         assert [sn.clusters[k].members[i] for i in range(sn.clusters[k].n_members)] == N[k].members
+
+
+def test_dbscan_over_csr_matches_oracle(pkg, oracle):
+    """mnhn_dbscan_csr (host code only) over epsilon lists built on the CPU
+    from the oracle's distance matrix gives the oracle's clusters, member
+    order included, for every epsilon up to the lists' bound."""
+    path = os.path.join(GOLDEN, "inputs", "c2_small.fasta")
+    seqs = oracle.read_fasta(path)[:160]
+    dist = oracle.sw_matrix(seqs)
+    n = len(seqs)
+    eps_max = 500
+    keep = (dist <= eps_max) & ~np.eye(n, dtype=bool)
+    rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+    col = np.nonzero(keep)[1].astype(np.int32)
+    d = dist[keep].astype(np.int16)
+    for eps, minpts in ((100, 3), (250.5, 5), (400, 2), (500, 10), (0, 0)):
+        got = pkg.dbscan_csr(rp, col, d, eps_max, eps, minpts)
+        want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(dist))
+        assert got == [c.members for c in want.clusters], (eps, minpts)

@@ -136,6 +136,197 @@ def run_dbscan(seqs, eps_start, eps_inc, minpts):
             "stdout_tail": r.stdout.decode().splitlines()[-1:]}
 
 
+# ------------------------------------------------------ learned proposer runs
+
+def _bf16_round(x):
+    """float32 -> nearest-even bf16, returned as float32 (the embeddings the
+    pairwise kernel reads)."""
+    a = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+    return ((a + 0x7FFF + ((a >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)
+
+
+def radius_excess(emb16, L, i, j, eps, match=4, chunk=1 << 20):
+    """||e_i - e_j|| - eps / (match max(l_i, l_j)) per pair: a neighbour pair
+    is proposed iff its excess is <= the encoder's margin."""
+    out = np.empty(len(i), np.float32)
+    for s in range(0, len(i), chunk):
+        a, b = i[s:s + chunk], j[s:s + chunk]
+        de = np.sqrt(((emb16[a] - emb16[b]) ** 2).sum(1))
+        out[s:s + chunk] = de - eps / (match * np.maximum(np.maximum(L[a], L[b]), 1))
+    return out
+
+
+def sample_training_pairs(rng, group_levels, n, m):
+    """m labelled-pair indices: for each (groups, fraction) level that fraction
+    drawn inside the same group, the rest uniform random."""
+    parts, used = [], 0
+    for groups, frac in group_levels:
+        k = int(round(frac * m))
+        parts.append(synth.labelled_pairs(rng, n, groups, k, within_frac=1.0))
+        used += k
+    parts.append(synth.labelled_pairs(rng, n, np.arange(n), m - used, within_frac=0.0))
+    ij = np.concatenate(parts)
+    return ij[rng.permutation(len(ij))]
+
+
+def train_encoder(pkg, seqs, group_levels, n_pairs, eps, seed, epochs, log):
+    """Label n_pairs pairs of seqs with the exact kernel, train an encoder on
+    90% of them and set its margin from the held-out 10%: the largest radius
+    excess of a held-out neighbour pair (d <= eps) plus a small slack."""
+    rng = np.random.default_rng(seed)
+    n = len(seqs)
+    L = np.array([len(x) for x in seqs], dtype=np.int64)
+    ij = sample_training_pairs(rng, group_levels, n, n_pairs)
+    eng = pkg.Engine(seqs)
+    t0 = time.perf_counter()
+    d = eng.pairs(ij)
+    label_s = time.perf_counter() - t0
+    cut = int(0.9 * len(ij))
+    log(f"labelled {len(ij)} pairs in {label_s:.2f} s; neighbours (d <= {eps}): "
+        f"{int((d <= eps).sum())}")
+    enc = pkg.Encoder(seed=seed)
+    st = enc.train(eng, ij[:cut], d[:cut], epochs=epochs, batch=1024, lr=2e-3, eps_train=eps,
+                   neighbor_weight=4.0, seed=seed)
+    log(f"trained: loss {st['first_loss']:.4f} -> {st['final_loss']:.4f}, {st['steps']} steps, "
+        f"{st['seconds']:.1f} s")
+    emb16 = _bf16_round(enc.embed(eng))
+    hv, dv = ij[cut:], d[cut:]
+    nb = dv <= eps
+    ex = radius_excess(emb16, L, hv[nb, 0], hv[nb, 1], eps)
+    q = {f"q{k}": float(np.quantile(ex, k / 100.0)) for k in (50, 99, 99.9)}
+    q["max"] = float(ex.max()) if len(ex) else 0.0
+    margin = max(q["max"], 0.0) + 0.01
+    enc.margin = margin
+    log(f"margin {margin:.4f} from {int(nb.sum())} held-out neighbour pairs (excess {q})")
+    eng.close()
+    info = {"labelled_pairs": int(len(ij)), "train_pairs": cut, "label_s": label_s,
+            "train_s": st["seconds"], "epochs": epochs, "steps": st["steps"],
+            "loss_first": st["first_loss"], "loss_final": st["final_loss"],
+            "margin": margin, "heldout_neighbour_pairs": int(nb.sum()), "heldout_excess": q}
+    return enc, info
+
+
+def pair_set_recall(rp_p, col_p, dist_p, rp_e, col_e, dist_e, sample_rows=20000, seed=0):
+    """Recall of the prefilter lists against the exact lists, and whether every
+    prefilter entry (row, col, d) is an exact entry, checked on a random sample
+    of rows (all rows when there are fewer).  Columns ascend within each row in
+    both lists."""
+    n = len(rp_e) - 1
+    rows = np.arange(n) if n <= sample_rows else \
+        np.random.default_rng(seed).choice(n, sample_rows, replace=False)
+    subset = True
+    for r in rows:
+        a, b = slice(rp_p[r], rp_p[r + 1]), slice(rp_e[r], rp_e[r + 1])
+        ce, de = col_e[b], dist_e[b]
+        pos = np.searchsorted(ce, col_p[a])
+        ok = (pos < len(ce))
+        ok &= ce[np.minimum(pos, len(ce) - 1)] == col_p[a] if len(ce) else False
+        if not np.all(ok) or not np.array_equal(de[pos], dist_p[a]):
+            subset = False
+            break
+    return len(col_p) / max(len(col_e), 1), subset
+
+
+def run_proposer_bench(args, pkg, log):
+    eps = args.eps if args.eps else (300 if args.config == "c4" else 200)
+    minpts = 5
+    if args.config == "c4":
+        # train on the amplicon-scale generator with its own seed, evaluate on
+        # 100,000 held-out sequences of another draw
+        train = synth.config3(seed=4)
+        groups = synth.config3_groups()
+        enc, info = train_encoder(pkg, train, [(groups, 0.5)], args.pairs, eps, 4, args.epochs, log)
+        del train
+        full = synth.config3(seed=44)
+        pick = np.sort(np.random.default_rng(44).choice(len(full), args.n or 100_000, replace=False))
+        seqs = [full[k] for k in pick]
+        workload = ("learned proposer: encoder trained on 2M exact-labelled pairs of the "
+                    "amplicon-scale generator (seed 4; 50% within-ancestor, 50% random), "
+                    "evaluated on 100,000 held-out sequences of another draw")
+    else:
+        tr, g = synth.config5(seed=50, n=args.train_n, with_groups=True)
+        is_tree = g < 2000
+        child = np.where(is_tree, g // 5, g)
+        root = np.where(is_tree, g // 40, g)
+        enc, info = train_encoder(pkg, tr, [(g, 0.4), (child, 0.15), (root, 0.1)], args.pairs,
+                                  eps, 5, args.epochs, log)
+        del tr
+        seqs = synth.config5(seed=5, n=args.n or 500_000)
+        workload = ("end-to-end learned-prefilter DBSCAN: heavy-tailed lengths 80-1500 "
+                    "(Pareto 1.5), 70% 4-level tree with 2,000 leaf clusters, 20% tandem "
+                    "repeats of 171-bp monomers (2-8 copies, 3%), 10% noise")
+    n = len(seqs)
+    L = np.array([len(x) for x in seqs], dtype=np.float64)
+    log(f"evaluation set: n={n}, mean length {L.mean():.0f}, max {int(L.max())}; eps {eps}")
+    eng = pkg.Engine(seqs)
+    if not args.no_autotune:
+        log(f"autotune: {eng.autotune(use_cache=True)}")
+    # exact path: all-pairs epsilon lists (length-bound pruned) + DBSCAN
+    t0 = time.perf_counter()
+    erp, ecol, edist = eng.eps_csr(eps)
+    exact_lists_s = time.perf_counter() - t0
+    et = eng.last_timing()
+    log(f"exact eps lists: {exact_lists_s:.2f} s (kernel {et['kernel_ms'] / 1e3:.2f} s, "
+        f"{et['pairs']:.3g} pairs, {et['cells']:.3g} cells), nnz {len(ecol)}")
+    out = {"metric": "", "n_gpus": 1, "steps": 1, "warmup": 0, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic"}
+    # learned prefilter + exact verification
+    t0 = time.perf_counter()
+    prp, pcol, pdist, pst = enc.prefilter_csr(eng, eps)
+    pref_lists_s = time.perf_counter() - t0
+    log(f"prefilter lists: {pref_lists_s:.2f} s {pst}, nnz {len(pcol)}")
+    recall, subset = pair_set_recall(prp, pcol, pdist, erp, ecol, edist)
+    precision = pst["neighbours"] / max(pst["candidates"], 1)
+    log(f"recall {recall:.6f} (proposed subset of exact: {subset}), precision {precision:.4f}")
+    all_pairs = n * (n - 1) / 2
+    mfma = {"bound": "mfma", "kernel": "pair_candidates_kernel",
+            "achieved": all_pairs * 2 * 128 / (pst["propose_ms"] / 1e3) / 1e12,
+            "peak": 2500.0, "unit": "TFLOP/s (bf16 MFMA; 2*128 flop per unique pair)",
+            "traffic": None}
+    mfma["frac"] = mfma["achieved"] / mfma["peak"]
+    res = {"eps": eps, "recall": recall, "proposed_subset_of_exact": bool(subset),
+           "precision": precision, "candidates": pst["candidates"],
+           "neighbour_pairs": pst["neighbours"], "exact_neighbour_pairs": len(ecol) // 2,
+           "exact_lists_s": exact_lists_s, "prefilter_lists_s": pref_lists_s,
+           "exact_kernel_s": et["kernel_ms"] / 1e3, "exact_pairs_computed": et["pairs"],
+           "exact_cells_computed": et["cells"], "all_pairs": all_pairs,
+           "all_pairs_cells": 0.5 * (L.sum() ** 2 - (L * L).sum()),
+           "prefilter_phases_ms": pst, "training": info}
+    if args.config == "c4":
+        out.update({"metric": "learned-prefilter eps-neighbour recall (held-out)",
+                    "value": recall, "unit": "fraction", "higher_is_better": True,
+                    "ms_per_step": pref_lists_s * 1e3, "dtype": "bf16 (encoder) / int16 (verify)",
+                    "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma})
+    else:
+        t0 = time.perf_counter()
+        ecl = pkg.dbscan_csr(erp, ecol, edist, eps, eps, minpts)
+        exact_db_s = time.perf_counter() - t0
+        del ecol, edist
+        t0 = time.perf_counter()
+        pcl = pkg.dbscan_csr(prp, pcol, pdist, eps, eps, minpts)
+        pref_db_s = time.perf_counter() - t0
+        exact_s = exact_lists_s + exact_db_s
+        pref_s = pref_lists_s + pref_db_s
+        same = ecl == pcl
+        log(f"DBSCAN(eps {eps}, minpts {minpts}): exact {exact_s:.2f} s ({len(ecl)} clusters), "
+            f"prefilter {pref_s:.2f} s ({len(pcl)} clusters); identical: {same}")
+        res.update({"minpts": minpts, "exact_dbscan_s": exact_s, "prefilter_dbscan_s": pref_s,
+                    "speedup": exact_s / pref_s, "host_dbscan_s": [exact_db_s, pref_db_s],
+                    "clusters_exact": len(ecl), "clusters_prefilter": len(pcl),
+                    "identical_clusters": bool(same),
+                    "clustered_points": int(sum(len(c) for c in ecl))})
+        out.update({"metric": "DBSCAN s/run exact vs learned-prefilter", "value": pref_s,
+                    "unit": "s", "higher_is_better": False, "ms_per_step": pref_s * 1e3,
+                    "dtype": "bf16 (encoder) / int16 (verify)",
+                    "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma})
+    eng.close()
+    enc.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(seqs, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,8 +338,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3"],
-                    help="c2: 20,000 uneven-length (headline); c3: 105,000 amplicon-scale")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: 20,000 uneven-length (headline); c3: 105,000 amplicon-scale; "
+                         "c4: learned-proposer training + held-out recall; c5: end-to-end "
+                         "learned-prefilter DBSCAN vs the exact path")
+    ap.add_argument("--eps", type=int, default=0,
+                    help="c4/c5 epsilon (default 300 for c4, 200 for c5)")
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="c4/c5 labelled training pairs")
+    ap.add_argument("--epochs", type=int, default=8, help="c4/c5 training epochs")
+    ap.add_argument("--train-n", type=int, default=100_000, help="c5 training-set size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (torch) 3 times, "
                          "to calibrate FETCH_SIZE / WRITE_SIZE counters")
@@ -169,6 +367,11 @@ def main():
         sys.exit(2)
 
     pkg = load_pkg()
+    if args.config in ("c4", "c5"):
+        def log(msg):
+            print(msg, file=sys.stderr, flush=True)
+        run_proposer_bench(args, pkg, log)
+        return
     print("generating workload ...", file=sys.stderr, flush=True)
     seqs = synth.config2() if args.config == "c2" else synth.config3()
     if args.n:

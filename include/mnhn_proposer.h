@@ -6,7 +6,8 @@
  * (1024 counts over the 2-bit residues, L1-normalised) to a 128-d embedding:
  *   MLP 1024 -> 512 -> 256 -> 128, GELU, bf16 weights, fp32 accumulation,
  * trained on Smith-Waterman distances computed by the exact engine so that
- * ||e_i - e_j|| approximates d(i, j) / (4 max(l_i, l_j)).  It only PROPOSES
+ * ||e_i - e_j|| approximates d(i, j) / (match max(l_i, l_j)) (match = 4 with
+ * the default scores).  It only PROPOSES
  * pairs: every candidate is re-scored by the exact integer kernel, and only
  * exact distances enter the epsilon lists.  All kernels are hand-written
  * gfx950 HIP (bf16 MFMA GEMMs with fused bias/GELU, pairwise distance with
@@ -66,8 +67,8 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *ctx, const int32_t *ij,
 int mnhn_encoder_embed(mnhn_encoder *enc, mnhn_sw_ctx *ctx, float *out);
 
 /* Radius slack: a pair is proposed when
- *   ||e_i - e_j|| <= eps / (4 max(l_i, l_j)) + margin
- * and 4 |l_i - l_j| <= eps (an exact lower bound of d). */
+ *   ||e_i - e_j|| <= eps / (match max(l_i, l_j)) + margin
+ * and match |l_i - l_j| <= eps (an exact lower bound of d). */
 int mnhn_encoder_set_margin(mnhn_encoder *enc, float margin);
 float mnhn_encoder_margin(const mnhn_encoder *enc);
 

@@ -262,11 +262,11 @@ __global__ void row_norm_kernel(const bf16 *__restrict__ E, int n, float *__rest
 // -------------------------------------------------------- pairwise proposals
 // Upper-triangle 128 x 128 tiles of E E^T (K = 128), 4 waves of 64 x 64.
 // Pair (i, j), i < j, is proposed when
-//   |e_i|^2 + |e_j|^2 - 2 e_i.e_j <= (eps / (4 max(l_i, l_j)) + margin)^2
-// and 4 |l_i - l_j| <= eps.
+//   |e_i|^2 + |e_j|^2 - 2 e_i.e_j <= (eps / (match max(l_i, l_j)) + margin)^2
+// and match |l_i - l_j| <= eps (match = 4 with the default scores).
 __global__ __launch_bounds__(256) void pair_candidates_kernel(
     const bf16 *__restrict__ E, const float *__restrict__ norm, const int32_t *__restrict__ len, int n,
-    int tiles, float eps, float margin, unsigned long long *__restrict__ out,
+    int tiles, float eps, float match, float margin, unsigned long long *__restrict__ out,
     unsigned long long *__restrict__ count, unsigned long long cap) {
   constexpr int BT = 128, KH = 64, LK = KH + 8;
   __shared__ __attribute__((aligned(16))) bf16 sA[BT * LK];
@@ -342,9 +342,9 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
         if (vi && gj < n && gi < gj) {
           const int lj = len[gj];
           const int lm = max(max(li, lj), 1);
-          const float rad = eps / (4.0f * (float)lm) + margin;
+          const float rad = eps / (match * (float)lm) + margin;
           const float d2 = ni + norm[gj] - 2.0f * acc[i][j][r];
-          hit = d2 <= rad * rad && 4.0f * (float)abs(li - lj) <= eps;
+          hit = d2 <= rad * rad && match * (float)abs(li - lj) <= eps;
         }
         const unsigned long long bal = __ballot(hit);
         if (bal == 0ull) continue;
@@ -645,7 +645,7 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
         const int ia = ij[2 * pk], ib = ij[2 * pk + 1];
         h_idx[2 * k] = ia;
         h_idx[2 * k + 1] = ib;
-        const double L4 = 4.0 * std::max(std::max(c->len[ia], c->len[ib]), 1);
+        const double L4 = (double)c->match * std::max(std::max(c->len[ia], c->len[ib]), 1);
         h_tgt[k] = (float)(dist[pk] / L4);
         h_rad[k] = dist[pk] <= p->eps_train ? (float)(p->eps_train / L4) : -1.f;
       }
@@ -845,7 +845,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     hipError_t e = hipMemsetAsync(d_cnt64, 0, 8, st);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(pair_candidates_kernel, dim3((unsigned)nblk), dim3(256), 0, st, E16, norm, c->d_len, n,
-                         tiles, (float)eps_max, enc->margin, cand, d_cnt64, cap);
+                         tiles, (float)eps_max, (float)c->match, enc->margin, cand, d_cnt64, cap);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&m, d_cnt64, 8, hipMemcpyDeviceToHost, st);

@@ -1038,31 +1038,52 @@ void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
 
 // All unique pairs: group g holds sequences at length-order positions
 // [128g, 128g+128); its queries are the positions before its last member.
-Work triangle_work(const mnhn_sw_ctx *c, int S) {
+// Upper-triangle work over the length-sorted order.  With eps >= 0 the pairs
+// that the length bound already places beyond eps are left out: a pair's
+// local score is at most match * min(la, lb), so d >= match * |la - lb|.
+// Lengths fall along the order, so for target group g the queries that the
+// bound removes (match * (len_p - gmax[g]) > eps) are a prefix of positions.
+Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
   Work w;
   w.qids = c->order;
+  std::vector<int> first(c->groups, 0);
+  if (eps >= 0) {
+    for (int g = 0; g < c->groups; ++g) {
+      const long long gm = c->gmax[g];
+      int lo = 0, hi = std::min((g + 1) * kGroup, c->n);
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if ((long long)c->match * ((long long)c->len[c->order[mid]] - gm) > (long long)eps) lo = mid + 1;
+        else hi = mid;
+      }
+      first[g] = lo;
+    }
+  }
   double total = 0;
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     const int gm = c->gmax[g];
     const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
-    for (int p = 0; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
+    for (int p = first[g]; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
   }
   // ~256 items per CU: enough for the longest-first queue to balance the tail
   const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 256.0), 2.0e5);
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
-    if (last <= 0) continue;
-    add_group_items(c, w, g, 0, last, target, S);
+    if (last <= first[g]) continue;
+    add_group_items(c, w, g, first[g], last, target, S);
   }
-  // algorithmic cells over unique pairs
-  double sumL = 0, sumL2 = 0;
-  for (int i = 0; i < c->n; ++i) {
-    sumL += c->len[i];
-    sumL2 += (double)c->len[i] * c->len[i];
+  // algorithmic cells and pairs of the unique pairs the items cover
+  std::vector<double> prefix((size_t)c->n + 1, 0.0);
+  for (int p = 0; p < c->n; ++p) prefix[p + 1] = prefix[p] + c->len[c->order[p]];
+  double cells = 0, pairs = 0;
+  for (int t = 0; t < c->n; ++t) {
+    const int f = std::min(first[t / kGroup], t);
+    cells += (double)c->len[c->order[t]] * (prefix[t] - prefix[f]);
+    pairs += (double)(t - f);
   }
-  w.cells = 0.5 * (sumL * sumL - sumL2);
-  w.pairs = 0.5 * (double)c->n * (c->n - 1);
+  w.cells = cells;
+  w.pairs = pairs;
   return w;
 }
 
@@ -1337,7 +1358,7 @@ int mnhn_sw_matrix(mnhn_sw_ctx *c, int32_t *dist_out) {
   int rc = check_range(c);
   if (rc) return rc;
   const int S = pick_strip(c);
-  Work w = triangle_work(c, S);
+  Work w = triangle_work(c, S, -1);
   finalize_items(c, w, S, true);
   size_t out_n = 0;
   for (const auto &it : w.items) out_n += (size_t)(it.z - it.y) * kGroup;
@@ -1373,11 +1394,18 @@ namespace {
 
 // Run the all-pairs epsilon pass; on return the context's d_keys holds nkeys
 // unsorted keys (two per unordered pair with d <= eps_max).
+// MNHN_SW_LENGTH_PRUNE=0 keeps every unique pair in the epsilon passes (the
+// bound is exact, so the lists are the same either way).
+bool length_prune() {
+  const char *e = getenv("MNHN_SW_LENGTH_PRUNE");
+  return !(e && e[0] == '0');
+}
+
 int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
   int rc = check_range(c);
   if (rc) return rc;
   const int S = pick_strip(c);
-  Work w = triangle_work(c, S);
+  Work w = triangle_work(c, S, length_prune() ? eps_max : -1);
   finalize_items(c, w, S, false);
   const unsigned long long all = (unsigned long long)c->n * (unsigned long long)(c->n > 0 ? c->n - 1 : 0);
   unsigned long long cap = c->keys_cap;

@@ -120,7 +120,7 @@ def pareto_lengths(rng: np.random.Generator, n: int, alpha: float = 1.5, xm: flo
 
 def config5(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 8,
             per_child: int = 5, tree_frac: float = 0.70, repeat_frac: float = 0.20,
-            n_monomer_families: int = 200, monomer: int = 171) -> List[bytes]:
+            n_monomer_families: int = 200, monomer: int = 171, with_groups: bool = False):
     """End-to-end learned-prefilter workload (heavy-tailed lengths 80-1500):
       70% from a 4-level tree with 2,000 leaf clusters: 50 roots (Pareto
          lengths, alpha 1.5, x_m 80, clipped to 1,500) -> 8 children at 8% ->
@@ -128,7 +128,8 @@ def config5(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 
       20% tandem repeats: 2-8 copies of one of 200 random 171-bp monomers, every
          copy at 3% divergence;
       10% uniform random sequences with Pareto lengths.
-    (mutation: sub:indel 4:1)."""
+    (mutation: sub:indel 4:1).  with_groups=True also returns one label per
+    sequence: its leaf cluster, its monomer family, or a unique label for noise."""
     rng = _rng(seed)
     n_tree = int(round(tree_frac * n))
     n_rep = int(round(repeat_frac * n))
@@ -152,7 +153,22 @@ def config5(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 
             [mutate(rng, mono, 0.8 * 0.03, 0.2 * 0.03) for _ in range(copies)])))
     for L in pareto_lengths(rng, n_noise):
         out.append(to_bytes(random_codes(rng, L)))
-    return out
+    if not with_groups:
+        return out
+    k = np.arange(n)
+    groups = np.where(k < n_tree, k % len(leaves),
+                      np.where(k < n_tree + n_rep, len(leaves) + (k - n_tree) % n_monomer_families,
+                               len(leaves) + n_monomer_families + k))
+    return out, groups
+
+
+def config3_groups(n_anc: int = 500, per_anc: int = 200, noise_frac: float = 0.05) -> np.ndarray:
+    """Ancestor label of every sequence of config3() with the same sizes
+    (noise sequences get unique labels)."""
+    n_desc = n_anc * per_anc
+    n = n_desc + int(round(noise_frac * n_desc))
+    k = np.arange(n)
+    return np.where(k < n_desc, k // per_anc, n_anc + k)
 
 
 def labelled_pairs(rng: np.random.Generator, n: int, group: np.ndarray, m: int,

@@ -125,6 +125,35 @@ def test_eps_csr_vs_oracle(pkg, oracle, eps):
     assert np.array_equal(dense, expect)
 
 
+@pytest.mark.parametrize("eps", [0, 40, 250, 900])
+def test_eps_csr_length_pruning_is_exact(pkg, oracle, monkeypatch, eps):
+    """The epsilon pass leaves out pairs with match * |la - lb| > eps; over
+    widely spread lengths the lists must equal the oracle's with and without
+    that pruning, and the pruned pass must compute fewer pairs."""
+    rng = np.random.default_rng(100 + eps)
+    seqs = _rand_set(rng, 300, 0, 700)
+    base = _rand_set(rng, 6, 50, 650, b"ACGT")
+    for b in base:
+        for _ in range(12):
+            seqs.append(synth.to_bytes(synth.mutate(
+                rng, np.frombuffer(b, np.uint8).copy() % 4, 0.03, 0.01)))
+    n = len(seqs)
+    _say(f"eps csr length pruning n={n} eps={eps}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    expect = np.where(want <= eps, want, -1)
+    np.fill_diagonal(expect, -1)
+    got = {}
+    with pkg.Engine(seqs) as eng:
+        for prune in ("0", "1"):
+            monkeypatch.setenv("MNHN_SW_LENGTH_PRUNE", prune)
+            rp, col, dist = eng.eps_csr(eps)
+            got[prune] = (eng.last_timing()["pairs"], _csr_to_dense(rp, col, dist, n))
+    assert got["0"][0] == n * (n - 1) // 2
+    assert got["1"][0] < got["0"][0]
+    assert np.array_equal(got["0"][1], expect)
+    assert np.array_equal(got["1"][1], expect)
+
+
 def test_config1_full_matrix_and_eps(pkg, oracle):
     seqs = synth.config1(seed=1)
     n = len(seqs)

@@ -169,41 +169,75 @@ def sample_training_pairs(rng, group_levels, n, m):
     return ij[rng.permutation(len(ij))]
 
 
-def train_encoder(pkg, seqs, group_levels, n_pairs, eps, seed, epochs, log):
-    """Label n_pairs pairs of seqs with the exact kernel, train an encoder on
-    90% of them and set its margin from the held-out 10%: the largest radius
-    excess of a held-out neighbour pair (d <= eps) plus a small slack."""
+def calibrate_margins(emb16, L, i, j, eps, q=0.9999, buckets=8, slack=0.005):
+    """Margin table from neighbour pairs (i, j) of held-out sequences: the
+    pairs are split into length buckets of equal size by their longer length,
+    and each bucket's margin is the q-quantile of its radius excess (its
+    maximum when the bucket is too small for that quantile) plus a slack."""
+    ex = radius_excess(emb16, L, i, j, eps)
+    lmax = np.maximum(L[i], L[j])
+    if len(ex) == 0:
+        return np.array([2**31 - 1], np.int32), np.array([slack], np.float32), {}
+    qs = np.unique(np.quantile(lmax, np.linspace(0, 1, buckets + 1)[1:-1]).astype(np.int64))
+    upper = np.concatenate([qs, [2**31 - 1]]).astype(np.int64)
+    margins, lo = [], -1
+    for u in upper:
+        sel = (lmax > lo) & (lmax <= u)
+        e = ex[sel]
+        if len(e) == 0:
+            m = float(np.quantile(ex, q))
+        elif len(e) * (1 - q) < 10:
+            m = float(e.max())
+        else:
+            m = float(np.quantile(e, q))
+        margins.append(max(m, 0.0) + slack)
+        lo = u
+    info = {"pairs": int(len(ex)), "excess_q50": float(np.quantile(ex, 0.5)),
+            "excess_q999": float(np.quantile(ex, 0.999)), "excess_max": float(ex.max()),
+            "max_len": upper.tolist(), "margins": margins}
+    return upper.astype(np.int32), np.array(margins, np.float32), info
+
+
+def train_encoder(pkg, seqs, group_levels, holdout_groups, n_pairs, eps, seed, epochs, log):
+    """Label n_pairs pairs of seqs with the exact kernel and train an encoder on
+    the pairs between sequences outside a held-out 10% of `holdout_groups`;
+    the margin table comes from neighbour pairs (d <= eps) inside the held-out
+    groups, which the encoder never saw."""
     rng = np.random.default_rng(seed)
     n = len(seqs)
     L = np.array([len(x) for x in seqs], dtype=np.int64)
+    ug = np.unique(holdout_groups)
+    held = np.isin(holdout_groups, rng.choice(ug, max(1, len(ug) // 10), replace=False))
     ij = sample_training_pairs(rng, group_levels, n, n_pairs)
     eng = pkg.Engine(seqs)
     t0 = time.perf_counter()
     d = eng.pairs(ij)
     label_s = time.perf_counter() - t0
-    cut = int(0.9 * len(ij))
-    log(f"labelled {len(ij)} pairs in {label_s:.2f} s; neighbours (d <= {eps}): "
-        f"{int((d <= eps).sum())}")
+    tr = ~held[ij[:, 0]] & ~held[ij[:, 1]]
+    cal = held[ij[:, 0]] & held[ij[:, 1]]
+    log(f"labelled {len(ij)} pairs in {label_s:.2f} s; training pairs {int(tr.sum())}, "
+        f"held-out pairs {int(cal.sum())}; neighbours (d <= {eps}): {int((d <= eps).sum())}")
     enc = pkg.Encoder(seed=seed)
-    st = enc.train(eng, ij[:cut], d[:cut], epochs=epochs, batch=1024, lr=2e-3, eps_train=eps,
+    st = enc.train(eng, ij[tr], d[tr], epochs=epochs, batch=1024, lr=2e-3, eps_train=eps,
                    neighbor_weight=4.0, seed=seed)
     log(f"trained: loss {st['first_loss']:.4f} -> {st['final_loss']:.4f}, {st['steps']} steps, "
         f"{st['seconds']:.1f} s")
     emb16 = _bf16_round(enc.embed(eng))
-    hv, dv = ij[cut:], d[cut:]
+    hv, dv = ij[cal], d[cal]
     nb = dv <= eps
-    ex = radius_excess(emb16, L, hv[nb, 0], hv[nb, 1], eps)
-    q = {f"q{k}": float(np.quantile(ex, k / 100.0)) for k in (50, 99, 99.9)}
-    q["max"] = float(ex.max()) if len(ex) else 0.0
-    margin = max(q["max"], 0.0) + 0.01
-    enc.margin = margin
-    log(f"margin {margin:.4f} from {int(nb.sum())} held-out neighbour pairs (excess {q})")
+    upper, margins, cinfo = calibrate_margins(emb16, L, hv[nb, 0], hv[nb, 1], eps)
+    enc.set_margin_table(upper, margins)
+    log(f"margin table from {int(nb.sum())} held-out neighbour pairs: {cinfo}")
     eng.close()
-    info = {"labelled_pairs": int(len(ij)), "train_pairs": cut, "label_s": label_s,
+    info = {"labelled_pairs": int(len(ij)), "train_pairs": int(tr.sum()), "label_s": label_s,
             "train_s": st["seconds"], "epochs": epochs, "steps": st["steps"],
             "loss_first": st["first_loss"], "loss_final": st["final_loss"],
-            "margin": margin, "heldout_neighbour_pairs": int(nb.sum()), "heldout_excess": q}
+            "calibration": cinfo}
     return enc, info
+
+
+def same_partition(a, b):
+    return sorted(tuple(sorted(c)) for c in a) == sorted(tuple(sorted(c)) for c in b)
 
 
 def pair_set_recall(rp_p, col_p, dist_p, rp_e, col_e, dist_e, sample_rows=20000, seed=0):
@@ -231,30 +265,41 @@ def run_proposer_bench(args, pkg, log):
     eps = args.eps if args.eps else (300 if args.config == "c4" else 200)
     minpts = 5
     if args.config == "c4":
-        # train on the amplicon-scale generator with its own seed, evaluate on
-        # 100,000 held-out sequences of another draw
-        train = synth.config3(seed=4)
-        groups = synth.config3_groups()
-        enc, info = train_encoder(pkg, train, [(groups, 0.5)], args.pairs, eps, 4, args.epochs, log)
+        # train on the amplicon-scale generator (its lengths, mutation rates and
+        # noise) drawn with 2,000 ancestors under a new seed, evaluate on
+        # 100,000 sequences of another draw
+        train = synth.config3(seed=4, n_anc=args.train_anc)
+        groups = synth.config3_groups(n_anc=args.train_anc)
+        enc, info = train_encoder(pkg, train, [(groups, 0.5)], groups, args.pairs, eps, 4,
+                                  args.epochs, log)
         del train
         full = synth.config3(seed=44)
         pick = np.sort(np.random.default_rng(44).choice(len(full), args.n or 100_000, replace=False))
         seqs = [full[k] for k in pick]
-        workload = ("learned proposer: encoder trained on 2M exact-labelled pairs of the "
-                    "amplicon-scale generator (seed 4; 50% within-ancestor, 50% random), "
-                    "evaluated on 100,000 held-out sequences of another draw")
+        workload = (f"learned proposer: encoder trained on {args.pairs} exact-labelled pairs "
+                    f"(50% within-ancestor, 50% random) of the amplicon-scale generator drawn "
+                    f"with {args.train_anc} ancestors (seed 4), evaluated on 100,000 sequences "
+                    f"of another draw (seed 44)")
     else:
-        tr, g = synth.config5(seed=50, n=args.train_n, with_groups=True)
-        is_tree = g < 2000
-        child = np.where(is_tree, g // 5, g)
-        root = np.where(is_tree, g // 40, g)
-        enc, info = train_encoder(pkg, tr, [(g, 0.4), (child, 0.15), (root, 0.1)], args.pairs,
-                                  eps, 5, args.epochs, log)
+        # training draws of the same generator under other seeds
+        parts, labels = [], []
+        for k in range(args.train_draws):
+            tr, g = synth.config5(seed=50 + k, n=args.train_n, with_groups=True)
+            parts += tr
+            labels.append((g, k))
         del tr
+        big = 1 << 40
+        leaf = np.concatenate([g + k * big for g, k in labels])
+        child = np.concatenate([np.where(g < 2000, g // 5, g) + k * big for g, k in labels])
+        root = np.concatenate([np.where(g < 2000, g // 40, g) + k * big for g, k in labels])
+        enc, info = train_encoder(pkg, parts, [(leaf, 0.4), (child, 0.15), (root, 0.1)], root,
+                                  args.pairs, eps, 5, args.epochs, log)
+        del parts
         seqs = synth.config5(seed=5, n=args.n or 500_000)
         workload = ("end-to-end learned-prefilter DBSCAN: heavy-tailed lengths 80-1500 "
                     "(Pareto 1.5), 70% 4-level tree with 2,000 leaf clusters, 20% tandem "
-                    "repeats of 171-bp monomers (2-8 copies, 3%), 10% noise")
+                    "repeats of 171-bp monomers (2-8 copies, 3%), 10% noise; encoder trained "
+                    f"on {args.train_draws} other draws of {args.train_n}")
     n = len(seqs)
     L = np.array([len(x) for x in seqs], dtype=np.float64)
     log(f"evaluation set: n={n}, mean length {L.mean():.0f}, max {int(L.max())}; eps {eps}")
@@ -268,8 +313,7 @@ def run_proposer_bench(args, pkg, log):
     et = eng.last_timing()
     log(f"exact eps lists: {exact_lists_s:.2f} s (kernel {et['kernel_ms'] / 1e3:.2f} s, "
         f"{et['pairs']:.3g} pairs, {et['cells']:.3g} cells), nnz {len(ecol)}")
-    out = {"metric": "", "n_gpus": 1, "steps": 1, "warmup": 0, "scaling": "weak",
-           "vs_baseline": None, "data": "synthetic"}
+    out = {}
     # learned prefilter + exact verification
     t0 = time.perf_counter()
     prp, pcol, pdist, pst = enc.prefilter_csr(eng, eps)
@@ -279,9 +323,11 @@ def run_proposer_bench(args, pkg, log):
     precision = pst["neighbours"] / max(pst["candidates"], 1)
     log(f"recall {recall:.6f} (proposed subset of exact: {subset}), precision {precision:.4f}")
     all_pairs = n * (n - 1) / 2
+    # the pairwise kernel's algorithmic work: 128 x 128 x 128 multiply-adds per
+    # tile of the length band
     mfma = {"bound": "mfma", "kernel": "pair_candidates_kernel",
-            "achieved": all_pairs * 2 * 128 / (pst["propose_ms"] / 1e3) / 1e12,
-            "peak": 2500.0, "unit": "TFLOP/s (bf16 MFMA; 2*128 flop per unique pair)",
+            "achieved": pst["tiles"] * 2 * 128 ** 3 / (pst["propose_ms"] / 1e3) / 1e12,
+            "peak": 2500.0, "unit": "TFLOP/s (bf16 MFMA; 2*128^3 flop per band tile)",
             "traffic": None}
     mfma["frac"] = mfma["achieved"] / mfma["peak"]
     res = {"eps": eps, "recall": recall, "proposed_subset_of_exact": bool(subset),
@@ -292,33 +338,34 @@ def run_proposer_bench(args, pkg, log):
            "exact_cells_computed": et["cells"], "all_pairs": all_pairs,
            "all_pairs_cells": 0.5 * (L.sum() ** 2 - (L * L).sum()),
            "prefilter_phases_ms": pst, "training": info}
+    t0 = time.perf_counter()
+    ecl = pkg.dbscan_csr(erp, ecol, edist, eps, eps, minpts)
+    exact_db_s = time.perf_counter() - t0
+    del ecol, edist
+    t0 = time.perf_counter()
+    pcl = pkg.dbscan_csr(prp, pcol, pdist, eps, eps, minpts)
+    pref_db_s = time.perf_counter() - t0
+    exact_s = exact_lists_s + exact_db_s
+    pref_s = pref_lists_s + pref_db_s
+    same = same_partition(ecl, pcl)
+    log(f"DBSCAN(eps {eps}, minpts {minpts}): exact {exact_s:.2f} s ({len(ecl)} clusters), "
+        f"prefilter {pref_s:.2f} s ({len(pcl)} clusters); same partition: {same}, "
+        f"same member order: {ecl == pcl}")
+    res.update({"minpts": minpts, "exact_dbscan_s": exact_s, "prefilter_dbscan_s": pref_s,
+                "speedup": exact_s / pref_s, "host_dbscan_s": [exact_db_s, pref_db_s],
+                "clusters_exact": len(ecl), "clusters_prefilter": len(pcl),
+                "identical_clusters": bool(same), "identical_member_order": ecl == pcl,
+                "clustered_points": int(sum(len(c) for c in ecl))})
+    common = {"n_gpus": 1, "steps": 1, "warmup": 0, "scaling": "weak", "vs_baseline": None,
+              "data": "synthetic", "dtype": "bf16 (encoder) / int16 (verify)",
+              "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma}
     if args.config == "c4":
         out.update({"metric": "learned-prefilter eps-neighbour recall (held-out)",
                     "value": recall, "unit": "fraction", "higher_is_better": True,
-                    "ms_per_step": pref_lists_s * 1e3, "dtype": "bf16 (encoder) / int16 (verify)",
-                    "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma})
+                    "ms_per_step": pref_lists_s * 1e3, **common})
     else:
-        t0 = time.perf_counter()
-        ecl = pkg.dbscan_csr(erp, ecol, edist, eps, eps, minpts)
-        exact_db_s = time.perf_counter() - t0
-        del ecol, edist
-        t0 = time.perf_counter()
-        pcl = pkg.dbscan_csr(prp, pcol, pdist, eps, eps, minpts)
-        pref_db_s = time.perf_counter() - t0
-        exact_s = exact_lists_s + exact_db_s
-        pref_s = pref_lists_s + pref_db_s
-        same = ecl == pcl
-        log(f"DBSCAN(eps {eps}, minpts {minpts}): exact {exact_s:.2f} s ({len(ecl)} clusters), "
-            f"prefilter {pref_s:.2f} s ({len(pcl)} clusters); identical: {same}")
-        res.update({"minpts": minpts, "exact_dbscan_s": exact_s, "prefilter_dbscan_s": pref_s,
-                    "speedup": exact_s / pref_s, "host_dbscan_s": [exact_db_s, pref_db_s],
-                    "clusters_exact": len(ecl), "clusters_prefilter": len(pcl),
-                    "identical_clusters": bool(same),
-                    "clustered_points": int(sum(len(c) for c in ecl))})
         out.update({"metric": "DBSCAN s/run exact vs learned-prefilter", "value": pref_s,
-                    "unit": "s", "higher_is_better": False, "ms_per_step": pref_s * 1e3,
-                    "dtype": "bf16 (encoder) / int16 (verify)",
-                    "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma})
+                    "unit": "s", "higher_is_better": False, "ms_per_step": pref_s * 1e3, **common})
     eng.close()
     enc.close()
     if not args.no_cpu_baseline:
@@ -346,7 +393,9 @@ def main():
                     help="c4/c5 epsilon (default 300 for c4, 200 for c5)")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="c4/c5 labelled training pairs")
     ap.add_argument("--epochs", type=int, default=8, help="c4/c5 training epochs")
-    ap.add_argument("--train-n", type=int, default=100_000, help="c5 training-set size")
+    ap.add_argument("--train-n", type=int, default=100_000, help="c5 sequences per training draw")
+    ap.add_argument("--train-draws", type=int, default=4, help="c5 training draws")
+    ap.add_argument("--train-anc", type=int, default=2000, help="c4 training ancestors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (torch) 3 times, "
                          "to calibrate FETCH_SIZE / WRITE_SIZE counters")

@@ -48,6 +48,7 @@ typedef struct {
   int64_t candidates;     /* unordered pairs proposed (i < j) */
   int64_t neighbours;     /* of those, exact d <= eps_max */
   double profile_ms, embed_ms, propose_ms, verify_ms, csr_ms; /* device time per phase */
+  int64_t tiles;          /* 128 x 128 embedding tiles the length band leaves to compare */
 } mnhn_prefilter_stats;
 
 /* 5-mer profiles of every sequence (n x 1024 fp32, host buffer). */
@@ -70,9 +71,19 @@ int mnhn_encoder_embed(mnhn_encoder *enc, mnhn_sw_ctx *ctx, float *out);
  *   ||e_i - e_j|| <= eps / (match max(l_i, l_j)) + margin
  * and match |l_i - l_j| <= eps (an exact lower bound of d). */
 int mnhn_encoder_set_margin(mnhn_encoder *enc, float margin);
+
+/* Margin per length bucket: a pair whose longer sequence has length l uses
+ * margin[b] of the first bucket b with l <= max_len[b] (margin[n-1] above the
+ * last bound).  1 <= n <= 16, max_len strictly ascending, margins >= 0.
+ * mnhn_encoder_set_margin(m) is the one-bucket table. */
+int mnhn_encoder_set_margin_table(mnhn_encoder *enc, const int32_t *max_len, const float *margin,
+                                  int n);
+
+/* Largest margin of the table. */
 float mnhn_encoder_margin(const mnhn_encoder *enc);
 
-/* Save / load the weights (little-endian fp32 blob with a header). */
+/* Save / load the weights and the margin table (little-endian fp32 blob with
+ * a header; files with a single margin load as a one-bucket table). */
 int mnhn_encoder_save(const mnhn_encoder *enc, const char *path);
 int mnhn_encoder_load(mnhn_encoder *enc, const char *path);
 

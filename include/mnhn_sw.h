@@ -28,7 +28,8 @@ extern "C" {
 #define MNHN_ERR_DEVICE 2    /* HIP runtime / kernel launch error */
 #define MNHN_ERR_NOMEM 3     /* device or host allocation failed */
 #define MNHN_ERR_NODEVICE 4  /* no usable GPU */
-#define MNHN_ERR_RANGE 5     /* a score would not fit the 16-bit lane format */
+#define MNHN_ERR_RANGE 5     /* a score would not fit the 16-bit lane format, or a
+                                  work list exceeds its 32-bit offsets */
 
 /* Scoring parameters.  The reference hard-codes match 4 / mismatch -4 / gap 3
  * (smith-waterman.c:4-14); NULL selects those.  Requires match > 0,
@@ -93,7 +94,8 @@ int mnhn_sw_pairs(mnhn_sw_ctx *ctx, const int32_t *ij, int64_t m,
 
 /* Full n x n distance matrix (row-major, host buffer of n*n int32).  Every
  * unordered pair is computed once and mirrored; the diagonal is 0.  For
- * moderate n (tests, small data sets). */
+ * moderate n (tests, small data sets): MNHN_ERR_RANGE when the unordered
+ * pairs exceed 2^31 / 128 work slots. */
 int mnhn_sw_matrix(mnhn_sw_ctx *ctx, int32_t *dist_out);
 
 /* All-pairs epsilon-neighbour lists with d <= eps_max, computed in one

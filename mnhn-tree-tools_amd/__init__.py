@@ -36,7 +36,7 @@ ABI_SYMBOLS = [
 PROPOSER_SYMBOLS = [
     "mnhn_kmer_profiles", "mnhn_encoder_create", "mnhn_encoder_destroy",
     "mnhn_encoder_train", "mnhn_encoder_embed", "mnhn_encoder_set_margin",
-    "mnhn_encoder_margin", "mnhn_encoder_save", "mnhn_encoder_load",
+    "mnhn_encoder_set_margin_table", "mnhn_encoder_margin", "mnhn_encoder_save", "mnhn_encoder_load",
     "mnhn_prefilter_eps_csr",
 ]
 HOST_SYMBOLS = [
@@ -95,7 +95,7 @@ class PrefilterStats(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_int64), ("neighbours", ctypes.c_int64),
                 ("profile_ms", ctypes.c_double), ("embed_ms", ctypes.c_double),
                 ("propose_ms", ctypes.c_double), ("verify_ms", ctypes.c_double),
-                ("csr_ms", ctypes.c_double)]
+                ("csr_ms", ctypes.c_double), ("tiles", ctypes.c_int64)]
 
 
 _LIB = None
@@ -136,6 +136,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                        ctypes.POINTER(TrainStats)]
     lib.mnhn_encoder_embed.argtypes = [vp, vp, vp]
     lib.mnhn_encoder_set_margin.argtypes = [vp, ctypes.c_float]
+    lib.mnhn_encoder_set_margin_table.argtypes = [vp, vp, vp, ctypes.c_int]
     lib.mnhn_encoder_margin.argtypes = [vp]
     lib.mnhn_encoder_margin.restype = ctypes.c_float
     lib.mnhn_encoder_save.argtypes = [vp, ctypes.c_char_p]
@@ -320,6 +321,16 @@ class Encoder:
     def margin(self, v: float) -> None:
         _check(self.lib.mnhn_encoder_set_margin(self.h, float(v)), "mnhn_encoder_set_margin")
 
+    def set_margin_table(self, max_len, margins) -> None:
+        """Margin per length bucket of the pair's longer sequence (first bucket
+        with length <= max_len[b]; the last margin above the last bound)."""
+        ml = np.ascontiguousarray(max_len, dtype=np.int32)
+        mg = np.ascontiguousarray(margins, dtype=np.float32)
+        if ml.shape != mg.shape:
+            raise MnhnError("max_len and margins differ in length")
+        _check(self.lib.mnhn_encoder_set_margin_table(self.h, ml.ctypes.data, mg.ctypes.data, len(ml)),
+               "mnhn_encoder_set_margin_table")
+
     def save(self, path: str) -> None:
         _check(self.lib.mnhn_encoder_save(self.h, path.encode()), "mnhn_encoder_save")
 
@@ -337,18 +348,24 @@ class Encoder:
 
 
 def load_encoder_weights(path: str):
-    """Read a file written by Encoder.save: (margin, [(Wt [out][in], b [out]) x 3])."""
+    """Read a file written by Encoder.save: ((max_len[], margin[]),
+    [(Wt [out][in], b [out]) x 3])."""
     with open(path, "rb") as f:
-        if f.read(8) != b"MNHNENC1":
+        magic = f.read(8)
+        if magic == b"MNHNENC1":
+            table = (np.array([2**31 - 1], np.int32), np.frombuffer(f.read(4), "<f4").copy())
+        elif magic == b"MNHNENC2":
+            nb = int(np.frombuffer(f.read(4), "<i4")[0])
+            table = (np.frombuffer(f.read(4 * nb), "<i4").copy(), np.frombuffer(f.read(4 * nb), "<f4").copy())
+        else:
             raise MnhnError(f"{path}: not an encoder file")
-        margin = float(np.frombuffer(f.read(4), "<f4")[0])
         layers = []
         for _ in range(3):
             din, dout = np.frombuffer(f.read(8), "<i4")
             w = np.frombuffer(f.read(4 * din * dout), "<f4").reshape(dout, din)
             b = np.frombuffer(f.read(4 * dout), "<f4")
             layers.append((w, b))
-    return margin, layers
+    return table, layers
 
 
 class _Cluster(ctypes.Structure):

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "mnhn_sw.h"
@@ -42,6 +43,22 @@ namespace mnhn_detail {
     hipError_t e_ = (expr);                            \
     if (e_ != hipSuccess) return MNHN_ERR_DEVICE;      \
   } while (0)
+
+// Wave-uniform copy of a 64-bit value held by the first active lane.  The
+// readfirstlane builtin works on 32-bit ints, so each half is reinterpreted
+// as unsigned before it is widened (a signed low half would sign-extend).
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Workgroups for a grid-stride loop over n elements: element launches stay far
+// below 2^31 work-items whatever n is.
+inline unsigned stride_blocks(long long n, int tb) {
+  const long long b = (n + tb - 1) / tb;
+  return (unsigned)std::min<long long>(std::max<long long>(b, 1), 1 << 16);
+}
 
 template <typename T>
 int dev_alloc(T **p, size_t count) {

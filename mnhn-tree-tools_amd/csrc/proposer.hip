@@ -248,45 +248,69 @@ __global__ void refresh_weights_kernel(const float *__restrict__ wt, int out_d, 
   w16[(size_t)c * out_d + o] = b;
 }
 
-__global__ void row_norm_kernel(const bf16 *__restrict__ E, int n, float *__restrict__ norm) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  float s = 0.f;
-  for (int k = 0; k < D3; ++k) {
-    const float x = (float)E[(size_t)r * D3 + k];
-    s += x * x;
+// -------------------------------------------------------- pairwise proposals
+// The pairwise stage works in length order (longest first): row p of Es is
+// sequence order[p].  A pair (p, q), p < q, is proposed when
+//   match |l_p - l_q| <= eps                        (exact bound: d >= match |dl|)
+//   ||e_p - e_q||^2 <= rad^2,  rad = eps / (match l_long) + margin(l_long)
+// where l_long is the longer length of the pair and margin(l) comes from the
+// encoder's margin table (first bucket whose upper length bound holds l).
+constexpr int kMaxMarginBuckets = 16;
+struct MarginTable {
+  int n;
+  int upper[kMaxMarginBuckets];
+  float margin[kMaxMarginBuckets];
+};
+
+// One wave per sorted row: copy of the bf16 embedding, its squared norm (the
+// same bf16 values the MFMA tiles read), length, and the row's radius term.
+__global__ __launch_bounds__(256) void sorted_rows_kernel(const bf16 *__restrict__ E,
+                                                          const int32_t *__restrict__ order,
+                                                          const int32_t *__restrict__ len, int n,
+                                                          float eps_over_match, MarginTable mt,
+                                                          bf16 *__restrict__ Es, float *__restrict__ norm,
+                                                          float *__restrict__ srad, int32_t *__restrict__ lens) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= n) return;
+  const int id = order[p];
+  const bf16 x0 = E[(size_t)id * D3 + lane], x1 = E[(size_t)id * D3 + lane + 64];
+  Es[(size_t)p * D3 + lane] = x0;
+  Es[(size_t)p * D3 + lane + 64] = x1;
+  float ss = (float)x0 * (float)x0 + (float)x1 * (float)x1;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+  if (lane == 0) {
+    const int L = len[id];
+    float m = mt.margin[mt.n - 1];
+    for (int k = mt.n - 1; k >= 0; --k)
+      if (L <= mt.upper[k]) m = mt.margin[k];
+    norm[p] = ss;
+    lens[p] = L;
+    srad[p] = eps_over_match / (float)max(L, 1) + m;
   }
-  norm[r] = s;
 }
 
-// -------------------------------------------------------- pairwise proposals
-// Upper-triangle 128 x 128 tiles of E E^T (K = 128), 4 waves of 64 x 64.
-// Pair (i, j), i < j, is proposed when
-//   |e_i|^2 + |e_j|^2 - 2 e_i.e_j <= (eps / (match max(l_i, l_j)) + margin)^2
-// and match |l_i - l_j| <= eps (match = 4 with the default scores).
+// 128 x 128 tiles of Es Es^T (K = 128) over the length band: the host lists,
+// per row tile ti, the column tiles ti .. that the length bound can reach;
+// tile_off[ti] is the first block of row tile ti.  4 waves of 64 x 64 per
+// tile.  Hits are counted first and reserved with one atomic per wave, then
+// written (sorted positions p << 32 | q) at consecutive addresses.
 __global__ __launch_bounds__(256) void pair_candidates_kernel(
-    const bf16 *__restrict__ E, const float *__restrict__ norm, const int32_t *__restrict__ len, int n,
-    int tiles, float eps, float match, float margin, unsigned long long *__restrict__ out,
-    unsigned long long *__restrict__ count, unsigned long long cap) {
+    const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
+    const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
+    unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap) {
   constexpr int BT = 128, KH = 64, LK = KH + 8;
   __shared__ __attribute__((aligned(16))) bf16 sA[BT * LK];
   __shared__ __attribute__((aligned(16))) bf16 sB[BT * LK];
-  // block -> (ti, tj), ti <= tj, row-major over the upper triangle: row t
-  // starts at offset(t) = t*tiles - t(t-1)/2
-  long long b = blockIdx.x;
-  int ti;
-  {
-    auto offset = [tiles](long long t) { return t * (long long)tiles - t * (t - 1) / 2; };
-    const double T2 = 2.0 * (double)tiles + 1.0;
-    long long t = (long long)floor((T2 - sqrt(fmax(T2 * T2 - 8.0 * (double)b, 0.0))) * 0.5);
-    if (t < 0) t = 0;
-    if (t > tiles - 1) t = tiles - 1;
-    while (t + 1 < tiles && offset(t + 1) <= b) ++t;
-    while (t > 0 && offset(t) > b) --t;
-    ti = (int)t;
-    b -= offset(t);
+  const int blk = blockIdx.x;
+  int lo = 0, hi = tiles;  // tile_off[lo] <= blk < tile_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= blk) lo = mid;
+    else hi = mid;
   }
-  const int tj = ti + (int)b;
+  const int ti = lo, tj = ti + (blk - tile_off[ti]);
   const int i0 = ti * BT, j0 = tj * BT;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -306,8 +330,8 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
       bf16x8 va, vb;
 #pragma unroll
       for (int q = 0; q < 8; ++q) { va[q] = (bf16)0.0f; vb[q] = (bf16)0.0f; }
-      if (i0 + r < n) va = *reinterpret_cast<const bf16x8 *>(E + (size_t)(i0 + r) * D3 + kh + k);
-      if (j0 + r < n) vb = *reinterpret_cast<const bf16x8 *>(E + (size_t)(j0 + r) * D3 + kh + k);
+      if (i0 + r < n) va = *reinterpret_cast<const bf16x8 *>(Es + (size_t)(i0 + r) * D3 + kh + k);
+      if (j0 + r < n) vb = *reinterpret_cast<const bf16x8 *>(Es + (size_t)(j0 + r) * D3 + kh + k);
       *reinterpret_cast<bf16x8 *>(sA + r * LK + k) = va;
       *reinterpret_cast<bf16x8 *>(sB + r * LK + k) = vb;
     }
@@ -326,59 +350,92 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
   }
-  // epilogue: radius test and wave-aggregated emission
+  // per-lane row / column terms (out-of-range rows and columns get lengths
+  // that no |dl| <= max_dl can match)
+  float ni[4][4], si[4][4], nj[4], sj[4];
+  int li[4][4], lj[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-      const bool vi = gi < n;
-      const float ni = vi ? norm[gi] : 0.f;
-      const int li = vi ? len[gi] : 0;
+      const bool v = gi < n;
+      ni[i][r] = v ? norm[gi] : 0.f;
+      si[i][r] = v ? srad[gi] : 0.f;
+      li[i][r] = v ? lens[gi] : -(1 << 28);
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+    const bool v = gj < n;
+    nj[j] = v ? norm[gj] : 0.f;
+    sj[j] = v ? srad[gj] : 0.f;
+    lj[j] = v ? lens[gj] : (1 << 29);
+  }
+  const bool diag = ti == tj;
+  auto hit = [&](int i, int r, int j) -> bool {
+    const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+    const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+    const int dl = li[i][r] - lj[j];
+    const float rad = dl >= 0 ? si[i][r] : sj[j];
+    const float d2 = ni[i][r] + nj[j] - 2.0f * acc[i][j][r];
+    return (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad * rad;
+  };
+  unsigned total = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) total += (unsigned)__popcll(__ballot(hit(i, r, j)));
+  if (total == 0) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(count, (unsigned long long)total);
+  base = uniform_u64(base);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-        bool hit = false;
-        if (vi && gj < n && gi < gj) {
-          const int lj = len[gj];
-          const int lm = max(max(li, lj), 1);
-          const float rad = eps / (match * (float)lm) + margin;
-          const float d2 = ni + norm[gj] - 2.0f * acc[i][j][r];
-          hit = d2 <= rad * rad && match * (float)abs(li - lj) <= eps;
+        const bool h = hit(i, r, j);
+        const unsigned long long bal = __ballot(h);
+        if (h) {
+          const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+          const unsigned long long pos = base + below;
+          const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+          const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+          if (pos < cap) out[pos] = ((unsigned long long)gi << 32) | (unsigned)gj;
         }
-        const unsigned long long bal = __ballot(hit);
-        if (bal == 0ull) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(bal));
-        base = __builtin_amdgcn_readfirstlane((unsigned)base) |
-               ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
-        const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-        const unsigned long long pos = base + below;
-        if (hit && pos < cap) out[pos] = ((unsigned long long)gi << 32) | (unsigned)gj;
+        base += (unsigned long long)__popcll(bal);
       }
-    }
 }
 
-// candidate keys (i << 32 | j) -> per-row counts
-__global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long long m, int32_t *__restrict__ cnt) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) atomicAdd(&cnt[(int)(c[k] >> 32)], 1);
+// candidate keys (sorted positions p << 32 | q) -> per-row counts of the
+// longer sequence's id order[p]
+__global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long long m,
+                                  const int32_t *__restrict__ order, int32_t *__restrict__ cnt) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += (long long)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[order[(int)(c[k] >> 32)]], 1);
 }
 
 // scatter candidates into their row's target groups of 128
 __global__ void cand_groups_kernel(const unsigned long long *__restrict__ c, long long m,
-                                   const int32_t *__restrict__ gbase, int32_t *__restrict__ cursor,
-                                   const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
-                                   int32_t *__restrict__ gmax, int32_t *__restrict__ gmin) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const int i = (int)(c[k] >> 32), j = (int)(c[k] & 0xffffffffull);
-  const int o = atomicAdd(&cursor[i], 1);
-  const int g = gbase[i] + o / kGroupSize;
-  gtgt[(size_t)g * kGroupSize + (o % kGroupSize)] = j;
-  atomicMax(&gmax[g], len[j]);
-  atomicMin(&gmin[g], len[j]);
+                                   const int32_t *__restrict__ order, const int32_t *__restrict__ gbase,
+                                   int32_t *__restrict__ cursor, const int32_t *__restrict__ len,
+                                   int32_t *__restrict__ gtgt, int32_t *__restrict__ gmax,
+                                   int32_t *__restrict__ gmin) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int i = order[(int)(c[k] >> 32)], j = order[(int)(c[k] & 0xffffffffull)];
+    const int o = atomicAdd(&cursor[i], 1);
+    const int g = gbase[i] + o / kGroupSize;
+    gtgt[(size_t)g * kGroupSize + (o % kGroupSize)] = j;
+    atomicMax(&gmax[g], len[j]);
+    atomicMin(&gmin[g], len[j]);
+  }
 }
 
 __global__ void cand_items_kernel(const int32_t *__restrict__ gbase, int n, int4 *__restrict__ items) {
@@ -393,8 +450,9 @@ __global__ void iota_kernel(int32_t *a, int n) {
 }
 
 __global__ void fill_i32_kernel(int32_t *a, long long n, int32_t v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] = v;
 }
 
 unsigned blocks_for(long long n, int tb) { return (unsigned)std::max<long long>((n + tb - 1) / tb, 1); }
@@ -411,7 +469,7 @@ struct Layer {
 
 struct mnhn_encoder {
   Layer L[3];
-  float margin = 0.02f;
+  MarginTable margins{1, {INT32_MAX}, {0.02f}};
   int64_t adam_t = 0;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -581,12 +639,33 @@ void mnhn_encoder_destroy(mnhn_encoder *enc) {
 }
 
 int mnhn_encoder_set_margin(mnhn_encoder *enc, float margin) {
-  if (!enc || !(margin >= 0.f)) return MNHN_ERR_ARG;
-  enc->margin = margin;
+  if (!enc || !(margin >= 0.f) || !std::isfinite(margin)) return MNHN_ERR_ARG;
+  enc->margins.n = 1;
+  enc->margins.upper[0] = INT32_MAX;
+  enc->margins.margin[0] = margin;
   return MNHN_OK;
 }
 
-float mnhn_encoder_margin(const mnhn_encoder *enc) { return enc ? enc->margin : -1.f; }
+int mnhn_encoder_set_margin_table(mnhn_encoder *enc, const int32_t *max_len, const float *margin, int n) {
+  if (!enc || !max_len || !margin || n < 1 || n > kMaxMarginBuckets) return MNHN_ERR_ARG;
+  for (int k = 0; k < n; ++k) {
+    if (!(margin[k] >= 0.f) || !std::isfinite(margin[k])) return MNHN_ERR_ARG;
+    if (k > 0 && max_len[k] <= max_len[k - 1]) return MNHN_ERR_ARG;
+  }
+  enc->margins.n = n;
+  for (int k = 0; k < n; ++k) {
+    enc->margins.upper[k] = max_len[k];
+    enc->margins.margin[k] = margin[k];
+  }
+  return MNHN_OK;
+}
+
+float mnhn_encoder_margin(const mnhn_encoder *enc) {
+  if (!enc) return -1.f;
+  float m = 0.f;
+  for (int k = 0; k < enc->margins.n; ++k) m = std::max(m, enc->margins.margin[k]);
+  return m;
+}
 
 int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, const int32_t *dist,
                        int64_t m, const mnhn_train_params *p, mnhn_train_stats *stats) {
@@ -734,7 +813,11 @@ int mnhn_encoder_embed(mnhn_encoder *enc, mnhn_sw_ctx *c, float *out) {
   return rc;
 }
 
-static const char kEncMagic[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '1'};
+// File layout (little-endian): "MNHNENC2", int32 n, int32 upper[n],
+// float margin[n], then per layer int32 in, int32 out, float Wt[out][in],
+// float b[out].  "MNHNENC1" files hold one float margin in place of the table.
+static const char kEncMagic[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '2'};
+static const char kEncMagicV1[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '1'};
 
 int mnhn_encoder_save(const mnhn_encoder *enc, const char *path) {
   if (!enc || !path) return MNHN_ERR_ARG;
@@ -742,7 +825,9 @@ int mnhn_encoder_save(const mnhn_encoder *enc, const char *path) {
   if (!f) return MNHN_ERR_ARG;
   int rc = MNHN_OK;
   fwrite(kEncMagic, 1, 8, f);
-  fwrite(&enc->margin, 4, 1, f);
+  fwrite(&enc->margins.n, 4, 1, f);
+  fwrite(enc->margins.upper, 4, (size_t)enc->margins.n, f);
+  fwrite(enc->margins.margin, 4, (size_t)enc->margins.n, f);
   for (const Layer &L : enc->L) {
     std::vector<float> w((size_t)L.in * L.out), b(L.out);
     if (hipMemcpy(w.data(), L.wt, w.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -764,9 +849,19 @@ int mnhn_encoder_load(mnhn_encoder *enc, const char *path) {
   FILE *f = fopen(path, "rb");
   if (!f) return MNHN_ERR_ARG;
   char magic[8];
-  float margin = 0;
+  MarginTable mt{1, {INT32_MAX}, {0.f}};
   int rc = MNHN_OK;
-  if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kEncMagic, 8) != 0 || fread(&margin, 4, 1, f) != 1) rc = MNHN_ERR_ARG;
+  if (fread(magic, 1, 8, f) != 8) {
+    rc = MNHN_ERR_ARG;
+  } else if (memcmp(magic, kEncMagicV1, 8) == 0) {
+    if (fread(&mt.margin[0], 4, 1, f) != 1) rc = MNHN_ERR_ARG;
+  } else if (memcmp(magic, kEncMagic, 8) == 0) {
+    if (fread(&mt.n, 4, 1, f) != 1 || mt.n < 1 || mt.n > kMaxMarginBuckets ||
+        fread(mt.upper, 4, (size_t)mt.n, f) != (size_t)mt.n || fread(mt.margin, 4, (size_t)mt.n, f) != (size_t)mt.n)
+      rc = MNHN_ERR_ARG;
+  } else {
+    rc = MNHN_ERR_ARG;
+  }
   for (Layer &L : enc->L) {
     if (rc) break;
     int32_t dims[2];
@@ -777,8 +872,8 @@ int mnhn_encoder_load(mnhn_encoder *enc, const char *path) {
         hipMemcpy(L.b, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
   }
   fclose(f);
+  if (rc == MNHN_OK) rc = mnhn_encoder_set_margin_table(enc, mt.upper, mt.margin, mt.n);
   if (rc == MNHN_OK) {
-    enc->margin = margin;
     rc = refresh(enc);
     if (rc == MNHN_OK && hipStreamSynchronize(enc->stream) != hipSuccess) rc = MNHN_ERR_DEVICE;
   }
@@ -809,43 +904,78 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     (void)hipEventRecord(e0, st);
     return (double)ms;
   };
-  bf16 *X = nullptr, *E16 = nullptr;
-  float *norm = nullptr;
+  bf16 *X = nullptr, *E16 = nullptr, *Es = nullptr;
+  float *norm = nullptr, *srad = nullptr;
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
-          *gmin = nullptr, *qids = nullptr;
+          *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr;
   int4 *items = nullptr;
   auto cleanup = [&]() {
-    dev_free(X); dev_free(E16); dev_free(norm); dev_free(cand); dev_free(d_cnt64); dev_free(rowcnt);
-    dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax); dev_free(gmin); dev_free(qids);
-    dev_free(items);
+    dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
+    dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
+    dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   };
+  auto fail = [&](int code) {
+    cleanup();
+    mnhn_csr_free(out);
+    return code;
+  };
+  if (eps_max < 0) {  // no pair has a negative distance
+    cleanup();
+    if (stats) *stats = stt;
+    return MNHN_OK;
+  }
   int rc = MNHN_OK;
   (void)hipEventRecord(e0, st);
-  if ((rc = profiles_dev(c, &X, nullptr))) { cleanup(); mnhn_csr_free(out); return rc; }
+  if ((rc = profiles_dev(c, &X, nullptr))) return fail(rc);
   stt.profile_ms = phase_ms();
-  if ((rc = dev_alloc(&E16, (size_t)n * D3)) || (rc = dev_alloc(&norm, (size_t)n)) ||
-      (rc = embed_dev(enc, c, X, E16, nullptr))) {
-    cleanup(); mnhn_csr_free(out); return rc;
-  }
+  if ((rc = dev_alloc(&E16, (size_t)n * D3)) || (rc = embed_dev(enc, c, X, E16, nullptr))) return fail(rc);
   dev_free(X);
-  hipLaunchKernelGGL(row_norm_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, E16, n, norm);
+  // length-ordered copies for the pairwise stage
+  if ((rc = dev_alloc(&Es, (size_t)n * D3)) || (rc = dev_alloc(&norm, (size_t)n)) ||
+      (rc = dev_alloc(&srad, (size_t)n)) || (rc = dev_alloc(&lens, (size_t)n)) ||
+      (rc = dev_alloc(&d_order, (size_t)n)))
+    return fail(rc);
+  if (hipMemcpyAsync(d_order, c->order.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    return fail(MNHN_ERR_DEVICE);
+  const int max_dl = eps_max / c->match;
+  hipLaunchKernelGGL(sorted_rows_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, st, E16, d_order, c->d_len, n,
+                     (float)eps_max / (float)c->match, enc->margins, Es, norm, srad, lens);
+  if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
+  dev_free(E16);
   stt.embed_ms = phase_ms();
-  // ---- proposals
+  // ---- proposals over the length band: row tile ti reaches column tiles
+  // ti .. tj_max(ti), the last whose longest length is >= shortest(ti) - max_dl
   const int tiles = (n + 127) / 128;
-  const long long nblk = (long long)tiles * (tiles + 1) / 2;
-  if (nblk > 0x7fffffffLL) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_RANGE; }
+  std::vector<int32_t> toff((size_t)tiles + 1, 0);
+  {
+    long long acc = 0;
+    int tj = 0;
+    for (int ti = 0; ti < tiles; ++ti) {
+      const long long shortest = c->len[c->order[std::min((ti + 1) * 128, n) - 1]];
+      tj = std::max(tj, ti);
+      while (tj + 1 < tiles && (long long)c->len[c->order[(tj + 1) * 128]] >= shortest - max_dl) ++tj;
+      toff[ti] = (int32_t)acc;
+      acc += tj - ti + 1;
+      if (acc > 0x7fffffffLL) return fail(MNHN_ERR_RANGE);
+    }
+    toff[tiles] = (int32_t)acc;
+  }
+  const unsigned nblk = (unsigned)toff[tiles];
+  stt.tiles = nblk;
+  if ((rc = dev_alloc(&d_toff, (size_t)tiles + 1)) || (rc = dev_alloc(&d_cnt64, 1))) return fail(rc);
+  if (hipMemcpyAsync(d_toff, toff.data(), ((size_t)tiles + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    return fail(MNHN_ERR_DEVICE);
   unsigned long long cap = std::max<unsigned long long>(1ull << 20, 64ull * (unsigned long long)n), m = 0;
-  if ((rc = dev_alloc(&d_cnt64, 1))) { cleanup(); mnhn_csr_free(out); return rc; }
   for (int attempt = 0; attempt < 2; ++attempt) {
     dev_free(cand);
     if ((rc = dev_alloc(&cand, cap))) break;
     hipError_t e = hipMemsetAsync(d_cnt64, 0, 8, st);
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(pair_candidates_kernel, dim3((unsigned)nblk), dim3(256), 0, st, E16, norm, c->d_len, n,
-                         tiles, (float)eps_max, (float)c->match, enc->margin, cand, d_cnt64, cap);
+      hipLaunchKernelGGL(pair_candidates_kernel, dim3(nblk), dim3(256), 0, st, Es, norm, srad, lens, n, d_toff,
+                         tiles, max_dl, cand, d_cnt64, cap);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&m, d_cnt64, 8, hipMemcpyDeviceToHost, st);
@@ -854,7 +984,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (m <= cap) break;
     cap = m;  // overflowed: exact size, recompute once
   }
-  if (rc || m > cap) { cleanup(); mnhn_csr_free(out); return rc ? rc : MNHN_ERR_NOMEM; }
+  if (rc || m > cap) return fail(rc ? rc : MNHN_ERR_NOMEM);
+  dev_free(Es); dev_free(norm); dev_free(srad); dev_free(lens);
   stt.candidates = (int64_t)m;
   stt.propose_ms = phase_ms();
   // ---- exact verification: candidates grouped per row into targets of 128
@@ -866,7 +997,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       cleanup(); mnhn_csr_free(out); return rc;
     }
     hipError_t e = hipMemsetAsync(rowcnt, 0, (size_t)n * 4, st);
-    hipLaunchKernelGGL(cand_count_kernel, dim3(blocks_for((long long)m, 256)), dim3(256), 0, st, cand, (long long)m, rowcnt);
+    hipLaunchKernelGGL(cand_count_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand, (long long)m,
+                       d_order, rowcnt);
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
@@ -879,11 +1011,11 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, (size_t)n * 4, st);
     if (e == hipSuccess) e = hipMemsetAsync(gmax, 0, (size_t)G * 4, st);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(G, 256)), dim3(256), 0, st, gmin, (long long)G, 0x7fffffff);
-    hipLaunchKernelGGL(cand_groups_kernel, dim3(blocks_for((long long)m, 256)), dim3(256), 0, st, cand,
-                       (long long)m, gbase, cursor, c->d_len, gtgt, gmax, gmin);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks(G, 256)), dim3(256), 0, st, gmin, (long long)G, 0x7fffffff);
+    hipLaunchKernelGGL(cand_groups_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand,
+                       (long long)m, d_order, gbase, cursor, c->d_len, gtgt, gmax, gmin);
     hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, n, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
@@ -900,12 +1032,20 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     w.dev_kernel = KERNEL_LDS;
     w.max_q_len = c->maxlen;
     w.pairs = (double)m;
-    const unsigned long long kcap = 2ull * m;
-    if ((rc = ensure_keys(c, kcap))) { cleanup(); mnhn_csr_free(out); return rc; }
-    if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
-    rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, kcap);
-    if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
-    if (rc) { cleanup(); mnhn_csr_free(out); return rc; }
+    // key buffer: both directions of every neighbour; start from at most 2^28
+    // keys and, if the neighbours need more, grow to the exact count and verify
+    // once more
+    unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if ((rc = ensure_keys(c, kcap))) return fail(rc);
+      if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
+      rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
+      if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
+      if (rc) return fail(rc);
+      if (nk <= c->keys_cap) break;
+      kcap = nk;
+    }
+    if (nk > c->keys_cap) return fail(MNHN_ERR_NOMEM);
   }
   stt.verify_ms = phase_ms();
   stt.neighbours = (int64_t)(nk / 2);

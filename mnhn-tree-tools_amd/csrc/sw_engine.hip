@@ -175,8 +175,7 @@ __device__ __forceinline__ void finish_query_int(const KArgs &a, int lane, int q
   if (total == 0) return;
   unsigned long long base = 0;
   if (lane == 0) base = atomicAdd(a.nkeys, (unsigned long long)total);
-  base = __builtin_amdgcn_readfirstlane((unsigned)base) |
-         ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
+  base = uniform_u64(base);
   const unsigned mlo = __builtin_amdgcn_mbcnt_hi((unsigned)(b_lo >> 32),
                         __builtin_amdgcn_mbcnt_lo((unsigned)b_lo, 0u));
   const unsigned mhi = __builtin_amdgcn_mbcnt_hi((unsigned)(b_hi >> 32),
@@ -232,8 +231,8 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
     const int qe = __builtin_amdgcn_readfirstlane(it.z);
     const int ob = __builtin_amdgcn_readfirstlane(it.w);
 
-    const int t_lo = a.gtgt[g * kGroup + lane];
-    const int t_hi = a.gtgt[g * kGroup + kWave + lane];
+    const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
+    const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
     const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
     const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
     const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
@@ -397,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
 
     const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
     const int gmn = __builtin_amdgcn_readfirstlane(a.gmin[g]);
-    const int t_lo = a.gtgt[g * kGroup + lane];
-    const int t_hi = a.gtgt[g * kGroup + kWave + lane];
+    const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
+    const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
     const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
     const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
     // empty slots are treated as full length (their results are discarded)
@@ -583,8 +582,8 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
 
     const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
     const int gmn = __builtin_amdgcn_readfirstlane(a.gmin[g]);
-    const int t_lo = a.gtgt[g * kGroup + lane];
-    const int t_hi = a.gtgt[g * kGroup + kWave + lane];
+    const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
+    const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
     const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
     const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
     const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
@@ -695,21 +694,22 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
 // ---- epsilon-list compaction: keys -> CSR rows in ascending column order ----
 
 __global__ void count_rows_kernel(const unsigned long long *keys, long long n, int32_t *cnt) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  atomicAdd(&cnt[(int)(keys[k] >> 40)], 1);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[(int)(keys[k] >> 40)], 1);
 }
 
 __global__ void scatter_rows_kernel(const unsigned long long *keys, long long n,
                                     const long long *row_ptr, int32_t *cursor,
                                     int32_t *seg_col, int16_t *seg_d) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const unsigned long long key = keys[k];
-  const int row = (int)(key >> 40);
-  const long long pos = row_ptr[row] + atomicAdd(&cursor[row], 1);
-  seg_col[pos] = (int32_t)((key >> 16) & 0xffffffull);
-  seg_d[pos] = (int16_t)(key & 0xffffull);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = keys[k];
+    const int row = (int)(key >> 40);
+    const long long pos = row_ptr[row] + atomicAdd(&cursor[row], 1);
+    seg_col[pos] = (int32_t)((key >> 16) & 0xffffffull);
+    seg_d[pos] = (int16_t)(key & 0xffffull);
+  }
 }
 
 constexpr int kSortThreads = 256;
@@ -1104,7 +1104,7 @@ const char *mnhn_sw_strerror(int code) {
     case MNHN_ERR_DEVICE: return "HIP runtime error";
     case MNHN_ERR_NOMEM: return "out of memory";
     case MNHN_ERR_NODEVICE: return "no usable GPU";
-    case MNHN_ERR_RANGE: return "score range exceeds the 16-bit lane format";
+    case MNHN_ERR_RANGE: return "score range exceeds the 16-bit lane format, or work exceeds 32-bit offsets";
     default: return "unknown error";
   }
 }
@@ -1333,6 +1333,7 @@ int mnhn_sw_pairs(mnhn_sw_ctx *c, const int32_t *ij, int64_t m, int32_t *dist_ou
     }
     k = e;
   }
+  if (w.gmax.size() * (size_t)kGroup > (size_t)INT32_MAX) return MNHN_ERR_RANGE;  // 32-bit output offsets
   for (int64_t k = 0; k < m; ++k) w.cells += (double)c->len[ij[2 * k]] * c->len[ij[2 * k + 1]];
   w.pairs = (double)m;
   finalize_items(c, w, S, false);
@@ -1362,6 +1363,7 @@ int mnhn_sw_matrix(mnhn_sw_ctx *c, int32_t *dist_out) {
   finalize_items(c, w, S, true);
   size_t out_n = 0;
   for (const auto &it : w.items) out_n += (size_t)(it.z - it.y) * kGroup;
+  if (out_n > (size_t)INT32_MAX) return MNHN_ERR_RANGE;  // item output offsets are 32-bit
   int32_t *d_out = nullptr;
   if ((rc = dev_alloc(&d_out, out_n))) return rc;
   rc = launch_tiles(c, w, MODE_DIST, 1, 0, d_out, nullptr, 0);
@@ -1495,7 +1497,7 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
     cleanup(); mnhn_csr_free(out); return rc;
   }
   const int tb = 256;
-  const unsigned kblocks = (unsigned)std::max<long long>(((long long)nk + tb - 1) / tb, 1);
+  const unsigned kblocks = stride_blocks((long long)nk, tb);
   std::vector<int32_t> cnt(n);
   (void)hipEventRecord(c->ev[2], c->stream);
   hipError_t e = hipMemsetAsync(d_cnt, 0, (size_t)n * sizeof(int32_t), c->stream);

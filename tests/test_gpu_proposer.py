@@ -140,3 +140,72 @@ def test_prefilter_trained_recall(pkg, oracle):
     recall = len(got) / max(len(want), 1)
     _say(f"recall {recall:.5f}, candidates {st['candidates']}, neighbours {st['neighbours']}")
     assert recall > 0.99
+
+
+def test_proposal_rule_and_margin_table_match_numpy(pkg, tmp_path):
+    """Candidate count and verified lists against a numpy evaluation of the
+    proposal rule (length bound, radius eps / (4 l_long) + margin(l_long) with
+    a three-bucket margin table) on bf16-rounded embeddings; pairs within
+    floating-point rounding of the radius are left out of the comparison."""
+    seqs = synth.config2(seed=46, n_roots=8, n_children=4, n_leaves=12)
+    eps = 300
+    n = len(seqs)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    _say(f"proposal rule n={n}")
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=9) as enc:
+        e = _bf16(enc.embed(eng)).astype(np.float64)
+        d2 = ((e[:, None, :] - e[None, :, :]) ** 2).sum(-1)
+        iu = np.triu_indices(n, 1)
+        base = np.sqrt(np.quantile(d2[iu], [0.2, 0.35, 0.5]))
+        upper = np.array([250, 400, 2**31 - 1])
+        margins = base.astype(np.float32)
+        enc.set_margin_table(upper, margins)
+        assert abs(enc.margin - margins.max()) < 1e-7
+        rp, col, dist, st = enc.prefilter_csr(eng, eps)
+        erp, ecol, edist = eng.eps_csr(eps)
+        enc.save(str(tmp_path / "enc.bin"))
+        with pkg.Encoder(seed=1) as enc2:
+            enc2.load(str(tmp_path / "enc.bin"))
+            rp2, col2, dist2, st2 = enc2.prefilter_csr(eng, eps)
+    (tu, tm), _ = pkg.load_encoder_weights(str(tmp_path / "enc.bin"))
+    assert np.array_equal(tu, upper.astype(np.int32)) and np.array_equal(tm, margins)
+    assert st2["candidates"] == st["candidates"] and np.array_equal(col2, col)
+    lmax = np.maximum(L[:, None], L[None, :])
+    marg = margins[np.searchsorted(upper, lmax)]
+    rad = eps / (4.0 * lmax) + marg
+    nrm = (e * e).sum(1)
+    tol = 1e-4 * (nrm[:, None] + nrm[None, :]) + 1e-9
+    prop = (4 * np.abs(L[:, None] - L[None, :]) <= eps) & (d2 <= rad ** 2)
+    border = np.abs(d2 - rad ** 2) <= tol
+    prop_u, border_u = prop[iu], border[iu]
+    _say(f"proposed {int(prop_u.sum())} of {len(prop_u)}, border {int(border_u.sum())}, "
+         f"kernel {st['candidates']}")
+    assert 0 < prop_u.sum() < len(prop_u)
+    assert abs(int(st["candidates"]) - int(prop_u.sum())) <= int(border_u.sum())
+    # verified lists: exactly the exact neighbours that the rule proposes
+    exact = np.zeros((n, n), bool)
+    exact[np.repeat(np.arange(n), np.diff(erp)), ecol] = True
+    got = np.zeros((n, n), bool)
+    got[np.repeat(np.arange(n), np.diff(rp)), col] = True
+    want = exact & (prop | prop.T)
+    sure = ~(border | border.T)
+    assert np.array_equal(got & sure, want & sure)
+    assert not np.any(got & ~exact)
+
+
+def test_prefilter_positions_beyond_2e31(pkg):
+    """80,000 amplicon-like sequences with an unbounded radius propose 2.35e9
+    pairs, so candidate positions pass 2^31 (wave-uniform 64-bit offsets);
+    the verified lists must still equal the exact all-pairs lists."""
+    full = synth.config3(seed=44)
+    pick = np.sort(np.random.default_rng(44).choice(len(full), 80000, replace=False))
+    seqs = [full[k] for k in pick]
+    eps = 300
+    _say("prefilter beyond 2^31 candidates, n=80000")
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=3) as enc:
+        enc.margin = 1e4
+        rp, col, dist, st = enc.prefilter_csr(eng, eps)
+        _say(f"candidates {st['candidates']}, neighbours {st['neighbours']}")
+        erp, ecol, edist = eng.eps_csr(eps)
+    assert st["candidates"] > 2**31
+    assert np.array_equal(rp, erp) and np.array_equal(col, ecol) and np.array_equal(dist, edist)

@@ -169,40 +169,80 @@ def sample_training_pairs(rng, group_levels, n, m):
     return ij[rng.permutation(len(ij))]
 
 
-def calibrate_margins(emb16, L, i, j, eps, q=0.9999, buckets=8, slack=0.005):
-    """Margin table from neighbour pairs (i, j) of held-out sequences: the
-    pairs are split into length buckets of equal size by their longer length,
-    and each bucket's margin is the q-quantile of its radius excess (its
-    maximum when the bucket is too small for that quantile) plus a slack."""
+def profile_cosine(P, i, j, chunk=1 << 18):
+    """Cosine of the 5-mer profiles of pairs (i, j) (rows of P)."""
+    out = np.empty(len(i), np.float32)
+    nrm = np.sqrt((P.astype(np.float64) ** 2).sum(1))
+    for s in range(0, len(i), chunk):
+        a, b = i[s:s + chunk], j[s:s + chunk]
+        out[s:s + chunk] = (P[a] * P[b]).sum(1) / np.maximum(nrm[a] * nrm[b], 1e-30)
+    return out
+
+
+def bucket_edges(lmax, buckets=8):
+    qs = np.unique(np.quantile(lmax, np.linspace(0, 1, buckets + 1)[1:-1]).astype(np.int64))
+    return np.concatenate([qs, [2**31 - 1]]).astype(np.int64)
+
+
+def per_bucket(values, lmax, upper, q, high):
+    """Per length bucket: the q-quantile of values (high=True) or the
+    (1-q)-quantile (high=False); the extreme when the bucket is too small."""
+    out, lo = [], -1
+    allq = float(np.quantile(values, q if high else 1 - q))
+    for u in upper:
+        v = values[(lmax > lo) & (lmax <= u)]
+        if len(v) == 0:
+            out.append(allq)
+        elif len(v) * (1 - q) < 10:
+            out.append(float(v.max() if high else v.min()))
+        else:
+            out.append(float(np.quantile(v, q if high else 1 - q)))
+        lo = u
+    return out
+
+
+def calibrate_tables(emb16, P, L, i, j, eps, recall=0.9999, slack=0.005, exact_below=0):
+    """Margin and cosine tables from neighbour pairs (i, j) of held-out
+    sequences, per length bucket of the longer sequence (buckets of equal pair
+    counts).  Each test keeps all but (1 - recall) / 2 of the pairs of its
+    bucket, so the pair passes both with probability >= recall.  Pairs whose
+    longer sequence is at most exact_below long get their own buckets with
+    both tests off: every pair of them that the length bound allows goes to
+    the exact kernel."""
+    q = 1 - (1 - recall) / 2
+    if len(i) == 0:
+        one = np.array([2**31 - 1], np.int32)
+        return one, np.array([slack], np.float32), one, np.array([-1.0], np.float32), {}
     ex = radius_excess(emb16, L, i, j, eps)
     lmax = np.maximum(L[i], L[j])
-    if len(ex) == 0:
-        return np.array([2**31 - 1], np.int32), np.array([slack], np.float32), {}
-    qs = np.unique(np.quantile(lmax, np.linspace(0, 1, buckets + 1)[1:-1]).astype(np.int64))
-    upper = np.concatenate([qs, [2**31 - 1]]).astype(np.int64)
-    margins, lo = [], -1
-    for u in upper:
-        sel = (lmax > lo) & (lmax <= u)
-        e = ex[sel]
-        if len(e) == 0:
-            m = float(np.quantile(ex, q))
-        elif len(e) * (1 - q) < 10:
-            m = float(e.max())
-        else:
-            m = float(np.quantile(e, q))
-        margins.append(max(m, 0.0) + slack)
-        lo = u
+    upper = bucket_edges(lmax)
+    if exact_below > 0:
+        upper = np.unique(np.concatenate([[exact_below], upper[upper > exact_below]])).astype(np.int64)
+    fallback = upper <= exact_below
+    margins = [1e3 if fb else max(m, 0.0) + slack
+               for m, fb in zip(per_bucket(ex, lmax, upper, q, True), fallback)]
     info = {"pairs": int(len(ex)), "excess_q50": float(np.quantile(ex, 0.5)),
             "excess_q999": float(np.quantile(ex, 0.999)), "excess_max": float(ex.max()),
             "max_len": upper.tolist(), "margins": margins}
-    return upper.astype(np.int32), np.array(margins, np.float32), info
+    if P is None:
+        cmin = [-1.0] * len(upper)
+    else:
+        cs = profile_cosine(P, i, j)
+        cmin = [-1.0 if fb else max(c - slack, -1.0)
+                for c, fb in zip(per_bucket(cs, lmax, upper, q, False), fallback)]
+        info.update({"cos_q001": float(np.quantile(cs, 0.001)), "cos_min": float(cs.min()),
+                     "cos_table": cmin})
+    return (upper.astype(np.int32), np.array(margins, np.float32), upper.astype(np.int32),
+            np.array(cmin, np.float32), info)
 
 
-def train_encoder(pkg, seqs, group_levels, holdout_groups, n_pairs, eps, seed, epochs, log):
+def train_encoder(pkg, seqs, group_levels, holdout_groups, n_pairs, eps, seed, epochs, log,
+                  use_cosine=True, recall=0.9999, exact_below=0):
     """Label n_pairs pairs of seqs with the exact kernel and train an encoder on
-    the pairs between sequences outside a held-out 10% of `holdout_groups`;
-    the margin table comes from neighbour pairs (d <= eps) inside the held-out
-    groups, which the encoder never saw."""
+    the pairs between sequences outside a held-out 10% of `holdout_groups`.
+    The margin and cosine tables come from every neighbour pair (d <= eps,
+    exact all-pairs lists) among the held-out sequences, which the encoder
+    never saw."""
     rng = np.random.default_rng(seed)
     n = len(seqs)
     L = np.array([len(x) for x in seqs], dtype=np.int64)
@@ -219,20 +259,34 @@ def train_encoder(pkg, seqs, group_levels, holdout_groups, n_pairs, eps, seed, e
         f"held-out pairs {int(cal.sum())}; neighbours (d <= {eps}): {int((d <= eps).sum())}")
     enc = pkg.Encoder(seed=seed)
     st = enc.train(eng, ij[tr], d[tr], epochs=epochs, batch=1024, lr=2e-3, eps_train=eps,
-                   neighbor_weight=4.0, seed=seed)
+                   neighbor_weight=4.0, seed=seed, weight_decay=0.05, lr_decay=True,
+                   far_weight=1.0, far_gap=0.3)
     log(f"trained: loss {st['first_loss']:.4f} -> {st['final_loss']:.4f}, {st['steps']} steps, "
         f"{st['seconds']:.1f} s")
     emb16 = _bf16_round(enc.embed(eng))
-    hv, dv = ij[cal], d[cal]
-    nb = dv <= eps
-    upper, margins, cinfo = calibrate_margins(emb16, L, hv[nb, 0], hv[nb, 1], eps)
+    P = eng.kmer_profiles() if use_cosine else None
+    # calibration set: every neighbour pair (exact all-pairs lists) among the
+    # held-out sequences, at most 60,000 of them
+    hidx = np.nonzero(held)[0]
+    if len(hidx) > 60000:
+        hidx = np.sort(rng.choice(hidx, 60000, replace=False))
+    with pkg.Engine([seqs[k] for k in hidx]) as heng:
+        hrp, hcol, _ = heng.eps_csr(eps)
+    hrows = np.repeat(np.arange(len(hidx)), np.diff(hrp))
+    up = hrows < hcol
+    ci, cj = hidx[hrows[up]], hidx[hcol[up]]
+    del hrp, hcol, hrows, up
+    upper, margins, cupper, cmin, cinfo = calibrate_tables(emb16, P, L, ci, cj, eps, recall=recall,
+                                                           exact_below=exact_below)
+    del P
     enc.set_margin_table(upper, margins)
-    log(f"margin table from {int(nb.sum())} held-out neighbour pairs: {cinfo}")
+    enc.set_cosine_table(cupper, cmin)
+    log(f"tables from {len(ci)} neighbour pairs of {len(hidx)} held-out sequences: {cinfo}")
     eng.close()
     info = {"labelled_pairs": int(len(ij)), "train_pairs": int(tr.sum()), "label_s": label_s,
             "train_s": st["seconds"], "epochs": epochs, "steps": st["steps"],
             "loss_first": st["first_loss"], "loss_final": st["final_loss"],
-            "calibration": cinfo}
+            "calibration": cinfo, "calibration_sequences": int(len(hidx))}
     return enc, info
 
 
@@ -241,24 +295,27 @@ def same_partition(a, b):
 
 
 def pair_set_recall(rp_p, col_p, dist_p, rp_e, col_e, dist_e, sample_rows=20000, seed=0):
-    """Recall of the prefilter lists against the exact lists, and whether every
-    prefilter entry (row, col, d) is an exact entry, checked on a random sample
-    of rows (all rows when there are fewer).  Columns ascend within each row in
-    both lists."""
+    """Recall of the prefilter lists against the exact lists, whether every
+    prefilter entry (row, col, d) is an exact entry, and the exact entries the
+    prefilter missed, checked on a random sample of rows (all rows when there
+    are fewer).  Columns ascend within each row in both lists."""
     n = len(rp_e) - 1
     rows = np.arange(n) if n <= sample_rows else \
         np.random.default_rng(seed).choice(n, sample_rows, replace=False)
     subset = True
+    missed = []
     for r in rows:
         a, b = slice(rp_p[r], rp_p[r + 1]), slice(rp_e[r], rp_e[r + 1])
-        ce, de = col_e[b], dist_e[b]
-        pos = np.searchsorted(ce, col_p[a])
+        ce, de, cp = col_e[b], dist_e[b], col_p[a]
+        pos = np.searchsorted(ce, cp)
         ok = (pos < len(ce))
-        ok &= ce[np.minimum(pos, len(ce) - 1)] == col_p[a] if len(ce) else False
+        ok &= ce[np.minimum(pos, len(ce) - 1)] == cp if len(ce) else False
         if not np.all(ok) or not np.array_equal(de[pos], dist_p[a]):
             subset = False
-            break
-    return len(col_p) / max(len(col_e), 1), subset
+        if len(cp) < len(ce):
+            for c in np.setdiff1d(ce, cp):
+                missed.append((int(r), int(c)))
+    return len(col_p) / max(len(col_e), 1), subset, missed
 
 
 def run_proposer_bench(args, pkg, log):
@@ -268,18 +325,20 @@ def run_proposer_bench(args, pkg, log):
         # train on the amplicon-scale generator (its lengths, mutation rates and
         # noise) drawn with 2,000 ancestors under a new seed, evaluate on
         # 100,000 sequences of another draw
-        train = synth.config3(seed=4, n_anc=args.train_anc)
-        groups = synth.config3_groups(n_anc=args.train_anc)
+        train = synth.config3(seed=4, n_anc=args.train_anc, per_anc=args.train_per)
+        groups = synth.config3_groups(n_anc=args.train_anc, per_anc=args.train_per)
         enc, info = train_encoder(pkg, train, [(groups, 0.5)], groups, args.pairs, eps, 4,
-                                  args.epochs, log)
+                                  args.epochs, log, use_cosine=not args.no_cosine,
+                                  recall=args.recall_target or 0.9999,
+                                  exact_below=max(args.exact_below, 0))
         del train
         full = synth.config3(seed=44)
         pick = np.sort(np.random.default_rng(44).choice(len(full), args.n or 100_000, replace=False))
         seqs = [full[k] for k in pick]
         workload = (f"learned proposer: encoder trained on {args.pairs} exact-labelled pairs "
                     f"(50% within-ancestor, 50% random) of the amplicon-scale generator drawn "
-                    f"with {args.train_anc} ancestors (seed 4), evaluated on 100,000 sequences "
-                    f"of another draw (seed 44)")
+                    f"with {args.train_anc} ancestors x {args.train_per} (seed 4), evaluated on "
+                    f"{len(seqs)} sequences of another draw (seed 44)")
     else:
         # training draws of the same generator under other seeds
         parts, labels = [], []
@@ -293,7 +352,9 @@ def run_proposer_bench(args, pkg, log):
         child = np.concatenate([np.where(g < 2000, g // 5, g) + k * big for g, k in labels])
         root = np.concatenate([np.where(g < 2000, g // 40, g) + k * big for g, k in labels])
         enc, info = train_encoder(pkg, parts, [(leaf, 0.4), (child, 0.15), (root, 0.1)], root,
-                                  args.pairs, eps, 5, args.epochs, log)
+                                  args.pairs, eps, 5, args.epochs, log, use_cosine=not args.no_cosine,
+                                  recall=args.recall_target or 0.999999,
+                                  exact_below=args.exact_below if args.exact_below >= 0 else 100)
         del parts
         seqs = synth.config5(seed=5, n=args.n or 500_000)
         workload = ("end-to-end learned-prefilter DBSCAN: heavy-tailed lengths 80-1500 "
@@ -313,15 +374,37 @@ def run_proposer_bench(args, pkg, log):
     et = eng.last_timing()
     log(f"exact eps lists: {exact_lists_s:.2f} s (kernel {et['kernel_ms'] / 1e3:.2f} s, "
         f"{et['pairs']:.3g} pairs, {et['cells']:.3g} cells), nnz {len(ecol)}")
+    unpruned = None
+    if args.exact_unpruned:
+        # the same lists from every unique pair (no length-bound pruning)
+        os.environ["MNHN_SW_LENGTH_PRUNE"] = "0"
+        try:
+            t0 = time.perf_counter()
+            urp, ucol, udist = eng.eps_csr(eps)
+            unpruned = {"lists_s": time.perf_counter() - t0,
+                        "kernel_s": eng.last_timing()["kernel_ms"] / 1e3,
+                        "pairs": eng.last_timing()["pairs"],
+                        "identical_lists": bool(np.array_equal(urp, erp) and np.array_equal(ucol, ecol)
+                                                and np.array_equal(udist, edist))}
+            del urp, ucol, udist
+        finally:
+            del os.environ["MNHN_SW_LENGTH_PRUNE"]
+        log(f"exact all-pairs lists (no length pruning): {unpruned}")
     out = {}
     # learned prefilter + exact verification
     t0 = time.perf_counter()
     prp, pcol, pdist, pst = enc.prefilter_csr(eng, eps)
     pref_lists_s = time.perf_counter() - t0
     log(f"prefilter lists: {pref_lists_s:.2f} s {pst}, nnz {len(pcol)}")
-    recall, subset = pair_set_recall(prp, pcol, pdist, erp, ecol, edist)
+    recall, subset, missed = pair_set_recall(prp, pcol, pdist, erp, ecol, edist)
     precision = pst["neighbours"] / max(pst["candidates"], 1)
     log(f"recall {recall:.6f} (proposed subset of exact: {subset}), precision {precision:.4f}")
+    if missed:
+        mi = np.array(missed)
+        lm = np.maximum(L[mi[:, 0]], L[mi[:, 1]]).astype(np.int64)
+        log(f"missed pairs in sampled rows: {len(mi)}; longer length quantiles "
+            f"{np.quantile(lm, [0, 0.25, 0.5, 0.75, 1]).tolist()}; first "
+            f"{[(int(a), int(b), int(L[a]), int(L[b])) for a, b in mi[:8]]}")
     all_pairs = n * (n - 1) / 2
     # the pairwise kernel's algorithmic work: 128 x 128 x 128 multiply-adds per
     # tile of the length band
@@ -351,6 +434,10 @@ def run_proposer_bench(args, pkg, log):
     log(f"DBSCAN(eps {eps}, minpts {minpts}): exact {exact_s:.2f} s ({len(ecl)} clusters), "
         f"prefilter {pref_s:.2f} s ({len(pcl)} clusters); same partition: {same}, "
         f"same member order: {ecl == pcl}")
+    if unpruned:
+        unpruned["dbscan_s"] = unpruned["lists_s"] + exact_db_s
+        unpruned["speedup_of_prefilter"] = unpruned["dbscan_s"] / pref_s
+        res["exact_all_pairs"] = unpruned
     res.update({"minpts": minpts, "exact_dbscan_s": exact_s, "prefilter_dbscan_s": pref_s,
                 "speedup": exact_s / pref_s, "host_dbscan_s": [exact_db_s, pref_db_s],
                 "clusters_exact": len(ecl), "clusters_prefilter": len(pcl),
@@ -392,10 +479,20 @@ def main():
     ap.add_argument("--eps", type=int, default=0,
                     help="c4/c5 epsilon (default 300 for c4, 200 for c5)")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="c4/c5 labelled training pairs")
-    ap.add_argument("--epochs", type=int, default=8, help="c4/c5 training epochs")
+    ap.add_argument("--epochs", type=int, default=3, help="c4/c5 training epochs")
     ap.add_argument("--train-n", type=int, default=100_000, help="c5 sequences per training draw")
     ap.add_argument("--train-draws", type=int, default=4, help="c5 training draws")
-    ap.add_argument("--train-anc", type=int, default=2000, help="c4 training ancestors")
+    ap.add_argument("--train-anc", type=int, default=20000, help="c4 training ancestors")
+    ap.add_argument("--train-per", type=int, default=20, help="c4 training descendants per ancestor")
+    ap.add_argument("--no-cosine", action="store_true", help="c4/c5: embedding radius test only")
+    ap.add_argument("--exact-unpruned", action="store_true",
+                    help="c4/c5: also time the exact lists over every unique pair (no length pruning)")
+    ap.add_argument("--recall-target", type=float, default=0,
+                    help="c4/c5: calibrated per-bucket recall of the proposal tests "
+                         "(default 0.9999 for c4, 0.999999 for c5)")
+    ap.add_argument("--exact-below", type=int, default=-1,
+                    help="c4/c5: pairs whose longer sequence is at most this long skip the "
+                         "proposal tests (default 0 for c4, 100 for c5)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (torch) 3 times, "
                          "to calibrate FETCH_SIZE / WRITE_SIZE counters")

@@ -36,6 +36,10 @@ typedef struct {
   int32_t eps_train;     /* distance below which a pair counts as a neighbour */
   float neighbor_weight; /* weight of the hinge keeping neighbours inside their radius */
   uint64_t seed;         /* pair shuffling */
+  float weight_decay;    /* decoupled (AdamW) decay of the weights, 0 = plain Adam */
+  int32_t lr_decay;      /* nonzero: rate decays linearly to 10% over the run */
+  float far_weight;      /* weight of the hinge keeping non-neighbours beyond radius + far_gap */
+  float far_gap;
 } mnhn_train_params;
 
 typedef struct {
@@ -79,11 +83,20 @@ int mnhn_encoder_set_margin(mnhn_encoder *enc, float margin);
 int mnhn_encoder_set_margin_table(mnhn_encoder *enc, const int32_t *max_len, const float *margin,
                                   int n);
 
+/* Second proposal test, per length bucket of the pair's longer sequence (same
+ * bucket rule as the margin table): the cosine of the two sequences' raw 5-mer
+ * count vectors must be >= cos_min[b]; cos_min[b] < 0 switches the test off
+ * for that bucket (the default: off everywhere).  Computed on the same
+ * 128 x 128 MFMA tiles as the embedding distance. */
+int mnhn_encoder_set_cosine_table(mnhn_encoder *enc, const int32_t *max_len, const float *cos_min,
+                                  int n);
+
 /* Largest margin of the table. */
 float mnhn_encoder_margin(const mnhn_encoder *enc);
 
-/* Save / load the weights and the margin table (little-endian fp32 blob with
- * a header; files with a single margin load as a one-bucket table). */
+/* Save / load the weights, the margin table and the cosine table
+ * (little-endian fp32 blob with a header; older files without a cosine table
+ * load with the test off, files with a single margin as a one-bucket table). */
 int mnhn_encoder_save(const mnhn_encoder *enc, const char *path);
 int mnhn_encoder_load(mnhn_encoder *enc, const char *path);
 

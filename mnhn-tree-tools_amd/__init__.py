@@ -36,7 +36,7 @@ ABI_SYMBOLS = [
 PROPOSER_SYMBOLS = [
     "mnhn_kmer_profiles", "mnhn_encoder_create", "mnhn_encoder_destroy",
     "mnhn_encoder_train", "mnhn_encoder_embed", "mnhn_encoder_set_margin",
-    "mnhn_encoder_set_margin_table", "mnhn_encoder_margin", "mnhn_encoder_save", "mnhn_encoder_load",
+    "mnhn_encoder_set_margin_table", "mnhn_encoder_set_cosine_table", "mnhn_encoder_margin", "mnhn_encoder_save", "mnhn_encoder_load",
     "mnhn_prefilter_eps_csr",
 ]
 HOST_SYMBOLS = [
@@ -83,7 +83,9 @@ class Tuning(ctypes.Structure):
 class TrainParams(ctypes.Structure):
     _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
                 ("eps_train", ctypes.c_int32), ("neighbor_weight", ctypes.c_float),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("weight_decay", ctypes.c_float),
+                ("lr_decay", ctypes.c_int32), ("far_weight", ctypes.c_float),
+                ("far_gap", ctypes.c_float)]
 
 
 class TrainStats(ctypes.Structure):
@@ -137,6 +139,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_encoder_embed.argtypes = [vp, vp, vp]
     lib.mnhn_encoder_set_margin.argtypes = [vp, ctypes.c_float]
     lib.mnhn_encoder_set_margin_table.argtypes = [vp, vp, vp, ctypes.c_int]
+    lib.mnhn_encoder_set_cosine_table.argtypes = [vp, vp, vp, ctypes.c_int]
     lib.mnhn_encoder_margin.argtypes = [vp]
     lib.mnhn_encoder_margin.restype = ctypes.c_float
     lib.mnhn_encoder_save.argtypes = [vp, ctypes.c_char_p]
@@ -298,10 +301,12 @@ class Encoder:
 
     def train(self, eng: Engine, ij: np.ndarray, dist: np.ndarray, epochs: int = 4,
               batch: int = 4096, lr: float = 1e-3, eps_train: int = 400,
-              neighbor_weight: float = 4.0, seed: int = 0) -> dict:
+              neighbor_weight: float = 4.0, seed: int = 0, weight_decay: float = 0.0,
+              lr_decay: bool = False, far_weight: float = 0.0, far_gap: float = 0.0) -> dict:
         ij = np.ascontiguousarray(ij, dtype=np.int32).reshape(-1, 2)
         dist = np.ascontiguousarray(dist, dtype=np.int32)
-        p = TrainParams(epochs, batch, lr, eps_train, neighbor_weight, seed)
+        p = TrainParams(epochs, batch, lr, eps_train, neighbor_weight, seed, weight_decay,
+                        int(bool(lr_decay)), far_weight, far_gap)
         st = TrainStats()
         _check(self.lib.mnhn_encoder_train(self.h, eng.ctx, ij.ctypes.data, dist.ctypes.data,
                                            len(ij), ctypes.byref(p), ctypes.byref(st)),
@@ -331,6 +336,16 @@ class Encoder:
         _check(self.lib.mnhn_encoder_set_margin_table(self.h, ml.ctypes.data, mg.ctypes.data, len(ml)),
                "mnhn_encoder_set_margin_table")
 
+    def set_cosine_table(self, max_len, cos_min) -> None:
+        """Minimum raw 5-mer profile cosine per length bucket of the pair's
+        longer sequence (negative: no test in that bucket)."""
+        ml = np.ascontiguousarray(max_len, dtype=np.int32)
+        cm = np.ascontiguousarray(cos_min, dtype=np.float32)
+        if ml.shape != cm.shape:
+            raise MnhnError("max_len and cos_min differ in length")
+        _check(self.lib.mnhn_encoder_set_cosine_table(self.h, ml.ctypes.data, cm.ctypes.data, len(ml)),
+               "mnhn_encoder_set_cosine_table")
+
     def save(self, path: str) -> None:
         _check(self.lib.mnhn_encoder_save(self.h, path.encode()), "mnhn_encoder_save")
 
@@ -348,15 +363,23 @@ class Encoder:
 
 
 def load_encoder_weights(path: str):
-    """Read a file written by Encoder.save: ((max_len[], margin[]),
-    [(Wt [out][in], b [out]) x 3])."""
+    """Read a file written by Encoder.save: ({"margin": (max_len[], margin[]),
+    "cosine": (max_len[], cos_min[])}, [(Wt [out][in], b [out]) x 3])."""
+    def table(f):
+        nb = int(np.frombuffer(f.read(4), "<i4")[0])
+        return (np.frombuffer(f.read(4 * nb), "<i4").copy(), np.frombuffer(f.read(4 * nb), "<f4").copy())
+
+    off = (np.array([2**31 - 1], np.int32), np.array([-1.0], np.float32))
     with open(path, "rb") as f:
         magic = f.read(8)
         if magic == b"MNHNENC1":
-            table = (np.array([2**31 - 1], np.int32), np.frombuffer(f.read(4), "<f4").copy())
+            tables = {"margin": (np.array([2**31 - 1], np.int32), np.frombuffer(f.read(4), "<f4").copy()),
+                      "cosine": off}
         elif magic == b"MNHNENC2":
-            nb = int(np.frombuffer(f.read(4), "<i4")[0])
-            table = (np.frombuffer(f.read(4 * nb), "<i4").copy(), np.frombuffer(f.read(4 * nb), "<f4").copy())
+            tables = {"margin": table(f), "cosine": off}
+        elif magic == b"MNHNENC3":
+            m = table(f)
+            tables = {"margin": m, "cosine": table(f)}
         else:
             raise MnhnError(f"{path}: not an encoder file")
         layers = []
@@ -365,7 +388,7 @@ def load_encoder_weights(path: str):
             w = np.frombuffer(f.read(4 * din * dout), "<f4").reshape(dout, din)
             b = np.frombuffer(f.read(4 * dout), "<f4")
             layers.append((w, b))
-    return table, layers
+    return tables, layers
 
 
 class _Cluster(ctypes.Structure):

@@ -183,11 +183,13 @@ __global__ void gather_rows_kernel(const bf16 *__restrict__ X, const int32_t *__
 }
 
 // One wave per pair: ||e_a - e_b|| regressed onto the normalised distance,
-// plus a hinge keeping neighbours inside their radius.
+// plus a hinge keeping neighbours inside their radius r and, with fw > 0, one
+// keeping non-neighbours beyond r + gap.  radius[k] = r >= 0 marks a
+// neighbour, radius[k] = -1 - r a non-neighbour.
 __global__ __launch_bounds__(256) void pair_loss_kernel(const float *__restrict__ emb,
                                                         const float *__restrict__ target,
                                                         const float *__restrict__ radius, int npairs,
-                                                        float nbw, float inv_batch,
+                                                        float nbw, float fw, float gap, float inv_batch,
                                                         bf16 *__restrict__ demb, float *__restrict__ loss) {
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -205,6 +207,12 @@ __global__ __launch_bounds__(256) void pair_loss_kernel(const float *__restrict_
     const float h = dist - rad;
     L += nbw * h * h;
     g += 2.0f * nbw * h;
+  } else if (rad < 0.0f && fw > 0.0f) {
+    const float h = (-1.0f - rad) + gap - dist;
+    if (h > 0.0f) {
+      L += fw * h * h;
+      g -= 2.0f * fw * h;
+    }
   }
   const float s = g * inv_batch / dist;
   bf16 *da = demb + (size_t)(2 * k) * D3, *db = demb + (size_t)(2 * k + 1) * D3;
@@ -224,9 +232,10 @@ __global__ void col_sum_kernel(const bf16 *__restrict__ dY, int R, int C, float 
   out[c] = s;
 }
 
+// Adam with decoupled weight decay wd (0 for biases)
 __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                             float *__restrict__ v, int n, float lr, float b1, float b2, float c1,
-                            float c2) {
+                            float c2, float wd) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -234,7 +243,8 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+  const float pi = p[i];
+  p[i] = pi - lr * ((mi / c1) / (sqrtf(vi / c2) + 1e-8f) + wd * pi);
 }
 
 // fp32 master Wt [out][in] -> bf16 Wt [out][in] and bf16 W [in][out]
@@ -262,14 +272,23 @@ struct MarginTable {
   float margin[kMaxMarginBuckets];
 };
 
+__device__ __forceinline__ float bucket_value(const MarginTable &t, int L) {
+  float m = t.margin[t.n - 1];
+  for (int k = t.n - 1; k >= 0; --k)
+    if (L <= t.upper[k]) m = t.margin[k];
+  return m;
+}
+
 // One wave per sorted row: copy of the bf16 embedding, its squared norm (the
-// same bf16 values the MFMA tiles read), length, and the row's radius term.
+// same bf16 values the MFMA tiles read), length, the row's radius term and its
+// minimum profile cosine (both taken from the row's length bucket).
 __global__ __launch_bounds__(256) void sorted_rows_kernel(const bf16 *__restrict__ E,
                                                           const int32_t *__restrict__ order,
                                                           const int32_t *__restrict__ len, int n,
-                                                          float eps_over_match, MarginTable mt,
+                                                          float eps_over_match, MarginTable mt, MarginTable ct,
                                                           bf16 *__restrict__ Es, float *__restrict__ norm,
-                                                          float *__restrict__ srad, int32_t *__restrict__ lens) {
+                                                          float *__restrict__ srad, float *__restrict__ cmin,
+                                                          int32_t *__restrict__ lens) {
   const int lane = threadIdx.x & 63;
   const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= n) return;
@@ -282,47 +301,69 @@ __global__ __launch_bounds__(256) void sorted_rows_kernel(const bf16 *__restrict
   for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
   if (lane == 0) {
     const int L = len[id];
-    float m = mt.margin[mt.n - 1];
-    for (int k = mt.n - 1; k >= 0; --k)
-      if (L <= mt.upper[k]) m = mt.margin[k];
     norm[p] = ss;
     lens[p] = L;
-    srad[p] = eps_over_match / (float)max(L, 1) + m;
+    srad[p] = eps_over_match / (float)max(L, 1) + bucket_value(mt, L);
+    cmin[p] = bucket_value(ct, L);
   }
 }
 
-// 128 x 128 tiles of Es Es^T (K = 128) over the length band: the host lists,
-// per row tile ti, the column tiles ti .. that the length bound can reach;
-// tile_off[ti] is the first block of row tile ti.  4 waves of 64 x 64 per
-// tile.  Hits are counted first and reserved with one atomic per wave, then
-// written (sorted positions p << 32 | q) at consecutive addresses.
-__global__ __launch_bounds__(256) void pair_candidates_kernel(
-    const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
-    const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
-    unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap) {
-  constexpr int BT = 128, KH = 64, LK = KH + 8;
-  __shared__ __attribute__((aligned(16))) bf16 sA[BT * LK];
-  __shared__ __attribute__((aligned(16))) bf16 sB[BT * LK];
-  const int blk = blockIdx.x;
-  int lo = 0, hi = tiles;  // tile_off[lo] <= blk < tile_off[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tile_off[mid] <= blk) lo = mid;
-    else hi = mid;
+// One wave per sorted row p: raw 5-mer counts of sequence order[p] as bf16
+// (integers up to 256 are exact) and their squared norm, for the profile
+// cosine of the pairwise stage.
+__global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t *__restrict__ words,
+                                                                 const uint32_t *__restrict__ woff,
+                                                                 const int32_t *__restrict__ len,
+                                                                 const int32_t *__restrict__ order, int n,
+                                                                 bf16 *__restrict__ Xs, float *__restrict__ pnorm) {
+  __shared__ int hist[D0];
+  __shared__ float red[256];
+  for (int p = blockIdx.x; p < n; p += gridDim.x) {
+    const int s = order[p];
+    for (int c = threadIdx.x; c < D0; c += 256) hist[c] = 0;
+    __syncthreads();
+    const int R = len[s] > 1 ? len[s] - 1 : 0;
+    const int nk = R >= 5 ? R - 4 : 0;
+    const uint32_t wo = woff[s];
+    for (int q = threadIdx.x; q < nk; q += 256) {
+      uint32_t code = 0;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const uint32_t r = (uint32_t)(q + i);
+        code = (code << 2) | ((words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u);
+      }
+      atomicAdd(&hist[code], 1);
+    }
+    __syncthreads();
+    float ss = 0.f;
+    for (int c = threadIdx.x; c < D0; c += 256) {
+      const bf16 v = (bf16)(float)hist[c];
+      Xs[(size_t)p * D0 + c] = v;
+      ss += (float)v * (float)v;
+    }
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) pnorm[p] = red[0];
+    __syncthreads();
   }
-  const int ti = lo, tj = ti + (blk - tile_off[ti]);
-  const int i0 = ti * BT, j0 = tj * BT;
+}
+
+// acc += A[i0 .. i0+128) . B[j0 .. j0+128)^T over k in [0, K) (row stride K),
+// staged through LDS 64 k at a time; each wave owns a 64 x 64 sub-tile.
+template <int K>
+__device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int n,
+                                         int i0, int j0, bf16 *sA, bf16 *sB, f32x4 (&acc)[4][4]) {
+  constexpr int KH = 64, LK = KH + 8;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 8 * (lane >> 4);
-  f32x4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  for (int kh = 0; kh < D3; kh += KH) {
+  for (int kh = 0; kh < K; kh += KH) {
     __syncthreads();
-    // 128 rows x 64 k = 1024 chunks of 8 per tile; 4 per thread
+    // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
@@ -330,8 +371,8 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
       bf16x8 va, vb;
 #pragma unroll
       for (int q = 0; q < 8; ++q) { va[q] = (bf16)0.0f; vb[q] = (bf16)0.0f; }
-      if (i0 + r < n) va = *reinterpret_cast<const bf16x8 *>(Es + (size_t)(i0 + r) * D3 + kh + k);
-      if (j0 + r < n) vb = *reinterpret_cast<const bf16x8 *>(Es + (size_t)(j0 + r) * D3 + kh + k);
+      if (i0 + r < n) va = *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + r) * K + kh + k);
+      if (j0 + r < n) vb = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + r) * K + kh + k);
       *reinterpret_cast<bf16x8 *>(sA + r * LK + k) = va;
       *reinterpret_cast<bf16x8 *>(sB + r * LK + k) = vb;
     }
@@ -350,6 +391,47 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
   }
+}
+
+// 128 x 128 tiles over the length band: the host lists, per row tile ti, the
+// column tiles ti .. that the length bound can reach; tile_off[ti] is the
+// first block of row tile ti.  4 waves of 64 x 64 per tile.  Stage 1: Es Es^T
+// (K = 128) and the radius test.  Stage 2 (kCos, skipped for a tile with no
+// stage-1 survivor): Xs Xs^T over the 1024 raw 5-mer counts and the cosine
+// test dot >= cmin * |x_p| |x_q|, cmin = cmin of the longer sequence (< 0:
+// no test).  Hits are counted first and reserved with one atomic per wave,
+// then written (sorted positions p << 32 | q) at consecutive addresses.
+template <bool kCos>
+__global__ __launch_bounds__(256) void pair_candidates_kernel(
+    const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
+    const bf16 *__restrict__ Xs, const float *__restrict__ pnorm, const float *__restrict__ cmin,
+    const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
+    unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap) {
+  constexpr int BT = 128, LK = 64 + 8;
+  __shared__ __attribute__((aligned(16))) bf16 sA[BT * LK];
+  __shared__ __attribute__((aligned(16))) bf16 sB[BT * LK];
+  __shared__ int any_hit;
+  const int blk = blockIdx.x;
+  int lo = 0, hi = tiles;  // tile_off[lo] <= blk < tile_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= blk) lo = mid;
+    else hi = mid;
+  }
+  const int ti = lo, tj = ti + (blk - tile_off[ti]);
+  const int i0 = ti * BT, j0 = tj * BT;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  f32x4 acc[4][4], accx[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      accx[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    }
+  if (tid == 0) any_hit = 0;
+  tile_dot<D3>(Es, Es, n, i0, j0, sA, sB, acc);
   // per-lane row / column terms (out-of-range rows and columns get lengths
   // that no |dl| <= max_dl can match)
   float ni[4][4], si[4][4], nj[4], sj[4];
@@ -373,13 +455,52 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
     lj[j] = v ? lens[gj] : (1 << 29);
   }
   const bool diag = ti == tj;
-  auto hit = [&](int i, int r, int j) -> bool {
+  auto hit1 = [&](int i, int r, int j) -> bool {
     const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
     const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
     const int dl = li[i][r] - lj[j];
     const float rad = dl >= 0 ? si[i][r] : sj[j];
     const float d2 = ni[i][r] + nj[j] - 2.0f * acc[i][j][r];
     return (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad * rad;
+  };
+  float pi[4][4], ci[4][4], pj[4], cj[4];
+  if constexpr (kCos) {
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) any |= hit1(i, r, j);
+    if (__ballot(any) != 0ull && lane == 0) any_hit = 1;
+    __syncthreads();
+    if (any_hit == 0) return;  // uniform over the block
+    tile_dot<D0>(Xs, Xs, n, i0, j0, sA, sB, accx);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+        const bool v = gi < n;
+        pi[i][r] = v ? pnorm[gi] : 0.f;
+        ci[i][r] = v ? cmin[gi] : -1.f;
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+      const bool v = gj < n;
+      pj[j] = v ? pnorm[gj] : 0.f;
+      cj[j] = v ? cmin[gj] : -1.f;
+    }
+  }
+  auto hit = [&](int i, int r, int j) -> bool {
+    if (!hit1(i, r, j)) return false;
+    if constexpr (kCos) {
+      const float c = li[i][r] - lj[j] >= 0 ? ci[i][r] : cj[j];
+      const float dot = accx[i][j][r];
+      if (c >= 0.0f && !(dot >= 0.0f && dot * dot >= c * c * pi[i][r] * pj[j])) return false;
+    }
+    return true;
   };
   unsigned total = 0;
 #pragma unroll
@@ -470,6 +591,7 @@ struct Layer {
 struct mnhn_encoder {
   Layer L[3];
   MarginTable margins{1, {INT32_MAX}, {0.02f}};
+  MarginTable cosmin{1, {INT32_MAX}, {-1.0f}};  // minimum profile cosine per bucket (< 0: none)
   int64_t adam_t = 0;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -660,6 +782,20 @@ int mnhn_encoder_set_margin_table(mnhn_encoder *enc, const int32_t *max_len, con
   return MNHN_OK;
 }
 
+int mnhn_encoder_set_cosine_table(mnhn_encoder *enc, const int32_t *max_len, const float *cos_min, int n) {
+  if (!enc || !max_len || !cos_min || n < 1 || n > kMaxMarginBuckets) return MNHN_ERR_ARG;
+  for (int k = 0; k < n; ++k) {
+    if (!(cos_min[k] <= 1.f) || !std::isfinite(cos_min[k])) return MNHN_ERR_ARG;
+    if (k > 0 && max_len[k] <= max_len[k - 1]) return MNHN_ERR_ARG;
+  }
+  enc->cosmin.n = n;
+  for (int k = 0; k < n; ++k) {
+    enc->cosmin.upper[k] = max_len[k];
+    enc->cosmin.margin[k] = cos_min[k] < 0.f ? -1.f : cos_min[k];
+  }
+  return MNHN_OK;
+}
+
 float mnhn_encoder_margin(const mnhn_encoder *enc) {
   if (!enc) return -1.f;
   float m = 0.f;
@@ -670,7 +806,7 @@ float mnhn_encoder_margin(const mnhn_encoder *enc) {
 int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, const int32_t *dist,
                        int64_t m, const mnhn_train_params *p, mnhn_train_stats *stats) {
   if (!enc || !c || !p || m <= 0 || !ij || !dist || p->batch < 32 || p->batch % 32 || p->epochs < 1 ||
-      !(p->lr > 0.f))
+      !(p->lr > 0.f) || !(p->weight_decay >= 0.f) || !(p->far_weight >= 0.f))
     return MNHN_ERR_ARG;
   if (!c->acgt) return MNHN_ERR_ARG;
   for (int64_t k = 0; k < 2 * m; ++k)
@@ -713,6 +849,7 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
   double first_loss = 0, last_loss = 0;
   int64_t steps = 0;
   const int64_t steps_per_epoch = (m + B - 1) / B;
+  const int64_t total_steps = steps_per_epoch * p->epochs;
   for (int ep = 0; ep < p->epochs && rc == MNHN_OK; ++ep) {
     std::shuffle(order.begin(), order.end(), rng);
     double ep_loss = 0;
@@ -726,7 +863,8 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
         h_idx[2 * k + 1] = ib;
         const double L4 = (double)c->match * std::max(std::max(c->len[ia], c->len[ib]), 1);
         h_tgt[k] = (float)(dist[pk] / L4);
-        h_rad[k] = dist[pk] <= p->eps_train ? (float)(p->eps_train / L4) : -1.f;
+        const float r = (float)(p->eps_train / L4);
+        h_rad[k] = dist[pk] <= p->eps_train ? r : -1.f - r;
       }
       hipError_t e = hipMemcpyAsync(d_idx, h_idx.data(), R * 4, hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipMemcpyAsync(d_tgt, h_tgt.data(), B * 4, hipMemcpyHostToDevice, st);
@@ -738,7 +876,7 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       b.emb16 = nullptr;
       if ((rc = forward(enc, st, Xb, R, b))) break;
       hipLaunchKernelGGL(pair_loss_kernel, dim3(blocks_for(B, 4)), dim3(256), 0, st, a.emb, d_tgt, d_rad, B,
-                         p->neighbor_weight, 1.0f / (float)B, demb, d_loss);
+                         p->neighbor_weight, p->far_weight, p->far_gap, 1.0f / (float)B, demb, d_loss);
       // layer 3: dW3 = dEmb^T a2, db3 = colsum dEmb, dpre2 = (dEmb W3) * gelu'(pre2)
       if ((rc = transpose(st, demb, R, D3, dembT)) || (rc = transpose(st, a.a2, R, D2, a2T))) break;
       Epi g3; g3.c32 = enc->L[2].gw; g3.ldc = D2;
@@ -762,12 +900,15 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       enc->adam_t++;
       const float b1 = 0.9f, b2 = 0.999f;
       const float c1 = 1.f - (float)std::pow(b1, (double)enc->adam_t), c2 = 1.f - (float)std::pow(b2, (double)enc->adam_t);
+      // optional linear decay of the rate to 10% over the run
+      const double frac = (double)steps / (double)std::max<int64_t>(total_steps, 1);
+      const float lr = p->lr_decay ? (float)(p->lr * (1.0 - 0.9 * frac)) : p->lr;
       for (Layer &L : enc->L) {
         const int nw = L.in * L.out;
         hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(nw, 256)), dim3(256), 0, st, L.wt, L.gw, L.mw, L.vw, nw,
-                           p->lr, b1, b2, c1, c2);
+                           lr, b1, b2, c1, c2, p->weight_decay);
         hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(L.out, 256)), dim3(256), 0, st, L.b, L.gb, L.mb, L.vb,
-                           L.out, p->lr, b1, b2, c1, c2);
+                           L.out, lr, b1, b2, c1, c2, 0.0f);
         hipLaunchKernelGGL(refresh_weights_kernel, dim3(blocks_for(nw, 256)), dim3(256), 0, st, L.wt, L.out,
                            L.in, L.wt16, L.w16);
       }
@@ -813,11 +954,24 @@ int mnhn_encoder_embed(mnhn_encoder *enc, mnhn_sw_ctx *c, float *out) {
   return rc;
 }
 
-// File layout (little-endian): "MNHNENC2", int32 n, int32 upper[n],
-// float margin[n], then per layer int32 in, int32 out, float Wt[out][in],
-// float b[out].  "MNHNENC1" files hold one float margin in place of the table.
-static const char kEncMagic[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '2'};
+// File layout (little-endian): "MNHNENC3", the margin table (int32 n, int32
+// upper[n], float margin[n]), the cosine table (same form), then per layer
+// int32 in, int32 out, float Wt[out][in], float b[out].  "MNHNENC2" files have
+// no cosine table; "MNHNENC1" files hold one float margin in place of both.
+static const char kEncMagic[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '3'};
+static const char kEncMagicV2[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '2'};
 static const char kEncMagicV1[8] = {'M', 'N', 'H', 'N', 'E', 'N', 'C', '1'};
+
+static void write_table(FILE *f, const MarginTable &t) {
+  fwrite(&t.n, 4, 1, f);
+  fwrite(t.upper, 4, (size_t)t.n, f);
+  fwrite(t.margin, 4, (size_t)t.n, f);
+}
+
+static bool read_table(FILE *f, MarginTable &t) {
+  return fread(&t.n, 4, 1, f) == 1 && t.n >= 1 && t.n <= kMaxMarginBuckets &&
+         fread(t.upper, 4, (size_t)t.n, f) == (size_t)t.n && fread(t.margin, 4, (size_t)t.n, f) == (size_t)t.n;
+}
 
 int mnhn_encoder_save(const mnhn_encoder *enc, const char *path) {
   if (!enc || !path) return MNHN_ERR_ARG;
@@ -825,9 +979,8 @@ int mnhn_encoder_save(const mnhn_encoder *enc, const char *path) {
   if (!f) return MNHN_ERR_ARG;
   int rc = MNHN_OK;
   fwrite(kEncMagic, 1, 8, f);
-  fwrite(&enc->margins.n, 4, 1, f);
-  fwrite(enc->margins.upper, 4, (size_t)enc->margins.n, f);
-  fwrite(enc->margins.margin, 4, (size_t)enc->margins.n, f);
+  write_table(f, enc->margins);
+  write_table(f, enc->cosmin);
   for (const Layer &L : enc->L) {
     std::vector<float> w((size_t)L.in * L.out), b(L.out);
     if (hipMemcpy(w.data(), L.wt, w.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -849,16 +1002,16 @@ int mnhn_encoder_load(mnhn_encoder *enc, const char *path) {
   FILE *f = fopen(path, "rb");
   if (!f) return MNHN_ERR_ARG;
   char magic[8];
-  MarginTable mt{1, {INT32_MAX}, {0.f}};
+  MarginTable mt{1, {INT32_MAX}, {0.f}}, ct{1, {INT32_MAX}, {-1.f}};
   int rc = MNHN_OK;
   if (fread(magic, 1, 8, f) != 8) {
     rc = MNHN_ERR_ARG;
   } else if (memcmp(magic, kEncMagicV1, 8) == 0) {
     if (fread(&mt.margin[0], 4, 1, f) != 1) rc = MNHN_ERR_ARG;
+  } else if (memcmp(magic, kEncMagicV2, 8) == 0) {
+    if (!read_table(f, mt)) rc = MNHN_ERR_ARG;
   } else if (memcmp(magic, kEncMagic, 8) == 0) {
-    if (fread(&mt.n, 4, 1, f) != 1 || mt.n < 1 || mt.n > kMaxMarginBuckets ||
-        fread(mt.upper, 4, (size_t)mt.n, f) != (size_t)mt.n || fread(mt.margin, 4, (size_t)mt.n, f) != (size_t)mt.n)
-      rc = MNHN_ERR_ARG;
+    if (!read_table(f, mt) || !read_table(f, ct)) rc = MNHN_ERR_ARG;
   } else {
     rc = MNHN_ERR_ARG;
   }
@@ -873,6 +1026,7 @@ int mnhn_encoder_load(mnhn_encoder *enc, const char *path) {
   }
   fclose(f);
   if (rc == MNHN_OK) rc = mnhn_encoder_set_margin_table(enc, mt.upper, mt.margin, mt.n);
+  if (rc == MNHN_OK) rc = mnhn_encoder_set_cosine_table(enc, ct.upper, ct.margin, ct.n);
   if (rc == MNHN_OK) {
     rc = refresh(enc);
     if (rc == MNHN_OK && hipStreamSynchronize(enc->stream) != hipSuccess) rc = MNHN_ERR_DEVICE;
@@ -904,14 +1058,15 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     (void)hipEventRecord(e0, st);
     return (double)ms;
   };
-  bf16 *X = nullptr, *E16 = nullptr, *Es = nullptr;
-  float *norm = nullptr, *srad = nullptr;
+  bf16 *X = nullptr, *E16 = nullptr, *Es = nullptr, *Xs = nullptr;
+  float *norm = nullptr, *srad = nullptr, *cmin = nullptr, *pnorm = nullptr;
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr;
   int4 *items = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
+    dev_free(Xs); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
     (void)hipEventDestroy(e0);
@@ -935,15 +1090,23 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   dev_free(X);
   // length-ordered copies for the pairwise stage
   if ((rc = dev_alloc(&Es, (size_t)n * D3)) || (rc = dev_alloc(&norm, (size_t)n)) ||
-      (rc = dev_alloc(&srad, (size_t)n)) || (rc = dev_alloc(&lens, (size_t)n)) ||
-      (rc = dev_alloc(&d_order, (size_t)n)))
+      (rc = dev_alloc(&srad, (size_t)n)) || (rc = dev_alloc(&cmin, (size_t)n)) ||
+      (rc = dev_alloc(&lens, (size_t)n)) || (rc = dev_alloc(&d_order, (size_t)n)))
     return fail(rc);
+  bool use_cos = false;
+  for (int k = 0; k < enc->cosmin.n; ++k) use_cos |= enc->cosmin.margin[k] >= 0.f;
   if (hipMemcpyAsync(d_order, c->order.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess)
     return fail(MNHN_ERR_DEVICE);
   const int max_dl = eps_max / c->match;
   hipLaunchKernelGGL(sorted_rows_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, st, E16, d_order, c->d_len, n,
-                     (float)eps_max / (float)c->match, enc->margins, Es, norm, srad, lens);
+                     (float)eps_max / (float)c->match, enc->margins, enc->cosmin, Es, norm, srad, cmin, lens);
   if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
+  if (use_cos) {
+    if ((rc = dev_alloc(&Xs, (size_t)n * D0)) || (rc = dev_alloc(&pnorm, (size_t)n))) return fail(rc);
+    hipLaunchKernelGGL(kmer_counts_sorted_kernel, dim3((unsigned)std::min(n, 16384)), dim3(256), 0, st,
+                       c->d_words, c->d_woff, c->d_len, d_order, n, Xs, pnorm);
+    if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
+  }
   dev_free(E16);
   stt.embed_ms = phase_ms();
   // ---- proposals over the length band: row tile ti reaches column tiles
@@ -974,8 +1137,12 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if ((rc = dev_alloc(&cand, cap))) break;
     hipError_t e = hipMemsetAsync(d_cnt64, 0, 8, st);
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(pair_candidates_kernel, dim3(nblk), dim3(256), 0, st, Es, norm, srad, lens, n, d_toff,
-                         tiles, max_dl, cand, d_cnt64, cap);
+      if (use_cos)
+        hipLaunchKernelGGL(pair_candidates_kernel<true>, dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs, pnorm,
+                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap);
+      else
+        hipLaunchKernelGGL(pair_candidates_kernel<false>, dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs, pnorm,
+                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&m, d_cnt64, 8, hipMemcpyDeviceToHost, st);
@@ -985,7 +1152,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     cap = m;  // overflowed: exact size, recompute once
   }
   if (rc || m > cap) return fail(rc ? rc : MNHN_ERR_NOMEM);
-  dev_free(Es); dev_free(norm); dev_free(srad); dev_free(lens);
+  dev_free(Es); dev_free(norm); dev_free(srad); dev_free(lens); dev_free(Xs); dev_free(pnorm); dev_free(cmin);
   stt.candidates = (int64_t)m;
   stt.propose_ms = phase_ms();
   // ---- exact verification: candidates grouped per row into targets of 128

@@ -167,8 +167,10 @@ def test_proposal_rule_and_margin_table_match_numpy(pkg, tmp_path):
         with pkg.Encoder(seed=1) as enc2:
             enc2.load(str(tmp_path / "enc.bin"))
             rp2, col2, dist2, st2 = enc2.prefilter_csr(eng, eps)
-    (tu, tm), _ = pkg.load_encoder_weights(str(tmp_path / "enc.bin"))
+    tables, _ = pkg.load_encoder_weights(str(tmp_path / "enc.bin"))
+    tu, tm = tables["margin"]
     assert np.array_equal(tu, upper.astype(np.int32)) and np.array_equal(tm, margins)
+    assert np.all(tables["cosine"][1] < 0)
     assert st2["candidates"] == st["candidates"] and np.array_equal(col2, col)
     lmax = np.maximum(L[:, None], L[None, :])
     marg = margins[np.searchsorted(upper, lmax)]
@@ -209,3 +211,41 @@ def test_prefilter_positions_beyond_2e31(pkg):
         erp, ecol, edist = eng.eps_csr(eps)
     assert st["candidates"] > 2**31
     assert np.array_equal(rp, erp) and np.array_equal(col, ecol) and np.array_equal(dist, edist)
+
+
+def test_cosine_rule_matches_numpy(pkg):
+    """With a cosine table set, a pair is proposed only if it also passes the
+    raw 5-mer count cosine bound of its longer sequence's length bucket:
+    candidate count and verified lists against numpy."""
+    seqs = synth.config2(seed=47, n_roots=8, n_children=4, n_leaves=12)
+    eps = 400
+    n = len(seqs)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    _say(f"cosine rule n={n}")
+    nk = np.maximum(L - 5, 0).astype(np.float64)
+    counts = np.rint(_kmer_ref(seqs) * nk[:, None])
+    nrm = np.sqrt((counts ** 2).sum(1))
+    cos = (counts @ counts.T) / np.maximum(nrm[:, None] * nrm[None, :], 1e-30)
+    iu = np.triu_indices(n, 1)
+    upper = np.array([300, 2**31 - 1])
+    cmin = np.quantile(cos[iu], [0.6, 0.8]).astype(np.float32)
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=9) as enc:
+        enc.margin = 1e3  # the embedding test passes every pair
+        enc.set_cosine_table(upper, cmin)
+        rp, col, dist, st = enc.prefilter_csr(eng, eps)
+        erp, ecol, edist = eng.eps_csr(eps)
+    lmax = np.maximum(L[:, None], L[None, :])
+    cm = cmin[np.searchsorted(upper, lmax)]
+    prop = (4 * np.abs(L[:, None] - L[None, :]) <= eps) & (cos >= cm)
+    border = np.abs(cos - cm) <= 1e-5
+    _say(f"proposed {int(prop[iu].sum())} of {len(iu[0])}, border {int(border[iu].sum())}, "
+         f"kernel {st['candidates']}")
+    assert 0 < prop[iu].sum() < len(iu[0])
+    assert abs(int(st["candidates"]) - int(prop[iu].sum())) <= int(border[iu].sum())
+    exact = np.zeros((n, n), bool)
+    exact[np.repeat(np.arange(n), np.diff(erp)), ecol] = True
+    got = np.zeros((n, n), bool)
+    got[np.repeat(np.arange(n), np.diff(rp)), col] = True
+    sure = ~(border | border.T)
+    assert np.array_equal(got & sure, exact & (prop | prop.T) & sure)
+    assert not np.any(got & ~exact)

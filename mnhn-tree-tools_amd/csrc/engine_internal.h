@@ -88,7 +88,10 @@ struct Work {
   const int4 *d_items = nullptr;
   int n_items = 0;
   const int32_t *d_qids = nullptr, *d_gtgt = nullptr, *d_gmax = nullptr, *d_gmin = nullptr;
-  int dev_kernel = 0;  // kernel variant for device work (KERNEL_LDS or KERNEL_REG)
+  int dev_kernel = 0;  // kernel variant for device work (KERNEL_LDS, KERNEL_LDS_F16 or KERNEL_REG)
+  // KERNEL_LDS_F16 device work: every pair of every item satisfies
+  // match * min(la, lb) <= 2048 (otherwise the int16 LDS kernel runs it)
+  bool dev_f16_ok = false;
 };
 
 

@@ -559,10 +559,12 @@ __global__ void cand_groups_kernel(const unsigned long long *__restrict__ c, lon
   }
 }
 
-__global__ void cand_items_kernel(const int32_t *__restrict__ gbase, int n, int4 *__restrict__ items) {
+// items of row i (one per target group) at ibase[i] ..
+__global__ void cand_items_kernel(const int32_t *__restrict__ gbase, const int32_t *__restrict__ ibase, int n,
+                                  int4 *__restrict__ items) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (int g = gbase[i]; g < gbase[i + 1]; ++g) items[g] = make_int4(g, i, i + 1, 0);
+  for (int g = gbase[i]; g < gbase[i + 1]; ++g) items[ibase[i] + (g - gbase[i])] = make_int4(g, i, i + 1, 0);
 }
 
 __global__ void iota_kernel(int32_t *a, int n) {
@@ -1062,13 +1064,15 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   float *norm = nullptr, *srad = nullptr, *cmin = nullptr, *pnorm = nullptr;
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
-          *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr;
+          *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr,
+          *ibase = nullptr;
   int4 *items = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
     dev_free(Xs); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
+    dev_free(ibase);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   };
@@ -1160,22 +1164,31 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   if (m > 0) {
     std::vector<int32_t> cnt(n), gb(n + 1, 0);
     if ((rc = dev_alloc(&rowcnt, (size_t)n)) || (rc = dev_alloc(&gbase, (size_t)n + 1)) ||
-        (rc = dev_alloc(&cursor, (size_t)n)) || (rc = dev_alloc(&qids, (size_t)n))) {
-      cleanup(); mnhn_csr_free(out); return rc;
-    }
+        (rc = dev_alloc(&cursor, (size_t)n)) || (rc = dev_alloc(&qids, (size_t)n)))
+      return fail(rc);
     hipError_t e = hipMemsetAsync(rowcnt, 0, (size_t)n * 4, st);
     hipLaunchKernelGGL(cand_count_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand, (long long)m,
                        d_order, rowcnt);
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+    if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + (cnt[i] + kGroupSize - 1) / kGroupSize;
     const int G = gb[n];
+    // rows whose pairs all fit the exact fp16 form (the row's sequence is the
+    // longer of each of its pairs, so match * len <= 2048 suffices) take the
+    // first G16 items and run the fp16 kernel; the rest run the int16 kernel
+    std::vector<int32_t> ib(n);
+    int G16 = 0;
+    for (int i = 0; i < n; ++i)
+      if ((long long)c->match * c->len[i] <= 2048) { ib[i] = G16; G16 += gb[i + 1] - gb[i]; }
+    for (int i = 0, o = G16; i < n; ++i)
+      if ((long long)c->match * c->len[i] > 2048) { ib[i] = o; o += gb[i + 1] - gb[i]; }
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
-        (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G))) {
-      cleanup(); mnhn_csr_free(out); return rc;
-    }
+        (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||
+        (rc = dev_alloc(&ibase, (size_t)n)))
+      return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, (size_t)n * 4, st);
     if (e == hipSuccess) e = hipMemsetAsync(gmax, 0, (size_t)G * 4, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
@@ -1183,36 +1196,44 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks(G, 256)), dim3(256), 0, st, gmin, (long long)G, 0x7fffffff);
     hipLaunchKernelGGL(cand_groups_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand,
                        (long long)m, d_order, gbase, cursor, c->d_len, gtgt, gmax, gmin);
-    hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, n, items);
+    hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, ibase, n, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
-    if (e != hipSuccess) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+    if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     dev_free(cand);
     Work w;
     w.dev = true;
-    w.d_items = items;
-    w.n_items = G;
     w.d_qids = qids;
     w.d_gtgt = gtgt;
     w.d_gmax = gmax;
     w.d_gmin = gmin;
-    w.dev_kernel = KERNEL_LDS;
     w.max_q_len = c->maxlen;
-    w.pairs = (double)m;
     // key buffer: both directions of every neighbour; start from at most 2^28
     // keys and, if the neighbours need more, grow to the exact count and verify
     // once more
     unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap));
+    double kernel_ms = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
-      rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
+      kernel_ms = 0;
+      for (int part = 0; part < 2 && rc == MNHN_OK; ++part) {
+        w.d_items = part == 0 ? items : items + G16;
+        w.n_items = part == 0 ? G16 : G - G16;
+        w.dev_kernel = part == 0 ? KERNEL_LDS_F16 : KERNEL_LDS;
+        w.dev_f16_ok = part == 0;
+        if (w.n_items == 0) continue;
+        rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
+        kernel_ms += c->last.kernel_ms;
+      }
       if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
       if (rc) return fail(rc);
       if (nk <= c->keys_cap) break;
       kcap = nk;
     }
     if (nk > c->keys_cap) return fail(MNHN_ERR_NOMEM);
+    c->last.kernel_ms = kernel_ms;
+    c->last.pairs = (double)m;
   }
   stt.verify_ms = phase_ms();
   stt.neighbours = (int64_t)(nk / 2);

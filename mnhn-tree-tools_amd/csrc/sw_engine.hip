@@ -888,7 +888,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   // all satisfy match * min(la, lb) <= 2048; the rest run the int16 LDS kernel.
   std::vector<std::pair<int, std::vector<int4>>> passes;
   if (w.dev) {
-    passes.push_back({K == KERNEL_LDS_F16 ? KERNEL_LDS : K, {}});
+    passes.push_back({(K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K, {}});
   } else if (K == KERNEL_LDS_F16) {
     std::vector<int4> f16, i16;
     for (const int4 &it : w.items) {

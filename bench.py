@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
-"""Benchmark: all-pairs Smith-Waterman GCUPS on one MI355X.
+"""Benchmark: all-pairs Smith-Waterman GCUPS on one MI355X (default), and the
+learned-prefilter runs.
 
-Workload (one step = one pass of the hot path over the whole batch):
+Default workload (one step = one pass of the hot path over the whole batch):
   uneven-length set of 20,000 sequences from a 3-level evolution tree
   (100 roots with lognormal lengths, median 300, clipped to [100, 600];
   10 children at 8% mutation; 20 leaves per child at 3%; sub:indel 4:1),
@@ -9,7 +10,7 @@ Workload (one step = one pass of the hot path over the whole batch):
   kernel, with the epsilon-neighbour lists (d <= eps_max) emitted on the
   device.  Sequences are uploaded once before timing (resident in HBM).
 
-GCUPS = sum over unique pairs of la*lb / step seconds / 1e9.
+GCUPS = sum over the computed unique pairs of la*lb / step seconds / 1e9.
 
 Also reported:
   roofline     -- the SW kernel's integer-op rate (6 ops per cell, measured
@@ -17,11 +18,21 @@ Also reported:
                   int16 VALU peak of the chip;
   cpu_baseline -- the reference's own score() (compiled from its source into
                   oracle/_ref/) on a bounded random sample of the same
-                  sequences, single thread, as the reference's CPU path runs;
+                  sequences, single thread, as the reference's CPU path runs
+                  (cpu_baseline_all_cores: the same on all host cores);
   dbscan       -- wall seconds of the drop-in adaptive_clustering_SW CLI on the
                   same 20,000 sequences (FASTA in, split-set files out).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other configurations:
+  --config c3  105,000 amplicon-like sequences, same measurement;
+  --config c4  learned proposer: train on 2M exact-labelled pairs, calibrate on
+               held-out sequences, then recall / precision / time of the
+               prefilter lists against the exact lists on 100,000 held-out
+               sequences (and DBSCAN on both);
+  --config c5  end-to-end DBSCAN on 500,000 heavy-tailed sequences: exact
+               lists + DBSCAN against prefilter lists + DBSCAN.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4|c5]
 """
 import argparse
 import ctypes
@@ -74,10 +85,10 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(seqs, sample_n, seed=7):
+def cpu_baseline(seqs, sample_n, seed=7, threads=1):
     """Reference score() (oracle/_ref/libref_sw.so) over all unique pairs of
-    a random sample, one thread; the oracle's C restatement if the reference
-    build is absent."""
+    a random sample on `threads` host threads (pairs split statically); the
+    oracle's C restatement if the reference build is absent."""
     rng = random.Random(seed)
     sample = rng.sample(range(len(seqs)), sample_n)
     sub = [seqs[i] for i in sample]
@@ -95,10 +106,10 @@ def cpu_baseline(seqs, sample_n, seed=7):
     f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                   ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
     cs = ctypes.c_uint64(0)
-    secs = f(ptrs, lens, len(sub), 1, 0, None, ctypes.byref(cs))
+    secs = f(ptrs, lens, len(sub), threads, 0, None, ctypes.byref(cs))
     L = np.array([len(s) for s in sub], dtype=np.float64)
     cells = 0.5 * (L.sum() ** 2 - (L * L).sum())
-    return {"value": cells / secs / 1e9, "unit": "GCUPS", "cores": 1, "kind": kind,
+    return {"value": cells / secs / 1e9, "unit": "GCUPS", "cores": threads, "kind": kind,
             "sample": f"all {sample_n * (sample_n - 1) // 2} unique pairs of {sample_n} "
                       f"random sequences of the workload ({cells:.3g} cells, {secs:.1f} s); "
                       f"host: {cpu_model()}, nproc {os.cpu_count()}"}
@@ -406,11 +417,12 @@ def run_proposer_bench(args, pkg, log):
             f"{np.quantile(lm, [0, 0.25, 0.5, 0.75, 1]).tolist()}; first "
             f"{[(int(a), int(b), int(L[a]), int(L[b])) for a, b in mi[:8]]}")
     all_pairs = n * (n - 1) / 2
-    # the pairwise kernel's algorithmic work: 128 x 128 x 128 multiply-adds per
-    # tile of the length band
+    # the pairwise kernel's algorithmic work per band tile: 128 x 128 dot
+    # products of the 128-d embeddings, plus (cosine stage) of the 1024-d counts
+    kdim = 128 + (0 if args.no_cosine else 1024)
     mfma = {"bound": "mfma", "kernel": "pair_candidates_kernel",
-            "achieved": pst["tiles"] * 2 * 128 ** 3 / (pst["propose_ms"] / 1e3) / 1e12,
-            "peak": 2500.0, "unit": "TFLOP/s (bf16 MFMA; 2*128^3 flop per band tile)",
+            "achieved": pst["tiles"] * 2 * 128 * 128 * kdim / (pst["propose_ms"] / 1e3) / 1e12,
+            "peak": 2500.0, "unit": f"TFLOP/s (bf16 MFMA; 2*128*128*{kdim} flop per band tile)",
             "traffic": None}
     mfma["frac"] = mfma["achieved"] / mfma["peak"]
     res = {"eps": eps, "recall": recall, "proposed_subset_of_exact": bool(subset),
@@ -420,6 +432,7 @@ def run_proposer_bench(args, pkg, log):
            "exact_kernel_s": et["kernel_ms"] / 1e3, "exact_pairs_computed": et["pairs"],
            "exact_cells_computed": et["cells"], "all_pairs": all_pairs,
            "all_pairs_cells": 0.5 * (L.sum() ** 2 - (L * L).sum()),
+           "verify_gcups": pst["cells"] / max(pst["verify_kernel_ms"], 1e-9) / 1e6,
            "prefilter_phases_ms": pst, "training": info}
     t0 = time.perf_counter()
     ecl = pkg.dbscan_csr(erp, ecol, edist, eps, eps, minpts)
@@ -544,28 +557,32 @@ def main():
         t = eng.last_timing()
         kms.append(t["kernel_ms"])
         print(f"step {k}: {times[-1] * 1e3:.1f} ms (kernel {kms[-1]:.1f} ms), "
-              f"{cells / times[-1] / 1e9:.1f} GCUPS, nnz {nnz}", file=sys.stderr, flush=True)
+              f"{t['cells'] / times[-1] / 1e9:.1f} GCUPS, nnz {nnz}", file=sys.stderr, flush=True)
     t = eng.last_timing()
+    # GCUPS counts the cells of the pairs the step computed: every unique pair,
+    # unless eps_max is small enough for the length bound to leave some out
+    comp_cells, comp_pairs = t["cells"], t["pairs"]
     # the same step with the int16 form of the LDS kernel only (integer VALU path)
     eng.set_tuning(2, 32, 16)
     t0 = time.perf_counter()
     eng.eps_count(args.eps_max)
     int16_s = time.perf_counter() - t0
     int16_kernel_ms = eng.last_timing()["kernel_ms"]
-    print(f"int16-only step: {int16_s * 1e3:.1f} ms, {cells / int16_s / 1e9:.1f} GCUPS",
+    print(f"int16-only step: {int16_s * 1e3:.1f} ms, {comp_cells / int16_s / 1e9:.1f} GCUPS",
           file=sys.stderr, flush=True)
     eng.close()
     step_s = float(np.mean(times))
     kernel_s = float(np.mean(kms)) / 1e3
-    gcups = cells / step_s / 1e9
-    achieved = OPS_PER_CELL * cells / kernel_s / 1e12
+    gcups = comp_cells / step_s / 1e9
+    achieved = OPS_PER_CELL * comp_cells / kernel_s / 1e12
     out = {
         "metric": "all-pairs SW GCUPS", "value": gcups, "unit": "GCUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_s * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config],
-                   "n_seqs": len(seqs), "unique_pairs": pairs, "cells": cells,
+                   "n_seqs": len(seqs), "unique_pairs": pairs, "all_pairs_cells": cells,
+                   "computed_pairs": comp_pairs, "computed_cells": comp_cells,
                    "eps_max": args.eps_max, "eps_list_entries": nnz,
                    "kernel": {1: "register-profile int16", 2: "lds-profile int16",
                               3: "lds-profile exact fp16 (+int16 pass)"}.get(t["kernel"]),
@@ -578,13 +595,16 @@ def main():
                      "kernel": "sw_lds_f16_kernel + sw_lds_kernel" if t["kernel"] == 3
                                else {1: "sw_tiles_kernel", 2: "sw_lds_kernel"}.get(t["kernel"]),
                      "kernel_ms": kernel_s * 1e3,
-                     "kernel_gcups": cells / kernel_s / 1e9,
-                     "int16_only_gcups": cells / int16_s / 1e9,
+                     "kernel_gcups": comp_cells / kernel_s / 1e9,
+                     "int16_only_gcups": comp_cells / int16_s / 1e9,
                      "int16_only_kernel_ms": int16_kernel_ms},
     }
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(seqs, args.cpu_sample)
+        # best effort on the host: every core the box gives this process (at most 16)
+        nt = max(1, min(16, len(os.sched_getaffinity(0))))
+        out["cpu_baseline_all_cores"] = cpu_baseline(seqs, 3 * args.cpu_sample, threads=nt)
     if not args.no_dbscan and args.config == "c2":
         print("adaptive DBSCAN run ...", file=sys.stderr, flush=True)
         out["dbscan"] = run_dbscan(seqs, 100, 300, 5)

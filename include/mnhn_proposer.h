@@ -53,6 +53,8 @@ typedef struct {
   int64_t neighbours;     /* of those, exact d <= eps_max */
   double profile_ms, embed_ms, propose_ms, verify_ms, csr_ms; /* device time per phase */
   int64_t tiles;          /* 128 x 128 embedding tiles the length band leaves to compare */
+  double cells;           /* sum of la * lb over the candidates (verification work) */
+  double verify_kernel_ms; /* SW kernel time of the verification (HIP events) */
 } mnhn_prefilter_stats;
 
 /* 5-mer profiles of every sequence (n x 1024 fp32, host buffer). */

@@ -97,7 +97,8 @@ class PrefilterStats(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_int64), ("neighbours", ctypes.c_int64),
                 ("profile_ms", ctypes.c_double), ("embed_ms", ctypes.c_double),
                 ("propose_ms", ctypes.c_double), ("verify_ms", ctypes.c_double),
-                ("csr_ms", ctypes.c_double), ("tiles", ctypes.c_int64)]
+                ("csr_ms", ctypes.c_double), ("tiles", ctypes.c_int64), ("cells", ctypes.c_double),
+                ("verify_kernel_ms", ctypes.c_double)]
 
 
 _LIB = None

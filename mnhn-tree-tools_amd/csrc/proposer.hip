@@ -542,12 +542,13 @@ __global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long
     atomicAdd(&cnt[order[(int)(c[k] >> 32)]], 1);
 }
 
-// scatter candidates into their row's target groups of 128
+// scatter candidates into their row's target groups of 128; tlen[i] sums the
+// target lengths of row i (for the cell count)
 __global__ void cand_groups_kernel(const unsigned long long *__restrict__ c, long long m,
                                    const int32_t *__restrict__ order, const int32_t *__restrict__ gbase,
                                    int32_t *__restrict__ cursor, const int32_t *__restrict__ len,
                                    int32_t *__restrict__ gtgt, int32_t *__restrict__ gmax,
-                                   int32_t *__restrict__ gmin) {
+                                   int32_t *__restrict__ gmin, unsigned long long *__restrict__ tlen) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
        k += (long long)gridDim.x * blockDim.x) {
     const int i = order[(int)(c[k] >> 32)], j = order[(int)(c[k] & 0xffffffffull)];
@@ -556,6 +557,7 @@ __global__ void cand_groups_kernel(const unsigned long long *__restrict__ c, lon
     gtgt[(size_t)g * kGroupSize + (o % kGroupSize)] = j;
     atomicMax(&gmax[g], len[j]);
     atomicMin(&gmin[g], len[j]);
+    atomicAdd(&tlen[i], (unsigned long long)len[j]);
   }
 }
 
@@ -1066,13 +1068,14 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr,
           *ibase = nullptr;
+  unsigned long long *tlen = nullptr;
   int4 *items = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
     dev_free(Xs); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
-    dev_free(ibase);
+    dev_free(ibase); dev_free(tlen);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   };
@@ -1185,21 +1188,26 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if ((long long)c->match * c->len[i] > 2048) { ib[i] = o; o += gb[i + 1] - gb[i]; }
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||
-        (rc = dev_alloc(&ibase, (size_t)n)))
+        (rc = dev_alloc(&ibase, (size_t)n)) || (rc = dev_alloc(&tlen, (size_t)n)))
       return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, (size_t)n * 4, st);
     if (e == hipSuccess) e = hipMemsetAsync(gmax, 0, (size_t)G * 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(tlen, 0, (size_t)n * 8, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks(G, 256)), dim3(256), 0, st, gmin, (long long)G, 0x7fffffff);
     hipLaunchKernelGGL(cand_groups_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand,
-                       (long long)m, d_order, gbase, cursor, c->d_len, gtgt, gmax, gmin);
+                       (long long)m, d_order, gbase, cursor, c->d_len, gtgt, gmax, gmin, tlen);
     hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, ibase, n, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
+    std::vector<unsigned long long> h_tlen(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, (size_t)n * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    for (int i = 0; i < n; ++i) stt.cells += (double)c->len[i] * (double)h_tlen[i];
     dev_free(cand);
     Work w;
     w.dev = true;
@@ -1234,6 +1242,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (nk > c->keys_cap) return fail(MNHN_ERR_NOMEM);
     c->last.kernel_ms = kernel_ms;
     c->last.pairs = (double)m;
+    c->last.cells = stt.cells;
+    stt.verify_kernel_ms = kernel_ms;
   }
   stt.verify_ms = phase_ms();
   stt.neighbours = (int64_t)(nk / 2);

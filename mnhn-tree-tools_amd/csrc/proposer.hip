@@ -223,13 +223,21 @@ __global__ __launch_bounds__(256) void pair_loss_kernel(const float *__restrict_
   if (lane == 0) atomicAdd(loss, L);
 }
 
-// db[c] = sum_r dY[r][c]
-__global__ void col_sum_kernel(const bf16 *__restrict__ dY, int R, int C, float *__restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
+// out[c] += sum_r dY[r][c] (out zeroed by the caller).  256 threads = 64
+// columns x 4 row lanes per block, 128 rows per block (grid.y), one atomic per
+// column per block.
+__global__ __launch_bounds__(256) void col_sum_kernel(const bf16 *__restrict__ dY, int R, int C,
+                                                      float *__restrict__ out) {
+  __shared__ float part[4][64];
+  const int cl = threadIdx.x & 63, rl = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  const int r0 = blockIdx.y * 128, r1 = min(R, r0 + 128);
   float s = 0.f;
-  for (int r = 0; r < R; ++r) s += (float)dY[(size_t)r * C + c];
-  out[c] = s;
+  if (c < C)
+    for (int r = r0 + rl; r < r1; r += 4) s += (float)dY[(size_t)r * C + c];
+  part[rl][cl] = s;
+  __syncthreads();
+  if (rl == 0 && c < C) atomicAdd(&out[c], part[0][cl] + part[1][cl] + part[2][cl] + part[3][cl]);
 }
 
 // Adam with decoupled weight decay wd (0 for biases)
@@ -885,21 +893,27 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       if ((rc = transpose(st, demb, R, D3, dembT)) || (rc = transpose(st, a.a2, R, D2, a2T))) break;
       Epi g3; g3.c32 = enc->L[2].gw; g3.ldc = D2;
       if ((rc = gemm(st, EPI_F32, dembT, R, a2T, R, D3, D2, R, g3))) break;
-      hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(D3, 128)), dim3(128), 0, st, demb, R, D3, enc->L[2].gb);
+      if (hipMemsetAsync(enc->L[2].gb, 0, D3 * sizeof(float), st) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+      hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((D3 + 63) / 64), (unsigned)((R + 127) / 128)), dim3(256), 0, st,
+                         demb, R, D3, enc->L[2].gb);
       Epi d2; d2.c16 = dp2; d2.ldc16 = D2; d2.pre = a.pre2; d2.ldp = D2;
       if ((rc = gemm(st, EPI_GELU_BWD, demb, D3, enc->L[2].w16, D3, R, D2, D3, d2))) break;
       // layer 2
       if ((rc = transpose(st, dp2, R, D2, dp2T)) || (rc = transpose(st, a.a1, R, D1, a1T))) break;
       Epi g2; g2.c32 = enc->L[1].gw; g2.ldc = D1;
       if ((rc = gemm(st, EPI_F32, dp2T, R, a1T, R, D2, D1, R, g2))) break;
-      hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(D2, 128)), dim3(128), 0, st, dp2, R, D2, enc->L[1].gb);
+      if (hipMemsetAsync(enc->L[1].gb, 0, D2 * sizeof(float), st) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+      hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((D2 + 63) / 64), (unsigned)((R + 127) / 128)), dim3(256), 0, st,
+                         dp2, R, D2, enc->L[1].gb);
       Epi d1; d1.c16 = dp1; d1.ldc16 = D1; d1.pre = a.pre1; d1.ldp = D1;
       if ((rc = gemm(st, EPI_GELU_BWD, dp2, D2, enc->L[1].w16, D2, R, D1, D2, d1))) break;
       // layer 1
       if ((rc = transpose(st, dp1, R, D1, dp1T)) || (rc = transpose(st, Xb, R, D0, XbT))) break;
       Epi g1; g1.c32 = enc->L[0].gw; g1.ldc = D0;
       if ((rc = gemm(st, EPI_F32, dp1T, R, XbT, R, D1, D0, R, g1))) break;
-      hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(D1, 128)), dim3(128), 0, st, dp1, R, D1, enc->L[0].gb);
+      if (hipMemsetAsync(enc->L[0].gb, 0, D1 * sizeof(float), st) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+      hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((D1 + 63) / 64), (unsigned)((R + 127) / 128)), dim3(256), 0, st,
+                         dp1, R, D1, enc->L[0].gb);
       // Adam
       enc->adam_t++;
       const float b1 = 0.9f, b2 = 0.999f;
@@ -1138,7 +1152,15 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   if ((rc = dev_alloc(&d_toff, (size_t)tiles + 1)) || (rc = dev_alloc(&d_cnt64, 1))) return fail(rc);
   if (hipMemcpyAsync(d_toff, toff.data(), ((size_t)tiles + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess)
     return fail(MNHN_ERR_DEVICE);
-  unsigned long long cap = std::max<unsigned long long>(1ull << 20, 64ull * (unsigned long long)n), m = 0;
+  // candidate buffer: the band's pair count, at most MNHN_PREFILTER_CAND_GB
+  // (default 8) GB; grown to the exact count and recomputed once on overflow
+  unsigned long long band_pairs = 0;
+  for (int ti = 0; ti < tiles; ++ti)
+    band_pairs += (unsigned long long)(toff[ti + 1] - toff[ti]) * 128ull * 128ull;
+  const char *cg = getenv("MNHN_PREFILTER_CAND_GB");
+  const double cand_gb = cg ? atof(cg) : 8.0;
+  unsigned long long cap = std::max<unsigned long long>(
+      1ull << 20, std::min<unsigned long long>(band_pairs, (unsigned long long)(cand_gb * 1e9 / 8.0))), m = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
     dev_free(cand);
     if ((rc = dev_alloc(&cand, cap))) break;

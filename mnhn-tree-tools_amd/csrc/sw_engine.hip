@@ -1664,9 +1664,10 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
   for (int w : {4, 8}) cands.push_back({KERNEL_REG, 32, w});
   for (int w : {8, 16}) cands.push_back({KERNEL_REG, 16, w});
   // sample of the all-pairs work: single-query items (one query against one
-  // target group) strided over all (group, query) pairs, about 8 per resident
-  // wave at the largest candidate occupancy -- a few hundred milliseconds of
-  // work per candidate whatever the data-set size
+  // target group) strided over all (group, query) pairs, about 32 per
+  // resident wave at the largest candidate occupancy (so that the queue's tail
+  // weighs little) -- a few hundred milliseconds of work per candidate
+  // whatever the data-set size
   std::vector<double> gsum(c->groups, 0.0);
   double units = 0;
   for (int g = 0; g < c->groups; ++g) {
@@ -1676,7 +1677,7 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
     }
     units += std::max(std::min((g + 1) * kGroup, c->n) - 1, 0);
   }
-  const double want = (double)std::max(c->num_cu, 1) * 24 * 8;
+  const double want = (double)std::max(c->num_cu, 1) * 24 * 32;
   const double stride = std::max(1.0, units / want);
   Work sample;
   sample.qids = c->order;
@@ -1697,7 +1698,7 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
     c->strip = cd.strip;
     c->waves_per_cu = cd.wpc;
     double ms = 1e30;
-    for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
+    for (int rep = 0; rep < 3; ++rep) {  // first run warms caches, code and clocks
       rc = launch_tiles(c, sample, MODE_EPS, 1, -1, nullptr, nullptr, 0);
       if (rc) break;
       ms = std::min(ms, c->last.kernel_ms);

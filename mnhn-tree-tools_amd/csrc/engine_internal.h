@@ -34,6 +34,10 @@ struct mnhn_sw_ctx {
   // device-resident epsilon lists of the last mnhn_sw_eps_count call
   unsigned long long *d_keys = nullptr;
   size_t keys_cap = 0;
+  // pinned staging buffers for large device-to-host copies (allocated on
+  // first use, kStageBytes each)
+  void *h_stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace mnhn_detail {
@@ -99,6 +103,10 @@ struct Work {
 // item list, or device-resident items and target groups when w.dev is set).
 int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
                  int32_t *d_dist, unsigned long long *d_keys, unsigned long long key_cap);
+
+// Device-to-host copy through the ctx's two pinned staging buffers: chunk k+1
+// is in flight on the ctx's stream while chunk k is copied to dst (pageable).
+int copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes);
 
 // Group the ctx's epsilon keys (c->d_keys, nk of them) into CSR rows.
 int csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out);

@@ -361,7 +361,9 @@ __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t 
 }
 
 // acc += A[i0 .. i0+128) . B[j0 .. j0+128)^T over k in [0, K) (row stride K),
-// staged through LDS 64 k at a time; each wave owns a 64 x 64 sub-tile.
+// staged through LDS 64 k at a time; each wave owns a 64 x 64 sub-tile.  The
+// next chunk's global loads are issued before the current chunk's MFMAs, so
+// their latency hides behind the math.
 template <int K>
 __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int n,
                                          int i0, int j0, bf16 *sA, bf16 *sB, f32x4 (&acc)[4][4]) {
@@ -369,22 +371,31 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 8 * (lane >> 4);
-  for (int kh = 0; kh < K; kh += KH) {
-    __syncthreads();
-    // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread
+  // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread
+  bf16x8 va[4], vb[4];
+  auto load = [&](int kh) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
       const int r = ch >> 3, k = (ch & 7) * 8;
-      bf16x8 va, vb;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) { va[q] = (bf16)0.0f; vb[q] = (bf16)0.0f; }
-      if (i0 + r < n) va = *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + r) * K + kh + k);
-      if (j0 + r < n) vb = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + r) * K + kh + k);
-      *reinterpret_cast<bf16x8 *>(sA + r * LK + k) = va;
-      *reinterpret_cast<bf16x8 *>(sB + r * LK + k) = vb;
+      for (int q = 0; q < 8; ++q) { va[t][q] = (bf16)0.0f; vb[t][q] = (bf16)0.0f; }
+      if (i0 + r < n) va[t] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + r) * K + kh + k);
+      if (j0 + r < n) vb[t] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + r) * K + kh + k);
+    }
+  };
+  load(0);
+  for (int kh = 0; kh < K; kh += KH) {
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ch = tid + 256 * t;
+      const int r = ch >> 3, k = (ch & 7) * 8;
+      *reinterpret_cast<bf16x8 *>(sA + r * LK + k) = va[t];
+      *reinterpret_cast<bf16x8 *>(sB + r * LK + k) = vb[t];
     }
     __syncthreads();
+    if (kh + KH < K) load(kh + KH);
 #pragma unroll
     for (int ks = 0; ks < KH; ks += 32) {
       bf16x8 af[4], bfr[4];

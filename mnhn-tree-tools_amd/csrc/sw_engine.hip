@@ -1239,6 +1239,10 @@ void mnhn_sw_destroy(mnhn_sw_ctx *c) {
   dev_free(c->d_bound); dev_free(c->d_keys);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; ++k) {
+    if (c->h_stage[k]) (void)hipHostFree(c->h_stage[k]);
+    if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1536,14 +1540,47 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
     out->col = (int32_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
     out->dist = (int16_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
     if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
-    e = hipMemcpy(out->col, d_col, nk * sizeof(int32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(out->dist, d_dist, nk * sizeof(int16_t), hipMemcpyDeviceToHost);
+    rc = copy_d2h(c, out->col, d_col, nk * sizeof(int32_t));
+    if (rc == MNHN_OK) rc = copy_d2h(c, out->dist, d_dist, nk * sizeof(int16_t));
+    if (rc != MNHN_OK) e = hipErrorUnknown;
   }
   cleanup();
   if (e != hipSuccess) { mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
   out->nnz = (int64_t)nk;
   c->last.kernel_ms = kernel_ms;
   c->last.total_ms = kernel_ms + post_ms;
+  return MNHN_OK;
+}
+
+int mnhn_detail::copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes) {
+  constexpr size_t kStageBytes = 64u << 20;
+  if (bytes <= kStageBytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (!c->h_stage[k] && hipHostMalloc(&c->h_stage[k], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+      c->h_stage[k] = nullptr;
+      return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
+    }
+    if (!c->stage_ev[k] && hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming) != hipSuccess) {
+      c->stage_ev[k] = nullptr;
+      return MNHN_ERR_DEVICE;
+    }
+  }
+  const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t k) -> bool {
+    const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    return hipMemcpyAsync(c->h_stage[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, c->stream) ==
+               hipSuccess &&
+           hipEventRecord(c->stage_ev[k & 1], c->stream) == hipSuccess;
+  };
+  if (!issue(0)) return MNHN_ERR_DEVICE;
+  for (size_t k = 0; k < chunks; ++k) {
+    if (k + 1 < chunks && !issue(k + 1)) return MNHN_ERR_DEVICE;
+    if (hipEventSynchronize(c->stage_ev[k & 1]) != hipSuccess) return MNHN_ERR_DEVICE;
+    const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    memcpy((char *)dst + off, c->h_stage[k & 1], len);
+  }
   return MNHN_OK;
 }
 

@@ -96,6 +96,7 @@ struct Work {
   // KERNEL_LDS_F16 device work: every pair of every item satisfies
   // match * min(la, lb) <= 2048 (otherwise the int16 LDS kernel runs it)
   bool dev_f16_ok = false;
+  int dev_strip = 0;  // strip width for device work (0: the context's)
 };
 
 
@@ -103,6 +104,9 @@ struct Work {
 // item list, or device-resident items and target groups when w.dev is set).
 int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
                  int32_t *d_dist, unsigned long long *d_keys, unsigned long long key_cap);
+
+// Register-strip width the context's launches use (tuning, environment or default).
+int context_strip(const mnhn_sw_ctx *c);
 
 // Device-to-host copy through the ctx's two pinned staging buffers: chunk k+1
 // is in flight on the ctx's stream while chunk k is copied to dst (pageable).

@@ -1210,15 +1210,26 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + (cnt[i] + kGroupSize - 1) / kGroupSize;
     const int G = gb[n];
-    // rows whose pairs all fit the exact fp16 form (the row's sequence is the
-    // longer of each of its pairs, so match * len <= 2048 suffices) take the
-    // first G16 items and run the fp16 kernel; the rest run the int16 kernel
+    // item order by row class: rows whose pairs all fit the exact fp16 form
+    // (the row's sequence is the longer of each of its pairs, so
+    // match * len <= 2048 suffices), first those short enough for a 32-column
+    // strip, then the rest; then the rows that need the int16 form
+    const int S = context_strip(c);
+    auto row_class = [&](int i) {
+      if ((long long)c->match * c->len[i] > 2048) return 2;
+      return (S == 64 && c->len[i] <= 128) ? 0 : 1;
+    };
     std::vector<int32_t> ib(n);
-    int G16 = 0;
-    for (int i = 0; i < n; ++i)
-      if ((long long)c->match * c->len[i] <= 2048) { ib[i] = G16; G16 += gb[i + 1] - gb[i]; }
-    for (int i = 0, o = G16; i < n; ++i)
-      if ((long long)c->match * c->len[i] > 2048) { ib[i] = o; o += gb[i + 1] - gb[i]; }
+    int part_begin[4] = {0, 0, 0, 0};
+    {
+      int o = 0;
+      for (int cls = 0; cls < 3; ++cls) {
+        part_begin[cls] = o;
+        for (int i = 0; i < n; ++i)
+          if (row_class(i) == cls) { ib[i] = o; o += gb[i + 1] - gb[i]; }
+      }
+      part_begin[3] = o;
+    }
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||
         (rc = dev_alloc(&ibase, (size_t)n)) || (rc = dev_alloc(&tlen, (size_t)n)))
@@ -1258,11 +1269,12 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
-      for (int part = 0; part < 2 && rc == MNHN_OK; ++part) {
-        w.d_items = part == 0 ? items : items + G16;
-        w.n_items = part == 0 ? G16 : G - G16;
-        w.dev_kernel = part == 0 ? KERNEL_LDS_F16 : KERNEL_LDS;
-        w.dev_f16_ok = part == 0;
+      for (int part = 0; part < 3 && rc == MNHN_OK; ++part) {
+        w.d_items = items + part_begin[part];
+        w.n_items = part_begin[part + 1] - part_begin[part];
+        w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : KERNEL_LDS;
+        w.dev_f16_ok = part < 2;
+        w.dev_strip = part == 0 ? 32 : 0;
         if (w.n_items == 0) continue;
         rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
         kernel_ms += c->last.kernel_ms;

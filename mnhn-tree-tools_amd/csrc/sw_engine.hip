@@ -886,9 +886,13 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   }
   // passes: (kernel, items).  The fp16 variant takes the items whose pairs
   // all satisfy match * min(la, lb) <= 2048; the rest run the int16 LDS kernel.
-  std::vector<std::pair<int, std::vector<int4>>> passes;
+  struct Pass {
+    int kernel, strip;
+    std::vector<int4> items;
+  };
+  std::vector<Pass> passes;
   if (w.dev) {
-    passes.push_back({(K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K, {}});
+    passes.push_back({(K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K, w.dev_strip ? w.dev_strip : S, {}});
   } else if (K == KERNEL_LDS_F16) {
     std::vector<int4> f16, i16;
     for (const int4 &it : w.items) {
@@ -898,10 +902,32 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
       const bool ok = (long long)c->match * gm <= 2048 || (long long)c->match * maxla <= 2048;
       (ok ? f16 : i16).push_back(it);
     }
-    if (!f16.empty()) passes.push_back({KERNEL_LDS_F16, std::move(f16)});
-    if (!i16.empty()) passes.push_back({KERNEL_LDS, std::move(i16)});
+    if (!f16.empty()) passes.push_back({KERNEL_LDS_F16, S, std::move(f16)});
+    if (!i16.empty()) passes.push_back({KERNEL_LDS, S, std::move(i16)});
   } else {
-    passes.push_back({K, w.items});
+    passes.push_back({K, S, w.items});
+  }
+  // LDS kernels at S = 64: items whose queries are all at most kShortQuery
+  // long run at S = 32 (right-aligned 64-column strips would pad a 90-residue
+  // query to 128 columns)
+  if (!w.dev && S == 64) {
+    constexpr int kShortQuery = 128;
+    std::vector<Pass> split;
+    for (Pass &ps : passes) {
+      if (ps.kernel != KERNEL_LDS && ps.kernel != KERNEL_LDS_F16) {
+        split.push_back(std::move(ps));
+        continue;
+      }
+      std::vector<int4> shorter, rest;
+      for (const int4 &it : ps.items) {
+        int maxla = 0;
+        for (int qi = it.y; qi < it.z; ++qi) maxla = std::max(maxla, c->len[w.qids[qi]]);
+        (maxla <= kShortQuery ? shorter : rest).push_back(it);
+      }
+      if (!rest.empty()) split.push_back({ps.kernel, S, std::move(rest)});
+      if (!shorter.empty()) split.push_back({ps.kernel, 32, std::move(shorter)});
+    }
+    passes.swap(split);
   }
 
   // strip-boundary workspace, [row][lane] per wave.  Rows are query residues
@@ -911,14 +937,14 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   int waves_used = 0;
   (void)hipEventRecord(c->ev[0], c->stream);
   for (auto &pass : passes) {
-    const int PK = pass.first;
-    const std::vector<int4> &items = pass.second;
+    const int PK = pass.kernel, PS = pass.strip;
+    const std::vector<int4> &items = pass.items;
     const size_t n_items = w.dev ? (size_t)w.n_items : items.size();
-    void *kp = kernel_ptr(PK, S, c->acgt);
+    void *kp = kernel_ptr(PK, PS, c->acgt);
     int wpc = wpc_req;
     // measured on MI355X (config-2 workload): the LDS kernels at S = 32 run
     // best with 3 of their 4 possible waves per SIMD resident
-    if (wpc <= 0 && (PK == KERNEL_LDS || PK == KERNEL_LDS_F16) && S == 32) wpc = 12;
+    if (wpc <= 0 && (PK == KERNEL_LDS || PK == KERNEL_LDS_F16) && PS == 32) wpc = 12;
     if (wpc <= 0) {
       int blocks_per_cu = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) !=
@@ -1551,6 +1577,8 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
   c->last.total_ms = kernel_ms + post_ms;
   return MNHN_OK;
 }
+
+int mnhn_detail::context_strip(const mnhn_sw_ctx *c) { return pick_strip(c); }
 
 int mnhn_detail::copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes) {
   constexpr size_t kStageBytes = 64u << 20;

@@ -98,14 +98,23 @@ def test_training_reduces_loss(pkg):
     assert de[fam[a] == fam[b]].mean() < 0.5 * de[fam[a] != fam[b]].mean()
 
 
-def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg):
+@pytest.mark.parametrize("tuning", [None, (3, 64, 8), (2, 16, 16)])
+def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, tuning):
     """A huge margin proposes every pair that passes the exact length bound,
-    so the verified lists must equal the exact all-pairs lists."""
-    seqs = synth.config2(seed=44, n_roots=6, n_children=4, n_leaves=10)
+    so the verified lists must equal the exact all-pairs lists (verification
+    in every kernel form and strip width, incl. the 32-column strip the short
+    rows take when the context runs at 64)."""
+    rng = np.random.default_rng(44)
+    seqs = []
+    for L0 in (70, 90, 110, 125, 140, 300, 520, 560):  # short, mid and int16-only rows
+        anc = synth.random_codes(rng, L0)
+        seqs += [synth.to_bytes(synth.mutate(rng, anc, 0.03, 0.01)) for _ in range(30)]
     eps = 350
-    _say(f"prefilter (unbounded radius) n={len(seqs)}")
+    _say(f"prefilter (unbounded radius) n={len(seqs)} tuning={tuning}")
     L = np.array([len(s) for s in seqs])
     with pkg.Engine(seqs) as eng, pkg.Encoder(seed=3) as enc:
+        if tuning:
+            eng.set_tuning(*tuning)
         enc.margin = 1e4
         rp, col, dist, st = enc.prefilter_csr(eng, eps)
         erp, ecol, edist = eng.eps_csr(eps)

@@ -231,16 +231,7 @@ class Engine:
         csr = Csr()
         _check(self.lib.mnhn_sw_eps_csr(self.ctx, int(eps_max), ctypes.byref(csr)),
                "mnhn_sw_eps_csr")
-        try:
-            nnz = int(csr.nnz)
-            rp = np.ctypeslib.as_array(csr.row_ptr, shape=(self.n + 1,)).copy()
-            col = (np.ctypeslib.as_array(csr.col, shape=(nnz,)).copy() if nnz
-                   else np.zeros(0, np.int32))
-            dist = (np.ctypeslib.as_array(csr.dist, shape=(nnz,)).copy() if nnz
-                    else np.zeros(0, np.int16))
-        finally:
-            self.lib.mnhn_csr_free(ctypes.byref(csr))
-        return rp, col, dist
+        return _csr_to_numpy(self.lib, csr, self.n)
 
     def eps_count(self, eps_max: int) -> int:
         v = ctypes.c_int64(0)
@@ -260,16 +251,37 @@ class Engine:
         return out
 
 
+class _CsrOwner:
+    """Keeps a library-owned CSR alive; frees it with mnhn_csr_free when the
+    last array viewing its buffers is gone."""
+
+    def __init__(self, lib, csr):
+        self.lib, self.csr = lib, csr
+
+    def __del__(self):
+        try:
+            self.lib.mnhn_csr_free(ctypes.byref(self.csr))
+        except Exception:
+            pass
+
+
+class _OwnedArray(np.ndarray):
+    """ndarray view over library memory that holds the memory's owner."""
+
+
+def _view(ptr, n, owner):
+    a = np.ctypeslib.as_array(ptr, shape=(n,)).view(_OwnedArray)
+    a._owner = owner
+    return a
+
+
 def _csr_to_numpy(lib, csr, n):
-    try:
-        nnz = int(csr.nnz)
-        rp = np.ctypeslib.as_array(csr.row_ptr, shape=(n + 1,)).copy()
-        col = (np.ctypeslib.as_array(csr.col, shape=(nnz,)).copy() if nnz
-               else np.zeros(0, np.int32))
-        dist = (np.ctypeslib.as_array(csr.dist, shape=(nnz,)).copy() if nnz
-                else np.zeros(0, np.int16))
-    finally:
-        lib.mnhn_csr_free(ctypes.byref(csr))
+    """(row_ptr, col, dist) as zero-copy views of the library's CSR buffers."""
+    owner = _CsrOwner(lib, csr)
+    nnz = int(csr.nnz)
+    rp = _view(csr.row_ptr, n + 1, owner)
+    col = _view(csr.col, nnz, owner) if nnz else np.zeros(0, np.int32)
+    dist = _view(csr.dist, nnz, owner) if nnz else np.zeros(0, np.int16)
     return rp, col, dist
 
 

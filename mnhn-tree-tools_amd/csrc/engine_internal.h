@@ -105,6 +105,14 @@ struct Work {
 int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
                  int32_t *d_dist, unsigned long long *d_keys, unsigned long long key_cap);
 
+// Sort each row segment [d_rp[r], d_rp[r+1]) of distinct column values in
+// [0, n_cols) ascending into out (same positions), on the ctx's stream.
+int sort_row_segments(mnhn_sw_ctx *c, const long long *d_rp, const int32_t *seg, int32_t *out, int n_rows,
+                      int n_cols);
+
+// Host allocation for large result arrays (free() releases it).
+void *host_alloc(size_t bytes);
+
 // Register-strip width the context's launches use (tuning, environment or default).
 int context_strip(const mnhn_sw_ctx *c);
 

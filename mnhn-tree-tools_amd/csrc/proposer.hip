@@ -561,23 +561,50 @@ __global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long
     atomicAdd(&cnt[order[(int)(c[k] >> 32)]], 1);
 }
 
-// scatter candidates into their row's target groups of 128; tlen[i] sums the
-// target lengths of row i (for the cell count)
-__global__ void cand_groups_kernel(const unsigned long long *__restrict__ c, long long m,
-                                   const int32_t *__restrict__ order, const int32_t *__restrict__ gbase,
-                                   int32_t *__restrict__ cursor, const int32_t *__restrict__ len,
-                                   int32_t *__restrict__ gtgt, int32_t *__restrict__ gmax,
-                                   int32_t *__restrict__ gmin, unsigned long long *__restrict__ tlen) {
+// scatter candidates (sorted positions p << 32 | q) into the row segments of
+// the longer sequence's id order[p]: seg[rs[i] ..] holds the q of row i
+__global__ void cand_scatter_kernel(const unsigned long long *__restrict__ c, long long m,
+                                    const int32_t *__restrict__ order, const long long *__restrict__ rs,
+                                    int32_t *__restrict__ cursor, int32_t *__restrict__ seg) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
        k += (long long)gridDim.x * blockDim.x) {
-    const int i = order[(int)(c[k] >> 32)], j = order[(int)(c[k] & 0xffffffffull)];
-    const int o = atomicAdd(&cursor[i], 1);
-    const int g = gbase[i] + o / kGroupSize;
-    gtgt[(size_t)g * kGroupSize + (o % kGroupSize)] = j;
-    atomicMax(&gmax[g], len[j]);
-    atomicMin(&gmin[g], len[j]);
-    atomicAdd(&tlen[i], (unsigned long long)len[j]);
+    const int i = order[(int)(c[k] >> 32)];
+    seg[rs[i] + atomicAdd(&cursor[i], 1)] = (int32_t)(c[k] & 0xffffffffull);
   }
+}
+
+// One workgroup per row i: its ascending q (= descending target length) in
+// target groups of 128, so each group holds targets of similar length; the
+// group's first target is its longest, its last the shortest.  tlen[i] = sum
+// of the row's target lengths.
+__global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restrict__ sorted,
+                                                         const long long *__restrict__ rs,
+                                                         const int32_t *__restrict__ gbase,
+                                                         const int32_t *__restrict__ order,
+                                                         const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
+                                                         int32_t *__restrict__ gmax, int32_t *__restrict__ gmin,
+                                                         unsigned long long *__restrict__ tlen) {
+  __shared__ unsigned long long red[256];
+  const int i = blockIdx.x;
+  const long long b = rs[i];
+  const int cnt = (int)(rs[i + 1] - b);
+  unsigned long long sum = 0;
+  for (int k = threadIdx.x; k < cnt; k += 256) {
+    const int j = order[sorted[b + k]];
+    const int g = gbase[i] + k / kGroupSize, slot = k % kGroupSize;
+    const int lj = len[j];
+    gtgt[(size_t)g * kGroupSize + slot] = j;
+    if (slot == 0) gmax[g] = lj;
+    if (slot == kGroupSize - 1 || k == cnt - 1) gmin[g] = lj;
+    sum += (unsigned long long)lj;
+  }
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tlen[i] = red[0];
 }
 
 // items of row i (one per target group) at ibase[i] ..
@@ -1092,15 +1119,16 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr,
-          *ibase = nullptr;
+          *ibase = nullptr, *seg = nullptr, *sorted = nullptr;
   unsigned long long *tlen = nullptr;
+  long long *d_rs = nullptr;
   int4 *items = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
     dev_free(Xs); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
-    dev_free(ibase); dev_free(tlen);
+    dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   };
@@ -1230,20 +1258,30 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       }
       part_begin[3] = o;
     }
+    // row segments of the candidates, sorted by target position within rows
+    std::vector<long long> rsh((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) rsh[i + 1] = rsh[i] + cnt[i];
+    if ((rc = dev_alloc(&d_rs, (size_t)n + 1)) || (rc = dev_alloc(&seg, (size_t)m))) return fail(rc);
+    e = hipMemcpyAsync(d_rs, rsh.data(), ((size_t)n + 1) * sizeof(long long), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, (size_t)n * 4, st);
+    if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    hipLaunchKernelGGL(cand_scatter_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand,
+                       (long long)m, d_order, d_rs, cursor, seg);
+    if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    dev_free(cand);
+    if ((rc = dev_alloc(&sorted, (size_t)m))) return fail(rc);
+    if ((rc = sort_row_segments(c, d_rs, seg, sorted, n, n))) return fail(rc);
+    dev_free(seg);
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||
         (rc = dev_alloc(&ibase, (size_t)n)) || (rc = dev_alloc(&tlen, (size_t)n)))
       return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemsetAsync(cursor, 0, (size_t)n * 4, st);
-    if (e == hipSuccess) e = hipMemsetAsync(gmax, 0, (size_t)G * 4, st);
-    if (e == hipSuccess) e = hipMemsetAsync(tlen, 0, (size_t)n * 8, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks(G, 256)), dim3(256), 0, st, gmin, (long long)G, 0x7fffffff);
-    hipLaunchKernelGGL(cand_groups_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand,
-                       (long long)m, d_order, gbase, cursor, c->d_len, gtgt, gmax, gmin, tlen);
+    hipLaunchKernelGGL(row_groups_kernel, dim3((unsigned)n), dim3(256), 0, st, sorted, d_rs, gbase, d_order,
+                       c->d_len, gtgt, gmax, gmin, tlen);
     hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, ibase, n, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
@@ -1252,7 +1290,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     for (int i = 0; i < n; ++i) stt.cells += (double)c->len[i] * (double)h_tlen[i];
-    dev_free(cand);
+    dev_free(sorted);
     Work w;
     w.dev = true;
     w.d_qids = qids;

@@ -34,6 +34,7 @@
 //    column order by ranking its columns (comparison ranks for short rows,
 //    a column bitmap + popcount prefix in LDS for long ones).
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <cmath>
@@ -716,7 +717,8 @@ constexpr int kSortThreads = 256;
 constexpr int kShortRow = 256;
 
 // One workgroup per row: columns are distinct, so each entry's position in
-// the sorted row is the number of smaller columns in the row.
+// the sorted row is the number of smaller columns in the row.  seg_d / out_d
+// may be null (columns only).
 __global__ __launch_bounds__(kSortThreads) void sort_rows_kernel(
     const long long *row_ptr, const int32_t *seg_col, const int16_t *seg_d,
     int32_t *out_col, int16_t *out_d, int n_cols) {
@@ -734,7 +736,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_rows_kernel(
       int r = 0;
       for (int m = 0; m < len; ++m) r += lds[m] < c ? 1 : 0;
       out_col[b + r] = (int32_t)c;
-      out_d[b + r] = seg_d[b + t];
+      if (seg_d) out_d[b + r] = seg_d[b + t];
     }
     return;
   }
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_rows_kernel(
     const uint32_t w = c >> 5;
     const uint32_t r = prefix[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
     out_col[b + r] = (int32_t)c;
-    out_d[b + r] = seg_d[b + t];
+    if (seg_d) out_d[b + r] = seg_d[b + t];
   }
 }
 
@@ -1563,8 +1565,8 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
   float post_ms = 0.f;
   if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
   if (e == hipSuccess) {
-    out->col = (int32_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
-    out->dist = (int16_t *)malloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
+    out->col = (int32_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
+    out->dist = (int16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
     if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
     rc = copy_d2h(c, out->col, d_col, nk * sizeof(int32_t));
     if (rc == MNHN_OK) rc = copy_d2h(c, out->dist, d_dist, nk * sizeof(int16_t));
@@ -1579,6 +1581,35 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
 }
 
 int mnhn_detail::context_strip(const mnhn_sw_ctx *c) { return pick_strip(c); }
+
+int mnhn_detail::sort_row_segments(mnhn_sw_ctx *c, const long long *d_rp, const int32_t *seg, int32_t *out,
+                                   int n_rows, int n_cols) {
+  const size_t lds_bytes = std::max<size_t>(
+      (size_t)(2 * ((n_cols + 31) / 32) + kSortThreads) * sizeof(uint32_t), kShortRow * sizeof(uint32_t));
+  if (lds_bytes > 160 * 1024) return MNHN_ERR_RANGE;
+  hipError_t e = hipSuccess;
+  if (lds_bytes > 64 * 1024)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e == hipSuccess && n_rows > 0) {
+    hipLaunchKernelGGL(sort_rows_kernel, dim3((unsigned)n_rows), dim3(kSortThreads), lds_bytes, c->stream, d_rp,
+                       seg, (const int16_t *)nullptr, out, (int16_t *)nullptr, n_cols);
+    e = hipGetLastError();
+  }
+  return e == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
+}
+
+// Host buffers of the CSR results (released with free()).  Large ones are
+// 2 MB-aligned and marked for transparent huge pages, so filling them takes a
+// page fault per 2 MB instead of per 4 KB.
+void *mnhn_detail::host_alloc(size_t bytes) {
+  constexpr size_t kHuge = 2u << 20;
+  if (bytes < 8 * kHuge) return malloc(bytes);
+  const size_t rounded = (bytes + kHuge - 1) / kHuge * kHuge;
+  void *p = aligned_alloc(kHuge, rounded);
+  if (p) (void)madvise(p, rounded, MADV_HUGEPAGE);
+  return p;
+}
 
 int mnhn_detail::copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes) {
   constexpr size_t kStageBytes = 64u << 20;

@@ -22,6 +22,7 @@
 #define MNHN_HOST_H
 
 #include <stddef.h>
+#include <stdint.h>
 #include <stdio.h>
 
 #include "mnhn_sw.h"
@@ -101,6 +102,18 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
  * learned prefilter; requires eps <= csr->eps_max.  No counterpart in the
  * reference, whose passes always recompute distances. */
 split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts);
+
+/* ---- persisted epsilon lists (.ecsr) ----
+ * No counterpart in the reference, which recomputes every distance per run.
+ * File: "MNHNECSR", uint32 version 1, int32 n_rows, int32 eps_max, int32 0,
+ * int64 nnz, uint64 data_hash (mnhn_sw_data_hash), int64 row_ptr[n_rows+1],
+ * int32 col[nnz], int16 dist[nnz]; little-endian.  Loaded lists are released
+ * with mnhn_csr_free.  Both return MNHN_OK or MNHN_ERR_ARG (I/O error, bad or
+ * truncated file).  With MNHN_ECSR=<path> in the environment, the DBSCAN
+ * entry points load the lists from that file when it matches the data and
+ * covers the epsilon, and otherwise compute them and write the file. */
+int mnhn_csr_save(const mnhn_csr *csr, uint64_t data_hash, const char *path);
+int mnhn_csr_load(const char *path, uint64_t *data_hash, mnhn_csr *out);
 
 void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
                      dataset ds, float epsilon_start, float epsilon_inc, int minpts,

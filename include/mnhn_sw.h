@@ -115,6 +115,10 @@ void mnhn_csr_free(mnhn_csr *csr);
 int mnhn_sw_size(const mnhn_sw_ctx *ctx);
 int mnhn_sw_alphabet(const mnhn_sw_ctx *ctx);
 
+/* 64-bit FNV-1a of the scores, the sequence count and every sequence's length
+ * and bytes: identifies the data a saved epsilon-list file belongs to. */
+uint64_t mnhn_sw_data_hash(const mnhn_sw_ctx *ctx);
+
 /* Timing of the last one_vs_all / pairs / matrix / eps call. */
 int mnhn_sw_last_timing(const mnhn_sw_ctx *ctx, mnhn_sw_timing *out);
 

@@ -30,7 +30,7 @@ ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of memory",
 ABI_SYMBOLS = [
     "mnhn_sw_create", "mnhn_sw_destroy", "mnhn_sw_one_vs_all", "mnhn_sw_pairs",
     "mnhn_sw_matrix", "mnhn_sw_eps_csr", "mnhn_sw_eps_count", "mnhn_csr_free",
-    "mnhn_sw_size", "mnhn_sw_alphabet", "mnhn_sw_last_timing",
+    "mnhn_sw_size", "mnhn_sw_data_hash", "mnhn_sw_alphabet", "mnhn_sw_last_timing",
     "mnhn_sw_set_tuning", "mnhn_sw_autotune", "mnhn_sw_strerror",
 ]
 PROPOSER_SYMBOLS = [
@@ -45,7 +45,7 @@ HOST_SYMBOLS = [
     "data_not_in_clusters", "intersection_of_clusters",
     "generate_split_set_relation", "free_split_set_and_associated_clusters",
     "mnhn_gpu_initialization", "mnhn_gpu_destroy", "mnhn_gpu_set_lookahead",
-    "dbscan_SW_GPU", "mnhn_dbscan_csr", "adaptive_dbscan",
+    "dbscan_SW_GPU", "mnhn_dbscan_csr", "mnhn_csr_save", "mnhn_csr_load", "adaptive_dbscan",
 ]
 
 
@@ -125,6 +125,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_csr_free.argtypes = [ctypes.POINTER(Csr)]
     lib.mnhn_csr_free.restype = None
     lib.mnhn_sw_size.argtypes = [vp]
+    lib.mnhn_sw_data_hash.argtypes = [vp]
+    lib.mnhn_sw_data_hash.restype = ctypes.c_uint64
+    lib.mnhn_csr_save.argtypes = [ctypes.POINTER(Csr), ctypes.c_uint64, ctypes.c_char_p]
+    lib.mnhn_csr_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Csr)]
     lib.mnhn_sw_alphabet.argtypes = [vp]
     lib.mnhn_sw_last_timing.argtypes = [vp, ctypes.POINTER(Timing)]
     lib.mnhn_sw_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -225,6 +229,11 @@ class Engine:
         out = np.empty((self.n, self.n), dtype=np.int32)
         _check(self.lib.mnhn_sw_matrix(self.ctx, out.ctypes.data), "mnhn_sw_matrix")
         return out
+
+    @property
+    def data_hash(self) -> int:
+        """FNV-1a of scores, count, lengths and bytes (the .ecsr file key)."""
+        return int(self.lib.mnhn_sw_data_hash(self.ctx))
 
     def eps_csr(self, eps_max: int):
         """Returns (row_ptr int64[n+1], col int32[nnz], dist int16[nnz])."""
@@ -439,3 +448,30 @@ def dbscan_csr(rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int, 
     finally:
         lib.free_split_set_and_associated_clusters(s)
     return out
+
+
+def csr_save(path: str, rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int,
+             data_hash: int) -> None:
+    """Write epsilon lists to an .ecsr file (mnhn_csr_save)."""
+    lib = load_library()
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    dist = np.ascontiguousarray(dist, dtype=np.int16)
+    csr = Csr(len(rp) - 1, int(eps_max), len(col),
+              rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+              col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+    _check(lib.mnhn_csr_save(ctypes.byref(csr), ctypes.c_uint64(data_hash), path.encode()),
+           "mnhn_csr_save")
+
+
+def csr_load(path: str):
+    """Read an .ecsr file: (row_ptr, col, dist, eps_max, data_hash)."""
+    lib = load_library()
+    csr = Csr()
+    h = ctypes.c_uint64(0)
+    _check(lib.mnhn_csr_load(path.encode(), ctypes.byref(h), ctypes.byref(csr)), "mnhn_csr_load")
+    eps_max = int(csr.eps_max)
+    rp, col, dist = _csr_to_numpy(lib, csr, int(csr.n_rows))
+    return rp, col, dist, eps_max, int(h.value)
+

@@ -34,6 +34,7 @@ struct mnhn_sw_ctx {
   // device-resident epsilon lists of the last mnhn_sw_eps_count call
   unsigned long long *d_keys = nullptr;
   size_t keys_cap = 0;
+  uint64_t data_hash = 0;  // scores, count, lengths and bytes of the sequences
   // pinned staging buffers for large device-to-host copies (allocated on
   // first use, kStageBytes each)
   void *h_stage[2] = {nullptr, nullptr};

@@ -93,9 +93,28 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
   double want = (double)need + (double)c->lookahead;
   if (need < 0) want = -1;
   int32_t eps_max = want > 2.0e9 ? INT32_MAX : (int32_t)floor(want);
+  const char *path = getenv("MNHN_ECSR");
+  const uint64_t hash = mnhn_sw_data_hash(g.sw);
+  if (path && *path) {
+    uint64_t fh = 0;
+    if (mnhn_csr_load(path, &fh, &c->csr) == MNHN_OK) {
+      if (fh == hash && c->csr.n_rows == mnhn_sw_size(g.sw) && c->csr.eps_max >= need) {
+        fprintf(stderr, "epsilon lists (eps_max %d) read from %s\n", (int)c->csr.eps_max, path);
+        c->have = 1;
+        return;
+      }
+      mnhn_csr_free(&c->csr);
+    }
+  }
   int rc = mnhn_sw_eps_csr(g.sw, eps_max, &c->csr);
   if (rc) die_engine("mnhn_sw_eps_csr", rc);
   c->have = 1;
+  if (path && *path) {
+    if (mnhn_csr_save(&c->csr, hash, path) == MNHN_OK)
+      fprintf(stderr, "epsilon lists (eps_max %d) written to %s\n", (int)eps_max, path);
+    else
+      fprintf(stderr, "could not write epsilon lists to %s\n", path);
+  }
 }
 
 /* N(p) from epsilon lists (ascending, p included); returns |N(p)|. */

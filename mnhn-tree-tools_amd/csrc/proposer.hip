@@ -552,13 +552,43 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
       }
 }
 
+// Wave-aggregated per-row atomics: candidates next to each other in the
+// buffer mostly share a row (a wave's 64 hits of one emission step cover at
+// most 4 rows), so each distinct row of the wave takes one atomic; every lane
+// gets its own slot (base + its rank among the lanes of its row).
+__device__ __forceinline__ int row_slot(int32_t *__restrict__ ctr, int row, bool valid) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(valid);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lrow = __shfl(row, leader);
+    const unsigned long long same = __ballot(valid && row == lrow);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lrow], __popcll(same));
+    base = __shfl(base, leader);
+    if (valid && row == lrow)
+      slot = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(same >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)same, 0u));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 // candidate keys (sorted positions p << 32 | q) -> per-row counts of the
-// longer sequence's id order[p]
+// longer sequence's id order[p].  The loop runs whole waves to the end (the
+// aggregation needs every lane).
 __global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long long m,
                                   const int32_t *__restrict__ order, int32_t *__restrict__ cnt) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
-       k += (long long)gridDim.x * blockDim.x)
-    atomicAdd(&cnt[order[(int)(c[k] >> 32)]], 1);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long first = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long wave0 = first - (threadIdx.x & 63);
+  for (long long w0 = wave0; w0 < m; w0 += stride) {
+    const long long k = w0 + (threadIdx.x & 63);
+    const bool valid = k < m;
+    const int i = valid ? order[(int)(c[k] >> 32)] : 0;
+    (void)row_slot(cnt, i, valid);
+  }
 }
 
 // scatter candidates (sorted positions p << 32 | q) into the row segments of
@@ -566,10 +596,16 @@ __global__ void cand_count_kernel(const unsigned long long *__restrict__ c, long
 __global__ void cand_scatter_kernel(const unsigned long long *__restrict__ c, long long m,
                                     const int32_t *__restrict__ order, const long long *__restrict__ rs,
                                     int32_t *__restrict__ cursor, int32_t *__restrict__ seg) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
-       k += (long long)gridDim.x * blockDim.x) {
-    const int i = order[(int)(c[k] >> 32)];
-    seg[rs[i] + atomicAdd(&cursor[i], 1)] = (int32_t)(c[k] & 0xffffffffull);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long first = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long wave0 = first - (threadIdx.x & 63);
+  for (long long w0 = wave0; w0 < m; w0 += stride) {
+    const long long k = w0 + (threadIdx.x & 63);
+    const bool valid = k < m;
+    const unsigned long long key = valid ? c[k] : 0ull;
+    const int i = valid ? order[(int)(key >> 32)] : 0;
+    const int slot = row_slot(cursor, i, valid);
+    if (valid) seg[rs[i] + slot] = (int32_t)(key & 0xffffffffull);
   }
 }
 

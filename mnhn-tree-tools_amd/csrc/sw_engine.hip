@@ -1163,6 +1163,22 @@ int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
     c->len[i] = (int32_t)lens[i];
     c->maxlen = std::max(c->maxlen, c->len[i]);
   }
+  // FNV-1a over the scores, the count and every sequence's length and bytes
+  {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void *p, size_t k) {
+      const unsigned char *b = (const unsigned char *)p;
+      for (size_t t = 0; t < k; ++t) { h ^= b[t]; h *= 1099511628211ull; }
+    };
+    const int32_t hdr[4] = {c->match, c->mismatch, c->gap, n};
+    mix(hdr, sizeof(hdr));
+    for (int i = 0; i < n; ++i) {
+      const uint64_t L = (uint64_t)lens[i];
+      mix(&L, sizeof(L));
+      if (L) mix(seqs[i], (size_t)L);
+    }
+    c->data_hash = h;
+  }
   // alphabet of the compared residues (index >= 1)
   bool seen[256] = {false};
   int distinct = 0;
@@ -1276,6 +1292,7 @@ void mnhn_sw_destroy(mnhn_sw_ctx *c) {
 }
 
 int mnhn_sw_size(const mnhn_sw_ctx *c) { return c ? c->n : -1; }
+uint64_t mnhn_sw_data_hash(const mnhn_sw_ctx *c) { return c ? c->data_hash : 0; }
 int mnhn_sw_alphabet(const mnhn_sw_ctx *c) { return c ? (c->acgt ? 4 : 0) : -1; }
 
 int mnhn_sw_last_timing(const mnhn_sw_ctx *c, mnhn_sw_timing *out) {

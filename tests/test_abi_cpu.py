@@ -205,3 +205,30 @@ def test_dbscan_over_csr_matches_oracle(pkg, oracle):
         got = pkg.dbscan_csr(rp, col, d, eps_max, eps, minpts)
         want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(dist))
         assert got == [c.members for c in want.clusters], (eps, minpts)
+
+
+def test_ecsr_roundtrip_and_rejects_damaged_files(pkg, tmp_path):
+    """.ecsr files (host code only): exact round trip, and truncated files,
+    a wrong magic and out-of-range columns are refused."""
+    rng = np.random.default_rng(5)
+    n = 50
+    counts = rng.integers(0, 7, n)
+    rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(n, c, replace=False)) for c in counts]).astype(np.int32)
+    dist = rng.integers(0, 3000, len(col)).astype(np.int16)
+    path = str(tmp_path / "a.ecsr")
+    pkg.csr_save(path, rp, col, dist, 2999, 0x1234567890ABCDEF)
+    r2, c2, d2, e2, h2 = pkg.csr_load(path)
+    assert np.array_equal(r2, rp) and np.array_equal(c2, col) and np.array_equal(d2, dist)
+    assert (e2, h2) == (2999, 0x1234567890ABCDEF)
+    data = open(path, "rb").read()
+    assert data[:8] == b"MNHNECSR" and len(data) == 40 + 8 * (n + 1) + 6 * len(col)
+    bad = tmp_path / "b.ecsr"
+    for blob in (data[:-1], b"X" + data[1:], data[:40 + 8 * (n + 1)] + struct.pack("<i", n) + data[44 + 8 * (n + 1):]):
+        bad.write_bytes(blob)
+        with pytest.raises(pkg.MnhnError):
+            pkg.csr_load(str(bad))
+    # empty lists
+    pkg.csr_save(path, np.zeros(4, np.int64), np.zeros(0, np.int32), np.zeros(0, np.int16), 10, 7)
+    r3, c3, d3, e3, h3 = pkg.csr_load(path)
+    assert len(r3) == 4 and len(c3) == 0 and (e3, h3) == (10, 7)

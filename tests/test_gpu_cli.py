@@ -65,3 +65,39 @@ def test_split_set_no_truncate(tmp_path):
     data = (tmp_path / first).read_bytes()
     assert data[:len(files[first])] == files[first]
     assert data[len(files[first]):] == junk[len(files[first]):]
+
+
+@pytest.mark.parametrize("name", ["dbscan_c1small_e400", "adaptive_testxs33"])
+def test_cli_persisted_eps_lists(name, tmp_path):
+    """MNHN_ECSR: the first run computes and writes the lists, the second reads
+    them instead of running Smith-Waterman; both outputs stay byte-identical to
+    the reference's; a file from other data is not used."""
+    meta = dict(cli_cases())[name]
+    path = str(tmp_path.parent / f"{name}.ecsr")
+    for k, msg in ((0, b"written to"), (1, b"read from")):
+        run_dir = tmp_path / f"run{k}"
+        run_dir.mkdir()
+        stdout, files = read_case_files(name, meta)
+        exe = os.path.join(PKG_DIR, "bin", meta["program"])
+        fasta = run_dir / "in.fasta"
+        shutil.copyfile(os.path.join(GOLDEN, "inputs", meta["input"]), fasta)
+        if meta["program"] == "cluster_dbscan_SW":
+            argv = [exe, str(fasta)] + meta["args"] + [meta["prefix"]]
+        else:
+            argv = [exe, str(fasta)] + [a if a != "PREFIX" else meta["prefix"] for a in meta["args"]]
+        env = dict(os.environ, MNHN_ECSR=path)
+        print(f"ecsr case {name} run {k}", flush=True)
+        r = subprocess.run(argv, cwd=run_dir, capture_output=True, env=env, timeout=300)
+        assert r.returncode == meta["returncode"]
+        assert msg in r.stderr, r.stderr.decode()[-500:]
+        assert r.stdout == stdout
+        for fn, data in files.items():
+            assert (run_dir / fn).read_bytes() == data, fn
+    # other data (one sequence dropped): the file does not match, lists are recomputed
+    other = tmp_path / "other.fasta"
+    seqs = open(os.path.join(GOLDEN, "inputs", meta["input"]), "rb").read().split(b">")[1:]
+    other.write_bytes(b"".join(b">" + s for s in seqs[:-1]))
+    exe = os.path.join(PKG_DIR, "bin", "cluster_dbscan_SW")
+    r = subprocess.run([exe, str(other), "400", "5", "o"], cwd=tmp_path, capture_output=True,
+                       env=dict(os.environ, MNHN_ECSR=path), timeout=300)
+    assert r.returncode == 0 and b"written to" in r.stderr

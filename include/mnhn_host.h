@@ -103,6 +103,31 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
  * reference, whose passes always recompute distances. */
 split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts);
 
+/* ---- the other Smith-Waterman consumers (distances from the gfx950 engine;
+ * n_threads arguments are accepted and unused: one device pass) ---- */
+/* dataset.c:85-110: ">sequence_<j>" records, lines wrapped at 50 residues */
+void dataset_to_fasta(FILE *f, dataset ds);
+/* filter.c:18-48: lengths in [size - minus, size + plus], size_t arithmetic */
+dataset filter_ds_by_size(dataset in_set, size_t size, size_t plus, size_t minus);
+/* filter.c:73-140: sequences with d(seq, target) <= max_distance, in order */
+dataset filter_by_SW_distance(dataset in_set, char *target, size_t target_length,
+                              unsigned int max_distance, int n_threads);
+/* comparison.c:77-103: n1 x n2 distances, row-major */
+unsigned long *smith_waterman_distances_matrix(dataset ds_one, dataset ds_two);
+/* comparison.c:152-188, 240-277, 279-313 (accumulators as in the reference's
+ * one-thread run) */
+double mean_from_smith_waterman_datasets(dataset ds_one, dataset ds_two, size_t n_threads);
+double sigma_from_smith_waterman_datasets(double mean, dataset ds_one, dataset ds_two,
+                                          size_t n_threads);
+double silhouette_from_smith_waterman_datasets(dataset ds_one, dataset ds_two, size_t n_threads);
+/* comparison.c:315-341, 389-403 (host only) */
+double mean_from_smith_waterman_distance_matrix(unsigned long *matrix, dataset ds_one,
+                                                dataset ds_two);
+double sigma_from_smith_waterman_distance_matrix(unsigned long *matrix, double mean,
+                                                 dataset ds_one, dataset ds_two);
+void shortest_longest_distance_in_matrix(unsigned long *matrix, unsigned long *min,
+                                         unsigned long *max, dataset ds_one, dataset ds_two);
+
 /* ---- persisted epsilon lists (.ecsr) ----
  * No counterpart in the reference, which recomputes every distance per run.
  * File: "MNHNECSR", uint32 version 1, int32 n_rows, int32 eps_max, int32 0,

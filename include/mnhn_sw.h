@@ -92,6 +92,16 @@ int mnhn_sw_one_vs_all(mnhn_sw_ctx *ctx, int query, int32_t *dist_out);
 int mnhn_sw_pairs(mnhn_sw_ctx *ctx, const int32_t *ij, int64_t m,
                   int32_t *dist_out);
 
+/* Rectangular block: dist_out[r * nc + k] = d(seq[rows[r]], seq[cols[k]]) for
+ * r < nr, k < nc (host buffer of nr * nc int32).  Replaces the nested score()
+ * loops of smith_waterman_distances_matrix and the comparison statistics
+ * (comparison.c:20-313) and the one-against-many scans of
+ * find_closest_sequence_SW.c:26-63 and filter.c:52-71; rows and columns may
+ * repeat or overlap.  MNHN_ERR_RANGE when nr * ceil(nc / 128) * 128 exceeds
+ * 2^31 (split the rows). */
+int mnhn_sw_block(mnhn_sw_ctx *ctx, const int32_t *rows, int nr, const int32_t *cols, int nc,
+                  int32_t *dist_out);
+
 /* Full n x n distance matrix (row-major, host buffer of n*n int32).  Every
  * unordered pair is computed once and mirrored; the diagonal is 0.  For
  * moderate n (tests, small data sets): MNHN_ERR_RANGE when the unordered

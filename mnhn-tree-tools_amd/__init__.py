@@ -29,7 +29,7 @@ ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of memory",
 # every entry point declared in include/mnhn_sw.h and include/mnhn_host.h
 ABI_SYMBOLS = [
     "mnhn_sw_create", "mnhn_sw_destroy", "mnhn_sw_one_vs_all", "mnhn_sw_pairs",
-    "mnhn_sw_matrix", "mnhn_sw_eps_csr", "mnhn_sw_eps_count", "mnhn_csr_free",
+    "mnhn_sw_matrix", "mnhn_sw_block", "mnhn_sw_eps_csr", "mnhn_sw_eps_count", "mnhn_csr_free",
     "mnhn_sw_size", "mnhn_sw_data_hash", "mnhn_sw_alphabet", "mnhn_sw_last_timing",
     "mnhn_sw_set_tuning", "mnhn_sw_autotune", "mnhn_sw_strerror",
 ]
@@ -46,6 +46,11 @@ HOST_SYMBOLS = [
     "generate_split_set_relation", "free_split_set_and_associated_clusters",
     "mnhn_gpu_initialization", "mnhn_gpu_destroy", "mnhn_gpu_set_lookahead",
     "dbscan_SW_GPU", "mnhn_dbscan_csr", "mnhn_csr_save", "mnhn_csr_load", "adaptive_dbscan",
+    "dataset_to_fasta", "filter_ds_by_size", "filter_by_SW_distance",
+    "smith_waterman_distances_matrix", "mean_from_smith_waterman_datasets",
+    "sigma_from_smith_waterman_datasets", "silhouette_from_smith_waterman_datasets",
+    "mean_from_smith_waterman_distance_matrix", "sigma_from_smith_waterman_distance_matrix",
+    "shortest_longest_distance_in_matrix",
 ]
 
 
@@ -125,6 +130,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_csr_free.argtypes = [ctypes.POINTER(Csr)]
     lib.mnhn_csr_free.restype = None
     lib.mnhn_sw_size.argtypes = [vp]
+    lib.mnhn_sw_block.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, vp]
     lib.mnhn_sw_data_hash.argtypes = [vp]
     lib.mnhn_sw_data_hash.restype = ctypes.c_uint64
     lib.mnhn_csr_save.argtypes = [ctypes.POINTER(Csr), ctypes.c_uint64, ctypes.c_char_p]
@@ -228,6 +234,15 @@ class Engine:
     def matrix(self) -> np.ndarray:
         out = np.empty((self.n, self.n), dtype=np.int32)
         _check(self.lib.mnhn_sw_matrix(self.ctx, out.ctypes.data), "mnhn_sw_matrix")
+        return out
+
+    def block(self, rows, cols) -> np.ndarray:
+        """int32 [len(rows), len(cols)] distances d(seq[rows[r]], seq[cols[k]])."""
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.empty((len(rows), len(cols)), dtype=np.int32)
+        _check(self.lib.mnhn_sw_block(self.ctx, rows.ctypes.data, len(rows), cols.ctypes.data, len(cols),
+                                      out.ctypes.data), "mnhn_sw_block")
         return out
 
     @property

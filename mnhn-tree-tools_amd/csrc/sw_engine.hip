@@ -1340,6 +1340,86 @@ int mnhn_sw_one_vs_all(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
   return rc;
 }
 
+int mnhn_sw_block(mnhn_sw_ctx *c, const int32_t *rows, int nr, const int32_t *cols, int nc,
+                  int32_t *dist_out) {
+  if (!c || nr < 0 || nc < 0 || (nr > 0 && !rows) || (nc > 0 && !cols) || (nr > 0 && nc > 0 && !dist_out))
+    return MNHN_ERR_ARG;
+  for (int r = 0; r < nr; ++r)
+    if (rows[r] < 0 || rows[r] >= c->n) return MNHN_ERR_ARG;
+  for (int k = 0; k < nc; ++k)
+    if (cols[k] < 0 || cols[k] >= c->n) return MNHN_ERR_ARG;
+  if (nr == 0 || nc == 0) return MNHN_OK;
+  int rc = check_range(c);
+  if (rc) return rc;
+  const int S = pick_strip(c);
+  // targets: the columns by length (longest first) in groups of 128; queries:
+  // the rows by length (longest first)
+  std::vector<int> cperm(nc), rperm(nr);
+  std::iota(cperm.begin(), cperm.end(), 0);
+  std::iota(rperm.begin(), rperm.end(), 0);
+  std::stable_sort(cperm.begin(), cperm.end(), [&](int x, int y) { return c->len[cols[x]] > c->len[cols[y]]; });
+  std::stable_sort(rperm.begin(), rperm.end(), [&](int x, int y) { return c->len[rows[x]] > c->len[rows[y]]; });
+  const int G = (nc + kGroup - 1) / kGroup;
+  if ((size_t)G * kGroup * (size_t)nr > (size_t)INT32_MAX) return MNHN_ERR_RANGE;  // 32-bit output offsets
+  Work w;
+  w.gtgt.assign((size_t)G * kGroup, -1);
+  w.gmax.assign(G, 0);
+  w.gmin.assign(G, INT32_MAX);
+  for (int k = 0; k < nc; ++k) {
+    const int g = k / kGroup, id = cols[cperm[k]];
+    w.gtgt[(size_t)g * kGroup + (k % kGroup)] = id;
+    w.gmax[g] = std::max(w.gmax[g], c->len[id]);
+    w.gmin[g] = std::min(w.gmin[g], c->len[id]);
+  }
+  for (int r = 0; r < nr; ++r) w.qids.push_back(rows[rperm[r]]);
+  double total = 0, sum_lb = 0, sum_la = 0;
+  for (int g = 0; g < G; ++g) {
+    const int strips = w.gmax[g] > 1 ? (w.gmax[g] - 1 + S - 1) / S : 0;
+    for (int r = 0; r < nr; ++r) total += (double)std::max(c->len[w.qids[r]] - 1, 0) * strips * S;
+  }
+  for (int k = 0; k < nc; ++k) sum_lb += c->len[cols[k]];
+  for (int r = 0; r < nr; ++r) sum_la += c->len[rows[r]];
+  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 256.0), 2.0e5);
+  for (int g = 0; g < G; ++g) {
+    const int strips = w.gmax[g] > 1 ? (w.gmax[g] - 1 + S - 1) / S : 0;
+    int qb = 0;
+    double acc = 0;
+    for (int qi = 0; qi < nr; ++qi) {
+      acc += (double)std::max(c->len[w.qids[qi]] - 1, 0) * strips * S + 64.0;
+      if (acc >= target || qi + 1 == nr) {
+        w.items.push_back(make_int4(g, qb, qi + 1, 0));
+        qb = qi + 1;
+        acc = 0;
+      }
+    }
+  }
+  w.cells = sum_la * sum_lb;
+  w.pairs = (double)nr * nc;
+  finalize_items(c, w, S, true);
+  const size_t out_n = (size_t)G * kGroup * nr;
+  int32_t *d_out = nullptr;
+  if ((rc = dev_alloc(&d_out, out_n))) return rc;
+  rc = launch_tiles(c, w, MODE_DIST, 0, 0, d_out, nullptr, 0);
+  if (rc == MNHN_OK) {
+    std::vector<int32_t> h(out_n);
+    if (copy_d2h(c, h.data(), d_out, out_n * sizeof(int32_t)) != MNHN_OK) {
+      rc = MNHN_ERR_DEVICE;
+    } else {
+      for (const auto &it : w.items)
+        for (int qi = it.y; qi < it.z; ++qi) {
+          int32_t *orow = dist_out + (size_t)rperm[qi] * nc;
+          const int32_t *hrow = h.data() + (size_t)it.w + (size_t)(qi - it.y) * kGroup;
+          for (int t = 0; t < kGroup; ++t) {
+            const int k = it.x * kGroup + t;
+            if (k < nc) orow[cperm[k]] = hrow[t];
+          }
+        }
+    }
+  }
+  dev_free(d_out);
+  return rc;
+}
+
 int mnhn_sw_pairs(mnhn_sw_ctx *c, const int32_t *ij, int64_t m, int32_t *dist_out) {
   if (!c || m < 0 || (m > 0 && (!ij || !dist_out))) return MNHN_ERR_ARG;
   for (int64_t k = 0; k < 2 * m; ++k)

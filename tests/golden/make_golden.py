@@ -15,6 +15,10 @@ Run in a container where the reference sources exist (default
   cli/<case>/             -- stdout and every output file of the reference's
                              cluster_dbscan_SW / adaptive_clustering_SW run on
                              that input (cmd.json names the arguments)
+  consumers/<case>/       -- the same for the reference's other score()
+                             consumers: find_closest_sequence_SW,
+                             find_satellite, compareSW, silhouette (cmd.json:
+                             argv with IN:<input> for copied inputs)
 
 The fixtures are data (inputs and expected outputs) only.
 """
@@ -135,6 +139,25 @@ def make_inputs(ref_dir: str) -> dict:
     synth.write_fasta(os.path.join(d, "c2_small.fasta"),
                       synth.config2(seed=12, n_roots=8, n_children=4, n_leaves=8))
     synth.write_fasta(os.path.join(d, "c1_full.fasta"), synth.config1(seed=1))
+    # queries / targets for the consumers: substitution-only copies of members
+    # of c2_small (so that close matches exist; the third query has the first
+    # one's length), and one random sequence
+    import numpy as np
+    rng = np.random.default_rng(13)
+    c2 = synth.config2(seed=12, n_roots=8, n_children=4, n_leaves=8)
+    def subs(seq: bytes, p: float) -> bytes:
+        codes = np.frombuffer(seq, np.uint8).copy()
+        lut = {b: k for k, b in enumerate(b"ACGT")}
+        c = np.array([lut[x] for x in codes], np.uint8)
+        return synth.to_bytes(synth.mutate(rng, c, p, 0.0))
+    q1 = subs(c2[5], 0.05)
+    queries = [q1, synth.to_bytes(synth.random_codes(rng, 50)), subs(q1, 0.10)]
+    synth.write_fasta(os.path.join(d, "queries.fasta"), queries)
+    synth.write_fasta(os.path.join(d, "targets.fasta"), [subs(c2[77], 0.03), c2[3]])
+    # two sets of random 10-mers, large enough that compareSW's int sums wrap
+    for name, seed in (("kmers10_a.fasta", 21), ("kmers10_b.fasta", 22)):
+        r = np.random.default_rng(seed)
+        synth.write_fasta(os.path.join(d, name), [synth.to_bytes(synth.random_codes(r, 10)) for _ in range(9000)])
     return {n: os.path.join(d, n) for n in os.listdir(d)}
 
 
@@ -188,6 +211,65 @@ def run_case(case, inputs) -> str:
     return name
 
 
+# (case, program, argv after the program; IN:<name> = copied input, returncode kept)
+CONSUMER_CASES = [
+    ("closest_c2_q1", "find_closest_sequence_SW",
+     ["IN:c2_small.fasta", "IN:queries.fasta", "1", "1", "closest.fasta"]),
+    ("closest_c2_q2_ties", "find_closest_sequence_SW",
+     ["IN:c2_small.fasta", "IN:queries.fasta", "2", "3", "closest.fasta"]),
+    ("closest_c2_q3", "find_closest_sequence_SW",
+     ["IN:c2_small.fasta", "IN:queries.fasta", "3", "2", "closest.fasta"]),
+    ("closest_test3_in_test4", "find_closest_sequence_SW",
+     ["IN:test4.fasta", "IN:test3.fasta", "1", "1", "closest.fasta"]),
+    ("satellite_window", "find_satellite",
+     ["IN:c2_small.fasta", "IN:targets.fasta", "300", "150", "150", "600", "2"]),
+    ("satellite_wide", "find_satellite",
+     ["IN:c2_small.fasta", "IN:targets.fasta", "250", "400", "200", "100000", "1"]),
+    ("satellite_underflow", "find_satellite",
+     ["IN:c2_small.fasta", "IN:targets.fasta", "5", "1000", "10", "100000", "1"]),
+    ("satellite_testx", "find_satellite",
+     ["IN:testx_300.fasta", "IN:test2.fasta", "60", "200", "40", "400", "4"]),
+    ("compare_c1_c2", "compareSW", ["IN:c1_small.fasta", "IN:c2_small.fasta", "1"]),
+    ("compare_test3_test4", "compareSW", ["IN:test3.fasta", "IN:test4.fasta", "1"]),
+    ("compare_test3_test4_t4", "compareSW", ["IN:test3.fasta", "IN:test4.fasta", "4"]),
+    ("compare_kmers10_wrap", "compareSW", ["IN:kmers10_a.fasta", "IN:kmers10_b.fasta", "1"]),
+    ("silhouette_c1_c2", "silhouette", ["IN:c1_small.fasta", "IN:c2_small.fasta", "1"]),
+    ("silhouette_c2_c1", "silhouette", ["IN:c2_small.fasta", "IN:c1_small.fasta", "1"]),
+    ("silhouette_test3_test4", "silhouette", ["IN:test3.fasta", "IN:test4.fasta", "1"]),
+]
+
+
+def run_consumer_case(case, inputs) -> str:
+    name, prog, argv = case
+    exe = os.path.join(ROOT, "oracle", "_ref", prog)
+    outdir = os.path.join(HERE, "consumers", name)
+    if os.path.exists(os.path.join(outdir, "cmd.json")):
+        return name + " (kept)"
+    with tempfile.TemporaryDirectory() as tmp:
+        args = []
+        for a in argv:
+            if a.startswith("IN:"):
+                dst = os.path.join(tmp, "in_" + a[3:])
+                shutil.copyfile(inputs[a[3:]], dst)
+                args.append(dst)
+            else:
+                args.append(a)
+        before = set(os.listdir(tmp))
+        r = subprocess.run([exe] + args, cwd=tmp, capture_output=True)
+        files = sorted(set(os.listdir(tmp)) - before)
+        if os.path.exists(outdir):
+            shutil.rmtree(outdir)
+        os.makedirs(outdir)
+        for f in files:
+            shutil.copyfile(os.path.join(tmp, f), os.path.join(outdir, f))
+        with open(os.path.join(outdir, "stdout.txt"), "wb") as g:
+            g.write(r.stdout)
+        with open(os.path.join(outdir, "cmd.json"), "w") as g:
+            json.dump({"program": prog, "argv": argv, "returncode": r.returncode, "files": files}, g,
+                      indent=1)
+    return name
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -205,6 +287,8 @@ def main() -> None:
     cases.sort(key=lambda c: ("full" not in c[0], "adaptive" not in c[0]))
     with ThreadPoolExecutor(a.jobs) as ex:
         for name in ex.map(lambda c: run_case(c, inputs), cases):
+            print("done", name, flush=True)
+        for name in ex.map(lambda c: run_consumer_case(c, inputs), CONSUMER_CASES):
             print("done", name, flush=True)
 
 

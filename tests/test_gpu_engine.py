@@ -239,3 +239,31 @@ def test_every_tuning_is_exact(pkg, oracle, kernel, strip, wpc):
     assert (t["kernel"], t["strip"]) == (kernel, strip)
     assert np.array_equal(got, want)
     assert np.array_equal(row, want[5])
+
+
+@pytest.mark.parametrize("tuning", [None, (3, 64, 8), (2, 16, 16)])
+def test_block_vs_oracle(pkg, oracle, tuning):
+    """Rectangular blocks with repeated and overlapping rows/columns, ragged
+    lengths (incl. empty sequences), and a column count that is no multiple
+    of 128."""
+    rng = np.random.default_rng(61)
+    seqs = _rand_set(rng, 240, 0, 600)
+    base = _rand_set(rng, 3, 80, 500, b"ACGT")
+    for b in base:
+        for _ in range(12):
+            seqs.append(synth.to_bytes(synth.mutate(
+                rng, np.frombuffer(b, np.uint8).copy() % 4, 0.04, 0.01)))
+    n = len(seqs)
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    rows = np.concatenate([rng.choice(n, 90), [0, 0, n - 1]]).astype(np.int32)
+    cols = np.concatenate([rng.choice(n, 200), rows[:7]]).astype(np.int32)
+    _say(f"block {len(rows)} x {len(cols)} tuning={tuning}")
+    with pkg.Engine(seqs) as eng:
+        if tuning:
+            eng.set_tuning(*tuning)
+        got = eng.block(rows, cols)
+        one = eng.block(rows[:1], np.arange(n, dtype=np.int32))
+        empty = eng.block(rows[:0], cols)
+    assert np.array_equal(got, want[np.ix_(rows, cols)])
+    assert np.array_equal(one[0], want[rows[0]])
+    assert empty.shape == (0, len(cols))

@@ -355,3 +355,92 @@ def adaptive_clustering(seqs: Sequence[bytes], eps_start: float, eps_inc: float,
             out.append("".join("%i " % c for c in conns[i - 1][j]))
             out.append("\n")
     return "".join(out), files, kept
+
+
+# --------------------------------------------------------------------------
+# the other score() consumers: find_closest_sequence_SW.c, filter.c,
+# comparison.c, dataset.c:85-110
+# --------------------------------------------------------------------------
+
+def _cross(a: Sequence[bytes], b: Sequence[bytes]) -> np.ndarray:
+    """d(a_i, b_j) as an int64 matrix (oracle distances)."""
+    seqs = list(a) + list(b)
+    ij = np.array([[i, len(a) + j] for i in range(len(a)) for j in range(len(b))], np.int32)
+    if len(ij) == 0:
+        return np.zeros((len(a), len(b)), np.int64)
+    return sw_pairs(seqs, ij).astype(np.int64).reshape(len(a), len(b))
+
+
+def find_closest(db: Sequence[bytes], query: bytes):
+    """find_closest_sequence_SW.c:26-63, 126-153 (query index 1): the smallest
+    distance and the lowest index that has it; stdout text and output file."""
+    d = _cross([query], db)[0]
+    best = int(np.argmin(d))
+    text = "Smith Waterman distance = %i \n" % int(d[best])
+    return text, cluster_file_text(Cluster([best], 0), db)
+
+
+def dataset_to_fasta(seqs: Sequence[bytes]) -> bytes:
+    """dataset.c:85-110 for non-empty sequences."""
+    return cluster_file_text(Cluster(list(range(len(seqs))), 0), seqs) if seqs else b""
+
+
+def find_satellite(ds: Sequence[bytes], target: bytes, size: int, plus: int, minus: int,
+                   max_distance: int) -> bytes:
+    """find_satellite.c:12-74 with filter.c:18-140: the size window in size_t
+    arithmetic, then d(seq, target) <= max_distance, printed in order."""
+    m64 = (1 << 64) - 1
+    hi, lo = (size + plus) & m64, (size - minus) & m64
+    sized = [s for s in ds if lo <= len(s) <= hi]
+    if not sized:
+        return b""
+    d = _cross(sized, [target])[:, 0]
+    return dataset_to_fasta([s for s, x in zip(sized, d) if max_distance >= int(x)])
+
+
+def _to_int_x86(t: float) -> int:
+    """cvttsd2si: truncation toward zero, INT_MIN outside the int range."""
+    if t != t or not (-2147483649.0 < t < 2147483648.0):
+        return -2147483648
+    return int(t)
+
+
+def compare_sw(a: Sequence[bytes], b: Sequence[bytes]) -> str:
+    """compareSW.c with comparison.c:113-277 run by one thread: the mean from
+    an int sum of the distances (modulo 2^32), sigma from an int sum that
+    takes each (d - mean)^2 through a double; rows then columns in order."""
+    d = _cross(a, b).ravel()
+    s32 = int(d.sum()) & 0xFFFFFFFF
+    s32 = s32 - (1 << 32) if s32 >= (1 << 31) else s32
+    mean = float(s32) / (float(len(a)) * float(len(b)))
+    acc = 0
+    for v in d.tolist():
+        x = (float(v) - mean) * (float(v) - mean)
+        acc = _to_int_x86(float(acc) + x)
+    sigma = float(np.sqrt(np.float64(acc))) / (len(a) + len(b))
+    return "mean = %s, sigma = %s \n" % (_fmt_lf(mean), _fmt_lf(sigma))
+
+
+def silhouette(a: Sequence[bytes], b: Sequence[bytes]) -> str:
+    """silhouette.c with comparison.c:20-75, 279-313 run by one thread."""
+    internal = _cross(a, a).sum(1)
+    external = _cross(a, b).sum(1)
+    s_mean = 0.0
+    for i in range(len(a)):
+        av = float(internal[i]) / (float(len(a)) - 1.0)
+        bv = float(external[i]) / float(len(b))
+        if av < bv:
+            s = 1 - av / bv
+        elif av == bv:
+            s = 0.0
+        else:
+            s = bv / av - 1
+        s_mean += s
+    return "%s\n" % _fmt_lf(s_mean / float(len(a)))
+
+
+def _fmt_lf(x: float) -> str:
+    """printf("%lf") as glibc prints it (nan keeps its sign)."""
+    if x != x:
+        return "-nan" if np.signbit(x) else "nan"
+    return "%f" % x

@@ -151,8 +151,11 @@ def make_inputs(ref_dir: str) -> dict:
         c = np.array([lut[x] for x in codes], np.uint8)
         return synth.to_bytes(synth.mutate(rng, c, p, 0.0))
     q1 = subs(c2[5], 0.05)
-    queries = [q1, synth.to_bytes(synth.random_codes(rng, 50)), subs(q1, 0.10)]
-    synth.write_fasta(os.path.join(d, "queries.fasta"), queries)
+    rand50 = synth.to_bytes(synth.random_codes(rng, 50))
+    # the reference scores query k with the length of query 1
+    # (find_closest_sequence_SW.c:52): both queries have that length
+    synth.write_fasta(os.path.join(d, "queries.fasta"), [q1, subs(q1, 0.10)])
+    synth.write_fasta(os.path.join(d, "random_query.fasta"), [rand50])
     synth.write_fasta(os.path.join(d, "targets.fasta"), [subs(c2[77], 0.03), c2[3]])
     # two sets of random 10-mers, large enough that compareSW's int sums wrap
     for name, seed in (("kmers10_a.fasta", 21), ("kmers10_b.fasta", 22)):
@@ -215,10 +218,10 @@ def run_case(case, inputs) -> str:
 CONSUMER_CASES = [
     ("closest_c2_q1", "find_closest_sequence_SW",
      ["IN:c2_small.fasta", "IN:queries.fasta", "1", "1", "closest.fasta"]),
-    ("closest_c2_q2_ties", "find_closest_sequence_SW",
-     ["IN:c2_small.fasta", "IN:queries.fasta", "2", "3", "closest.fasta"]),
-    ("closest_c2_q3", "find_closest_sequence_SW",
-     ["IN:c2_small.fasta", "IN:queries.fasta", "3", "2", "closest.fasta"]),
+    ("closest_c2_random_ties", "find_closest_sequence_SW",
+     ["IN:c2_small.fasta", "IN:random_query.fasta", "1", "3", "closest.fasta"]),
+    ("closest_c2_q2", "find_closest_sequence_SW",
+     ["IN:c2_small.fasta", "IN:queries.fasta", "2", "2", "closest.fasta"]),
     ("closest_test3_in_test4", "find_closest_sequence_SW",
      ["IN:test4.fasta", "IN:test3.fasta", "1", "1", "closest.fasta"]),
     ("satellite_window", "find_satellite",

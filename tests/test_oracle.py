@@ -1,6 +1,7 @@
 """CPU tests that pin the oracle (C restatement of smith-waterman.c and the
 Python restatement of dbscan.c / cluster_io.c) to the compiled reference's
 golden outputs, closed forms and brute force."""
+import json
 import os
 import random
 
@@ -127,3 +128,37 @@ def test_oracle_cli_matches_reference(oracle, name, meta):
     assert sorted(got) == sorted(files)
     for k in files:
         assert got[k] == files[k], k
+
+
+def _consumer_cases():
+    d = os.path.join(GOLDEN, "consumers")
+    for name in sorted(os.listdir(d)):
+        meta = json.load(open(os.path.join(d, name, "cmd.json")))
+        yield name, meta
+
+
+@pytest.mark.parametrize("name,meta", [c for c in _consumer_cases() if "kmers10" not in c[0]],
+                         ids=[c[0] for c in _consumer_cases() if "kmers10" not in c[0]])
+def test_oracle_consumers_match_reference(oracle, name, meta):
+    """The oracle's restatement of the other score() consumers reproduces the
+    compiled reference's stdout and files (the 81M-pair wrap case runs on the
+    GPU only)."""
+    d = os.path.join(GOLDEN, "consumers", name)
+    want = open(os.path.join(d, "stdout.txt"), "rb").read()
+    argv = meta["argv"]
+    inp = lambda k: oracle.read_fasta(os.path.join(GOLDEN, "inputs", argv[k][3:]))
+    prog = meta["program"]
+    if prog == "find_closest_sequence_SW":
+        db, q = inp(0), inp(1)
+        idx = int(argv[2])
+        assert idx == 1 or len(q[idx - 1]) == len(q[0])
+        text, f = oracle.find_closest(db, q[idx - 1])
+        assert text.encode() == want
+        assert f == open(os.path.join(d, argv[4]), "rb").read()
+    elif prog == "find_satellite":
+        got = oracle.find_satellite(inp(0), inp(1)[0], *[int(x) for x in argv[2:6]])
+        assert got == want
+    elif prog == "compareSW":
+        assert oracle.compare_sw(inp(0), inp(1)).encode() == want
+    else:
+        assert oracle.silhouette(inp(0), inp(1)).encode() == want

@@ -127,6 +127,11 @@ double sigma_from_smith_waterman_distance_matrix(unsigned long *matrix, double m
                                                  dataset ds_one, dataset ds_two);
 void shortest_longest_distance_in_matrix(unsigned long *matrix, unsigned long *min,
                                          unsigned long *max, dataset ds_one, dataset ds_two);
+/* dataset.c:427-469, 629-635: PCA projections (values[dim][row]) */
+void load_projections_from_file_into_dataset(FILE *projections, size_t dimensions, dataset *ds);
+void free_values_from_dataset(dataset ds);
+/* comparison.c:343-387: "i j SW L1 L2" header, then every pair j < i */
+void print_SW_PCA_L_comparison(FILE *f, dataset ds);
 
 /* ---- persisted epsilon lists (.ecsr) ----
  * No counterpart in the reference, which recomputes every distance per run.

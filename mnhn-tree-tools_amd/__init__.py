@@ -50,7 +50,8 @@ HOST_SYMBOLS = [
     "smith_waterman_distances_matrix", "mean_from_smith_waterman_datasets",
     "sigma_from_smith_waterman_datasets", "silhouette_from_smith_waterman_datasets",
     "mean_from_smith_waterman_distance_matrix", "sigma_from_smith_waterman_distance_matrix",
-    "shortest_longest_distance_in_matrix",
+    "shortest_longest_distance_in_matrix", "load_projections_from_file_into_dataset",
+    "free_values_from_dataset", "print_SW_PCA_L_comparison",
 ]
 
 

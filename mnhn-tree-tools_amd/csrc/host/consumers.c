@@ -297,3 +297,46 @@ void shortest_longest_distance_in_matrix(unsigned long *matrix, unsigned long *m
     if (matrix[i] > max[0]) max[0] = matrix[i];
   }
 }
+
+/* comparison.c:343-387: every pair j < i with its Smith-Waterman distance and
+ * the L2 / L1 distances of the projections, "%i %i %lf %lf %lf\n". */
+void print_SW_PCA_L_comparison(FILE *f, dataset ds) {
+  fprintf(f, "i j SW L1 L2\n");
+  const int n = ds.n_values;
+  if (n < 2) return;
+  mnhn_sw_ctx *sw = NULL;
+  int rc = mnhn_sw_create((const char *const *)ds.sequences, ds.sequence_lengths, n, NULL, &sw);
+  if (rc) die("mnhn_sw_create", rc);
+  /* rows in blocks against the columns j < the block's last row */
+  int32_t *cols = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  for (int k = 0; k < n; k++) cols[k] = k;
+  const int rows_per_block = 1024;
+  int32_t *rows = (int32_t *)malloc(sizeof(int32_t) * rows_per_block);
+  int32_t *dist = (int32_t *)malloc(sizeof(int32_t) * (size_t)rows_per_block * (size_t)n);
+  for (int r0 = 1; r0 < n; r0 += rows_per_block) {
+    const int nr = n - r0 < rows_per_block ? n - r0 : rows_per_block;
+    const int nc = r0 + nr - 1; /* columns 0 .. last row - 1 */
+    for (int r = 0; r < nr; r++) rows[r] = r0 + r;
+    rc = mnhn_sw_block(sw, rows, nr, cols, nc, dist);
+    if (rc) die("mnhn_sw_block", rc);
+    for (int r = 0; r < nr; r++) {
+      const int i = r0 + r;
+      for (int j = 0; j < i; j++) {
+        const double dist_sw = (double)(uint32_t)dist[(size_t)r * nc + j];
+        double l2 = 0, l1 = 0;
+        for (int k = 0; k < ds.n_dimensions; k++) {
+          const double buf = (double)(ds.values[k][j] - ds.values[k][i]);
+          l2 += buf * buf;
+          l1 += fabs(buf);
+        }
+        l2 = sqrt(l2);
+        fprintf(f, "%i %i %lf %lf %lf\n", i, j, dist_sw, l2, l1);
+      }
+    }
+  }
+  free(rows);
+  free(dist);
+  free(cols);
+  mnhn_sw_destroy(sw);
+}
+

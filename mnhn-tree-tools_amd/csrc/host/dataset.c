@@ -14,6 +14,7 @@
 #include <ctype.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <unistd.h>
 #include <string.h>
 
 #include "mnhn_host.h"
@@ -96,3 +97,33 @@ void free_sequences_from_dataset(dataset ds) {
   free(ds.sequence_lengths);
   free(ds.sequences);
 }
+
+/* dataset.c:427-469: n_values rows of `dimensions` whitespace-separated
+ * floats; values[dimension][row].  A short row prints the reference's message
+ * and exits. */
+void load_projections_from_file_into_dataset(FILE *projections, size_t dimensions, dataset *ds) {
+  ds->n_dimensions = (int)dimensions;
+  ds->values = (float **)malloc(sizeof(float *) * (dimensions ? dimensions : 1));
+  for (size_t i = 0; i < dimensions; i++) {
+    ds->values[i] = (float *)malloc(sizeof(float) * (size_t)(ds->n_values + 8));
+    if (!ds->values[i]) {
+      printf("Could not allocate memory to load projections \n");
+      _exit(1);
+    }
+  }
+  for (int i = 0; i < ds->n_values; i++) {
+    size_t counter = 0;
+    for (size_t j = 0; j < dimensions; j++) counter += (size_t)fscanf(projections, "%f", ds->values[j] + i);
+    if (counter != dimensions) {
+      printf("Error reading projections file \n");
+      _exit(1);
+    }
+  }
+}
+
+/* dataset.c:629-635 */
+void free_values_from_dataset(dataset ds) {
+  for (int i = 0; i < ds.n_dimensions; i++) free(ds.values[i]);
+  free(ds.values);
+}
+

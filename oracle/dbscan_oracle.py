@@ -444,3 +444,24 @@ def _fmt_lf(x: float) -> str:
     if x != x:
         return "-nan" if np.signbit(x) else "nan"
     return "%f" % x
+
+
+def compare_norms(seqs: Sequence[bytes], proj: np.ndarray) -> str:
+    """compare_norms.c + print_SW_PCA_L_comparison (comparison.c:343-387):
+    header, then per pair j < i the distance, the L2 and the L1 norm of the
+    float32 projection differences (each difference taken in float32)."""
+    n = len(seqs)
+    d = sw_matrix(seqs) if n > 1 else np.zeros((n, n), np.int64)
+    p32 = proj.astype(np.float32)
+    out = ["i j SW L1 L2\n"]
+    for i in range(n):
+        for j in range(i):
+            buf = (p32[j] - p32[i]).astype(np.float64)
+            l2 = 0.0
+            l1 = 0.0
+            for x in buf.tolist():
+                l2 += x * x
+                l1 += abs(x)
+            out.append("%i %i %s %s %s\n" % (i, j, _fmt_lf(float(d[i, j])), _fmt_lf(float(np.sqrt(l2))),
+                                             _fmt_lf(l1)))
+    return "".join(out)

@@ -17,7 +17,8 @@ Run in a container where the reference sources exist (default
                              that input (cmd.json names the arguments)
   consumers/<case>/       -- the same for the reference's other score()
                              consumers: find_closest_sequence_SW,
-                             find_satellite, compareSW, silhouette (cmd.json:
+                             find_satellite, compareSW, silhouette,
+                             compare_norms (cmd.json:
                              argv with IN:<input> for copied inputs)
 
 The fixtures are data (inputs and expected outputs) only.
@@ -157,6 +158,16 @@ def make_inputs(ref_dir: str) -> dict:
     synth.write_fasta(os.path.join(d, "queries.fasta"), [q1, subs(q1, 0.10)])
     synth.write_fasta(os.path.join(d, "random_query.fasta"), [rand50])
     synth.write_fasta(os.path.join(d, "targets.fasta"), [subs(c2[77], 0.03), c2[3]])
+    # PCA-projection files for compare_norms (values of several magnitudes)
+    c1 = open(os.path.join(d, "c1_small.fasta"), "rb").read().split(b">")[1:61]
+    with open(os.path.join(d, "c1_60.fasta"), "wb") as g:
+        g.write(b"".join(b">" + r for r in c1))
+    for name, rows, dims in (("test3_proj.txt", 5, 3), ("c1_60_proj.txt", 60, 4)):
+        r = np.random.default_rng(rows)
+        v = r.normal(size=(rows, dims)) * np.array([10.0 ** k for k in range(dims)])[None, :dims]
+        with open(os.path.join(d, name), "w") as g:
+            for row in v:
+                g.write(" ".join("%.6f" % x for x in row) + "\n")
     # two sets of random 10-mers, large enough that compareSW's int sums wrap
     for name, seed in (("kmers10_a.fasta", 21), ("kmers10_b.fasta", 22)):
         r = np.random.default_rng(seed)
@@ -239,6 +250,8 @@ CONSUMER_CASES = [
     ("silhouette_c1_c2", "silhouette", ["IN:c1_small.fasta", "IN:c2_small.fasta", "1"]),
     ("silhouette_c2_c1", "silhouette", ["IN:c2_small.fasta", "IN:c1_small.fasta", "1"]),
     ("silhouette_test3_test4", "silhouette", ["IN:test3.fasta", "IN:test4.fasta", "1"]),
+    ("norms_test3", "compare_norms", ["IN:test3.fasta", "IN:test3_proj.txt", "3"]),
+    ("norms_c1_60", "compare_norms", ["IN:c1_60.fasta", "IN:c1_60_proj.txt", "4"]),
 ]
 
 

@@ -160,5 +160,8 @@ def test_oracle_consumers_match_reference(oracle, name, meta):
         assert got == want
     elif prog == "compareSW":
         assert oracle.compare_sw(inp(0), inp(1)).encode() == want
+    elif prog == "compare_norms":
+        proj = np.loadtxt(os.path.join(GOLDEN, "inputs", argv[1][3:]), ndmin=2)
+        assert oracle.compare_norms(inp(0), proj[:, :int(argv[2])]).encode() == want
     else:
         assert oracle.silhouette(inp(0), inp(1)).encode() == want

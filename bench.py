@@ -30,7 +30,9 @@ Other configurations:
                prefilter lists against the exact lists on 100,000 held-out
                sequences (and DBSCAN on both);
   --config c5  end-to-end DBSCAN on 500,000 heavy-tailed sequences: exact
-               lists + DBSCAN against prefilter lists + DBSCAN.
+               lists + DBSCAN against prefilter lists + DBSCAN;
+  --config consumers  wall time of the drop-in compareSW, silhouette and
+               find_closest_sequence_SW CLIs on two 10,000-sequence sets.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4|c5]
 """
@@ -474,6 +476,56 @@ def run_proposer_bench(args, pkg, log):
     print(json.dumps(out), flush=True)
 
 
+# ------------------------------------------------- other score() consumers
+
+def run_consumers_bench(args, log):
+    """Wall time of the drop-in compareSW, silhouette and
+    find_closest_sequence_SW CLIs on two uneven-length sets (two draws of the
+    20,000-sequence tree generator, n of each), against the reference's
+    score() rate on the same data (its CPU path computes the same cells)."""
+    n = args.n or 10000
+    a = synth.config2(seed=2)[::2][:n]
+    b = synth.config2(seed=3)[::2][:n]
+    La = np.array([len(x) for x in a], np.float64)
+    Lb = np.array([len(x) for x in b], np.float64)
+    cross = float(La.sum() * Lb.sum())
+    within = float(La.sum() ** 2)
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        fa, fb, fq = (os.path.join(tmp, x) for x in ("a.fasta", "b.fasta", "q.fasta"))
+        synth.write_fasta(fa, a)
+        synth.write_fasta(fb, b)
+        synth.write_fasta(fq, b[:1])
+        runs = [("compareSW", [fa, fb, "1"], 2 * cross),  # mean pass + sigma pass
+                ("silhouette", [fa, fb, "1"], within + cross),
+                ("find_closest_sequence_SW", [fa, fq, "1", "1", os.path.join(tmp, "c.fasta")],
+                 float(La.sum() * Lb[0]))]
+        for prog, argv, cells in runs:
+            exe = os.path.join(PKG_DIR, "bin", prog)
+            t0 = time.perf_counter()
+            r = subprocess.run([exe] + argv, capture_output=True, timeout=900)
+            secs = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout.decode()[-500:] + r.stderr.decode()[-500:])
+            res[prog] = {"s": secs, "cells": cells, "gcups_wall": cells / secs / 1e9,
+                         "stdout": r.stdout.decode().strip()[:120]}
+            log(f"{prog}: {secs:.2f} s, {cells / secs / 1e9:.0f} GCUPS (wall), {res[prog]['stdout']}")
+    out = {"metric": "SW consumer CLIs wall seconds (compareSW)", "value": res["compareSW"]["s"],
+           "unit": "s", "higher_is_better": False, "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": res["compareSW"]["s"] * 1e3, "scaling": "weak", "vs_baseline": None,
+           "dtype": "int16", "data": "synthetic",
+           "config": {"workload": f"compareSW / silhouette / find_closest_sequence_SW on two sets of "
+                                  f"{n} uneven-length sequences (tree generator draws)",
+                      "runs": res}}
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cb = cpu_baseline(a + b, args.cpu_sample)
+        out["cpu_baseline"] = cb
+        out["config"]["reference_cpu_estimate_s"] = {
+            k: v["cells"] / (cb["value"] * 1e9) for k, v in res.items()}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -485,7 +537,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "consumers"],
                     help="c2: 20,000 uneven-length (headline); c3: 105,000 amplicon-scale; "
                          "c4: learned-proposer training + held-out recall; c5: end-to-end "
                          "learned-prefilter DBSCAN vs the exact path")
@@ -526,10 +578,13 @@ def main():
         sys.exit(2)
 
     pkg = load_pkg()
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
     if args.config in ("c4", "c5"):
-        def log(msg):
-            print(msg, file=sys.stderr, flush=True)
         run_proposer_bench(args, pkg, log)
+        return
+    if args.config == "consumers":
+        run_consumers_bench(args, log)
         return
     print("generating workload ...", file=sys.stderr, flush=True)
     seqs = synth.config2() if args.config == "c2" else synth.config3()

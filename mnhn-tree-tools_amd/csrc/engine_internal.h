@@ -107,9 +107,10 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
                  int32_t *d_dist, unsigned long long *d_keys, unsigned long long key_cap);
 
 // Sort each row segment [d_rp[r], d_rp[r+1]) of distinct column values in
-// [0, n_cols) ascending into out (same positions), on the ctx's stream.
-int sort_row_segments(mnhn_sw_ctx *c, const long long *d_rp, const int32_t *seg, int32_t *out, int n_rows,
-                      int n_cols);
+// [0, n_cols) ascending into out (same positions), moving the optional 32-bit
+// payload along (pay_in / pay_out may be null), on stream st.
+int sort_row_segments(hipStream_t st, const long long *d_rp, const int32_t *seg, int32_t *out,
+                      const uint32_t *pay_in, uint32_t *pay_out, int n_rows, int n_cols);
 
 // Host allocation for large result arrays (free() releases it).
 void *host_alloc(size_t bytes);

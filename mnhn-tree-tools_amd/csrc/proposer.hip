@@ -1306,7 +1306,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
     dev_free(cand);
     if ((rc = dev_alloc(&sorted, (size_t)m))) return fail(rc);
-    if ((rc = sort_row_segments(c, d_rs, seg, sorted, n, n))) return fail(rc);
+    if ((rc = sort_row_segments(st, d_rs, seg, sorted, nullptr, nullptr, n, n))) return fail(rc);
     dev_free(seg);
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||

@@ -719,9 +719,10 @@ constexpr int kShortRow = 256;
 // One workgroup per row: columns are distinct, so each entry's position in
 // the sorted row is the number of smaller columns in the row.  seg_d / out_d
 // may be null (columns only).
+template <typename P>
 __global__ __launch_bounds__(kSortThreads) void sort_rows_kernel(
-    const long long *row_ptr, const int32_t *seg_col, const int16_t *seg_d,
-    int32_t *out_col, int16_t *out_d, int n_cols) {
+    const long long *row_ptr, const int32_t *seg_col, const P *seg_d,
+    int32_t *out_col, P *out_d, int n_cols) {
   extern __shared__ uint32_t lds[];
   const int row = blockIdx.x;
   const long long b = row_ptr[row];
@@ -1650,10 +1651,10 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
     e = hipGetLastError();
   }
   if (e == hipSuccess && lds_bytes > 64 * 1024)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel),
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel<int16_t>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e == hipSuccess && nk > 0) {
-    hipLaunchKernelGGL(sort_rows_kernel, dim3((unsigned)n), dim3(kSortThreads), lds_bytes,
+    hipLaunchKernelGGL(sort_rows_kernel<int16_t>, dim3((unsigned)n), dim3(kSortThreads), lds_bytes,
                        c->stream, d_rp, d_seg_col, d_seg_d, d_col, d_dist, n);
     e = hipGetLastError();
   }
@@ -1679,18 +1680,18 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
 
 int mnhn_detail::context_strip(const mnhn_sw_ctx *c) { return pick_strip(c); }
 
-int mnhn_detail::sort_row_segments(mnhn_sw_ctx *c, const long long *d_rp, const int32_t *seg, int32_t *out,
-                                   int n_rows, int n_cols) {
+int mnhn_detail::sort_row_segments(hipStream_t st, const long long *d_rp, const int32_t *seg, int32_t *out,
+                                   const uint32_t *pay_in, uint32_t *pay_out, int n_rows, int n_cols) {
   const size_t lds_bytes = std::max<size_t>(
       (size_t)(2 * ((n_cols + 31) / 32) + kSortThreads) * sizeof(uint32_t), kShortRow * sizeof(uint32_t));
   if (lds_bytes > 160 * 1024) return MNHN_ERR_RANGE;
   hipError_t e = hipSuccess;
   if (lds_bytes > 64 * 1024)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel),
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel<uint32_t>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e == hipSuccess && n_rows > 0) {
-    hipLaunchKernelGGL(sort_rows_kernel, dim3((unsigned)n_rows), dim3(kSortThreads), lds_bytes, c->stream, d_rp,
-                       seg, (const int16_t *)nullptr, out, (int16_t *)nullptr, n_cols);
+    hipLaunchKernelGGL(sort_rows_kernel<uint32_t>, dim3((unsigned)n_rows), dim3(kSortThreads), lds_bytes, st,
+                       d_rp, seg, pay_in, out, pay_out, n_cols);
     e = hipGetLastError();
   }
   return e == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;

@@ -14,9 +14,11 @@
  *   opencl_initialization/destroy       -> mnhn_gpu_initialization/destroy
  *                                          (dbscan.h:32-37; dbscan.c:97-298)
  *   dbscan_SW_GPU                         dbscan.h:36-37; dbscan.c:950-1047
+ *   dbscan_L1 / dbscan_L2                 dbscan.h:30-31; dbscan.c:563-780, 950-1047
  *   adaptive_dbscan                       dbscan.h:38-48; dbscan.c:1117-1380
  * The Smith-Waterman distances all come from the gfx950 engine
- * (mnhn_sw.h); there is no CPU distance path in this library.
+ * (mnhn_sw.h) and the L1 / L2 vector distances from the gfx950 vector
+ * engine (mnhn_vec.h); there is no CPU distance path in this library.
  */
 #ifndef MNHN_HOST_H
 #define MNHN_HOST_H
@@ -26,6 +28,7 @@
 #include <stdio.h>
 
 #include "mnhn_sw.h"
+#include "mnhn_vec.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -60,14 +63,23 @@ typedef struct {
   int n_connections;
 } cluster_connections;
 
+/* dataset.h:37-40 */
+typedef struct {
+  size_t n_features;
+  size_t n_samples;
+} data_shape;
+
 /* Device context of a dataset: the engine plus the cached epsilon-neighbour
  * lists of the dataset.  Passed by value like the reference's opencl_stuff;
- * copies share the same engine and cache. */
+ * copies share the same engine and cache.  Sequence datasets use sw; vector
+ * datasets (mnhn_gpu_initialization_values) use vec with norm L1 or L2. */
 typedef struct mnhn_eps_cache mnhn_eps_cache;
 typedef struct {
   mnhn_sw_ctx *sw;
   mnhn_eps_cache *cache;
   int num_devs;
+  mnhn_vec_ctx *vec;
+  int norm; /* MNHN_NORM_L1 / MNHN_NORM_L2 for vector datasets, 0 otherwise */
 } mnhn_gpu_stuff;
 
 /* ---- dataset ---- */
@@ -88,6 +100,8 @@ void free_split_set_and_associated_clusters(split_set s);
  * On failure (no GPU, HIP error) these print the reason and _exit(1), the
  * reference's error convention (dbscan.c:149-152). */
 mnhn_gpu_stuff mnhn_gpu_initialization(dataset ds);
+/* Context over ds.values (n_dimensions x n_values) for the L1 or L2 scans. */
+mnhn_gpu_stuff mnhn_gpu_initialization_values(dataset ds, int norm);
 void mnhn_gpu_destroy(mnhn_gpu_stuff g);
 
 /* epsilon-neighbour lists are computed once up to eps + lookahead and reused
@@ -102,6 +116,35 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
  * learned prefilter; requires eps <= csr->eps_max.  No counterpart in the
  * reference, whose passes always recompute distances. */
 split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts);
+
+/* L1 / L2 DBSCAN over ds.values: region_query of dbscan.c:563-780 (every j,
+ * the point itself included, whose accumulator is <= eps (L1) or <= eps * eps
+ * (L2)), then the same seeding and expansion as dbscan_SW_GPU.  The _GPU forms
+ * take a context from mnhn_gpu_initialization_values with the matching norm
+ * and are the dbscanner arguments of adaptive_dbscan for vector datasets. */
+split_set dbscan_L1_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g);
+split_set dbscan_L2_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g);
+/* dbscan.h:30-31: one pass with a context of its own */
+split_set dbscan_L1(dataset ds, float epsilon, int minpts);
+split_set dbscan_L2(dataset ds, float epsilon, int minpts);
+/* One L1 / L2 DBSCAN pass over lists from mnhn_vec_eps_csr; requires
+ * eps <= csr->thr (L1) or eps * eps <= csr->thr (L2).  No counterpart in the
+ * reference. */
+split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts);
+
+/* ---- vector inputs and outputs ---- */
+/* dataset.c:346-375: features = tabs on the first line, samples = line
+ * breaks from the end of the first line on (the first one included) */
+data_shape shape_from_kmer_file(int infile);
+/* dataset.c:378-425: per sample a label token, then n_features floats
+ * (values[feature][sample]); sequences are left empty */
+dataset load_kmer_from_file_into_dataset(FILE *in_file, data_shape shape);
+/* cluster_io.c:343-379: "%f\t" per value and "%f\n" after the last, files
+ * prefix-%03d */
+void create_single_cluster_file_with_values(char *filename, cluster cl, dataset ds);
+void create_cluster_files_with_values(char *prefix, split_set s, dataset ds);
+/* dataset.c:646-649 */
+void free_dataset(dataset ds);
 
 /* ---- the other Smith-Waterman consumers (distances from the gfx950 engine;
  * n_threads arguments are accepted and unused: one device pass) ---- */

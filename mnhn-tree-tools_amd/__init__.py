@@ -39,6 +39,9 @@ PROPOSER_SYMBOLS = [
     "mnhn_encoder_set_margin_table", "mnhn_encoder_set_cosine_table", "mnhn_encoder_margin", "mnhn_encoder_save", "mnhn_encoder_load",
     "mnhn_prefilter_eps_csr",
 ]
+VEC_SYMBOLS = [
+    "mnhn_vec_create", "mnhn_vec_destroy", "mnhn_vec_eps_csr", "mnhn_vcsr_free", "mnhn_vec_last_timing",
+]
 HOST_SYMBOLS = [
     "dataset_from_fasta", "free_sequences_from_dataset", "store_split_set",
     "read_split_set", "create_single_cluster_file", "create_cluster_files",
@@ -52,6 +55,9 @@ HOST_SYMBOLS = [
     "mean_from_smith_waterman_distance_matrix", "sigma_from_smith_waterman_distance_matrix",
     "shortest_longest_distance_in_matrix", "load_projections_from_file_into_dataset",
     "free_values_from_dataset", "print_SW_PCA_L_comparison",
+    "mnhn_gpu_initialization_values", "dbscan_L1_GPU", "dbscan_L2_GPU", "dbscan_L1", "dbscan_L2",
+    "mnhn_dbscan_vcsr", "shape_from_kmer_file", "load_kmer_from_file_into_dataset",
+    "create_single_cluster_file_with_values", "create_cluster_files_with_values", "free_dataset",
 ]
 
 
@@ -70,6 +76,16 @@ class Csr(ctypes.Structure):
                 ("row_ptr", ctypes.POINTER(ctypes.c_int64)),
                 ("col", ctypes.POINTER(ctypes.c_int32)),
                 ("dist", ctypes.POINTER(ctypes.c_int16))]
+
+
+class Vcsr(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_int32), ("norm", ctypes.c_int32), ("thr", ctypes.c_float),
+                ("reserved", ctypes.c_int32), ("nnz", ctypes.c_int64),
+                ("row_ptr", ctypes.POINTER(ctypes.c_int64)), ("col", ctypes.POINTER(ctypes.c_int32)),
+                ("val", ctypes.POINTER(ctypes.c_float))]
+
+
+NORM_L1, NORM_L2 = 1, 2
 
 
 class Timing(ctypes.Structure):
@@ -158,6 +174,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_encoder_load.argtypes = [vp, ctypes.c_char_p]
     lib.mnhn_prefilter_eps_csr.argtypes = [vp, vp, i32, ctypes.POINTER(Csr),
                                            ctypes.POINTER(PrefilterStats)]
+    lib.mnhn_vec_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.mnhn_vec_destroy.argtypes = [vp]
+    lib.mnhn_vec_destroy.restype = None
+    lib.mnhn_vec_eps_csr.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(Vcsr)]
+    lib.mnhn_vcsr_free.argtypes = [ctypes.POINTER(Vcsr)]
+    lib.mnhn_vcsr_free.restype = None
+    lib.mnhn_vec_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
     _LIB = lib
     return lib
 
@@ -310,6 +334,77 @@ def _csr_to_numpy(lib, csr, n):
     return rp, col, dist
 
 
+def vec_threshold(norm: int, eps: float) -> np.float32:
+    """The value the accumulator is compared with: eps * eps in float32 for
+    L2 (dbscan.c:577), eps for L1."""
+    e = np.float32(eps)
+    return np.float32(e * e) if norm == NORM_L2 else e
+
+
+class _VcsrOwner:
+    def __init__(self, lib, csr):
+        self.lib, self.csr = lib, csr
+
+    def __del__(self):
+        try:
+            self.lib.mnhn_vcsr_free(ctypes.byref(self.csr))
+        except Exception:
+            pass
+
+
+class VecEngine:
+    """Epsilon lists of float vectors under the reference's L1 / L2 scans on
+    the GPU (mnhn_vec.h).  points: (n, dims) float32 array."""
+
+    def __init__(self, points: np.ndarray):
+        self.lib = load_library()
+        pts = np.asarray(points, dtype=np.float32)
+        if pts.ndim != 2:
+            raise MnhnError("points must be (n, dims)")
+        self.n, self.dims = pts.shape
+        cols = [np.ascontiguousarray(pts[:, k]) for k in range(self.dims)]
+        ptrs = (ctypes.c_void_p * max(self.dims, 1))(*[c.ctypes.data for c in cols])
+        self.ctx = ctypes.c_void_p()
+        _check(self.lib.mnhn_vec_create(ptrs, self.dims, self.n, ctypes.byref(self.ctx)),
+               "mnhn_vec_create")
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.mnhn_vec_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def eps_csr(self, norm: int, thr: float):
+        """(row_ptr int64[n+1], col int32[nnz], val float32[nnz]): per row every
+        point (itself included) whose accumulator is <= thr, ascending."""
+        csr = Vcsr()
+        _check(self.lib.mnhn_vec_eps_csr(self.ctx, int(norm), ctypes.c_float(thr), ctypes.byref(csr)),
+               "mnhn_vec_eps_csr")
+        owner = _VcsrOwner(self.lib, csr)
+        nnz = int(csr.nnz)
+        rp = _view(csr.row_ptr, self.n + 1, owner)
+        col = _view(csr.col, nnz, owner) if nnz else np.zeros(0, np.int32)
+        val = _view(csr.val, nnz, owner) if nnz else np.zeros(0, np.float32)
+        return rp, col, val
+
+    def last_timing(self) -> dict:
+        ms, pairs = ctypes.c_double(0), ctypes.c_double(0)
+        _check(self.lib.mnhn_vec_last_timing(self.ctx, ctypes.byref(ms), ctypes.byref(pairs)),
+               "mnhn_vec_last_timing")
+        return {"kernel_ms": ms.value, "pairs": pairs.value}
+
+
 class Encoder:
     """Learned candidate proposer (mnhn_encoder): 5-mer profile -> 128-d MLP."""
 
@@ -455,6 +550,34 @@ def dbscan_csr(rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int, 
               col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
               dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
     s = lib.mnhn_dbscan_csr(ctypes.byref(csr), float(eps), int(minpts))
+    try:
+        out = []
+        for k in range(s.n_clusters):
+            c = s.clusters[k]
+            out.append(np.ctypeslib.as_array(c.members, shape=(c.n_members,)).tolist()
+                       if c.n_members else [])
+    finally:
+        lib.free_split_set_and_associated_clusters(s)
+    return out
+
+
+def dbscan_vcsr(rp: np.ndarray, col: np.ndarray, val: np.ndarray, norm: int, thr: float, eps: float,
+                minpts: int):
+    """One L1 / L2 DBSCAN pass (the library's C host code) over vector
+    epsilon lists computed up to thr; returns the clusters' member lists."""
+    lib = load_library()
+    lib.mnhn_dbscan_vcsr.restype = _SplitSet
+    lib.mnhn_dbscan_vcsr.argtypes = [ctypes.POINTER(Vcsr), ctypes.c_float, ctypes.c_int]
+    lib.free_split_set_and_associated_clusters.argtypes = [_SplitSet]
+    lib.free_split_set_and_associated_clusters.restype = None
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float32)
+    csr = Vcsr(len(rp) - 1, int(norm), float(thr), 0, len(col),
+               rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+               col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+               val.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    s = lib.mnhn_dbscan_vcsr(ctypes.byref(csr), float(eps), int(minpts))
     try:
         out = []
         for k in range(s.n_clusters):

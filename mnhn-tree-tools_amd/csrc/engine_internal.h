@@ -10,6 +10,13 @@
 
 #include "mnhn_sw.h"
 
+// Pinned staging buffers for large device-to-host copies (allocated on first
+// use, 64 MB each; released by stage_free).
+struct mnhn_detail_stage {
+  void *h[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
 struct mnhn_sw_ctx {
   int n = 0;
   bool acgt = true;
@@ -35,10 +42,7 @@ struct mnhn_sw_ctx {
   unsigned long long *d_keys = nullptr;
   size_t keys_cap = 0;
   uint64_t data_hash = 0;  // scores, count, lengths and bytes of the sequences
-  // pinned staging buffers for large device-to-host copies (allocated on
-  // first use, kStageBytes each)
-  void *h_stage[2] = {nullptr, nullptr};
-  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  mnhn_detail_stage stage;
 };
 
 namespace mnhn_detail {
@@ -118,9 +122,13 @@ void *host_alloc(size_t bytes);
 // Register-strip width the context's launches use (tuning, environment or default).
 int context_strip(const mnhn_sw_ctx *c);
 
-// Device-to-host copy through the ctx's two pinned staging buffers: chunk k+1
-// is in flight on the ctx's stream while chunk k is copied to dst (pageable).
-int copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes);
+// Device-to-host copy through two pinned staging buffers: chunk k+1 is in
+// flight on stream st while chunk k is copied to dst (pageable).
+int copy_d2h(hipStream_t st, mnhn_detail_stage &s, void *dst, const void *src, size_t bytes);
+inline int copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes) {
+  return copy_d2h(c->stream, c->stage, dst, src, bytes);
+}
+void stage_free(mnhn_detail_stage &s);
 
 // Group the ctx's epsilon keys (c->d_keys, nk of them) into CSR rows.
 int csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out);

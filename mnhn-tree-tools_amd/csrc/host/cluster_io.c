@@ -181,3 +181,26 @@ void free_split_set_and_associated_clusters(split_set s) {
   for (int i = 0; i < s.n_clusters; i++) free(s.clusters[i].members);
   free(s.clusters);
 }
+
+/* cluster_io.c:343-358: one line per member, its values "%f\t"-separated
+ * with "%f\n" after the last dimension */
+void create_single_cluster_file_with_values(char *filename, cluster cl, dataset ds) {
+  FILE *f = fopen(filename, "w");
+  if (!f) return;
+  for (int j = 0; j < cl.n_members; j++) {
+    for (int k = 0; k < ds.n_dimensions - 1; k++) fprintf(f, "%f\t", ds.values[k][cl.members[j]]);
+    if (ds.n_dimensions > 0) fprintf(f, "%f\n", ds.values[ds.n_dimensions - 1][cl.members[j]]);
+  }
+  fclose(f);
+}
+
+/* cluster_io.c:360-379: prefix-%03d per cluster */
+void create_cluster_files_with_values(char *prefix, split_set s, dataset ds) {
+  size_t plen = strlen(prefix);
+  char *name = (char *)malloc(plen + 32);
+  for (int i = 0; i < s.n_clusters; i++) {
+    snprintf(name, plen + 32, "%s-%03d", prefix, i);
+    create_single_cluster_file_with_values(name, s.clusters[i], ds);
+  }
+  free(name);
+}

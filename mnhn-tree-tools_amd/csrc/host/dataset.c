@@ -127,3 +127,81 @@ void free_values_from_dataset(dataset ds) {
   free(ds.values);
 }
 
+/* dataset.c:646-649 */
+void free_dataset(dataset ds) {
+  free_values_from_dataset(ds);
+  free_sequences_from_dataset(ds);
+}
+
+/* dataset.c:346-375: tabs before the first line break are the features; the
+ * line breaks from that one to the end of the file are the samples.  Read in
+ * blocks instead of whole; a file without a line break has 0 samples (the
+ * reference reads past its buffer there). */
+data_shape shape_from_kmer_file(int infile) {
+  data_shape s = {0, 0};
+  char buf[1 << 16];
+  ssize_t got;
+  int first_line = 1;
+  if (lseek(infile, 0, SEEK_SET) < 0) return s;
+  while ((got = read(infile, buf, sizeof(buf))) > 0) {
+    for (ssize_t i = 0; i < got; i++) {
+      if (first_line) {
+        if (buf[i] == '\t') s.n_features++;
+        if (buf[i] == '\n') {
+          first_line = 0;
+          s.n_samples++;
+        }
+      } else if (buf[i] == '\n') {
+        s.n_samples++;
+      }
+    }
+  }
+  return s;
+}
+
+/* dataset.c:378-425: per sample one whitespace-delimited label token, then
+ * n_features values as fscanf("%f") reads them (strtof accepts the same
+ * decimal, hexadecimal, inf and nan forms and rounds the same way).  The file
+ * is parsed from memory; values missing at the end of the file stay 0. */
+dataset load_kmer_from_file_into_dataset(FILE *in_file, data_shape shape) {
+  dataset ds;
+  memset(&ds, 0, sizeof(ds));
+  ds.n_dimensions = (int)shape.n_features;
+  ds.n_values = (int)shape.n_samples;
+  ds.values = (float **)malloc(sizeof(float *) * (shape.n_features ? shape.n_features : 1));
+  for (size_t k = 0; k < shape.n_features; k++) {
+    ds.values[k] = (float *)calloc((size_t)ds.n_values + 8, sizeof(float));
+    if (!ds.values[k]) {
+      printf("Error allocating aligned memory for kmers \n");
+      _exit(1);
+    }
+  }
+  rewind(in_file);
+  size_t cap = 1 << 20, len = 0;
+  char *text = (char *)xrealloc(NULL, cap);
+  size_t r;
+  while ((r = fread(text + len, 1, cap - len - 1, in_file)) > 0) {
+    len += r;
+    if (cap - len - 1 == 0) {
+      cap *= 2;
+      text = (char *)xrealloc(text, cap);
+    }
+  }
+  text[len] = 0;
+  char *p = text;
+  for (size_t i = 0; i < shape.n_samples; i++) {
+    while (*p && isspace((unsigned char)*p)) p++; /* label token */
+    while (*p && !isspace((unsigned char)*p)) p++;
+    for (size_t k = 0; k < shape.n_features; k++) {
+      char *end;
+      const float v = strtof(p, &end);
+      if (end == p) goto done;
+      ds.values[k][i] = v;
+      p = end;
+    }
+  }
+done:
+  free(text);
+  return ds;
+}
+

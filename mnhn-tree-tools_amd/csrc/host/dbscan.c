@@ -20,6 +20,10 @@
  *     the halving retries, float32 epsilon accumulation, the strictly
  *     decreasing layer filter, the relation side effect and %04d split files.
  *
+ * Vector datasets (dbscan_L1 / dbscan_L2, dbscan.c:563-780) follow the same
+ * pass with the accumulator lists of the vector engine (mnhn_vec_eps_csr),
+ * computed once up to the threshold of eps + lookahead.
+ *
  * Where the distances come from: every region query of a pass reads the
  * epsilon-neighbour lists (CSR) that the engine computes once for all unique
  * pairs up to eps + lookahead (mnhn_sw_eps_csr); later passes with a smaller
@@ -42,6 +46,8 @@ struct mnhn_eps_cache {
   float lookahead;
   int per_query_launch;
   int32_t *dist_row; /* per-query launch buffer [n] */
+  mnhn_vcsr vcsr;    /* vector datasets: lists up to vcsr.thr */
+  int have_v;
 };
 
 static void die_engine(const char *what, int rc) {
@@ -63,13 +69,27 @@ mnhn_gpu_stuff mnhn_gpu_initialization(dataset ds) {
   return g;
 }
 
+mnhn_gpu_stuff mnhn_gpu_initialization_values(dataset ds, int norm) {
+  mnhn_gpu_stuff g;
+  memset(&g, 0, sizeof(g));
+  if (norm != MNHN_NORM_L1 && norm != MNHN_NORM_L2) die_engine("mnhn_gpu_initialization_values", MNHN_ERR_ARG);
+  int rc = mnhn_vec_create((const float *const *)ds.values, ds.n_dimensions, ds.n_values, &g.vec);
+  if (rc) die_engine("mnhn_vec_create", rc);
+  g.cache = (mnhn_eps_cache *)calloc(1, sizeof(mnhn_eps_cache));
+  g.norm = norm;
+  g.num_devs = 1;
+  return g;
+}
+
 void mnhn_gpu_destroy(mnhn_gpu_stuff g) {
   if (g.cache) {
     if (g.cache->have) mnhn_csr_free(&g.cache->csr);
+    if (g.cache->have_v) mnhn_vcsr_free(&g.cache->vcsr);
     free(g.cache->dist_row);
     free(g.cache);
   }
   mnhn_sw_destroy(g.sw);
+  mnhn_vec_destroy(g.vec);
 }
 
 void mnhn_gpu_set_lookahead(mnhn_gpu_stuff g, float lookahead) {
@@ -222,18 +242,23 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   return ret;
 }
 
+/* dbscan.c:980-989: the -1 defaults and their warnings */
+static void apply_defaults(float *epsilon, int *minpts) {
+  if (*epsilon == -1) {
+    printf("Warning @dbscan: epsilon value is: %f\n", *epsilon);
+    *epsilon = 0.015f;
+    printf("Warning @dbscan: epsilon was changed to: %f\n", *epsilon);
+  }
+  if (*minpts == -1) {
+    printf("Warning @dbscan: minpts value is: %i\n", *minpts);
+    *minpts = 5;
+    printf("Warning @dbscan: minpts was changed to: %i\n", *minpts);
+  }
+}
+
 split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
   const int n = ds.n_values;
-  if (epsilon == -1) {
-    printf("Warning @dbscan: epsilon value is: %f\n", epsilon);
-    epsilon = 0.015f;
-    printf("Warning @dbscan: epsilon was changed to: %f\n", epsilon);
-  }
-  if (minpts == -1) {
-    printf("Warning @dbscan: minpts value is: %i\n", minpts);
-    minpts = 5;
-    printf("Warning @dbscan: minpts was changed to: %i\n", minpts);
-  }
+  apply_defaults(&epsilon, &minpts);
   if (n > 0) {
     if (g.cache->per_query_launch) {
       if (!g.cache->dist_row) g.cache->dist_row = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
@@ -248,11 +273,92 @@ split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts) {
   return dbscan_core(csr ? csr->n_rows : 0, epsilon, minpts, rq_csr, (void *)csr);
 }
 
+/* ---- vector datasets (dbscan.c:563-780) ---- */
+
+/* The threshold the accumulator is compared with: eps * eps in float for L2
+ * (dbscan.c:577), eps for L1. */
+static float vec_thr(int norm, float eps) { return norm == MNHN_NORM_L2 ? eps * eps : eps; }
+
+/* Lists up to the threshold of eps + lookahead (never below eps's own);
+ * passes whose threshold is <= the lists' reuse them. */
+static void ensure_vec_lists(mnhn_gpu_stuff g, float eps) {
+  mnhn_eps_cache *c = g.cache;
+  const float thr = vec_thr(g.norm, eps);
+  if (c->have_v && thr <= c->vcsr.thr) return;
+  if (c->have_v) mnhn_vcsr_free(&c->vcsr);
+  c->have_v = 0;
+  float t_max = vec_thr(g.norm, eps + c->lookahead);
+  if (!(t_max >= thr)) t_max = thr;
+  int rc = mnhn_vec_eps_csr(g.vec, g.norm, t_max, &c->vcsr);
+  if (rc) die_engine("mnhn_vec_eps_csr", rc);
+  c->have_v = 1;
+}
+
+/* N(p): row p's entries (ascending, p itself listed when its own accumulator
+ * qualifies) with accumulator <= thr. */
+static int region_query_vcsr(const mnhn_vcsr *csr, int p, float thr, int *out) {
+  int k = 0;
+  for (int64_t t = csr->row_ptr[p], e = csr->row_ptr[p + 1]; t < e; t++)
+    if (csr->val[t] <= thr) out[k++] = csr->col[t];
+  return k;
+}
+
+static int rq_vec(void *src, int n, int p, float eps, int *out) {
+  (void)n;
+  const mnhn_gpu_stuff *g = (const mnhn_gpu_stuff *)src;
+  return region_query_vcsr(&g->cache->vcsr, p, vec_thr(g->norm, eps), out);
+}
+
+static int rq_vcsr(void *src, int n, int p, float eps, int *out) {
+  (void)n;
+  const mnhn_vcsr *csr = (const mnhn_vcsr *)src;
+  return region_query_vcsr(csr, p, vec_thr(csr->norm, eps), out);
+}
+
+static split_set dbscan_vec(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g, int norm) {
+  if (!g.vec || g.norm != norm) {
+    printf("dbscan_L%d_GPU needs a context from mnhn_gpu_initialization_values with the same norm\n", norm);
+    fflush(stdout);
+    _exit(1);
+  }
+  apply_defaults(&epsilon, &minpts);
+  if (ds.n_values > 0) ensure_vec_lists(g, epsilon);
+  return dbscan_core(ds.n_values, epsilon, minpts, rq_vec, &g);
+}
+
+split_set dbscan_L1_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
+  return dbscan_vec(ds, epsilon, minpts, g, MNHN_NORM_L1);
+}
+
+split_set dbscan_L2_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
+  return dbscan_vec(ds, epsilon, minpts, g, MNHN_NORM_L2);
+}
+
+split_set dbscan_L1(dataset ds, float epsilon, int minpts) {
+  mnhn_gpu_stuff g = mnhn_gpu_initialization_values(ds, MNHN_NORM_L1);
+  split_set s = dbscan_L1_GPU(ds, epsilon, minpts, g);
+  mnhn_gpu_destroy(g);
+  return s;
+}
+
+split_set dbscan_L2(dataset ds, float epsilon, int minpts) {
+  mnhn_gpu_stuff g = mnhn_gpu_initialization_values(ds, MNHN_NORM_L2);
+  split_set s = dbscan_L2_GPU(ds, epsilon, minpts, g);
+  mnhn_gpu_destroy(g);
+  return s;
+}
+
+split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts) {
+  return dbscan_core(csr ? csr->n_rows : 0, epsilon, minpts, rq_vcsr, (void *)csr);
+}
+
 void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
                      dataset ds, float epsilon_start, float epsilon_inc, int minpts,
                      char *split_files_prefix, int n_threads) {
   (void)n_threads; /* epsilon values are scanned in order by one thread */
-  mnhn_gpu_stuff g = mnhn_gpu_initialization(ds);
+  mnhn_gpu_stuff g = dbscanner == dbscan_L2_GPU   ? mnhn_gpu_initialization_values(ds, MNHN_NORM_L2)
+                     : dbscanner == dbscan_L1_GPU ? mnhn_gpu_initialization_values(ds, MNHN_NORM_L1)
+                                                  : mnhn_gpu_initialization(ds);
   /* neighbour lists are computed up to 8 increments beyond the epsilon asked
    * for, so the sweep's next passes reuse them */
   mnhn_gpu_set_lookahead(g, 8.0f * (epsilon_inc > 0 ? epsilon_inc : 0));

@@ -1284,10 +1284,7 @@ void mnhn_sw_destroy(mnhn_sw_ctx *c) {
   dev_free(c->d_bound); dev_free(c->d_keys);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  for (int k = 0; k < 2; ++k) {
-    if (c->h_stage[k]) (void)hipHostFree(c->h_stage[k]);
-    if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
-  }
+  stage_free(c->stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1709,36 +1706,44 @@ void *mnhn_detail::host_alloc(size_t bytes) {
   return p;
 }
 
-int mnhn_detail::copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes) {
+int mnhn_detail::copy_d2h(hipStream_t st, mnhn_detail_stage &s, void *dst, const void *src, size_t bytes) {
   constexpr size_t kStageBytes = 64u << 20;
   if (bytes <= kStageBytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
   }
   for (int k = 0; k < 2; ++k) {
-    if (!c->h_stage[k] && hipHostMalloc(&c->h_stage[k], kStageBytes, hipHostMallocDefault) != hipSuccess) {
-      c->h_stage[k] = nullptr;
+    if (!s.h[k] && hipHostMalloc(&s.h[k], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+      s.h[k] = nullptr;
       return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
     }
-    if (!c->stage_ev[k] && hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming) != hipSuccess) {
-      c->stage_ev[k] = nullptr;
+    if (!s.ev[k] && hipEventCreateWithFlags(&s.ev[k], hipEventDisableTiming) != hipSuccess) {
+      s.ev[k] = nullptr;
       return MNHN_ERR_DEVICE;
     }
   }
   const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
   auto issue = [&](size_t k) -> bool {
     const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
-    return hipMemcpyAsync(c->h_stage[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, c->stream) ==
-               hipSuccess &&
-           hipEventRecord(c->stage_ev[k & 1], c->stream) == hipSuccess;
+    return hipMemcpyAsync(s.h[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipEventRecord(s.ev[k & 1], st) == hipSuccess;
   };
   if (!issue(0)) return MNHN_ERR_DEVICE;
   for (size_t k = 0; k < chunks; ++k) {
     if (k + 1 < chunks && !issue(k + 1)) return MNHN_ERR_DEVICE;
-    if (hipEventSynchronize(c->stage_ev[k & 1]) != hipSuccess) return MNHN_ERR_DEVICE;
+    if (hipEventSynchronize(s.ev[k & 1]) != hipSuccess) return MNHN_ERR_DEVICE;
     const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
-    memcpy((char *)dst + off, c->h_stage[k & 1], len);
+    memcpy((char *)dst + off, s.h[k & 1], len);
   }
   return MNHN_OK;
+}
+
+void mnhn_detail::stage_free(mnhn_detail_stage &s) {
+  for (int k = 0; k < 2; ++k) {
+    if (s.h[k]) (void)hipHostFree(s.h[k]);
+    if (s.ev[k]) (void)hipEventDestroy(s.ev[k]);
+    s.h[k] = nullptr;
+    s.ev[k] = nullptr;
+  }
 }
 
 extern "C" {

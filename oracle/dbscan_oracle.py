@@ -44,6 +44,10 @@ def _lib():
         lib.oracle_sw_matrix.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_int]
+        lib.oracle_vec_rows.restype = ctypes.c_int
+        lib.oracle_vec_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int]
         lib.oracle_cpu_allpairs.restype = ctypes.c_double
         lib.oracle_cpu_allpairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_int, ctypes.c_int,
@@ -286,15 +290,20 @@ def cluster_dbscan_sw(seqs: Sequence[bytes], eps: float, minpts: int,
 
 def adaptive_clustering(seqs: Sequence[bytes], eps_start: float, eps_inc: float,
                         minpts: int, dist: Optional[np.ndarray] = None,
-                        max_passes: int = 100000):
+                        max_passes: int = 100000,
+                        rq: Optional[Callable[[int, float], List[int]]] = None,
+                        n: Optional[int] = None):
     """adaptive_dbscan (dbscan.c:1117-1380) with one worker thread, and the
     main() prints of adaptive_clustering.c.  Returns (stdout text,
     {suffix: split-set bytes}, kept layers) or raises on the reference's
-    _exit(1) paths."""
-    if dist is None:
-        dist = sw_matrix(seqs)
-    n = len(seqs)
-    rq = region_query_from_matrix(dist)
+    _exit(1) paths.  rq / n replace the Smith-Waterman region query (e.g.
+    region_query_from_acc for the L1 / L2 builds)."""
+    if rq is None:
+        if dist is None:
+            dist = sw_matrix(seqs)
+        rq = region_query_from_matrix(dist)
+    if n is None:
+        n = len(seqs)
     f32 = np.float32
     eps_start = f32(eps_start)
     eps_inc = f32(eps_inc)
@@ -465,3 +474,113 @@ def compare_norms(seqs: Sequence[bytes], proj: np.ndarray) -> str:
             out.append("%i %i %s %s %s\n" % (i, j, _fmt_lf(float(d[i, j])), _fmt_lf(float(np.sqrt(l2))),
                                              _fmt_lf(l1)))
     return "".join(out)
+
+
+# --------------------------------------------------------------------------
+# L1 / L2 vector scans: dbscan.c:563-780 (-D_SCAN_L1 / -D_SCAN_L2),
+# dataset.c:346-469 (k-mer and projection files), cluster_io.c:343-379
+# --------------------------------------------------------------------------
+
+_STRTOF = None
+
+
+def _strtof(tok: bytes) -> np.float32:
+    """libc strtof, the conversion fscanf("%f") performs."""
+    global _STRTOF
+    if _STRTOF is None:
+        f = ctypes.CDLL(None).strtof
+        f.restype = ctypes.c_float
+        f.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+        _STRTOF = f
+    return np.float32(_STRTOF(tok, None))
+
+
+def read_projections(path: str, dims: int, n: int) -> np.ndarray:
+    """load_projections_from_file_into_dataset (dataset.c:427-469): n rows of
+    dims whitespace-separated floats; returns values[dim][row] (float32)."""
+    toks = open(path, "rb").read().split()
+    if len(toks) < n * dims:
+        raise ValueError("Error reading projections file")
+    v = np.array([_strtof(t) for t in toks[:n * dims]], np.float32)
+    return np.ascontiguousarray(v.reshape(n, dims).T)
+
+
+def kmer_shape(data: bytes):
+    """shape_from_kmer_file (dataset.c:346-375): (features = tabs before the
+    first line break, samples = line breaks from that one on)."""
+    first = data.index(b"\n")
+    return data[:first].count(b"\t"), data[first:].count(b"\n")
+
+
+def read_kmers(path: str) -> np.ndarray:
+    """load_kmer_from_file_into_dataset (dataset.c:378-425): per sample a
+    label token and n_features floats; returns values[feature][sample]."""
+    data = open(path, "rb").read()
+    nf, ns = kmer_shape(data)
+    toks = data.split()
+    v = np.zeros((nf, ns), np.float32)
+    k = 0
+    for i in range(ns):
+        k += 1  # label
+        for j in range(nf):
+            v[j, i] = _strtof(toks[k])
+            k += 1
+    return v
+
+
+def vec_acc(values: np.ndarray, norm: int, rows: Optional[Sequence[int]] = None,
+            nthreads: int = 8) -> np.ndarray:
+    """Accumulators of the query rows against every point, in the order of
+    region_query's AVX2 + FMA branch (vec_oracle.c); values is [dims][n]."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    dims, n = v.shape
+    rows = np.arange(n, dtype=np.int32) if rows is None else np.asarray(rows, np.int32)
+    ptrs = (ctypes.c_void_p * max(dims, 1))(*[v[k].ctypes.data for k in range(dims)])
+    out = np.empty((len(rows), n), np.float32)
+    rc = _lib().oracle_vec_rows(ptrs, dims, n, norm, rows.ctypes.data, len(rows), out.ctypes.data, nthreads)
+    if rc:
+        raise ValueError("norm must be 1 or 2")
+    return out
+
+
+def vec_threshold(norm: int, eps: float) -> np.float32:
+    """dbscan.c:577, 676-679: eps * eps in float32 for L2, eps for L1."""
+    e = np.float32(eps)
+    return np.float32(e * e) if norm == 2 else e
+
+
+def region_query_from_acc(acc: np.ndarray, norm: int) -> Callable[[int, float], List[int]]:
+    """region_query (dbscan.c:563-780): ascending x with acc(p, x) <= thr."""
+    def rq(point: int, eps: float) -> List[int]:
+        return np.nonzero(acc[point] <= vec_threshold(norm, eps))[0].tolist()
+
+    return rq
+
+
+def values_file_text(cl: Cluster, values: np.ndarray) -> bytes:
+    """create_single_cluster_file_with_values (cluster_io.c:343-358)."""
+    dims = values.shape[0]
+    out = []
+    for m in cl.members:
+        row = [float(values[k, m]) for k in range(dims)]
+        out.append("".join("%f\t" % x for x in row[:-1]) + "%f\n" % row[-1])
+    return "".join(out).encode()
+
+
+def cluster_dbscan_vec(seqs: Sequence[bytes], values: np.ndarray, norm: int, eps: float,
+                       minpts: int, with_values: bool = False):
+    """cluster_dbscan_pca.c:16-68 (norm 2) and cluster_dbscan_kmer.c:18-77:
+    returns (stdout text, {file: bytes}, split set); file keys are
+    'out-%03d' and, with_values, 'vals-%03d'."""
+    acc = vec_acc(values, norm)
+    lines: List[str] = []
+    s = dbscan(values.shape[1], eps, minpts, region_query_from_acc(acc, norm), lines)
+    lines.append("%u clusters obtained \n" % s.n_clusters)
+    files = {}
+    if s.n_clusters < 500:
+        for i, c in enumerate(s.clusters):
+            files["out-%03d" % i] = cluster_file_text(c, seqs)
+        if with_values:
+            for i, c in enumerate(s.clusters):
+                files["vals-%03d" % i] = values_file_text(c, values)
+    return "".join(lines), files, s

@@ -70,3 +70,26 @@ def read_case_files(name, meta):
         with open(os.path.join(d, fn), "rb") as f:
             files[fn] = f.read()
     return stdout, files
+
+
+def vector_cases(programs=None):
+    """(name, cmd.json) of the L1 / L2 vector-tool fixtures (and fasta2kmer)."""
+    d = os.path.join(GOLDEN, "vectors")
+    out = []
+    for name in sorted(os.listdir(d)):
+        with open(os.path.join(d, name, "cmd.json")) as f:
+            meta = json.load(f)
+        if programs is None or meta["program"] in programs:
+            out.append((name, meta))
+    return out
+
+
+def golden_arg(a):
+    """A fixture argument as the program received it: IN:<f> -> the input
+    path, EPS:<f> -> the epsilon text stored in that input."""
+    if a.startswith("IN:"):
+        return os.path.join(GOLDEN, "inputs", a[3:])
+    if a.startswith("EPS:"):
+        with open(os.path.join(GOLDEN, "inputs", a[4:])) as f:
+            return f.read().strip()
+    return a

@@ -20,6 +20,11 @@ Run in a container where the reference sources exist (default
                              find_satellite, compareSW, silhouette,
                              compare_norms (cmd.json:
                              argv with IN:<input> for copied inputs)
+  vectors/<case>/         -- the same for the L1 / L2 vector tools
+                             (cluster_dbscan_pca, cluster_dbscan_kmerL1/L2,
+                             adaptive_clustering_PCA / _kmer_L1 / _kmer_L2)
+                             and fasta2kmer; the k-mer inputs of the vector
+                             cases are fasta2kmer outputs of the reference
 
 The fixtures are data (inputs and expected outputs) only.
 """
@@ -128,6 +133,37 @@ def stride_fasta(src: str, dst: str, stride: int, n_records: int) -> None:
             g.write(b"".join(rec))
 
 
+def fma_pair(fused_in: bool, seed: int):
+    """Two 2-d float32 points p, x and an epsilon e whose float square t
+    separates the fused L2 accumulator of (p, x) from the unfused one: the
+    fused value is <= t < the unfused value when fused_in, the reverse
+    otherwise.  (The fused step is emulated in float64 here; the reference's
+    own output on the written files is what the fixtures record.)"""
+    import numpy as np
+    f32 = np.float32
+    rng = np.random.default_rng(seed)
+    while True:
+        p = rng.uniform(0, 4, size=2).astype(f32)
+        x = rng.uniform(0, 4, size=2).astype(f32)
+        d = (p - x).astype(f32)
+        acc_u = f32(0)
+        acc_f = f32(0)
+        for k in range(2):
+            acc_u = f32(f32(d[k] * d[k]) + acc_u)
+            acc_f = f32(np.float64(d[k]) * np.float64(d[k]) + np.float64(acc_f))
+        if acc_u == acc_f or (acc_f < acc_u) != fused_in:
+            continue
+        lo, hi = min(acc_u, acc_f), max(acc_u, acc_f)
+        e0 = f32(np.sqrt(np.float64(lo)))
+        for step in range(-4, 5):
+            e = e0
+            for _ in range(abs(step)):
+                e = np.nextafter(e, f32(np.inf) if step > 0 else f32(0))
+            t = f32(e * e)
+            if lo <= t < hi:
+                return p, x, float(e)
+
+
 def make_inputs(ref_dir: str) -> dict:
     d = os.path.join(HERE, "inputs")
     os.makedirs(d, exist_ok=True)
@@ -168,6 +204,28 @@ def make_inputs(ref_dir: str) -> dict:
         with open(os.path.join(d, name), "w") as g:
             for row in v:
                 g.write(" ".join("%.6f" % x for x in row) + "\n")
+    # PCA-like projections of c1_small: one centre per block of 40 records
+    r = np.random.default_rng(5)
+    centres = r.normal(scale=3.0, size=(8, 5))
+    v = np.repeat(centres, 40, axis=0) + r.normal(scale=0.5, size=(320, 5))
+    with open(os.path.join(d, "c1_small_proj5.txt"), "w") as g:
+        for row in v.astype(np.float32):
+            g.write("\t".join("%.7g" % x for x in row) + "\n")
+    # two-point cases whose L2 neighbourhood differs between the fused step
+    # fma(d, d, acc) and the unfused round(round(d * d) + acc)
+    for name, fused_in, seed in (("fma_in", True, 31), ("fma_out", False, 32)):
+        p_, x_, e_ = fma_pair(fused_in, seed)
+        synth.write_fasta(os.path.join(d, name + ".fasta"), [b"ACGTACGT", b"ACGTACGA"])
+        with open(os.path.join(d, name + "_proj.txt"), "w") as g:
+            for row in (p_, x_):
+                g.write(" ".join("%.9g" % x for x in row) + "\n")
+        with open(os.path.join(d, name + "_eps.txt"), "w") as g:
+            g.write("%.9g\n" % e_)
+    # k-mer count files written by the reference's fasta2kmer
+    for src, k, dst in (("c1_small.fasta", 3, "c1_small_k3.kmer"), ("c2_small.fasta", 4, "c2_small_k4.kmer")):
+        with open(os.path.join(d, dst), "wb") as g:
+            subprocess.run([os.path.join(ROOT, "oracle", "_ref", "fasta2kmer"), os.path.join(d, src), str(k), "1",
+                            "0"], stdout=g, check=True)
     # two sets of random 10-mers, large enough that compareSW's int sums wrap
     for name, seed in (("kmers10_a.fasta", 21), ("kmers10_b.fasta", 22)):
         r = np.random.default_rng(seed)
@@ -255,10 +313,41 @@ CONSUMER_CASES = [
 ]
 
 
-def run_consumer_case(case, inputs) -> str:
+# L1 / L2 vector tools and fasta2kmer; EPS:<file> = the epsilon written in that input
+VECTOR_CASES = [
+    ("pca_c1_e1", "cluster_dbscan_pca",
+     ["IN:c1_small.fasta", "IN:c1_small_proj5.txt", "5", "1.0", "5", "out", "vals"]),
+    ("pca_c1_e07_m3", "cluster_dbscan_pca",
+     ["IN:c1_small.fasta", "IN:c1_small_proj5.txt", "5", "0.7", "3", "out"]),
+    ("pca_c1_defaults", "cluster_dbscan_pca",
+     ["IN:c1_small.fasta", "IN:c1_small_proj5.txt", "5", "-1", "-1", "out", "vals"]),
+    ("pca_c1_3dims", "cluster_dbscan_pca",
+     ["IN:c1_small.fasta", "IN:c1_small_proj5.txt", "3", "1.2", "4", "out", "vals"]),
+    ("pca_test3", "cluster_dbscan_pca",
+     ["IN:test3.fasta", "IN:test3_proj.txt", "3", "120", "1", "out", "vals"]),
+    ("pca_fma_in", "cluster_dbscan_pca",
+     ["IN:fma_in.fasta", "IN:fma_in_proj.txt", "2", "EPS:fma_in_eps.txt", "1", "out", "vals"]),
+    ("pca_fma_out", "cluster_dbscan_pca",
+     ["IN:fma_out.fasta", "IN:fma_out_proj.txt", "2", "EPS:fma_out_eps.txt", "1", "out", "vals"]),
+    ("kmerL1_c1_k3_e60", "cluster_dbscan_kmerL1", ["IN:c1_small.fasta", "IN:c1_small_k3.kmer", "60", "5", "out"]),
+    ("kmerL1_c1_k3_e40", "cluster_dbscan_kmerL1", ["IN:c1_small.fasta", "IN:c1_small_k3.kmer", "40", "5", "out"]),
+    ("kmerL2_c1_k3_e8", "cluster_dbscan_kmerL2", ["IN:c1_small.fasta", "IN:c1_small_k3.kmer", "8", "5", "out"]),
+    ("kmerL2_c2_k4_e14", "cluster_dbscan_kmerL2", ["IN:c2_small.fasta", "IN:c2_small_k4.kmer", "14", "4", "out"]),
+    ("kmerL1_c2_k4_neg", "cluster_dbscan_kmerL1", ["IN:c2_small.fasta", "IN:c2_small_k4.kmer", "-3", "0", "out"]),
+    ("kmerL2_c2_k4_neg", "cluster_dbscan_kmerL2", ["IN:c2_small.fasta", "IN:c2_small_k4.kmer", "-10", "4", "out"]),
+    ("adaptive_pca_c1", "adaptive_clustering_PCA",
+     ["IN:c1_small.fasta", "0.9", "0.15", "5", "out", "1", "5", "IN:c1_small_proj5.txt"]),
+    ("adaptive_kmerL1_c1", "adaptive_clustering_kmer_L1", ["IN:c1_small_k3.kmer", "60", "4", "5", "out", "1"]),
+    ("adaptive_kmerL2_c2", "adaptive_clustering_kmer_L2", ["IN:c2_small_k4.kmer", "14", "3", "4", "out", "1"]),
+    ("fasta2kmer_test3_k5", "fasta2kmer", ["IN:test3.fasta", "5", "1", "0"]),
+    ("fasta2kmer_c2_k2", "fasta2kmer", ["IN:c2_small.fasta", "2", "3", "0"]),
+]
+
+
+def run_consumer_case(case, inputs, base: str = "consumers") -> str:
     name, prog, argv = case
     exe = os.path.join(ROOT, "oracle", "_ref", prog)
-    outdir = os.path.join(HERE, "consumers", name)
+    outdir = os.path.join(HERE, base, name)
     if os.path.exists(os.path.join(outdir, "cmd.json")):
         return name + " (kept)"
     with tempfile.TemporaryDirectory() as tmp:
@@ -268,6 +357,8 @@ def run_consumer_case(case, inputs) -> str:
                 dst = os.path.join(tmp, "in_" + a[3:])
                 shutil.copyfile(inputs[a[3:]], dst)
                 args.append(dst)
+            elif a.startswith("EPS:"):
+                args.append(open(inputs[a[4:]]).read().strip())
             else:
                 args.append(a)
         before = set(os.listdir(tmp))
@@ -305,6 +396,8 @@ def main() -> None:
         for name in ex.map(lambda c: run_case(c, inputs), cases):
             print("done", name, flush=True)
         for name in ex.map(lambda c: run_consumer_case(c, inputs), CONSUMER_CASES):
+            print("done", name, flush=True)
+        for name in ex.map(lambda c: run_consumer_case(c, inputs, "vectors"), VECTOR_CASES):
             print("done", name, flush=True)
 
 

@@ -33,12 +33,12 @@ def _declared_functions():
 
 def test_headers_declare_the_exported_list(pkg):
     assert _declared_functions() == (set(pkg.ABI_SYMBOLS) | set(pkg.HOST_SYMBOLS)
-                                     | set(pkg.PROPOSER_SYMBOLS))
+                                     | set(pkg.PROPOSER_SYMBOLS) | set(pkg.VEC_SYMBOLS))
 
 
 def test_library_loads_and_exports_every_symbol(pkg):
     lib = pkg.load_library()
-    for name in pkg.ABI_SYMBOLS + pkg.HOST_SYMBOLS + pkg.PROPOSER_SYMBOLS:
+    for name in pkg.ABI_SYMBOLS + pkg.HOST_SYMBOLS + pkg.PROPOSER_SYMBOLS + pkg.VEC_SYMBOLS:
         assert hasattr(lib, name), name
     assert lib.mnhn_sw_strerror(0) == b"ok"
     assert lib.mnhn_sw_strerror(5).startswith(b"score range")
@@ -232,3 +232,81 @@ def test_ecsr_roundtrip_and_rejects_damaged_files(pkg, tmp_path):
     pkg.csr_save(path, np.zeros(4, np.int64), np.zeros(0, np.int32), np.zeros(0, np.int16), 10, 7)
     r3, c3, d3, e3, h3 = pkg.csr_load(path)
     assert len(r3) == 4 and len(c3) == 0 and (e3, h3) == (10, 7)
+
+
+# ---- vector datasets (host code only) -----------------------------------------
+
+class DataShape(ctypes.Structure):
+    _fields_ = [("n_features", ctypes.c_size_t), ("n_samples", ctypes.c_size_t)]
+
+
+def test_kmer_loader_matches_oracle(host, oracle):
+    """shape_from_kmer_file / load_kmer_from_file_into_dataset read the
+    reference's fasta2kmer files as the oracle does (dataset.c:346-425)."""
+    lib, libc = host
+    lib.shape_from_kmer_file.restype = DataShape
+    lib.shape_from_kmer_file.argtypes = [ctypes.c_int]
+    lib.load_kmer_from_file_into_dataset.restype = Dataset
+    lib.load_kmer_from_file_into_dataset.argtypes = [ctypes.c_void_p, DataShape]
+    lib.free_values_from_dataset.argtypes = [Dataset]
+    libc.fdopen.restype = ctypes.c_void_p
+    libc.fdopen.argtypes = [ctypes.c_int, ctypes.c_char_p]
+    for fn in ("c1_small_k3.kmer", "c2_small_k4.kmer"):
+        path = os.path.join(GOLDEN, "inputs", fn)
+        fd = os.open(path, os.O_RDONLY)
+        shape = lib.shape_from_kmer_file(fd)
+        f = libc.fdopen(fd, b"r")
+        ds = lib.load_kmer_from_file_into_dataset(f, shape)
+        libc.fclose(f)
+        want = oracle.read_kmers(path)
+        assert (shape.n_features, shape.n_samples) == want.shape
+        vals = ctypes.cast(ds.values, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)))
+        got = np.array([[vals[k][i] for i in range(ds.n_values)] for k in range(ds.n_dimensions)],
+                       np.float32)
+        assert np.array_equal(got, want)
+        lib.free_values_from_dataset(ds)
+
+
+def test_values_cluster_files_match_oracle(host, oracle, tmp_path):
+    """create_cluster_files_with_values (cluster_io.c:343-379): prefix-%03d
+    files, "%f\\t" between values and "%f\\n" after the last."""
+    lib, _ = host
+    lib.create_cluster_files_with_values.argtypes = [ctypes.c_char_p, SplitSet, Dataset]
+    rng = np.random.default_rng(8)
+    dims, n = 3, 12
+    v = (rng.normal(size=(dims, n)) * 1000).astype(np.float32)
+    v[1, 3] = -0.0
+    v[2, 5] = 1e-8
+    rows = [np.ascontiguousarray(v[k]) for k in range(dims)]
+    ptrs = (ctypes.POINTER(ctypes.c_float) * dims)(
+        *[r.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for r in rows])
+    ds = Dataset(None, None, None, n, ctypes.cast(ptrs, ctypes.c_void_p), dims, 0)
+    cl = [([3, 1, 2], 0), ([5, 11, 0, 4], 1)]
+    s, keep = _mk_split(cl)
+    prefix = str(tmp_path / "vals")
+    lib.create_cluster_files_with_values(prefix.encode(), s, ds)
+    for i, (m, cid) in enumerate(cl):
+        got = open("%s-%03d" % (prefix, i), "rb").read()
+        assert got == oracle.values_file_text(oracle.Cluster(m, cid), v)
+
+
+def test_dbscan_over_vcsr_matches_oracle(pkg, oracle):
+    """mnhn_dbscan_vcsr (host code only) over vector lists built on the CPU
+    from the oracle's accumulators at a larger threshold gives the oracle's
+    clusters for every epsilon under it, L1 and L2."""
+    values = oracle.read_kmers(os.path.join(GOLDEN, "inputs", "c1_small_k3.kmer"))
+    n = values.shape[1]
+    for norm, e_max, passes in ((1, 70.0, ((40, 5), (55.5, 3), (70, 8), (-2.0, 0))),
+                                (2, 12.0, ((8, 5), (10.25, 4), (12, 2), (-9.0, 5)))):
+        acc = oracle.vec_acc(values, norm)
+        t_max = oracle.vec_threshold(norm, e_max)
+        keep = acc <= t_max
+        rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+        col = np.nonzero(keep)[1].astype(np.int32)
+        val = acc[keep].astype(np.float32)
+        for eps, minpts in passes:
+            if oracle.vec_threshold(norm, eps) > t_max:
+                continue
+            got = pkg.dbscan_vcsr(rp, col, val, norm, t_max, eps, minpts)
+            want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_acc(acc, norm))
+            assert got == [c.members for c in want.clusters], (norm, eps, minpts)

@@ -8,7 +8,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, cli_cases, read_case_files
+from conftest import GOLDEN, cli_cases, golden_arg, read_case_files, vector_cases
 
 
 # --------------------------------------------------------------------------
@@ -165,3 +165,129 @@ def test_oracle_consumers_match_reference(oracle, name, meta):
         assert oracle.compare_norms(inp(0), proj[:, :int(argv[2])]).encode() == want
     else:
         assert oracle.silhouette(inp(0), inp(1)).encode() == want
+
+
+# --------------------------------------------------------------------------
+# L1 / L2 vector tools (dbscan.c:563-780) against the compiled reference
+# --------------------------------------------------------------------------
+
+VEC_PROGRAMS = {"cluster_dbscan_pca": 2, "cluster_dbscan_kmerL1": 1, "cluster_dbscan_kmerL2": 2,
+                "adaptive_clustering_PCA": 2, "adaptive_clustering_kmer_L1": 1,
+                "adaptive_clustering_kmer_L2": 2}
+VEC_CASES = vector_cases(VEC_PROGRAMS)
+
+
+def vec_case_inputs(oracle, meta):
+    """(sequences or None, values[dims][n], norm, eps, minpts) of a fixture,
+    parsed as the reference's main() parses them."""
+    prog, a = meta["program"], [golden_arg(x) for x in meta["argv"]]
+    norm = VEC_PROGRAMS[prog]
+    f32 = lambda t: float(oracle._strtof(t.encode()))
+    if prog == "cluster_dbscan_pca":
+        seqs = oracle.read_fasta(a[0])
+        return seqs, oracle.read_projections(a[1], int(a[2]), len(seqs)), norm, f32(a[3]), int(a[4], 0)
+    if prog.startswith("cluster_dbscan_kmer"):
+        return oracle.read_fasta(a[0]), oracle.read_kmers(a[1]), norm, f32(a[2]), int(a[3], 0)
+    if prog == "adaptive_clustering_PCA":
+        seqs = oracle.read_fasta(a[0])
+        return seqs, oracle.read_projections(a[7], int(a[6]), len(seqs)), norm, f32(a[1]), int(a[3], 0)
+    return None, oracle.read_kmers(a[0]), norm, f32(a[1]), int(a[3], 0)
+
+
+def vec_case_expected(name, meta):
+    d = os.path.join(GOLDEN, "vectors", name)
+    want = open(os.path.join(d, "stdout.txt"), "rb").read()
+    files = {f: open(os.path.join(d, f), "rb").read() for f in meta["files"]}
+    return want, files
+
+
+@pytest.mark.parametrize("name,meta", VEC_CASES, ids=[c[0] for c in VEC_CASES])
+def test_oracle_vector_tools_match_reference(oracle, name, meta):
+    """The oracle's restatement of the L1 / L2 scans, DBSCAN and the tools'
+    main() reproduces the compiled reference's stdout and every file."""
+    want, files = vec_case_expected(name, meta)
+    seqs, values, norm, eps, minpts = vec_case_inputs(oracle, meta)
+    prog, argv = meta["program"], meta["argv"]
+    if prog.startswith("cluster_dbscan"):
+        with_values = prog == "cluster_dbscan_pca" and len(argv) == 7
+        text, got, _ = oracle.cluster_dbscan_vec(seqs, values, norm, eps, minpts, with_values)
+    else:
+        a = [golden_arg(x) for x in argv]
+        acc = oracle.vec_acc(values, norm)
+        text, out, _ = oracle.adaptive_clustering(None, float(oracle._strtof(a[1].encode())),
+                                                  float(oracle._strtof(a[2].encode())), minpts,
+                                                  rq=oracle.region_query_from_acc(acc, norm),
+                                                  n=values.shape[1])
+        got = {"out" + k: v for k, v in out.items()}
+    assert text.encode() == want
+    assert sorted(got) == sorted(files)
+    for k in files:
+        assert got[k] == files[k], k
+
+
+def _unfused_l2(values):
+    """The L2 accumulator with the multiply rounded before the add (the
+    reference's build without FMA), for the pinning test below."""
+    v = values.astype(np.float32)
+    n = v.shape[1]
+    acc = np.zeros((n, n), np.float32)
+    for k in range(v.shape[0]):
+        d = (v[k][:, None] - v[k][None, :]).astype(np.float32)
+        acc = (np.float32(1) * (d * d).astype(np.float32) + acc).astype(np.float32)
+    return acc
+
+
+@pytest.mark.parametrize("case,fused_in", [("pca_fma_in", True), ("pca_fma_out", False)])
+def test_vector_fma_cases_pin_the_fused_step(oracle, case, fused_in):
+    """The two-point fixtures separate the fused L2 step from the unfused
+    one: the reference's cluster count agrees with the fused accumulator and
+    not with the unfused one."""
+    meta = dict(vector_cases())[case]
+    seqs, values, norm, eps, minpts = vec_case_inputs(oracle, meta)
+    thr = oracle.vec_threshold(2, eps)
+    fused = oracle.vec_acc(values, 2)[0, 1]
+    unfused = _unfused_l2(values)[0, 1]
+    assert (fused <= thr) == fused_in and (unfused <= thr) != fused_in
+    want, _ = vec_case_expected(case, meta)
+    assert want == (b"1 clusters obtained \n" if fused_in else b"0 clusters obtained \n")
+
+
+def test_vector_oracle_closed_forms(oracle):
+    """Integer-valued vectors: every partial sum is an integer below 2^24, so
+    the float32 accumulators equal the exact L1 / squared L2 distances."""
+    rng = np.random.default_rng(3)
+    for dims, n in ((1, 7), (3, 40), (64, 50), (300, 20)):
+        x = rng.integers(-40, 40, size=(dims, n))
+        d = x[:, :, None] - x[:, None, :]
+        l1 = np.abs(d).sum(axis=0)
+        l2 = (d * d).sum(axis=0)
+        assert np.array_equal(oracle.vec_acc(x.astype(np.float32), 1), l1.astype(np.float32))
+        assert np.array_equal(oracle.vec_acc(x.astype(np.float32), 2), l2.astype(np.float32))
+        rows = rng.choice(n, size=min(n, 5), replace=False)
+        assert np.array_equal(oracle.vec_acc(x.astype(np.float32), 2, rows), l2[rows].astype(np.float32))
+    # non-integer data: within float32 rounding of the float64 distances, with
+    # the difference taken as query minus candidate
+    x = rng.normal(size=(5, 30)).astype(np.float32)
+    d = x[:, :, None].astype(np.float64) - x[:, None, :].astype(np.float64)
+    assert np.allclose(oracle.vec_acc(x, 2), (d * d).sum(axis=0), rtol=1e-5, atol=1e-6)
+    assert np.allclose(oracle.vec_acc(x, 1), np.abs(d).sum(axis=0), rtol=1e-5, atol=1e-6)
+    # NaN / inf inputs: the comparisons fail, the point is nobody's neighbour
+    y = x.copy()
+    y[2, 4] = np.nan
+    y[0, 7] = np.inf
+    acc = oracle.vec_acc(y, 2)
+    assert np.isnan(acc[4]).all() and np.isnan(acc[:, 4]).all() and np.isnan(acc[7, 7])
+
+
+def test_vector_oracle_parsers(oracle):
+    """k-mer shape / values and projection parsing against the fasta2kmer
+    format (kmers.c:378-389) and numpy."""
+    path = os.path.join(GOLDEN, "inputs", "c2_small_k4.kmer")
+    data = open(path, "rb").read()
+    assert oracle.kmer_shape(data) == (256, 256)
+    rows = [ln.split(b"\t") for ln in data.decode().encode().split(b"\n") if ln]
+    v = oracle.read_kmers(path)
+    assert np.array_equal(v, np.array([[float(t) for t in r[1:]] for r in rows], np.float32).T)
+    p = oracle.read_projections(os.path.join(GOLDEN, "inputs", "c1_small_proj5.txt"), 5, 320)
+    ref = np.loadtxt(os.path.join(GOLDEN, "inputs", "c1_small_proj5.txt"), dtype=np.float64)
+    assert np.array_equal(p, ref.astype(np.float32).T)

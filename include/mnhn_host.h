@@ -16,6 +16,8 @@
  *   dbscan_SW_GPU                         dbscan.h:36-37; dbscan.c:950-1047
  *   dbscan_L1 / dbscan_L2                 dbscan.h:30-31; dbscan.c:563-780, 950-1047
  *   adaptive_dbscan                       dbscan.h:38-48; dbscan.c:1117-1380
+ *   kmer_frequencies / frequencies_from_dataset / write_kmer_base
+ *                                         kmers.h:19-23, 72-113; kmers.c:67-84, 288-398
  * The Smith-Waterman distances all come from the gfx950 engine
  * (mnhn_sw.h) and the L1 / L2 vector distances from the gfx950 vector
  * engine (mnhn_vec.h); there is no CPU distance path in this library.
@@ -29,6 +31,7 @@
 
 #include "mnhn_sw.h"
 #include "mnhn_vec.h"
+#include "mnhn_kmer.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -145,6 +148,28 @@ void create_single_cluster_file_with_values(char *filename, cluster cl, dataset 
 void create_cluster_files_with_values(char *prefix, split_set s, dataset ds);
 /* dataset.c:646-649 */
 void free_dataset(dataset ds);
+
+/* ---- k-mer frequencies (fasta2kmer) ----
+ * kmers.h:19-23: frequencies[sequence][k-mer] in the column order of
+ * mnhn_kmer.h.  Here the rows share one block: release the table with
+ * free_kmer_frequencies, not row by row. */
+typedef struct {
+  short int **frequencies;
+  size_t n_kmers;
+  size_t n_seq;
+} kmer_frequencies;
+/* kmers.c:67-84: 4^k and 20^k; 0 for k = 0 (the reference loops there) */
+size_t number_of_kmers_from_length(size_t kmer_length);
+size_t number_of_protein_kmers_from_length(size_t protein_kmer_length);
+/* kmers.c:288-376, counted on the GPU (mnhn_kmer_frequencies); n_threads is
+ * accepted and unused.  A k outside 1..15 (DNA) / 1..7 (protein) or a device
+ * failure prints the reason and exits. */
+kmer_frequencies frequencies_from_dataset(dataset ds, size_t kmer_length, int n_threads, int protein);
+/* kmers.c:378-389: "sequence_%i\t", then the counts "%hi" tab-separated and
+ * "\n" after the last */
+void write_kmer_base(FILE *f, kmer_frequencies freq);
+/* kmers.c:391-398 */
+void free_kmer_frequencies(kmer_frequencies freq);
 
 /* ---- the other Smith-Waterman consumers (distances from the gfx950 engine;
  * n_threads arguments are accepted and unused: one device pass) ---- */

@@ -41,6 +41,7 @@ PROPOSER_SYMBOLS = [
 ]
 VEC_SYMBOLS = [
     "mnhn_vec_create", "mnhn_vec_destroy", "mnhn_vec_eps_csr", "mnhn_vcsr_free", "mnhn_vec_last_timing",
+    "mnhn_kmer_columns", "mnhn_kmer_frequencies",
 ]
 HOST_SYMBOLS = [
     "dataset_from_fasta", "free_sequences_from_dataset", "store_split_set",
@@ -58,6 +59,8 @@ HOST_SYMBOLS = [
     "mnhn_gpu_initialization_values", "dbscan_L1_GPU", "dbscan_L2_GPU", "dbscan_L1", "dbscan_L2",
     "mnhn_dbscan_vcsr", "shape_from_kmer_file", "load_kmer_from_file_into_dataset",
     "create_single_cluster_file_with_values", "create_cluster_files_with_values", "free_dataset",
+    "number_of_kmers_from_length", "number_of_protein_kmers_from_length", "frequencies_from_dataset",
+    "write_kmer_base", "free_kmer_frequencies",
 ]
 
 
@@ -182,6 +185,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mnhn_vcsr_free.restype = None
     lib.mnhn_vec_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
+    lib.mnhn_kmer_columns.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.mnhn_kmer_columns.restype = ctypes.c_size_t
+    lib.mnhn_kmer_frequencies.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     _LIB = lib
     return lib
 
@@ -403,6 +409,23 @@ class VecEngine:
         _check(self.lib.mnhn_vec_last_timing(self.ctx, ctypes.byref(ms), ctypes.byref(pairs)),
                "mnhn_vec_last_timing")
         return {"kernel_ms": ms.value, "pairs": pairs.value}
+
+
+def kmer_frequencies(seqs: Sequence[bytes], k: int, protein: bool = False) -> np.ndarray:
+    """(n, 4^k or 20^k) int16 k-mer counts on the GPU (mnhn_kmer.h): the
+    table of the reference's frequencies_from_dataset."""
+    lib = load_library()
+    nk = int(lib.mnhn_kmer_columns(int(k), int(bool(protein))))
+    if nk == 0:
+        raise MnhnError(f"k-mer length {k} not supported")
+    n = len(seqs)
+    bufs = [ctypes.create_string_buffer(x, len(x) + 1) for x in seqs]
+    ptrs = (ctypes.c_char_p * max(n, 1))(*[ctypes.cast(b, ctypes.c_char_p) for b in bufs])
+    lens = (ctypes.c_size_t * max(n, 1))(*[len(x) for x in seqs])
+    out = np.empty((n, nk), np.int16)
+    _check(lib.mnhn_kmer_frequencies(ptrs, lens, n, int(k), int(bool(protein)), out.ctypes.data),
+           "mnhn_kmer_frequencies")
+    return out
 
 
 class Encoder:

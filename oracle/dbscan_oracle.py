@@ -584,3 +584,47 @@ def cluster_dbscan_vec(seqs: Sequence[bytes], values: np.ndarray, norm: int, eps
             for i, c in enumerate(s.clusters):
                 files["vals-%03d" % i] = values_file_text(c, values)
     return "".join(lines), files, s
+
+
+# --------------------------------------------------------------------------
+# k-mer frequencies: kmers.c:26-243, 288-389 (fasta2kmer.c)
+# --------------------------------------------------------------------------
+
+PROTEIN_ALPHABET = b"ACDEFGHIKLMNPQRSTVWY"  # kmers.c:139-142
+
+
+def kmer_names(k: int, protein: bool) -> List[bytes]:
+    """The k-mer strings in output order: DNA index i spells base p from bits
+    2p, 2p+1 of i (A C G T = 0..3, first base in the low bits;
+    gen_dna_string_from_binary, kmers.c:26-65); protein k-mers in the
+    recursion's order, first residue varying slowest (kmers.c:105-157)."""
+    if protein:
+        out = [b""]
+        for _ in range(k):
+            out = [w + bytes([c]) for w in out for c in PROTEIN_ALPHABET]
+        return out
+    return [bytes(b"ACGT"[(i >> (2 * p)) & 3] for p in range(k)) for i in range(4 ** k)]
+
+
+def kmer_frequencies(seqs: Sequence[bytes], k: int, protein: bool) -> np.ndarray:
+    """frequence_of_kmer_in_sequence (kmers.c:159-243) for every sequence and
+    k-mer: occurrences at the start positions i < len - k (the window that
+    ends at the last residue is not counted), in a short int that wraps."""
+    names = kmer_names(k, protein)
+    index = {w: j for j, w in enumerate(names)}
+    out = np.zeros((len(seqs), len(names)), np.int64)
+    for s, seq in enumerate(seqs):
+        for i in range(max(0, len(seq) - k)):
+            j = index.get(seq[i:i + k])
+            if j is not None:
+                out[s, j] += 1
+    return ((out + 32768) % 65536 - 32768).astype(np.int16)
+
+
+def kmer_text(freq: np.ndarray) -> bytes:
+    """write_kmer_base (kmers.c:378-389): "sequence_<i>\\t", the counts
+    "%hi"-formatted and tab-separated, a line break after the last."""
+    lines = []
+    for i, row in enumerate(freq.tolist()):
+        lines.append("sequence_%i\t" % i + "\t".join(str(v) for v in row) + "\n")
+    return "".join(lines).encode()

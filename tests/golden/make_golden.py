@@ -221,6 +221,20 @@ def make_inputs(ref_dir: str) -> dict:
                 g.write(" ".join("%.9g" % x for x in row) + "\n")
         with open(os.path.join(d, name + "_eps.txt"), "w") as g:
             g.write("%.9g\n" % e_)
+    # k-mer counting edge cases: empty record, shorter than / equal to k,
+    # N and other bytes, lower case, a poly-A run long enough that the short
+    # counter wraps, sequences above and below the reference's 16-byte path
+    r = np.random.default_rng(41)
+    rnd = lambda m: synth.to_bytes(synth.random_codes(r, m))
+    edge = [b"", b"AC", b"ACG", b"ACGT", b"ACGTNACGTTTN-ACG", b"acgtacgtAAACCCGGGTTT", rnd(15), rnd(16),
+            rnd(17), rnd(300), b"A" * 40000, b"ACGT" * 9000]
+    synth.write_fasta(os.path.join(d, "kmer_edge.fasta"), edge)
+    synth.write_fasta(os.path.join(d, "one_seq.fasta"), [rnd(2000)])
+    prot = b"ACDEFGHIKLMNPQRSTVWYXBZ*"
+    pseqs = [bytes(r.choice(list(prot), size=int(r.integers(0, 80))).astype(np.uint8)) for _ in range(20)]
+    pseqs[3] = pseqs[3].lower()
+    synth.write_fasta(os.path.join(d, "protein.fasta"), pseqs)
+    synth.write_fasta(os.path.join(d, "protein8.fasta"), pseqs[:8])
     # k-mer count files written by the reference's fasta2kmer
     for src, k, dst in (("c1_small.fasta", 3, "c1_small_k3.kmer"), ("c2_small.fasta", 4, "c2_small_k4.kmer")):
         with open(os.path.join(d, dst), "wb") as g:
@@ -341,6 +355,12 @@ VECTOR_CASES = [
     ("adaptive_kmerL2_c2", "adaptive_clustering_kmer_L2", ["IN:c2_small_k4.kmer", "14", "3", "4", "out", "1"]),
     ("fasta2kmer_test3_k5", "fasta2kmer", ["IN:test3.fasta", "5", "1", "0"]),
     ("fasta2kmer_c2_k2", "fasta2kmer", ["IN:c2_small.fasta", "2", "3", "0"]),
+    ("fasta2kmer_edge_k3", "fasta2kmer", ["IN:kmer_edge.fasta", "3", "2", "0"]),
+    ("fasta2kmer_edge_k1", "fasta2kmer", ["IN:kmer_edge.fasta", "1", "1", "0"]),
+    ("fasta2kmer_test3_k7", "fasta2kmer", ["IN:test3.fasta", "7", "1", "0"]),
+    ("fasta2kmer_one_k8", "fasta2kmer", ["IN:one_seq.fasta", "8", "1", "0"]),
+    ("fasta2kmer_prot_k2", "fasta2kmer", ["IN:protein.fasta", "2", "4", "1"]),
+    ("fasta2kmer_prot_k3", "fasta2kmer", ["IN:protein8.fasta", "3", "1", "1"]),
 ]
 
 

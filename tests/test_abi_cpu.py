@@ -310,3 +310,41 @@ def test_dbscan_over_vcsr_matches_oracle(pkg, oracle):
             got = pkg.dbscan_vcsr(rp, col, val, norm, t_max, eps, minpts)
             want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_acc(acc, norm))
             assert got == [c.members for c in want.clusters], (norm, eps, minpts)
+
+
+class KmerFreq(ctypes.Structure):
+    _fields_ = [("frequencies", ctypes.POINTER(ctypes.POINTER(ctypes.c_short))),
+                ("n_kmers", ctypes.c_size_t), ("n_seq", ctypes.c_size_t)]
+
+
+def test_write_kmer_base_matches_oracle(host, oracle, tmp_path):
+    """write_kmer_base (kmers.c:378-389), host only: the hand formatter gives
+    printf's "%hi" text, extremes and negative (wrapped) counts included."""
+    lib, libc = host
+    lib.write_kmer_base.argtypes = [ctypes.c_void_p, KmerFreq]
+    rng = np.random.default_rng(2)
+    for n_seq, nk in ((3, 4), (5, 64), (2, 1)):
+        tab = rng.integers(-32768, 32768, size=(n_seq, nk)).astype(np.int16)
+        tab[0, 0] = -32768
+        tab[-1, -1] = 32767
+        tab[0, -1] = 0
+        rows = (ctypes.POINTER(ctypes.c_short) * n_seq)(
+            *[tab[i].ctypes.data_as(ctypes.POINTER(ctypes.c_short)) for i in range(n_seq)])
+        path = str(tmp_path / "k.txt")
+        f = libc.fopen(path.encode(), b"w")
+        lib.write_kmer_base(f, KmerFreq(rows, nk, n_seq))
+        libc.fclose(f)
+        assert open(path, "rb").read() == oracle.kmer_text(tab)
+
+
+def test_kmer_column_counts(pkg):
+    lib = pkg.load_library()
+    lib.number_of_kmers_from_length.restype = ctypes.c_size_t
+    lib.number_of_kmers_from_length.argtypes = [ctypes.c_size_t]
+    lib.number_of_protein_kmers_from_length.restype = ctypes.c_size_t
+    lib.number_of_protein_kmers_from_length.argtypes = [ctypes.c_size_t]
+    for k in range(1, 12):
+        assert lib.number_of_kmers_from_length(k) == 4 ** k
+        assert lib.number_of_protein_kmers_from_length(k) == 20 ** k
+    assert [lib.mnhn_kmer_columns(k, 0) for k in (0, 1, 15, 16)] == [0, 4, 4 ** 15, 0]
+    assert [lib.mnhn_kmer_columns(k, 1) for k in (0, 1, 7, 8)] == [0, 20, 20 ** 7, 0]

@@ -291,3 +291,42 @@ def test_vector_oracle_parsers(oracle):
     p = oracle.read_projections(os.path.join(GOLDEN, "inputs", "c1_small_proj5.txt"), 5, 320)
     ref = np.loadtxt(os.path.join(GOLDEN, "inputs", "c1_small_proj5.txt"), dtype=np.float64)
     assert np.array_equal(p, ref.astype(np.float32).T)
+
+
+KMER_CASES = vector_cases({"fasta2kmer"})
+
+
+@pytest.mark.parametrize("name,meta", KMER_CASES, ids=[c[0] for c in KMER_CASES])
+def test_oracle_fasta2kmer_matches_reference(oracle, name, meta):
+    """The oracle's k-mer counting (kmers.c:159-243 with its quirks: the last
+    window not counted, wrapping short counts, column orders) reproduces the
+    compiled reference's fasta2kmer output."""
+    a = meta["argv"]
+    seqs = oracle.read_fasta(golden_arg(a[0]))
+    want, _ = vec_case_expected(name, meta)
+    assert oracle.kmer_text(oracle.kmer_frequencies(seqs, int(a[1]), bool(int(a[3])))) == want
+
+
+def test_oracle_kmer_inputs_are_reference_outputs(oracle):
+    """The k-mer inputs of the vector fixtures were written by the reference's
+    fasta2kmer; the oracle reproduces them too."""
+    for fasta, k, kmer in (("c1_small.fasta", 3, "c1_small_k3.kmer"), ("c2_small.fasta", 4, "c2_small_k4.kmer")):
+        seqs = oracle.read_fasta(os.path.join(GOLDEN, "inputs", fasta))
+        want = open(os.path.join(GOLDEN, "inputs", kmer), "rb").read()
+        assert oracle.kmer_text(oracle.kmer_frequencies(seqs, k, False)) == want
+
+
+def test_kmer_oracle_closed_forms(oracle):
+    """Brute force on tiny inputs: row sums are the number of valid windows
+    below len - k, DNA column names decode little-endian, protein names are
+    lexicographic in the reference's alphabet."""
+    names = oracle.kmer_names(2, False)
+    assert names[:5] == [b"AA", b"CA", b"GA", b"TA", b"AC"]
+    assert oracle.kmer_names(2, True)[:3] == [b"AA", b"AC", b"AD"] and oracle.kmer_names(1, True)[-1] == b"Y"
+    rng = np.random.default_rng(9)
+    for _ in range(30):
+        s = bytes(rng.choice(list(b"ACGTN"), size=int(rng.integers(0, 40))).astype(np.uint8))
+        for k in (1, 2, 3):
+            f = oracle.kmer_frequencies([s], k, False)[0]
+            valid = sum(1 for i in range(max(0, len(s) - k)) if b"N" not in s[i:i + k])
+            assert int(f.astype(np.int64).sum()) == valid

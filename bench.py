@@ -32,7 +32,13 @@ Other configurations:
   --config c5  end-to-end DBSCAN on 500,000 heavy-tailed sequences: exact
                lists + DBSCAN against prefilter lists + DBSCAN;
   --config consumers  wall time of the drop-in compareSW, silhouette and
-               find_closest_sequence_SW CLIs on two 10,000-sequence sets.
+               find_closest_sequence_SW CLIs on two 10,000-sequence sets;
+  --config vec  the L1 / L2 vector path: epsilon lists of 200,000 clustered
+               5-d projections (L2) and of 1024-d 5-mer count vectors of the
+               20,000-sequence tree set (L1), fasta2kmer on that set, and the
+               drop-in cluster_dbscan_pca / cluster_dbscan_kmerL1 / fasta2kmer
+               CLIs against the compiled reference on bounded samples (outputs
+               compared byte for byte there).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4|c5]
 """
@@ -526,6 +532,173 @@ def run_consumers_bench(args, log):
     print(json.dumps(out), flush=True)
 
 
+# MI355X published fp32 vector peak (an FMA counts 2 flops)
+PEAK_FP32_TFLOPS = 157.3
+
+
+def _run_cli(argv, cwd, timeout=1800):
+    t0 = time.perf_counter()
+    r = subprocess.run(argv, cwd=cwd, capture_output=True, timeout=timeout)
+    secs = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(" ".join(argv[:1]) + ": " + r.stdout.decode()[-300:] + r.stderr.decode()[-300:])
+    return secs, r.stdout
+
+
+def _outputs(d, prefix):
+    return {f: open(os.path.join(d, f), "rb").read() for f in sorted(os.listdir(d)) if f.startswith(prefix)}
+
+
+def run_vec_bench(args, pkg, log):
+    """Epsilon-list kernel rates of the vector engine and the drop-in vector
+    CLIs against the compiled reference (oracle/_ref) on bounded samples."""
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    have_ref = os.path.exists(os.path.join(ref_dir, "cluster_dbscan_pca"))
+    rng = np.random.default_rng(11)
+    # clustered 5-d projections: 2,000 centres with lognormal weights
+    n, dims = (args.n or 200_000), 5
+    centres = rng.normal(scale=12.0, size=(2000, dims))
+    w = rng.lognormal(0.0, 1.0, 2000)
+    lab = rng.choice(2000, n, p=w / w.sum())
+    pts = (centres[lab] + rng.normal(scale=0.3, size=(n, dims))).astype(np.float32)
+    eps_pca, minpts = 0.5, 5
+    thr = pkg.vec_threshold(2, eps_pca)
+    res = {}
+    with pkg.VecEngine(pts) as eng:
+        for _ in range(args.warmup):
+            eng.eps_csr(2, thr)
+        times, kms = [], []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            rp, col, val = eng.eps_csr(2, thr)
+            times.append(time.perf_counter() - t0)
+            kms.append(eng.last_timing()["kernel_ms"])
+            log(f"L2 step {k}: {times[-1] * 1e3:.1f} ms (tile kernel {kms[-1]:.1f} ms), nnz {len(col)}")
+        nnz = len(col)
+        del rp, col, val
+    pairs = n * (n + 1) / 2
+    kernel_s = float(np.mean(kms)) / 1e3
+    res["l2_pca"] = {"n": n, "dims": dims, "eps": eps_pca, "list_entries": nnz,
+                     "mean_neighbours": nnz / n, "step_ms": float(np.mean(times)) * 1e3,
+                     "kernel_ms": kernel_s * 1e3, "pair_dims_per_s": pairs * dims / kernel_s,
+                     "pairs_per_s_step": pairs / float(np.mean(times))}
+    # 5-mer count vectors of the tree set (counted on the GPU)
+    seqs = synth.config2()
+    t0 = time.perf_counter()
+    km = pkg.kmer_frequencies(seqs, 5, False)
+    kmer_s = time.perf_counter() - t0
+    kv = km.astype(np.float32)
+    eps_l1 = 150.0
+    with pkg.VecEngine(kv) as eng:
+        eng.eps_csr(1, eps_l1)
+        t0 = time.perf_counter()
+        rp, col, val = eng.eps_csr(1, eps_l1)
+        l1_s = time.perf_counter() - t0
+        l1_kernel_s = eng.last_timing()["kernel_ms"] / 1e3
+    m = len(seqs)
+    res["l1_kmer"] = {"n": m, "dims": 1024, "eps": eps_l1, "list_entries": int(len(col)),
+                      "step_ms": l1_s * 1e3, "kernel_ms": l1_kernel_s * 1e3,
+                      "pair_dims_per_s": m * (m + 1) / 2 * 1024 / l1_kernel_s,
+                      "kmer_counts_s": kmer_s}
+    log(f"L1 k-mer lists: {l1_s * 1e3:.1f} ms (tile kernel {l1_kernel_s * 1e3:.1f} ms), nnz {len(col)}; "
+        f"5-mer table of {m} sequences {kmer_s * 1e3:.1f} ms")
+    # flops per pair-dimension: sub + fma (L2) = 3, sub + add (L1) = 2
+    achieved = 3 * res["l2_pca"]["pair_dims_per_s"] / 1e12
+    res["l1_kmer"]["tflops"] = 2 * res["l1_kmer"]["pair_dims_per_s"] / 1e12
+    res["l1_kmer"]["frac_of_published_fp32"] = res["l1_kmer"]["tflops"] / PEAK_FP32_TFLOPS
+    out = {"metric": "L2 epsilon-list pair-dimensions per second (vector DBSCAN scans)",
+           "value": res["l2_pca"]["pair_dims_per_s"], "unit": "pair-dims/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["l2_pca"]["step_ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"L2 epsilon lists of {n} clustered 5-d projections (2,000 centres), "
+                                  f"eps {eps_pca}; L1 lists of 1024-d 5-mer vectors of 20,000 tree sequences",
+                      "runs": res},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s (fp32 VALU; per pair-dimension a sub and an fma = 3 flop)",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                        "kernel": "vec_tiles_kernel<2>"}}
+    # drop-in CLIs end to end, and the reference on samples (outputs compared)
+    with tempfile.TemporaryDirectory() as tmp:
+        fa = os.path.join(tmp, "p.fasta")
+        fp = os.path.join(tmp, "p.txt")
+        synth.write_fasta(fa, [b"ACGT"] * n)
+        np.savetxt(fp, pts, fmt="%.7g")
+        os.makedirs(os.path.join(tmp, "g"))
+        secs, so = _run_cli([os.path.join(PKG_DIR, "bin", "cluster_dbscan_pca"), fa, fp, "5", str(eps_pca),
+                             str(minpts), "o"], os.path.join(tmp, "g"))
+        res["cli_pca"] = {"n": n, "s": secs, "stdout": so.decode().strip()}
+        log(f"cluster_dbscan_pca {n}: {secs:.2f} s, {so.decode().strip()}")
+        fk = os.path.join(tmp, "k.fasta")
+        synth.write_fasta(fk, seqs)
+        secs, ktext = _run_cli([os.path.join(PKG_DIR, "bin", "fasta2kmer"), fk, "5", "1", "0"], tmp)
+        res["cli_fasta2kmer"] = {"n": m, "s": secs, "bytes": len(ktext)}
+        log(f"fasta2kmer {m} seqs k=5: {secs:.2f} s ({len(ktext) / 1e6:.0f} MB)")
+        kf = os.path.join(tmp, "k.kmer")
+        with open(kf, "wb") as f:
+            f.write(ktext)
+        os.makedirs(os.path.join(tmp, "gk"))
+        secs, so = _run_cli([os.path.join(PKG_DIR, "bin", "cluster_dbscan_kmerL1"), fk, kf, str(eps_l1), "5",
+                             "o"], os.path.join(tmp, "gk"))
+        res["cli_kmerL1"] = {"n": m, "s": secs, "stdout": so.decode().strip()}
+        log(f"cluster_dbscan_kmerL1 {m}: {secs:.2f} s, {so.decode().strip()}")
+        if have_ref and not args.no_cpu_baseline:
+            cmp = {}
+            # L2 scans on a 40,000-point sample: the reference's pass costs
+            # sample^2 * dims pair-dimensions (one region query per point)
+            ns = min(n, 40000)
+            fa2, fp2 = os.path.join(tmp, "s.fasta"), os.path.join(tmp, "s.txt")
+            synth.write_fasta(fa2, [b"ACGT"] * ns)
+            np.savetxt(fp2, pts[:ns], fmt="%.7g")
+            for tag, exe_dir in (("ours", os.path.join(PKG_DIR, "bin")), ("ref", ref_dir)):
+                d = os.path.join(tmp, "s_" + tag)
+                os.makedirs(d)
+                cmp[tag] = _run_cli([os.path.join(exe_dir, "cluster_dbscan_pca"), fa2, fp2, "5", str(eps_pca),
+                                     str(minpts), "o"], d)
+                cmp[tag + "_files"] = _outputs(d, "o")
+            same_pca = cmp["ours"][1] == cmp["ref"][1] and cmp["ours_files"] == cmp["ref_files"]
+            ref_rate = ns * ns * dims / cmp["ref"][0]
+            log(f"reference cluster_dbscan_pca {ns}: {cmp['ref'][0]:.2f} s; ours {cmp['ours'][0]:.2f} s; "
+                f"identical outputs: {same_pca}")
+            # fasta2kmer on a 1,000-sequence sample
+            fk2 = os.path.join(tmp, "ks.fasta")
+            synth.write_fasta(fk2, seqs[:1000])
+            rs, rtext = _run_cli([os.path.join(ref_dir, "fasta2kmer"), fk2, "5", "1", "0"], tmp)
+            os_, otext = _run_cli([os.path.join(PKG_DIR, "bin", "fasta2kmer"), fk2, "5", "1", "0"], tmp)
+            log(f"reference fasta2kmer 1000 seqs: {rs:.2f} s; ours {os_:.2f} s; identical: {rtext == otext}")
+            # L1 k-mer DBSCAN on a 4,000-sequence sample
+            ks = 4000
+            fk3, kf3 = os.path.join(tmp, "k3.fasta"), os.path.join(tmp, "k3.kmer")
+            synth.write_fasta(fk3, seqs[:ks])
+            with open(kf3, "wb") as f:
+                f.write(b"\n".join(ktext.split(b"\n", ks)[:ks]) + b"\n")
+            for tag, exe_dir in (("kours", os.path.join(PKG_DIR, "bin")), ("kref", ref_dir)):
+                d = os.path.join(tmp, "s_" + tag)
+                os.makedirs(d)
+                cmp[tag] = _run_cli([os.path.join(exe_dir, "cluster_dbscan_kmerL1"), fk3, kf3, str(eps_l1), "5",
+                                     "o"], d)
+                cmp[tag + "_files"] = _outputs(d, "o")
+            same_k = cmp["kours"][1] == cmp["kref"][1] and cmp["kours_files"] == cmp["kref_files"]
+            log(f"reference cluster_dbscan_kmerL1 {ks}: {cmp['kref'][0]:.2f} s; ours {cmp['kours'][0]:.2f} s; "
+                f"identical outputs: {same_k}")
+            out["cpu_baseline"] = {
+                "value": ref_rate, "unit": "pair-dims/s", "cores": 1, "kind": "reference",
+                "sample": f"the reference's cluster_dbscan_pca (oracle/_ref, one thread, AVX2+FMA build) on the "
+                          f"first {ns} points ({cmp['ref'][0]:.2f} s wall incl. file parsing; one region "
+                          f"query per point = {ns}^2 x {dims} pair-dims); host: {cpu_model()}"}
+            res["reference_samples"] = {
+                "cluster_dbscan_pca": {"n": ns, "ref_s": cmp["ref"][0], "ours_s": cmp["ours"][0],
+                                       "identical": same_pca},
+                "fasta2kmer": {"n": 1000, "ref_s": rs, "ours_s": os_, "identical": rtext == otext},
+                "cluster_dbscan_kmerL1": {"n": ks, "ref_s": cmp["kref"][0], "ours_s": cmp["kours"][0],
+                                          "identical": same_k},
+                "reference_estimate_full_s": {
+                    "cluster_dbscan_pca": n * n * dims / ref_rate,
+                    "fasta2kmer": rs * m / 1000,
+                    "cluster_dbscan_kmerL1": cmp["kref"][0] * (m / ks) ** 2}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,7 +710,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "consumers"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "consumers", "vec"],
                     help="c2: 20,000 uneven-length (headline); c3: 105,000 amplicon-scale; "
                          "c4: learned-proposer training + held-out recall; c5: end-to-end "
                          "learned-prefilter DBSCAN vs the exact path")
@@ -585,6 +758,9 @@ def main():
         return
     if args.config == "consumers":
         run_consumers_bench(args, log)
+        return
+    if args.config == "vec":
+        run_vec_bench(args, pkg, log)
         return
     print("generating workload ...", file=sys.stderr, flush=True)
     seqs = synth.config2() if args.config == "c2" else synth.config3()

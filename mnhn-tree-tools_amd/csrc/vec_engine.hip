@@ -16,8 +16,11 @@
 // walked by a grid-stride loop; a 256-thread workgroup stages up to 16
 // dimensions of both tiles in LDS per step and every thread keeps an 8 x 8 block of
 // accumulators in registers (two VALU ops per pair and dimension: the
-// subtraction and the fma / abs-add).  Pairs under the threshold are appended
-// as (i << 32 | j, acc) with one atomic per wave.
+// subtraction and the fma / abs-add); the next chunk -- across the tile
+// boundary too -- is loaded into registers while the current one computes.
+// Pairs under the threshold are collected per wave in an LDS buffer and
+// appended as (i << 32 | j, acc) with one atomic reservation per flush of up
+// to 512 pairs, so sparse lists do not serialise on the output counter.
 //
 // Lists.  Pairs are expanded to both directions as keys row * 2^b + col, radix
 // sorted (rocPRIM) with the accumulator as payload, and cut into CSR rows by a
@@ -57,6 +60,7 @@ namespace {
 constexpr int kTile = 128;
 constexpr int kThreads = 256;
 constexpr int kChunk = 16;
+constexpr int kWaveBuf = 512;  // listed pairs a wave collects in LDS before one reservation
 
 __device__ __forceinline__ long long tri_start(long long t) { return t * (t + 1) / 2; }
 
@@ -76,58 +80,89 @@ __device__ __forceinline__ float step(float acc, float a, float b) {
   return acc + fabsf(d);
 }
 
+// Staging of one chunk (up to kChunk dimensions of both tiles): thread tid
+// holds rows lk0 and lk0 + 8 of float4 column lc of each tile.
+struct Stage {
+  float4 a0, a1, b0, b1;
+};
+
+__device__ __forceinline__ Stage load_chunk(const float *__restrict__ x, long long n_pad, int dims_pad, int i0, int j0,
+                                            int k0, int lk0, int lc) {
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  Stage g{zero4, zero4, zero4, zero4};
+  const float4 *xa = reinterpret_cast<const float4 *>(x + i0);
+  const float4 *xb = reinterpret_cast<const float4 *>(x + j0);
+  const long long row4 = n_pad / 4;
+  if (k0 + lk0 < dims_pad) {
+    g.a0 = xa[(long long)(k0 + lk0) * row4 + lc];
+    g.b0 = xb[(long long)(k0 + lk0) * row4 + lc];
+  }
+  if (k0 + lk0 + 8 < dims_pad) {
+    g.a1 = xa[(long long)(k0 + lk0 + 8) * row4 + lc];
+    g.b1 = xb[(long long)(k0 + lk0 + 8) * row4 + lc];
+  }
+  return g;
+}
+
 template <int NORM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void vec_tiles_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void vec_tiles_kernel(
     const float *__restrict__ x, long long n_pad, int n, int dims_pad, long long n_tiles, float thr,
     unsigned long long *__restrict__ pairs, float *__restrict__ pval, unsigned long long cap,
     unsigned long long *__restrict__ count) {
   __shared__ float4 sa[kChunk][kTile / 4];
   __shared__ float4 sb[kChunk][kTile / 4];
+  __shared__ unsigned long long bkey[kThreads / 64][kWaveBuf];
+  __shared__ float bval[kThreads / 64][kWaveBuf];
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
-  const int lane = tid & 63;
-  // staging assignment: two float4 of each tile per thread and chunk
-  const int lk0 = tid >> 5, lc = tid & 31;  // rows lk0 and lk0 + 8, float4 column lc
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lk0 = tid >> 5, lc = tid & 31;
+  // pairs buffered by this wave (the same value in every lane)
+  int bcount = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, (unsigned long long)bcount);
+    base = __shfl(base, 0, 64);
+    for (int e = lane; e < bcount; e += 64) {
+      if (base + (unsigned long long)e < cap) {
+        pairs[base + e] = bkey[wave][e];
+        pval[base + e] = bval[wave][e];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    bcount = 0;
+  };
 
-  for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    int ti, tj;
-    tile_of(t, ti, tj);
-    const long long i0 = (long long)ti * kTile, j0 = (long long)tj * kTile;
+  long long t = blockIdx.x;
+  if (t >= n_tiles) return;
+  int ti, tj;
+  tile_of(t, ti, tj);
+  // the first chunk of the next tile is loaded while this tile computes
+  Stage g = load_chunk(x, n_pad, dims_pad, ti * kTile, tj * kTile, 0, lk0, lc);
+  for (;;) {
+    const int i0 = ti * kTile, j0 = tj * kTile;
+    const long long t_next = t + gridDim.x;
+    int ti_n = 0, tj_n = 0;
+    if (t_next < n_tiles) tile_of(t_next, ti_n, tj_n);
     float acc[8][8];
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
       for (int c = 0; c < 8; ++c) acc[r][c] = 0.f;
-
-    const float4 *xa = reinterpret_cast<const float4 *>(x + i0);
-    const float4 *xb = reinterpret_cast<const float4 *>(x + j0);
-    const long long row4 = n_pad / 4;
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 pa0 = zero4, pa1 = zero4, pb0 = zero4, pb1 = zero4;
-    if (lk0 < dims_pad) {
-      pa0 = xa[(long long)lk0 * row4 + lc];
-      pb0 = xb[(long long)lk0 * row4 + lc];
-    }
-    if (lk0 + 8 < dims_pad) {
-      pa1 = xa[(long long)(lk0 + 8) * row4 + lc];
-      pb1 = xb[(long long)(lk0 + 8) * row4 + lc];
-    }
     for (int k0 = 0; k0 < dims_pad; k0 += kChunk) {
       __syncthreads();
-      sa[lk0][lc] = pa0;
-      sa[lk0 + 8][lc] = pa1;
-      sb[lk0][lc] = pb0;
-      sb[lk0 + 8][lc] = pb1;
+      sa[lk0][lc] = g.a0;
+      sa[lk0 + 8][lc] = g.a1;
+      sb[lk0][lc] = g.b0;
+      sb[lk0 + 8][lc] = g.b1;
       __syncthreads();
-      const int kk = k0 + kChunk;
-      if (kk + lk0 < dims_pad) {
-        pa0 = xa[(long long)(kk + lk0) * row4 + lc];
-        pb0 = xb[(long long)(kk + lk0) * row4 + lc];
-      }
-      if (kk + lk0 + 8 < dims_pad) {
-        pa1 = xa[(long long)(kk + lk0 + 8) * row4 + lc];
-        pb1 = xb[(long long)(kk + lk0 + 8) * row4 + lc];
-      }
+      if (k0 + kChunk < dims_pad)
+        g = load_chunk(x, n_pad, dims_pad, i0, j0, k0 + kChunk, lk0, lc);
+      else if (t_next < n_tiles)
+        g = load_chunk(x, n_pad, dims_pad, ti_n * kTile, tj_n * kTile, 0, lk0, lc);
       const int kc = min(kChunk, dims_pad - k0);  // even
 #pragma unroll 2
       for (int k = 0; k < kc; ++k) {
@@ -142,48 +177,62 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       }
     }
 
-    // emission: pair (i, j) with i <= j < n and acc <= thr
+    // emission: pair (i, j) with i <= j < n and acc <= thr, into the wave's
+    // buffer one tile row at a time (at most 64 lanes x 8 = kWaveBuf pairs)
+    // the smallest accumulator first (NaN never passes a threshold and
+    // fminf skips it): most waves of a sparse list stop here
+    float mn = acc[0][0];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) mn = fminf(mn, acc[r][c]);
     unsigned long long hitmask = 0ull;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const long long i = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const long long j = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
-        const bool hit = i <= j && j < n && acc[r][c] <= thr;
-        hitmask |= (unsigned long long)hit << (r * 8 + c);
-      }
-    }
-    const int cnt = __popcll(hitmask);
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += y;
-    }
-    const int total = __shfl(incl, 63, 64);
-    if (total > 0) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(count, (unsigned long long)total);
-      base = __shfl(base, 0, 64);
-      unsigned long long pos = base + (unsigned long long)(incl - cnt);
+    if (__ballot(mn <= thr)) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        const long long i = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
+        const int i = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-          if ((hitmask >> (r * 8 + c)) & 1ull) {
-            const long long j = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
-            if (pos < cap) {
-              pairs[pos] = ((unsigned long long)i << 32) | (unsigned long long)j;
-              pval[pos] = acc[r][c];
-            }
-            ++pos;
-          }
+          const int j = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
+          const bool hit = i <= j && j < n && acc[r][c] <= thr;
+          hitmask |= (unsigned long long)hit << (r * 8 + c);
         }
       }
     }
+    if (__ballot(hitmask != 0ull)) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const unsigned rm = (unsigned)(hitmask >> (8 * r)) & 0xffu;
+        if (!__ballot(rm != 0u)) continue;
+        const int rc = __popc(rm);
+        int incl = rc;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += y;
+        }
+        const int rtotal = __shfl(incl, 63, 64);
+        if (bcount + rtotal > kWaveBuf) flush();
+        int p = bcount + incl - rc;
+        const int i = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          if ((rm >> c) & 1u) {
+            const int j = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
+            bkey[wave][p] = ((unsigned long long)(unsigned)i << 32) | (unsigned long long)(unsigned)j;
+            bval[wave][p] = acc[r][c];
+            ++p;
+          }
+        }
+        bcount += rtotal;
+      }
+    }
+    if (t_next >= n_tiles) break;
+    t = t_next;
+    ti = ti_n;
+    tj = tj_n;
   }
+  if (bcount > 0) flush();
 }
 
 // Both directions of every pair as row * 2^b + col; the second copy of a

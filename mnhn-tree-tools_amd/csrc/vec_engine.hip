@@ -273,6 +273,15 @@ __global__ void row_ptr_kernel(const unsigned long long *__restrict__ keys, long
   }
 }
 
+// Sets *bad when the sorted keys are not strictly increasing (a checked
+// postcondition of the radix sort: keys are distinct).
+__global__ void check_sorted_kernel(const unsigned long long *__restrict__ keys, long long nnz,
+                                    unsigned long long *__restrict__ bad) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k + 1 < nnz;
+       k += (long long)gridDim.x * blockDim.x)
+    if (keys[k] >= keys[k + 1]) atomicOr(bad, 1ull);
+}
+
 __global__ void col_kernel(const unsigned long long *__restrict__ keys, long long nnz, int b,
                            int32_t *__restrict__ col) {
   const unsigned long long mask = (1ull << b) - 1ull;
@@ -454,6 +463,18 @@ int mnhn_vec_eps_csr(mnhn_vec_ctx *c, int norm, float thr, mnhn_vcsr *out) {
     e = hipMemcpyAsync(&diag, c->d_count + 1, sizeof(diag), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   const long long nnz = m2 - (long long)diag;
+  // d_count[0] is free again: reuse it as the sortedness flag
+  if (e == hipSuccess) e = hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream);
+  if (e == hipSuccess && nnz > 1) {
+    hipLaunchKernelGGL(check_sorted_kernel, dim3(stride_blocks(nnz, tb)), dim3(tb), 0, c->stream, kbuf.current(),
+                       nnz, c->d_count);
+    e = hipGetLastError();
+  }
+  unsigned long long unsorted = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&unsorted, c->d_count, sizeof(unsorted), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && unsorted) e = hipErrorUnknown;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(row_ptr_kernel, dim3(stride_blocks((long long)n + 1, tb)), dim3(tb), 0, c->stream,
                        kbuf.current(), nnz, n, b, d_rp);

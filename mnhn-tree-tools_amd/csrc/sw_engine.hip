@@ -34,6 +34,7 @@
 //    column order by ranking its columns (comparison ranks for short rows,
 //    a column bitmap + popcount prefix in LDS for long ones).
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <sys/mman.h>
 
 #include <algorithm>
@@ -1603,6 +1604,130 @@ int mnhn_detail::ensure_keys(mnhn_sw_ctx *c, unsigned long long cap) {
 }
 
 // out->row_ptr must be allocated ([n+1], zeroed) and out->n_rows set.
+namespace {
+
+// Radix-sort form of the key -> CSR step, for any n (the row-ranking form
+// needs an LDS bitmap of n bits): keys row<<40 | col<<16 | d become
+// row<<24 | col with d as payload, are sorted over their 24 + b significant
+// bits (n < 2^b), checked on the device, and cut into rows by binary search.
+__global__ void split_keys_kernel(const unsigned long long *__restrict__ keys, long long nk,
+                                  unsigned long long *__restrict__ k_out, int16_t *__restrict__ d_out) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nk;
+       k += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = keys[k];
+    k_out[k] = key >> 16;
+    d_out[k] = (int16_t)(key & 0xffffull);
+  }
+}
+
+__global__ void sorted_check_kernel(const unsigned long long *__restrict__ keys, long long nk,
+                                    unsigned long long *__restrict__ bad) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k + 1 < nk;
+       k += (long long)gridDim.x * blockDim.x)
+    if (keys[k] >= keys[k + 1]) atomicOr(bad, 1ull);
+}
+
+__global__ void cut_rows_kernel(const unsigned long long *__restrict__ keys, long long nk, int n,
+                                long long *__restrict__ row_ptr) {
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r <= n;
+       r += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long want = (unsigned long long)r << 24;
+    long long lo = 0, hi = nk;
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if (keys[mid] < want) lo = mid + 1;
+      else hi = mid;
+    }
+    row_ptr[r] = lo;
+  }
+}
+
+__global__ void low_col_kernel(const unsigned long long *__restrict__ keys, long long nk,
+                               int32_t *__restrict__ col) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nk;
+       k += (long long)gridDim.x * blockDim.x)
+    col[k] = (int32_t)(keys[k] & 0xffffffull);
+}
+
+int csr_from_keys_radix(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
+  const int n = c->n;
+  const double kernel_ms = c->last.kernel_ms;
+  int b = 1;
+  while ((1ll << b) <= (long long)n) ++b;
+  unsigned long long *d_k0 = nullptr, *d_k1 = nullptr, *d_flag = nullptr;
+  int16_t *d_d0 = nullptr, *d_d1 = nullptr;
+  long long *d_rp = nullptr;
+  void *d_tmp = nullptr;
+  auto cleanup = [&]() {
+    dev_free(d_k0); dev_free(d_k1); dev_free(d_d0); dev_free(d_d1); dev_free(d_rp); dev_free(d_flag);
+    if (d_tmp) (void)hipFree(d_tmp);
+    d_tmp = nullptr;
+  };
+  int rc;
+  if ((rc = dev_alloc(&d_k0, nk)) || (rc = dev_alloc(&d_k1, nk)) || (rc = dev_alloc(&d_d0, nk)) ||
+      (rc = dev_alloc(&d_d1, nk)) || (rc = dev_alloc(&d_rp, (size_t)n + 1)) || (rc = dev_alloc(&d_flag, 1))) {
+    cleanup(); mnhn_csr_free(out); return rc;
+  }
+  const int tb = 256;
+  (void)hipEventRecord(c->ev[2], c->stream);
+  hipError_t e = hipSuccess;
+  if (nk > 0) {
+    hipLaunchKernelGGL(split_keys_kernel, dim3(stride_blocks((long long)nk, tb)), dim3(tb), 0, c->stream, c->d_keys,
+                       (long long)nk, d_k0, d_d0);
+    e = hipGetLastError();
+  }
+  rocprim::double_buffer<unsigned long long> kbuf(d_k0, d_k1);
+  rocprim::double_buffer<int16_t> dbuf(d_d0, d_d1);
+  size_t tmp_bytes = 0;
+  if (e == hipSuccess && nk > 0)
+    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kbuf, dbuf, (size_t)nk, 0u, (unsigned)(24 + b), c->stream);
+  if (e == hipSuccess && nk > 0) e = hipMalloc(&d_tmp, std::max<size_t>(tmp_bytes, 1));
+  if (e != hipSuccess) d_tmp = nullptr;
+  if (e == hipSuccess && nk > 0)
+    e = rocprim::radix_sort_pairs(d_tmp, tmp_bytes, kbuf, dbuf, (size_t)nk, 0u, (unsigned)(24 + b), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_flag, 0, sizeof(unsigned long long), c->stream);
+  if (e == hipSuccess && nk > 1) {
+    hipLaunchKernelGGL(sorted_check_kernel, dim3(stride_blocks((long long)nk, tb)), dim3(tb), 0, c->stream,
+                       kbuf.current(), (long long)nk, d_flag);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(cut_rows_kernel, dim3(stride_blocks((long long)n + 1, tb)), dim3(tb), 0, c->stream,
+                       kbuf.current(), (long long)nk, n, d_rp);
+    e = hipGetLastError();
+  }
+  int32_t *d_col = reinterpret_cast<int32_t *>(kbuf.alternate());
+  if (e == hipSuccess && nk > 0) {
+    hipLaunchKernelGGL(low_col_kernel, dim3(stride_blocks((long long)nk, tb)), dim3(tb), 0, c->stream,
+                       kbuf.current(), (long long)nk, d_col);
+    e = hipGetLastError();
+  }
+  unsigned long long unsorted = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&unsorted, d_flag, sizeof(unsorted), hipMemcpyDeviceToHost, c->stream);
+  (void)hipEventRecord(c->ev[3], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[3]);
+  if (e == hipSuccess && unsorted) e = hipErrorUnknown;
+  float post_ms = 0.f;
+  if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
+  if (e == hipSuccess) {
+    out->col = (int32_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
+    out->dist = (int16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
+    if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
+    rc = copy_d2h(c, out->row_ptr, d_rp, ((size_t)n + 1) * sizeof(long long));
+    if (rc == MNHN_OK && nk) rc = copy_d2h(c, out->col, d_col, nk * sizeof(int32_t));
+    if (rc == MNHN_OK && nk) rc = copy_d2h(c, out->dist, dbuf.current(), nk * sizeof(int16_t));
+    if (rc != MNHN_OK) e = hipErrorUnknown;
+  }
+  cleanup();
+  if (e != hipSuccess) { mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
+  out->nnz = (int64_t)nk;
+  c->last.kernel_ms = kernel_ms;
+  c->last.total_ms = kernel_ms + post_ms;
+  return MNHN_OK;
+}
+
+}  // namespace
+
 int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
   const int n = c->n;
   int rc = MNHN_OK;
@@ -1610,7 +1735,8 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
   // LDS of the long-row path: column bitmap + per-word prefix + thread totals
   const size_t lds_bytes = std::max<size_t>(
       (size_t)(2 * ((n + 31) / 32) + kSortThreads) * sizeof(uint32_t), kShortRow * sizeof(uint32_t));
-  if (lds_bytes > 160 * 1024) { mnhn_csr_free(out); return MNHN_ERR_RANGE; }
+  const char *form = getenv("MNHN_CSR_SORT");
+  if (lds_bytes > 160 * 1024 || (form && strcmp(form, "radix") == 0)) return csr_from_keys_radix(c, nk, out);
   int32_t *d_cnt = nullptr, *d_seg_col = nullptr, *d_col = nullptr;
   int16_t *d_seg_d = nullptr, *d_dist = nullptr;
   long long *d_rp = nullptr;

@@ -20,6 +20,9 @@ Run in a container where the reference sources exist (default
                              find_satellite, compareSW, silhouette,
                              compare_norms (cmd.json:
                              argv with IN:<input> for copied inputs)
+  newick/<case>/          -- the reference's split_sets_to_newick on the split
+                             sets and stdout of an adaptive case above (the
+                             downstream tree tool that reads those files)
   vectors/<case>/         -- the same for the L1 / L2 vector tools
                              (cluster_dbscan_pca, cluster_dbscan_kmerL1/L2,
                              adaptive_clustering_PCA / _kmer_L1 / _kmer_L2)
@@ -397,6 +400,38 @@ def run_consumer_case(case, inputs, base: str = "consumers") -> str:
     return name
 
 
+# (case, source fixture dir, newick argv before the split-set files; STDOUT =
+# the source's stdout.txt)
+NEWICK_CASES = [
+    ("newick_adaptive_c2small", "cli/adaptive_c2small", ["0", "2", "1", "STDOUT", "4"]),
+    ("newick_adaptive_c1small_min5", "cli/adaptive_c1small", ["5", "1", "2", "STDOUT", "5"]),
+    ("newick_adaptive_pca_c1", "vectors/adaptive_pca_c1", ["0", "2", "5", "STDOUT", "5"]),
+    ("newick_adaptive_kmerL2_c2", "vectors/adaptive_kmerL2_c2", ["0", "0"]),
+]
+
+
+def run_newick_case(case) -> str:
+    name, src, args = case
+    outdir = os.path.join(HERE, "newick", name)
+    if os.path.exists(os.path.join(outdir, "cmd.json")):
+        return name + " (kept)"
+    srcdir = os.path.join(HERE, src)
+    meta = json.load(open(os.path.join(srcdir, "cmd.json")))
+    sets = sorted(f for f in meta["files"] if f[len(f.rstrip("0123456789")):].isdigit() and len(f) > 4)
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in sets + ["stdout.txt"]:
+            shutil.copyfile(os.path.join(srcdir, f), os.path.join(tmp, f))
+        argv = [os.path.join(ROOT, "oracle", "_ref", "split_sets_to_newick")]
+        argv += ["stdout.txt" if a == "STDOUT" else a for a in args] + sets
+        r = subprocess.run(argv, cwd=tmp, capture_output=True)
+    os.makedirs(outdir, exist_ok=True)
+    with open(os.path.join(outdir, "stdout.txt"), "wb") as g:
+        g.write(r.stdout)
+    with open(os.path.join(outdir, "cmd.json"), "w") as g:
+        json.dump({"source": src, "args": args, "sets": sets, "returncode": r.returncode}, g, indent=1)
+    return name
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -419,6 +454,8 @@ def main() -> None:
             print("done", name, flush=True)
         for name in ex.map(lambda c: run_consumer_case(c, inputs, "vectors"), VECTOR_CASES):
             print("done", name, flush=True)
+    for c in NEWICK_CASES:
+        print("done", run_newick_case(c), flush=True)
 
 
 if __name__ == "__main__":

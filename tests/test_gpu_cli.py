@@ -101,3 +101,38 @@ def test_cli_persisted_eps_lists(name, tmp_path):
     r = subprocess.run([exe, str(other), "400", "5", "o"], cwd=tmp_path, capture_output=True,
                        env=dict(os.environ, MNHN_ECSR=path), timeout=300)
     assert r.returncode == 0 and b"written to" in r.stderr
+
+
+# ---- downstream: the reference's split_sets_to_newick on our outputs --------------
+
+NEWICK_DIR = os.path.join(GOLDEN, "newick")
+NEWICK_CASES = sorted(os.listdir(NEWICK_DIR)) if os.path.isdir(NEWICK_DIR) else []
+NEWICK_TOOL = os.path.join(os.path.dirname(PKG_DIR), "oracle", "_ref", "split_sets_to_newick")
+
+
+@pytest.mark.parametrize("name", NEWICK_CASES)
+def test_newick_tool_reads_our_outputs(name, tmp_path):
+    """The split sets and stdout of the drop-in adaptive CLIs feed the
+    reference's split_sets_to_newick (compiled from its source) to the same
+    tree as the reference's own outputs do."""
+    if not os.path.exists(NEWICK_TOOL):
+        pytest.skip("reference split_sets_to_newick not built (make -C oracle ref)")
+    import json
+    meta = json.load(open(os.path.join(NEWICK_DIR, name, "cmd.json")))
+    src = json.load(open(os.path.join(GOLDEN, meta["source"], "cmd.json")))
+    exe = os.path.join(PKG_DIR, "bin", src["program"])
+    if "argv" in src:  # vector fixture: argv with IN:/EPS: inputs
+        from conftest import golden_arg
+        args = [golden_arg(a) for a in src["argv"]]
+    else:
+        fasta = tmp_path / "in.fasta"
+        shutil.copyfile(os.path.join(GOLDEN, "inputs", src["input"]), fasta)
+        args = [str(fasta)] + [a if a != "PREFIX" else src["prefix"] for a in src["args"]]
+    print(f"newick case {name}: {src['program']}", flush=True)
+    r = subprocess.run([exe] + args, cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()[-2000:]
+    (tmp_path / "stdout.txt").write_bytes(r.stdout)
+    argv = [NEWICK_TOOL] + ["stdout.txt" if a == "STDOUT" else a for a in meta["args"]] + meta["sets"]
+    t = subprocess.run(argv, cwd=tmp_path, capture_output=True, timeout=120)
+    assert t.returncode == meta["returncode"]
+    assert t.stdout == open(os.path.join(NEWICK_DIR, name, "stdout.txt"), "rb").read()

@@ -267,3 +267,63 @@ def test_block_vs_oracle(pkg, oracle, tuning):
     assert np.array_equal(got, want[np.ix_(rows, cols)])
     assert np.array_equal(one[0], want[rows[0]])
     assert empty.shape == (0, len(cols))
+
+
+@pytest.mark.parametrize("eps", [60, 400])
+def test_eps_csr_radix_form_equals_rank_form(pkg, oracle, monkeypatch, eps):
+    """The radix-sort key -> CSR form (used when n is too large for the
+    row-ranking form's LDS bitmap; forced here with MNHN_CSR_SORT=radix)
+    gives the same lists, and both equal the oracle's."""
+    rng = np.random.default_rng(7 + eps)
+    seqs = _rand_set(rng, 200, 0, 400)
+    for b in _rand_set(rng, 8, 60, 300):
+        for _ in range(25):
+            seqs.append(synth.to_bytes(synth.mutate(rng, np.frombuffer(b, np.uint8).copy() % 4, 0.04, 0.01)))
+    n = len(seqs)
+    _say(f"eps csr radix form n={n} eps={eps}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    expect = np.where(want <= eps, want, -1)
+    np.fill_diagonal(expect, -1)
+    with pkg.Engine(seqs) as eng:
+        monkeypatch.delenv("MNHN_CSR_SORT", raising=False)
+        a = eng.eps_csr(eps)
+        monkeypatch.setenv("MNHN_CSR_SORT", "radix")
+        b = eng.eps_csr(eps)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(_csr_to_dense(*b, n), expect)
+
+
+def test_eps_csr_beyond_row_bitmap_limit(pkg, oracle):
+    """700,000 short sequences (more than the 651,000 columns the row-ranking
+    form can hold in LDS): the radix form is taken; rows are sorted and 40
+    sampled rows equal the oracle's."""
+    rng = np.random.default_rng(70)
+    n_anc, per = 20000, 35
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    seqs = []
+    for _ in range(n_anc):
+        a = rng.integers(0, 4, int(rng.integers(10, 21)))
+        copies = np.tile(a, (per, 1))
+        hit = rng.random(copies.shape) < 0.05
+        copies[hit] = rng.integers(0, 4, int(hit.sum()))
+        seqs.extend(letters[row].tobytes() for row in copies)
+    n = len(seqs)
+    eps = 12
+    _say(f"eps csr beyond bitmap limit n={n} eps={eps}")
+    with pkg.Engine(seqs) as eng:
+        rp, col, dist = eng.eps_csr(eps)
+    assert len(rp) == n + 1 and rp[-1] == len(col)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    key = rows * n + col
+    assert np.all(np.diff(key) > 0)
+    assert np.all(dist <= eps)
+    sample = rng.choice(n, 40, replace=False)
+    all_j = np.arange(n, dtype=np.int32)
+    ij = np.concatenate([np.stack([np.full(n - 1, p, np.int32), all_j[all_j != p]], 1) for p in sample])
+    d = oracle.sw_pairs(seqs, ij, nthreads=16).reshape(len(sample), n - 1)
+    for k, p in enumerate(sample):
+        others = all_j[all_j != p]
+        want = others[d[k] <= eps]
+        assert np.array_equal(col[rp[p]:rp[p + 1]], want), p
+        assert np.array_equal(dist[rp[p]:rp[p + 1]], d[k][d[k] <= eps].astype(np.int16)), p

@@ -71,6 +71,8 @@ WORKLOADS = {
 # MI355X: 256 CUs x 64 lanes x 2 (packed int16) x 2.4 GHz
 PEAK_INT16_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 OPS_PER_CELL = 6
+# the exact fp16 form: 3.5 packed instructions per 2 cells at 64 lanes x 2 halves per CU-clock
+FP16_FORM_ROOFLINE_GCUPS = 256 * 64 * 2 * 2.4e9 / 1.75 / 1e9
 
 
 def load_pkg():
@@ -827,6 +829,15 @@ def main():
                                else {1: "sw_tiles_kernel", 2: "sw_lds_kernel"}.get(t["kernel"]),
                      "kernel_ms": kernel_s * 1e3,
                      "kernel_gcups": comp_cells / kernel_s / 1e9,
+                     # frac > 1 is expected for the fp16 form: it retires the 6 algorithmic ops
+                     # of a cell in 1.75 packed instructions (max3 and the clamp fold two each);
+                     # against its own instruction roofline (3.5 packed ops per 2 cells,
+                     # 18,725 GCUPS) the fraction is:
+                     "instruction_roofline_gcups": FP16_FORM_ROOFLINE_GCUPS if t["kernel"] == 3
+                     else PEAK_INT16_TOPS * 1e12 / OPS_PER_CELL / 1e9,
+                     "instruction_frac": comp_cells / kernel_s / 1e9 / (
+                         FP16_FORM_ROOFLINE_GCUPS if t["kernel"] == 3
+                         else PEAK_INT16_TOPS * 1e12 / OPS_PER_CELL / 1e9),
                      "int16_only_gcups": comp_cells / int16_s / 1e9,
                      "int16_only_kernel_ms": int16_kernel_ms},
     }

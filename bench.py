@@ -71,8 +71,9 @@ WORKLOADS = {
 # MI355X: 256 CUs x 64 lanes x 2 (packed int16) x 2.4 GHz
 PEAK_INT16_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 OPS_PER_CELL = 6
-# the exact fp16 form: 3.5 packed instructions per 2 cells at 64 lanes x 2 halves per CU-clock
-FP16_FORM_ROOFLINE_GCUPS = 256 * 64 * 2 * 2.4e9 / 1.75 / 1e9
+# the exact fp16 form: 3.5 packed DP instructions per 2 cells (one cell per 16-bit half), at 64
+# lanes per CU-clock (the 6-op count above is 6 packed instructions per 2 cells: 13,107 GCUPS)
+FP16_FORM_ROOFLINE_GCUPS = 256 * 64 * 2.4e9 * 2 / 3.5 / 1e9
 
 
 def load_pkg():
@@ -830,9 +831,8 @@ def main():
                      "kernel_ms": kernel_s * 1e3,
                      "kernel_gcups": comp_cells / kernel_s / 1e9,
                      # frac > 1 is expected for the fp16 form: it retires the 6 algorithmic ops
-                     # of a cell in 1.75 packed instructions (max3 and the clamp fold two each);
-                     # against its own instruction roofline (3.5 packed ops per 2 cells,
-                     # 18,725 GCUPS) the fraction is:
+                     # of a cell in 3.5 packed instructions per 2 cells (max3 and the clamp fold
+                     # two each); against that instruction roofline (22,469 GCUPS) the fraction is:
                      "instruction_roofline_gcups": FP16_FORM_ROOFLINE_GCUPS if t["kernel"] == 3
                      else PEAK_INT16_TOPS * 1e12 / OPS_PER_CELL / 1e9,
                      "instruction_frac": comp_cells / kernel_s / 1e9 / (

@@ -32,10 +32,12 @@
  * launches its kernel once per query.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include <unistd.h>
 
 #include "mnhn_host.h"
@@ -49,6 +51,22 @@ struct mnhn_eps_cache {
   mnhn_vcsr vcsr;    /* vector datasets: lists up to vcsr.thr */
   int have_v;
 };
+
+/* MNHN_TIMING=1: phase times on stderr (stdout is unchanged) */
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static int timing_on(void) {
+  static int on = -1;
+  if (on < 0) {
+    const char *v = getenv("MNHN_TIMING");
+    on = v && *v && strcmp(v, "0") != 0;
+  }
+  return on;
+}
 
 static void die_engine(const char *what, int rc) {
   printf("%s failed: %s\n", what, mnhn_sw_strerror(rc));
@@ -126,9 +144,13 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
       mnhn_csr_free(&c->csr);
     }
   }
+  const double t0 = now_s();
   int rc = mnhn_sw_eps_csr(g.sw, eps_max, &c->csr);
   if (rc) die_engine("mnhn_sw_eps_csr", rc);
   c->have = 1;
+  if (timing_on())
+    fprintf(stderr, "[mnhn] epsilon lists up to %d: %lld entries, %.3f s\n", (int)eps_max, (long long)c->csr.nnz,
+            now_s() - t0);
   if (path && *path) {
     if (mnhn_csr_save(&c->csr, hash, path) == MNHN_OK)
       fprintf(stderr, "epsilon lists (eps_max %d) written to %s\n", (int)eps_max, path);
@@ -169,15 +191,144 @@ static int region_query(mnhn_gpu_stuff g, int n, int p, float eps, int *out) {
   return region_query_csr(&c->csr, p, eps, out);
 }
 
-typedef int (*region_fn)(void *src, int n, int p, float eps, int *out);
+/* A region query fills out[] (or points *res at a stored row) with N(p),
+ * ascending, and returns |N(p)|. */
+typedef int (*region_fn)(void *src, int n, int p, float eps, int *out, const int **res);
 
-static int rq_gpu(void *src, int n, int p, float eps, int *out) {
+static int rq_gpu(void *src, int n, int p, float eps, int *out, const int **res) {
+  *res = out;
   return region_query(*(mnhn_gpu_stuff *)src, n, p, eps, out);
 }
 
-static int rq_csr(void *src, int n, int p, float eps, int *out) {
+/* ---- the lists of one pass, materialised in parallel ----
+ * N(p) of every point for one epsilon: the stored lists filtered by the
+ * pass's threshold ((float)dist <= eps for Smith-Waterman lists, acc <= thr
+ * for vector lists), with p itself inserted in order where the stored lists
+ * leave it out.  Rows are independent, so host threads build them (count,
+ * scan, fill) and the sequential expansion then reads each row in place. */
+typedef struct {
+  int64_t *rp;
+  int32_t *col;
+} pass_lists;
+
+typedef struct {
+  const int64_t *rp;
+  const int32_t *col;
+  const int16_t *dist; /* Smith-Waterman lists, or */
+  const float *val;    /* vector lists */
+  float thr;
+  int self; /* insert p itself */
+  int64_t *cnt;
+  pass_lists *out;
+  int r0, r1, phase;
+} pass_job;
+
+static inline int keep_entry(const pass_job *jb, int64_t t) {
+  return jb->dist ? (float)jb->dist[t] <= jb->thr : jb->val[t] <= jb->thr;
+}
+
+static void *pass_worker(void *arg) {
+  pass_job *jb = (pass_job *)arg;
+  for (int p = jb->r0; p < jb->r1; p++) {
+    const int64_t b = jb->rp[p], e = jb->rp[p + 1];
+    if (jb->phase == 0) {
+      int64_t k = jb->self;
+      for (int64_t t = b; t < e; t++) k += keep_entry(jb, t);
+      jb->cnt[p] = k;
+    } else {
+      int32_t *o = jb->out->col + jb->out->rp[p];
+      int placed = !jb->self;
+      for (int64_t t = b; t < e; t++) {
+        const int j = jb->col[t];
+        if (!placed && j > p) {
+          *o++ = p;
+          placed = 1;
+        }
+        if (keep_entry(jb, t)) *o++ = j;
+      }
+      if (!placed) *o++ = p;
+    }
+  }
+  return NULL;
+}
+
+static int host_threads(int64_t work) {
+  if (work < (1 << 20)) return 1;
+  const char *v = getenv("MNHN_HOST_THREADS");
+  long t = v && *v ? strtol(v, NULL, 10) : sysconf(_SC_NPROCESSORS_ONLN);
+  if (t < 1) t = 1;
+  if (t > 16) t = 16;
+  return (int)t;
+}
+
+static pass_lists build_pass_lists(int n, const int64_t *rp, const int32_t *col, const int16_t *dist,
+                                   const float *val, float thr, int self) {
+  pass_lists pl;
+  pl.rp = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+  int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  const int T = host_threads(rp[n]);
+  pthread_t th[16];
+  pass_job jobs[16];
+  /* rows split into T ranges of about equal stored entries */
+  int bounds[17];
+  bounds[0] = 0;
+  for (int k = 1; k < T; k++) {
+    const int64_t want = rp[n] / T * k;
+    int lo = bounds[k - 1], hi = n;
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if (rp[mid] < want) lo = mid + 1;
+      else hi = mid;
+    }
+    bounds[k] = lo;
+  }
+  bounds[T] = n;
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1) {
+      pl.rp[0] = 0;
+      for (int p = 0; p < n; p++) pl.rp[p + 1] = pl.rp[p] + cnt[p];
+      pl.col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(pl.rp[n] ? pl.rp[n] : 1));
+      if (!pl.col) {
+        printf("out of memory building the neighbour lists of a pass\n");
+        fflush(stdout);
+        _exit(1);
+      }
+    }
+    for (int k = 0; k < T; k++) {
+      jobs[k] = (pass_job){rp, col, dist, val, thr, self, cnt, &pl, bounds[k], bounds[k + 1], phase};
+      if (T > 1) pthread_create(&th[k], NULL, pass_worker, &jobs[k]);
+      else pass_worker(&jobs[k]);
+    }
+    if (T > 1)
+      for (int k = 0; k < T; k++) pthread_join(th[k], NULL);
+  }
+  free(cnt);
+  return pl;
+}
+
+static void free_pass_lists(pass_lists *pl) {
+  free(pl->rp);
+  free(pl->col);
+}
+
+static int rq_pass(void *src, int n, int p, float eps, int *out, const int **res) {
   (void)n;
-  return region_query_csr((const mnhn_csr *)src, p, eps, out);
+  (void)eps;
+  (void)out;
+  const pass_lists *pl = (const pass_lists *)src;
+  *res = pl->col + pl->rp[p];
+  return (int)(pl->rp[p + 1] - pl->rp[p]);
+}
+
+/* x in the ascending row a[0..m)? */
+static inline int in_row(const int *a, int m, int x) {
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < m && a[lo] == x;
 }
 
 static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, void *src) {
@@ -191,11 +342,13 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   int *stamp = (int *)calloc((size_t)n, sizeof(int)); /* union membership, per cluster */
   int *nb = (int *)malloc(sizeof(int) * (size_t)n);
   int *queue = (int *)malloc(sizeof(int) * (size_t)n);
+  int *rest = (int *)malloc(sizeof(int) * (size_t)n); /* points outside the union, ascending */
 
   for (int i = 0; i < n; i++) {
     if (visited[i]) continue;
     visited[i] = 1;
-    int cnt = rq(src, n, i, epsilon, nb);
+    const int *nbi;
+    int cnt = rq(src, n, i, epsilon, nb, &nbi);
     if (!(minpts < cnt)) continue;
     /* expand_cluster */
     const int id = ret.n_clusters;
@@ -208,20 +361,50 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
     member[i] = 1;
     int qn = 0;
     for (int t = 0; t < cnt; t++) {
-      queue[qn++] = nb[t];
-      stamp[nb[t]] = mark;
+      queue[qn++] = nbi[t];
+      stamp[nbi[t]] = mark;
     }
+    int in_union = cnt; /* the union holds queue[0 .. qn) */
+    int n_rest = -1;    /* rest[] not built yet for this cluster */
     for (int qi = 0; qi < qn; qi++) {
       const int q = queue[qi];
       if (!visited[q]) {
         visited[q] = 1;
-        int cq = rq(src, n, q, epsilon, nb);
+        const int *nbq;
+        int cq = rq(src, n, q, epsilon, nb, &nbq);
         if (cq >= minpts) {
-          for (int t = 0; t < cq; t++) {
-            const int x = nb[t];
-            if (stamp[x] != mark) {
-              stamp[x] = mark;
-              queue[qn++] = x;
+          /* the members of N(q) outside the union, in ascending order: by
+           * scanning N(q), or -- when N(q) is much longer than the set of
+           * points still outside -- by scanning that set (kept ascending)
+           * and looking each point up in N(q); both append the same points
+           * in the same order */
+          if (cq > 8 * (n - in_union)) {
+            if (n_rest < 0) {
+              n_rest = 0;
+              for (int x = 0; x < n; x++)
+                if (stamp[x] != mark) rest[n_rest++] = x;
+            } else {
+              int k = 0;
+              for (int t = 0; t < n_rest; t++)
+                if (stamp[rest[t]] != mark) rest[k++] = rest[t];
+              n_rest = k;
+            }
+            for (int t = 0; t < n_rest; t++) {
+              const int x = rest[t];
+              if (in_row(nbq, cq, x)) {
+                stamp[x] = mark;
+                queue[qn++] = x;
+                in_union++;
+              }
+            }
+          } else {
+            for (int t = 0; t < cq; t++) {
+              const int x = nbq[t];
+              if (stamp[x] != mark) {
+                stamp[x] = mark;
+                queue[qn++] = x;
+                in_union++;
+              }
             }
           }
         }
@@ -239,6 +422,7 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   free(stamp);
   free(nb);
   free(queue);
+  free(rest);
   return ret;
 }
 
@@ -266,11 +450,28 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
       ensure_lists(g, epsilon);
     }
   }
-  return dbscan_core(n, epsilon, minpts, rq_gpu, &g);
+  const double t0 = now_s();
+  split_set s;
+  if (n > 0 && !g.cache->per_query_launch) {
+    const mnhn_csr *c = &g.cache->csr;
+    pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, c->dist, NULL, epsilon, (float)0 <= epsilon);
+    s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+    free_pass_lists(&pl);
+  } else {
+    s = dbscan_core(n, epsilon, minpts, rq_gpu, &g);
+  }
+  if (timing_on())
+    fprintf(stderr, "[mnhn] dbscan pass eps %g: %d clusters, %.3f s\n", epsilon, s.n_clusters, now_s() - t0);
+  return s;
 }
 
 split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts) {
-  return dbscan_core(csr ? csr->n_rows : 0, epsilon, minpts, rq_csr, (void *)csr);
+  const int n = csr ? csr->n_rows : 0;
+  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL);
+  pass_lists pl = build_pass_lists(n, csr->row_ptr, csr->col, csr->dist, NULL, epsilon, (float)0 <= epsilon);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+  free_pass_lists(&pl);
+  return s;
 }
 
 /* ---- vector datasets (dbscan.c:563-780) ---- */
@@ -294,27 +495,6 @@ static void ensure_vec_lists(mnhn_gpu_stuff g, float eps) {
   c->have_v = 1;
 }
 
-/* N(p): row p's entries (ascending, p itself listed when its own accumulator
- * qualifies) with accumulator <= thr. */
-static int region_query_vcsr(const mnhn_vcsr *csr, int p, float thr, int *out) {
-  int k = 0;
-  for (int64_t t = csr->row_ptr[p], e = csr->row_ptr[p + 1]; t < e; t++)
-    if (csr->val[t] <= thr) out[k++] = csr->col[t];
-  return k;
-}
-
-static int rq_vec(void *src, int n, int p, float eps, int *out) {
-  (void)n;
-  const mnhn_gpu_stuff *g = (const mnhn_gpu_stuff *)src;
-  return region_query_vcsr(&g->cache->vcsr, p, vec_thr(g->norm, eps), out);
-}
-
-static int rq_vcsr(void *src, int n, int p, float eps, int *out) {
-  (void)n;
-  const mnhn_vcsr *csr = (const mnhn_vcsr *)src;
-  return region_query_vcsr(csr, p, vec_thr(csr->norm, eps), out);
-}
-
 static split_set dbscan_vec(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g, int norm) {
   if (!g.vec || g.norm != norm) {
     printf("dbscan_L%d_GPU needs a context from mnhn_gpu_initialization_values with the same norm\n", norm);
@@ -322,8 +502,15 @@ static split_set dbscan_vec(dataset ds, float epsilon, int minpts, mnhn_gpu_stuf
     _exit(1);
   }
   apply_defaults(&epsilon, &minpts);
-  if (ds.n_values > 0) ensure_vec_lists(g, epsilon);
-  return dbscan_core(ds.n_values, epsilon, minpts, rq_vec, &g);
+  const int n = ds.n_values;
+  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL);
+  ensure_vec_lists(g, epsilon);
+  /* the stored rows list p itself when its own accumulator qualifies */
+  const mnhn_vcsr *c = &g.cache->vcsr;
+  pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, NULL, c->val, vec_thr(norm, epsilon), 0);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+  free_pass_lists(&pl);
+  return s;
 }
 
 split_set dbscan_L1_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
@@ -349,19 +536,26 @@ split_set dbscan_L2(dataset ds, float epsilon, int minpts) {
 }
 
 split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts) {
-  return dbscan_core(csr ? csr->n_rows : 0, epsilon, minpts, rq_vcsr, (void *)csr);
+  const int n = csr ? csr->n_rows : 0;
+  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL);
+  pass_lists pl = build_pass_lists(n, csr->row_ptr, csr->col, NULL, csr->val, vec_thr(csr->norm, epsilon), 0);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+  free_pass_lists(&pl);
+  return s;
 }
 
 void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
                      dataset ds, float epsilon_start, float epsilon_inc, int minpts,
                      char *split_files_prefix, int n_threads) {
   (void)n_threads; /* epsilon values are scanned in order by one thread */
+  const double t_init = now_s();
   mnhn_gpu_stuff g = dbscanner == dbscan_L2_GPU   ? mnhn_gpu_initialization_values(ds, MNHN_NORM_L2)
                      : dbscanner == dbscan_L1_GPU ? mnhn_gpu_initialization_values(ds, MNHN_NORM_L1)
                                                   : mnhn_gpu_initialization(ds);
   /* neighbour lists are computed up to 8 increments beyond the epsilon asked
    * for, so the sweep's next passes reuse them */
   mnhn_gpu_set_lookahead(g, 8.0f * (epsilon_inc > 0 ? epsilon_inc : 0));
+  if (timing_on()) fprintf(stderr, "[mnhn] device context: %.3f s\n", now_s() - t_init);
 
   printf("Performing adaptive clustering with parameters: \n"
          "minpts: %i\n"
@@ -435,6 +629,7 @@ void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff)
     }
   }
   mnhn_gpu_destroy(g);
+  const double t_out = now_s();
 
   printf("Connections found: \n");
   size_t plen = strlen(split_files_prefix);
@@ -457,6 +652,7 @@ void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff)
     free(conn[i - 1]);
   }
   for (int i = 0; i < nsets; i++) free_split_set_and_associated_clusters(sets[i]);
+  if (timing_on()) fprintf(stderr, "[mnhn] split sets and relations written: %.3f s\n", now_s() - t_out);
   free(is_kept);
   free(conn);
   free(kept);

@@ -348,3 +348,55 @@ def test_kmer_column_counts(pkg):
         assert lib.number_of_protein_kmers_from_length(k) == 20 ** k
     assert [lib.mnhn_kmer_columns(k, 0) for k in (0, 1, 15, 16)] == [0, 4, 4 ** 15, 0]
     assert [lib.mnhn_kmer_columns(k, 1) for k in (0, 1, 7, 8)] == [0, 20, 20 ** 7, 0]
+
+
+def test_dbscan_threaded_pass_lists_match_oracle(pkg, oracle, monkeypatch):
+    """Lists large enough (> 2^20 stored entries) that each pass's neighbour
+    lists are built by several host threads: clusters equal the oracle's for
+    Smith-Waterman-style lists (self inserted) and vector lists (self stored)."""
+    monkeypatch.setenv("MNHN_HOST_THREADS", "7")
+    rng = np.random.default_rng(12)
+    n = 3000
+    centre = rng.integers(0, 60, n)
+    d = np.abs(centre[:, None] - centre[None, :]) * 40 + rng.integers(0, 120, (n, n))
+    d = np.triu(d, 1)
+    d = (d + d.T).astype(np.int64)
+    eps_max = 2000
+    keep = (d <= eps_max) & ~np.eye(n, dtype=bool)
+    assert keep.sum() > (1 << 20)
+    rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+    col = np.nonzero(keep)[1].astype(np.int32)
+    dist = d[keep].astype(np.int16)
+    for eps, minpts in ((60, 4), (150.5, 10), (-2.0, 3)):
+        got = pkg.dbscan_csr(rp, col, dist, eps_max, eps, minpts)
+        want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(d))
+        assert got == [c.members for c in want.clusters], (eps, minpts)
+    acc = d.astype(np.float32)
+    keepv = acc <= np.float32(eps_max)
+    rpv = np.concatenate([[0], np.cumsum(keepv.sum(1))]).astype(np.int64)
+    colv = np.nonzero(keepv)[1].astype(np.int32)
+    val = acc[keepv]
+    for eps, minpts in ((60.0, 4), (140.0, 12)):
+        got = pkg.dbscan_vcsr(rpv, colv, val, 1, float(eps_max), eps, minpts)
+        want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_acc(acc, 1))
+        assert got == [c.members for c in want.clusters], (eps, minpts)
+
+
+def test_dbscan_dense_neighbourhoods_match_oracle(pkg, oracle):
+    """Neighbourhoods covering most points: the expansion walks the points
+    still outside the cluster and looks them up in N(q) instead of scanning
+    N(q); the clusters and member order equal the oracle's."""
+    rng = np.random.default_rng(21)
+    n = 1200
+    centre = rng.integers(0, 8, n)
+    d = np.abs(centre[:, None] - centre[None, :]) * 100 + rng.integers(0, 300, (n, n))
+    d = np.triu(d, 1)
+    d = (d + d.T).astype(np.int64)
+    keep = (d <= 5000) & ~np.eye(n, dtype=bool)
+    rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+    col = np.nonzero(keep)[1].astype(np.int32)
+    dist = d[keep].astype(np.int16)
+    for eps, minpts in ((900, 5), (600, 50), (400, 5), (1500, 2)):
+        got = pkg.dbscan_csr(rp, col, dist, 5000, eps, minpts)
+        want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(d))
+        assert got == [c.members for c in want.clusters], (eps, minpts)

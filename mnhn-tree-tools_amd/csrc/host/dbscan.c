@@ -148,9 +148,13 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
   int rc = mnhn_sw_eps_csr(g.sw, eps_max, &c->csr);
   if (rc) die_engine("mnhn_sw_eps_csr", rc);
   c->have = 1;
-  if (timing_on())
-    fprintf(stderr, "[mnhn] epsilon lists up to %d: %lld entries, %.3f s\n", (int)eps_max, (long long)c->csr.nnz,
-            now_s() - t0);
+  if (timing_on()) {
+    mnhn_sw_timing tm;
+    memset(&tm, 0, sizeof(tm));
+    (void)mnhn_sw_last_timing(g.sw, &tm);
+    fprintf(stderr, "[mnhn] epsilon lists up to %d: %lld entries, %.3f s (SW kernels %.3f s, device lists %.3f s)\n",
+            (int)eps_max, (long long)c->csr.nnz, now_s() - t0, tm.kernel_ms / 1e3, (tm.total_ms - tm.kernel_ms) / 1e3);
+  }
   if (path && *path) {
     if (mnhn_csr_save(&c->csr, hash, path) == MNHN_OK)
       fprintf(stderr, "epsilon lists (eps_max %d) written to %s\n", (int)eps_max, path);

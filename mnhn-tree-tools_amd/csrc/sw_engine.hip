@@ -45,6 +45,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -1853,12 +1854,31 @@ int mnhn_detail::copy_d2h(hipStream_t st, mnhn_detail_stage &s, void *dst, const
     return hipMemcpyAsync(s.h[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, st) == hipSuccess &&
            hipEventRecord(s.ev[k & 1], st) == hipSuccess;
   };
+  // the pinned -> pageable copy of a chunk is split over host threads (the
+  // destination pages are touched for the first time here, which one thread
+  // does at about 10 GB/s) while the next chunk's DMA is in flight
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<unsigned>(8u, hw);
   if (!issue(0)) return MNHN_ERR_DEVICE;
   for (size_t k = 0; k < chunks; ++k) {
     if (k + 1 < chunks && !issue(k + 1)) return MNHN_ERR_DEVICE;
     if (hipEventSynchronize(s.ev[k & 1]) != hipSuccess) return MNHN_ERR_DEVICE;
     const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
-    memcpy((char *)dst + off, s.h[k & 1], len);
+    char *d = (char *)dst + off;
+    const char *src_h = (const char *)s.h[k & 1];
+    if (nt > 1 && len >= (8u << 20)) {
+      const size_t part = (len / nt + 4095) / 4096 * 4096;
+      std::vector<std::thread> th;
+      for (int t = 1; t < nt; ++t) {
+        const size_t b = (size_t)t * part;
+        if (b >= len) break;
+        th.emplace_back([=]() { memcpy(d + b, src_h + b, std::min(part, len - b)); });
+      }
+      memcpy(d, src_h, std::min(part, len));
+      for (auto &x : th) x.join();
+    } else {
+      memcpy(d, src_h, len);
+    }
   }
   return MNHN_OK;
 }

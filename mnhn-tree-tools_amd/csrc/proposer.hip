@@ -421,7 +421,7 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
 // no test).  Hits are counted first and reserved with one atomic per wave,
 // then written (sorted positions p << 32 | q) at consecutive addresses.
 template <bool kCos>
-__global__ __launch_bounds__(256) void pair_candidates_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void pair_candidates_kernel(
     const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
     const bf16 *__restrict__ Xs, const float *__restrict__ pnorm, const float *__restrict__ cmin,
     const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
@@ -441,69 +441,65 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
   const int i0 = ti * BT, j0 = tj * BT;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
-  f32x4 acc[4][4], accx[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-      accx[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    }
   if (tid == 0) any_hit = 0;
-  tile_dot<D3>(Es, Es, n, i0, j0, sA, sB, acc);
-  // per-lane row / column terms (out-of-range rows and columns get lengths
-  // that no |dl| <= max_dl can match)
-  float ni[4][4], si[4][4], nj[4], sj[4];
-  int li[4][4], lj[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-      const bool v = gi < n;
-      ni[i][r] = v ? norm[gi] : 0.f;
-      si[i][r] = v ? srad[gi] : 0.f;
-      li[i][r] = v ? lens[gi] : -(1 << 28);
-    }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-    const bool v = gj < n;
-    nj[j] = v ? norm[gj] : 0.f;
-    sj[j] = v ? srad[gj] : 0.f;
-    lj[j] = v ? lens[gj] : (1 << 29);
-  }
-  const bool diag = ti == tj;
-  auto hit1 = [&](int i, int r, int j) -> bool {
-    const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-    const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-    const int dl = li[i][r] - lj[j];
-    const float rad = dl >= 0 ? si[i][r] : sj[j];
-    const float d2 = ni[i][r] + nj[j] - 2.0f * acc[i][j][r];
-    return (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad * rad;
-  };
-  float pi[4][4], ci[4][4], pj[4], cj[4];
-  if constexpr (kCos) {
-    bool any = false;
+  // Stage 1 is folded into two 64-bit masks per lane (bit 16 i + 4 r + j):
+  // the radius test, and which side is the longer sequence; its accumulators
+  // and row / column terms are dead before stage 2 starts, which keeps the
+  // kernel at 4 waves per SIMD.
+  unsigned long long m1 = 0ull, mlong = 0ull;
+  {
+    f32x4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    tile_dot<D3>(Es, Es, n, i0, j0, sA, sB, acc);
+    // out-of-range rows and columns get lengths that no |dl| <= max_dl matches
+    float nj[4], sj[4];
+    int lj[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) any |= hit1(i, r, j);
-    if (__ballot(any) != 0ull && lane == 0) any_hit = 1;
-    __syncthreads();
-    if (any_hit == 0) return;  // uniform over the block
-    tile_dot<D0>(Xs, Xs, n, i0, j0, sA, sB, accx);
+    for (int j = 0; j < 4; ++j) {
+      const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+      const bool v = gj < n;
+      nj[j] = v ? norm[gj] : 0.f;
+      sj[j] = v ? srad[gj] : 0.f;
+      lj[j] = v ? lens[gj] : (1 << 29);
+    }
+    const bool diag = ti == tj;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
         const bool v = gi < n;
-        pi[i][r] = v ? pnorm[gi] : 0.f;
-        ci[i][r] = v ? cmin[gi] : -1.f;
+        const float ni = v ? norm[gi] : 0.f;
+        const float si = v ? srad[gi] : 0.f;
+        const int li = v ? lens[gi] : -(1 << 28);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+          const int dl = li - lj[j];
+          const float rad = dl >= 0 ? si : sj[j];
+          const float d2 = ni + nj[j] - 2.0f * acc[i][j][r];
+          const bool h = (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad * rad;
+          const int bit = 16 * i + 4 * r + j;
+          m1 |= (unsigned long long)h << bit;
+          mlong |= (unsigned long long)(dl >= 0) << bit;
+        }
       }
+  }
+  unsigned long long m2 = m1;  // hits after both stages
+  if constexpr (kCos) {
+    if (__ballot(m1 != 0ull) != 0ull && lane == 0) any_hit = 1;
+    __syncthreads();
+    if (any_hit == 0) return;  // uniform over the block
+    f32x4 accx[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) accx[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    tile_dot<D0>(Xs, Xs, n, i0, j0, sA, sB, accx);
+    float pj[4], cj[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
@@ -511,23 +507,26 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
       pj[j] = v ? pnorm[gj] : 0.f;
       cj[j] = v ? cmin[gj] : -1.f;
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+        const bool v = gi < n;
+        const float pi = v ? pnorm[gi] : 0.f;
+        const float ci = v ? cmin[gi] : -1.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int bit = 16 * i + 4 * r + j;
+          const float c = ((mlong >> bit) & 1ull) ? ci : cj[j];
+          const float dot = accx[i][j][r];
+          if (c >= 0.0f && !(dot >= 0.0f && dot * dot >= c * c * pi * pj[j])) m2 &= ~(1ull << bit);
+        }
+      }
   }
-  auto hit = [&](int i, int r, int j) -> bool {
-    if (!hit1(i, r, j)) return false;
-    if constexpr (kCos) {
-      const float c = li[i][r] - lj[j] >= 0 ? ci[i][r] : cj[j];
-      const float dot = accx[i][j][r];
-      if (c >= 0.0f && !(dot >= 0.0f && dot * dot >= c * c * pi[i][r] * pj[j])) return false;
-    }
-    return true;
-  };
   unsigned total = 0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) total += (unsigned)__popcll(__ballot(hit(i, r, j)));
+  for (int b = 0; b < 64; ++b) total += (unsigned)__popcll(__ballot((m2 >> b) & 1ull));
   if (total == 0) return;
   unsigned long long base = 0;
   if (lane == 0) base = atomicAdd(count, (unsigned long long)total);
@@ -538,7 +537,8 @@ __global__ __launch_bounds__(256) void pair_candidates_kernel(
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const bool h = hit(i, r, j);
+        const int bit = 16 * i + 4 * r + j;
+        const bool h = (m2 >> bit) & 1ull;
         const unsigned long long bal = __ballot(h);
         if (h) {
           const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),

@@ -367,7 +367,10 @@ __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t 
 template <int K>
 __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int n,
                                          int i0, int j0, bf16 *sA, bf16 *sB, f32x4 (&acc)[4][4]) {
-  constexpr int KH = 64, LK = KH + 8;
+  // sA / sB hold two stages of 128 x (64 + 8) each: chunk c computes from
+  // stage c & 1 while chunk c + 1 is loaded and stored into the other, so one
+  // barrier per chunk suffices
+  constexpr int KH = 64, LK = KH + 8, STG = 128 * LK;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 8 * (lane >> 4);
@@ -384,31 +387,39 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
       if (j0 + r < n) vb[t] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + r) * K + kh + k);
     }
   };
-  load(0);
-  for (int kh = 0; kh < K; kh += KH) {
-    __syncthreads();
+  auto store = [&](int stage) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
       const int r = ch >> 3, k = (ch & 7) * 8;
-      *reinterpret_cast<bf16x8 *>(sA + r * LK + k) = va[t];
-      *reinterpret_cast<bf16x8 *>(sB + r * LK + k) = vb[t];
+      *reinterpret_cast<bf16x8 *>(sA + stage * STG + r * LK + k) = va[t];
+      *reinterpret_cast<bf16x8 *>(sB + stage * STG + r * LK + k) = vb[t];
     }
-    __syncthreads();
-    if (kh + KH < K) load(kh + KH);
+  };
+  __syncthreads();  // the previous user of both stages is done
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int c = 0; c * KH < K; ++c) {
+    const int cur = c & 1;
+    const bool more = (c + 1) * KH < K;
+    if (more) load((c + 1) * KH);
+    const bf16 *a = sA + cur * STG, *b = sB + cur * STG;
 #pragma unroll
     for (int ks = 0; ks < KH; ks += 32) {
       bf16x8 af[4], bfr[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(sA + (64 * wm + 16 * i + fr) * LK + ks + fk);
+      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(a + (64 * wm + 16 * i + fr) * LK + ks + fk);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(sB + (64 * wn + 16 * j + fr) * LK + ks + fk);
+      for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(b + (64 * wn + 16 * j + fr) * LK + ks + fk);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
+    if (more) store(cur ^ 1);
+    __syncthreads();
   }
 }
 
@@ -427,8 +438,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
     const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
     unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap) {
   constexpr int BT = 128, LK = 64 + 8;
-  __shared__ __attribute__((aligned(16))) bf16 sA[BT * LK];
-  __shared__ __attribute__((aligned(16))) bf16 sB[BT * LK];
+  __shared__ __attribute__((aligned(16))) bf16 sA[2 * BT * LK];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2 * BT * LK];
   __shared__ int any_hit;
   const int blk = blockIdx.x;
   int lo = 0, hi = tiles;  // tile_off[lo] <= blk < tile_off[hi]

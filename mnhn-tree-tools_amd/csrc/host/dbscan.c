@@ -298,13 +298,15 @@ static pass_lists build_pass_lists(int n, const int64_t *rp, const int32_t *col,
         _exit(1);
       }
     }
+    int started[16];
     for (int k = 0; k < T; k++) {
       jobs[k] = (pass_job){rp, col, dist, val, thr, self, cnt, &pl, bounds[k], bounds[k + 1], phase};
-      if (T > 1) pthread_create(&th[k], NULL, pass_worker, &jobs[k]);
-      else pass_worker(&jobs[k]);
+      /* a thread that cannot be started does its rows here */
+      started[k] = T > 1 && pthread_create(&th[k], NULL, pass_worker, &jobs[k]) == 0;
+      if (!started[k]) pass_worker(&jobs[k]);
     }
-    if (T > 1)
-      for (int k = 0; k < T; k++) pthread_join(th[k], NULL);
+    for (int k = 0; k < T; k++)
+      if (started[k]) pthread_join(th[k], NULL);
   }
   free(cnt);
   return pl;

@@ -1869,12 +1869,18 @@ int mnhn_detail::copy_d2h(hipStream_t st, mnhn_detail_stage &s, void *dst, const
     if (nt > 1 && len >= (8u << 20)) {
       const size_t part = (len / nt + 4095) / 4096 * 4096;
       std::vector<std::thread> th;
-      for (int t = 1; t < nt; ++t) {
+      size_t done_from = len;  // parts from here on are copied by threads
+      for (int t = nt - 1; t >= 1; --t) {
         const size_t b = (size_t)t * part;
-        if (b >= len) break;
-        th.emplace_back([=]() { memcpy(d + b, src_h + b, std::min(part, len - b)); });
+        if (b >= len) continue;
+        try {
+          th.emplace_back([=]() { memcpy(d + b, src_h + b, std::min(part, len - b)); });
+          done_from = b;
+        } catch (...) {
+          break;  // no more threads: the rest is copied below
+        }
       }
-      memcpy(d, src_h, std::min(part, len));
+      memcpy(d, src_h, std::min(done_from, len));
       for (auto &x : th) x.join();
     } else {
       memcpy(d, src_h, len);

@@ -86,6 +86,15 @@ void mnhn_sw_destroy(mnhn_sw_ctx *ctx);
  * region_query (dbscan.c:471-513, smith-waterman-gpu.cl:13-98). */
 int mnhn_sw_one_vs_all(mnhn_sw_ctx *ctx, int query, int32_t *dist_out);
 
+/* The reference's scalar seam, score() (smith-waterman.h:1,
+ * smith-waterman.c:16-81), with the same signature and result: one pair
+ * computed on the GPU (a two-sequence context per call; work is accepted and
+ * unused).  Reference objects that call score() bind to it unchanged with
+ * -Wl,--defsym,score=mnhn_score.  There is no error path in the reference's
+ * signature: a failure prints the reason and _exit(1)s.  One call costs a
+ * context set-up, so batch callers should use mnhn_sw_pairs. */
+unsigned int mnhn_score(const char *A, const char *B, size_t len_a, size_t len_b, int *work);
+
 /* dist_out[k] = d(seq[ij[2k]], seq[ij[2k+1]]) for k in [0, m).  Replaces the
  * loops of score() calls in the reference's CPU region_query
  * (dbscan.c:546-555) and other batch callers of score() (smith-waterman.h:1). */

@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <sys/mman.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
@@ -1418,6 +1419,24 @@ int mnhn_sw_block(mnhn_sw_ctx *c, const int32_t *rows, int nr, const int32_t *co
   }
   dev_free(d_out);
   return rc;
+}
+
+unsigned int mnhn_score(const char *A, const char *B, size_t len_a, size_t len_b, int *work) {
+  (void)work;
+  const char *seqs[2] = {A, B};
+  const size_t lens[2] = {len_a, len_b};
+  const int32_t ij[2] = {0, 1};
+  int32_t d = 0;
+  mnhn_sw_ctx *c = nullptr;
+  int rc = mnhn_sw_create(seqs, lens, 2, nullptr, &c);
+  if (rc == MNHN_OK) rc = mnhn_sw_pairs(c, ij, 1, &d);
+  mnhn_sw_destroy(c);
+  if (rc != MNHN_OK) {
+    printf("mnhn_score failed: %s\n", mnhn_sw_strerror(rc));
+    fflush(stdout);
+    _exit(1);
+  }
+  return (unsigned int)d;
 }
 
 int mnhn_sw_pairs(mnhn_sw_ctx *c, const int32_t *ij, int64_t m, int32_t *dist_out) {

@@ -136,3 +136,45 @@ def test_newick_tool_reads_our_outputs(name, tmp_path):
     t = subprocess.run(argv, cwd=tmp_path, capture_output=True, timeout=120)
     assert t.returncode == meta["returncode"]
     assert t.stdout == open(os.path.join(NEWICK_DIR, name, "stdout.txt"), "rb").read()
+
+
+# ---- the scalar seam: the reference's own CLIs with score() bound to mnhn_score ----
+
+REF_DIR_BUILT = os.path.join(os.path.dirname(PKG_DIR), "oracle", "_ref")
+SCORE_CASES = [c for c in cli_cases() if c[0] in ("dbscan_test2", "dbscan_test3", "dbscan_test4",
+                                                  "adaptive_test3")]
+
+
+@pytest.mark.parametrize("name,meta", SCORE_CASES, ids=[c[0] for c in SCORE_CASES])
+def test_reference_cli_with_score_bound_to_engine(name, meta, tmp_path):
+    """The reference's cluster_dbscan_SW / adaptive_clustering_SW compiled from
+    its sources with score() defined by csrc/binding/reference_score.c (one
+    GPU pair per call through mnhn_score) reproduce the reference's outputs."""
+    exe = os.path.join(REF_DIR_BUILT, meta["program"] + "_mnhn_score")
+    if not os.path.exists(exe):
+        pytest.skip("relinked reference CLIs not built (make -C oracle relink)")
+    stdout, files = read_case_files(name, meta)
+    fasta = tmp_path / "in.fasta"
+    shutil.copyfile(os.path.join(GOLDEN, "inputs", meta["input"]), fasta)
+    if meta["program"] == "cluster_dbscan_SW":
+        argv = [exe, str(fasta)] + meta["args"] + [meta["prefix"]]
+    else:
+        argv = [exe, str(fasta)] + [a if a != "PREFIX" else meta["prefix"] for a in meta["args"]]
+    print(f"relinked reference case {name}", flush=True)
+    r = subprocess.run(argv, cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == meta["returncode"], r.stderr.decode()[-2000:]
+    assert r.stdout == stdout
+    for fn, data in files.items():
+        assert (tmp_path / fn).read_bytes() == data, fn
+
+
+def test_mnhn_score_matches_golden_pairs(pkg, golden_pairs):
+    """mnhn_score, the scalar seam, against the reference's score() vectors."""
+    import ctypes
+    lib = pkg.load_library()
+    lib.mnhn_score.restype = ctypes.c_uint
+    lib.mnhn_score.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_size_t,
+                               ctypes.c_void_p]
+    for a, b, d in golden_pairs[:120]:
+        a, b = a.encode(), b.encode()
+        assert lib.mnhn_score(a, b, len(a), len(b), None) == d, (a[:20], b[:20])

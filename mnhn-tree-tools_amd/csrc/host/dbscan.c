@@ -163,36 +163,16 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
   }
 }
 
-/* N(p) from epsilon lists (ascending, p included); returns |N(p)|. */
-static int region_query_csr(const mnhn_csr *csr, int p, float eps, int *out) {
-  int k = 0;
-  const int64_t b = csr->row_ptr[p], e = csr->row_ptr[p + 1];
-  const int self = (float)0 <= eps;
-  int placed = !self;
-  for (int64_t t = b; t < e; t++) {
-    const int j = csr->col[t];
-    if (!placed && j > p) {
-      out[k++] = p;
-      placed = 1;
-    }
-    if ((float)csr->dist[t] <= eps) out[k++] = j;
-  }
-  if (!placed) out[k++] = p;
-  return k;
-}
-
-/* N(p) for threshold eps into out[] (ascending, p included); returns |N(p)|. */
+/* N(p) for threshold eps into out[] (ascending, p included) with one
+ * one-vs-all launch (MNHN_REGION_QUERY=launch); returns |N(p)|.  Passes over
+ * stored lists read their rows through pass_lists instead. */
 static int region_query(mnhn_gpu_stuff g, int n, int p, float eps, int *out) {
-  mnhn_eps_cache *c = g.cache;
   int k = 0;
-  if (c->per_query_launch) {
-    int rc = mnhn_sw_one_vs_all(g.sw, p, c->dist_row);
-    if (rc) die_engine("mnhn_sw_one_vs_all", rc);
-    for (int j = 0; j < n; j++)
-      if ((float)c->dist_row[j] <= eps) out[k++] = j;
-    return k;
-  }
-  return region_query_csr(&c->csr, p, eps, out);
+  int rc = mnhn_sw_one_vs_all(g.sw, p, g.cache->dist_row);
+  if (rc) die_engine("mnhn_sw_one_vs_all", rc);
+  for (int j = 0; j < n; j++)
+    if ((float)g.cache->dist_row[j] <= eps) out[k++] = j;
+  return k;
 }
 
 /* A region query fills out[] (or points *res at a stored row) with N(p),

@@ -709,7 +709,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--eps-max", type=int, default=2500)
     ap.add_argument("--n", type=int, default=0, help="use only the first n sequences")
-    ap.add_argument("--cpu-sample", type=int, default=200)
+    ap.add_argument("--cpu-sample", type=int, default=260,
+                    help="sequences in the CPU-baseline sample (all their unique pairs; about 10 s single-thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")

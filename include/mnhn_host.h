@@ -135,6 +135,24 @@ split_set dbscan_L2(dataset ds, float epsilon, int minpts);
  * reference. */
 split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts);
 
+/* n_eps DBSCAN passes over the same lists, out[k] the pass at epsilon[k]
+ * (each as mnhn_dbscan_csr / mnhn_dbscan_vcsr would return it).  The stored
+ * rows are ordered by distance once, so each pass costs what its expansion
+ * reads rather than a filter of every stored entry; when that copy of the
+ * lists would exceed a quarter of host memory the passes filter as the
+ * single-pass calls do.  symmetric != 0 declares that every stored (p, q, d)
+ * has its (q, p, d) with the same d -- true of all lists the engines return
+ * (mnhn_sw_eps_csr, mnhn_prefilter_eps_csr, mnhn_vec_eps_csr, .ecsr files of
+ * them) -- and lets large clusters find the points still outside them from
+ * those points' own short lists; lists declared symmetric that are not give
+ * unspecified clusters.  Returns MNHN_ERR_ARG for a NULL csr, or for NULL
+ * epsilon / out with n_eps > 0.  No counterpart in the reference, whose
+ * adaptive sweep (dbscan.c:1061-1115) recomputes distances every pass. */
+int mnhn_dbscan_csr_sweep(const mnhn_csr *csr, const float *epsilon, int n_eps, int minpts, int symmetric,
+                          split_set *out);
+int mnhn_dbscan_vcsr_sweep(const mnhn_vcsr *csr, const float *epsilon, int n_eps, int minpts, int symmetric,
+                           split_set *out);
+
 /* ---- vector inputs and outputs ---- */
 /* dataset.c:346-375: features = tabs on the first line, samples = line
  * breaks from the end of the first line on (the first one included) */

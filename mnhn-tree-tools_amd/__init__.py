@@ -57,7 +57,7 @@ HOST_SYMBOLS = [
     "shortest_longest_distance_in_matrix", "load_projections_from_file_into_dataset",
     "free_values_from_dataset", "print_SW_PCA_L_comparison",
     "mnhn_gpu_initialization_values", "dbscan_L1_GPU", "dbscan_L2_GPU", "dbscan_L1", "dbscan_L2",
-    "mnhn_dbscan_vcsr", "shape_from_kmer_file", "load_kmer_from_file_into_dataset",
+    "mnhn_dbscan_vcsr", "mnhn_dbscan_csr_sweep", "mnhn_dbscan_vcsr_sweep", "shape_from_kmer_file", "load_kmer_from_file_into_dataset",
     "create_single_cluster_file_with_values", "create_cluster_files_with_values", "free_dataset",
     "number_of_kmers_from_length", "number_of_protein_kmers_from_length", "frequencies_from_dataset",
     "write_kmer_base", "free_kmer_frequencies",
@@ -610,6 +610,58 @@ def dbscan_vcsr(rp: np.ndarray, col: np.ndarray, val: np.ndarray, norm: int, thr
     finally:
         lib.free_split_set_and_associated_clusters(s)
     return out
+
+
+def _sweep(fn, csr, eps, minpts, symmetric):
+    lib = load_library()
+    eps = np.ascontiguousarray(eps, dtype=np.float32)
+    sets = (_SplitSet * max(len(eps), 1))()
+    f = getattr(lib, fn)
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.POINTER(type(csr)), ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                  ctypes.c_int, ctypes.POINTER(_SplitSet)]
+    lib.free_split_set_and_associated_clusters.argtypes = [_SplitSet]
+    lib.free_split_set_and_associated_clusters.restype = None
+    _check(f(ctypes.byref(csr), eps.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(eps), int(minpts),
+             int(bool(symmetric)), sets), fn)
+    out = []
+    for k in range(len(eps)):
+        s = sets[k]
+        try:
+            out.append([np.ctypeslib.as_array(s.clusters[c].members, shape=(s.clusters[c].n_members,)).tolist()
+                        if s.clusters[c].n_members else [] for c in range(s.n_clusters)])
+        finally:
+            lib.free_split_set_and_associated_clusters(s)
+    return out
+
+
+def dbscan_csr_sweep(rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int, eps, minpts: int,
+                     symmetric: bool = True):
+    """DBSCAN passes at every epsilon of eps over the same Smith-Waterman
+    lists (mnhn_dbscan_csr_sweep: the lists ordered by distance once); one
+    list of clusters per epsilon, each as dbscan_csr returns it.  symmetric:
+    every (p, q, d) is listed as (q, p, d) too, as in the engine's lists."""
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    dist = np.ascontiguousarray(dist, dtype=np.int16)
+    csr = Csr(len(rp) - 1, int(eps_max), len(col), rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+              col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+    return _sweep("mnhn_dbscan_csr_sweep", csr, eps, minpts, symmetric)
+
+
+def dbscan_vcsr_sweep(rp: np.ndarray, col: np.ndarray, val: np.ndarray, norm: int, thr: float, eps,
+                      minpts: int, symmetric: bool = True):
+    """L1 / L2 DBSCAN passes at every epsilon of eps over the same vector
+    lists (mnhn_dbscan_vcsr_sweep); one list of clusters per epsilon."""
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float32)
+    csr = Vcsr(len(rp) - 1, int(norm), float(thr), 0, len(col),
+               rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+               col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+               val.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return _sweep("mnhn_dbscan_vcsr_sweep", csr, eps, minpts, symmetric)
 
 
 def csr_save(path: str, rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int,

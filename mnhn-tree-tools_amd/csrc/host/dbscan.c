@@ -27,9 +27,11 @@
  * Where the distances come from: every region query of a pass reads the
  * epsilon-neighbour lists (CSR) that the engine computes once for all unique
  * pairs up to eps + lookahead (mnhn_sw_eps_csr); later passes with a smaller
- * or equal epsilon reuse them.  With MNHN_REGION_QUERY=launch each region
- * query instead launches the one-vs-all kernel, as the reference's GPU build
- * launches its kernel once per query.
+ * or equal epsilon reuse them.  During a sweep the stored rows are also
+ * ordered by distance once, and each pass reads N(p) as a prefix of that
+ * order.  With MNHN_REGION_QUERY=launch each region query instead launches
+ * the one-vs-all kernel, as the reference's GPU build launches its kernel
+ * once per query.
  */
 #include <math.h>
 #include <pthread.h>
@@ -50,7 +52,12 @@ struct mnhn_eps_cache {
   int32_t *dist_row; /* per-query launch buffer [n] */
   mnhn_vcsr vcsr;    /* vector datasets: lists up to vcsr.thr */
   int have_v;
+  int sweep;         /* many passes follow: order the stored lists by distance */
+  void *index;       /* the distance-ordered index of the stored lists, or NULL */
+  int index_tried;   /* built, or found too large, for the stored lists */
 };
+
+static void drop_index(mnhn_eps_cache *c);
 
 /* MNHN_TIMING=1: phase times on stderr (stdout is unchanged) */
 static double now_s(void) {
@@ -101,6 +108,7 @@ mnhn_gpu_stuff mnhn_gpu_initialization_values(dataset ds, int norm) {
 
 void mnhn_gpu_destroy(mnhn_gpu_stuff g) {
   if (g.cache) {
+    drop_index(g.cache);
     if (g.cache->have) mnhn_csr_free(&g.cache->csr);
     if (g.cache->have_v) mnhn_vcsr_free(&g.cache->vcsr);
     free(g.cache->dist_row);
@@ -126,6 +134,7 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
   mnhn_eps_cache *c = g.cache;
   const int32_t need = eps_floor(eps);
   if (c->have && c->csr.eps_max >= need) return;
+  drop_index(c);
   if (c->have) mnhn_csr_free(&c->csr);
   c->have = 0;
   double want = (double)need + (double)c->lookahead;
@@ -317,7 +326,376 @@ static inline int in_row(const int *a, int m, int x) {
   return lo < m && a[lo] == x;
 }
 
-static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, void *src) {
+/* ---- distance-ordered index: many passes over the same stored lists ----
+ * For an epsilon sweep every stored row is kept a second time with its
+ * entries ordered by distance (ties in ascending column).  For any threshold
+ * the stored lists cover, |N(p)| is then one binary search and the stored
+ * entries of N(p) are a prefix of the row, so a pass reads what its expansion
+ * uses instead of filtering every stored entry. */
+typedef struct {
+  int n;
+  const int64_t *rp;
+  const int32_t *col;  /* stored rows, ascending columns */
+  const int16_t *dist; /* Smith-Waterman lists, or */
+  const float *val;    /* vector lists */
+  int32_t *dcol;       /* the same rows ordered by distance */
+  int16_t *ddist;
+  float *dval;
+  int symmetric;       /* every stored (p, q, d) has its (q, p, d) */
+} dist_index;
+
+/* one pass over the index: keep entries <= thr, insert p itself when self */
+typedef struct {
+  const dist_index *ix;
+  float thr;
+  int self;
+} ix_pass;
+
+/* sort key whose unsigned order is the order of the distances: signed int16,
+ * or float with every NaN last */
+static inline uint32_t dist_key(const dist_index *ix, int64_t t) {
+  if (ix->dist) return (uint32_t)(uint16_t)((uint16_t)ix->dist[t] ^ 0x8000u);
+  uint32_t u;
+  memcpy(&u, &ix->val[t], sizeof(u));
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+typedef struct {
+  dist_index *ix;
+  int r0, r1;
+  int64_t max_len;
+} index_job;
+
+static void *index_worker(void *arg) {
+  index_job *jb = (index_job *)arg;
+  dist_index *ix = jb->ix;
+  const int bytes = ix->dist ? 2 : 4;
+  const size_t m = (size_t)(jb->max_len ? jb->max_len : 1);
+  uint32_t *k0 = (uint32_t *)malloc(sizeof(uint32_t) * m), *k1 = (uint32_t *)malloc(sizeof(uint32_t) * m);
+  int32_t *i0 = (int32_t *)malloc(sizeof(int32_t) * m), *i1 = (int32_t *)malloc(sizeof(int32_t) * m);
+  int *hist = ix->dist ? (int *)malloc(sizeof(int) * 65536) : NULL;
+  if (!k0 || !k1 || !i0 || !i1 || (ix->dist && !hist)) {
+    printf("out of memory ordering the epsilon lists by distance\n");
+    fflush(stdout);
+    _exit(1);
+  }
+  for (int p = jb->r0; p < jb->r1; p++) {
+    const int64_t b = ix->rp[p];
+    const int len = (int)(ix->rp[p + 1] - b);
+    if (len == 0) continue;
+    if (ix->dist && len > 32) {
+      /* integer distances over a short range: one counting sort, written in
+       * place (stable, so ties keep ascending columns) */
+      int lo = ix->dist[b], hi = lo;
+      for (int r = 1; r < len; r++) {
+        const int d = ix->dist[b + r];
+        lo = d < lo ? d : lo;
+        hi = d > hi ? d : hi;
+      }
+      const int range = hi - lo + 1;
+      if (range <= 4 * len + 256) {
+        memset(hist, 0, sizeof(int) * (size_t)range);
+        for (int r = 0; r < len; r++) hist[ix->dist[b + r] - lo]++;
+        int sum = 0;
+        for (int d = 0; d < range; d++) {
+          const int c = hist[d];
+          hist[d] = sum;
+          sum += c;
+        }
+        for (int r = 0; r < len; r++) {
+          const int d = ix->dist[b + r];
+          const int64_t o = b + hist[d - lo]++;
+          ix->dcol[o] = ix->col[b + r];
+          ix->ddist[o] = (int16_t)d;
+        }
+        continue;
+      }
+    }
+    uint32_t *ka = k0, *kb = k1;
+    int32_t *ia = i0, *ib = i1;
+    for (int r = 0; r < len; r++) {
+      ka[r] = dist_key(ix, b + r);
+      ia[r] = r;
+    }
+    if (len <= 32) {
+      /* insertion sort: stable, ties keep ascending columns */
+      for (int r = 1; r < len; r++) {
+        const uint32_t k = ka[r];
+        const int32_t v = ia[r];
+        int s = r - 1;
+        while (s >= 0 && ka[s] > k) {
+          ka[s + 1] = ka[s];
+          ia[s + 1] = ia[s];
+          s--;
+        }
+        ka[s + 1] = k;
+        ia[s + 1] = v;
+      }
+    } else {
+      /* least significant digit first, 8 bits a pass: stable */
+      for (int sh = 0; sh < 8 * bytes; sh += 8) {
+        int cnt[256];
+        memset(cnt, 0, sizeof(cnt));
+        for (int r = 0; r < len; r++) cnt[(ka[r] >> sh) & 255u]++;
+        if (cnt[(ka[0] >> sh) & 255u] == len) continue; /* one digit value: order unchanged */
+        int sum = 0;
+        for (int d = 0; d < 256; d++) {
+          const int c = cnt[d];
+          cnt[d] = sum;
+          sum += c;
+        }
+        for (int r = 0; r < len; r++) {
+          const int o = cnt[(ka[r] >> sh) & 255u]++;
+          kb[o] = ka[r];
+          ib[o] = ia[r];
+        }
+        uint32_t *tk = ka;
+        ka = kb;
+        kb = tk;
+        int32_t *ti = ia;
+        ia = ib;
+        ib = ti;
+      }
+    }
+    for (int r = 0; r < len; r++) {
+      const int64_t t = b + ia[r];
+      ix->dcol[b + r] = ix->col[t];
+      if (ix->dist) ix->ddist[b + r] = ix->dist[t];
+      else ix->dval[b + r] = ix->val[t];
+    }
+  }
+  free(k0);
+  free(k1);
+  free(i0);
+  free(i1);
+  free(hist);
+  return NULL;
+}
+
+/* Builds the index of stored lists; 0 (and no index) when its copy of the
+ * lists would take more than a quarter of the host's memory.  symmetric:
+ * the caller vouches that every stored (p, q, d) has its (q, p, d). */
+static int build_dist_index(dist_index *ix, int n, const int64_t *rp, const int32_t *col, const int16_t *dist,
+                            const float *val, int symmetric) {
+  memset(ix, 0, sizeof(*ix));
+  const int64_t nnz = rp[n];
+  const size_t per = sizeof(int32_t) + (dist ? sizeof(int16_t) : sizeof(float));
+  const double phys = (double)sysconf(_SC_PHYS_PAGES) * (double)sysconf(_SC_PAGE_SIZE);
+  if (phys > 0 && (double)nnz * (double)per > phys / 4) return 0;
+  ix->n = n;
+  ix->rp = rp;
+  ix->col = col;
+  ix->dist = dist;
+  ix->val = val;
+  const size_t cap = (size_t)(nnz ? nnz : 1);
+  ix->dcol = (int32_t *)malloc(sizeof(int32_t) * cap);
+  if (dist) ix->ddist = (int16_t *)malloc(sizeof(int16_t) * cap);
+  else ix->dval = (float *)malloc(sizeof(float) * cap);
+  if (!ix->dcol || (dist && !ix->ddist) || (!dist && !ix->dval)) {
+    free(ix->dcol);
+    free(ix->ddist);
+    free(ix->dval);
+    memset(ix, 0, sizeof(*ix));
+    return 0;
+  }
+  const int T = host_threads(nnz);
+  pthread_t th[16];
+  index_job jobs[16];
+  int started[16];
+  int lo = 0;
+  for (int k = 0; k < T; k++) {
+    /* rows split into T ranges of about equal stored entries */
+    int hi = n;
+    if (k + 1 < T) {
+      const int64_t want = nnz / T * (k + 1);
+      int a = lo, z = n;
+      while (a < z) {
+        const int mid = a + (z - a) / 2;
+        if (rp[mid] < want) a = mid + 1;
+        else z = mid;
+      }
+      hi = a;
+    }
+    int64_t ml = 0;
+    for (int p = lo; p < hi; p++)
+      if (rp[p + 1] - rp[p] > ml) ml = rp[p + 1] - rp[p];
+    jobs[k] = (index_job){ix, lo, hi, ml};
+    lo = hi;
+  }
+  for (int k = 0; k < T; k++) {
+    started[k] = T > 1 && pthread_create(&th[k], NULL, index_worker, &jobs[k]) == 0;
+    if (!started[k]) index_worker(&jobs[k]);
+  }
+  for (int k = 0; k < T; k++)
+    if (started[k]) pthread_join(th[k], NULL);
+  ix->symmetric = symmetric != 0;
+  return 1;
+}
+
+static void free_dist_index(dist_index *ix) {
+  free(ix->dcol);
+  free(ix->ddist);
+  free(ix->dval);
+  memset(ix, 0, sizeof(*ix));
+}
+
+/* MNHN_SWEEP_INDEX=0: every pass filters the stored lists (no index) */
+static int index_allowed(void) {
+  const char *v = getenv("MNHN_SWEEP_INDEX");
+  return !(v && strcmp(v, "0") == 0);
+}
+
+static void drop_index(mnhn_eps_cache *c) {
+  if (c->index) {
+    free_dist_index((dist_index *)c->index);
+    free(c->index);
+  }
+  c->index = NULL;
+  c->index_tried = 0;
+}
+
+/* The index of the cache's stored lists for a sweep, built on first use;
+ * NULL outside a sweep or when it does not fit. */
+static const dist_index *cache_index(mnhn_eps_cache *c, int n, const int64_t *rp, const int32_t *col,
+                                     const int16_t *dist, const float *val) {
+  if (!c->sweep || !index_allowed()) return NULL;
+  if (!c->index_tried) {
+    c->index_tried = 1;
+    const double t0 = now_s();
+    dist_index *ix = (dist_index *)malloc(sizeof(dist_index));
+    /* the engines' lists hold both directions of every pair */
+    if (ix && build_dist_index(ix, n, rp, col, dist, val, 1)) {
+      c->index = ix;
+      if (timing_on())
+        fprintf(stderr, "[mnhn] epsilon lists ordered by distance: %.3f s\n", now_s() - t0);
+    } else {
+      free(ix);
+    }
+  }
+  return (const dist_index *)c->index;
+}
+
+static inline int ix_keep_sorted(const dist_index *ix, int64_t t, float thr) {
+  return ix->ddist ? (float)ix->ddist[t] <= thr : ix->dval[t] <= thr;
+}
+
+/* |N(p)|: the kept prefix of the distance-ordered row, plus p itself */
+static int ix_count(const ix_pass *ps, int p) {
+  const dist_index *ix = ps->ix;
+  int64_t lo = ix->rp[p], hi = ix->rp[p + 1];
+  const int64_t b = lo;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (ix_keep_sorted(ix, mid, ps->thr)) lo = mid + 1;
+    else hi = mid;
+  }
+  return (int)(lo - b) + ps->self;
+}
+
+/* x in N(q)?  A binary search of the row in column order. */
+static int ix_member(const ix_pass *ps, int q, int x) {
+  const dist_index *ix = ps->ix;
+  if (x == q && ps->self) return 1;
+  int64_t lo = ix->rp[q], hi = ix->rp[q + 1];
+  const int64_t e = hi;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (ix->col[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo == e || ix->col[lo] != x) return 0;
+  return ix->dist ? (float)ix->dist[lo] <= ps->thr : ix->val[lo] <= ps->thr;
+}
+
+/* ---- triggers: the points outside a large cluster, seen from their side ----
+ * With symmetric lists, x is in N(q) exactly when q is in N(x).  Points still
+ * outside the cluster whose own N(x) is short are registered once under each
+ * of their neighbours; a core point q then finds them in its trigger row
+ * instead of a binary search of N(q) per outside point.  Outside points with
+ * long N(x) keep the binary search. */
+typedef struct {
+  int *px;     /* [n] stored prefix length of each outside point */
+  int64_t *rp; /* [n + 1] trigger rows */
+  int *col;    /* outside points, ascending within each row */
+  int64_t cap;
+} triggers;
+
+/* Registers the outside points rest[0..n_rest) with short lists and returns
+ * how many (those with long lists, still ascending) remain in rest[]. */
+static int build_triggers(const ix_pass *ps, int n, int *rest, int n_rest, triggers *tg) {
+  const dist_index *ix = ps->ix;
+  for (int t = 0; t < n_rest; t++) tg->px[t] = ix_count(ps, rest[t]) - ps->self;
+  /* the longest list registered: as long as the triggers fit in cap */
+  int K = 4096;
+  for (;;) {
+    int64_t sum = 0;
+    for (int t = 0; t < n_rest; t++)
+      if (tg->px[t] <= K) sum += tg->px[t];
+    if (sum <= tg->cap || K == 0) break;
+    K >>= 1;
+  }
+  memset(tg->rp, 0, sizeof(int64_t) * ((size_t)n + 1));
+  for (int t = 0; t < n_rest; t++) {
+    if (tg->px[t] > K) continue;
+    const int x = rest[t];
+    const int32_t *d = ix->dcol + ix->rp[x];
+    for (int u = 0; u < tg->px[t]; u++)
+      if (d[u] != x) tg->rp[d[u] + 1]++;
+  }
+  for (int y = 0; y < n; y++) tg->rp[y + 1] += tg->rp[y];
+  for (int t = 0; t < n_rest; t++) {
+    if (tg->px[t] > K) continue;
+    const int x = rest[t];
+    const int32_t *d = ix->dcol + ix->rp[x];
+    for (int u = 0; u < tg->px[t]; u++)
+      if (d[u] != x) tg->col[tg->rp[d[u]]++] = x;
+  }
+  for (int y = n; y > 0; y--) tg->rp[y] = tg->rp[y - 1];
+  tg->rp[0] = 0;
+  int k = 0;
+  for (int t = 0; t < n_rest; t++)
+    if (tg->px[t] > K) rest[k++] = rest[t];
+  return k;
+}
+
+/* a[0..k) ascending (values >= 0); tmp holds k ints */
+static void sort_ascending(int *a, int k, int *tmp) {
+  if (k <= 48) {
+    for (int r = 1; r < k; r++) {
+      const int v = a[r];
+      int s = r - 1;
+      while (s >= 0 && a[s] > v) {
+        a[s + 1] = a[s];
+        s--;
+      }
+      a[s + 1] = v;
+    }
+    return;
+  }
+  int mx = 0;
+  for (int r = 0; r < k; r++)
+    if (a[r] > mx) mx = a[r];
+  int *src = a, *dst = tmp;
+  for (int sh = 0; sh < 31 && (mx >> sh) > 0; sh += 11) {
+    int cnt[2048];
+    memset(cnt, 0, sizeof(cnt));
+    for (int r = 0; r < k; r++) cnt[(src[r] >> sh) & 2047]++;
+    int sum = 0;
+    for (int d = 0; d < 2048; d++) {
+      const int c = cnt[d];
+      cnt[d] = sum;
+      sum += c;
+    }
+    for (int r = 0; r < k; r++) dst[cnt[(src[r] >> sh) & 2047]++] = src[r];
+    int *t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != a) memcpy(a, src, sizeof(int) * (size_t)k);
+}
+
+static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, void *src, const ix_pass *ix) {
   split_set ret;
   ret.n_clusters = 0;
   ret.clusters = (cluster *)malloc(sizeof(cluster) * (size_t)(n ? n : 1));
@@ -329,13 +707,29 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   int *nb = (int *)malloc(sizeof(int) * (size_t)n);
   int *queue = (int *)malloc(sizeof(int) * (size_t)n);
   int *rest = (int *)malloc(sizeof(int) * (size_t)n); /* points outside the union, ascending */
+  int *tmp = ix ? (int *)malloc(sizeof(int) * (size_t)n) : NULL;
+  triggers tg = {NULL, NULL, NULL, 0};
+  const int use_triggers = ix && ix->ix->symmetric;
+  if (use_triggers) {
+    tg.cap = (int64_t)n * 32 + 4096;
+    tg.px = (int *)malloc(sizeof(int) * (size_t)n);
+    tg.rp = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    tg.col = (int *)malloc(sizeof(int) * (size_t)tg.cap);
+  }
 
   for (int i = 0; i < n; i++) {
     if (visited[i]) continue;
     visited[i] = 1;
-    const int *nbi;
-    int cnt = rq(src, n, i, epsilon, nb, &nbi);
+    const int *nbi = nb;
+    int cnt = ix ? ix_count(ix, i) : rq(src, n, i, epsilon, nb, &nbi);
     if (!(minpts < cnt)) continue;
+    if (ix) {
+      /* N(i) in ascending order: the kept prefix, i itself, sorted */
+      const int m = cnt - ix->self;
+      memcpy(nb, ix->ix->dcol + ix->ix->rp[i], sizeof(int) * (size_t)m);
+      if (ix->self) nb[m] = i;
+      sort_ascending(nb, cnt, tmp);
+    }
     /* expand_cluster */
     const int id = ret.n_clusters;
     const int mark = id + 1;
@@ -356,8 +750,8 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
       const int q = queue[qi];
       if (!visited[q]) {
         visited[q] = 1;
-        const int *nbq;
-        int cq = rq(src, n, q, epsilon, nb, &nbq);
+        const int *nbq = nb;
+        int cq = ix ? ix_count(ix, q) : rq(src, n, q, epsilon, nb, &nbq);
         if (cq >= minpts) {
           /* the members of N(q) outside the union, in ascending order: by
            * scanning N(q), or -- when N(q) is much longer than the set of
@@ -369,20 +763,54 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
               n_rest = 0;
               for (int x = 0; x < n; x++)
                 if (stamp[x] != mark) rest[n_rest++] = x;
+              if (use_triggers) n_rest = build_triggers(ix, n, rest, n_rest, &tg);
             } else {
               int k = 0;
               for (int t = 0; t < n_rest; t++)
                 if (stamp[rest[t]] != mark) rest[k++] = rest[t];
               n_rest = k;
             }
-            for (int t = 0; t < n_rest; t++) {
-              const int x = rest[t];
-              if (in_row(nbq, cq, x)) {
+            if (use_triggers) {
+              /* outside points with long lists by binary search, those with
+               * short lists from q's trigger row; both ascending, disjoint */
+              int kl = 0, ks = 0;
+              for (int t = 0; t < n_rest; t++)
+                if (ix_member(ix, q, rest[t])) nb[kl++] = rest[t];
+              for (int64_t u = tg.rp[q]; u < tg.rp[q + 1]; u++)
+                if (stamp[tg.col[u]] != mark) tmp[ks++] = tg.col[u];
+              int a = 0, c = 0;
+              while (a < kl || c < ks) {
+                const int x = c == ks || (a < kl && nb[a] < tmp[c]) ? nb[a++] : tmp[c++];
                 stamp[x] = mark;
                 queue[qn++] = x;
-                in_union++;
+              }
+              in_union += kl + ks;
+            } else {
+              for (int t = 0; t < n_rest; t++) {
+                const int x = rest[t];
+                if (ix ? ix_member(ix, q, x) : in_row(nbq, cq, x)) {
+                  stamp[x] = mark;
+                  queue[qn++] = x;
+                  in_union++;
+                }
               }
             }
+          } else if (ix) {
+            /* the kept prefix outside the union, sorted; q itself is in it */
+            const int32_t *d = ix->ix->dcol + ix->ix->rp[q];
+            const int m = cq - ix->self;
+            int k = 0;
+            for (int t = 0; t < m; t++) {
+              const int x = d[t];
+              if (stamp[x] != mark) {
+                stamp[x] = mark;
+                nb[k++] = x;
+              }
+            }
+            sort_ascending(nb, k, tmp);
+            memcpy(queue + qn, nb, sizeof(int) * (size_t)k);
+            qn += k;
+            in_union += k;
           } else {
             for (int t = 0; t < cq; t++) {
               const int x = nbq[t];
@@ -409,6 +837,10 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   free(nb);
   free(queue);
   free(rest);
+  free(tmp);
+  free(tg.px);
+  free(tg.rp);
+  free(tg.col);
   return ret;
 }
 
@@ -440,11 +872,17 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
   split_set s;
   if (n > 0 && !g.cache->per_query_launch) {
     const mnhn_csr *c = &g.cache->csr;
-    pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, c->dist, NULL, epsilon, (float)0 <= epsilon);
-    s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
-    free_pass_lists(&pl);
+    const dist_index *ix = cache_index(g.cache, n, c->row_ptr, c->col, c->dist, NULL);
+    if (ix) {
+      const ix_pass ps = {ix, epsilon, (float)0 <= epsilon};
+      s = dbscan_core(n, epsilon, minpts, NULL, NULL, &ps);
+    } else {
+      pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, c->dist, NULL, epsilon, (float)0 <= epsilon);
+      s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
+      free_pass_lists(&pl);
+    }
   } else {
-    s = dbscan_core(n, epsilon, minpts, rq_gpu, &g);
+    s = dbscan_core(n, epsilon, minpts, rq_gpu, &g, NULL);
   }
   if (timing_on())
     fprintf(stderr, "[mnhn] dbscan pass eps %g: %d clusters, %.3f s\n", epsilon, s.n_clusters, now_s() - t0);
@@ -453,9 +891,9 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
 
 split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts) {
   const int n = csr ? csr->n_rows : 0;
-  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL);
+  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL, NULL);
   pass_lists pl = build_pass_lists(n, csr->row_ptr, csr->col, csr->dist, NULL, epsilon, (float)0 <= epsilon);
-  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
   free_pass_lists(&pl);
   return s;
 }
@@ -472,6 +910,7 @@ static void ensure_vec_lists(mnhn_gpu_stuff g, float eps) {
   mnhn_eps_cache *c = g.cache;
   const float thr = vec_thr(g.norm, eps);
   if (c->have_v && thr <= c->vcsr.thr) return;
+  drop_index(c);
   if (c->have_v) mnhn_vcsr_free(&c->vcsr);
   c->have_v = 0;
   float t_max = vec_thr(g.norm, eps + c->lookahead);
@@ -489,12 +928,17 @@ static split_set dbscan_vec(dataset ds, float epsilon, int minpts, mnhn_gpu_stuf
   }
   apply_defaults(&epsilon, &minpts);
   const int n = ds.n_values;
-  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL);
+  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL, NULL);
   ensure_vec_lists(g, epsilon);
   /* the stored rows list p itself when its own accumulator qualifies */
   const mnhn_vcsr *c = &g.cache->vcsr;
+  const dist_index *ix = cache_index(g.cache, n, c->row_ptr, c->col, NULL, c->val);
+  if (ix) {
+    const ix_pass ps = {ix, vec_thr(norm, epsilon), 0};
+    return dbscan_core(n, epsilon, minpts, NULL, NULL, &ps);
+  }
   pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, NULL, c->val, vec_thr(norm, epsilon), 0);
-  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
   free_pass_lists(&pl);
   return s;
 }
@@ -523,11 +967,55 @@ split_set dbscan_L2(dataset ds, float epsilon, int minpts) {
 
 split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts) {
   const int n = csr ? csr->n_rows : 0;
-  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL);
+  if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL, NULL);
   pass_lists pl = build_pass_lists(n, csr->row_ptr, csr->col, NULL, csr->val, vec_thr(csr->norm, epsilon), 0);
-  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
   free_pass_lists(&pl);
   return s;
+}
+
+/* Several passes over the same lists: through the distance-ordered index when
+ * it fits, else one materialised pass each. */
+static int dbscan_sweep(int n, const int64_t *rp, const int32_t *col, const int16_t *dist, const float *val,
+                        int norm, const float *eps, int n_eps, int minpts, int symmetric, split_set *out) {
+  if (n_eps < 0 || (n_eps > 0 && (!eps || !out))) return MNHN_ERR_ARG;
+  dist_index ix;
+  double t0 = now_s();
+  const int have = n > 0 && n_eps > 0 && index_allowed() && build_dist_index(&ix, n, rp, col, dist, val, symmetric);
+  if (timing_on() && have) fprintf(stderr, "[mnhn] epsilon lists ordered by distance: %.3f s\n", now_s() - t0);
+  for (int k = 0; k < n_eps; k++) {
+    t0 = now_s();
+    const float thr = dist ? eps[k] : vec_thr(norm, eps[k]);
+    const int self = dist ? (float)0 <= eps[k] : 0;
+    if (n == 0) {
+      out[k] = dbscan_core(0, eps[k], minpts, rq_pass, NULL, NULL);
+    } else if (have) {
+      const ix_pass ps = {&ix, thr, self};
+      out[k] = dbscan_core(n, eps[k], minpts, NULL, NULL, &ps);
+    } else {
+      pass_lists pl = build_pass_lists(n, rp, col, dist, val, thr, self);
+      out[k] = dbscan_core(n, eps[k], minpts, rq_pass, &pl, NULL);
+      free_pass_lists(&pl);
+    }
+    if (timing_on())
+      fprintf(stderr, "[mnhn] dbscan pass eps %g: %d clusters, %.3f s\n", eps[k], out[k].n_clusters, now_s() - t0);
+  }
+  if (have) free_dist_index(&ix);
+  return MNHN_OK;
+}
+
+int mnhn_dbscan_csr_sweep(const mnhn_csr *csr, const float *epsilon, int n_eps, int minpts, int symmetric,
+                          split_set *out) {
+  if (!csr) return MNHN_ERR_ARG;
+  return dbscan_sweep(csr->n_rows, csr->row_ptr, csr->col, csr->dist, NULL, 0, epsilon, n_eps, minpts, symmetric,
+                      out);
+}
+
+int mnhn_dbscan_vcsr_sweep(const mnhn_vcsr *csr, const float *epsilon, int n_eps, int minpts, int symmetric,
+                           split_set *out) {
+  if (!csr) return MNHN_ERR_ARG;
+  return dbscan_sweep(csr->n_rows, csr->row_ptr, csr->col, NULL, csr->val, csr->norm, epsilon, n_eps, minpts,
+                      symmetric, out);
 }
 
 void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff),
@@ -541,6 +1029,7 @@ void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff)
   /* neighbour lists are computed up to 8 increments beyond the epsilon asked
    * for, so the sweep's next passes reuse them */
   mnhn_gpu_set_lookahead(g, 8.0f * (epsilon_inc > 0 ? epsilon_inc : 0));
+  g.cache->sweep = 1;
   if (timing_on()) fprintf(stderr, "[mnhn] device context: %.3f s\n", now_s() - t_init);
 
   printf("Performing adaptive clustering with parameters: \n"

@@ -400,3 +400,122 @@ def test_dbscan_dense_neighbourhoods_match_oracle(pkg, oracle):
         got = pkg.dbscan_csr(rp, col, dist, 5000, eps, minpts)
         want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(d))
         assert got == [c.members for c in want.clusters], (eps, minpts)
+
+
+def _lists_from_matrix(d, bound):
+    keep = (d <= bound) & ~np.eye(len(d), dtype=bool)
+    rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+    return rp, np.nonzero(keep)[1].astype(np.int32), d[keep].astype(np.int16)
+
+
+def test_dbscan_sweep_matches_oracle(pkg, oracle, monkeypatch):
+    """mnhn_dbscan_csr_sweep: passes over lists ordered by distance once give
+    the oracle's clusters, member order included, for every epsilon; rows
+    cover the three orderings (short rows, integer ranges short enough for
+    one counting pass, wide ranges by radix), large clusters that find the
+    points outside them through those points' short lists (symmetric lists)
+    or by binary search (not declared symmetric), and lists big enough for
+    the threaded build."""
+    seqs = oracle.read_fasta(os.path.join(GOLDEN, "inputs", "c2_small.fasta"))[:160]
+    dist = oracle.sw_matrix(seqs)
+    rp, col, d = _lists_from_matrix(dist, 500)
+    eps = [100, 250.5, 400, 500, 0, -2.0, 330]
+    for minpts in (3, 0):
+        got = pkg.dbscan_csr_sweep(rp, col, d, 500, eps, minpts)
+        for e, g in zip(eps, got):
+            want = oracle.dbscan(len(seqs), e, minpts, oracle.region_query_from_matrix(dist))
+            assert g == [c.members for c in want.clusters], (e, minpts)
+
+    rng = np.random.default_rng(31)
+    n = 1500
+    centre = rng.integers(0, 8, n)
+    m = np.abs(centre[:, None] - centre[None, :]) * 100 + rng.integers(0, 300, (n, n))
+    wide = rng.random(n) < 0.3  # rows whose far entries spread over 0..30000
+    m = np.where(wide[:, None] | wide[None, :], m * 6 + rng.integers(0, 9000, (n, n)), m)
+    m = np.triu(m, 1)
+    m = (m + m.T).astype(np.int64)
+    rp, col, d = _lists_from_matrix(m, 30000)
+    eps = [900, 600, 400, 1500, 3000, 20000]
+    for minpts in (5, 50, 2):
+        got = pkg.dbscan_csr_sweep(rp, col, d, 30000, eps, minpts)
+        for e, g in zip(eps, got):
+            want = oracle.dbscan(n, e, minpts, oracle.region_query_from_matrix(m))
+            assert g == [c.members for c in want.clusters], (e, minpts)
+        # without the symmetric declaration (no triggers): the same clusters
+        assert pkg.dbscan_csr_sweep(rp, col, d, 30000, eps, minpts, symmetric=False) == got
+
+    monkeypatch.setenv("MNHN_HOST_THREADS", "7")
+    n = 3000
+    centre = rng.integers(0, 60, n)
+    m = np.abs(centre[:, None] - centre[None, :]) * 40 + rng.integers(0, 120, (n, n))
+    m = np.triu(m, 1)
+    m = (m + m.T).astype(np.int64)
+    rp, col, d = _lists_from_matrix(m, 2000)
+    assert len(col) > (1 << 20)
+    eps = [60, 150.5, 1000, 2000]
+    got = pkg.dbscan_csr_sweep(rp, col, d, 2000, eps, 4)
+    for e, g in zip(eps, got):
+        want = oracle.dbscan(n, e, 4, oracle.region_query_from_matrix(m))
+        assert g == [c.members for c in want.clusters], e
+    # the same passes without the index (MNHN_SWEEP_INDEX=0) agree
+    monkeypatch.setenv("MNHN_SWEEP_INDEX", "0")
+    assert pkg.dbscan_csr_sweep(rp, col, d, 2000, eps, 4) == got
+
+
+def test_dbscan_vcsr_sweep_matches_oracle(pkg, oracle):
+    """mnhn_dbscan_vcsr_sweep over vector lists (self stored, float keys,
+    infinities and exact zeros among them) gives the oracle's clusters for
+    every epsilon, L1 and L2."""
+    values = oracle.read_kmers(os.path.join(GOLDEN, "inputs", "c1_small_k3.kmer"))
+    n = values.shape[1]
+    for norm, e_max, eps in ((1, 70.0, [40, 55.5, 70, -2.0, 0.0]), (2, 12.0, [8, 10.25, 12, -9.0, 0.0])):
+        acc = oracle.vec_acc(values, norm)
+        t_max = oracle.vec_threshold(norm, e_max)
+        keep = acc <= t_max
+        rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+        col = np.nonzero(keep)[1].astype(np.int32)
+        for minpts in (5, 0):
+            got = pkg.dbscan_vcsr_sweep(rp, col, acc[keep].astype(np.float32), norm, t_max, eps, minpts)
+            for e, g in zip(eps, got):
+                want = oracle.dbscan(n, e, minpts, oracle.region_query_from_acc(acc, norm))
+                assert g == [c.members for c in want.clusters], (norm, e, minpts)
+    rng = np.random.default_rng(3)
+    pts = rng.integers(0, 4, size=(300, 6)).astype(np.float32)
+    pts[7, 2] = np.inf
+    pts[50:60] = pts[49]
+    acc = oracle.vec_acc(pts.T, 1)
+    keep = acc <= np.float32(np.inf)
+    rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+    col = np.nonzero(keep)[1].astype(np.int32)
+    eps = [0.0, 2.0, 5.0, float(np.inf)]
+    got = pkg.dbscan_vcsr_sweep(rp, col, acc[keep].astype(np.float32), 1, float(np.inf), eps, 3)
+    for e, g in zip(eps, got):
+        want = oracle.dbscan(300, e, 3, oracle.region_query_from_acc(acc, 1))
+        assert g == [c.members for c in want.clusters], e
+
+
+def test_dbscan_sweep_long_lists_outside_a_cluster(pkg, oracle):
+    """Points on a line with a sparse tail: when the cluster has almost
+    everything, the points still outside have long lists (binary search) or
+    short ones (found from their own lists), both merged in ascending order;
+    clusters equal the oracle's."""
+    pos = np.concatenate([np.arange(4000), 3999 + 50 * np.arange(1, 31)]).astype(np.int64)
+    n = len(pos)
+    eps_max = 1800
+    rows = [np.nonzero(np.abs(pos - pos[p]) <= eps_max)[0] for p in range(n)]
+    rows = [r[r != p] for p, r in enumerate(rows)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    col = np.concatenate(rows).astype(np.int32)
+    d = np.abs(pos[np.repeat(np.arange(n), np.diff(rp))] - pos[col]).astype(np.int16)
+
+    def rq(p, e):
+        lo = np.searchsorted(pos, pos[p] - np.float32(e), side="left")
+        hi = np.searchsorted(pos, pos[p] + np.float32(e), side="right")
+        return list(range(lo, hi))
+
+    eps = [1800, 1200, 700]
+    got = pkg.dbscan_csr_sweep(rp, col, d, eps_max, eps, 4)
+    for e, g in zip(eps, got):
+        want = oracle.dbscan(n, e, 4, rq)
+        assert g == [c.members for c in want.clusters], e
+    assert pkg.dbscan_csr_sweep(rp, col, d, eps_max, eps, 4, symmetric=False) == got

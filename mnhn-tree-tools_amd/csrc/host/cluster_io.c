@@ -11,7 +11,9 @@
  *     tie) is sorted in place when it has more than 20 members -- a side effect
  *     that changes the member order later written by store_split_set.
  *   generate_split_set_relation (:407-445): new cluster i is connected to old
- *     cluster j when |old_j & new_i| > 0.8 |old_j|.
+ *     cluster j when |old_j & new_i| > 0.8 |old_j|, with the sort side effect
+ *     of every pair's intersection (computed in one labelled pass when each
+ *     split set holds every point at most once).
  */
 #include <fcntl.h>
 #include <stdint.h>
@@ -161,9 +163,89 @@ cluster intersection_of_clusters(cluster a, cluster b) {
   return r;
 }
 
+/* Largest member + 1 of s when every member is >= 0 and appears in only one
+ * cluster, once (as in every split set DBSCAN produces); -1 otherwise. */
+static int members_once(split_set s) {
+  int mx = -1;
+  for (int i = 0; i < s.n_clusters; i++)
+    for (int k = 0; k < s.clusters[i].n_members; k++) {
+      const int x = s.clusters[i].members[k];
+      if (x < 0) return -1;
+      mx = x > mx ? x : mx;
+    }
+  unsigned char *seen = (unsigned char *)calloc((size_t)mx + 1, 1);
+  if (!seen) return -1;
+  for (int i = 0; i < s.n_clusters; i++)
+    for (int k = 0; k < s.clusters[i].n_members; k++) {
+      const int x = s.clusters[i].members[k];
+      if (seen[x]) {
+        free(seen);
+        return -1;
+      }
+      seen[x] = 1;
+    }
+  free(seen);
+  return mx + 1;
+}
+
+/* The relation when each split set holds every point at most once: one pass
+ * over the old clusters against a label per point gives every |old_j & new_i|
+ * at once.  The in-place sorts of intersection_of_clusters are idempotent and
+ * leave the counts unchanged, so their end state is reproduced directly: a
+ * cluster of more than 20 members ends sorted when it is the larger side
+ * (the old one on a tie) of at least one pair. */
+static void relation_by_labels(split_set ancient, split_set new_set, int n_points, cluster_connections *c) {
+  int *label = (int *)malloc(sizeof(int) * (size_t)(n_points ? n_points : 1));
+  int *cnt = (int *)calloc((size_t)(new_set.n_clusters ? new_set.n_clusters : 1), sizeof(int));
+  int *touched = (int *)malloc(sizeof(int) * (size_t)(new_set.n_clusters ? new_set.n_clusters : 1));
+  for (int x = 0; x < n_points; x++) label[x] = -1;
+  for (int i = 0; i < new_set.n_clusters; i++)
+    for (int k = 0; k < new_set.clusters[i].n_members; k++) label[new_set.clusters[i].members[k]] = i;
+  for (int j = 0; j < ancient.n_clusters; j++) {
+    const cluster o = ancient.clusters[j];
+    int nt = 0;
+    for (int k = 0; k < o.n_members; k++) {
+      const int x = o.members[k];
+      const int i = x < n_points ? label[x] : -1;
+      if (i >= 0 && cnt[i]++ == 0) touched[nt++] = i;
+    }
+    for (int t = 0; t < nt; t++) {
+      const int i = touched[t];
+      if ((double)cnt[i] > (double)o.n_members * 0.8) c[i].connections[c[i].n_connections++] = j;
+      cnt[i] = 0;
+    }
+  }
+  free(label);
+  free(cnt);
+  free(touched);
+  if (ancient.n_clusters == 0 || new_set.n_clusters == 0) return;
+  int min_old = ancient.clusters[0].n_members, min_new = new_set.clusters[0].n_members;
+  for (int j = 1; j < ancient.n_clusters; j++)
+    min_old = ancient.clusters[j].n_members < min_old ? ancient.clusters[j].n_members : min_old;
+  for (int i = 1; i < new_set.n_clusters; i++)
+    min_new = new_set.clusters[i].n_members < min_new ? new_set.clusters[i].n_members : min_new;
+  for (int j = 0; j < ancient.n_clusters; j++) {
+    const cluster o = ancient.clusters[j];
+    if (o.n_members > 20 && o.n_members >= min_new) qsort(o.members, (size_t)o.n_members, sizeof(int), cmp_int);
+  }
+  for (int i = 0; i < new_set.n_clusters; i++) {
+    const cluster w = new_set.clusters[i];
+    if (w.n_members > 20 && w.n_members > min_old) qsort(w.members, (size_t)w.n_members, sizeof(int), cmp_int);
+  }
+}
+
 cluster_connections *generate_split_set_relation(split_set ancient, split_set new_set) {
   cluster_connections *c =
       (cluster_connections *)malloc(sizeof(cluster_connections) * (size_t)(new_set.n_clusters ? new_set.n_clusters : 1));
+  const int na = members_once(ancient), nn = na < 0 ? -1 : members_once(new_set);
+  if (na >= 0 && nn >= 0) {
+    for (int i = 0; i < new_set.n_clusters; i++) {
+      c[i].connections = (int *)malloc(sizeof(int) * (size_t)(ancient.n_clusters ? ancient.n_clusters : 1));
+      c[i].n_connections = 0;
+    }
+    relation_by_labels(ancient, new_set, nn, c);
+    return c;
+  }
   for (int i = 0; i < new_set.n_clusters; i++) {
     c[i].connections = (int *)malloc(sizeof(int) * (size_t)(ancient.n_clusters ? ancient.n_clusters : 1));
     int count = 0;

@@ -1082,6 +1082,7 @@ void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff)
   }
   const int stop = nsets - 1;
 
+  const double t_layers = now_s();
   split_set *kept = (split_set *)malloc(sizeof(split_set) * (size_t)nsets);
   cluster_connections **conn = (cluster_connections **)malloc(sizeof(*conn) * (size_t)nsets);
   unsigned char *is_kept = (unsigned char *)calloc((size_t)nsets, 1);
@@ -1103,7 +1104,10 @@ void adaptive_dbscan(split_set (*dbscanner)(dataset, float, int, mnhn_gpu_stuff)
       is_kept[i] = 1;
     }
   }
+  if (timing_on()) fprintf(stderr, "[mnhn] layers and relations: %.3f s\n", now_s() - t_layers);
+  const double t_free = now_s();
   mnhn_gpu_destroy(g);
+  if (timing_on()) fprintf(stderr, "[mnhn] device context and lists released: %.3f s\n", now_s() - t_free);
   const double t_out = now_s();
 
   printf("Connections found: \n");

@@ -188,6 +188,51 @@ def test_relation_and_sort_side_effect(host, oracle):
         assert [sn.clusters[k].members[i] for i in range(sn.clusters[k].n_members)] == N[k].members
 
 
+def test_relation_of_disjoint_split_sets(host, oracle):
+    """Split sets that hold every point at most once (DBSCAN output): the
+    connections and the member order left by the in-place sorts equal the
+    oracle's pairwise restatement, sizes on both sides of 20 and ties of
+    equal-sized old and new clusters included."""
+    lib, _ = host
+    rng = np.random.default_rng(7)
+    for trial in range(12):
+        n = int(rng.integers(200, 3000))
+        perm = rng.permutation(n)
+        sizes = rng.integers(1, 60, size=200)
+        old, at = [], 0
+        for k in sizes:
+            if at + k > n * 0.9:
+                break
+            old.append(perm[at:at + k].tolist())
+            at += int(k)
+        # new clusters: unions of consecutive old clusters, part of a cluster
+        # moved to the next one, some points dropped, one copy of an old
+        # cluster (equal sizes)
+        new, k = [], 0
+        while k < len(old):
+            step = int(rng.integers(1, 4))
+            m = sum(old[k:k + step], [])
+            if rng.random() < 0.3:
+                m = m[: int(len(m) * rng.uniform(0.5, 1.0))]
+            new.append(rng.permutation(m).tolist() if trial % 2 else m)
+            k += step
+        if trial % 3 == 0 and new:
+            new[0] = list(old[0])
+        new = [m for m in new if m]
+        O = [oracle.Cluster(list(m), i) for i, m in enumerate(old)]
+        N = [oracle.Cluster(list(m), i) for i, m in enumerate(new)]
+        want = oracle.split_set_relation(oracle.SplitSet(O), oracle.SplitSet(N))
+        so, ko = _mk_split([(m, i) for i, m in enumerate(old)])
+        sn, kn = _mk_split([(m, i) for i, m in enumerate(new)])
+        conn = lib.generate_split_set_relation(so, sn)
+        got = [[conn[i].connections[k] for k in range(conn[i].n_connections)] for i in range(len(new))]
+        assert got == want, trial
+        for k in range(len(old)):
+            assert [so.clusters[k].members[i] for i in range(so.clusters[k].n_members)] == O[k].members
+        for k in range(len(new)):
+            assert [sn.clusters[k].members[i] for i in range(sn.clusters[k].n_members)] == N[k].members
+
+
 def test_dbscan_over_csr_matches_oracle(pkg, oracle):
     """mnhn_dbscan_csr (host code only) over epsilon lists built on the CPU
     from the oracle's distance matrix gives the oracle's clusters, member

@@ -167,12 +167,16 @@ cluster intersection_of_clusters(cluster a, cluster b) {
  * cluster, once (as in every split set DBSCAN produces); -1 otherwise. */
 static int members_once(split_set s) {
   int mx = -1;
+  int64_t total = 0;
   for (int i = 0; i < s.n_clusters; i++)
     for (int k = 0; k < s.clusters[i].n_members; k++) {
       const int x = s.clusters[i].members[k];
       if (x < 0) return -1;
       mx = x > mx ? x : mx;
+      total++;
     }
+  /* members are point indices; sparse ids beyond that take the pairwise path */
+  if ((int64_t)mx > 16 * total + (1 << 20)) return -1;
   unsigned char *seen = (unsigned char *)calloc((size_t)mx + 1, 1);
   if (!seen) return -1;
   for (int i = 0; i < s.n_clusters; i++)

@@ -524,6 +524,12 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
 // per two cells instead of 5.  (clamp01 is the local-alignment floor; its upper
 // bound 1.0 = 2048 is never exceeded.)
 // ---------------------------------------------------------------------------
+// compile-time strip width for the strip lambda of the fp16 LDS kernel
+template <int W>
+struct Width {
+  static constexpr int value = W;
+};
+
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 constexpr float kF16Scale = 1.0f / 2048.0f;
 
@@ -607,11 +613,13 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
       const int nstrips = (ncols + S - 1) / S;
       f16x2 best = zero2;
 
-      for (int s = 0; s < nstrips && nrows > 0; ++s) {
-        const int j0 = ncols - (nstrips - s) * S + 1;
+      // One strip of W query columns whose slot 0 is column j0 (columns < 1
+      // are pre-padding); first: no strip to its left.
+      auto strip = [&](auto wc, int j0, bool first) {
+        constexpr int W = decltype(wc)::value;
         __builtin_amdgcn_wave_barrier();
-        for (int e = lane; e < kPairs * S; e += kWave) {
-          const int p = e / S, jj = e - p * S;
+        for (int e = lane; e < kPairs * W; e += kWave) {
+          const int p = e / W, jj = e - p * W;
           const int k = j0 + jj;
           uint32_t v = spre | (spre << 16);
           if (k >= 1) {
@@ -625,9 +633,9 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
 
-        f16x2 G[S];
+        f16x2 G[W];
 #pragma unroll
-        for (int jj = 0; jj < S; ++jj) G[jj] = g0;
+        for (int jj = 0; jj < W; ++jj) G[jj] = g0;
         f16x2 gdiag = g0;
         uint32_t w_lo = a.words[wo_lo], w_hi = a.words[wo_hi];
         auto do_row = [&](int j, uint32_t left_u, bool pad) {
@@ -639,9 +647,9 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
             if (j > rows_hi) c_hi = 4u;
           }
           const uint4 *prow = reinterpret_cast<const uint4 *>(tab + (c_lo * 5u + c_hi) * RS);
-          f16x2 P[S];
+          f16x2 P[W];
 #pragma unroll
-          for (int w = 0; w < S / 4; ++w) {
+          for (int w = 0; w < W / 4; ++w) {
             const uint4 v = prow[w];
             P[4 * w + 0] = as_f16x2(v.x);
             P[4 * w + 1] = as_f16x2(v.y);
@@ -649,12 +657,12 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
             P[4 * w + 3] = as_f16x2(v.w);
           }
           const f16x2 gleft = as_f16x2(left_u);
-          dp_row_f16<S>(G, P, gleft, gdiag, gapv, best);
-          bnd[(uint32_t)j * kWave + (uint32_t)lane] = f16_u32(G[S - 1]);
+          dp_row_f16<W>(G, P, gleft, gdiag, gapv, best);
+          bnd[(uint32_t)j * kWave + (uint32_t)lane] = f16_u32(G[W - 1]);
           gdiag = gleft;
         };
         uint32_t bA = f16_u32(g0), bB = f16_u32(g0);
-        if (s > 0) {
+        if (!first) {
           bA = bnd[kWave + (uint32_t)lane];
           if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
         }
@@ -665,14 +673,28 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
           int j = blk * 16 + 1;
           for (; j + 1 <= jend; j += 2) {
             do_row(j, bA, j > nfast);
-            if (s > 0 && j + 2 <= nrows) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
+            if (!first && j + 2 <= nrows) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
             do_row(j + 1, bB, j + 1 > nfast);
-            if (s > 0 && j + 3 <= nrows) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
+            if (!first && j + 3 <= nrows) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
           }
           if (j <= jend) do_row(j, bA, true);
           w_lo = nw_lo;
           w_hi = nw_hi;
         }
+      };
+      if (nstrips > 0 && nrows > 0) {
+        // strips are right-aligned: the first holds the rem = 1..S leftmost
+        // columns; when rem <= S/2 it runs S/2 wide, halving its padding
+        const int rem = ncols - (nstrips - 1) * S;
+        bool narrow = false;
+        if constexpr (S >= 32) {
+          if (rem <= S / 2) {
+            strip(Width<S / 2>{}, rem - S / 2 + 1, true);
+            narrow = true;
+          }
+        }
+        if (!narrow) strip(Width<S>{}, rem - S + 1, true);
+        for (int s = 1; s < nstrips; ++s) strip(Width<S>{}, rem + (s - 1) * S + 1, false);
       }
       // back to integers: H = 2048 * value; corner H = G + gap at row lb-1
       const float sc = 2048.0f;

@@ -684,16 +684,17 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
       };
       if (nstrips > 0 && nrows > 0) {
         // strips are right-aligned: the first holds the rem = 1..S leftmost
-        // columns; when rem <= S/2 it runs S/2 wide, halving its padding
+        // columns and runs in the narrowest of S/4, S/2, 3S/4 and S that
+        // holds them (multiples of 8 columns), so it pads fewer than S/4
         const int rem = ncols - (nstrips - 1) * S;
-        bool narrow = false;
         if constexpr (S >= 32) {
-          if (rem <= S / 2) {
-            strip(Width<S / 2>{}, rem - S / 2 + 1, true);
-            narrow = true;
-          }
+          if (rem <= S / 4) strip(Width<S / 4>{}, rem - S / 4 + 1, true);
+          else if (rem <= S / 2) strip(Width<S / 2>{}, rem - S / 2 + 1, true);
+          else if (rem <= 3 * S / 4) strip(Width<3 * S / 4>{}, rem - 3 * S / 4 + 1, true);
+          else strip(Width<S>{}, rem - S + 1, true);
+        } else {
+          strip(Width<S>{}, rem - S + 1, true);
         }
-        if (!narrow) strip(Width<S>{}, rem - S + 1, true);
         for (int s = 1; s < nstrips; ++s) strip(Width<S>{}, rem + (s - 1) * S + 1, false);
       }
       // back to integers: H = 2048 * value; corner H = G + gap at row lb-1
@@ -2056,31 +2057,35 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
   for (int w : {4, 8}) cands.push_back({KERNEL_REG, 32, w});
   for (int w : {8, 16}) cands.push_back({KERNEL_REG, 16, w});
   // sample of the all-pairs work: single-query items (one query against one
-  // target group) strided over all (group, query) pairs, about 32 per
-  // resident wave at the largest candidate occupancy (so that the queue's tail
-  // weighs little) -- a few hundred milliseconds of work per candidate
-  // whatever the data-set size
+  // target group), about 32 per resident wave at the largest candidate
+  // occupancy (so that the queue's tail weighs little) -- a few hundred
+  // milliseconds of work per candidate whatever the data-set size.  They are
+  // strided over the cells of all (group, query) pairs, so the sample has the
+  // length mix of the work it stands for.
   std::vector<double> gsum(c->groups, 0.0);
-  double units = 0;
+  double total_cells = 0;
   for (int g = 0; g < c->groups; ++g) {
     for (int t = 0; t < kGroup; ++t) {
       const int id = c->gtgt[(size_t)g * kGroup + t];
       if (id >= 0) gsum[g] += c->len[id];
     }
-    units += std::max(std::min((g + 1) * kGroup, c->n) - 1, 0);
+    const int last = std::min((g + 1) * kGroup, c->n) - 1;
+    for (int p = 0; p < last; ++p) total_cells += (double)c->len[c->order[p]] * gsum[g];
   }
   const double want = (double)std::max(c->num_cu, 1) * 24 * 32;
-  const double stride = std::max(1.0, units / want);
+  const double stride = total_cells / want;
   Work sample;
   sample.qids = c->order;
   double next = 0, u = 0;
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
-    for (int p = 0; p < last; ++p, u += 1) {
-      if (u < next) continue;
-      next += stride;
+    for (int p = 0; p < last; ++p) {
+      const double cells = (double)c->len[c->order[p]] * gsum[g];
+      u += cells;
+      if (u <= next || cells <= 0) continue;
+      while (next < u) next += stride;  // once per pair, however many strides it spans
       sample.items.push_back(make_int4(g, p, p + 1, 0));
-      sample.cells += (double)c->len[c->order[p]] * gsum[g];
+      sample.cells += cells;
     }
   }
   finalize_items(c, sample, 32, false);
@@ -2097,6 +2102,10 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
     }
     if (rc) break;
     const double g = sample.cells / std::max(ms, 1e-6) / 1e6;
+    const char *tv = getenv("MNHN_TIMING");
+    if (tv && *tv && strcmp(tv, "0") != 0)
+      fprintf(stderr, "[mnhn] autotune kernel %d strip %d waves/CU %d: %.0f GCUPS on %.3g sample cells\n",
+              cd.kernel, cd.strip, cd.wpc, g, sample.cells);
     if (g > best_gcups) {
       best_gcups = g;
       best = cd;

@@ -971,7 +971,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     const size_t n_items = w.dev ? (size_t)w.n_items : items.size();
     void *kp = kernel_ptr(PK, PS, c->acgt);
     int wpc = wpc_req;
-    // measured on MI355X (config-2 workload): the LDS kernels at S = 32 run
+    // measured on MI355X (20,000 sequences of 100-600 residues): the LDS kernels at S = 32 run
     // best with 3 of their 4 possible waves per SIMD resident
     if (wpc <= 0 && (PK == KERNEL_LDS || PK == KERNEL_LDS_F16) && PS == 32) wpc = 12;
     if (wpc <= 0) {
@@ -1069,15 +1069,25 @@ double item_cost(const mnhn_sw_ctx *c, const Work &w, const int4 &it, int S) {
   return acc;
 }
 
+// Items longest first, so the queue's last items are its smallest: a
+// counting sort on 1/16-octave cost classes (stable within a class), linear
+// in the item count (a comparison sort of the ~10^6 items of a large run took
+// ~0.1 s of host time per call).
 void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
-  std::vector<double> cost(w.items.size());
-  for (size_t k = 0; k < w.items.size(); ++k) cost[k] = item_cost(c, w, w.items[k], S);
-  std::vector<size_t> idx(w.items.size());
-  std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-  std::vector<int4> sorted;
-  sorted.reserve(idx.size());
-  for (size_t k : idx) sorted.push_back(w.items[k]);
+  constexpr int kClasses = 16 * 64;
+  const size_t m = w.items.size();
+  std::vector<uint16_t> cls(m);
+  std::vector<size_t> start(kClasses + 1, 0);
+  for (size_t k = 0; k < m; ++k) {
+    const double cost = item_cost(c, w, w.items[k], S);
+    int q = cost > 1.0 ? (int)(16.0 * std::log2(cost)) : 0;
+    q = std::min(std::max(q, 0), kClasses - 1);
+    cls[k] = (uint16_t)(kClasses - 1 - q);  // larger cost, smaller class index
+    ++start[cls[k] + 1];
+  }
+  for (int q = 0; q < kClasses; ++q) start[q + 1] += start[q];
+  std::vector<int4> sorted(m);
+  for (size_t k = 0; k < m; ++k) sorted[start[cls[k]]++] = w.items[k];
   w.items.swap(sorted);
   if (dense_out) {
     int off = 0;
@@ -1121,8 +1131,13 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
     const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
     for (int p = first[g]; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
   }
-  // ~256 items per CU: enough for the longest-first queue to balance the tail
-  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 256.0), 2.0e5);
+  // ~4096 items per CU (MNHN_SW_ITEMS_PER_CU): on 20,000 sequences of 100-600
+  // residues, 256 items per CU left the longest-first queue's tail at ~5 % of
+  // the run (17,081 GCUPS at strip 32) and 4096 runs at 17,876; items stay one
+  // or more whole queries
+  const char *ipc_env = getenv("MNHN_SW_ITEMS_PER_CU");
+  const double ipc = ipc_env && atof(ipc_env) > 0 ? atof(ipc_env) : 4096.0;
+  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * ipc), 2.0e5);
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     if (last <= first[g]) continue;

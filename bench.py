@@ -24,12 +24,12 @@ Also reported:
                   same 20,000 sequences (FASTA in, split-set files out).
 
 Other configurations:
-  --config c3  105,000 amplicon-like sequences, same measurement;
-  --config c4  learned proposer: train on 2M exact-labelled pairs, calibrate on
+  --config amplicon105k  105,000 amplicon-like sequences, same measurement;
+  --config proposer100k  learned proposer: train on 2M exact-labelled pairs, calibrate on
                held-out sequences, then recall / precision / time of the
                prefilter lists against the exact lists on 100,000 held-out
                sequences (and DBSCAN on both);
-  --config c5  end-to-end DBSCAN on 500,000 heavy-tailed sequences: exact
+  --config mix500k  end-to-end DBSCAN on 500,000 heavy-tailed sequences: exact
                lists + DBSCAN against prefilter lists + DBSCAN;
   --config consumers  wall time of the drop-in compareSW, silhouette and
                find_closest_sequence_SW CLIs on two 10,000-sequence sets;
@@ -40,7 +40,7 @@ Other configurations:
                CLIs against the compiled reference on bounded samples (outputs
                compared byte for byte there).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4|c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config tree20k|amplicon105k|proposer100k|mix500k]
 """
 import argparse
 import ctypes
@@ -62,10 +62,10 @@ sys.path.insert(0, PKG_DIR)
 import synth  # noqa: E402
 
 WORKLOADS = {
-    "c2": "uneven-length all-pairs SW: 20,000 seqs, 3-level tree, lognormal lengths "
-          "median 300 in [100,600]",
-    "c3": "amplicon-scale all-pairs SW: 105,000 seqs (500 ancestors x 200 at 2-10% "
-          "mutation + 5% noise), lengths uniform in [150,300]",
+    "tree20k": "uneven-length all-pairs SW: 20,000 seqs, 3-level tree, lognormal lengths "
+               "median 300 in [100,600]",
+    "amplicon105k": "amplicon-scale all-pairs SW: 105,000 seqs (500 ancestors x 200 at 2-10% "
+                    "mutation + 5% noise), lengths uniform in [150,300]",
 }
 
 # MI355X: 256 CUs x 64 lanes x 2 (packed int16) x 2.4 GHz
@@ -130,7 +130,7 @@ def profiled_traffic(config):
     """HBM bytes per step of the SW kernels, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
     plain copy kernel); None if there is no summary for it."""
-    path = os.path.join(ROOT, "profiles", {"c2": "sw_lds_f16_kernel_config2_20k.md"}.get(config, "-"))
+    path = os.path.join(ROOT, "profiles", {"tree20k": "sw_lds_f16_kernel_tree20k.md"}.get(config, "-"))
     try:
         with open(path) as f:
             for line in f:
@@ -341,20 +341,20 @@ def pair_set_recall(rp_p, col_p, dist_p, rp_e, col_e, dist_e, sample_rows=20000,
 
 
 def run_proposer_bench(args, pkg, log):
-    eps = args.eps if args.eps else (300 if args.config == "c4" else 200)
+    eps = args.eps if args.eps else (300 if args.config == "proposer100k" else 200)
     minpts = 5
-    if args.config == "c4":
+    if args.config == "proposer100k":
         # train on the amplicon-scale generator (its lengths, mutation rates and
         # noise) drawn with 2,000 ancestors under a new seed, evaluate on
         # 100,000 sequences of another draw
-        train = synth.config3(seed=4, n_anc=args.train_anc, per_anc=args.train_per)
-        groups = synth.config3_groups(n_anc=args.train_anc, per_anc=args.train_per)
+        train = synth.amplicon_families(seed=4, n_anc=args.train_anc, per_anc=args.train_per)
+        groups = synth.amplicon_family_labels(n_anc=args.train_anc, per_anc=args.train_per)
         enc, info = train_encoder(pkg, train, [(groups, 0.5)], groups, args.pairs, eps, 4,
                                   args.epochs, log, use_cosine=not args.no_cosine,
                                   recall=args.recall_target or 0.9999,
                                   exact_below=max(args.exact_below, 0))
         del train
-        full = synth.config3(seed=44)
+        full = synth.amplicon_families(seed=44)
         pick = np.sort(np.random.default_rng(44).choice(len(full), args.n or 100_000, replace=False))
         seqs = [full[k] for k in pick]
         workload = (f"learned proposer: encoder trained on {args.pairs} exact-labelled pairs "
@@ -365,7 +365,7 @@ def run_proposer_bench(args, pkg, log):
         # training draws of the same generator under other seeds
         parts, labels = [], []
         for k in range(args.train_draws):
-            tr, g = synth.config5(seed=50 + k, n=args.train_n, with_groups=True)
+            tr, g = synth.heavy_tail_mix(seed=50 + k, n=args.train_n, with_groups=True)
             parts += tr
             labels.append((g, k))
         del tr
@@ -378,7 +378,7 @@ def run_proposer_bench(args, pkg, log):
                                   recall=args.recall_target or 0.999999,
                                   exact_below=args.exact_below if args.exact_below >= 0 else 100)
         del parts
-        seqs = synth.config5(seed=5, n=args.n or 500_000)
+        seqs = synth.heavy_tail_mix(seed=5, n=args.n or 500_000)
         workload = ("end-to-end learned-prefilter DBSCAN: heavy-tailed lengths 80-1500 "
                     "(Pareto 1.5), 70% 4-level tree with 2,000 leaf clusters, 20% tandem "
                     "repeats of 171-bp monomers (2-8 copies, 3%), 10% noise; encoder trained "
@@ -470,7 +470,7 @@ def run_proposer_bench(args, pkg, log):
     common = {"n_gpus": 1, "steps": 1, "warmup": 0, "scaling": "weak", "vs_baseline": None,
               "data": "synthetic", "dtype": "bf16 (encoder) / int16 (verify)",
               "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma}
-    if args.config == "c4":
+    if args.config == "proposer100k":
         out.update({"metric": "learned-prefilter eps-neighbour recall (held-out)",
                     "value": recall, "unit": "fraction", "higher_is_better": True,
                     "ms_per_step": pref_lists_s * 1e3, **common})
@@ -493,8 +493,8 @@ def run_consumers_bench(args, log):
     20,000-sequence tree generator, n of each), against the reference's
     score() rate on the same data (its CPU path computes the same cells)."""
     n = args.n or 10000
-    a = synth.config2(seed=2)[::2][:n]
-    b = synth.config2(seed=3)[::2][:n]
+    a = synth.evolution_tree(seed=2)[::2][:n]
+    b = synth.evolution_tree(seed=3)[::2][:n]
     La = np.array([len(x) for x in a], np.float64)
     Lb = np.array([len(x) for x in b], np.float64)
     cross = float(La.sum() * Lb.sum())
@@ -586,7 +586,7 @@ def run_vec_bench(args, pkg, log):
                      "kernel_ms": kernel_s * 1e3, "pair_dims_per_s": pairs * dims / kernel_s,
                      "pairs_per_s_step": pairs / float(np.mean(times))}
     # 5-mer count vectors of the tree set (counted on the GPU)
-    seqs = synth.config2()
+    seqs = synth.evolution_tree()
     t0 = time.perf_counter()
     km = pkg.kmer_frequencies(seqs, 5, False)
     kmer_s = time.perf_counter() - t0
@@ -714,27 +714,28 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "consumers", "vec"],
-                    help="c2: 20,000 uneven-length (headline); c3: 105,000 amplicon-scale; "
-                         "c4: learned-proposer training + held-out recall; c5: end-to-end "
+    ap.add_argument("--config", default="tree20k",
+                    choices=["tree20k", "amplicon105k", "proposer100k", "mix500k", "consumers", "vec"],
+                    help="tree20k: 20,000 uneven-length (headline); amplicon105k: 105,000 amplicon-scale; "
+                         "proposer100k: learned-proposer training + held-out recall; mix500k: end-to-end "
                          "learned-prefilter DBSCAN vs the exact path")
     ap.add_argument("--eps", type=int, default=0,
-                    help="c4/c5 epsilon (default 300 for c4, 200 for c5)")
-    ap.add_argument("--pairs", type=int, default=2_000_000, help="c4/c5 labelled training pairs")
-    ap.add_argument("--epochs", type=int, default=3, help="c4/c5 training epochs")
-    ap.add_argument("--train-n", type=int, default=100_000, help="c5 sequences per training draw")
-    ap.add_argument("--train-draws", type=int, default=4, help="c5 training draws")
-    ap.add_argument("--train-anc", type=int, default=20000, help="c4 training ancestors")
-    ap.add_argument("--train-per", type=int, default=20, help="c4 training descendants per ancestor")
-    ap.add_argument("--no-cosine", action="store_true", help="c4/c5: embedding radius test only")
+                    help="proposer100k/mix500k epsilon (default 300 for proposer100k, 200 for mix500k)")
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="proposer100k/mix500k labelled training pairs")
+    ap.add_argument("--epochs", type=int, default=3, help="proposer100k/mix500k training epochs")
+    ap.add_argument("--train-n", type=int, default=100_000, help="mix500k sequences per training draw")
+    ap.add_argument("--train-draws", type=int, default=4, help="mix500k training draws")
+    ap.add_argument("--train-anc", type=int, default=20000, help="proposer100k training ancestors")
+    ap.add_argument("--train-per", type=int, default=20, help="proposer100k training descendants per ancestor")
+    ap.add_argument("--no-cosine", action="store_true", help="proposer100k/mix500k: embedding radius test only")
     ap.add_argument("--exact-unpruned", action="store_true",
-                    help="c4/c5: also time the exact lists over every unique pair (no length pruning)")
+                    help="proposer100k/mix500k: also time the exact lists over every unique pair (no length pruning)")
     ap.add_argument("--recall-target", type=float, default=0,
-                    help="c4/c5: calibrated per-bucket recall of the proposal tests "
-                         "(default 0.9999 for c4, 0.999999 for c5)")
+                    help="proposer100k/mix500k: calibrated per-bucket recall of the proposal tests "
+                         "(default 0.9999 for proposer100k, 0.999999 for mix500k)")
     ap.add_argument("--exact-below", type=int, default=-1,
-                    help="c4/c5: pairs whose longer sequence is at most this long skip the "
-                         "proposal tests (default 0 for c4, 100 for c5)")
+                    help="proposer100k/mix500k: pairs whose longer sequence is at most this long skip the "
+                         "proposal tests (default 0 for proposer100k, 100 for mix500k)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (torch) 3 times, "
                          "to calibrate FETCH_SIZE / WRITE_SIZE counters")
@@ -757,7 +758,7 @@ def main():
     pkg = load_pkg()
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
-    if args.config in ("c4", "c5"):
+    if args.config in ("proposer100k", "mix500k"):
         run_proposer_bench(args, pkg, log)
         return
     if args.config == "consumers":
@@ -767,7 +768,7 @@ def main():
         run_vec_bench(args, pkg, log)
         return
     print("generating workload ...", file=sys.stderr, flush=True)
-    seqs = synth.config2() if args.config == "c2" else synth.config3()
+    seqs = synth.evolution_tree() if args.config == "tree20k" else synth.amplicon_families()
     if args.n:
         seqs = seqs[:args.n]
     L = np.array([len(s) for s in seqs], dtype=np.float64)
@@ -851,7 +852,7 @@ def main():
         # best effort on the host: every core the box gives this process (at most 16)
         nt = max(1, min(16, len(os.sched_getaffinity(0))))
         out["cpu_baseline_all_cores"] = cpu_baseline(seqs, 3 * args.cpu_sample, threads=nt)
-    if not args.no_dbscan and args.config == "c2":
+    if not args.no_dbscan and args.config == "tree20k":
         print("adaptive DBSCAN run ...", file=sys.stderr, flush=True)
         out["dbscan"] = run_dbscan(seqs, 100, 300, 5)
     print(json.dumps(out), flush=True)

@@ -61,7 +61,7 @@ def to_bytes(codes: np.ndarray) -> bytes:
     return ALPHABET[codes].tobytes()
 
 
-def config1(seed: int = 1, n_anc: int = 20, per_anc: int = 100,
+def families_200bp(seed: int = 1, n_anc: int = 20, per_anc: int = 100,
             length: int = 200) -> List[bytes]:
     """Plain exact path: 20 ancestors (iid ACGT, length 200), 100 children each
     with 5% substitutions and 1% single-base indels; N = 2,000."""
@@ -74,7 +74,7 @@ def config1(seed: int = 1, n_anc: int = 20, per_anc: int = 100,
     return out
 
 
-def config2(seed: int = 2, n_roots: int = 100, n_children: int = 10,
+def evolution_tree(seed: int = 2, n_roots: int = 100, n_children: int = 10,
             n_leaves: int = 20, median: float = 300.0, sigma: float = 0.4,
             lo: int = 100, hi: int = 600) -> List[bytes]:
     """Uneven-length batching: 3-level tree.  Roots have lognormal lengths
@@ -93,7 +93,7 @@ def config2(seed: int = 2, n_roots: int = 100, n_children: int = 10,
     return out
 
 
-def config3(seed: int = 3, n_anc: int = 500, per_anc: int = 200,
+def amplicon_families(seed: int = 3, n_anc: int = 500, per_anc: int = 200,
             noise_frac: float = 0.05, lo: int = 150, hi: int = 300) -> List[bytes]:
     """Amplicon-scale: 500 ancestors with length uniform in [150, 300]; 200
     descendants each at a mutation rate drawn uniformly in [2%, 10%]
@@ -118,7 +118,7 @@ def pareto_lengths(rng: np.random.Generator, n: int, alpha: float = 1.5, xm: flo
     return np.minimum(np.rint(xm * (1.0 + rng.pareto(alpha, size=n))), hi).astype(np.int64)
 
 
-def config5(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 8,
+def heavy_tail_mix(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 8,
             per_child: int = 5, tree_frac: float = 0.70, repeat_frac: float = 0.20,
             n_monomer_families: int = 200, monomer: int = 171, with_groups: bool = False):
     """End-to-end learned-prefilter workload (heavy-tailed lengths 80-1500):
@@ -162,8 +162,8 @@ def config5(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 
     return out, groups
 
 
-def config3_groups(n_anc: int = 500, per_anc: int = 200, noise_frac: float = 0.05) -> np.ndarray:
-    """Ancestor label of every sequence of config3() with the same sizes
+def amplicon_family_labels(n_anc: int = 500, per_anc: int = 200, noise_frac: float = 0.05) -> np.ndarray:
+    """Ancestor label of every sequence of amplicon_families() with the same sizes
     (noise sequences get unique labels)."""
     n_desc = n_anc * per_anc
     n = n_desc + int(round(noise_frac * n_desc))

@@ -175,16 +175,16 @@ def make_inputs(ref_dir: str) -> dict:
     slice_fasta(os.path.join(ref_dir, "test-x.fasta"), os.path.join(d, "testx_300.fasta"), 300)
     stride_fasta(os.path.join(ref_dir, "test-x.fasta"), os.path.join(d, "testx_s33.fasta"), 33, 300)
     synth.write_fasta(os.path.join(d, "c1_small.fasta"),
-                      synth.config1(seed=11, n_anc=8, per_anc=40))
+                      synth.families_200bp(seed=11, n_anc=8, per_anc=40))
     synth.write_fasta(os.path.join(d, "c2_small.fasta"),
-                      synth.config2(seed=12, n_roots=8, n_children=4, n_leaves=8))
-    synth.write_fasta(os.path.join(d, "c1_full.fasta"), synth.config1(seed=1))
+                      synth.evolution_tree(seed=12, n_roots=8, n_children=4, n_leaves=8))
+    synth.write_fasta(os.path.join(d, "c1_full.fasta"), synth.families_200bp(seed=1))
     # queries / targets for the consumers: substitution-only copies of members
     # of c2_small (so that close matches exist; the third query has the first
     # one's length), and one random sequence
     import numpy as np
     rng = np.random.default_rng(13)
-    c2 = synth.config2(seed=12, n_roots=8, n_children=4, n_leaves=8)
+    c2 = synth.evolution_tree(seed=12, n_roots=8, n_children=4, n_leaves=8)
     def subs(seq: bytes, p: float) -> bytes:
         codes = np.frombuffer(seq, np.uint8).copy()
         lut = {b: k for k, b in enumerate(b"ACGT")}

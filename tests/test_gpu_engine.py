@@ -108,7 +108,7 @@ def _csr_to_dense(rp, col, dist, n):
 
 @pytest.mark.parametrize("eps", [0, 150, 400, 100000])
 def test_eps_csr_vs_oracle(pkg, oracle, eps):
-    seqs = synth.config1(seed=21, n_anc=6, per_anc=50)
+    seqs = synth.families_200bp(seed=21, n_anc=6, per_anc=50)
     _say(f"eps csr n={len(seqs)} eps={eps}")
     want = oracle.sw_matrix(seqs, nthreads=16)
     with pkg.Engine(seqs) as eng:
@@ -154,10 +154,10 @@ def test_eps_csr_length_pruning_is_exact(pkg, oracle, monkeypatch, eps):
     assert np.array_equal(got["1"][1], expect)
 
 
-def test_config1_full_matrix_and_eps(pkg, oracle):
-    seqs = synth.config1(seed=1)
+def test_families_200bp_full_matrix_and_eps(pkg, oracle):
+    seqs = synth.families_200bp(seed=1)
     n = len(seqs)
-    _say(f"config1 full n={n}")
+    _say(f"families_200bp full n={n}")
     want = oracle.sw_matrix(seqs, nthreads=16)
     with pkg.Engine(seqs) as eng:
         got = eng.matrix()
@@ -168,20 +168,20 @@ def test_config1_full_matrix_and_eps(pkg, oracle):
     expect = np.where(want <= 400, want, -1)
     np.fill_diagonal(expect, -1)
     assert np.array_equal(dense, expect)
-    _say(f"config1 eps kernel {t['kernel_ms']:.2f} ms, {t['cells'] / t['kernel_ms'] / 1e6:.1f} GCUPS")
+    _say(f"families_200bp eps kernel {t['kernel_ms']:.2f} ms, {t['cells'] / t['kernel_ms'] / 1e6:.1f} GCUPS")
 
 
-def test_config2_full_size_properties(pkg, oracle):
+def test_evolution_tree_full_size_properties(pkg, oracle):
     """Full 20,000-sequence uneven-length set: size-independent properties
     plus oracle checks on sampled pairs (neighbours and non-neighbours)."""
-    seqs = synth.config2()
+    seqs = synth.evolution_tree()
     n = len(seqs)
     eps = 600
-    _say(f"config2 full n={n}")
+    _say(f"evolution_tree full n={n}")
     with pkg.Engine(seqs) as eng:
         rp, col, dist = eng.eps_csr(eps)
         t = eng.last_timing()
-        _say(f"config2 eps kernel {t['kernel_ms']:.1f} ms, {t['cells'] / t['kernel_ms'] / 1e6:.1f} GCUPS, nnz={len(col)}")
+        _say(f"evolution_tree eps kernel {t['kernel_ms']:.1f} ms, {t['cells'] / t['kernel_ms'] / 1e6:.1f} GCUPS, nnz={len(col)}")
         rng = np.random.default_rng(3)
         ij = rng.integers(0, n, size=(4000, 2)).astype(np.int32)
         d_pairs = eng.pairs(ij)
@@ -209,7 +209,7 @@ def test_config2_full_size_properties(pkg, oracle):
 
 def test_autotune_choice_is_exact_and_cached(pkg, oracle, tmp_path, monkeypatch):
     monkeypatch.setenv("MNHN_SW_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
-    seqs = synth.config1(seed=31, n_anc=8, per_anc=60)
+    seqs = synth.families_200bp(seed=31, n_anc=8, per_anc=60)
     _say(f"autotune n={len(seqs)}")
     want = oracle.sw_matrix(seqs, nthreads=16)
     with pkg.Engine(seqs) as eng:

@@ -48,7 +48,7 @@ def _gelu(x):
 
 
 def test_kmer_profiles_match_numpy(pkg):
-    seqs = synth.config2(seed=41, n_roots=4, n_children=3, n_leaves=5) + [b"A", b"ACGTAC", b""]
+    seqs = synth.evolution_tree(seed=41, n_roots=4, n_children=3, n_leaves=5) + [b"A", b"ACGTAC", b""]
     _say(f"kmer profiles n={len(seqs)}")
     with pkg.Engine(seqs) as eng:
         got = eng.kmer_profiles()
@@ -57,7 +57,7 @@ def test_kmer_profiles_match_numpy(pkg):
 
 
 def test_embedding_matches_numpy_forward(pkg, tmp_path):
-    seqs = synth.config1(seed=42, n_anc=4, per_anc=30)
+    seqs = synth.families_200bp(seed=42, n_anc=4, per_anc=30)
     _say(f"embedding n={len(seqs)}")
     with pkg.Engine(seqs) as eng, pkg.Encoder(seed=7) as enc:
         got = enc.embed(eng)
@@ -76,7 +76,7 @@ def test_embedding_matches_numpy_forward(pkg, tmp_path):
 
 
 def test_training_reduces_loss(pkg):
-    seqs = synth.config1(seed=43, n_anc=10, per_anc=40, length=160)
+    seqs = synth.families_200bp(seed=43, n_anc=10, per_anc=40, length=160)
     rng = np.random.default_rng(0)
     n = len(seqs)
     ij = rng.integers(0, n, size=(20000, 2)).astype(np.int32)
@@ -127,7 +127,7 @@ def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, tuning):
 
 
 def test_prefilter_trained_recall(pkg, oracle):
-    seqs = synth.config3(seed=45, n_anc=20, per_anc=40, noise_frac=0.05)
+    seqs = synth.amplicon_families(seed=45, n_anc=20, per_anc=40, noise_frac=0.05)
     n = len(seqs)
     eps = 300
     rng = np.random.default_rng(2)
@@ -156,7 +156,7 @@ def test_proposal_rule_and_margin_table_match_numpy(pkg, tmp_path):
     proposal rule (length bound, radius eps / (4 l_long) + margin(l_long) with
     a three-bucket margin table) on bf16-rounded embeddings; pairs within
     floating-point rounding of the radius are left out of the comparison."""
-    seqs = synth.config2(seed=46, n_roots=8, n_children=4, n_leaves=12)
+    seqs = synth.evolution_tree(seed=46, n_roots=8, n_children=4, n_leaves=12)
     eps = 300
     n = len(seqs)
     L = np.array([len(s) for s in seqs], dtype=np.int64)
@@ -208,7 +208,7 @@ def test_prefilter_positions_beyond_2e31(pkg):
     """80,000 amplicon-like sequences with an unbounded radius propose 2.35e9
     pairs, so candidate positions pass 2^31 (wave-uniform 64-bit offsets);
     the verified lists must still equal the exact all-pairs lists."""
-    full = synth.config3(seed=44)
+    full = synth.amplicon_families(seed=44)
     pick = np.sort(np.random.default_rng(44).choice(len(full), 80000, replace=False))
     seqs = [full[k] for k in pick]
     eps = 300
@@ -226,7 +226,7 @@ def test_cosine_rule_matches_numpy(pkg):
     """With a cosine table set, a pair is proposed only if it also passes the
     raw 5-mer count cosine bound of its longer sequence's length bucket:
     candidate count and verified lists against numpy."""
-    seqs = synth.config2(seed=47, n_roots=8, n_children=4, n_leaves=12)
+    seqs = synth.evolution_tree(seed=47, n_roots=8, n_children=4, n_leaves=12)
     eps = 400
     n = len(seqs)
     L = np.array([len(s) for s in seqs], dtype=np.int64)

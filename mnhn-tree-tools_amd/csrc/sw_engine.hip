@@ -1042,6 +1042,16 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
 
 namespace {
 
+// Work items per CU in the persistent queue (MNHN_SW_ITEMS_PER_CU, default
+// 4096).  On 20,000 sequences of 100-600 residues, 256 items per CU left the
+// longest-first queue's tail at ~5 % of the run (17,081 GCUPS at strip 32)
+// and 4096 runs at 17,876; items stay one or more whole queries.
+double items_per_cu() {
+  const char *e = getenv("MNHN_SW_ITEMS_PER_CU");
+  const double v = e ? atof(e) : 0.0;
+  return v > 0 ? v : 4096.0;
+}
+
 // Split the query ranges [0, nq_g) of each group into items of similar cost.
 void add_group_items(const mnhn_sw_ctx *c, Work &w, int g, int q_begin, int q_end,
                      double target_cost, int S) {
@@ -1131,13 +1141,7 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
     const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
     for (int p = first[g]; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
   }
-  // ~4096 items per CU (MNHN_SW_ITEMS_PER_CU): on 20,000 sequences of 100-600
-  // residues, 256 items per CU left the longest-first queue's tail at ~5 % of
-  // the run (17,081 GCUPS at strip 32) and 4096 runs at 17,876; items stay one
-  // or more whole queries
-  const char *ipc_env = getenv("MNHN_SW_ITEMS_PER_CU");
-  const double ipc = ipc_env && atof(ipc_env) > 0 ? atof(ipc_env) : 4096.0;
-  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * ipc), 2.0e5);
+  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * items_per_cu()), 2.0e5);
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     if (last <= first[g]) continue;
@@ -1418,7 +1422,7 @@ int mnhn_sw_block(mnhn_sw_ctx *c, const int32_t *rows, int nr, const int32_t *co
   }
   for (int k = 0; k < nc; ++k) sum_lb += c->len[cols[k]];
   for (int r = 0; r < nr; ++r) sum_la += c->len[rows[r]];
-  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * 256.0), 2.0e5);
+  const double target = std::max(total / ((double)std::max(c->num_cu, 1) * items_per_cu()), 2.0e5);
   for (int g = 0; g < G; ++g) {
     const int strips = w.gmax[g] > 1 ? (w.gmax[g] - 1 + S - 1) / S : 0;
     int qb = 0;

@@ -468,7 +468,7 @@ def run_proposer_bench(args, pkg, log):
                 "identical_clusters": bool(same), "identical_member_order": ecl == pcl,
                 "clustered_points": int(sum(len(c) for c in ecl))})
     common = {"n_gpus": 1, "steps": 1, "warmup": 0, "scaling": "weak", "vs_baseline": None,
-              "data": "synthetic", "dtype": "bf16 (encoder) / int16 (verify)",
+              "data": "synthetic", "dtype": "bf16 (encoder) / f16 exact integers + int16 (verify)",
               "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma}
     if args.config == "proposer100k":
         out.update({"metric": "learned-prefilter eps-neighbour recall (held-out)",
@@ -522,7 +522,7 @@ def run_consumers_bench(args, log):
     out = {"metric": "SW consumer CLIs wall seconds (compareSW)", "value": res["compareSW"]["s"],
            "unit": "s", "higher_is_better": False, "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": res["compareSW"]["s"] * 1e3, "scaling": "weak", "vs_baseline": None,
-           "dtype": "int16", "data": "synthetic",
+           "dtype": "f16 (exact integers) + int16", "data": "synthetic",
            "config": {"workload": f"compareSW / silhouette / find_closest_sequence_SW on two sets of "
                                   f"{n} uneven-length sequences (tree generator draws)",
                       "runs": res}}

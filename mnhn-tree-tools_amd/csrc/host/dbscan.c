@@ -344,12 +344,30 @@ typedef struct {
   int symmetric;       /* every stored (p, q, d) has its (q, p, d) */
 } dist_index;
 
-/* one pass over the index: keep entries <= thr, insert p itself when self */
+/* one pass over the index: keep entries <= thr, insert p itself when self.
+ * all: the threshold keeps every stored entry; the index is then a view of the
+ * stored rows themselves (column order is a valid order of a row kept whole),
+ * so a pass reads the lists in place with no copy or filter. */
 typedef struct {
   const dist_index *ix;
   float thr;
   int self;
+  int all;
 } ix_pass;
+
+static dist_index rows_view(int n, const int64_t *rp, const int32_t *col, const int16_t *dist, const float *val,
+                            int symmetric) {
+  dist_index v;
+  memset(&v, 0, sizeof(v));
+  v.n = n;
+  v.rp = rp;
+  v.col = col;
+  v.dist = dist;
+  v.val = val;
+  v.dcol = (int32_t *)col; /* read only in passes */
+  v.symmetric = symmetric;
+  return v;
+}
 
 /* sort key whose unsigned order is the order of the distances: signed int16,
  * or float with every NaN last */
@@ -583,6 +601,7 @@ static inline int ix_keep_sorted(const dist_index *ix, int64_t t, float thr) {
 /* |N(p)|: the kept prefix of the distance-ordered row, plus p itself */
 static int ix_count(const ix_pass *ps, int p) {
   const dist_index *ix = ps->ix;
+  if (ps->all) return (int)(ix->rp[p + 1] - ix->rp[p]) + ps->self;
   int64_t lo = ix->rp[p], hi = ix->rp[p + 1];
   const int64_t b = lo;
   while (lo < hi) {
@@ -605,6 +624,7 @@ static int ix_member(const ix_pass *ps, int q, int x) {
     else hi = mid;
   }
   if (lo == e || ix->col[lo] != x) return 0;
+  if (ps->all) return 1;
   return ix->dist ? (float)ix->dist[lo] <= ps->thr : ix->val[lo] <= ps->thr;
 }
 
@@ -807,7 +827,7 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
                 nb[k++] = x;
               }
             }
-            sort_ascending(nb, k, tmp);
+            if (!ix->all) sort_ascending(nb, k, tmp); /* a whole row is already ascending */
             memcpy(queue + qn, nb, sizeof(int) * (size_t)k);
             qn += k;
             in_union += k;
@@ -858,6 +878,23 @@ static void apply_defaults(float *epsilon, int *minpts) {
   }
 }
 
+/* One pass over stored lists.  all_kept: the threshold keeps every stored
+ * entry, so the rows are read in place (no filtered copy); otherwise each
+ * pass materialises its lists. */
+static split_set stored_pass(int n, float epsilon, int minpts, const int64_t *rp, const int32_t *col,
+                             const int16_t *dist, const float *val, float thr, int self, int all_kept,
+                             int symmetric) {
+  if (all_kept) {
+    const dist_index v = rows_view(n, rp, col, dist, val, symmetric);
+    const ix_pass ps = {&v, thr, self, 1};
+    return dbscan_core(n, epsilon, minpts, NULL, NULL, &ps);
+  }
+  pass_lists pl = build_pass_lists(n, rp, col, dist, val, thr, self);
+  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
+  free_pass_lists(&pl);
+  return s;
+}
+
 split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
   const int n = ds.n_values;
   apply_defaults(&epsilon, &minpts);
@@ -874,12 +911,11 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
     const mnhn_csr *c = &g.cache->csr;
     const dist_index *ix = cache_index(g.cache, n, c->row_ptr, c->col, c->dist, NULL);
     if (ix) {
-      const ix_pass ps = {ix, epsilon, (float)0 <= epsilon};
+      const ix_pass ps = {ix, epsilon, (float)0 <= epsilon, 0};
       s = dbscan_core(n, epsilon, minpts, NULL, NULL, &ps);
     } else {
-      pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, c->dist, NULL, epsilon, (float)0 <= epsilon);
-      s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
-      free_pass_lists(&pl);
+      s = stored_pass(n, epsilon, minpts, c->row_ptr, c->col, c->dist, NULL, epsilon, (float)0 <= epsilon,
+                      eps_floor(epsilon) >= c->eps_max, 1);
     }
   } else {
     s = dbscan_core(n, epsilon, minpts, rq_gpu, &g, NULL);
@@ -892,10 +928,8 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
 split_set mnhn_dbscan_csr(const mnhn_csr *csr, float epsilon, int minpts) {
   const int n = csr ? csr->n_rows : 0;
   if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL, NULL);
-  pass_lists pl = build_pass_lists(n, csr->row_ptr, csr->col, csr->dist, NULL, epsilon, (float)0 <= epsilon);
-  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
-  free_pass_lists(&pl);
-  return s;
+  return stored_pass(n, epsilon, minpts, csr->row_ptr, csr->col, csr->dist, NULL, epsilon, (float)0 <= epsilon,
+                     eps_floor(epsilon) >= csr->eps_max, 0);
 }
 
 /* ---- vector datasets (dbscan.c:563-780) ---- */
@@ -934,13 +968,11 @@ static split_set dbscan_vec(dataset ds, float epsilon, int minpts, mnhn_gpu_stuf
   const mnhn_vcsr *c = &g.cache->vcsr;
   const dist_index *ix = cache_index(g.cache, n, c->row_ptr, c->col, NULL, c->val);
   if (ix) {
-    const ix_pass ps = {ix, vec_thr(norm, epsilon), 0};
+    const ix_pass ps = {ix, vec_thr(norm, epsilon), 0, 0};
     return dbscan_core(n, epsilon, minpts, NULL, NULL, &ps);
   }
-  pass_lists pl = build_pass_lists(n, c->row_ptr, c->col, NULL, c->val, vec_thr(norm, epsilon), 0);
-  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
-  free_pass_lists(&pl);
-  return s;
+  const float thr = vec_thr(norm, epsilon);
+  return stored_pass(n, epsilon, minpts, c->row_ptr, c->col, NULL, c->val, thr, 0, thr >= c->thr, 1);
 }
 
 split_set dbscan_L1_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g) {
@@ -968,10 +1000,8 @@ split_set dbscan_L2(dataset ds, float epsilon, int minpts) {
 split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts) {
   const int n = csr ? csr->n_rows : 0;
   if (n == 0) return dbscan_core(0, epsilon, minpts, rq_pass, NULL, NULL);
-  pass_lists pl = build_pass_lists(n, csr->row_ptr, csr->col, NULL, csr->val, vec_thr(csr->norm, epsilon), 0);
-  split_set s = dbscan_core(n, epsilon, minpts, rq_pass, &pl, NULL);
-  free_pass_lists(&pl);
-  return s;
+  const float thr = vec_thr(csr->norm, epsilon);
+  return stored_pass(n, epsilon, minpts, csr->row_ptr, csr->col, NULL, csr->val, thr, 0, thr >= csr->thr, 0);
 }
 
 /* Several passes over the same lists: through the distance-ordered index when
@@ -990,7 +1020,7 @@ static int dbscan_sweep(int n, const int64_t *rp, const int32_t *col, const int1
     if (n == 0) {
       out[k] = dbscan_core(0, eps[k], minpts, rq_pass, NULL, NULL);
     } else if (have) {
-      const ix_pass ps = {&ix, thr, self};
+      const ix_pass ps = {&ix, thr, self, 0};
       out[k] = dbscan_core(n, eps[k], minpts, NULL, NULL, &ps);
     } else {
       pass_lists pl = build_pass_lists(n, rp, col, dist, val, thr, self);

@@ -564,3 +564,27 @@ def test_dbscan_sweep_long_lists_outside_a_cluster(pkg, oracle):
         want = oracle.dbscan(n, e, 4, rq)
         assert g == [c.members for c in want.clusters], e
     assert pkg.dbscan_csr_sweep(rp, col, d, eps_max, eps, 4, symmetric=False) == got
+
+
+def test_dbscan_passes_over_whole_lists_match_oracle(pkg, oracle):
+    """A pass whose epsilon keeps every stored entry reads the rows in place
+    (no filtered copy): Smith-Waterman lists (self inserted) and vector lists
+    (self stored), dense neighbourhoods included, give the oracle's clusters."""
+    rng = np.random.default_rng(23)
+    n = 1400
+    centre = rng.integers(0, 8, n)
+    d = np.abs(centre[:, None] - centre[None, :]) * 100 + rng.integers(0, 300, (n, n))
+    d = np.triu(d, 1)
+    d = (d + d.T).astype(np.int64)
+    for eps, minpts in ((900, 5), (400, 5), (1500, 2), (250, 40)):
+        rp, col, dist = _lists_from_matrix(d, eps)
+        got = pkg.dbscan_csr(rp, col, dist, eps, eps, minpts)
+        want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(d))
+        assert got == [c.members for c in want.clusters], (eps, minpts)
+        acc = d.astype(np.float32)
+        keep = acc <= np.float32(eps)
+        rpv = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+        colv = np.nonzero(keep)[1].astype(np.int32)
+        got = pkg.dbscan_vcsr(rpv, colv, acc[keep], 1, float(eps), eps, minpts)
+        want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_acc(acc, 1))
+        assert got == [c.members for c in want.clusters], ("L1", eps, minpts)

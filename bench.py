@@ -547,6 +547,8 @@ def run_consumers_bench(args, log):
 
 # MI355X published fp32 vector peak (an FMA counts 2 flops)
 PEAK_FP32_TFLOPS = 157.3
+# 1,024 SIMDs x 2.4 GHz x 64 lanes / 6 cycles per pair-dimension (sub 2 + fma 4)
+L2_INSTR_PAIR_DIMS_S = 1024 * 2.4e9 * 64 / 6
 
 
 def _run_cli(argv, cwd, timeout=1800):
@@ -630,7 +632,12 @@ def run_vec_bench(args, pkg, log):
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s (fp32 VALU; per pair-dimension a sub and an fma = 3 flop)",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                        "kernel": "vec_tiles_kernel<2>"}}
+                        "kernel": "vec_tiles_kernel<2>",
+                        # issue-rate bound of the kernel's instruction mix (measured
+                        # on gfx950 by bin/valu_rates): per pair-dimension a
+                        # v_sub_f32 (2 cycles per wave64) and a v_fmac_f32 (4)
+                        "instruction_roofline_pair_dims_s": L2_INSTR_PAIR_DIMS_S,
+                        "instruction_frac": res["l2_pca"]["pair_dims_per_s"] / L2_INSTR_PAIR_DIMS_S}}
     # drop-in CLIs end to end, and the reference on samples (outputs compared)
     with tempfile.TemporaryDirectory() as tmp:
         fa = os.path.join(tmp, "p.fasta")

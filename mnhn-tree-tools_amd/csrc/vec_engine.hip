@@ -65,8 +65,11 @@ constexpr int kWaveBuf = 512;  // listed pairs a wave collects in LDS before one
 __device__ __forceinline__ long long tri_start(long long t) { return t * (t + 1) / 2; }
 
 // Upper-triangle tile t (column-major: tiles (0,0), (0,1), (1,1), (0,2), ...).
+// A float estimate of the column (off by at most a few for any tile count
+// that fits the output keys), then exact integer correction.
 __device__ __forceinline__ void tile_of(long long t, int &ti, int &tj) {
-  long long c = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  long long c = (long long)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+  if (c < 0) c = 0;
   while (tri_start(c) > t) --c;
   while (tri_start(c + 1) <= t) ++c;
   tj = (int)c;
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
         g = load_chunk(x, n_pad, dims_pad, i0, j0, k0 + kChunk, lk0, lc);
       else if (t_next < n_tiles)
         g = load_chunk(x, n_pad, dims_pad, ti_n * kTile, tj_n * kTile, 0, lk0, lc);
-      const int kc = min(kChunk, dims_pad - k0);  // even
+      const int kc = min(kChunk, dims_pad - k0);
 #pragma unroll 2
       for (int k = 0; k < kc; ++k) {
         const float4 a0 = sa[k][ty], a1 = sa[k][16 + ty];
@@ -343,7 +346,7 @@ int mnhn_vec_create(const float *const *values, int dims, int n, mnhn_vec_ctx **
   mnhn_vec_ctx *c = new mnhn_vec_ctx();
   c->n = n;
   c->dims = dims;
-  c->dims_pad = std::max(2, (dims + 1) / 2 * 2);  // even: the dimension loop is unrolled by two
+  c->dims_pad = std::max(1, dims);  // no padding: a zero dimension would cost a step per pair
   c->n_pad = ((long long)n + kTile - 1) / kTile * kTile;
   int rc = MNHN_OK;
   auto fail = [&](int code) {

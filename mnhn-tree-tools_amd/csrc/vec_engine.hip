@@ -18,9 +18,11 @@
 // accumulators in registers (two VALU ops per pair and dimension: the
 // subtraction and the fma / abs-add); the next chunk -- across the tile
 // boundary too -- is loaded into registers while the current one computes.
-// Pairs under the threshold are collected per wave in an LDS buffer and
-// appended as (i << 32 | j, acc) with one atomic reservation per flush of up
-// to 512 pairs, so sparse lists do not serialise on the output counter.
+// A lane whose smallest accumulator passes the threshold hands its 8 x 8 block
+// to the whole wave through LDS, one accumulator per lane; pairs under the
+// threshold are collected per wave in an LDS buffer and appended as
+// (i << 32 | j, acc) with one atomic reservation per flush of up to 512
+// pairs, so sparse lists do not serialise on the output counter.
 //
 // Lists.  Pairs are expanded to both directions as keys row * 2^b + col, radix
 // sorted (rocPRIM) with the accumulator as payload, and cut into CSR rows by a
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
   __shared__ float4 sb[kChunk][kTile / 4];
   __shared__ unsigned long long bkey[kThreads / 64][kWaveBuf];
   __shared__ float bval[kThreads / 64][kWaveBuf];
+  __shared__ float4 spill[kThreads / 64][16];  // one lane's 8 x 8 accumulators, row-major
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int lane = tid & 63, wave = tid >> 6;
@@ -181,54 +184,49 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     }
 
     // emission: pair (i, j) with i <= j < n and acc <= thr, into the wave's
-    // buffer one tile row at a time (at most 64 lanes x 8 = kWaveBuf pairs)
-    // the smallest accumulator first (NaN never passes a threshold and
-    // fminf skips it): most waves of a sparse list stop here
+    // buffer.  The smallest accumulator first (NaN never passes a threshold
+    // and fminf skips it): most waves of a sparse list stop here.  Otherwise
+    // each lane that may hold a pair spills its 8 x 8 block to LDS in turn and
+    // the wave tests it one accumulator per lane, so the cost follows the
+    // lanes with pairs rather than 64 tests in every lane.
     float mn = acc[0][0];
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
       for (int c = 0; c < 8; ++c) mn = fminf(mn, acc[r][c]);
-    unsigned long long hitmask = 0ull;
-    if (__ballot(mn <= thr)) {
+    unsigned long long cand = __ballot(mn <= thr);
+    while (cand) {
+      const int h = __builtin_ctzll(cand);
+      cand &= cand - 1ull;
+      if (lane == h) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int i = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int j = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
-          const bool hit = i <= j && j < n && acc[r][c] <= thr;
-          hitmask |= (unsigned long long)hit << (r * 8 + c);
+        for (int r = 0; r < 8; ++r) {
+          spill[wave][r * 2 + 0] = make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+          spill[wave][r * 2 + 1] = make_float4(acc[r][4], acc[r][5], acc[r][6], acc[r][7]);
         }
       }
-    }
-    if (__ballot(hitmask != 0ull)) {
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const unsigned rm = (unsigned)(hitmask >> (8 * r)) & 0xffu;
-        if (!__ballot(rm != 0u)) continue;
-        const int rc = __popc(rm);
-        int incl = rc;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const int y = __shfl_up(incl, off, 64);
-          if (lane >= off) incl += y;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // lane e tests accumulator (r, c) = (e / 8, e % 8) of lane h
+      const int r = lane >> 3, c = lane & 7;
+      const float v = reinterpret_cast<const float *>(&spill[wave][0])[r * 8 + c];
+      const int hy = wave * 4 + (h >> 4), hx = h & 15;  // ty, tx of lane h
+      const int i = i0 + (r < 4 ? hy * 4 + r : 64 + hy * 4 + r - 4);
+      const int j = j0 + (c < 4 ? hx * 4 + c : 64 + hx * 4 + c - 4);
+      const bool hit = i <= j && j < n && v <= thr;
+      const unsigned long long hm = __ballot(hit);
+      const int cnt = __popcll(hm);
+      if (cnt > 0) {
+        if (bcount + cnt > kWaveBuf) flush();
+        if (hit) {
+          const int p = bcount + __popcll(hm & ((1ull << lane) - 1ull));
+          bkey[wave][p] = ((unsigned long long)(unsigned)i << 32) | (unsigned long long)(unsigned)j;
+          bval[wave][p] = v;
         }
-        const int rtotal = __shfl(incl, 63, 64);
-        if (bcount + rtotal > kWaveBuf) flush();
-        int p = bcount + incl - rc;
-        const int i = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          if ((rm >> c) & 1u) {
-            const int j = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
-            bkey[wave][p] = ((unsigned long long)(unsigned)i << 32) | (unsigned long long)(unsigned)j;
-            bval[wave][p] = acc[r][c];
-            ++p;
-          }
-        }
-        bcount += rtotal;
+        bcount += cnt;
       }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
     if (t_next >= n_tiles) break;
     t = t_next;

@@ -715,6 +715,205 @@ static void sort_ascending(int *a, int k, int *tmp) {
   if (src != a) memcpy(a, src, sizeof(int) * (size_t)k);
 }
 
+/* ---- parallel row scans for a batch of queue elements ----
+ * The expansion appends, for each core point q of the queue in order, the
+ * members of N(q) still outside the union, ascending.  For a batch of
+ * consecutive queue elements, host threads scan their rows in parallel
+ * against the union as it stands at the batch's start; at each element's turn
+ * the expansion keeps the scanned points still outside (the union only
+ * grows), which appends the same points in the same order as scanning the row
+ * then.  Rows come from the materialised pass lists or from the index. */
+typedef struct scanner scanner;
+typedef struct {
+  scanner *sc;
+  int w;
+} scan_arg;
+
+struct scanner {
+  int nth;      /* shares of a batch */
+  int started[16];
+  scan_arg args[16];
+  pthread_t th[16];
+  pthread_mutex_t mu;
+  pthread_cond_t cv;
+  int armed; /* barriers initialised for the threads that started */
+  pthread_barrier_t start, done;
+  int quit;
+  const pass_lists *pl; /* materialised rows (ascending, p included), or */
+  const ix_pass *ix;    /* index rows (kept prefix) */
+  int64_t cap;          /* scanned entries per batch */
+  int max_elems;
+  /* the batch */
+  const int *qs;
+  int nq;
+  const int *stamp;
+  int mark;
+  int *cq;      /* [max_elems] |N(q)| */
+  int64_t *off; /* [max_elems + 1] */
+  int *len;     /* [max_elems] scanned points, -1: not scanned */
+  int *out;     /* [cap] */
+  int *tmp[16]; /* per-share sort scratch, index rows */
+};
+
+/* Share w of the batch: the elements whose output starts in the w-th part of
+ * the batch's scanned entries. */
+static void scan_share(scanner *sc, int w) {
+  const int64_t tot = sc->off[sc->nq];
+  const int64_t lo = tot / sc->nth * w, hi = w + 1 == sc->nth ? tot : tot / sc->nth * (w + 1);
+  for (int k = 0; k < sc->nq; k++) {
+    if (sc->len[k] < 0 || sc->off[k] < lo || sc->off[k] >= hi) continue;
+    const int q = sc->qs[k];
+    const int32_t *row;
+    int m;
+    if (sc->pl) {
+      row = sc->pl->col + sc->pl->rp[q];
+      m = (int)(sc->pl->rp[q + 1] - sc->pl->rp[q]);
+    } else {
+      row = sc->ix->ix->dcol + sc->ix->ix->rp[q];
+      m = sc->cq[k] - sc->ix->self;
+    }
+    int *o = sc->out + sc->off[k];
+    int c = 0;
+    for (int t = 0; t < m; t++) {
+      const int x = row[t];
+      if (sc->stamp[x] != sc->mark) o[c++] = x;
+    }
+    if (sc->ix && !sc->ix->all) sort_ascending(o, c, sc->tmp[w]);
+    sc->len[k] = c;
+  }
+}
+
+static void *scan_worker(void *arg) {
+  scan_arg *a = (scan_arg *)arg;
+  scanner *sc = a->sc;
+  pthread_mutex_lock(&sc->mu);
+  while (!sc->armed) pthread_cond_wait(&sc->cv, &sc->mu);
+  pthread_mutex_unlock(&sc->mu);
+  for (;;) {
+    pthread_barrier_wait(&sc->start);
+    if (sc->quit) break;
+    scan_share(sc, a->w);
+    pthread_barrier_wait(&sc->done);
+  }
+  return NULL;
+}
+
+static void scanner_free(scanner *sc) {
+  for (int w = 0; w < 16; w++) free(sc->tmp[w]);
+  free(sc->cq);
+  free(sc->off);
+  free(sc->len);
+  free(sc->out);
+  free(sc);
+}
+
+/* A scanner for rows of total stored entries; NULL when one thread would do
+ * (small lists) or no helper thread starts.  Shares whose thread did not start
+ * are run by the caller. */
+static scanner *scanner_create(int n, int64_t total, const pass_lists *pl, const ix_pass *ix) {
+  const int T = host_threads(total >> 2);
+  if (T < 2 || n <= 0) return NULL;
+  scanner *sc = (scanner *)calloc(1, sizeof(scanner));
+  if (!sc) return NULL;
+  sc->nth = T;
+  sc->pl = pl;
+  sc->ix = ix;
+  sc->cap = (int64_t)1 << 22;
+  if (sc->cap < (int64_t)n + 1) sc->cap = (int64_t)n + 1;
+  sc->max_elems = n;
+  sc->cq = (int *)malloc(sizeof(int) * (size_t)n);
+  sc->off = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+  sc->len = (int *)malloc(sizeof(int) * (size_t)n);
+  sc->out = (int *)malloc(sizeof(int) * (size_t)sc->cap);
+  int ok = sc->cq && sc->off && sc->len && sc->out;
+  for (int w = 0; w < T && ok && ix && !ix->all; w++) ok = (sc->tmp[w] = (int *)malloc(sizeof(int) * (size_t)n)) != NULL;
+  if (!ok || pthread_mutex_init(&sc->mu, NULL) != 0) {
+    scanner_free(sc);
+    return NULL;
+  }
+  if (pthread_cond_init(&sc->cv, NULL) != 0) {
+    pthread_mutex_destroy(&sc->mu);
+    scanner_free(sc);
+    return NULL;
+  }
+  int nstarted = 0;
+  for (int w = 1; w < T; w++) {
+    sc->args[w] = (scan_arg){sc, w};
+    sc->started[w] = pthread_create(&sc->th[w], NULL, scan_worker, &sc->args[w]) == 0;
+    nstarted += sc->started[w];
+  }
+  if (nstarted == 0) {
+    pthread_cond_destroy(&sc->cv);
+    pthread_mutex_destroy(&sc->mu);
+    scanner_free(sc);
+    return NULL;
+  }
+  if (pthread_barrier_init(&sc->start, NULL, (unsigned)(nstarted + 1)) != 0 ||
+      pthread_barrier_init(&sc->done, NULL, (unsigned)(nstarted + 1)) != 0) {
+    printf("could not set up host threads for the DBSCAN expansion\n");
+    fflush(stdout);
+    _exit(1);
+  }
+  pthread_mutex_lock(&sc->mu);
+  sc->armed = 1;
+  pthread_cond_broadcast(&sc->cv);
+  pthread_mutex_unlock(&sc->mu);
+  return sc;
+}
+
+static void scanner_destroy(scanner *sc) {
+  if (!sc) return;
+  sc->quit = 1;
+  pthread_barrier_wait(&sc->start);
+  for (int w = 1; w < sc->nth; w++)
+    if (sc->started[w]) pthread_join(sc->th[w], NULL);
+  pthread_barrier_destroy(&sc->start);
+  pthread_barrier_destroy(&sc->done);
+  pthread_cond_destroy(&sc->cv);
+  pthread_mutex_destroy(&sc->mu);
+  scanner_free(sc);
+}
+
+/* Plans the batch starting at queue element qs[0] (avail elements queued)
+ * and scans its rows in parallel; returns how many elements it covers.  An
+ * element is scanned when it is unvisited, a core point, and its N(q) is not
+ * long enough for the walk over the points outside the union (that test only
+ * becomes more likely as the union grows, so a row not scanned here is not
+ * needed later in the batch either). */
+static int scan_batch(scanner *sc, const int *qs, int avail, const unsigned char *visited, const int *stamp,
+                      int mark, int in_union, int n, int minpts) {
+  int64_t tot = 0;
+  int k = 0;
+  for (; k < avail && k < sc->max_elems; k++) {
+    const int q = qs[k];
+    int c = -1;
+    if (!visited[q]) {
+      const int cq = sc->ix ? ix_count(sc->ix, q) : (int)(sc->pl->rp[q + 1] - sc->pl->rp[q]);
+      sc->cq[k] = cq;
+      if (cq >= minpts && cq <= 8 * (n - in_union)) c = sc->ix ? cq - sc->ix->self : cq;
+    }
+    if (c > 0 && k > 0 && tot + c > sc->cap) break;
+    sc->off[k] = tot;
+    sc->len[k] = c < 0 ? -1 : 0;
+    if (c > 0) tot += c;
+  }
+  sc->off[k] = tot;
+  sc->nq = k;
+  if (tot < (1 << 16)) {
+    for (int t = 0; t < k; t++) sc->len[t] = -1; /* not worth the threads: scan at each turn */
+    return k;
+  }
+  sc->qs = qs;
+  sc->stamp = stamp;
+  sc->mark = mark;
+  pthread_barrier_wait(&sc->start);
+  scan_share(sc, 0);
+  for (int w = 1; w < sc->nth; w++)
+    if (!sc->started[w]) scan_share(sc, w);
+  pthread_barrier_wait(&sc->done);
+  return k;
+}
+
 static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, void *src, const ix_pass *ix) {
   split_set ret;
   ret.n_clusters = 0;
@@ -730,6 +929,9 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   int *tmp = ix ? (int *)malloc(sizeof(int) * (size_t)n) : NULL;
   triggers tg = {NULL, NULL, NULL, 0};
   const int use_triggers = ix && ix->ix->symmetric;
+  scanner *sc = NULL;
+  if (ix) sc = scanner_create(n, ix->ix->rp[n], NULL, ix);
+  else if (rq == rq_pass && src) sc = scanner_create(n, ((const pass_lists *)src)->rp[n], (const pass_lists *)src, NULL);
   if (use_triggers) {
     tg.cap = (int64_t)n * 32 + 4096;
     tg.px = (int *)malloc(sizeof(int) * (size_t)n);
@@ -766,12 +968,20 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
     }
     int in_union = cnt; /* the union holds queue[0 .. qn) */
     int n_rest = -1;    /* rest[] not built yet for this cluster */
+    int bstart = 0, bend = 0; /* queue[bstart .. bend): the batch sc scanned */
     for (int qi = 0; qi < qn; qi++) {
+      if (sc && qi == bend) {
+        bstart = qi;
+        bend = qi + scan_batch(sc, queue + qi, qn - qi, visited, stamp, mark, in_union, n, minpts);
+      }
       const int q = queue[qi];
       if (!visited[q]) {
         visited[q] = 1;
         const int *nbq = nb;
-        int cq = ix ? ix_count(ix, q) : rq(src, n, q, epsilon, nb, &nbq);
+        const int bk = qi - bstart;
+        const int pre = sc && sc->len[bk] >= 0; /* row scanned with the batch */
+        int cq = pre ? sc->cq[bk] : ix ? ix_count(ix, q) : rq(src, n, q, epsilon, nb, &nbq);
+        if (pre && !ix) nbq = ((const pass_lists *)src)->col + ((const pass_lists *)src)->rp[q];
         if (cq >= minpts) {
           /* the members of N(q) outside the union, in ascending order: by
            * scanning N(q), or -- when N(q) is much longer than the set of
@@ -813,6 +1023,17 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
                   queue[qn++] = x;
                   in_union++;
                 }
+              }
+            }
+          } else if (pre) {
+            /* the batch's scan, less what joined the union since */
+            const int *o = sc->out + sc->off[bk];
+            for (int t = 0; t < sc->len[bk]; t++) {
+              const int x = o[t];
+              if (stamp[x] != mark) {
+                stamp[x] = mark;
+                queue[qn++] = x;
+                in_union++;
               }
             }
           } else if (ix) {
@@ -861,6 +1082,7 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   free(tg.px);
   free(tg.rp);
   free(tg.col);
+  scanner_destroy(sc);
   return ret;
 }
 

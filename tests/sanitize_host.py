@@ -21,7 +21,8 @@ PKG = os.path.join(ROOT, "mnhn-tree-tools_amd")
 TESTS = ["test_dbscan_over_csr_matches_oracle", "test_dbscan_over_vcsr_matches_oracle",
          "test_dbscan_dense_neighbourhoods_match_oracle", "test_dbscan_sweep_matches_oracle",
          "test_dbscan_vcsr_sweep_matches_oracle", "test_dbscan_sweep_long_lists_outside_a_cluster",
-         "test_dbscan_passes_over_whole_lists_match_oracle", "test_dbscan_threaded_pass_lists_match_oracle"]
+         "test_dbscan_passes_over_whole_lists_match_oracle", "test_dbscan_threaded_pass_lists_match_oracle",
+         "test_dbscan_parallel_row_scans_match_oracle"]
 
 CHILD = r"""
 import ctypes, inspect, os, sys

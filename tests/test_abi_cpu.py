@@ -588,3 +588,35 @@ def test_dbscan_passes_over_whole_lists_match_oracle(pkg, oracle):
         got = pkg.dbscan_vcsr(rpv, colv, acc[keep], 1, float(eps), eps, minpts)
         want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_acc(acc, 1))
         assert got == [c.members for c in want.clusters], ("L1", eps, minpts)
+
+
+def test_dbscan_parallel_row_scans_match_oracle(pkg, oracle, monkeypatch):
+    """Lists long enough (> 4M entries) that host threads scan batches of
+    queue rows ahead of the expansion: materialised pass lists, rows read in
+    place, and the sweep's distance-ordered rows all give the oracle's
+    clusters, member order included."""
+    monkeypatch.setenv("MNHN_HOST_THREADS", "7")
+    pos = np.arange(6000, dtype=np.int64)
+    pos[2500:] += 900   # a gap wider than every epsilon below: two clusters at least
+    pos[4200:] += 350   # a gap only the larger epsilons bridge
+    n = len(pos)
+    eps_max = 700
+    rows = [np.nonzero(np.abs(pos - pos[p]) <= eps_max)[0] for p in range(n)]
+    rows = [r[r != p] for p, r in enumerate(rows)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    col = np.concatenate(rows).astype(np.int32)
+    d = np.abs(pos[np.repeat(np.arange(n), np.diff(rp))] - pos[col]).astype(np.int16)
+    assert len(col) > (1 << 22)
+
+    def rq(p, e):
+        lo = np.searchsorted(pos, pos[p] - np.float32(e), side="left")
+        hi = np.searchsorted(pos, pos[p] + np.float32(e), side="right")
+        return list(range(lo, hi))
+
+    eps = [700, 500, 300]
+    want = {e: [c.members for c in oracle.dbscan(n, e, 6, rq).clusters] for e in eps}
+    for e in eps:
+        assert pkg.dbscan_csr(rp, col, d, eps_max, e, 6) == want[e], e
+    for sym in (True, False):
+        got = pkg.dbscan_csr_sweep(rp, col, d, eps_max, eps, 6, symmetric=sym)
+        assert got == [want[e] for e in eps], sym

@@ -866,7 +866,7 @@ def main():
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(seqs, args.cpu_sample)
-        # best effort on the host: every core the box gives this process (at most 16)
+        # best effort on the host: every core this process may run on (at most 16)
         nt = max(1, min(16, len(os.sched_getaffinity(0))))
         out["cpu_baseline_all_cores"] = cpu_baseline(seqs, 3 * args.cpu_sample, threads=nt)
     if not args.no_dbscan and args.config == "tree20k":

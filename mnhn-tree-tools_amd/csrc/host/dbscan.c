@@ -914,6 +914,11 @@ static int scan_batch(scanner *sc, const int *qs, int avail, const unsigned char
   return k;
 }
 
+/* One DBSCAN pass.  N(p) comes from rq (materialised pass lists, or one
+ * one-vs-all launch per query) or, with ix, from stored rows read through an
+ * index pass (distance-ordered rows, or the rows themselves when the pass
+ * keeps every entry).  With long lists, host threads scan batches of queue
+ * rows ahead of the expansion (sc). */
 static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, void *src, const ix_pass *ix) {
   split_set ret;
   ret.n_clusters = 0;

@@ -21,7 +21,9 @@ Also reported:
                   sequences, single thread, as the reference's CPU path runs
                   (cpu_baseline_all_cores: the same on all host cores);
   dbscan       -- wall seconds of the drop-in adaptive_clustering_SW CLI on the
-                  same 20,000 sequences (FASTA in, split-set files out).
+                  same 20,000 sequences (FASTA in, split-set files out);
+                  amplicon105k runs it on its 105,000 sequences (eps from
+                  100 in steps of 100).
 
 Other configurations:
   --config amplicon105k  105,000 amplicon-like sequences, same measurement;
@@ -869,9 +871,10 @@ def main():
         # best effort on the host: every core this process may run on (at most 16)
         nt = max(1, min(16, len(os.sched_getaffinity(0))))
         out["cpu_baseline_all_cores"] = cpu_baseline(seqs, 3 * args.cpu_sample, threads=nt)
-    if not args.no_dbscan and args.config == "tree20k":
+    if not args.no_dbscan:
         print("adaptive DBSCAN run ...", file=sys.stderr, flush=True)
-        out["dbscan"] = run_dbscan(seqs, 100, 300, 5)
+        out["dbscan"] = run_dbscan(seqs, 100, 300, 5) if args.config == "tree20k" else \
+            run_dbscan(seqs, 100, 100, 5)
     print(json.dumps(out), flush=True)
 
 

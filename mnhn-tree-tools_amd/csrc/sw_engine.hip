@@ -1606,6 +1606,30 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
     const double gb = e ? atof(e) : 32.0;
     const unsigned long long budget = (unsigned long long)(gb * 1e9 / 8.0);
     cap = std::max<unsigned long long>(std::min(all, budget), 1024ull);
+    if (all > budget && !e && w.items.size() >= 64) {
+      // the lists may not fit the default buffer: size it from a systematic
+      // sample of ~2 % of the items (keys per cell, times 1.3) instead of
+      // overflowing and computing every pair twice
+      Work smp;
+      smp.qids = w.qids;
+      const size_t stride = 50;
+      for (size_t k = 0; k < w.items.size(); k += stride) smp.items.push_back(w.items[k]);
+      smp.max_q_len = w.max_q_len;
+      double smp_cost = 0, all_cost = 0;
+      for (size_t k = 0; k < w.items.size(); ++k) {
+        const double ck = item_cost(c, w, w.items[k], S);
+        all_cost += ck;
+        if (k % stride == 0) smp_cost += ck;
+      }
+      if ((rc = ensure_keys(c, cap))) return rc;
+      HIP_TRY(hipMemsetAsync(c->d_nkeys, 0, sizeof(unsigned long long), c->stream));
+      rc = launch_tiles(c, smp, MODE_EPS, 1, eps_max, nullptr, c->d_keys, c->keys_cap);
+      if (rc) return rc;
+      unsigned long long sk = 0;
+      HIP_TRY(hipMemcpy(&sk, c->d_nkeys, sizeof(sk), hipMemcpyDeviceToHost));
+      const double est = (double)sk * all_cost / std::max(smp_cost, 1.0) * 1.3;
+      if (est > (double)cap) cap = std::min<unsigned long long>(all, (unsigned long long)est);
+    }
   }
   for (int attempt = 0; attempt < 2; ++attempt) {
     if ((rc = ensure_keys(c, cap))) return rc;

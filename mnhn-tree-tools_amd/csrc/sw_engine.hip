@@ -1606,13 +1606,14 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
     const double gb = e ? atof(e) : 32.0;
     const unsigned long long budget = (unsigned long long)(gb * 1e9 / 8.0);
     cap = std::max<unsigned long long>(std::min(all, budget), 1024ull);
-    if (all > budget && !e && w.items.size() >= 64) {
+    if (all > budget && !e && w.items.size() >= 4096) {
       // the lists may not fit the default buffer: size it from a systematic
-      // sample of ~2 % of the items (keys per cell, times 1.3) instead of
+      // sample of 0.5 % of the items (keys per cell, times 1.3; items are in
+      // cost order, so the sample spans every length class) instead of
       // overflowing and computing every pair twice
       Work smp;
       smp.qids = w.qids;
-      const size_t stride = 50;
+      const size_t stride = 200;
       for (size_t k = 0; k < w.items.size(); k += stride) smp.items.push_back(w.items[k]);
       smp.max_q_len = w.max_q_len;
       double smp_cost = 0, all_cost = 0;

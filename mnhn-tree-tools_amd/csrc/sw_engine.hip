@@ -1606,7 +1606,7 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
     const double gb = e ? atof(e) : 32.0;
     const unsigned long long budget = (unsigned long long)(gb * 1e9 / 8.0);
     cap = std::max<unsigned long long>(std::min(all, budget), 1024ull);
-    if (all > budget && !e && w.items.size() >= 4096) {
+    if (all > budget && w.items.size() >= 4096) {
       // the lists may not fit the default buffer: size it from a systematic
       // sample of 0.5 % of the items (keys per cell, times 1.3; items are in
       // cost order, so the sample spans every length class) instead of

@@ -327,3 +327,23 @@ def test_eps_csr_beyond_row_bitmap_limit(pkg, oracle):
         want = others[d[k] <= eps]
         assert np.array_equal(col[rp[p]:rp[p + 1]], want), p
         assert np.array_equal(dist[rp[p]:rp[p + 1]], d[k][d[k] <= eps].astype(np.int16)), p
+
+
+def test_eps_key_buffer_sized_from_sample(pkg, monkeypatch):
+    """A first epsilon pass whose keys may exceed the starting buffer sizes it
+    from a sample of the work (here the starting buffer holds 1,250 keys):
+    the lists equal those of a context that starts with room for every
+    pair, and a later, larger pass grows the buffer as needed."""
+    rng = np.random.default_rng(91)
+    seqs = _rand_set(rng, 3000, 60, 140)
+    with pkg.Engine(seqs) as eng:
+        want = eng.eps_csr(500)
+        want_wide = eng.eps_csr(700)
+    monkeypatch.setenv("MNHN_SW_KEY_BUDGET_GB", "0.00001")
+    with pkg.Engine(seqs) as eng:
+        got = eng.eps_csr(500)
+        got_wide = eng.eps_csr(700)
+    _say(f"sampled key buffer: nnz {len(got[1])} and {len(got_wide[1])}")
+    assert len(want[1]) > 100000
+    for g, w in ((got, want), (got_wide, want_wide)):
+        assert all(np.array_equal(a, b) for a, b in zip(g, w))

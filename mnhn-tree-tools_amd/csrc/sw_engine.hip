@@ -44,6 +44,7 @@
 #include <string>
 #include <chrono>
 #include <cstdint>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -541,9 +542,12 @@ __device__ __forceinline__ f16x2 clamp01(f16x2 x) {
 
 template <int S>
 __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f16x2 gleft,
-                                           f16x2 gdiag, f16x2 gapv, f16x2 &best) {
+                                           f16x2 gdiag, f16x2 gapv, f16x2 &best, f16x2 &best2) {
   f16x2 dnext = clamp01(gdiag + P[0]);
   f16x2 hprev = best;
+  // two running maxima, alternating every two columns: the compiler sinks
+  // them to the row's end, where one chain of dependent v_pk_maximum3 would
+  // need a wait state between each pair on gfx950 and two chains need none
 #pragma unroll
   for (int jj = 0; jj < S; ++jj) {
     const f16x2 D = dnext;
@@ -552,7 +556,8 @@ __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f
     const f16x2 g = H + gapv;
     G[jj] = g;
     gleft = g;
-    if (jj & 1) best = __builtin_elementwise_maximum(__builtin_elementwise_maximum(best, hprev), H);
+    if ((jj & 3) == 1) best = __builtin_elementwise_maximum(__builtin_elementwise_maximum(best, hprev), H);
+    else if ((jj & 3) == 3) best2 = __builtin_elementwise_maximum(__builtin_elementwise_maximum(best2, hprev), H);
     else hprev = H;
   }
 }
@@ -611,12 +616,13 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
       const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
       const int ncols = la > 1 ? la - 1 : 0;
       const int nstrips = (ncols + S - 1) / S;
-      f16x2 best = zero2;
+      f16x2 best = zero2, best2 = zero2;
 
       // One strip of W query columns whose slot 0 is column j0 (columns < 1
-      // are pre-padding); first: no strip to its left.
-      auto strip = [&](auto wc, int j0, bool first) {
+      // are pre-padding); first (compile-time): no strip to its left.
+      auto strip = [&](auto wc, int j0, auto firstc) {
         constexpr int W = decltype(wc)::value;
+        constexpr bool first = decltype(firstc)::value;
         __builtin_amdgcn_wave_barrier();
         for (int e = lane; e < kPairs * W; e += kWave) {
           const int p = e / W, jj = e - p * W;
@@ -657,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
             P[4 * w + 3] = as_f16x2(v.w);
           }
           const f16x2 gleft = as_f16x2(left_u);
-          dp_row_f16<W>(G, P, gleft, gdiag, gapv, best);
+          dp_row_f16<W>(G, P, gleft, gdiag, gapv, best, best2);
           bnd[(uint32_t)j * kWave + (uint32_t)lane] = f16_u32(G[W - 1]);
           gdiag = gleft;
         };
@@ -666,6 +672,11 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
           bA = bnd[kWave + (uint32_t)lane];
           if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
         }
+        // The boundary loads are unconditional so that both rows of an
+        // iteration share one basic block and the scheduler interleaves their
+        // dependency chains.  Rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay
+        // inside the wave's (maxlen + 2) * 64 workspace; a row past nrows is
+        // loaded but never used.
         for (int blk = 0; blk * 16 < nrows; ++blk) {
           const uint32_t nw_lo = a.words[wo_lo + (uint32_t)blk + 1u];
           const uint32_t nw_hi = a.words[wo_hi + (uint32_t)blk + 1u];
@@ -673,9 +684,9 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
           int j = blk * 16 + 1;
           for (; j + 1 <= jend; j += 2) {
             do_row(j, bA, j > nfast);
-            if (!first && j + 2 <= nrows) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
+            if (!first) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
             do_row(j + 1, bB, j + 1 > nfast);
-            if (!first && j + 3 <= nrows) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
+            if (!first) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
           }
           if (j <= jend) do_row(j, bA, true);
           w_lo = nw_lo;
@@ -688,15 +699,16 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
         // holds them (multiples of 8 columns), so it pads fewer than S/4
         const int rem = ncols - (nstrips - 1) * S;
         if constexpr (S >= 32) {
-          if (rem <= S / 4) strip(Width<S / 4>{}, rem - S / 4 + 1, true);
-          else if (rem <= S / 2) strip(Width<S / 2>{}, rem - S / 2 + 1, true);
-          else if (rem <= 3 * S / 4) strip(Width<3 * S / 4>{}, rem - 3 * S / 4 + 1, true);
-          else strip(Width<S>{}, rem - S + 1, true);
+          if (rem <= S / 4) strip(Width<S / 4>{}, rem - S / 4 + 1, std::true_type{});
+          else if (rem <= S / 2) strip(Width<S / 2>{}, rem - S / 2 + 1, std::true_type{});
+          else if (rem <= 3 * S / 4) strip(Width<3 * S / 4>{}, rem - 3 * S / 4 + 1, std::true_type{});
+          else strip(Width<S>{}, rem - S + 1, std::true_type{});
         } else {
-          strip(Width<S>{}, rem - S + 1, true);
+          strip(Width<S>{}, rem - S + 1, std::true_type{});
         }
-        for (int s = 1; s < nstrips; ++s) strip(Width<S>{}, rem + (s - 1) * S + 1, false);
+        for (int s = 1; s < nstrips; ++s) strip(Width<S>{}, rem + (s - 1) * S + 1, std::false_type{});
       }
+      best = __builtin_elementwise_maximum(best, best2);
       // back to integers: H = 2048 * value; corner H = G + gap at row lb-1
       const float sc = 2048.0f;
       int best_lo = (int)__builtin_rintf((float)best.x * sc);

@@ -123,6 +123,28 @@ __device__ __forceinline__ void dp_row(s16x2 (&H)[S], const s16x2 (&P)[S],
   }
 }
 
+// dp_row with the running maximum split over two accumulators by column
+// parity: the compiler sinks the maxima to the row's end, and one chain of
+// dependent packed ops there needs a wait state between each pair on gfx950
+template <int S>
+__device__ __forceinline__ void dp_row2(s16x2 (&H)[S], const s16x2 (&P)[S],
+                                        s16x2 left, s16x2 diag, s16x2 gapv,
+                                        s16x2 &best, s16x2 &best2) {
+  s16x2 dnext = __builtin_elementwise_add_sat(diag, P[0]);
+#pragma unroll
+  for (int jj = 0; jj < S; ++jj) {
+    const s16x2 D = dnext;
+    if (jj + 1 < S) dnext = __builtin_elementwise_add_sat(H[jj], P[jj + 1]);
+    const s16x2 M = __builtin_elementwise_max(left, H[jj]);
+    const s16x2 E = __builtin_elementwise_add_sat(M, gapv);
+    const s16x2 h = __builtin_elementwise_max(D, E);
+    if (jj & 1) best2 = __builtin_elementwise_max(best2, h);
+    else best = __builtin_elementwise_max(best, h);
+    H[jj] = h;
+    left = h;
+  }
+}
+
 // Generic alphabet: substitution score computed from 16-bit codes per cell.
 template <int S>
 __device__ __forceinline__ s16x2 code_score(uint32_t tc, uint32_t qq, s16x2 pm, s16x2 dmm) {
@@ -421,10 +443,12 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
       const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
       const int ncols = la > 1 ? la - 1 : 0;
       const int nstrips = (ncols + S - 1) / S;
-      s16x2 best = floor2;
+      s16x2 best = floor2, best2 = floor2;
       s16x2 corner = floor2;
 
-      for (int s = 0; s < nstrips && nrows > 0; ++s) {
+      // strip s of the query; first (compile-time): s == 0, no strip to its left
+      auto strip = [&](int s, auto firstc) {
+        constexpr bool first = decltype(firstc)::value;
         const int j0 = ncols - (nstrips - s) * S + 1;  // query column of slot 0
         // ---- score table of this strip
         __builtin_amdgcn_wave_barrier();
@@ -470,14 +494,18 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
             P[4 * w + 3] = as_s16x2(v.w);
           }
           const s16x2 left_b = as_s16x2(left_u);
-          dp_row<S>(H, P, left_b, diag_b, gapv, best);
+          dp_row2<S>(H, P, left_b, diag_b, gapv, best, best2);
           bnd[(uint32_t)j * kWave + (uint32_t)lane] = as_u32(H[S - 1]);
           diag_b = left_b;
         };
         // left-boundary values: even rows in bA, odd rows in bB, each loaded two
-        // rows ahead, so a row never waits on the store issued just before it
+        // rows ahead, so a row never waits on the store issued just before it.
+        // The loads are unconditional so that both rows of an iteration share
+        // one basic block and the scheduler interleaves their dependency
+        // chains; rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
+        // wave's (maxlen + 2) * 64 workspace, and a row past nrows is never used.
         uint32_t bA = as_u32(floor2), bB = as_u32(floor2);
-        if (s > 0) {
+        if (!first) {
           bA = bnd[kWave + (uint32_t)lane];
           if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
         }
@@ -488,9 +516,9 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
           int j = blk * 16 + 1;
           for (; j + 1 <= jend; j += 2) {
             do_row(j, bA);
-            if (s > 0 && j + 2 <= nrows) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
+            if (!first) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
             do_row(j + 1, bB);
-            if (s > 0 && j + 3 <= nrows) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
+            if (!first) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
           }
           if (j <= jend) {  // odd row count: the last row of the strip
             do_row(j, bA);
@@ -498,7 +526,12 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
           w_lo = nw_lo;
           w_hi = nw_hi;
         }
+      };
+      if (nstrips > 0 && nrows > 0) {
+        strip(0, std::true_type{});
+        for (int s = 1; s < nstrips; ++s) strip(s, std::false_type{});
       }
+      best = __builtin_elementwise_max(best, best2);
       if (nstrips > 0 && nrows > 0) {
         // corner H(la-1, lb-1): the last strip's right-most column at row lb-1
         uint32_t cu = as_u32(floor2);

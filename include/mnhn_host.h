@@ -13,7 +13,7 @@
  *   intersection_of_clusters              cluster_io.c:447-501
  *   opencl_initialization/destroy       -> mnhn_gpu_initialization/destroy
  *                                          (dbscan.h:32-37; dbscan.c:97-298)
- *   dbscan_SW_GPU                         dbscan.h:36-37; dbscan.c:950-1047
+ *   dbscan_SW_GPU / dbscan_SW             dbscan.h:30, 36-37; dbscan.c:950-1047
  *   dbscan_L1 / dbscan_L2                 dbscan.h:30-31; dbscan.c:563-780, 950-1047
  *   adaptive_dbscan                       dbscan.h:38-48; dbscan.c:1117-1380
  *   kmer_frequencies / frequencies_from_dataset / write_kmer_base
@@ -113,6 +113,10 @@ void mnhn_gpu_set_lookahead(mnhn_gpu_stuff g, float lookahead);
 
 /* ---- clustering ---- */
 split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g);
+/* dbscan.h:30, the CPU build's scanner (cluster_dbscan_SW.c:62): the same pass
+ * with a device context of its own, so that callers built against the
+ * reference's dbscan.h link against this library unchanged. */
+split_set dbscan_SW(dataset ds, float epsilon, int minpts);
 
 /* One DBSCAN pass (same semantics as dbscan_SW_GPU, without the -1 default
  * warnings) over precomputed epsilon lists, e.g. from mnhn_sw_eps_csr or the
@@ -223,7 +227,7 @@ void print_SW_PCA_L_comparison(FILE *f, dataset ds);
  * No counterpart in the reference, which recomputes every distance per run.
  * File: "MNHNECSR", uint32 version 1, int32 n_rows, int32 eps_max, int32 0,
  * int64 nnz, uint64 data_hash (mnhn_sw_data_hash), int64 row_ptr[n_rows+1],
- * int32 col[nnz], int16 dist[nnz]; little-endian.  Loaded lists are released
+ * int32 col[nnz], uint16 dist[nnz]; little-endian.  Loaded lists are released
  * with mnhn_csr_free.  Both return MNHN_OK or MNHN_ERR_ARG (I/O error, bad or
  * truncated file).  With MNHN_ECSR=<path> in the environment, the DBSCAN
  * entry points load the lists from that file when it matches the data and

@@ -28,8 +28,9 @@ extern "C" {
 #define MNHN_ERR_DEVICE 2    /* HIP runtime / kernel launch error */
 #define MNHN_ERR_NOMEM 3     /* device or host allocation failed */
 #define MNHN_ERR_NODEVICE 4  /* no usable GPU */
-#define MNHN_ERR_RANGE 5     /* a score would not fit the 16-bit lane format, or a
-                                  work list exceeds its 32-bit offsets */
+#define MNHN_ERR_RANGE 5     /* an epsilon list bound above 65535 where some pair's
+                                  distance can exceed 16 bits, or a work list exceeds
+                                  its 32-bit offsets */
 
 /* Scoring parameters.  The reference hard-codes match 4 / mismatch -4 / gap 3
  * (smith-waterman.c:4-14); NULL selects those.  Requires match > 0,
@@ -47,14 +48,15 @@ typedef struct mnhn_sw_ctx mnhn_sw_ctx;
 
 /* epsilon-neighbour lists in CSR form (library-owned; free with
  * mnhn_csr_free).  Row i lists every j != i with d(i, j) <= eps_max, in
- * ascending j, with its distance. */
+ * ascending j, with its distance (unsigned 16-bit: listed distances are at
+ * most 65535, see mnhn_sw_eps_csr). */
 typedef struct {
   int32_t n_rows;
   int32_t eps_max;
   int64_t nnz;
   int64_t *row_ptr; /* [n_rows + 1] */
   int32_t *col;     /* [nnz] */
-  int16_t *dist;    /* [nnz] */
+  uint16_t *dist;   /* [nnz]; eps_max <= 65535 keeps every listed d in range */
 } mnhn_csr;
 
 /* Timing of the last engine call (HIP events on the engine's stream). */
@@ -119,7 +121,9 @@ int mnhn_sw_matrix(mnhn_sw_ctx *ctx, int32_t *dist_out);
 
 /* All-pairs epsilon-neighbour lists with d <= eps_max, computed in one
  * persistent launch over unique pairs and compacted on the device; no n x n
- * matrix is stored.  Feeds every region_query of a DBSCAN pass (and of an
+ * matrix is stored.  eps_max above 65535 is accepted when match * (longest
+ * length) <= 65535 (no distance can exceed it) and returns MNHN_ERR_RANGE
+ * otherwise; distances of any size come from mnhn_sw_one_vs_all / _pairs.  Feeds every region_query of a DBSCAN pass (and of an
  * adaptive epsilon sweep up to eps_max) -- the reference recomputes n
  * distances per region_query (dbscan.c:471-561). */
 int mnhn_sw_eps_csr(mnhn_sw_ctx *ctx, int32_t eps_max, mnhn_csr *out);

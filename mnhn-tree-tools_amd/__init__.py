@@ -24,7 +24,7 @@ BIN_DIR = os.path.join(PKG_DIR, "bin")
 
 MNHN_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of memory",
-          4: "no usable GPU", 5: "score range exceeds the 16-bit lane format"}
+          4: "no usable GPU", 5: "epsilon above the 16-bit list range, or work beyond 32-bit offsets"}
 
 # every entry point declared in include/mnhn_sw.h and include/mnhn_host.h
 ABI_SYMBOLS = [
@@ -49,7 +49,7 @@ HOST_SYMBOLS = [
     "data_not_in_clusters", "intersection_of_clusters",
     "generate_split_set_relation", "free_split_set_and_associated_clusters",
     "mnhn_gpu_initialization", "mnhn_gpu_destroy", "mnhn_gpu_set_lookahead",
-    "dbscan_SW_GPU", "mnhn_dbscan_csr", "mnhn_csr_save", "mnhn_csr_load", "adaptive_dbscan",
+    "dbscan_SW_GPU", "dbscan_SW", "mnhn_dbscan_csr", "mnhn_csr_save", "mnhn_csr_load", "adaptive_dbscan",
     "dataset_to_fasta", "filter_ds_by_size", "filter_by_SW_distance",
     "smith_waterman_distances_matrix", "mean_from_smith_waterman_datasets",
     "sigma_from_smith_waterman_datasets", "silhouette_from_smith_waterman_datasets",
@@ -78,7 +78,7 @@ class Csr(ctypes.Structure):
                 ("nnz", ctypes.c_int64),
                 ("row_ptr", ctypes.POINTER(ctypes.c_int64)),
                 ("col", ctypes.POINTER(ctypes.c_int32)),
-                ("dist", ctypes.POINTER(ctypes.c_int16))]
+                ("dist", ctypes.POINTER(ctypes.c_uint16))]
 
 
 class Vcsr(ctypes.Structure):
@@ -282,7 +282,7 @@ class Engine:
         return int(self.lib.mnhn_sw_data_hash(self.ctx))
 
     def eps_csr(self, eps_max: int):
-        """Returns (row_ptr int64[n+1], col int32[nnz], dist int16[nnz])."""
+        """Returns (row_ptr int64[n+1], col int32[nnz], dist uint16[nnz])."""
         csr = Csr()
         _check(self.lib.mnhn_sw_eps_csr(self.ctx, int(eps_max), ctypes.byref(csr)),
                "mnhn_sw_eps_csr")
@@ -336,7 +336,7 @@ def _csr_to_numpy(lib, csr, n):
     nnz = int(csr.nnz)
     rp = _view(csr.row_ptr, n + 1, owner)
     col = _view(csr.col, nnz, owner) if nnz else np.zeros(0, np.int32)
-    dist = _view(csr.dist, nnz, owner) if nnz else np.zeros(0, np.int16)
+    dist = _view(csr.dist, nnz, owner) if nnz else np.zeros(0, np.uint16)
     return rp, col, dist
 
 
@@ -567,11 +567,11 @@ def dbscan_csr(rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max: int, 
     lib.free_split_set_and_associated_clusters.restype = None
     rp = np.ascontiguousarray(rp, dtype=np.int64)
     col = np.ascontiguousarray(col, dtype=np.int32)
-    dist = np.ascontiguousarray(dist, dtype=np.int16)
+    dist = np.ascontiguousarray(dist, dtype=np.uint16)
     csr = Csr(len(rp) - 1, int(eps_max), len(col),
               rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
               col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
     s = lib.mnhn_dbscan_csr(ctypes.byref(csr), float(eps), int(minpts))
     try:
         out = []
@@ -643,10 +643,10 @@ def dbscan_csr_sweep(rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_max:
     every (p, q, d) is listed as (q, p, d) too, as in the engine's lists."""
     rp = np.ascontiguousarray(rp, dtype=np.int64)
     col = np.ascontiguousarray(col, dtype=np.int32)
-    dist = np.ascontiguousarray(dist, dtype=np.int16)
+    dist = np.ascontiguousarray(dist, dtype=np.uint16)
     csr = Csr(len(rp) - 1, int(eps_max), len(col), rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
               col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
     return _sweep("mnhn_dbscan_csr_sweep", csr, eps, minpts, symmetric)
 
 
@@ -670,11 +670,11 @@ def csr_save(path: str, rp: np.ndarray, col: np.ndarray, dist: np.ndarray, eps_m
     lib = load_library()
     rp = np.ascontiguousarray(rp, dtype=np.int64)
     col = np.ascontiguousarray(col, dtype=np.int32)
-    dist = np.ascontiguousarray(dist, dtype=np.int16)
+    dist = np.ascontiguousarray(dist, dtype=np.uint16)
     csr = Csr(len(rp) - 1, int(eps_max), len(col),
               rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
               col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
     _check(lib.mnhn_csr_save(ctypes.byref(csr), ctypes.c_uint64(data_hash), path.encode()),
            "mnhn_csr_save")
 

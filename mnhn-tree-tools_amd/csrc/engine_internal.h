@@ -133,10 +133,16 @@ void stage_free(mnhn_detail_stage &s);
 // Group the ctx's epsilon keys (c->d_keys, nk of them) into CSR rows.
 int csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out);
 
+// The epsilon bound an all-pairs or prefilter list pass runs with: eps_max,
+// or 65535 when eps_max is larger and no distance can exceed 65535 (keys and
+// lists carry d in 16 bits); MNHN_ERR_RANGE when eps_max is larger and some
+// distance can exceed it.
+int list_bound(const mnhn_sw_ctx *c, int32_t eps_max, int32_t *eff);
+
 // Make sure c->d_keys holds at least cap keys.
 int ensure_keys(mnhn_sw_ctx *c, unsigned long long cap);
 
-enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3 };
+enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3, KERNEL_I32 = 4 };
 enum { MODE_DIST = 0, MODE_EPS = 1 };
 constexpr int kGroupSize = 128;
 

@@ -130,10 +130,27 @@ static int32_t eps_floor(float eps) {
   return (int32_t)floorf(eps);
 }
 
-static void ensure_lists(mnhn_gpu_stuff g, float eps) {
+/* Switch the context to one one-vs-all launch per region query (the
+ * reference's GPU scheme, dbscan.c:471-513): O(n) memory whatever epsilon
+ * and n are. */
+static void use_per_query_launch(mnhn_eps_cache *c, int n) {
+  c->per_query_launch = 1;
+  if (!c->dist_row) c->dist_row = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  if (!c->dist_row) {
+    printf("out of memory for a region query row\n");
+    fflush(stdout);
+    _exit(1);
+  }
+}
+
+/* The stored lists cover eps after this call, or -- when they do not fit in
+ * device or host memory (MNHN_ERR_NOMEM), or eps lies beyond the 16-bit list
+ * distances for these lengths (MNHN_ERR_RANGE) -- the context has switched to
+ * per-query launches (returns 0). */
+static int ensure_lists(mnhn_gpu_stuff g, float eps) {
   mnhn_eps_cache *c = g.cache;
   const int32_t need = eps_floor(eps);
-  if (c->have && c->csr.eps_max >= need) return;
+  if (c->have && c->csr.eps_max >= need) return 1;
   drop_index(c);
   if (c->have) mnhn_csr_free(&c->csr);
   c->have = 0;
@@ -148,13 +165,24 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
       if (fh == hash && c->csr.n_rows == mnhn_sw_size(g.sw) && c->csr.eps_max >= need) {
         fprintf(stderr, "epsilon lists (eps_max %d) read from %s\n", (int)c->csr.eps_max, path);
         c->have = 1;
-        return;
+        return 1;
       }
       mnhn_csr_free(&c->csr);
     }
   }
   const double t0 = now_s();
   int rc = mnhn_sw_eps_csr(g.sw, eps_max, &c->csr);
+  if ((rc == MNHN_ERR_NOMEM || rc == MNHN_ERR_RANGE) && eps_max > need && need >= 0) {
+    /* the lookahead may be what does not fit: lists up to eps alone */
+    eps_max = need;
+    rc = mnhn_sw_eps_csr(g.sw, eps_max, &c->csr);
+  }
+  if (rc == MNHN_ERR_NOMEM || rc == MNHN_ERR_RANGE) {
+    fprintf(stderr, "epsilon lists up to %d: %s; region queries recompute their distances from here on\n",
+            (int)eps_max, mnhn_sw_strerror(rc));
+    use_per_query_launch(c, mnhn_sw_size(g.sw));
+    return 0;
+  }
   if (rc) die_engine("mnhn_sw_eps_csr", rc);
   c->have = 1;
   if (timing_on()) {
@@ -170,6 +198,7 @@ static void ensure_lists(mnhn_gpu_stuff g, float eps) {
     else
       fprintf(stderr, "could not write epsilon lists to %s\n", path);
   }
+  return 1;
 }
 
 /* N(p) for threshold eps into out[] (ascending, p included) with one
@@ -207,7 +236,7 @@ typedef struct {
 typedef struct {
   const int64_t *rp;
   const int32_t *col;
-  const int16_t *dist; /* Smith-Waterman lists, or */
+  const uint16_t *dist; /* Smith-Waterman lists, or */
   const float *val;    /* vector lists */
   float thr;
   int self; /* insert p itself */
@@ -254,7 +283,7 @@ static int host_threads(int64_t work) {
   return (int)t;
 }
 
-static pass_lists build_pass_lists(int n, const int64_t *rp, const int32_t *col, const int16_t *dist,
+static pass_lists build_pass_lists(int n, const int64_t *rp, const int32_t *col, const uint16_t *dist,
                                    const float *val, float thr, int self) {
   pass_lists pl;
   pl.rp = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
@@ -336,10 +365,10 @@ typedef struct {
   int n;
   const int64_t *rp;
   const int32_t *col;  /* stored rows, ascending columns */
-  const int16_t *dist; /* Smith-Waterman lists, or */
+  const uint16_t *dist; /* Smith-Waterman lists, or */
   const float *val;    /* vector lists */
   int32_t *dcol;       /* the same rows ordered by distance */
-  int16_t *ddist;
+  uint16_t *ddist;
   float *dval;
   int symmetric;       /* every stored (p, q, d) has its (q, p, d) */
 } dist_index;
@@ -355,7 +384,7 @@ typedef struct {
   int all;
 } ix_pass;
 
-static dist_index rows_view(int n, const int64_t *rp, const int32_t *col, const int16_t *dist, const float *val,
+static dist_index rows_view(int n, const int64_t *rp, const int32_t *col, const uint16_t *dist, const float *val,
                             int symmetric) {
   dist_index v;
   memset(&v, 0, sizeof(v));
@@ -369,10 +398,10 @@ static dist_index rows_view(int n, const int64_t *rp, const int32_t *col, const 
   return v;
 }
 
-/* sort key whose unsigned order is the order of the distances: signed int16,
- * or float with every NaN last */
+/* sort key whose unsigned order is the order of the distances: the uint16
+ * distance itself, or float with every NaN last */
 static inline uint32_t dist_key(const dist_index *ix, int64_t t) {
-  if (ix->dist) return (uint32_t)(uint16_t)((uint16_t)ix->dist[t] ^ 0x8000u);
+  if (ix->dist) return (uint32_t)ix->dist[t];
   uint32_t u;
   memcpy(&u, &ix->val[t], sizeof(u));
   if ((u & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
@@ -425,7 +454,7 @@ static void *index_worker(void *arg) {
           const int d = ix->dist[b + r];
           const int64_t o = b + hist[d - lo]++;
           ix->dcol[o] = ix->col[b + r];
-          ix->ddist[o] = (int16_t)d;
+          ix->ddist[o] = (uint16_t)d;
         }
         continue;
       }
@@ -494,11 +523,11 @@ static void *index_worker(void *arg) {
 /* Builds the index of stored lists; 0 (and no index) when its copy of the
  * lists would take more than a quarter of the host's memory.  symmetric:
  * the caller vouches that every stored (p, q, d) has its (q, p, d). */
-static int build_dist_index(dist_index *ix, int n, const int64_t *rp, const int32_t *col, const int16_t *dist,
+static int build_dist_index(dist_index *ix, int n, const int64_t *rp, const int32_t *col, const uint16_t *dist,
                             const float *val, int symmetric) {
   memset(ix, 0, sizeof(*ix));
   const int64_t nnz = rp[n];
-  const size_t per = sizeof(int32_t) + (dist ? sizeof(int16_t) : sizeof(float));
+  const size_t per = sizeof(int32_t) + (dist ? sizeof(uint16_t) : sizeof(float));
   const double phys = (double)sysconf(_SC_PHYS_PAGES) * (double)sysconf(_SC_PAGE_SIZE);
   if (phys > 0 && (double)nnz * (double)per > phys / 4) return 0;
   ix->n = n;
@@ -508,7 +537,7 @@ static int build_dist_index(dist_index *ix, int n, const int64_t *rp, const int3
   ix->val = val;
   const size_t cap = (size_t)(nnz ? nnz : 1);
   ix->dcol = (int32_t *)malloc(sizeof(int32_t) * cap);
-  if (dist) ix->ddist = (int16_t *)malloc(sizeof(int16_t) * cap);
+  if (dist) ix->ddist = (uint16_t *)malloc(sizeof(uint16_t) * cap);
   else ix->dval = (float *)malloc(sizeof(float) * cap);
   if (!ix->dcol || (dist && !ix->ddist) || (!dist && !ix->dval)) {
     free(ix->dcol);
@@ -576,7 +605,7 @@ static void drop_index(mnhn_eps_cache *c) {
 /* The index of the cache's stored lists for a sweep, built on first use;
  * NULL outside a sweep or when it does not fit. */
 static const dist_index *cache_index(mnhn_eps_cache *c, int n, const int64_t *rp, const int32_t *col,
-                                     const int16_t *dist, const float *val) {
+                                     const uint16_t *dist, const float *val) {
   if (!c->sweep || !index_allowed()) return NULL;
   if (!c->index_tried) {
     c->index_tried = 1;
@@ -1109,7 +1138,7 @@ static void apply_defaults(float *epsilon, int *minpts) {
  * entry, so the rows are read in place (no filtered copy); otherwise each
  * pass materialises its lists. */
 static split_set stored_pass(int n, float epsilon, int minpts, const int64_t *rp, const int32_t *col,
-                             const int16_t *dist, const float *val, float thr, int self, int all_kept,
+                             const uint16_t *dist, const float *val, float thr, int self, int all_kept,
                              int symmetric) {
   if (all_kept) {
     const dist_index v = rows_view(n, rp, col, dist, val, symmetric);
@@ -1126,11 +1155,8 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
   const int n = ds.n_values;
   apply_defaults(&epsilon, &minpts);
   if (n > 0) {
-    if (g.cache->per_query_launch) {
-      if (!g.cache->dist_row) g.cache->dist_row = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-    } else {
-      ensure_lists(g, epsilon);
-    }
+    if (g.cache->per_query_launch) use_per_query_launch(g.cache, n);
+    else (void)ensure_lists(g, epsilon);
   }
   const double t0 = now_s();
   split_set s;
@@ -1149,6 +1175,15 @@ split_set dbscan_SW_GPU(dataset ds, float epsilon, int minpts, mnhn_gpu_stuff g)
   }
   if (timing_on())
     fprintf(stderr, "[mnhn] dbscan pass eps %g: %d clusters, %.3f s\n", epsilon, s.n_clusters, now_s() - t0);
+  return s;
+}
+
+/* The reference's CPU-build entry point (dbscan.h:30): one pass with a device
+ * context of its own. */
+split_set dbscan_SW(dataset ds, float epsilon, int minpts) {
+  mnhn_gpu_stuff g = mnhn_gpu_initialization(ds);
+  split_set s = dbscan_SW_GPU(ds, epsilon, minpts, g);
+  mnhn_gpu_destroy(g);
   return s;
 }
 
@@ -1233,7 +1268,7 @@ split_set mnhn_dbscan_vcsr(const mnhn_vcsr *csr, float epsilon, int minpts) {
 
 /* Several passes over the same lists: through the distance-ordered index when
  * it fits, else one materialised pass each. */
-static int dbscan_sweep(int n, const int64_t *rp, const int32_t *col, const int16_t *dist, const float *val,
+static int dbscan_sweep(int n, const int64_t *rp, const int32_t *col, const uint16_t *dist, const float *val,
                         int norm, const float *eps, int n_eps, int minpts, int symmetric, split_set *out) {
   if (n_eps < 0 || (n_eps > 0 && (!eps || !out))) return MNHN_ERR_ARG;
   dist_index ix;

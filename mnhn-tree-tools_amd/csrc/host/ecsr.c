@@ -37,7 +37,7 @@ int mnhn_csr_save(const mnhn_csr *csr, uint64_t data_hash, const char *path) {
   h.data_hash = data_hash;
   const size_t nr = (size_t)csr->n_rows + 1, nz = (size_t)csr->nnz;
   int ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(csr->row_ptr, sizeof(int64_t), nr, f) == nr &&
-           fwrite(csr->col, sizeof(int32_t), nz, f) == nz && fwrite(csr->dist, sizeof(int16_t), nz, f) == nz;
+           fwrite(csr->col, sizeof(int32_t), nz, f) == nz && fwrite(csr->dist, sizeof(uint16_t), nz, f) == nz;
   if (fclose(f) != 0) ok = 0;
   return ok ? MNHN_OK : MNHN_ERR_ARG;
 }
@@ -54,9 +54,9 @@ int mnhn_csr_load(const char *path, uint64_t *data_hash, mnhn_csr *out) {
     const size_t nr = (size_t)h.n_rows + 1, nz = (size_t)h.nnz;
     out->row_ptr = (int64_t *)malloc(nr * sizeof(int64_t));
     out->col = (int32_t *)malloc((nz ? nz : 1) * sizeof(int32_t));
-    out->dist = (int16_t *)malloc((nz ? nz : 1) * sizeof(int16_t));
+    out->dist = (uint16_t *)malloc((nz ? nz : 1) * sizeof(uint16_t));
     ok = out->row_ptr && out->col && out->dist && fread(out->row_ptr, sizeof(int64_t), nr, f) == nr &&
-         fread(out->col, sizeof(int32_t), nz, f) == nz && fread(out->dist, sizeof(int16_t), nz, f) == nz;
+         fread(out->col, sizeof(int32_t), nz, f) == nz && fread(out->dist, sizeof(uint16_t), nz, f) == nz;
     /* the row pointers must describe nnz ascending entries */
     for (size_t r = 0; ok && r < nr; ++r)
       if (out->row_ptr[r] < (r ? out->row_ptr[r - 1] : 0) || out->row_ptr[r] > h.nnz) ok = 0;

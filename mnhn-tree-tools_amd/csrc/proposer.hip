@@ -1189,7 +1189,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (stats) *stats = stt;
     return MNHN_OK;
   }
-  int rc = MNHN_OK;
+  int rc = list_bound(c, eps_max, &eps_max);
+  if (rc) return fail(rc);
   (void)hipEventRecord(e0, st);
   if ((rc = profiles_dev(c, &X, nullptr))) return fail(rc);
   stt.profile_ms = phase_ms();
@@ -1288,22 +1289,24 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     // item order by row class: rows whose pairs all fit the exact fp16 form
     // (the row's sequence is the longer of each of its pairs, so
     // match * len <= 2048 suffices), first those short enough for a 32-column
-    // strip, then the rest; then the rows that need the int16 form
+    // strip, then the rest; then the rows that need the int16 form; then the
+    // rows whose scores may pass 16 bits (the 32-bit kernel)
     const int S = context_strip(c);
     auto row_class = [&](int i) {
+      if ((long long)c->match * c->len[i] > 65535) return 3;
       if ((long long)c->match * c->len[i] > 2048) return 2;
       return (S == 64 && c->len[i] <= 128) ? 0 : 1;
     };
     std::vector<int32_t> ib(n);
-    int part_begin[4] = {0, 0, 0, 0};
+    int part_begin[5] = {0, 0, 0, 0, 0};
     {
       int o = 0;
-      for (int cls = 0; cls < 3; ++cls) {
+      for (int cls = 0; cls < 4; ++cls) {
         part_begin[cls] = o;
         for (int i = 0; i < n; ++i)
           if (row_class(i) == cls) { ib[i] = o; o += gb[i + 1] - gb[i]; }
       }
-      part_begin[3] = o;
+      part_begin[4] = o;
     }
     // row segments of the candidates, sorted by target position within rows
     std::vector<long long> rsh((size_t)n + 1, 0);
@@ -1354,10 +1357,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
-      for (int part = 0; part < 3 && rc == MNHN_OK; ++part) {
+      for (int part = 0; part < 4 && rc == MNHN_OK; ++part) {
         w.d_items = items + part_begin[part];
         w.n_items = part_begin[part + 1] - part_begin[part];
-        w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : KERNEL_LDS;
+        w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
         w.dev_f16_ok = part < 2;
         w.dev_strip = part == 0 ? 32 : 0;
         if (w.n_items == 0) continue;

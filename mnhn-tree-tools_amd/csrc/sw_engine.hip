@@ -69,6 +69,9 @@ constexpr int kGroup = 128;          // targets per work item (2 per lane)
 constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr short kFloor = -32768;     // biased representation of H = 0
 constexpr int kBias = 32768;
+constexpr long long kI16ScoreMax = 65535;  // largest score the biased 16-bit halves hold
+constexpr int kI32Strip = 16;              // register strip of the 32-bit kernel
+constexpr size_t kWorkspaceBytes = 8ull << 30;  // strip-boundary workspaces of one launch
 
 
 struct KArgs {
@@ -379,6 +382,166 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
 
       finish_query(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
                    best, corner);
+    }
+  }
+}
+
+// 32-bit form, for pairs whose local score can leave the 16-bit lane range
+// (match * min(la, lb) > 65535: both sequences longer than 16,383 residues at
+// the default scores).  The reference keeps its whole matrix in int32
+// (smith-waterman.c:16-81); here each lane takes its low-half target and then
+// its high-half target, one 32-bit DP at a time.  The query is the
+// wave-uniform DP row, a register strip holds S target columns, and the
+// strip's right-most H column is handed to the next strip through the
+// per-wave workspace [row][lane].  With at most four residue codes the lane
+// keeps the four substitution rows of its strip and the query code selects
+// one per DP row (uniform branch); otherwise each cell compares 16-bit codes.
+template <int S, bool kAcgt>
+__device__ __forceinline__ void dp_row_i32(int (&H)[S], const int (&P)[S], int left, int diag, int gap,
+                                           int &best) {
+  int dnext = diag + P[0];
+#pragma unroll
+  for (int jj = 0; jj < S; ++jj) {
+    const int D = dnext;
+    if (jj + 1 < S) dnext = H[jj] + P[jj + 1];
+    const int E = max(left, H[jj]) - gap;
+    const int h = max(max(D, E), 0);
+    best = max(best, h);
+    H[jj] = h;
+    left = h;
+  }
+}
+
+template <int S, bool kAcgt>
+__global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6));
+  const __attribute__((address_space(4))) uint32_t *cwords =
+      (const __attribute__((address_space(4))) uint32_t *)a.words;
+  uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+  constexpr int kPerWord = kAcgt ? 16 : 2;
+  constexpr int kWords = S / kPerWord;
+
+  for (;;) {
+    int item_idx = 0;
+    if (lane == 0) item_idx = atomicAdd(a.counter, 1);
+    item_idx = __builtin_amdgcn_readfirstlane(item_idx);
+    if (item_idx >= a.n_items) break;
+    const int4 it = a.items[item_idx];
+    const int g = __builtin_amdgcn_readfirstlane(it.x);
+    const int qb = __builtin_amdgcn_readfirstlane(it.y);
+    const int qe = __builtin_amdgcn_readfirstlane(it.z);
+    const int ob = __builtin_amdgcn_readfirstlane(it.w);
+
+    const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
+    const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
+    const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
+    const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
+    const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
+    const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
+    const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
+    const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
+    // strips of each half: its longest target's inner columns 1..lb-1 (a half
+    // without targets runs none)
+    int mlo = lb_lo, mhi = lb_hi;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      mlo = max(mlo, __shfl_xor(mlo, off));
+      mhi = max(mhi, __shfl_xor(mhi, off));
+    }
+    const int ns_lo = __builtin_amdgcn_readfirstlane(mlo > 1 ? (mlo - 1 + S - 1) / S : 0);
+    const int ns_hi = __builtin_amdgcn_readfirstlane(mhi > 1 ? (mhi - 1 + S - 1) / S : 0);
+
+    for (int qi = qb; qi < qe; ++qi) {
+      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
+      const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
+      const int nrows = la > 1 ? la - 1 : 0;
+      int best2[2] = {0, 0}, corner2[2] = {0, 0};
+#pragma unroll 1
+      for (int half = 0; half < 2; ++half) {
+        const int lb = half ? lb_hi : lb_lo;
+        const uint32_t wo = half ? wo_hi : wo_lo;
+        const int nstrips = half ? ns_hi : ns_lo;
+        int best = 0, corner = 0;
+        for (int s = 0; s < nstrips && nrows > 0; ++s) {
+          const int j0 = 1 + s * S;     // DP column of strip slot 0
+          const int cl = lb - 1 - j0;   // strip slot of the target's last inner column
+          uint32_t wd[kWords];
+          const uint32_t wbase = (uint32_t)(s * S) / kPerWord;
+#pragma unroll
+          for (int w = 0; w < kWords; ++w) wd[w] = a.words[wo + wbase + w];
+          int H[S];
+#pragma unroll
+          for (int jj = 0; jj < S; ++jj) H[jj] = 0;
+          int diag_b = 0;
+          int next_b = 0;
+          if (s > 0) next_b = (int)bnd[kWave + (uint32_t)lane];
+          if constexpr (kAcgt) {
+            // the lane's four substitution rows; columns past the target's
+            // end mismatch every query residue
+            int P0[S], P1[S], P2[S], P3[S];
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj) {
+              const uint32_t c = jj <= cl ? (wd[jj / 16] >> (2 * (jj % 16))) & 3u : 4u;
+              P0[jj] = c == 0u ? a.match : a.mismatch;
+              P1[jj] = c == 1u ? a.match : a.mismatch;
+              P2[jj] = c == 2u ? a.match : a.mismatch;
+              P3[jj] = c == 3u ? a.match : a.mismatch;
+            }
+            uint32_t qword = cwords[qwo];
+            uint32_t qnext = cwords[qwo + 1];
+            for (int i = 1; i <= nrows; ++i) {
+              const int left_b = next_b;
+              if (s > 0 && i < nrows) next_b = (int)bnd[(uint32_t)(i + 1) * kWave + (uint32_t)lane];
+              const int r = i - 1;
+              if ((r & 15) == 0 && r > 0) {
+                qword = qnext;
+                qnext = cwords[qwo + (uint32_t)(r >> 4) + 1];
+              }
+              const uint32_t code = (qword >> (2 * (r & 15))) & 3u;
+              switch (code) {
+                case 0: dp_row_i32<S, true>(H, P0, left_b, diag_b, a.gap, best); asm volatile("; i32 row c0"); break;
+                case 1: dp_row_i32<S, true>(H, P1, left_b, diag_b, a.gap, best); asm volatile("; i32 row c1"); break;
+                case 2: dp_row_i32<S, true>(H, P2, left_b, diag_b, a.gap, best); asm volatile("; i32 row c2"); break;
+                default: dp_row_i32<S, true>(H, P3, left_b, diag_b, a.gap, best); asm volatile("; i32 row c3"); break;
+              }
+              if (s + 1 < nstrips) bnd[(uint32_t)i * kWave + (uint32_t)lane] = (uint32_t)H[S - 1];
+              diag_b = left_b;
+            }
+          } else {
+            uint32_t TC[S];
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj)
+              TC[jj] = jj <= cl ? (wd[jj / 2] >> (16 * (jj & 1))) & 0xffffu : 0xffffffffu;
+            uint32_t qword = cwords[qwo];
+            uint32_t qnext = cwords[qwo + 1];
+            for (int i = 1; i <= nrows; ++i) {
+              const int left_b = next_b;
+              if (s > 0 && i < nrows) next_b = (int)bnd[(uint32_t)(i + 1) * kWave + (uint32_t)lane];
+              const int r = i - 1;
+              if ((r & 1) == 0 && r > 0) {
+                qword = qnext;
+                qnext = cwords[qwo + (uint32_t)(r >> 1) + 1];
+              }
+              const uint32_t code = (qword >> (16 * (r & 1))) & 0xffffu;
+              int P[S];
+#pragma unroll
+              for (int jj = 0; jj < S; ++jj) P[jj] = TC[jj] == code ? a.match : a.mismatch;
+              dp_row_i32<S, false>(H, P, left_b, diag_b, a.gap, best);
+              if (s + 1 < nstrips) bnd[(uint32_t)i * kWave + (uint32_t)lane] = (uint32_t)H[S - 1];
+              diag_b = left_b;
+            }
+          }
+          // corner H(la-1, lb-1): the last DP row at the target's last column
+#pragma unroll
+          for (int jj = 0; jj < S; ++jj)
+            if (jj == cl) corner = H[jj];
+        }
+        best2[half] = best;
+        corner2[half] = corner;
+      }
+      finish_query_int(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi, best2[0], best2[1],
+                       corner2[0], corner2[1]);
     }
   }
 }
@@ -773,14 +936,14 @@ __global__ void count_rows_kernel(const unsigned long long *keys, long long n, i
 
 __global__ void scatter_rows_kernel(const unsigned long long *keys, long long n,
                                     const long long *row_ptr, int32_t *cursor,
-                                    int32_t *seg_col, int16_t *seg_d) {
+                                    int32_t *seg_col, uint16_t *seg_d) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x) {
     const unsigned long long key = keys[k];
     const int row = (int)(key >> 40);
     const long long pos = row_ptr[row] + atomicAdd(&cursor[row], 1);
     seg_col[pos] = (int32_t)((key >> 16) & 0xffffffull);
-    seg_d[pos] = (int16_t)(key & 0xffffull);
+    seg_d[pos] = (uint16_t)(key & 0xffffull);
   }
 }
 
@@ -892,6 +1055,9 @@ int pick_strip(const mnhn_sw_ctx *c) {
 }
 
 void *kernel_ptr(int k, int S, bool acgt) {
+  if (k == KERNEL_I32)
+    return acgt ? reinterpret_cast<void *>(&sw_i32_kernel<16, true>)
+                : reinterpret_cast<void *>(&sw_i32_kernel<16, false>);
   if (k == KERNEL_LDS_F16) {
     if (S == 16) return reinterpret_cast<void *>(&sw_lds_f16_kernel<16>);
     if (S == 64) return reinterpret_cast<void *>(&sw_lds_f16_kernel<64>);
@@ -969,20 +1135,26 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   };
   std::vector<Pass> passes;
   if (w.dev) {
-    passes.push_back({(K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K, w.dev_strip ? w.dev_strip : S, {}});
-  } else if (K == KERNEL_LDS_F16) {
-    std::vector<int4> f16, i16;
+    const int DK = (K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K;
+    passes.push_back({DK, DK == KERNEL_I32 ? kI32Strip : (w.dev_strip ? w.dev_strip : S), {}});
+  } else {
+    // items by the lane format their pairs need: a pair's inner scores are at
+    // most match * min(la, lb), so items whose bound exceeds the 16-bit range
+    // run the 32-bit kernel; with the fp16 form, items whose bound is at most
+    // 2048 run exactly in fp16 and the rest in packed int16
+    std::vector<int4> f16, i16, i32;
     for (const int4 &it : w.items) {
       const int gm = own_groups ? w.gmax[it.x] : c->gmax[it.x];
       int maxla = 0;
       for (int qi = it.y; qi < it.z; ++qi) maxla = std::max(maxla, c->len[w.qids[qi]]);
-      const bool ok = (long long)c->match * gm <= 2048 || (long long)c->match * maxla <= 2048;
-      (ok ? f16 : i16).push_back(it);
+      const long long bound = (long long)c->match * std::min(gm, maxla);
+      if (bound > kI16ScoreMax) i32.push_back(it);
+      else if (K == KERNEL_LDS_F16 && bound <= 2048) f16.push_back(it);
+      else i16.push_back(it);
     }
     if (!f16.empty()) passes.push_back({KERNEL_LDS_F16, S, std::move(f16)});
-    if (!i16.empty()) passes.push_back({KERNEL_LDS, S, std::move(i16)});
-  } else {
-    passes.push_back({K, S, w.items});
+    if (!i16.empty()) passes.push_back({K == KERNEL_LDS_F16 ? KERNEL_LDS : K, S, std::move(i16)});
+    if (!i32.empty()) passes.push_back({KERNEL_I32, kI32Strip, std::move(i32)});
   }
   // LDS kernels at S = 64: items whose queries are all at most kShortQuery
   // long run at S = 32 (right-aligned 64-column strips would pad a 90-residue
@@ -1011,7 +1183,11 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   // in the register-profile kernel and target residues in the LDS kernels, so
   // it is sized for the longest sequence of the context.
   const int stride = (std::max(w.max_q_len, c->maxlen) + 2) * kWave;
+  // resident waves are capped so that their workspaces stay within
+  // kWorkspaceBytes (long sequences: 16,384 residues take 4 MB per wave)
+  const size_t ws_waves = std::max<size_t>(kWorkspaceBytes / ((size_t)stride * sizeof(uint32_t)), kBlock / kWave);
   int waves_used = 0;
+  bool any_i32 = false;
   (void)hipEventRecord(c->ev[0], c->stream);
   for (auto &pass : passes) {
     const int PK = pass.kernel, PS = pass.strip;
@@ -1032,6 +1208,8 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
     const int max_useful = (int)((n_items + 3) / 4);
     if (blocks > max_useful) blocks = std::max(1, max_useful);
+    blocks = (int)std::min<size_t>((size_t)blocks, ws_waves / (kBlock / kWave));
+    any_i32 |= PK == KERNEL_I32;
     const int waves = blocks * (kBlock / kWave);
     waves_used = std::max(waves_used, waves);
     const size_t need = (size_t)waves * stride * sizeof(uint32_t);
@@ -1080,7 +1258,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   c->last.total_ms = ms;
   c->last.cells = w.cells;
   c->last.pairs = w.pairs;
-  c->last.lane_bits = 16;
+  c->last.lane_bits = any_i32 ? 32 : 16;
   c->last.strip = S;
   c->last.kernel = K;
   c->last.waves = waves_used;
@@ -1209,13 +1387,21 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
   return w;
 }
 
-int check_range(const mnhn_sw_ctx *c) {
-  // biased 16-bit halves hold H in [0, 65535]; H <= match * min(la, lb)
-  if ((long long)c->match * (c->maxlen + 1) > 65535) return MNHN_ERR_RANGE;
-  return MNHN_OK;
-}
 
 }  // namespace
+
+// Epsilon-list keys carry d in 16 bits, so the lists hold d <= 65535.  Every
+// distance is at most match * max(la, lb): when that bound is within 16 bits
+// a larger eps_max lists the same pairs as 65535 and is lowered to it;
+// otherwise it is refused (a per-query pass, mnhn_sw_one_vs_all, returns any
+// distance in int32).
+int mnhn_detail::list_bound(const mnhn_sw_ctx *c, int32_t eps_max, int32_t *eff) {
+  *eff = eps_max;
+  if (eps_max <= kI16ScoreMax) return MNHN_OK;
+  if ((long long)c->match * c->maxlen > kI16ScoreMax) return MNHN_ERR_RANGE;
+  *eff = (int32_t)kI16ScoreMax;
+  return MNHN_OK;
+}
 
 extern "C" {
 
@@ -1226,7 +1412,7 @@ const char *mnhn_sw_strerror(int code) {
     case MNHN_ERR_DEVICE: return "HIP runtime error";
     case MNHN_ERR_NOMEM: return "out of memory";
     case MNHN_ERR_NODEVICE: return "no usable GPU";
-    case MNHN_ERR_RANGE: return "score range exceeds the 16-bit lane format, or work exceeds 32-bit offsets";
+    case MNHN_ERR_RANGE: return "epsilon above the 16-bit list range for these lengths, or work exceeds 32-bit offsets";
     default: return "unknown error";
   }
 }
@@ -1404,8 +1590,7 @@ int mnhn_sw_set_tuning(mnhn_sw_ctx *c, int kernel, int strip, int waves_per_cu) 
 
 int mnhn_sw_one_vs_all(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
   if (!c || !dist_out || query < 0 || query >= c->n) return MNHN_ERR_ARG;
-  int rc = check_range(c);
-  if (rc) return rc;
+  int rc = MNHN_OK;
   const int S = pick_strip(c);
   Work w;
   w.qids.push_back(query);
@@ -1440,8 +1625,7 @@ int mnhn_sw_block(mnhn_sw_ctx *c, const int32_t *rows, int nr, const int32_t *co
   for (int k = 0; k < nc; ++k)
     if (cols[k] < 0 || cols[k] >= c->n) return MNHN_ERR_ARG;
   if (nr == 0 || nc == 0) return MNHN_OK;
-  int rc = check_range(c);
-  if (rc) return rc;
+  int rc = MNHN_OK;
   const int S = pick_strip(c);
   // targets: the columns by length (longest first) in groups of 128; queries:
   // the rows by length (longest first)
@@ -1534,8 +1718,7 @@ int mnhn_sw_pairs(mnhn_sw_ctx *c, const int32_t *ij, int64_t m, int32_t *dist_ou
   for (int64_t k = 0; k < 2 * m; ++k)
     if (ij[k] < 0 || ij[k] >= c->n) return MNHN_ERR_ARG;
   if (m == 0) return MNHN_OK;
-  int rc = check_range(c);
-  if (rc) return rc;
+  int rc = MNHN_OK;
   const int S = pick_strip(c);
   // group pairs by first index; the first index is the wave-uniform query
   std::vector<int64_t> idx(m);
@@ -1594,8 +1777,7 @@ int mnhn_sw_matrix(mnhn_sw_ctx *c, int32_t *dist_out) {
   if (!c || (c->n > 0 && !dist_out)) return MNHN_ERR_ARG;
   const int n = c->n;
   if (n == 0) return MNHN_OK;
-  int rc = check_range(c);
-  if (rc) return rc;
+  int rc = MNHN_OK;
   const int S = pick_strip(c);
   Work w = triangle_work(c, S, -1);
   finalize_items(c, w, S, true);
@@ -1641,8 +1823,19 @@ bool length_prune() {
   return !(e && e[0] == '0');
 }
 
-int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
-  int rc = check_range(c);
+// Largest number of list entries (keys) a pass may hold; more returns
+// MNHN_ERR_NOMEM (the DBSCAN host then recomputes per region query).
+// MNHN_SW_LIST_MAX_ENTRIES sets it; by default only device and host memory
+// bound the lists.
+unsigned long long list_entry_cap() {
+  const char *e = getenv("MNHN_SW_LIST_MAX_ENTRIES");
+  if (e && *e) return strtoull(e, nullptr, 10);
+  return ~0ull;
+}
+
+int eps_pass(mnhn_sw_ctx *c, int32_t eps_req, unsigned long long *nkeys_out) {
+  int32_t eps_max = eps_req;
+  int rc = list_bound(c, eps_req, &eps_max);
   if (rc) return rc;
   const int S = pick_strip(c);
   Work w = triangle_work(c, S, length_prune() ? eps_max : -1);
@@ -1680,7 +1873,9 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
       if (est > (double)cap) cap = std::min<unsigned long long>(all, (unsigned long long)est);
     }
   }
+  const unsigned long long max_entries = list_entry_cap();
   for (int attempt = 0; attempt < 2; ++attempt) {
+    cap = std::max<unsigned long long>(std::min(cap, max_entries), 1);
     if ((rc = ensure_keys(c, cap))) return rc;
     HIP_TRY(hipMemsetAsync(c->d_nkeys, 0, sizeof(unsigned long long), c->stream));
     rc = launch_tiles(c, w, MODE_EPS, 1, eps_max, nullptr, c->d_keys, c->keys_cap);
@@ -1691,6 +1886,7 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_max, unsigned long long *nkeys_out) {
       *nkeys_out = nk;
       return MNHN_OK;
     }
+    if (nk > max_entries) return MNHN_ERR_NOMEM;
     cap = nk;  // overflowed: grow to the exact size and recompute once
   }
   return MNHN_ERR_NOMEM;
@@ -1746,12 +1942,12 @@ namespace {
 // row<<24 | col with d as payload, are sorted over their 24 + b significant
 // bits (n < 2^b), checked on the device, and cut into rows by binary search.
 __global__ void split_keys_kernel(const unsigned long long *__restrict__ keys, long long nk,
-                                  unsigned long long *__restrict__ k_out, int16_t *__restrict__ d_out) {
+                                  unsigned long long *__restrict__ k_out, uint16_t *__restrict__ d_out) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nk;
        k += (long long)gridDim.x * blockDim.x) {
     const unsigned long long key = keys[k];
     k_out[k] = key >> 16;
-    d_out[k] = (int16_t)(key & 0xffffull);
+    d_out[k] = (uint16_t)(key & 0xffffull);
   }
 }
 
@@ -1790,7 +1986,7 @@ int csr_from_keys_radix(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
   int b = 1;
   while ((1ll << b) <= (long long)n) ++b;
   unsigned long long *d_k0 = nullptr, *d_k1 = nullptr, *d_flag = nullptr;
-  int16_t *d_d0 = nullptr, *d_d1 = nullptr;
+  uint16_t *d_d0 = nullptr, *d_d1 = nullptr;
   long long *d_rp = nullptr;
   void *d_tmp = nullptr;
   auto cleanup = [&]() {
@@ -1812,7 +2008,7 @@ int csr_from_keys_radix(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
     e = hipGetLastError();
   }
   rocprim::double_buffer<unsigned long long> kbuf(d_k0, d_k1);
-  rocprim::double_buffer<int16_t> dbuf(d_d0, d_d1);
+  rocprim::double_buffer<uint16_t> dbuf(d_d0, d_d1);
   size_t tmp_bytes = 0;
   if (e == hipSuccess && nk > 0)
     e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kbuf, dbuf, (size_t)nk, 0u, (unsigned)(24 + b), c->stream);
@@ -1846,11 +2042,11 @@ int csr_from_keys_radix(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
   if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
   if (e == hipSuccess) {
     out->col = (int32_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
-    out->dist = (int16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
+    out->dist = (uint16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(uint16_t));
     if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
     rc = copy_d2h(c, out->row_ptr, d_rp, ((size_t)n + 1) * sizeof(long long));
     if (rc == MNHN_OK && nk) rc = copy_d2h(c, out->col, d_col, nk * sizeof(int32_t));
-    if (rc == MNHN_OK && nk) rc = copy_d2h(c, out->dist, dbuf.current(), nk * sizeof(int16_t));
+    if (rc == MNHN_OK && nk) rc = copy_d2h(c, out->dist, dbuf.current(), nk * sizeof(uint16_t));
     if (rc != MNHN_OK) e = hipErrorUnknown;
   }
   cleanup();
@@ -1866,6 +2062,13 @@ int csr_from_keys_radix(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
 int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *out) {
   const int n = c->n;
   int rc = MNHN_OK;
+  // host lists take 6 bytes per entry: refuse lists beyond half of the host's
+  // physical memory instead of running it out
+  const double phys = (double)sysconf(_SC_PHYS_PAGES) * (double)sysconf(_SC_PAGE_SIZE);
+  if (phys > 0 && (double)nk * (sizeof(int32_t) + sizeof(uint16_t)) > 0.5 * phys) {
+    mnhn_csr_free(out);
+    return MNHN_ERR_NOMEM;
+  }
   const double kernel_ms = c->last.kernel_ms;
   // LDS of the long-row path: column bitmap + per-word prefix + thread totals
   const size_t lds_bytes = std::max<size_t>(
@@ -1873,7 +2076,7 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
   const char *form = getenv("MNHN_CSR_SORT");
   if (lds_bytes > 160 * 1024 || (form && strcmp(form, "radix") == 0)) return csr_from_keys_radix(c, nk, out);
   int32_t *d_cnt = nullptr, *d_seg_col = nullptr, *d_col = nullptr;
-  int16_t *d_seg_d = nullptr, *d_dist = nullptr;
+  uint16_t *d_seg_d = nullptr, *d_dist = nullptr;
   long long *d_rp = nullptr;
   auto cleanup = [&]() {
     dev_free(d_cnt); dev_free(d_seg_col); dev_free(d_col); dev_free(d_seg_d); dev_free(d_dist);
@@ -1909,10 +2112,10 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
     e = hipGetLastError();
   }
   if (e == hipSuccess && lds_bytes > 64 * 1024)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel<int16_t>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sort_rows_kernel<uint16_t>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e == hipSuccess && nk > 0) {
-    hipLaunchKernelGGL(sort_rows_kernel<int16_t>, dim3((unsigned)n), dim3(kSortThreads), lds_bytes,
+    hipLaunchKernelGGL(sort_rows_kernel<uint16_t>, dim3((unsigned)n), dim3(kSortThreads), lds_bytes,
                        c->stream, d_rp, d_seg_col, d_seg_d, d_col, d_dist, n);
     e = hipGetLastError();
   }
@@ -1922,10 +2125,10 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
   if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
   if (e == hipSuccess) {
     out->col = (int32_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
-    out->dist = (int16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int16_t));
+    out->dist = (uint16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(uint16_t));
     if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
     rc = copy_d2h(c, out->col, d_col, nk * sizeof(int32_t));
-    if (rc == MNHN_OK) rc = copy_d2h(c, out->dist, d_dist, nk * sizeof(int16_t));
+    if (rc == MNHN_OK) rc = copy_d2h(c, out->dist, d_dist, nk * sizeof(uint16_t));
     if (rc != MNHN_OK) e = hipErrorUnknown;
   }
   cleanup();
@@ -2125,8 +2328,7 @@ void write_tune_cache(const std::string &key, const Cand &t, double gcups) {
 
 extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *out) {
   if (!c) return MNHN_ERR_ARG;
-  int rc = check_range(c);
-  if (rc) return rc;
+  int rc = MNHN_OK;
   const std::string key = tune_key(c);
   Cand best{0, 0, 0};
   double best_gcups = 0;

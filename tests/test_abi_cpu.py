@@ -41,7 +41,7 @@ def test_library_loads_and_exports_every_symbol(pkg):
     for name in pkg.ABI_SYMBOLS + pkg.HOST_SYMBOLS + pkg.PROPOSER_SYMBOLS + pkg.VEC_SYMBOLS:
         assert hasattr(lib, name), name
     assert lib.mnhn_sw_strerror(0) == b"ok"
-    assert lib.mnhn_sw_strerror(5).startswith(b"score range")
+    assert lib.mnhn_sw_strerror(5).startswith(b"epsilon above")
 
 
 def test_create_rejects_bad_arguments(pkg):
@@ -245,7 +245,7 @@ def test_dbscan_over_csr_matches_oracle(pkg, oracle):
     keep = (dist <= eps_max) & ~np.eye(n, dtype=bool)
     rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
     col = np.nonzero(keep)[1].astype(np.int32)
-    d = dist[keep].astype(np.int16)
+    d = dist[keep].astype(np.uint16)
     for eps, minpts in ((100, 3), (250.5, 5), (400, 2), (500, 10), (0, 0)):
         got = pkg.dbscan_csr(rp, col, d, eps_max, eps, minpts)
         want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(dist))
@@ -260,7 +260,7 @@ def test_ecsr_roundtrip_and_rejects_damaged_files(pkg, tmp_path):
     counts = rng.integers(0, 7, n)
     rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     col = np.concatenate([np.sort(rng.choice(n, c, replace=False)) for c in counts]).astype(np.int32)
-    dist = rng.integers(0, 3000, len(col)).astype(np.int16)
+    dist = rng.integers(0, 3000, len(col)).astype(np.uint16)
     path = str(tmp_path / "a.ecsr")
     pkg.csr_save(path, rp, col, dist, 2999, 0x1234567890ABCDEF)
     r2, c2, d2, e2, h2 = pkg.csr_load(path)
@@ -274,7 +274,7 @@ def test_ecsr_roundtrip_and_rejects_damaged_files(pkg, tmp_path):
         with pytest.raises(pkg.MnhnError):
             pkg.csr_load(str(bad))
     # empty lists
-    pkg.csr_save(path, np.zeros(4, np.int64), np.zeros(0, np.int32), np.zeros(0, np.int16), 10, 7)
+    pkg.csr_save(path, np.zeros(4, np.int64), np.zeros(0, np.int32), np.zeros(0, np.uint16), 10, 7)
     r3, c3, d3, e3, h3 = pkg.csr_load(path)
     assert len(r3) == 4 and len(c3) == 0 and (e3, h3) == (10, 7)
 
@@ -411,7 +411,7 @@ def test_dbscan_threaded_pass_lists_match_oracle(pkg, oracle, monkeypatch):
     assert keep.sum() > (1 << 20)
     rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
     col = np.nonzero(keep)[1].astype(np.int32)
-    dist = d[keep].astype(np.int16)
+    dist = d[keep].astype(np.uint16)
     for eps, minpts in ((60, 4), (150.5, 10), (-2.0, 3)):
         got = pkg.dbscan_csr(rp, col, dist, eps_max, eps, minpts)
         want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(d))
@@ -440,7 +440,7 @@ def test_dbscan_dense_neighbourhoods_match_oracle(pkg, oracle):
     keep = (d <= 5000) & ~np.eye(n, dtype=bool)
     rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
     col = np.nonzero(keep)[1].astype(np.int32)
-    dist = d[keep].astype(np.int16)
+    dist = d[keep].astype(np.uint16)
     for eps, minpts in ((900, 5), (600, 50), (400, 5), (1500, 2)):
         got = pkg.dbscan_csr(rp, col, dist, 5000, eps, minpts)
         want = oracle.dbscan(n, eps, minpts, oracle.region_query_from_matrix(d))
@@ -450,7 +450,7 @@ def test_dbscan_dense_neighbourhoods_match_oracle(pkg, oracle):
 def _lists_from_matrix(d, bound):
     keep = (d <= bound) & ~np.eye(len(d), dtype=bool)
     rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
-    return rp, np.nonzero(keep)[1].astype(np.int32), d[keep].astype(np.int16)
+    return rp, np.nonzero(keep)[1].astype(np.int32), d[keep].astype(np.uint16)
 
 
 def test_dbscan_sweep_matches_oracle(pkg, oracle, monkeypatch):
@@ -539,6 +539,36 @@ def test_dbscan_vcsr_sweep_matches_oracle(pkg, oracle):
         assert g == [c.members for c in want.clusters], e
 
 
+def test_dbscan_lists_with_distances_beyond_32767(pkg, oracle, tmp_path):
+    """List distances are unsigned 16-bit: pairs with d in [32768, 65535]
+    (sequences of 8,192 bp and more) keep their epsilon test and their place
+    in the distance order, in single passes, sweeps and .ecsr files.  A signed
+    16-bit distance would turn every such pair into a neighbour at any
+    epsilon and put it first in the distance order."""
+    rng = np.random.default_rng(41)
+    n = 500
+    centre = rng.integers(0, 12, n)
+    d = np.abs(centre[:, None] - centre[None, :]) * 2700 + 30000 + rng.integers(0, 1500, (n, n))
+    d = np.triu(d, 1)
+    d = (d + d.T).astype(np.int64)
+    assert d.max() <= 65535 and (d >= 32768).mean() > 0.5
+    rp, col, dist = _lists_from_matrix(d, 65535)
+    assert dist.dtype == np.uint16 and int(dist.max()) == int(d.max())
+    eps = [31000.0, 31500.5, 34000, 40000, 65535]
+    for minpts in (3, 12):
+        for e in eps:
+            want = oracle.dbscan(n, e, minpts, oracle.region_query_from_matrix(d))
+            assert pkg.dbscan_csr(rp, col, dist, 65535, e, minpts) == [c.members for c in want.clusters], (e, minpts)
+        got = pkg.dbscan_csr_sweep(rp, col, dist, 65535, eps, minpts)
+        for e, g in zip(eps, got):
+            want = oracle.dbscan(n, e, minpts, oracle.region_query_from_matrix(d))
+            assert g == [c.members for c in want.clusters], (e, minpts)
+    path = str(tmp_path / "wide.ecsr")
+    pkg.csr_save(path, rp, col, dist, 65535, 99)
+    r2, c2, d2, e2, _ = pkg.csr_load(path)
+    assert np.array_equal(r2, rp) and np.array_equal(c2, col) and np.array_equal(d2, dist) and e2 == 65535
+
+
 def test_dbscan_sweep_long_lists_outside_a_cluster(pkg, oracle):
     """Points on a line with a sparse tail: when the cluster has almost
     everything, the points still outside have long lists (binary search) or
@@ -551,7 +581,7 @@ def test_dbscan_sweep_long_lists_outside_a_cluster(pkg, oracle):
     rows = [r[r != p] for p, r in enumerate(rows)]
     rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
     col = np.concatenate(rows).astype(np.int32)
-    d = np.abs(pos[np.repeat(np.arange(n), np.diff(rp))] - pos[col]).astype(np.int16)
+    d = np.abs(pos[np.repeat(np.arange(n), np.diff(rp))] - pos[col]).astype(np.uint16)
 
     def rq(p, e):
         lo = np.searchsorted(pos, pos[p] - np.float32(e), side="left")
@@ -605,7 +635,7 @@ def test_dbscan_parallel_row_scans_match_oracle(pkg, oracle, monkeypatch):
     rows = [r[r != p] for p, r in enumerate(rows)]
     rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
     col = np.concatenate(rows).astype(np.int32)
-    d = np.abs(pos[np.repeat(np.arange(n), np.diff(rp))] - pos[col]).astype(np.int16)
+    d = np.abs(pos[np.repeat(np.arange(n), np.diff(rp))] - pos[col]).astype(np.uint16)
     assert len(col) > (1 << 22)
 
     def rq(p, e):

@@ -33,6 +33,7 @@ def _run(name, meta, tmp_path, env_extra=None):
     assert got == sorted(files)
     for fn, data in files.items():
         assert (tmp_path / fn).read_bytes() == data, fn
+    return r
 
 
 @pytest.mark.parametrize("name,meta", CASES, ids=[c[0] for c in CASES])
@@ -45,6 +46,20 @@ def test_cli_per_query_launch_mode(name, tmp_path):
     """The reference-shaped flow: one one-vs-all launch per region query."""
     meta = dict(cli_cases())[name]
     _run(name, meta, tmp_path, {"MNHN_REGION_QUERY": "launch"})
+
+
+@pytest.mark.parametrize("name,cap", [("adaptive_c1small", "0"), ("adaptive_c1small", "1000"),
+                                      ("adaptive_c1small", "20000"), ("dbscan_c1full_e400", "0")])
+def test_cli_lists_beyond_memory_fall_back(name, cap, tmp_path):
+    """Epsilon lists that do not fit (forced with MNHN_SW_LIST_MAX_ENTRIES)
+    switch the run to one one-vs-all launch per region query instead of
+    exiting, as the reference recomputes a row per query: stdout and files
+    stay byte-identical.  On the 320-sequence sweep (eps 100 + 60 k) a cap of
+    1000 keeps the lists at eps 100 and falls back from eps 160 on, 20000
+    keeps the lists up to eps 580 and falls back at eps 640."""
+    meta = dict(cli_cases())[name]
+    r = _run(name, meta, tmp_path, {"MNHN_SW_LIST_MAX_ENTRIES": cap})
+    assert b"region queries recompute their distances" in r.stderr
 
 
 def test_split_set_no_truncate(tmp_path):
@@ -164,6 +179,33 @@ def test_reference_cli_with_score_bound_to_engine(name, meta, tmp_path):
     r = subprocess.run(argv, cwd=tmp_path, capture_output=True, timeout=300)
     assert r.returncode == meta["returncode"], r.stderr.decode()[-2000:]
     assert r.stdout == stdout
+    for fn, data in files.items():
+        assert (tmp_path / fn).read_bytes() == data, fn
+
+
+LIB_CASES = [c for c in cli_cases() if c[0] in ("dbscan_c1full_e400", "dbscan_c1small_e400", "dbscan_c1small_e160",
+                                                "dbscan_test2")]
+
+
+@pytest.mark.parametrize("name,meta", LIB_CASES, ids=[c[0] for c in LIB_CASES])
+def test_reference_cli_linked_against_library_alone(name, meta, tmp_path):
+    """The reference's cluster_dbscan_SW main (CPU build, calling
+    dbscan_SW(dataset, float, int), dbscan.h:30) compiled from its source and
+    linked with -lmnhn_sw only -- no reference object -- reproduces the
+    reference's stdout and cluster files."""
+    exe = os.path.join(REF_DIR_BUILT, "cluster_dbscan_SW_mnhn_lib")
+    if not os.path.exists(exe):
+        pytest.skip("library-linked reference CLI not built (make -C oracle relink)")
+    stdout, files = read_case_files(name, meta)
+    fasta = tmp_path / "in.fasta"
+    shutil.copyfile(os.path.join(GOLDEN, "inputs", meta["input"]), fasta)
+    print(f"library-linked reference case {name}", flush=True)
+    r = subprocess.run([exe, str(fasta)] + meta["args"] + [meta["prefix"]], cwd=tmp_path, capture_output=True,
+                       timeout=300)
+    assert r.returncode == meta["returncode"], r.stderr.decode()[-2000:]
+    assert r.stdout == stdout
+    got = sorted(f for f in os.listdir(tmp_path) if f.startswith(meta["prefix"]))
+    assert got == sorted(files)
     for fn, data in files.items():
         assert (tmp_path / fn).read_bytes() == data, fn
 

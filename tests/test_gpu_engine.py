@@ -326,7 +326,7 @@ def test_eps_csr_beyond_row_bitmap_limit(pkg, oracle):
         others = all_j[all_j != p]
         want = others[d[k] <= eps]
         assert np.array_equal(col[rp[p]:rp[p + 1]], want), p
-        assert np.array_equal(dist[rp[p]:rp[p + 1]], d[k][d[k] <= eps].astype(np.int16)), p
+        assert np.array_equal(dist[rp[p]:rp[p + 1]], d[k][d[k] <= eps].astype(np.uint16)), p
 
 
 def test_eps_key_buffer_sized_from_sample(pkg, monkeypatch):
@@ -347,3 +347,55 @@ def test_eps_key_buffer_sized_from_sample(pkg, monkeypatch):
     assert len(want[1]) > 100000
     for g, w in ((got, want), (got_wide, want_wide)):
         assert all(np.array_equal(a, b) for a, b in zip(g, w))
+
+
+def _long_set(rng, alphabet):
+    """Sequences of 9,000-20,000 residues: mutated copies of one ancestor
+    (related pairs score far above 2^16), an unrelated one and a short one."""
+    alpha = np.frombuffer(alphabet, np.uint8)
+    anc = alpha[rng.integers(0, len(alpha), 20000)]
+    seqs = []
+    for L, p in ((20000, 0.02), (17200, 0.03), (16700, 0.05), (12000, 0.04), (9000, 0.02)):
+        s = anc[:L].copy()
+        flip = rng.random(L) < p
+        s[flip] = alpha[rng.integers(0, len(alpha), int(flip.sum()))]
+        seqs.append(s.tobytes())
+    seqs.append(alpha[rng.integers(0, len(alpha), 18000)].tobytes())
+    seqs.append(anc[5000:5300].tobytes())
+    return seqs
+
+
+@pytest.mark.parametrize("alphabet", [b"ACGT", b"ACGTN"])
+def test_long_sequences_run_the_32bit_path(pkg, oracle, alphabet):
+    """Pairs of sequences longer than 16,383 residues (match * min(la, lb) >
+    65535, beyond the 16-bit lanes) run the 32-bit kernel: the matrix, pairs,
+    one-vs-all rows and epsilon lists up to 65535 equal the oracle's
+    (distances up to 80,000), epsilon lists beyond 65535 are refused with
+    MNHN_ERR_RANGE, and a DBSCAN pass over the lists equals the oracle's."""
+    rng = np.random.default_rng(77 + len(alphabet))
+    seqs = _long_set(rng, alphabet)
+    n = len(seqs)
+    _say(f"long sequences ({alphabet.decode()}): lengths {[len(s) for s in seqs]}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    assert want.max() > 65535 and (want[want > 0] < 40000).any()
+    with pkg.Engine(seqs) as eng:
+        assert eng.alphabet == (4 if alphabet == b"ACGT" else 0)
+        got = eng.matrix()
+        assert eng.last_timing()["lane_bits"] == 32
+        assert np.array_equal(got, want)
+        ij = np.array([[0, 1], [2, 1], [5, 0], [6, 0], [3, 4], [1, 1]], np.int32)
+        assert np.array_equal(eng.pairs(ij), want[ij[:, 0], ij[:, 1]])
+        for q in (1, 6):
+            assert np.array_equal(eng.one_vs_all(q), want[q])
+        for eps in (40000, 65535):
+            rp, col, dist = eng.eps_csr(eps)
+            for p in range(n):
+                others = np.array([j for j in range(n) if j != p])
+                keep = want[p, others] <= eps
+                assert np.array_equal(col[rp[p]:rp[p + 1]], others[keep]), (eps, p)
+                assert np.array_equal(dist[rp[p]:rp[p + 1]], want[p, others][keep].astype(np.uint16)), (eps, p)
+            for e in (30000.5, eps):
+                want_cl = oracle.dbscan(n, e, 1, oracle.region_query_from_matrix(want))
+                assert pkg.dbscan_csr(rp, col, dist, eps, e, 1) == [c.members for c in want_cl.clusters], e
+        with pytest.raises(pkg.MnhnError):
+            eng.eps_csr(70000)

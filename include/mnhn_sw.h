@@ -65,12 +65,18 @@ typedef struct {
   double total_ms;    /* kernel + epsilon-list compaction/sort on device */
   double cells;       /* algorithmic cells: sum of la*lb over computed pairs */
   double pairs;       /* pairs computed */
-  int32_t lane_bits;  /* 16 (packed) or 32 */
+  int32_t lane_bits;  /* 16 (packed), or 32 when some pair ran the 32-bit form */
   int32_t kernel;     /* 1 = register profile, 2 = LDS profile (int16),
                          3 = LDS profile, exact fp16 where it applies */
   int32_t strip;      /* register strip width used */
   int32_t waves;      /* persistent waves launched */
   int32_t items;      /* work items in the queue */
+  int32_t reserved;
+  /* per lane format: [0] exact fp16 form, [1] packed int16 forms, [2] 32-bit
+   * form -- the cells each computed (sum of la*lb over its pairs) and its
+   * kernel time (HIP events around its launch) */
+  double form_cells[3];
+  double form_ms[3];
 } mnhn_sw_timing;
 
 /* Upload n sequences (bytes as read from FASTA; not necessarily

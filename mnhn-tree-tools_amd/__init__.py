@@ -96,7 +96,8 @@ class Timing(ctypes.Structure):
                 ("cells", ctypes.c_double), ("pairs", ctypes.c_double),
                 ("lane_bits", ctypes.c_int32), ("kernel", ctypes.c_int32),
                 ("strip", ctypes.c_int32),
-                ("waves", ctypes.c_int32), ("items", ctypes.c_int32)]
+                ("waves", ctypes.c_int32), ("items", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("form_cells", ctypes.c_double * 3), ("form_ms", ctypes.c_double * 3)]
 
 
 class Tuning(ctypes.Structure):
@@ -297,7 +298,10 @@ class Engine:
     def last_timing(self) -> dict:
         t = Timing()
         _check(self.lib.mnhn_sw_last_timing(self.ctx, ctypes.byref(t)), "last_timing")
-        return {f: getattr(t, f) for f, _ in Timing._fields_}
+        out = {f: getattr(t, f) for f, _ in Timing._fields_ if f != "reserved"}
+        for f in ("form_cells", "form_ms"):
+            out[f] = list(out[f])
+        return out
 
     def kmer_profiles(self) -> np.ndarray:
         """5-mer profiles (n x 1024 fp32, L1-normalised) computed on the GPU."""

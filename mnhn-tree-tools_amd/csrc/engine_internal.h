@@ -25,6 +25,9 @@ struct mnhn_sw_ctx {
   std::vector<uint32_t> woff;
   int groups = 0;
   std::vector<int32_t> gtgt, gmax, gmin;
+  // per target group, suffix sums of its slots' lengths: gsuf[g * 129 + t] =
+  // sum of len over slots t..127 (empty slots count 0)
+  std::vector<double> gsuf;
   int32_t maxlen = 0;
   int device = 0, num_cu = 0;
   int kernel = 0, strip = 0, waves_per_cu = 0;  // 0 = tuned default
@@ -37,6 +40,7 @@ struct mnhn_sw_ctx {
   size_t bound_bytes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t pev[12] = {};  // around each kernel pass of launch_tiles (created on first use)
   mnhn_sw_timing last{};
   // device-resident epsilon lists of the last mnhn_sw_eps_count call
   unsigned long long *d_keys = nullptr;

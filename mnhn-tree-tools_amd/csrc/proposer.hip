@@ -1339,7 +1339,12 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, (size_t)n * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    for (int i = 0; i < n; ++i) stt.cells += (double)c->len[i] * (double)h_tlen[i];
+    double part_cells[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      const double ci = (double)c->len[i] * (double)h_tlen[i];
+      stt.cells += ci;
+      part_cells[row_class(i)] += ci;
+    }
     dev_free(sorted);
     Work w;
     w.dev = true;
@@ -1352,11 +1357,12 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     // keys and, if the neighbours need more, grow to the exact count and verify
     // once more
     unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap));
-    double kernel_ms = 0;
+    double kernel_ms = 0, form_cells[3] = {0, 0, 0}, form_ms[3] = {0, 0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
+      for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
       for (int part = 0; part < 4 && rc == MNHN_OK; ++part) {
         w.d_items = items + part_begin[part];
         w.n_items = part_begin[part + 1] - part_begin[part];
@@ -1364,8 +1370,13 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
         w.dev_f16_ok = part < 2;
         w.dev_strip = part == 0 ? 32 : 0;
         if (w.n_items == 0) continue;
+        w.cells = part_cells[part];
         rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
         kernel_ms += c->last.kernel_ms;
+        for (int f = 0; f < 3; ++f) {
+          form_cells[f] += c->last.form_cells[f];
+          form_ms[f] += c->last.form_ms[f];
+        }
       }
       if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
       if (rc) return fail(rc);
@@ -1376,6 +1387,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     c->last.kernel_ms = kernel_ms;
     c->last.pairs = (double)m;
     c->last.cells = stt.cells;
+    for (int f = 0; f < 3; ++f) {
+      c->last.form_cells[f] = form_cells[f];
+      c->last.form_ms[f] = form_ms[f];
+    }
     stt.verify_kernel_ms = kernel_ms;
   }
   stt.verify_ms = phase_ms();

@@ -1132,6 +1132,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   struct Pass {
     int kernel, strip;
     std::vector<int4> items;
+    double cells = 0;
   };
   std::vector<Pass> passes;
   if (w.dev) {
@@ -1156,6 +1157,35 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     if (!i16.empty()) passes.push_back({K == KERNEL_LDS_F16 ? KERNEL_LDS : K, S, std::move(i16)});
     if (!i32.empty()) passes.push_back({KERNEL_I32, kI32Strip, std::move(i32)});
   }
+  // algorithmic cells of each pass (sum of la * lb over the pairs its items
+  // compute): with the context's groups from the targets' length suffix sums
+  // (the triangle keeps targets after the query in length order), with own
+  // groups from their slots' lengths
+  std::vector<double> own_gsum;
+  if (!w.dev && own_groups) {
+    own_gsum.assign(w.gmax.size(), 0.0);
+    for (size_t g = 0; g < w.gmax.size(); ++g)
+      for (int t = 0; t < kGroup; ++t) {
+        const int id = w.gtgt[g * kGroup + t];
+        if (id >= 0) own_gsum[g] += c->len[id];
+      }
+  }
+  auto item_cells = [&](const int4 &it) {
+    double s = 0;
+    for (int qi = it.y; qi < it.z; ++qi) {
+      const int q = w.qids[qi];
+      double tg;
+      if (own_groups) {
+        tg = own_gsum[it.x];
+      } else {
+        int from = 0;
+        if (triangle) from = std::max(0, std::min(kGroup, c->rank[q] - it.x * kGroup + 1));
+        tg = c->gsuf[(size_t)it.x * (kGroup + 1) + from];
+      }
+      s += (double)c->len[q] * tg;
+    }
+    return s;
+  };
   // LDS kernels at S = 64: items whose queries are all at most kShortQuery
   // long run at S = 32 (right-aligned 64-column strips would pad a 90-residue
   // query to 128 columns)
@@ -1178,6 +1208,23 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     }
     passes.swap(split);
   }
+  for (Pass &ps : passes) {
+    if (w.dev) {
+      ps.cells = w.cells;
+    } else {
+      for (const int4 &it : ps.items) ps.cells += item_cells(it);
+    }
+  }
+  if ((int)passes.size() * 2 > (int)(sizeof(c->pev) / sizeof(c->pev[0]))) {
+    cleanup();
+    return MNHN_ERR_ARG;
+  }
+  for (size_t k = 0; k < 2 * passes.size(); ++k)
+    if (!c->pev[k] && hipEventCreate(&c->pev[k]) != hipSuccess) {
+      c->pev[k] = nullptr;
+      cleanup();
+      return MNHN_ERR_DEVICE;
+    }
 
   // strip-boundary workspace, [row][lane] per wave.  Rows are query residues
   // in the register-profile kernel and target residues in the LDS kernels, so
@@ -1189,7 +1236,8 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   int waves_used = 0;
   bool any_i32 = false;
   (void)hipEventRecord(c->ev[0], c->stream);
-  for (auto &pass : passes) {
+  for (size_t pi = 0; pi < passes.size(); ++pi) {
+    Pass &pass = passes[pi];
     const int PK = pass.kernel, PS = pass.strip;
     const std::vector<int4> &items = pass.items;
     const size_t n_items = w.dev ? (size_t)w.n_items : items.size();
@@ -1240,7 +1288,9 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
     if (e == hipSuccess) {
       void *args[] = {&a};
+      (void)hipEventRecord(c->pev[2 * pi], c->stream);
       e = hipLaunchKernel(kp, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+      (void)hipEventRecord(c->pev[2 * pi + 1], c->stream);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) break;
@@ -1254,6 +1304,14 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   if (e == hipSuccess) (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
   cleanup();
   if (e != hipSuccess) return MNHN_ERR_DEVICE;
+  for (int f = 0; f < 3; ++f) c->last.form_cells[f] = c->last.form_ms[f] = 0;
+  for (size_t pi = 0; pi < passes.size(); ++pi) {
+    const int f = passes[pi].kernel == KERNEL_LDS_F16 ? 0 : passes[pi].kernel == KERNEL_I32 ? 2 : 1;
+    float pms = 0.f;
+    (void)hipEventElapsedTime(&pms, c->pev[2 * pi], c->pev[2 * pi + 1]);
+    c->last.form_cells[f] += passes[pi].cells;
+    c->last.form_ms[f] += pms;
+  }
   c->last.kernel_ms = ms;
   c->last.total_ms = ms;
   c->last.cells = w.cells;
@@ -1516,6 +1574,13 @@ int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
     c->gmax[g] = std::max(c->gmax[g], c->len[c->order[p]]);
     c->gmin[g] = std::min(c->gmin[g], c->len[c->order[p]]);
   }
+  c->gsuf.assign((size_t)c->groups * (kGroup + 1), 0.0);
+  for (int g = 0; g < c->groups; ++g)
+    for (int t = kGroup - 1; t >= 0; --t) {
+      const int id = c->gtgt[(size_t)g * kGroup + t];
+      c->gsuf[(size_t)g * (kGroup + 1) + t] =
+          c->gsuf[(size_t)g * (kGroup + 1) + t + 1] + (id >= 0 ? (double)c->len[id] : 0.0);
+    }
 
   int rc = MNHN_OK;
   hipDeviceProp_t prop;
@@ -1562,6 +1627,8 @@ void mnhn_sw_destroy(mnhn_sw_ctx *c) {
   dev_free(c->d_nkeys);
   dev_free(c->d_bound); dev_free(c->d_keys);
   for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->pev)
     if (e) (void)hipEventDestroy(e);
   stage_free(c->stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -55,6 +55,8 @@ typedef struct {
   int64_t tiles;          /* 128 x 128 embedding tiles the length band leaves to compare */
   double cells;           /* sum of la * lb over the candidates (verification work) */
   double verify_kernel_ms; /* SW kernel time of the verification (HIP events) */
+  int64_t stage2_tiles;    /* band tiles that ran the cosine stage (counted on the device) */
+  double propose_kernel_ms; /* pairwise kernel time alone (HIP events around its launch) */
 } mnhn_prefilter_stats;
 
 /* 5-mer profiles of every sequence (n x 1024 fp32, host buffer). */

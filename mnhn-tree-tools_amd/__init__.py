@@ -124,7 +124,8 @@ class PrefilterStats(ctypes.Structure):
                 ("profile_ms", ctypes.c_double), ("embed_ms", ctypes.c_double),
                 ("propose_ms", ctypes.c_double), ("verify_ms", ctypes.c_double),
                 ("csr_ms", ctypes.c_double), ("tiles", ctypes.c_int64), ("cells", ctypes.c_double),
-                ("verify_kernel_ms", ctypes.c_double)]
+                ("verify_kernel_ms", ctypes.c_double), ("stage2_tiles", ctypes.c_int64),
+                ("propose_kernel_ms", ctypes.c_double)]
 
 
 _LIB = None

@@ -436,7 +436,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
     const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
     const bf16 *__restrict__ Xs, const float *__restrict__ pnorm, const float *__restrict__ cmin,
     const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
-    unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap) {
+    unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap,
+    unsigned long long *__restrict__ stage2) {
   constexpr int BT = 128, LK = 64 + 8;
   __shared__ __attribute__((aligned(16))) bf16 sA[2 * BT * LK];
   __shared__ __attribute__((aligned(16))) bf16 sB[2 * BT * LK];
@@ -504,6 +505,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
     if (__ballot(m1 != 0ull) != 0ull && lane == 0) any_hit = 1;
     __syncthreads();
     if (any_hit == 0) return;  // uniform over the block
+    if (tid == 0) atomicAdd(stage2, 1ull);
     f32x4 accx[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -1150,9 +1152,11 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   if (!out->row_ptr) return MNHN_ERR_NOMEM;
   if (n == 0) return MNHN_OK;
   hipStream_t st = c->stream;
-  hipEvent_t e0, e1;
+  hipEvent_t e0, e1, ek0, ek1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
+  (void)hipEventCreate(&ek0);
+  (void)hipEventCreate(&ek1);
   auto phase_ms = [&]() {
     float ms = 0.f;
     (void)hipEventRecord(e1, st);
@@ -1178,6 +1182,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(ek0);
+    (void)hipEventDestroy(ek1);
   };
   auto fail = [&](int code) {
     cleanup();
@@ -1236,7 +1242,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   }
   const unsigned nblk = (unsigned)toff[tiles];
   stt.tiles = nblk;
-  if ((rc = dev_alloc(&d_toff, (size_t)tiles + 1)) || (rc = dev_alloc(&d_cnt64, 1))) return fail(rc);
+  // d_cnt64[0]: candidates, d_cnt64[1]: band tiles that ran the cosine stage
+  if ((rc = dev_alloc(&d_toff, (size_t)tiles + 1)) || (rc = dev_alloc(&d_cnt64, 2))) return fail(rc);
   if (hipMemcpyAsync(d_toff, toff.data(), ((size_t)tiles + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess)
     return fail(MNHN_ERR_DEVICE);
   // candidate buffer: the band's pair count, at most MNHN_PREFILTER_CAND_GB
@@ -1251,19 +1258,27 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   for (int attempt = 0; attempt < 2; ++attempt) {
     dev_free(cand);
     if ((rc = dev_alloc(&cand, cap))) break;
-    hipError_t e = hipMemsetAsync(d_cnt64, 0, 8, st);
+    hipError_t e = hipMemsetAsync(d_cnt64, 0, 16, st);
     if (e == hipSuccess) {
+      (void)hipEventRecord(ek0, st);
       if (use_cos)
         hipLaunchKernelGGL(pair_candidates_kernel<true>, dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs, pnorm,
-                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap);
+                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       else
         hipLaunchKernelGGL(pair_candidates_kernel<false>, dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs, pnorm,
-                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap);
+                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       e = hipGetLastError();
+      (void)hipEventRecord(ek1, st);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&m, d_cnt64, 8, hipMemcpyDeviceToHost, st);
+    unsigned long long cnt2[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt2, d_cnt64, 16, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
+    m = cnt2[0];
+    stt.stage2_tiles = (int64_t)cnt2[1];
+    float kms = 0.f;
+    (void)hipEventElapsedTime(&kms, ek0, ek1);
+    stt.propose_kernel_ms = kms;
     if (m <= cap) break;
     cap = m;  // overflowed: exact size, recompute once
   }

@@ -73,9 +73,12 @@ WORKLOADS = {
 # MI355X: 256 CUs x 64 lanes x 2 (packed int16) x 2.4 GHz
 PEAK_INT16_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 OPS_PER_CELL = 6
-# the exact fp16 form: 3.5 packed DP instructions per 2 cells (one cell per 16-bit half), at 64
-# lanes per CU-clock (the 6-op count above is 6 packed instructions per 2 cells: 13,107 GCUPS)
+# instruction rooflines of the kernel forms, at 64 lanes per CU-clock (packed 16-bit ops issue
+# every 4 cycles per wave64 on gfx950, measured): the exact fp16 form retires 3.5 packed DP
+# instructions per 2 cells (max3 and the clamp fold two of the 6 algorithmic ops each), the
+# packed int16 form 5 per 2 cells (saturating adds fold the floor)
 FP16_FORM_ROOFLINE_GCUPS = 256 * 64 * 2.4e9 * 2 / 3.5 / 1e9
+INT16_FORM_ROOFLINE_GCUPS = 256 * 64 * 2.4e9 * 2 / 5 / 1e9
 
 
 def load_pkg():
@@ -98,10 +101,12 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(seqs, sample_n, seed=7, threads=1):
-    """Reference score() (oracle/_ref/libref_sw.so) over all unique pairs of
-    a random sample on `threads` host threads (pairs split statically); the
-    oracle's C restatement if the reference build is absent."""
+def cpu_baseline(seqs, sample_n, seed=7, threads=1, port=False):
+    """Reference score() (oracle/_ref/libref_sw.so, compiled from the
+    reference's smith-waterman.c with its makefile flags) over all unique
+    pairs of a random sample on `threads` host threads (pairs split
+    statically); the oracle's clean-room C restatement (same flags) with
+    port=True or where the reference build is absent."""
     rng = random.Random(seed)
     sample = rng.sample(range(len(seqs)), sample_n)
     sub = [seqs[i] for i in sample]
@@ -109,7 +114,7 @@ def cpu_baseline(seqs, sample_n, seed=7, threads=1):
     ptrs = (ctypes.c_char_p * len(sub))(*[ctypes.cast(b, ctypes.c_char_p) for b in bufs])
     lens = (ctypes.c_size_t * len(sub))(*[len(s) for s in sub])
     ref = os.path.join(ROOT, "oracle", "_ref", "libref_sw.so")
-    if os.path.exists(ref):
+    if os.path.exists(ref) and not port:
         lib, fn, kind = ctypes.CDLL(ref), "ref_cpu_allpairs", "reference"
     else:
         lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle_sw.so"))
@@ -342,10 +347,13 @@ def pair_set_recall(rp_p, col_p, dist_p, rp_e, col_e, dist_e, sample_rows=20000,
     return len(col_p) / max(len(col_e), 1), subset, missed
 
 
-def run_proposer_bench(args, pkg, log):
-    eps = args.eps if args.eps else (300 if args.config == "proposer100k" else 200)
+def proposer_eval(args, pkg, log, config):
+    """Train the encoder, then time exact and learned-prefilter epsilon lists
+    + DBSCAN on the evaluation set of `config` (proposer100k or mix500k).
+    Returns (results, workload text, pairwise-kernel roofline, sequences)."""
+    eps = args.eps if args.eps else (300 if config == "proposer100k" else 200)
     minpts = 5
-    if args.config == "proposer100k":
+    if config == "proposer100k":
         # train on the amplicon-scale generator (its lengths, mutation rates and
         # noise) drawn with 2,000 ancestors under a new seed, evaluate on
         # 100,000 sequences of another draw
@@ -388,6 +396,9 @@ def run_proposer_bench(args, pkg, log):
     n = len(seqs)
     L = np.array([len(x) for x in seqs], dtype=np.float64)
     log(f"evaluation set: n={n}, mean length {L.mean():.0f}, max {int(L.max())}; eps {eps}")
+    hist_edges = [0, 100, 150, 200, 300, 400, 600, 800, 1000, 1500, 1 << 30]
+    length_hist = {f"{a}-{b - 1}" if b < (1 << 30) else f">={a}": int(((L >= a) & (L < b)).sum())
+                   for a, b in zip(hist_edges[:-1], hist_edges[1:])}
     eng = pkg.Engine(seqs)
     if not args.no_autotune:
         log(f"autotune: {eng.autotune(use_cache=True)}")
@@ -414,7 +425,6 @@ def run_proposer_bench(args, pkg, log):
         finally:
             del os.environ["MNHN_SW_LENGTH_PRUNE"]
         log(f"exact all-pairs lists (no length pruning): {unpruned}")
-    out = {}
     # learned prefilter + exact verification
     t0 = time.perf_counter()
     prp, pcol, pdist, pst = enc.prefilter_csr(eng, eps)
@@ -430,13 +440,16 @@ def run_proposer_bench(args, pkg, log):
             f"{np.quantile(lm, [0, 0.25, 0.5, 0.75, 1]).tolist()}; first "
             f"{[(int(a), int(b), int(L[a]), int(L[b])) for a, b in mi[:8]]}")
     all_pairs = n * (n - 1) / 2
-    # the pairwise kernel's algorithmic work per band tile: 128 x 128 dot
-    # products of the 128-d embeddings, plus (cosine stage) of the 1024-d counts
-    kdim = 128 + (0 if args.no_cosine else 1024)
+    # the pairwise kernel's executed MFMA work: every band tile runs the
+    # 128 x 128 dot products of the 128-d embeddings, the tiles with a stage-1
+    # survivor (counted on the device) also those of the 1024-d counts
+    flops = 2 * 128 * 128 * (128 * pst["tiles"] + 1024 * pst["stage2_tiles"])
     mfma = {"bound": "mfma", "kernel": "pair_candidates_kernel",
-            "achieved": pst["tiles"] * 2 * 128 * 128 * kdim / (pst["propose_ms"] / 1e3) / 1e12,
-            "peak": 2500.0, "unit": f"TFLOP/s (bf16 MFMA; 2*128*128*{kdim} flop per band tile)",
-            "traffic": None}
+            "achieved": flops / (pst["propose_kernel_ms"] / 1e3) / 1e12,
+            "peak": 2500.0, "unit": "TFLOP/s (bf16 MFMA, dense; 2*128*128*128 flop per band tile + "
+                                    "2*128*128*1024 per tile that runs the cosine stage)",
+            "traffic": None, "band_tiles": pst["tiles"], "cosine_stage_tiles": pst["stage2_tiles"],
+            "kernel_ms": pst["propose_kernel_ms"]}
     mfma["frac"] = mfma["achieved"] / mfma["peak"]
     res = {"eps": eps, "recall": recall, "proposed_subset_of_exact": bool(subset),
            "precision": precision, "candidates": pst["candidates"],
@@ -478,19 +491,27 @@ def run_proposer_bench(args, pkg, log):
                 "speedup": exact_s / pref_s, "host_dbscan_s": [exact_db_s, pref_db_s],
                 "clusters_exact": len(ecl), "clusters_prefilter": len(pcl),
                 "identical_clusters": bool(same), "identical_member_order": ecl == pcl,
-                "clustered_points": int(sum(len(c) for c in ecl))})
+                "clustered_points": int(sum(len(c) for c in ecl)), "length_histogram": length_hist})
+    eng.close()
+    enc.close()
+    return res, workload, mfma, seqs
+
+
+def run_proposer_bench(args, pkg, log):
+    res, workload, mfma, seqs = proposer_eval(args, pkg, log, args.config)
+    n = len(seqs)
+    out = {}
     common = {"n_gpus": 1, "steps": 1, "warmup": 0, "scaling": "weak", "vs_baseline": None,
               "data": "synthetic", "dtype": "bf16 (encoder) / f16 exact integers + int16 (verify)",
               "config": {"workload": workload, "n_seqs": n, **res}, "roofline": mfma}
     if args.config == "proposer100k":
         out.update({"metric": "learned-prefilter eps-neighbour recall (held-out)",
-                    "value": recall, "unit": "fraction", "higher_is_better": True,
-                    "ms_per_step": pref_lists_s * 1e3, **common})
+                    "value": res["recall"], "unit": "fraction", "higher_is_better": True,
+                    "ms_per_step": res["prefilter_lists_s"] * 1e3, **common})
     else:
-        out.update({"metric": "DBSCAN s/run exact vs learned-prefilter", "value": pref_s,
-                    "unit": "s", "higher_is_better": False, "ms_per_step": pref_s * 1e3, **common})
-    eng.close()
-    enc.close()
+        out.update({"metric": "DBSCAN s/run exact vs learned-prefilter", "value": res["prefilter_dbscan_s"],
+                    "unit": "s", "higher_is_better": False, "ms_per_step": res["prefilter_dbscan_s"] * 1e3,
+                    **common})
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(seqs, args.cpu_sample)
@@ -733,6 +754,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dbscan", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-prefilter", action="store_true",
+                    help="tree20k/amplicon105k: skip the learned-prefilter vs exact DBSCAN block (proposer100k)")
     ap.add_argument("--config", default="tree20k",
                     choices=["tree20k", "amplicon105k", "proposer100k", "mix500k", "consumers", "vec"],
                     help="tree20k: 20,000 uneven-length (headline); amplicon105k: 105,000 amplicon-scale; "
@@ -774,9 +797,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # the engine's phase and autotune lines (stderr) keep a long run visibly alive
+    os.environ.setdefault("MNHN_TIMING", "1")
     pkg = load_pkg()
+    t_start = time.perf_counter()
+
     def log(msg):
-        print(msg, file=sys.stderr, flush=True)
+        print(f"[{time.perf_counter() - t_start:7.1f} s] {msg}", file=sys.stderr, flush=True)
     if args.config in ("proposer100k", "mix500k"):
         run_proposer_bench(args, pkg, log)
         return
@@ -786,24 +813,26 @@ def main():
     if args.config == "vec":
         run_vec_bench(args, pkg, log)
         return
-    print("generating workload ...", file=sys.stderr, flush=True)
+    log("generating workload ...")
     seqs = synth.evolution_tree() if args.config == "tree20k" else synth.amplicon_families()
     if args.n:
         seqs = seqs[:args.n]
     L = np.array([len(s) for s in seqs], dtype=np.float64)
     cells = 0.5 * (L.sum() ** 2 - (L * L).sum())
     pairs = len(seqs) * (len(seqs) - 1) // 2
+    log(f"n={len(seqs)} pairs={pairs} cells={cells:.3e}; uploading")
 
     eng = pkg.Engine(seqs)
     tuning = None
     if not args.no_autotune:
+        log("autotune (one line per candidate) ...")
         tuning = eng.autotune(use_cache=True)
-        print(f"autotune: {tuning}", file=sys.stderr, flush=True)
-    print(f"n={len(seqs)} pairs={pairs} cells={cells:.3e}; warmup {args.warmup}", file=sys.stderr,
-          flush=True)
+        log(f"autotune: {tuning}")
+    log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
         eng.eps_count(args.eps_max)
     times, kms = [], []
+    form_cells, form_ms = np.zeros(3), np.zeros(3)
     nnz = 0
     for k in range(args.steps):
         t0 = time.perf_counter()
@@ -811,8 +840,11 @@ def main():
         times.append(time.perf_counter() - t0)
         t = eng.last_timing()
         kms.append(t["kernel_ms"])
-        print(f"step {k}: {times[-1] * 1e3:.1f} ms (kernel {kms[-1]:.1f} ms), "
-              f"{t['cells'] / times[-1] / 1e9:.1f} GCUPS, nnz {nnz}", file=sys.stderr, flush=True)
+        form_cells += np.array(t["form_cells"]) / args.steps
+        form_ms += np.array(t["form_ms"]) / args.steps
+        log(f"step {k}: {times[-1] * 1e3:.1f} ms (kernels {kms[-1]:.1f} ms: fp16 {t['form_ms'][0]:.1f}, "
+            f"int16 {t['form_ms'][1]:.1f}, int32 {t['form_ms'][2]:.1f}), "
+            f"{t['cells'] / times[-1] / 1e9:.1f} GCUPS, nnz {nnz}")
     t = eng.last_timing()
     # GCUPS counts the cells of the pairs the step computed: every unique pair,
     # unless eps_max is small enough for the length bound to leave some out
@@ -823,13 +855,25 @@ def main():
     eng.eps_count(args.eps_max)
     int16_s = time.perf_counter() - t0
     int16_kernel_ms = eng.last_timing()["kernel_ms"]
-    print(f"int16-only step: {int16_s * 1e3:.1f} ms, {comp_cells / int16_s / 1e9:.1f} GCUPS",
-          file=sys.stderr, flush=True)
+    log(f"int16-only step: {int16_s * 1e3:.1f} ms, {comp_cells / int16_s / 1e9:.1f} GCUPS")
     eng.close()
     step_s = float(np.mean(times))
     kernel_s = float(np.mean(kms)) / 1e3
     gcups = comp_cells / step_s / 1e9
-    achieved = OPS_PER_CELL * comp_cells / kernel_s / 1e12
+    peaks = [FP16_FORM_ROOFLINE_GCUPS, INT16_FORM_ROOFLINE_GCUPS, None]
+    forms = {}
+    for f, name in enumerate(("fp16", "int16", "int32")):
+        if form_cells[f] > 0:
+            g = form_cells[f] / (form_ms[f] / 1e3) / 1e9
+            forms[name] = {"cells": form_cells[f], "kernel_ms": form_ms[f], "gcups": g,
+                           "instruction_roofline_gcups": peaks[f],
+                           "frac": g / peaks[f] if peaks[f] else None}
+    # dominant kernel: the exact fp16 form (sw_lds_f16_kernel, strips 64 and 32)
+    f16 = forms.get("fp16") or forms.get("int16")
+    f16_peak = f16["instruction_roofline_gcups"]
+    # the step against each form's own roofline: time at roofline / kernel time
+    at_roof_s = sum(forms[k]["cells"] / (forms[k]["instruction_roofline_gcups"] * 1e9)
+                    for k in forms if forms[k]["instruction_roofline_gcups"])
     out = {
         "metric": "all-pairs SW GCUPS", "value": gcups, "unit": "GCUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_s * 1e3,
@@ -846,36 +890,102 @@ def main():
                               3: "lds-profile exact fp16 (+int16 pass)"}.get(t["kernel"]),
                    "strip": t["strip"], "waves": t["waves"], "items": t["items"],
                    "autotune": tuning},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT16_TOPS,
-                     "unit": "TOP/s (packed 16-bit VALU; 6 algorithmic ops per cell)",
-                     "frac": achieved / PEAK_INT16_TOPS,
+        "roofline": {"bound": "valu",
+                     "kernel": "sw_lds_f16_kernel (strips 64 and 32)" if "fp16" in forms else "sw_lds_kernel",
+                     "achieved": f16["gcups"], "peak": f16_peak,
+                     "unit": "GCUPS (cells of the pairs the kernel computed / its time, HIP events on the "
+                             "engine's stream; peak = the form's instruction roofline)",
+                     "frac": f16["gcups"] / f16_peak,
                      "traffic": profiled_traffic(args.config),
-                     "kernel": "sw_lds_f16_kernel + sw_lds_kernel" if t["kernel"] == 3
-                               else {1: "sw_tiles_kernel", 2: "sw_lds_kernel"}.get(t["kernel"]),
-                     "kernel_ms": kernel_s * 1e3,
-                     "kernel_gcups": comp_cells / kernel_s / 1e9,
-                     # frac > 1 is expected for the fp16 form: it retires the 6 algorithmic ops
-                     # of a cell in 3.5 packed instructions per 2 cells (max3 and the clamp fold
-                     # two each); against that instruction roofline (22,469 GCUPS) the fraction is:
-                     "instruction_roofline_gcups": FP16_FORM_ROOFLINE_GCUPS if t["kernel"] == 3
-                     else PEAK_INT16_TOPS * 1e12 / OPS_PER_CELL / 1e9,
-                     "instruction_frac": comp_cells / kernel_s / 1e9 / (
-                         FP16_FORM_ROOFLINE_GCUPS if t["kernel"] == 3
-                         else PEAK_INT16_TOPS * 1e12 / OPS_PER_CELL / 1e9),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload, calibrated on a "
+                                       "plain copy (profiles/); not measured in this run",
+                     "step_kernel_ms": kernel_s * 1e3,
+                     "step_kernel_gcups": comp_cells / kernel_s / 1e9,
+                     # every pass of the step against its own form's roofline
+                     "step_frac": at_roof_s / kernel_s,
+                     "forms": forms,
+                     # the fixed algorithmic count (6 int ops per cell) against the packed 16-bit
+                     # issue peak of the chip: the fp16 form exceeds 1 here because it retires
+                     # those 6 ops in 1.75 instructions per cell
+                     "algorithmic_tops": OPS_PER_CELL * comp_cells / kernel_s / 1e12,
+                     "algorithmic_peak_tops": PEAK_INT16_TOPS,
+                     "algorithmic_ratio": OPS_PER_CELL * comp_cells / kernel_s / 1e12 / PEAK_INT16_TOPS,
                      "int16_only_gcups": comp_cells / int16_s / 1e9,
                      "int16_only_kernel_ms": int16_kernel_ms},
     }
     if not args.no_cpu_baseline:
-        print("cpu baseline ...", file=sys.stderr, flush=True)
+        log("cpu baseline: the reference's score() on a sample, one thread ...")
         out["cpu_baseline"] = cpu_baseline(seqs, args.cpu_sample)
+        log(f"  {out['cpu_baseline']['value']:.3f} GCUPS; the clean-room port on the same sample ...")
+        out["cpu_baseline_port"] = cpu_baseline(seqs, args.cpu_sample, port=True)
+        out["cpu_baseline_port"]["port_over_reference"] = (out["cpu_baseline_port"]["value"]
+                                                           / out["cpu_baseline"]["value"])
         # best effort on the host: every core this process may run on (at most 16)
         nt = max(1, min(16, len(os.sched_getaffinity(0))))
+        log(f"  port {out['cpu_baseline_port']['value']:.3f} GCUPS; all {nt} cores ...")
         out["cpu_baseline_all_cores"] = cpu_baseline(seqs, 3 * args.cpu_sample, threads=nt)
+        log("reference-faithful DBSCAN pass (families set sample) ...")
+        out["cpu_baseline_reference_faithful"] = reference_faithful_dbscan(log)
     if not args.no_dbscan:
-        print("adaptive DBSCAN run ...", file=sys.stderr, flush=True)
+        log("adaptive DBSCAN CLI run ...")
         out["dbscan"] = run_dbscan(seqs, 100, 300, 5) if args.config == "tree20k" else \
             run_dbscan(seqs, 100, 100, 5)
+        log(f"  {out['dbscan']['exact_s_per_run']:.2f} s")
+    if not args.no_prefilter:
+        log("learned prefilter vs exact DBSCAN (proposer100k workload) ...")
+        pa = argparse.Namespace(**vars(args))
+        pa.n, pa.eps = 0, 0
+        res, workload, mfma, _ = proposer_eval(pa, pkg, log, "proposer100k")
+        keys = ("eps", "minpts", "recall", "proposed_subset_of_exact", "precision", "candidates",
+                "neighbour_pairs", "exact_neighbour_pairs", "exact_lists_s", "prefilter_lists_s",
+                "exact_dbscan_s", "prefilter_dbscan_s", "speedup", "exact_all_pairs_estimate",
+                "clusters_exact", "clusters_prefilter", "identical_clusters", "verify_gcups")
+        block = {"workload": workload, **{k: res[k] for k in keys if k in res}}
+        block["identical_partition"] = block.pop("identical_clusters")
+        block["recall_bar"] = 0.999
+        block["meets_recall_bar"] = bool(res["recall"] >= 0.999)
+        block["pairwise_roofline"] = mfma
+        out["dbscan_exact_vs_prefilter"] = block
     print(json.dumps(out), flush=True)
+
+
+def reference_faithful_dbscan(log, n_sample=300, eps=400, minpts=5):
+    """The reference's own cluster_dbscan_SW (CPU build, compiled from its
+    sources into oracle/_ref) on the first n_sample sequences of the
+    2,000-sequence 200-bp families set: one DBSCAN pass = n_sample region
+    queries of n_sample score() calls each (dbscan.c:525-561), one thread,
+    timed and extrapolated to N = 2,000 by N^2; the drop-in CLI is timed on
+    the same sample (outputs compared) and on all 2,000 sequences."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "cluster_dbscan_SW")
+    ours = os.path.join(PKG_DIR, "bin", "cluster_dbscan_SW")
+    if not os.path.exists(ref):
+        return None
+    seqs = synth.families_200bp()
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag, data in (("sample", seqs[:n_sample]), ("full", seqs)):
+            fa = os.path.join(tmp, tag + ".fasta")
+            synth.write_fasta(fa, data)
+            for who, exe in (("ref", ref), ("ours", ours)):
+                if who == "ref" and tag == "full":
+                    continue
+                d = os.path.join(tmp, f"{tag}_{who}")
+                os.makedirs(d)
+                secs, so = _run_cli([exe, fa, str(eps), str(minpts), "o"], d)
+                res[f"{tag}_{who}"] = (secs, so, _outputs(d, "o"))
+    ref_s = res["sample_ref"][0]
+    same = res["sample_ref"][1] == res["sample_ours"][1] and res["sample_ref"][2] == res["sample_ours"][2]
+    est = ref_s * (len(seqs) / n_sample) ** 2
+    log(f"  reference {n_sample} seqs: {ref_s:.1f} s (outputs identical to the drop-in: {same}); "
+        f"extrapolated to {len(seqs)}: {est:.0f} s; drop-in on {len(seqs)}: {res['full_ours'][0]:.2f} s")
+    return {"kind": "reference", "cores": 1, "unit": "s per DBSCAN pass",
+            "value": est, "label": "extrapolated by N^2 from a timed sample",
+            "sample": f"the reference's cluster_dbscan_SW (oracle/_ref, one thread) on the first {n_sample} "
+                      f"of the 2,000 families_200bp sequences, eps {eps}, minpts {minpts}: {ref_s:.2f} s "
+                      f"({n_sample}^2 score() calls); host: {cpu_model()}",
+            "sample_s": ref_s, "sample_outputs_identical": bool(same),
+            "drop_in_sample_s": res["sample_ours"][0], "drop_in_full_s": res["full_ours"][0],
+            "speedup_full_vs_extrapolated": est / res["full_ours"][0]}
 
 
 if __name__ == "__main__":

@@ -57,6 +57,43 @@ def mutate(rng: np.random.Generator, parent: np.ndarray, p_sub: float,
     return out
 
 
+def mutate_many(rng: np.random.Generator, parent: np.ndarray, p_sub, p_indel, k: int) -> List[np.ndarray]:
+    """k independent children of one parent under the model of `mutate`, drawn
+    as one (k, len) batch; p_sub / p_indel are scalars or one rate per child."""
+    n = len(parent)
+    if k <= 0:
+        return []
+    if n == 0:
+        return [parent.copy() for _ in range(k)]
+    ps = np.broadcast_to(np.asarray(p_sub, dtype=np.float64), (k,))[:, None]
+    pi = np.broadcast_to(np.asarray(p_indel, dtype=np.float64), (k,))[:, None]
+    r = rng.random((k, n))
+    child = np.repeat(parent[None, :], k, axis=0)
+    sub = r < ps
+    nsub = int(sub.sum())
+    if nsub:
+        child[sub] = (child[sub] + rng.integers(1, 4, size=nsub, dtype=np.uint8)) % 4
+    ind = (r >= ps) & (r < ps + pi)
+    coin = rng.random((k, n)) < 0.5
+    counts = 1 - (ind & coin).astype(np.int64) + (ind & ~coin).astype(np.int64)
+    idx = np.repeat(np.arange(k * n), counts.ravel())
+    out = child.ravel()[idx]
+    if len(out):
+        first = np.ones(len(out), dtype=bool)
+        first[1:] = idx[1:] != idx[:-1]
+        nins = int((~first).sum())
+        if nins:
+            out[~first] = rng.integers(0, 4, size=nins, dtype=np.uint8)
+    return np.split(out, np.cumsum(counts.sum(1))[:-1])
+
+
+def random_many(rng: np.random.Generator, lengths: Sequence[int]) -> List[np.ndarray]:
+    """Uniform random sequences of the given lengths (one draw for all)."""
+    lengths = np.asarray(lengths, dtype=np.int64)
+    codes = rng.integers(0, 4, size=int(lengths.sum()), dtype=np.uint8)
+    return np.split(codes, np.cumsum(lengths)[:-1]) if len(lengths) else []
+
+
 def to_bytes(codes: np.ndarray) -> bytes:
     return ALPHABET[codes].tobytes()
 
@@ -103,12 +140,10 @@ def amplicon_families(seed: int = 3, n_anc: int = 500, per_anc: int = 200,
     out = []
     for _ in range(n_anc):
         anc = random_codes(rng, rng.integers(lo, hi + 1))
-        for _ in range(per_anc):
-            m = rng.uniform(0.02, 0.10)
-            out.append(to_bytes(mutate(rng, anc, 0.8 * m, 0.2 * m)))
+        m = rng.uniform(0.02, 0.10, size=per_anc)
+        out += [to_bytes(c) for c in mutate_many(rng, anc, 0.8 * m, 0.2 * m, per_anc)]
     n_noise = int(round(noise_frac * n_anc * per_anc))
-    for _ in range(n_noise):
-        out.append(to_bytes(random_codes(rng, rng.integers(lo, hi + 1))))
+    out += [to_bytes(c) for c in random_many(rng, rng.integers(lo, hi + 1, size=n_noise))]
     return out
 
 
@@ -137,22 +172,27 @@ def heavy_tail_mix(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root:
     leaves = []
     for L in pareto_lengths(rng, n_roots):
         root = random_codes(rng, L)
-        for _ in range(per_root):
-            child = mutate(rng, root, 0.8 * 0.08, 0.2 * 0.08)
-            for _ in range(per_child):
-                leaves.append(mutate(rng, child, 0.8 * 0.04, 0.2 * 0.04))
-    out = []
-    for k in range(n_tree):
-        anc = leaves[k % len(leaves)]
-        out.append(to_bytes(mutate(rng, anc, 0.8 * 0.02, 0.2 * 0.02)))
+        for child in mutate_many(rng, root, 0.8 * 0.08, 0.2 * 0.08, per_root):
+            leaves += mutate_many(rng, child, 0.8 * 0.04, 0.2 * 0.04, per_child)
+    # sequence k of the tree part descends from leaf k % 2,000
+    out: List[bytes] = [b""] * n
+    nl = len(leaves)
+    for j, leaf in enumerate(leaves):
+        members = range(j, n_tree, nl)
+        for k, c in zip(members, mutate_many(rng, leaf, 0.8 * 0.02, 0.2 * 0.02, len(members))):
+            out[k] = to_bytes(c)
+    # sequence n_tree + k repeats monomer k % 200, 2-8 copies at 3% each
     monomers = [random_codes(rng, monomer) for _ in range(n_monomer_families)]
-    for k in range(n_rep):
-        mono = monomers[k % n_monomer_families]
-        copies = int(rng.integers(2, 9))
-        out.append(to_bytes(np.concatenate(
-            [mutate(rng, mono, 0.8 * 0.03, 0.2 * 0.03) for _ in range(copies)])))
-    for L in pareto_lengths(rng, n_noise):
-        out.append(to_bytes(random_codes(rng, L)))
+    copies = rng.integers(2, 9, size=n_rep)
+    for f, mono in enumerate(monomers):
+        members = np.arange(f, n_rep, n_monomer_families)
+        parts = mutate_many(rng, mono, 0.8 * 0.03, 0.2 * 0.03, int(copies[members].sum()))
+        at = 0
+        for k in members:
+            out[n_tree + k] = to_bytes(np.concatenate(parts[at:at + copies[k]]))
+            at += copies[k]
+    for k, c in enumerate(random_many(rng, pareto_lengths(rng, n_noise))):
+        out[n_tree + n_rep + k] = to_bytes(c)
     if not with_groups:
         return out
     k = np.arange(n)

@@ -94,6 +94,22 @@ void mnhn_sw_destroy(mnhn_sw_ctx *ctx);
  * region_query (dbscan.c:471-513, smith-waterman-gpu.cl:13-98). */
 int mnhn_sw_one_vs_all(mnhn_sw_ctx *ctx, int query, int32_t *dist_out);
 
+/* mnhn_sw_one_vs_all without any allocation: dist_out is DEVICE memory of n
+ * int32 (the caller's), and dist_out[j] = d(seq[j], seq[query]) as
+ * mnhn_sw_one_vs_all writes them to host memory, from the same kernels.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * call makes no hipMalloc / hipFree and no host heap allocation; its
+ * work-queue counters and strip-boundary workspaces live in a static device
+ * array of 64 MiB that each GPU running the library holds until the process
+ * ends.  Limit: the context's longest sequence must be at most 65,534
+ * residues (one workgroup's workspaces, 4 x (maxlen + 2) x 256 B, fit the
+ * array), else MNHN_ERR_RANGE.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (several host threads, several contexts) race and
+ * must be serialised by the caller.  Returns once dist_out holds the
+ * distances; mnhn_sw_last_timing is not updated. */
+int mnhn_sw_one_vs_all_noalloc(mnhn_sw_ctx *ctx, int query, int32_t *dist_out);
+
 /* The reference's scalar seam, score() (smith-waterman.h:1,
  * smith-waterman.c:16-81), with the same signature and result: one pair
  * computed on the GPU (a two-sequence context per call; work is accepted and

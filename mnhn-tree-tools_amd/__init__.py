@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "mnhn_sw_create", "mnhn_sw_destroy", "mnhn_sw_one_vs_all", "mnhn_sw_pairs",
     "mnhn_sw_matrix", "mnhn_sw_block", "mnhn_sw_eps_csr", "mnhn_sw_eps_count", "mnhn_csr_free",
     "mnhn_sw_size", "mnhn_sw_data_hash", "mnhn_sw_alphabet", "mnhn_sw_last_timing",
-    "mnhn_sw_set_tuning", "mnhn_sw_autotune", "mnhn_sw_strerror", "mnhn_score",
+    "mnhn_sw_set_tuning", "mnhn_sw_autotune", "mnhn_sw_strerror", "mnhn_score", "mnhn_sw_one_vs_all_noalloc",
 ]
 PROPOSER_SYMBOLS = [
     "mnhn_kmer_profiles", "mnhn_encoder_create", "mnhn_encoder_destroy",

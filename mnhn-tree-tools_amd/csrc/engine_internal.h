@@ -46,6 +46,7 @@ struct mnhn_sw_ctx {
   unsigned long long *d_keys = nullptr;
   size_t keys_cap = 0;
   uint64_t data_hash = 0;  // scores, count, lengths and bytes of the sequences
+  void *scratch = nullptr;  // this device's static scratch (mnhn_sw_one_vs_all_noalloc)
   mnhn_detail_stage stage;
 };
 

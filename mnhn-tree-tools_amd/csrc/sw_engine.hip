@@ -47,6 +47,7 @@
 #include <type_traits>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <new>
 #include <numeric>
@@ -92,11 +93,27 @@ struct KArgs {
   unsigned long long key_cap;
   int n_items;
   int bound_stride;        // uint32 words per wave
+  int item_base;           // group-item kernels: item k is target group item_base + k
+  int query;               // against this query
   int mode;
   int triangle;            // only pairs with rank[q] < rank[t]
   int eps_max;
   int match, mismatch, gap;
 };
+
+// Work item k: from the item list, or (kG: group items, the allocation-free
+// one-vs-all) target group item_base + k against the single query.  kG is a
+// template parameter so that the item-list kernels carry none of its code.
+template <bool kG>
+__device__ __forceinline__ int4 load_item(const KArgs &a, int k) {
+  if constexpr (kG) return make_int4(a.item_base + k, 0, 1, 0);
+  else return a.items[k];
+}
+template <bool kG>
+__device__ __forceinline__ int load_qid(const KArgs &a, int qi) {
+  if constexpr (kG) return a.query;
+  else return a.qids[qi];
+}
 
 __device__ __forceinline__ s16x2 as_s16x2(uint32_t v) {
   return __builtin_bit_cast(s16x2, v);
@@ -178,6 +195,7 @@ __device__ __forceinline__ void dp_row_codes(s16x2 (&H)[S], const uint32_t (&TC)
 
 // Distances of the lane's two targets against query q and their output:
 // MODE_DIST writes them, MODE_EPS emits keys for d <= eps_max.
+template <bool kG>
 __device__ __forceinline__ void finish_query_int(const KArgs &a, int lane, int q, int qrel, int ob,
                                                  int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
                                                  int rk_lo, int rk_hi, int best_lo, int best_hi,
@@ -188,6 +206,11 @@ __device__ __forceinline__ void finish_query_int(const KArgs &a, int lane, int q
   const int d_lo = a.match * max(la, lb_lo) - tot_lo;
   const int d_hi = a.match * max(la, lb_hi) - tot_hi;
 
+  if constexpr (kG) {  // group items: distances by target id
+    if (t_lo >= 0) a.dist_out[t_lo] = d_lo;
+    if (t_hi >= 0) a.dist_out[t_hi] = d_hi;
+    return;
+  }
   const int rq = a.triangle ? a.rank[q] : 0;
   const bool ok_lo = t_lo >= 0 && (!a.triangle || rq < rk_lo);
   const bool ok_hi = t_hi >= 0 && (!a.triangle || rq < rk_hi);
@@ -226,15 +249,16 @@ __device__ __forceinline__ void finish_query_int(const KArgs &a, int lane, int q
 }
 
 // int16 kernels: best / corner are biased by -32768
+template <bool kG>
 __device__ __forceinline__ void finish_query(const KArgs &a, int lane, int q, int qrel, int ob,
                                              int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
                                              int rk_lo, int rk_hi, s16x2 best, s16x2 corner) {
-  finish_query_int(a, lane, q, qrel, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+  finish_query_int<kG>(a, lane, q, qrel, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
                    (int)best.x + kBias, (int)best.y + kBias, (int)corner.x + kBias,
                    (int)corner.y + kBias);
 }
 
-template <int S, bool kAcgt>
+template <int S, bool kAcgt, bool kG = false>
 __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6));
@@ -255,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
     if (lane == 0) item_idx = atomicAdd(a.counter, 1);
     item_idx = __builtin_amdgcn_readfirstlane(item_idx);
     if (item_idx >= a.n_items) break;
-    const int4 it = a.items[item_idx];
+    const int4 it = load_item<kG>(a, item_idx);
     const int g = __builtin_amdgcn_readfirstlane(it.x);
     const int qb = __builtin_amdgcn_readfirstlane(it.y);
     const int qe = __builtin_amdgcn_readfirstlane(it.z);
@@ -274,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
     const int nstrips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
 
     for (int qi = qb; qi < qe; ++qi) {
-      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
       const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
       const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
       const int nrows = la > 1 ? la - 1 : 0;
@@ -380,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
         corner = as_s16x2(cu);
       }
 
-      finish_query(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+      finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
                    best, corner);
     }
   }
@@ -412,7 +436,7 @@ __device__ __forceinline__ void dp_row_i32(int (&H)[S], const int (&P)[S], int l
   }
 }
 
-template <int S, bool kAcgt>
+template <int S, bool kAcgt, bool kG = false>
 __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6));
@@ -427,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
     if (lane == 0) item_idx = atomicAdd(a.counter, 1);
     item_idx = __builtin_amdgcn_readfirstlane(item_idx);
     if (item_idx >= a.n_items) break;
-    const int4 it = a.items[item_idx];
+    const int4 it = load_item<kG>(a, item_idx);
     const int g = __builtin_amdgcn_readfirstlane(it.x);
     const int qb = __builtin_amdgcn_readfirstlane(it.y);
     const int qe = __builtin_amdgcn_readfirstlane(it.z);
@@ -452,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
     const int ns_hi = __builtin_amdgcn_readfirstlane(mhi > 1 ? (mhi - 1 + S - 1) / S : 0);
 
     for (int qi = qb; qi < qe; ++qi) {
-      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
       const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
       const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
       const int nrows = la > 1 ? la - 1 : 0;
@@ -540,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
         best2[half] = best;
         corner2[half] = corner;
       }
-      finish_query_int(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi, best2[0], best2[1],
+      finish_query_int<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi, best2[0], best2[1],
                        corner2[0], corner2[1]);
     }
   }
@@ -556,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
 // per-row selection, no register copies.  Columns left of the query's first
 // residue ("pre-padding" of the first strip) hold -32768, which keeps their
 // cells at the floor.
-template <int S>
+template <int S, bool kG = false>
 __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
@@ -578,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
     if (lane == 0) item_idx = atomicAdd(a.counter, 1);
     item_idx = __builtin_amdgcn_readfirstlane(item_idx);
     if (item_idx >= a.n_items) break;
-    const int4 it = a.items[item_idx];
+    const int4 it = load_item<kG>(a, item_idx);
     const int g = __builtin_amdgcn_readfirstlane(it.x);
     const int qb = __builtin_amdgcn_readfirstlane(it.y);
     const int qe = __builtin_amdgcn_readfirstlane(it.z);
@@ -601,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
     const int nfast = gmn > 1 ? gmn - 1 : 0;   // rows inside every target of the group
 
     for (int qi = qb; qi < qe; ++qi) {
-      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
       const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
       const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
       const int ncols = la > 1 ? la - 1 : 0;
@@ -702,7 +726,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
         if (lb_hi >= 2) cu = (cu & 0x0000ffffu) | (bnd[(uint32_t)(lb_hi - 1) * kWave + lane] & 0xffff0000u);
         corner = as_s16x2(cu);
       }
-      finish_query(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+      finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
                    best, corner);
     }
   }
@@ -758,7 +782,7 @@ __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f
   }
 }
 
-template <int S>
+template <int S, bool kG = false>
 __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;
@@ -785,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
     if (lane == 0) item_idx = atomicAdd(a.counter, 1);
     item_idx = __builtin_amdgcn_readfirstlane(item_idx);
     if (item_idx >= a.n_items) break;
-    const int4 it = a.items[item_idx];
+    const int4 it = load_item<kG>(a, item_idx);
     const int g = __builtin_amdgcn_readfirstlane(it.x);
     const int qb = __builtin_amdgcn_readfirstlane(it.y);
     const int qe = __builtin_amdgcn_readfirstlane(it.z);
@@ -807,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
     const int nfast = gmn > 1 ? gmn - 1 : 0;
 
     for (int qi = qb; qi < qe; ++qi) {
-      const int q = __builtin_amdgcn_readfirstlane(a.qids[qi]);
+      const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
       const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
       const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
       const int ncols = la > 1 ? la - 1 : 0;
@@ -920,7 +944,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
           cor_hi = (int)__builtin_rintf((float)gv.y * sc) + a.gap;
         }
       }
-      finish_query_int(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+      finish_query_int<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
                        best_lo, best_hi, cor_lo, cor_hi);
     }
   }
@@ -1054,26 +1078,89 @@ int pick_strip(const mnhn_sw_ctx *c) {
   return (v == 16 || v == 32) ? v : 32;
 }
 
-void *kernel_ptr(int k, int S, bool acgt) {
+template <bool kG>
+void *kernel_ptr_g(int k, int S, bool acgt) {
   if (k == KERNEL_I32)
-    return acgt ? reinterpret_cast<void *>(&sw_i32_kernel<16, true>)
-                : reinterpret_cast<void *>(&sw_i32_kernel<16, false>);
+    return acgt ? reinterpret_cast<void *>(&sw_i32_kernel<16, true, kG>)
+                : reinterpret_cast<void *>(&sw_i32_kernel<16, false, kG>);
   if (k == KERNEL_LDS_F16) {
-    if (S == 16) return reinterpret_cast<void *>(&sw_lds_f16_kernel<16>);
-    if (S == 64) return reinterpret_cast<void *>(&sw_lds_f16_kernel<64>);
-    return reinterpret_cast<void *>(&sw_lds_f16_kernel<32>);
+    if (S == 16) return reinterpret_cast<void *>(&sw_lds_f16_kernel<16, kG>);
+    if (S == 64) return reinterpret_cast<void *>(&sw_lds_f16_kernel<64, kG>);
+    return reinterpret_cast<void *>(&sw_lds_f16_kernel<32, kG>);
   }
   if (k == KERNEL_LDS) {
-    if (S == 16) return reinterpret_cast<void *>(&sw_lds_kernel<16>);
-    if (S == 64) return reinterpret_cast<void *>(&sw_lds_kernel<64>);
-    return reinterpret_cast<void *>(&sw_lds_kernel<32>);
+    if (S == 16) return reinterpret_cast<void *>(&sw_lds_kernel<16, kG>);
+    if (S == 64) return reinterpret_cast<void *>(&sw_lds_kernel<64, kG>);
+    return reinterpret_cast<void *>(&sw_lds_kernel<32, kG>);
   }
-  if (S == 16) return acgt ? reinterpret_cast<void *>(&sw_tiles_kernel<16, true>)
-                           : reinterpret_cast<void *>(&sw_tiles_kernel<16, false>);
-  return acgt ? reinterpret_cast<void *>(&sw_tiles_kernel<32, true>)
-              : reinterpret_cast<void *>(&sw_tiles_kernel<32, false>);
+  if (S == 16) return acgt ? reinterpret_cast<void *>(&sw_tiles_kernel<16, true, kG>)
+                           : reinterpret_cast<void *>(&sw_tiles_kernel<16, false, kG>);
+  return acgt ? reinterpret_cast<void *>(&sw_tiles_kernel<32, true, kG>)
+              : reinterpret_cast<void *>(&sw_tiles_kernel<32, false, kG>);
 }
 
+// Kernel of variant k at strip S; group_items: the allocation-free
+// one-vs-all instantiation.
+void *kernel_ptr(int k, int S, bool acgt, bool group_items = false) {
+  return group_items ? kernel_ptr_g<true>(k, S, acgt) : kernel_ptr_g<false>(k, S, acgt);
+}
+
+// Resident waves per CU for a launch of kernel variant k at strip S: the
+// tuned or requested value, else 12 for the LDS kernels at S = 32 (measured
+// on MI355X with 20,000 sequences of 100-600 residues: 3 of their 4 possible
+// waves per SIMD run best), else the occupancy limit.
+int launch_waves_per_cu(const mnhn_sw_ctx *c, int k, int S, const void *kp) {
+  int wpc = c->waves_per_cu;
+  if (wpc <= 0) {
+    const char *ev = getenv("MNHN_SW_WAVES_PER_CU");
+    wpc = ev ? atoi(ev) : 0;
+  }
+  if (wpc <= 0 && (k == KERNEL_LDS || k == KERNEL_LDS_F16) && S == 32) wpc = 12;
+  if (wpc <= 0) {
+    int blocks_per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) != hipSuccess ||
+        blocks_per_cu <= 0)
+      blocks_per_cu = 1;
+    wpc = blocks_per_cu * (kBlock / kWave);
+  }
+  return wpc;
+}
+
+}  // namespace
+
+// Device scratch of the allocation-free entry point (mnhn_sw_one_vs_all_noalloc):
+// its work-queue counters (first 256 bytes) and the strip-boundary workspaces
+// of its launches.  Each GPU that the library runs on holds its own copy, from
+// the code object's load until the process ends.  In host code `scratch` only
+// names the symbol for hipGetSymbolAddress: kernels and copies receive the
+// device address that call returns (scratch_address), never the name.
+static constexpr size_t kScratchBytes = 64ull << 20;
+static constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+bool g_scratch_done[kMaxDevices];
+
+// The address of `scratch` on device dev, which must be current; resolved
+// once per device.  nullptr if the runtime cannot resolve it.
+void *scratch_address(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = e == hipSuccess ? ptr : nullptr;
+    g_scratch_done[dev] = true;
+  }
+  return g_scratch_ptr[dev];
+}
+
+__global__ void zero_words_kernel(int *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
 }  // namespace
 
 int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_max,
@@ -1122,11 +1209,6 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
 
   const int K = w.dev ? (w.dev_kernel ? w.dev_kernel : KERNEL_LDS) : pick_kernel(c);
   const int S = pick_strip(c);
-  int wpc_req = c->waves_per_cu;
-  if (wpc_req <= 0) {
-    const char *ev = getenv("MNHN_SW_WAVES_PER_CU");
-    wpc_req = ev ? atoi(ev) : 0;
-  }
   // passes: (kernel, items).  The fp16 variant takes the items whose pairs
   // all satisfy match * min(la, lb) <= 2048; the rest run the int16 LDS kernel.
   struct Pass {
@@ -1242,17 +1324,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     const std::vector<int4> &items = pass.items;
     const size_t n_items = w.dev ? (size_t)w.n_items : items.size();
     void *kp = kernel_ptr(PK, PS, c->acgt);
-    int wpc = wpc_req;
-    // measured on MI355X (20,000 sequences of 100-600 residues): the LDS kernels at S = 32 run
-    // best with 3 of their 4 possible waves per SIMD resident
-    if (wpc <= 0 && (PK == KERNEL_LDS || PK == KERNEL_LDS_F16) && PS == 32) wpc = 12;
-    if (wpc <= 0) {
-      int blocks_per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) !=
-              hipSuccess || blocks_per_cu <= 0)
-        blocks_per_cu = 1;
-      wpc = blocks_per_cu * (kBlock / kWave);
-    }
+    const int wpc = launch_waves_per_cu(c, PK, PS, kp);
     int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
     const int max_useful = (int)((n_items + 3) / 4);
     if (blocks > max_useful) blocks = std::max(1, max_useful);
@@ -1590,6 +1662,8 @@ int mnhn_sw_create(const char *const *seqs, const size_t *lens, int n,
     return MNHN_ERR_DEVICE;
   }
   c->num_cu = prop.multiProcessorCount;
+  // scratch of the allocation-free entry point on this (current) device
+  c->scratch = scratch_address(c->device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return MNHN_ERR_DEVICE;
@@ -1681,6 +1755,70 @@ int mnhn_sw_one_vs_all(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
   }
   dev_free(d_out);
   return rc;
+}
+
+int mnhn_sw_one_vs_all_noalloc(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
+  if (!c || !dist_out || query < 0 || query >= c->n) return MNHN_ERR_ARG;
+  if (!c->scratch) return MNHN_ERR_DEVICE;
+  // workspace rows: target residues (LDS kernels) or query residues (register
+  // and 32-bit kernels), at most the longest sequence; one launch needs at
+  // least one workgroup's four workspaces in the scratch
+  const int stride = (c->maxlen + 2) * kWave;
+  const size_t ws_waves = (kScratchBytes - kScratchHead) / ((size_t)stride * sizeof(uint32_t));
+  if (ws_waves < (size_t)(kBlock / kWave)) return MNHN_ERR_RANGE;
+  const int la = c->len[query];
+  const int K = pick_kernel(c);
+  int S = pick_strip(c);
+  if ((K == KERNEL_LDS || K == KERNEL_LDS_F16) && S == 64 && la <= 128) S = 32;
+  // items are the target groups in length order (longest first), the query
+  // against each; a group's score bound match * min(gmax, la) falls along
+  // the order, so the groups needing 32-bit lanes are a prefix and (fp16
+  // form) those within 2048 a suffix
+  const int G = c->groups;
+  auto first_within = [&](int from, long long limit) {
+    int lo = from, hi = G;
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if ((long long)c->match * std::min(c->gmax[mid], la) > limit) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int gA = first_within(0, kI16ScoreMax);
+  const int gB = K == KERNEL_LDS_F16 ? first_within(gA, 2048) : G;
+  struct {
+    int kernel, strip, g0, g1;
+  } const passes[3] = {{KERNEL_I32, kI32Strip, 0, gA},
+                       {K == KERNEL_LDS_F16 ? KERNEL_LDS : K, S, gA, gB},
+                       {KERNEL_LDS_F16, S, gB, G}};
+  int *counters = reinterpret_cast<int *>(c->scratch);
+  uint32_t *ws = reinterpret_cast<uint32_t *>(static_cast<unsigned char *>(c->scratch) + kScratchHead);
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(kWave), 0, c->stream, counters, 3);
+  hipError_t e = hipGetLastError();
+  for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+    const int n_items = passes[k].g1 - passes[k].g0;
+    if (n_items <= 0) continue;
+    void *kp = kernel_ptr(passes[k].kernel, passes[k].strip, c->acgt, true);
+    const int wpc = launch_waves_per_cu(c, passes[k].kernel, passes[k].strip, kp);
+    int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
+    blocks = std::min(blocks, (n_items + 3) / 4);
+    blocks = std::max(1, (int)std::min<size_t>((size_t)blocks, ws_waves / (kBlock / kWave)));
+    KArgs a{};
+    a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
+    a.gtgt = c->d_gtgt; a.gmax = c->d_gmax; a.gmin = c->d_gmin;
+    a.counter = counters + k;
+    a.bound = ws; a.bound_stride = stride;
+    a.dist_out = dist_out;
+    a.n_items = n_items;
+    a.item_base = passes[k].g0; a.query = query;
+    a.mode = MODE_DIST; a.triangle = 0; a.eps_max = 0;
+    a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
+    void *args[] = {&a};
+    e = hipLaunchKernel(kp, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MNHN_OK : MNHN_ERR_DEVICE;
 }
 
 int mnhn_sw_block(mnhn_sw_ctx *c, const int32_t *rows, int nr, const int32_t *cols, int nc,

@@ -450,7 +450,7 @@ def run_vec_bench(args, pkg, log):
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                         "kernel": "vec_tiles_kernel<2>",
                         # issue-rate bound of the kernel's instruction mix (measured
-                        # on gfx950 by bin/valu_rates): per pair-dimension a
+                        # on gfx950 with an issue-rate probe): per pair-dimension a
                         # v_sub_f32 (2 cycles per wave64) and a v_fmac_f32 (4)
                         "instruction_roofline_pair_dims_s": L2_INSTR_PAIR_DIMS_S,
                         "instruction_frac": res["l2_pca"]["pair_dims_per_s"] / L2_INSTR_PAIR_DIMS_S}}

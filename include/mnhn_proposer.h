@@ -57,6 +57,8 @@ typedef struct {
   double verify_kernel_ms; /* SW kernel time of the verification (HIP events) */
   int64_t stage2_tiles;    /* band tiles that ran the cosine stage (counted on the device) */
   double propose_kernel_ms; /* pairwise kernel time alone (HIP events around its launch) */
+  int32_t counts_int8;     /* 1: the cosine stage ran on int8 counts (int8 MFMA), 0: bf16 */
+  int32_t reserved;
 } mnhn_prefilter_stats;
 
 /* 5-mer profiles of every sequence (n x 1024 fp32, host buffer). */

@@ -125,7 +125,8 @@ class PrefilterStats(ctypes.Structure):
                 ("propose_ms", ctypes.c_double), ("verify_ms", ctypes.c_double),
                 ("csr_ms", ctypes.c_double), ("tiles", ctypes.c_int64), ("cells", ctypes.c_double),
                 ("verify_kernel_ms", ctypes.c_double), ("stage2_tiles", ctypes.c_int64),
-                ("propose_kernel_ms", ctypes.c_double)]
+                ("propose_kernel_ms", ctypes.c_double), ("counts_int8", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 _LIB = None

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <random>
 #include <vector>
 
@@ -33,6 +34,7 @@ namespace {
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int D0 = MNHN_KMER_DIM, D1 = 512, D2 = 256, D3 = MNHN_EMBED_DIM;
 constexpr float kProfileScale = 64.0f;  // profile entries scaled from 1/L to O(0.1)
@@ -319,11 +321,17 @@ __global__ __launch_bounds__(256) void sorted_rows_kernel(const bf16 *__restrict
 // One wave per sorted row p: raw 5-mer counts of sequence order[p] as bf16
 // (integers up to 256 are exact) and their squared norm, for the profile
 // cosine of the pairwise stage.
+// Raw 5-mer counts of every sequence in length order (row p = sequence
+// order[p]) and their squared norms.  T = bf16 (integers exact up to 256), or
+// int8 for the int8 MFMA stage: counts above 127 set *over (the host then
+// builds the bf16 rows instead).
+template <typename T>
 __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t *__restrict__ words,
                                                                  const uint32_t *__restrict__ woff,
                                                                  const int32_t *__restrict__ len,
                                                                  const int32_t *__restrict__ order, int n,
-                                                                 bf16 *__restrict__ Xs, float *__restrict__ pnorm) {
+                                                                 T *__restrict__ Xs, float *__restrict__ pnorm,
+                                                                 int *__restrict__ over) {
   __shared__ int hist[D0];
   __shared__ float red[256];
   for (int p = blockIdx.x; p < n; p += gridDim.x) {
@@ -345,9 +353,18 @@ __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t 
     __syncthreads();
     float ss = 0.f;
     for (int c = threadIdx.x; c < D0; c += 256) {
-      const bf16 v = (bf16)(float)hist[c];
-      Xs[(size_t)p * D0 + c] = v;
-      ss += (float)v * (float)v;
+      const int h = hist[c];
+      float f;
+      if constexpr (std::is_same<T, int8_t>::value) {
+        if (h > 127) atomicOr(over, 1);
+        Xs[(size_t)p * D0 + c] = (int8_t)min(h, 127);
+        f = (float)h;
+      } else {
+        const bf16 v = (bf16)(float)h;
+        Xs[(size_t)p * D0 + c] = v;
+        f = (float)v;
+      }
+      ss += f * f;
     }
     red[threadIdx.x] = ss;
     __syncthreads();
@@ -374,17 +391,17 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 8 * (lane >> 4);
-  // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread
+  // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread.  Rows past
+  // n load row n - 1 instead (no branch): their products are never used, as
+  // the callers' tests reject every row or column at or beyond n.
   bf16x8 va[4], vb[4];
   auto load = [&](int kh) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
       const int r = ch >> 3, k = (ch & 7) * 8;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) { va[t][q] = (bf16)0.0f; vb[t][q] = (bf16)0.0f; }
-      if (i0 + r < n) va[t] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + r) * K + kh + k);
-      if (j0 + r < n) vb[t] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + r) * K + kh + k);
+      va[t] = *reinterpret_cast<const bf16x8 *>(A + (size_t)min(i0 + r, n - 1) * K + kh + k);
+      vb[t] = *reinterpret_cast<const bf16x8 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
     }
   };
   auto store = [&](int stage) {
@@ -423,6 +440,66 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
   }
 }
 
+// tile_dot over int8 operands (raw 5-mer counts <= 127) on the int8 MFMA
+// (v_mfma_i32_16x16x64_i8): int32 sums, exact.  Chunks of 128 bytes per row
+// (two MFMA k-steps of 64) in the same LDS footprint and double buffering as
+// the bf16 form, so a chunk moves the same bytes for twice the k.  Rows past
+// n load row n - 1, as in tile_dot.
+template <int K>
+__device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const int8_t *__restrict__ B, int n,
+                                            int i0, int j0, int8_t *sA, int8_t *sB, i32x4 (&acc)[4][4]) {
+  constexpr int KH = 128, LK = KH + 16, STG = 128 * LK;  // bytes
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int fr = lane & 15, fk = 16 * (lane >> 4);
+  // 128 rows x 128 bytes = 1024 pieces of 16 bytes per operand; 4 per thread
+  uint4 va[4], vb[4];
+  auto load = [&](int kh) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ch = tid + 256 * t;
+      const int r = ch >> 3, k = (ch & 7) * 16;
+      va[t] = *reinterpret_cast<const uint4 *>(A + (size_t)min(i0 + r, n - 1) * K + kh + k);
+      vb[t] = *reinterpret_cast<const uint4 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
+    }
+  };
+  auto store = [&](int stage) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ch = tid + 256 * t;
+      const int r = ch >> 3, k = (ch & 7) * 16;
+      *reinterpret_cast<uint4 *>(sA + stage * STG + r * LK + k) = va[t];
+      *reinterpret_cast<uint4 *>(sB + stage * STG + r * LK + k) = vb[t];
+    }
+  };
+  __syncthreads();  // the previous user of both stages is done
+  load(0);
+  store(0);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c * KH < K; ++c) {
+    const int cur = c & 1;
+    const bool more = (c + 1) * KH < K;
+    if (more) load((c + 1) * KH);
+    const int8_t *a = sA + cur * STG, *b = sB + cur * STG;
+#pragma unroll
+    for (int ks = 0; ks < KH; ks += 64) {
+      i32x4 af[4], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const i32x4 *>(a + (64 * wm + 16 * i + fr) * LK + ks + fk);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const i32x4 *>(b + (64 * wn + 16 * j + fr) * LK + ks + fk);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+}
+
 // 128 x 128 tiles over the length band: the host lists, per row tile ti, the
 // column tiles ti .. that the length bound can reach; tile_off[ti] is the
 // first block of row tile ti.  4 waves of 64 x 64 per tile.  Stage 1: Es Es^T
@@ -431,10 +508,10 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
 // test dot >= cmin * |x_p| |x_q|, cmin = cmin of the longer sequence (< 0:
 // no test).  Hits are counted first and reserved with one atomic per wave,
 // then written (sorted positions p << 32 | q) at consecutive addresses.
-template <bool kCos>
+template <bool kCos, bool kI8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void pair_candidates_kernel(
     const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
-    const bf16 *__restrict__ Xs, const float *__restrict__ pnorm, const float *__restrict__ cmin,
+    const void *__restrict__ Xv, const float *__restrict__ pnorm, const float *__restrict__ cmin,
     const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
     unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap,
     unsigned long long *__restrict__ stage2) {
@@ -442,7 +519,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
   __shared__ __attribute__((aligned(16))) bf16 sA[2 * BT * LK];
   __shared__ __attribute__((aligned(16))) bf16 sB[2 * BT * LK];
   __shared__ int any_hit;
-  const int blk = blockIdx.x;
+  // Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8),
+  // each XCD with its own L2.  Band tiles of one row tile are consecutive and
+  // share their row operand (and, with the next row tiles, most column
+  // operands), so each XCD takes a contiguous run of tiles: workgroup b
+  // computes tile (b % 8) * run + b / 8 (runs of per or per + 1 tiles).
+  int blk;
+  {
+    constexpr int kXcds = 8;
+    const int nb = (int)gridDim.x, per = nb / kXcds, rem = nb % kXcds;
+    const int x = (int)blockIdx.x % kXcds, k = (int)blockIdx.x / kXcds;
+    blk = x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+  }
   int lo = 0, hi = tiles;  // tile_off[lo] <= blk < tile_off[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -466,16 +554,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
     tile_dot<D3>(Es, Es, n, i0, j0, sA, sB, acc);
+    // row / column terms are loaded at clamped indices (no branches);
     // out-of-range rows and columns get lengths that no |dl| <= max_dl matches
-    float nj[4], sj[4];
+    float nj[4], sj2[4];
     int lj[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-      const bool v = gj < n;
-      nj[j] = v ? norm[gj] : 0.f;
-      sj[j] = v ? srad[gj] : 0.f;
-      lj[j] = v ? lens[gj] : (1 << 29);
+      const int cj = min(gj, n - 1);
+      const float sj = srad[cj];
+      nj[j] = norm[cj];
+      sj2[j] = sj * sj;
+      lj[j] = gj < n ? lens[cj] : (1 << 29);
     }
     const bool diag = ti == tj;
 #pragma unroll
@@ -483,20 +573,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-        const bool v = gi < n;
-        const float ni = v ? norm[gi] : 0.f;
-        const float si = v ? srad[gi] : 0.f;
-        const int li = v ? lens[gi] : -(1 << 28);
+        const int ci = min(gi, n - 1);
+        const float ni = norm[ci];
+        const float si = srad[ci];
+        const float si2 = si * si;
+        const int li = gi < n ? lens[ci] : -(1 << 28);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
           const int dl = li - lj[j];
-          const float rad = dl >= 0 ? si : sj[j];
+          const float rad2 = dl >= 0 ? si2 : sj2[j];
           const float d2 = ni + nj[j] - 2.0f * acc[i][j][r];
-          const bool h = (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad * rad;
+          const bool h = (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad2;
           const int bit = 16 * i + 4 * r + j;
-          m1 |= (unsigned long long)h << bit;
-          mlong |= (unsigned long long)(dl >= 0) << bit;
+          m1 |= h ? 1ull << bit : 0ull;
+          mlong |= dl >= 0 ? 1ull << bit : 0ull;
         }
       }
   }
@@ -507,62 +598,83 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
     if (any_hit == 0) return;  // uniform over the block
     if (tid == 0) atomicAdd(stage2, 1ull);
     f32x4 accx[4][4];
+    if constexpr (kI8) {
+      i32x4 acci[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) accx[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    tile_dot<D0>(Xs, Xs, n, i0, j0, sA, sB, accx);
+        for (int j = 0; j < 4; ++j) acci[i][j] = (i32x4){0, 0, 0, 0};
+      const int8_t *X8 = static_cast<const int8_t *>(Xv);
+      tile_dot_i8<D0>(X8, X8, n, i0, j0, reinterpret_cast<int8_t *>(sA), reinterpret_cast<int8_t *>(sB), acci);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) accx[i][j][r] = (float)acci[i][j][r];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) accx[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      const bf16 *Xs = static_cast<const bf16 *>(Xv);
+      tile_dot<D0>(Xs, Xs, n, i0, j0, sA, sB, accx);
+    }
     float pj[4], cj[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-      const bool v = gj < n;
-      pj[j] = v ? pnorm[gj] : 0.f;
-      cj[j] = v ? cmin[gj] : -1.f;
+      const int cc = min(gj, n - 1);
+      pj[j] = pnorm[cc];
+      cj[j] = gj < n ? cmin[cc] : -1.f;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-        const bool v = gi < n;
-        const float pi = v ? pnorm[gi] : 0.f;
-        const float ci = v ? cmin[gi] : -1.f;
+        const int cc = min(gi, n - 1);
+        const float pi = pnorm[cc];
+        const float ci = gi < n ? cmin[cc] : -1.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int bit = 16 * i + 4 * r + j;
           const float c = ((mlong >> bit) & 1ull) ? ci : cj[j];
           const float dot = accx[i][j][r];
-          if (c >= 0.0f && !(dot >= 0.0f && dot * dot >= c * c * pi * pj[j])) m2 &= ~(1ull << bit);
+          const bool fail = c >= 0.0f && !(dot >= 0.0f && dot * dot >= c * c * pi * pj[j]);
+          m2 &= fail ? ~(1ull << bit) : ~0ull;
         }
       }
   }
-  unsigned total = 0;
+  // Emission: each lane's hit count, a wave-wide prefix sum of the counts,
+  // one reservation per wave, then each lane writes its own hits (ascending
+  // bits) at consecutive positions.  Candidates are later grouped by row and
+  // sorted within rows, so their order in the buffer does not matter.
+  // The prefix sum is taken digit by digit: counts are at most 64, so
+  // prefix = sum over b of 2^b * (lanes below with bit b of their count set).
+  const unsigned cnt = (unsigned)__popcll(m2);
+  unsigned below = 0, total = 0;
 #pragma unroll
-  for (int b = 0; b < 64; ++b) total += (unsigned)__popcll(__ballot((m2 >> b) & 1ull));
+  for (int b = 0; b < 7; ++b) {
+    const unsigned long long bal = __ballot((cnt >> b) & 1u);
+    below += __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u)) << b;
+    total += (unsigned)__popcll(bal) << b;
+  }
   if (total == 0) return;
   unsigned long long base = 0;
   if (lane == 0) base = atomicAdd(count, (unsigned long long)total);
   base = uniform_u64(base);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int bit = 16 * i + 4 * r + j;
-        const bool h = (m2 >> bit) & 1ull;
-        const unsigned long long bal = __ballot(h);
-        if (h) {
-          const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-          const unsigned long long pos = base + below;
-          const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-          const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-          if (pos < cap) out[pos] = ((unsigned long long)gi << 32) | (unsigned)gj;
-        }
-        base += (unsigned long long)__popcll(bal);
-      }
+  unsigned long long pos = base + below;
+  unsigned long long m = m2;
+  while (m) {
+    const int bit = __builtin_ctzll(m);
+    m &= m - 1ull;
+    const int i = bit >> 4, r = (bit >> 2) & 3, j = bit & 3;
+    const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+    const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+    if (pos < cap) out[pos] = ((unsigned long long)gi << 32) | (unsigned)gj;
+    ++pos;
+  }
 }
 
 // Wave-aggregated per-row atomics: candidates next to each other in the
@@ -1165,7 +1277,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     (void)hipEventRecord(e0, st);
     return (double)ms;
   };
-  bf16 *X = nullptr, *E16 = nullptr, *Es = nullptr, *Xs = nullptr;
+  bf16 *X = nullptr, *E16 = nullptr, *Es = nullptr;
+  void *Xs = nullptr;  // length-ordered raw 5-mer counts: int8, or bf16 when a count exceeds 127
+  bool xs_i8 = true;
+  int *d_over = nullptr;
   float *norm = nullptr, *srad = nullptr, *cmin = nullptr, *pnorm = nullptr;
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
@@ -1176,7 +1291,9 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   int4 *items = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
-    dev_free(Xs); dev_free(cmin); dev_free(pnorm);
+    if (Xs) (void)hipFree(Xs);
+    Xs = nullptr;
+    dev_free(d_over); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
     dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs);
@@ -1216,10 +1333,36 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
                      (float)eps_max / (float)c->match, enc->margins, enc->cosmin, Es, norm, srad, cmin, lens);
   if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
   if (use_cos) {
-    if ((rc = dev_alloc(&Xs, (size_t)n * D0)) || (rc = dev_alloc(&pnorm, (size_t)n))) return fail(rc);
-    hipLaunchKernelGGL(kmer_counts_sorted_kernel, dim3((unsigned)std::min(n, 16384)), dim3(256), 0, st,
-                       c->d_words, c->d_woff, c->d_len, d_order, n, Xs, pnorm);
-    if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    // int8 counts for the int8 MFMA stage; bf16 rows (the same products,
+    // twice the bytes and MFMA time) only when some count exceeds 127
+    int8_t *X8 = nullptr;
+    if ((rc = dev_alloc(&X8, (size_t)n * D0)) || (rc = dev_alloc(&pnorm, (size_t)n)) ||
+        (rc = dev_alloc(&d_over, 1))) {
+      dev_free(X8);
+      return fail(rc);
+    }
+    Xs = X8;
+    int over = 0;
+    hipError_t e = hipMemsetAsync(d_over, 0, sizeof(int), st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(kmer_counts_sorted_kernel<int8_t>, dim3((unsigned)std::min(n, 16384)), dim3(256), 0, st,
+                         c->d_words, c->d_woff, c->d_len, d_order, n, X8, pnorm, d_over);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    if (over) {
+      (void)hipFree(Xs);
+      Xs = nullptr;
+      xs_i8 = false;
+      bf16 *X16 = nullptr;
+      if ((rc = dev_alloc(&X16, (size_t)n * D0))) return fail(rc);
+      Xs = X16;
+      hipLaunchKernelGGL(kmer_counts_sorted_kernel<bf16>, dim3((unsigned)std::min(n, 16384)), dim3(256), 0, st,
+                         c->d_words, c->d_woff, c->d_len, d_order, n, X16, pnorm, d_over);
+      if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    }
   }
   dev_free(E16);
   stt.embed_ms = phase_ms();
@@ -1261,12 +1404,15 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     hipError_t e = hipMemsetAsync(d_cnt64, 0, 16, st);
     if (e == hipSuccess) {
       (void)hipEventRecord(ek0, st);
-      if (use_cos)
-        hipLaunchKernelGGL(pair_candidates_kernel<true>, dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs, pnorm,
-                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+      if (use_cos && xs_i8)
+        hipLaunchKernelGGL((pair_candidates_kernel<true, true>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
+                           pnorm, cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+      else if (use_cos)
+        hipLaunchKernelGGL((pair_candidates_kernel<true, false>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
+                           pnorm, cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       else
-        hipLaunchKernelGGL(pair_candidates_kernel<false>, dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs, pnorm,
-                           cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+        hipLaunchKernelGGL((pair_candidates_kernel<false, false>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
+                           pnorm, cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       e = hipGetLastError();
       (void)hipEventRecord(ek1, st);
     }
@@ -1276,6 +1422,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
     m = cnt2[0];
     stt.stage2_tiles = (int64_t)cnt2[1];
+    stt.counts_int8 = use_cos && xs_i8 ? 1 : 0;
     float kms = 0.f;
     (void)hipEventElapsedTime(&kms, ek0, ek1);
     stt.propose_kernel_ms = kms;
@@ -1283,7 +1430,9 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     cap = m;  // overflowed: exact size, recompute once
   }
   if (rc || m > cap) return fail(rc ? rc : MNHN_ERR_NOMEM);
-  dev_free(Es); dev_free(norm); dev_free(srad); dev_free(lens); dev_free(Xs); dev_free(pnorm); dev_free(cmin);
+  dev_free(Es); dev_free(norm); dev_free(srad); dev_free(lens); dev_free(pnorm); dev_free(cmin);
+  if (Xs) (void)hipFree(Xs);
+  Xs = nullptr;
   stt.candidates = (int64_t)m;
   stt.propose_ms = phase_ms();
   // ---- exact verification: candidates grouped per row into targets of 128

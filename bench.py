@@ -221,7 +221,11 @@ def proposer_eval(args, pkg, log, config):
     t0 = time.perf_counter()
     prp, pcol, pdist, pst = enc.prefilter_csr(eng, eps)
     pref_lists_s = time.perf_counter() - t0
-    log(f"prefilter lists: {pref_lists_s:.2f} s {pst}, nnz {len(pcol)}")
+    vt = eng.last_timing()
+    verify_forms = {name: {"cells": vt["form_cells"][f], "kernel_ms": vt["form_ms"][f],
+                           "gcups": vt["form_cells"][f] / max(vt["form_ms"][f], 1e-9) / 1e6}
+                    for f, name in enumerate(("fp16", "int16", "int32")) if vt["form_cells"][f] > 0}
+    log(f"prefilter lists: {pref_lists_s:.2f} s {pst}, nnz {len(pcol)}; verification by form {verify_forms}")
     _, _, missed = pair_set_recall(prp, pcol, pdist, erp, ecol, edist)
     subset, recall = lists_subset_and_recall(prp, pcol, pdist, erp, ecol, edist)
     precision = pst["neighbours"] / max(pst["candidates"], 1)
@@ -258,6 +262,7 @@ def proposer_eval(args, pkg, log, config):
            "exact_cells_computed": et["cells"], "all_pairs": all_pairs,
            "all_pairs_cells": 0.5 * (L.sum() ** 2 - (L * L).sum()),
            "verify_gcups": pst["cells"] / max(pst["verify_kernel_ms"], 1e-9) / 1e6,
+           "verify_forms": verify_forms,
            "prefilter_phases_ms": pst, "training": info}
     t0 = time.perf_counter()
     ecl = pkg.dbscan_csr(erp, ecol, edist, eps, eps, minpts)

@@ -734,46 +734,88 @@ __global__ void cand_scatter_kernel(const unsigned long long *__restrict__ c, lo
   }
 }
 
+// Per row i, the targets' length bands of its candidates (ascending q =
+// descending target length, so each band is a run): kband[2i] targets longer
+// than t2 (pairs beyond the 16-bit lanes), kband[2i + 1] targets longer than
+// t1 (pairs beyond the exact fp16 form).  The row's sequence is the longer of
+// each of its pairs, so the target's length bounds the pair's score.
+__global__ void band_counts_kernel(const int32_t *__restrict__ sorted, const long long *__restrict__ rs,
+                                   const int32_t *__restrict__ order, const int32_t *__restrict__ len, int n,
+                                   int t1, int t2, int32_t *__restrict__ kband) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = rs[i];
+  const int cnt = (int)(rs[i + 1] - b);
+  auto first_at_most = [&](int t) {  // first k with len <= t
+    int lo = 0, hi = cnt;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (len[order[sorted[b + mid]]] > t) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  kband[2 * i] = first_at_most(t2);
+  kband[2 * i + 1] = first_at_most(t1);
+}
+
 // One workgroup per row i: its ascending q (= descending target length) in
-// target groups of 128, so each group holds targets of similar length; the
-// group's first target is its longest, its last the shortest.  tlen[i] = sum
-// of the row's target lengths.
+// target groups of 128 per length band (band 0: k < kband[2i], band 1: up to
+// kband[2i + 1], band 2: the rest), so each group holds targets of similar
+// length and one lane format; the group's first target is its longest, its
+// last the shortest.  tlen[3i + band] = sum of the band's target lengths.
 __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restrict__ sorted,
                                                          const long long *__restrict__ rs,
                                                          const int32_t *__restrict__ gbase,
+                                                         const int32_t *__restrict__ kband,
                                                          const int32_t *__restrict__ order,
                                                          const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
                                                          int32_t *__restrict__ gmax, int32_t *__restrict__ gmin,
                                                          unsigned long long *__restrict__ tlen) {
-  __shared__ unsigned long long red[256];
+  __shared__ unsigned long long red[3][256];
   const int i = blockIdx.x;
   const long long b = rs[i];
   const int cnt = (int)(rs[i + 1] - b);
-  unsigned long long sum = 0;
+  const int e0 = kband[2 * i], e1 = kband[2 * i + 1];
+  const int g0 = (e0 + kGroupSize - 1) / kGroupSize, g1 = (e1 - e0 + kGroupSize - 1) / kGroupSize;
+  unsigned long long sum[3] = {0, 0, 0};
   for (int k = threadIdx.x; k < cnt; k += 256) {
+    const int band = k < e0 ? 0 : k < e1 ? 1 : 2;
+    const int from = band == 0 ? 0 : band == 1 ? e0 : e1;
+    const int to = band == 0 ? e0 : band == 1 ? e1 : cnt;
+    const int gfirst = gbase[i] + (band == 0 ? 0 : band == 1 ? g0 : g0 + g1);
+    const int r = k - from;
+    const int g = gfirst + r / kGroupSize, slot = r % kGroupSize;
     const int j = order[sorted[b + k]];
-    const int g = gbase[i] + k / kGroupSize, slot = k % kGroupSize;
     const int lj = len[j];
     gtgt[(size_t)g * kGroupSize + slot] = j;
     if (slot == 0) gmax[g] = lj;
-    if (slot == kGroupSize - 1 || k == cnt - 1) gmin[g] = lj;
-    sum += (unsigned long long)lj;
+    if (slot == kGroupSize - 1 || k == to - 1) gmin[g] = lj;
+    sum[band] += (unsigned long long)lj;
   }
-  red[threadIdx.x] = sum;
+  for (int t = 0; t < 3; ++t) red[t][threadIdx.x] = sum[t];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if ((int)threadIdx.x < o)
+      for (int t = 0; t < 3; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) tlen[i] = red[0];
+  if (threadIdx.x < 3) tlen[3 * i + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// items of row i (one per target group) at ibase[i] ..
-__global__ void cand_items_kernel(const int32_t *__restrict__ gbase, const int32_t *__restrict__ ibase, int n,
+// items of row i, one per target group: band t's groups at ibase[3i + t] ..
+__global__ void cand_items_kernel(const int32_t *__restrict__ gbase, const int32_t *__restrict__ kband,
+                                  const long long *__restrict__ rs, const int32_t *__restrict__ ibase, int n,
                                   int4 *__restrict__ items) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (int g = gbase[i]; g < gbase[i + 1]; ++g) items[ibase[i] + (g - gbase[i])] = make_int4(g, i, i + 1, 0);
+  const int cnt = (int)(rs[i + 1] - rs[i]);
+  const int e[3] = {kband[2 * i], kband[2 * i + 1] - kband[2 * i], cnt - kband[2 * i + 1]};
+  int g = gbase[i];
+  for (int t = 0; t < 3; ++t) {
+    const int ng = (e[t] + kGroupSize - 1) / kGroupSize;
+    for (int q = 0; q < ng; ++q, ++g) items[ibase[3 * i + t] + q] = make_int4(g, i, i + 1, 0);
+  }
 }
 
 __global__ void iota_kernel(int32_t *a, int n) {
@@ -1285,7 +1327,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr,
-          *ibase = nullptr, *seg = nullptr, *sorted = nullptr;
+          *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr;
   unsigned long long *tlen = nullptr;
   long long *d_rs = nullptr;
   int4 *items = nullptr;
@@ -1296,7 +1338,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     dev_free(d_over); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
-    dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs);
+    dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs); dev_free(kband);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(ek0);
@@ -1440,7 +1482,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   if (m > 0) {
     std::vector<int32_t> cnt(n), gb(n + 1, 0);
     if ((rc = dev_alloc(&rowcnt, (size_t)n)) || (rc = dev_alloc(&gbase, (size_t)n + 1)) ||
-        (rc = dev_alloc(&cursor, (size_t)n)) || (rc = dev_alloc(&qids, (size_t)n)))
+        (rc = dev_alloc(&cursor, (size_t)n)) || (rc = dev_alloc(&qids, (size_t)n)) ||
+        (rc = dev_alloc(&kband, 2 * (size_t)n)))
       return fail(rc);
     hipError_t e = hipMemsetAsync(rowcnt, 0, (size_t)n * 4, st);
     hipLaunchKernelGGL(cand_count_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand, (long long)m,
@@ -1448,30 +1491,6 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), rowcnt, (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + (cnt[i] + kGroupSize - 1) / kGroupSize;
-    const int G = gb[n];
-    // item order by row class: rows whose pairs all fit the exact fp16 form
-    // (the row's sequence is the longer of each of its pairs, so
-    // match * len <= 2048 suffices), first those short enough for a 32-column
-    // strip, then the rest; then the rows that need the int16 form; then the
-    // rows whose scores may pass 16 bits (the 32-bit kernel)
-    const int S = context_strip(c);
-    auto row_class = [&](int i) {
-      if ((long long)c->match * c->len[i] > 65535) return 3;
-      if ((long long)c->match * c->len[i] > 2048) return 2;
-      return (S == 64 && c->len[i] <= 128) ? 0 : 1;
-    };
-    std::vector<int32_t> ib(n);
-    int part_begin[5] = {0, 0, 0, 0, 0};
-    {
-      int o = 0;
-      for (int cls = 0; cls < 4; ++cls) {
-        part_begin[cls] = o;
-        for (int i = 0; i < n; ++i)
-          if (row_class(i) == cls) { ib[i] = o; o += gb[i + 1] - gb[i]; }
-      }
-      part_begin[4] = o;
-    }
     // row segments of the candidates, sorted by target position within rows
     std::vector<long long> rsh((size_t)n + 1, 0);
     for (int i = 0; i < n; ++i) rsh[i + 1] = rsh[i] + cnt[i];
@@ -1486,29 +1505,73 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if ((rc = dev_alloc(&sorted, (size_t)m))) return fail(rc);
     if ((rc = sort_row_segments(st, d_rs, seg, sorted, nullptr, nullptr, n, n))) return fail(rc);
     dev_free(seg);
+    // each row's targets in length bands by the lane format their pairs need
+    // (the row's sequence is the longer of each pair, so the target's length
+    // bounds the score): match * len beyond 65535 -> 32-bit form, beyond 2048
+    // -> packed int16, else exact fp16
+    const int t1 = 2048 / c->match, t2 = 65535 / c->match;
+    std::vector<int32_t> kb(2 * (size_t)n);
+    hipLaunchKernelGGL(band_counts_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, sorted, d_rs, d_order,
+                       c->d_len, n, t1, t2, kband);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(kb.data(), kband, 2 * (size_t)n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    auto band_groups = [&](int i, int t) {
+      const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
+      const int k = t == 0 ? e0 : t == 1 ? e1 - e0 : cnt[i] - e1;
+      return (k + kGroupSize - 1) / kGroupSize;
+    };
+    for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + band_groups(i, 0) + band_groups(i, 1) + band_groups(i, 2);
+    const int G = gb[n];
+    // item order by class: fp16 groups of rows short enough for a 32-column
+    // strip, the other fp16 groups, the int16 groups, the 32-bit groups
+    const int S = context_strip(c);
+    auto band_class = [&](int i, int t) {
+      if (t == 0) return 3;
+      if (t == 1) return 2;
+      return (S == 64 && c->len[i] <= 128) ? 0 : 1;
+    };
+    std::vector<int32_t> ib(3 * (size_t)n, 0);
+    int part_begin[5] = {0, 0, 0, 0, 0};
+    {
+      int o = 0;
+      for (int cls = 0; cls < 4; ++cls) {
+        part_begin[cls] = o;
+        for (int i = 0; i < n; ++i)
+          for (int t = 0; t < 3; ++t)
+            if (band_class(i, t) == cls) {
+              ib[3 * (size_t)i + t] = o;
+              o += band_groups(i, t);
+            }
+      }
+      part_begin[4] = o;
+    }
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||
-        (rc = dev_alloc(&ibase, (size_t)n)) || (rc = dev_alloc(&tlen, (size_t)n)))
+        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 3 * (size_t)n)))
       return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
-    hipLaunchKernelGGL(row_groups_kernel, dim3((unsigned)n), dim3(256), 0, st, sorted, d_rs, gbase, d_order,
+    hipLaunchKernelGGL(row_groups_kernel, dim3((unsigned)n), dim3(256), 0, st, sorted, d_rs, gbase, kband, d_order,
                        c->d_len, gtgt, gmax, gmin, tlen);
-    hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, ibase, n, items);
+    hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, kband, d_rs, ibase, n,
+                       items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
-    std::vector<unsigned long long> h_tlen(n);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, (size_t)n * 8, hipMemcpyDeviceToHost, st);
+    std::vector<unsigned long long> h_tlen(3 * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 3 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     double part_cells[4] = {0, 0, 0, 0};
-    for (int i = 0; i < n; ++i) {
-      const double ci = (double)c->len[i] * (double)h_tlen[i];
-      stt.cells += ci;
-      part_cells[row_class(i)] += ci;
-    }
+    for (int i = 0; i < n; ++i)
+      for (int t = 0; t < 3; ++t) {
+        const double ci = (double)c->len[i] * (double)h_tlen[3 * (size_t)i + t];
+        stt.cells += ci;
+        part_cells[band_class(i, t)] += ci;
+      }
     dev_free(sorted);
     Work w;
     w.dev = true;

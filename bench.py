@@ -135,15 +135,17 @@ def cpu_baseline(seqs, sample_n, seed=7, threads=1, port=False):
                       f"host: {cpu_model()}, nproc {os.cpu_count()}"}
 
 
-def profiled_traffic(config):
-    """HBM bytes per step of the SW kernels, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
-    plain copy kernel); None if there is no summary for it."""
+def profiled_traffic(config, key="traffic_bytes_fp16_launches"):
+    """HBM bytes of the SW kernels from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a plain copy
+    kernel): `traffic_bytes_fp16_launches` = the fp16-form launches of one
+    step (the roofline's kernel), `traffic_bytes_per_step` = every SW launch
+    of the step; None if there is no summary for it."""
     path = os.path.join(ROOT, "profiles", {"tree20k": "sw_lds_f16_kernel_tree20k.md"}.get(config, "-"))
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_step:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -701,6 +703,7 @@ def main():
                              "engine's stream; peak = the form's instruction roofline)",
                      "frac": f16["gcups"] / f16_peak,
                      "traffic": profiled_traffic(args.config),
+                     "step_traffic": profiled_traffic(args.config, "traffic_bytes_per_step"),
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload, calibrated on a "
                                        "plain copy (profiles/); not measured in this run",
                      "step_kernel_ms": kernel_s * 1e3,

@@ -101,10 +101,10 @@ int mnhn_sw_one_vs_all(mnhn_sw_ctx *ctx, int query, int32_t *dist_out);
  * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * call makes no hipMalloc / hipFree and no host heap allocation; its
  * work-queue counters and strip-boundary workspaces live in a static device
- * array of 64 MiB that each GPU running the library holds until the process
- * ends.  Limit: the context's longest sequence must be at most 65,534
- * residues (one workgroup's workspaces, 4 x (maxlen + 2) x 256 B, fit the
- * array), else MNHN_ERR_RANGE.  All calls on one GPU share that array: calls
+ * array of 96 MiB (plus 256 B of counters) that each GPU running the library
+ * holds until the process ends.  Limit: the context's longest sequence must be
+ * at most 65,534 residues (one workgroup's workspaces, 4 x (maxlen + 2 +
+ * maxlen / 2 + 1) x 256 B, fit the array), else MNHN_ERR_RANGE.  All calls on one GPU share that array: calls
  * that overlap on one GPU (several host threads, several contexts) race and
  * must be serialised by the caller.  Returns once dist_out holds the
  * distances; mnhn_sw_last_timing is not updated. */

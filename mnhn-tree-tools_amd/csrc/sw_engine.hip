@@ -93,6 +93,7 @@ struct KArgs {
   unsigned long long key_cap;
   int n_items;
   int bound_stride;        // uint32 words per wave
+  int list_row;            // LDS kernels: first workspace row of the row-address list
   int item_base;           // group-item kernels: item k is target group item_base + k
   int query;               // against this query
   int mode;
@@ -113,6 +114,69 @@ template <bool kG>
 __device__ __forceinline__ int load_qid(const KArgs &a, int qi) {
   if constexpr (kG) return a.query;
   else return a.qids[qi];
+}
+
+// A wave's workspace as rows of 64 words, [row][lane], behind a buffer
+// resource: the row goes in the wave-uniform scalar offset and the lane's
+// byte offset is fixed, so a row access costs no vector address arithmetic.
+// (Word 3 of the resource is the raw-buffer format of the gfx9 family.)
+// The LDS kernels make every workspace access through it, so the compiler
+// keeps them in program order; a buffer store also keeps the next row's LDS
+// profile reads behind it, so the rows of a strip run one after another
+// (the registers of one row's profile suffice).
+constexpr int kRawBufferWord3 = 0x00020000;
+struct WaveRows {
+  __amdgpu_buffer_rsrc_t r;
+  int lane4;
+  __device__ __forceinline__ WaveRows(uint32_t *base, int words, int lane)
+      : r(__builtin_amdgcn_make_buffer_rsrc(base, (short)0, words * 4, kRawBufferWord3)), lane4(lane * 4) {}
+  __device__ __forceinline__ uint32_t load(int row) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, lane4, row * (kWave * 4), 0);
+  }
+  __device__ __forceinline__ void store(int row, uint32_t v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, lane4, row * (kWave * 4), 0);
+  }
+  // a row that differs between lanes
+  __device__ __forceinline__ uint32_t load_lane_row(int row) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, lane4 + row * (kWave * 4), 0, 0);
+  }
+};
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const u32x4 lds_u4;
+
+// LDS kernels: the DP rows of a work item's target group, decoded once per
+// item instead of once per strip and query.  Word m of the row-address list
+// (workspace row list_row + m) holds, for DP rows j = 2m + 1 (low half) and
+// 2m + 2 (high half), the LDS byte address of the profile row that the
+// lane's two target residues select, (c_lo * 5 + c_hi) * row_bytes past the
+// wave's table, with code 4 for rows past a target's end.  Rows are the
+// target residues 1..nrows; the words read are those of residues
+// 0..16 * ((npairs - 1) / 8) + 15, inside the words the row loop of the
+// per-strip decode read.
+__device__ __forceinline__ void build_row_list(const KArgs &a, const WaveRows &ws, int nrows, uint32_t wo_lo,
+                                               uint32_t wo_hi, int rows_lo, int rows_hi, uint32_t tab_lds,
+                                               uint32_t row_bytes) {
+  const int npairs = (nrows + 1) >> 1;
+  for (int mb = 0; mb * 8 < npairs; ++mb) {
+    const uint32_t w_lo = a.words[wo_lo + (uint32_t)mb], w_hi = a.words[wo_hi + (uint32_t)mb];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int m = mb * 8 + k;
+      if (m >= npairs) break;
+      const int j = 2 * m + 1;  // residue j - 1 = 16 * mb + 2k of both words
+      uint32_t c0l = (w_lo >> (4 * k)) & 3u, c1l = (w_lo >> (4 * k + 2)) & 3u;
+      uint32_t c0h = (w_hi >> (4 * k)) & 3u, c1h = (w_hi >> (4 * k + 2)) & 3u;
+      c0l = j > rows_lo ? 4u : c0l;
+      c1l = j + 1 > rows_lo ? 4u : c1l;
+      c0h = j > rows_hi ? 4u : c0h;
+      c1h = j + 1 > rows_hi ? 4u : c1h;
+      const uint32_t a0 = tab_lds + (c0l * 5u + c0h) * row_bytes;
+      const uint32_t a1 = tab_lds + (c1l * 5u + c1h) * row_bytes;
+      ws.store(a.list_row + m, a0 | (a1 << 16));
+    }
+  }
 }
 
 __device__ __forceinline__ s16x2 as_s16x2(uint32_t v) {
@@ -580,8 +644,12 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
 // per-row selection, no register copies.  Columns left of the query's first
 // residue ("pre-padding" of the first strip) hold -32768, which keeps their
 // cells at the floor.
+// Register budgets: 3 waves per SIMD at strip 64 (168 VGPRs), 5 at strip 32
+// (96).  Measured on MI355X, the extra resident waves hide the wait states
+// between one row's dependent packed ops better than interleaving two rows
+// in one wave (which needs registers for both rows' profiles).
 template <int S, bool kG = false>
-__global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
+__global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_kernel(KArgs a) {
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
   constexpr int kWaves = kBlock / kWave;
@@ -591,6 +659,9 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
   const int wave = blockIdx.x * kWaves + wv;
   uint32_t *tab = tab_all + wv * kPairs * RS;
   uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+  const WaveRows ws(bnd, a.bound_stride, lane);
+  static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
+  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab;
 
   const s16x2 floor2 = {kFloor, kFloor};
   const s16x2 gapv = {(short)-a.gap, (short)-a.gap};
@@ -609,7 +680,6 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
     const int ob = __builtin_amdgcn_readfirstlane(it.w);
 
     const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
-    const int gmn = __builtin_amdgcn_readfirstlane(a.gmin[g]);
     const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
     const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
     const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
@@ -622,7 +692,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
     const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
     const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
     const int nrows = gm > 1 ? gm - 1 : 0;     // DP rows = inner residues of the longest target
-    const int nfast = gmn > 1 ? gmn - 1 : 0;   // rows inside every target of the group
+    build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
 
     for (int qi = qb; qi < qe; ++qi) {
       const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
@@ -637,6 +707,16 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
       auto strip = [&](int s, auto firstc) {
         constexpr bool first = decltype(firstc)::value;
         const int j0 = ncols - (nstrips - s) * S + 1;  // query column of slot 0
+        // The first list word and left boundaries are loaded before the
+        // table build, whose own loads are waited for, so nothing is pending
+        // when the row loop starts: the loop's waits then count only the
+        // loads and stores of one iteration.
+        uint32_t ra = ws.load(a.list_row);
+        uint32_t bA = as_u32(floor2), bB = as_u32(floor2);
+        if (!first) {
+          bA = ws.load(1);
+          if (nrows >= 2) bB = ws.load(2);
+        }
         // ---- score table of this strip
         __builtin_amdgcn_wave_barrier();
         for (int e = lane; e < kPairs * S; e += kWave) {
@@ -658,23 +738,13 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
 #pragma unroll
         for (int jj = 0; jj < S; ++jj) H[jj] = floor2;
         s16x2 diag_b = floor2;
-        // target residues: 16 rows per word; the next block's words are
-        // fetched at the start of each block
-        uint32_t w_lo = a.words[wo_lo], w_hi = a.words[wo_hi];
-        // one DP row; returns the row's left-boundary value for the next row's diagonal
-        auto do_row = [&](int j, uint32_t left_u) {
-          const uint32_t sh = 2u * ((uint32_t)(j - 1) & 15u);
-          uint32_t c_lo = (w_lo >> sh) & 3u;
-          uint32_t c_hi = (w_hi >> sh) & 3u;
-          if (j > nfast) {
-            if (j > rows_lo) c_lo = 4u;
-            if (j > rows_hi) c_hi = 4u;
-          }
-          const uint4 *prow = reinterpret_cast<const uint4 *>(tab + (c_lo * 5u + c_hi) * RS);
+        // one DP row; its left boundary becomes the next row's diagonal
+        auto do_row = [&](int j, uint32_t left_u, uint32_t paddr) {
+          const lds_u4 *prow = (const lds_u4 *)(uintptr_t)paddr;
           s16x2 P[S];
 #pragma unroll
           for (int w = 0; w < S / 4; ++w) {
-            const uint4 v = prow[w];
+            const u32x4 v = prow[w];
             P[4 * w + 0] = as_s16x2(v.x);
             P[4 * w + 1] = as_s16x2(v.y);
             P[4 * w + 2] = as_s16x2(v.z);
@@ -682,37 +752,29 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
           }
           const s16x2 left_b = as_s16x2(left_u);
           dp_row2<S>(H, P, left_b, diag_b, gapv, best, best2);
-          bnd[(uint32_t)j * kWave + (uint32_t)lane] = as_u32(H[S - 1]);
+          ws.store(j, as_u32(H[S - 1]));
           diag_b = left_b;
         };
-        // left-boundary values: even rows in bA, odd rows in bB, each loaded two
-        // rows ahead, so a row never waits on the store issued just before it.
-        // The loads are unconditional so that both rows of an iteration share
-        // one basic block and the scheduler interleaves their dependency
-        // chains; rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
-        // wave's (maxlen + 2) * 64 workspace, and a row past nrows is never used.
-        uint32_t bA = as_u32(floor2), bB = as_u32(floor2);
-        if (!first) {
-          bA = bnd[kWave + (uint32_t)lane];
-          if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
+        // Two rows per iteration; their profile-row addresses and left
+        // boundaries are loaded one iteration ahead, each into the register
+        // whose previous value has just been used: a load that had to be
+        // copied into a loop-carried register would have to complete first.
+        // The loads are unconditional, keeping the iteration one basic block:
+        // boundary rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
+        // wave's maxlen + 2 boundary rows, and list word (j + 1) / 2 <=
+        // nrows / 2 inside its maxlen / 2 + 1 list rows; what is loaded past
+        // nrows is never used.
+        int j = 1;
+        for (; j + 1 <= nrows; j += 2) {
+          const uint32_t p0 = ra & 0xffffu, p1 = ra >> 16;
+          ra = ws.load(a.list_row + ((j + 1) >> 1));
+          __builtin_amdgcn_sched_barrier(0);  // issue the load here, a full iteration ahead
+          do_row(j, bA, p0);
+          if (!first) bA = ws.load(j + 2);
+          do_row(j + 1, bB, p1);
+          if (!first) bB = ws.load(j + 3);
         }
-        for (int blk = 0; blk * 16 < nrows; ++blk) {
-          const uint32_t nw_lo = a.words[wo_lo + (uint32_t)blk + 1u];
-          const uint32_t nw_hi = a.words[wo_hi + (uint32_t)blk + 1u];
-          const int jend = min(nrows, blk * 16 + 16);
-          int j = blk * 16 + 1;
-          for (; j + 1 <= jend; j += 2) {
-            do_row(j, bA);
-            if (!first) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
-            do_row(j + 1, bB);
-            if (!first) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
-          }
-          if (j <= jend) {  // odd row count: the last row of the strip
-            do_row(j, bA);
-          }
-          w_lo = nw_lo;
-          w_hi = nw_hi;
-        }
+        if (j <= nrows) do_row(j, bA, ra & 0xffffu);  // odd row count: the strip's last row
       };
       if (nstrips > 0 && nrows > 0) {
         strip(0, std::true_type{});
@@ -722,8 +784,8 @@ __global__ __launch_bounds__(kBlock) void sw_lds_kernel(KArgs a) {
       if (nstrips > 0 && nrows > 0) {
         // corner H(la-1, lb-1): the last strip's right-most column at row lb-1
         uint32_t cu = as_u32(floor2);
-        if (lb_lo >= 2) cu = (cu & 0xffff0000u) | (bnd[(uint32_t)(lb_lo - 1) * kWave + lane] & 0xffffu);
-        if (lb_hi >= 2) cu = (cu & 0x0000ffffu) | (bnd[(uint32_t)(lb_hi - 1) * kWave + lane] & 0xffff0000u);
+        if (lb_lo >= 2) cu = (cu & 0xffff0000u) | (ws.load_lane_row(lb_lo - 1) & 0xffffu);
+        if (lb_hi >= 2) cu = (cu & 0x0000ffffu) | (ws.load_lane_row(lb_hi - 1) & 0xffff0000u);
         corner = as_s16x2(cu);
       }
       finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
@@ -782,8 +844,9 @@ __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f
   }
 }
 
+// Register budgets as for sw_lds_kernel.
 template <int S, bool kG = false>
-__global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
+__global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_f16_kernel(KArgs a) {
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;
   constexpr int kWaves = kBlock / kWave;
@@ -794,6 +857,9 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
   const int wave = blockIdx.x * kWaves + wv;
   uint32_t *tab = tab_all + wv * kPairs * RS;
   uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+  const WaveRows ws(bnd, a.bound_stride, lane);
+  static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
+  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab;
 
   const _Float16 gap_h = (_Float16)((float)a.gap * kF16Scale);
   const f16x2 gapv = {(_Float16)-gap_h, (_Float16)-gap_h};
@@ -816,7 +882,6 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
     const int ob = __builtin_amdgcn_readfirstlane(it.w);
 
     const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
-    const int gmn = __builtin_amdgcn_readfirstlane(a.gmin[g]);
     const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
     const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
     const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
@@ -828,7 +893,7 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
     const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
     const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
     const int nrows = gm > 1 ? gm - 1 : 0;
-    const int nfast = gmn > 1 ? gmn - 1 : 0;
+    build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
 
     for (int qi = qb; qi < qe; ++qi) {
       const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
@@ -843,6 +908,16 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
       auto strip = [&](auto wc, int j0, auto firstc) {
         constexpr int W = decltype(wc)::value;
         constexpr bool first = decltype(firstc)::value;
+        // The first list word and left boundaries are loaded before the
+        // table build, whose own loads are waited for, so nothing is pending
+        // when the row loop starts: the loop's waits then count only the
+        // loads and stores of one iteration.
+        uint32_t ra = ws.load(a.list_row);
+        uint32_t bA = f16_u32(g0), bB = f16_u32(g0);
+        if (!first) {
+          bA = ws.load(1);
+          if (nrows >= 2) bB = ws.load(2);
+        }
         __builtin_amdgcn_wave_barrier();
         for (int e = lane; e < kPairs * W; e += kWave) {
           const int p = e / W, jj = e - p * W;
@@ -863,20 +938,12 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
 #pragma unroll
         for (int jj = 0; jj < W; ++jj) G[jj] = g0;
         f16x2 gdiag = g0;
-        uint32_t w_lo = a.words[wo_lo], w_hi = a.words[wo_hi];
-        auto do_row = [&](int j, uint32_t left_u, bool pad) {
-          const uint32_t sh = 2u * ((uint32_t)(j - 1) & 15u);
-          uint32_t c_lo = (w_lo >> sh) & 3u;
-          uint32_t c_hi = (w_hi >> sh) & 3u;
-          if (pad) {
-            if (j > rows_lo) c_lo = 4u;
-            if (j > rows_hi) c_hi = 4u;
-          }
-          const uint4 *prow = reinterpret_cast<const uint4 *>(tab + (c_lo * 5u + c_hi) * RS);
+        auto do_row = [&](int j, uint32_t left_u, uint32_t paddr) {
+          const lds_u4 *prow = (const lds_u4 *)(uintptr_t)paddr;
           f16x2 P[W];
 #pragma unroll
           for (int w = 0; w < W / 4; ++w) {
-            const uint4 v = prow[w];
+            const u32x4 v = prow[w];
             P[4 * w + 0] = as_f16x2(v.x);
             P[4 * w + 1] = as_f16x2(v.y);
             P[4 * w + 2] = as_f16x2(v.z);
@@ -884,34 +951,31 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
           }
           const f16x2 gleft = as_f16x2(left_u);
           dp_row_f16<W>(G, P, gleft, gdiag, gapv, best, best2);
-          bnd[(uint32_t)j * kWave + (uint32_t)lane] = f16_u32(G[W - 1]);
+          ws.store(j, f16_u32(G[W - 1]));
           gdiag = gleft;
         };
-        uint32_t bA = f16_u32(g0), bB = f16_u32(g0);
-        if (!first) {
-          bA = bnd[kWave + (uint32_t)lane];
-          if (nrows >= 2) bB = bnd[2u * kWave + (uint32_t)lane];
+        // Two rows per iteration; their profile-row addresses and left
+        // boundaries are loaded one iteration ahead, each into the register
+        // whose previous value has just been used: a load that had to be
+        // copied into a loop-carried register would have to complete first.
+        // The loads are unconditional, keeping the iteration one basic block:
+        // boundary rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
+        // wave's maxlen + 2 boundary rows, and list word (j + 1) / 2 <=
+        // nrows / 2 inside its maxlen / 2 + 1 list rows; what is loaded past
+        // nrows is never used.
+        int j = 1;
+        for (; j + 1 <= nrows; j += 2) {
+          const uint32_t p0 = ra & 0xffffu, p1 = ra >> 16;
+          ra = ws.load(a.list_row + ((j + 1) >> 1));
+          __builtin_amdgcn_sched_barrier(0);  // issue the load here, a full iteration ahead
+          // the second row's profile reads after the first row's first
+          // column adds, not in one burst with the first row's
+          do_row(j, bA, p0);
+          if (!first) bA = ws.load(j + 2);
+          do_row(j + 1, bB, p1);
+          if (!first) bB = ws.load(j + 3);
         }
-        // The boundary loads are unconditional so that both rows of an
-        // iteration share one basic block and the scheduler interleaves their
-        // dependency chains.  Rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay
-        // inside the wave's (maxlen + 2) * 64 workspace; a row past nrows is
-        // loaded but never used.
-        for (int blk = 0; blk * 16 < nrows; ++blk) {
-          const uint32_t nw_lo = a.words[wo_lo + (uint32_t)blk + 1u];
-          const uint32_t nw_hi = a.words[wo_hi + (uint32_t)blk + 1u];
-          const int jend = min(nrows, blk * 16 + 16);
-          int j = blk * 16 + 1;
-          for (; j + 1 <= jend; j += 2) {
-            do_row(j, bA, j > nfast);
-            if (!first) bA = bnd[(uint32_t)(j + 2) * kWave + (uint32_t)lane];
-            do_row(j + 1, bB, j + 1 > nfast);
-            if (!first) bB = bnd[(uint32_t)(j + 3) * kWave + (uint32_t)lane];
-          }
-          if (j <= jend) do_row(j, bA, true);
-          w_lo = nw_lo;
-          w_hi = nw_hi;
-        }
+        if (j <= nrows) do_row(j, bA, ra & 0xffffu);
       };
       if (nstrips > 0 && nrows > 0) {
         // strips are right-aligned: the first holds the rem = 1..S leftmost
@@ -936,11 +1000,11 @@ __global__ __launch_bounds__(kBlock) void sw_lds_f16_kernel(KArgs a) {
       int cor_lo = 0, cor_hi = 0;
       if (nstrips > 0 && nrows > 0) {
         if (lb_lo >= 2) {
-          const f16x2 gv = as_f16x2(bnd[(uint32_t)(lb_lo - 1) * kWave + lane]);
+          const f16x2 gv = as_f16x2(ws.load_lane_row(lb_lo - 1));
           cor_lo = (int)__builtin_rintf((float)gv.x * sc) + a.gap;
         }
         if (lb_hi >= 2) {
-          const f16x2 gv = as_f16x2(bnd[(uint32_t)(lb_hi - 1) * kWave + lane]);
+          const f16x2 gv = as_f16x2(ws.load_lane_row(lb_hi - 1));
           cor_hi = (int)__builtin_rintf((float)gv.y * sc) + a.gap;
         }
       }
@@ -1099,6 +1163,14 @@ void *kernel_ptr_g(int k, int S, bool acgt) {
               : reinterpret_cast<void *>(&sw_tiles_kernel<32, false, kG>);
 }
 
+// Workspace rows per wave for kernel variant k and sequences of at most len
+// residues: len + 2 strip-boundary rows, and for the kernels that decode
+// their target rows once per item, len / 2 + 1 rows of the row-address list
+// after them (first list row: len + 2).
+int wave_rows(int len, int k) {
+  return len + 2 + (k == KERNEL_LDS_F16 || k == KERNEL_LDS ? len / 2 + 1 : 0);
+}
+
 // Kernel of variant k at strip S; group_items: the allocation-free
 // one-vs-all instantiation.
 void *kernel_ptr(int k, int S, bool acgt, bool group_items = false) {
@@ -1106,16 +1178,15 @@ void *kernel_ptr(int k, int S, bool acgt, bool group_items = false) {
 }
 
 // Resident waves per CU for a launch of kernel variant k at strip S: the
-// tuned or requested value, else 12 for the LDS kernels at S = 32 (measured
-// on MI355X with 20,000 sequences of 100-600 residues: 3 of their 4 possible
-// waves per SIMD run best), else the occupancy limit.
+// tuned or requested value, else the occupancy limit.
 int launch_waves_per_cu(const mnhn_sw_ctx *c, int k, int S, const void *kp) {
+  (void)k;
+  (void)S;
   int wpc = c->waves_per_cu;
   if (wpc <= 0) {
     const char *ev = getenv("MNHN_SW_WAVES_PER_CU");
     wpc = ev ? atoi(ev) : 0;
   }
-  if (wpc <= 0 && (k == KERNEL_LDS || k == KERNEL_LDS_F16) && S == 32) wpc = 12;
   if (wpc <= 0) {
     int blocks_per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kp, kBlock, 0) != hipSuccess ||
@@ -1129,13 +1200,14 @@ int launch_waves_per_cu(const mnhn_sw_ctx *c, int k, int S, const void *kp) {
 }  // namespace
 
 // Device scratch of the allocation-free entry point (mnhn_sw_one_vs_all_noalloc):
-// its work-queue counters (first 256 bytes) and the strip-boundary workspaces
-// of its launches.  Each GPU that the library runs on holds its own copy, from
-// the code object's load until the process ends.  In host code `scratch` only
-// names the symbol for hipGetSymbolAddress: kernels and copies receive the
-// device address that call returns (scratch_address), never the name.
-static constexpr size_t kScratchBytes = 64ull << 20;
+// its work-queue counters (first 256 bytes) and 96 MiB of strip-boundary
+// workspaces for its launches (four waves' worth at 65,534 residues, 384 B per
+// residue and wave).  Each GPU that the library runs on holds its own copy,
+// from the code object's load until the process ends.  In host code `scratch`
+// only names the symbol for hipGetSymbolAddress: kernels and copies receive
+// the device address that call returns (scratch_address), never the name.
 static constexpr size_t kScratchHead = 256;
+static constexpr size_t kScratchBytes = (96ull << 20) + kScratchHead;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
@@ -1311,10 +1383,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   // strip-boundary workspace, [row][lane] per wave.  Rows are query residues
   // in the register-profile kernel and target residues in the LDS kernels, so
   // it is sized for the longest sequence of the context.
-  const int stride = (std::max(w.max_q_len, c->maxlen) + 2) * kWave;
-  // resident waves are capped so that their workspaces stay within
-  // kWorkspaceBytes (long sequences: 16,384 residues take 4 MB per wave)
-  const size_t ws_waves = std::max<size_t>(kWorkspaceBytes / ((size_t)stride * sizeof(uint32_t)), kBlock / kWave);
+  const int ws_len = std::max(w.max_q_len, c->maxlen);
   int waves_used = 0;
   bool any_i32 = false;
   (void)hipEventRecord(c->ev[0], c->stream);
@@ -1324,6 +1393,11 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     const std::vector<int4> &items = pass.items;
     const size_t n_items = w.dev ? (size_t)w.n_items : items.size();
     void *kp = kernel_ptr(PK, PS, c->acgt);
+    const int stride = wave_rows(ws_len, PK) * kWave;
+    // resident waves are capped so that their workspaces stay within
+    // kWorkspaceBytes (long sequences: 16,384 residues take 6 MB per wave)
+    const size_t ws_waves =
+        std::max<size_t>(kWorkspaceBytes / ((size_t)stride * sizeof(uint32_t)), kBlock / kWave);
     const int wpc = launch_waves_per_cu(c, PK, PS, kp);
     int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
     const int max_useful = (int)((n_items + 3) / 4);
@@ -1355,7 +1429,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     a.gmin = gn_ptr;
     a.items = w.dev ? w.d_items : d_items; a.qids = q_ptr; a.counter = c->d_counter;
     a.bound = c->d_bound; a.dist_out = d_dist; a.keys = d_keys; a.nkeys = c->d_nkeys;
-    a.key_cap = key_cap; a.n_items = (int)n_items; a.bound_stride = stride;
+    a.key_cap = key_cap; a.n_items = (int)n_items; a.bound_stride = stride; a.list_row = ws_len + 2;
     a.mode = mode; a.triangle = triangle; a.eps_max = eps_max;
     a.match = c->match; a.mismatch = c->mismatch; a.gap = c->gap;
     if (e == hipSuccess) {
@@ -1761,11 +1835,12 @@ int mnhn_sw_one_vs_all_noalloc(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
   if (!c || !dist_out || query < 0 || query >= c->n) return MNHN_ERR_ARG;
   if (!c->scratch) return MNHN_ERR_DEVICE;
   // workspace rows: target residues (LDS kernels) or query residues (register
-  // and 32-bit kernels), at most the longest sequence; one launch needs at
-  // least one workgroup's four workspaces in the scratch
-  const int stride = (c->maxlen + 2) * kWave;
-  const size_t ws_waves = (kScratchBytes - kScratchHead) / ((size_t)stride * sizeof(uint32_t));
-  if (ws_waves < (size_t)(kBlock / kWave)) return MNHN_ERR_RANGE;
+  // and 32-bit kernels), at most the longest sequence, plus the fp16 kernel's
+  // row-address list; one launch needs at least one workgroup's four
+  // workspaces in the scratch, checked for the largest of them
+  const size_t ws_bytes = kScratchBytes - kScratchHead;
+  const size_t wave_bytes_max = (size_t)wave_rows(c->maxlen, KERNEL_LDS_F16) * kWave * sizeof(uint32_t);
+  if (ws_bytes / wave_bytes_max < (size_t)(kBlock / kWave)) return MNHN_ERR_RANGE;
   const int la = c->len[query];
   const int K = pick_kernel(c);
   int S = pick_strip(c);
@@ -1799,6 +1874,8 @@ int mnhn_sw_one_vs_all_noalloc(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
     const int n_items = passes[k].g1 - passes[k].g0;
     if (n_items <= 0) continue;
     void *kp = kernel_ptr(passes[k].kernel, passes[k].strip, c->acgt, true);
+    const int stride = wave_rows(c->maxlen, passes[k].kernel) * kWave;
+    const size_t ws_waves = ws_bytes / ((size_t)stride * sizeof(uint32_t));
     const int wpc = launch_waves_per_cu(c, passes[k].kernel, passes[k].strip, kp);
     int blocks = (c->num_cu * wpc + (kBlock / kWave) - 1) / (kBlock / kWave);
     blocks = std::min(blocks, (n_items + 3) / 4);
@@ -1807,7 +1884,7 @@ int mnhn_sw_one_vs_all_noalloc(mnhn_sw_ctx *c, int query, int32_t *dist_out) {
     a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
     a.gtgt = c->d_gtgt; a.gmax = c->d_gmax; a.gmin = c->d_gmin;
     a.counter = counters + k;
-    a.bound = ws; a.bound_stride = stride;
+    a.bound = ws; a.bound_stride = stride; a.list_row = c->maxlen + 2;
     a.dist_out = dist_out;
     a.n_items = n_items;
     a.item_base = passes[k].g0; a.query = query;
@@ -2545,12 +2622,14 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
     return MNHN_OK;
   }
   std::vector<Cand> cands;
+  // resident-wave limits of the LDS kernels: 3 per SIMD at strip 64, 5 at
+  // 32, 6 at 16 (register budgets)
   if (c->acgt) {
-    for (int w : {8, 12, 16}) cands.push_back({KERNEL_LDS_F16, 32, w});
-    for (int w : {8, 16, 24}) cands.push_back({KERNEL_LDS_F16, 16, w});
+    for (int w : {12, 16, 20}) cands.push_back({KERNEL_LDS_F16, 32, w});
+    for (int w : {16, 24}) cands.push_back({KERNEL_LDS_F16, 16, w});
     for (int w : {8, 12}) cands.push_back({KERNEL_LDS_F16, 64, w});
-    for (int w : {8, 16, 24}) cands.push_back({KERNEL_LDS, 16, w});
-    for (int w : {8, 12, 16}) cands.push_back({KERNEL_LDS, 32, w});
+    for (int w : {16, 24}) cands.push_back({KERNEL_LDS, 16, w});
+    for (int w : {12, 16, 20}) cands.push_back({KERNEL_LDS, 32, w});
     for (int w : {8, 12}) cands.push_back({KERNEL_LDS, 64, w});
   }
   for (int w : {4, 8}) cands.push_back({KERNEL_REG, 32, w});

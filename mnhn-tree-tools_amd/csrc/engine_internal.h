@@ -107,6 +107,9 @@ struct Work {
   // match * min(la, lb) <= 2048 (otherwise the int16 LDS kernel runs it)
   bool dev_f16_ok = false;
   int dev_strip = 0;  // strip width for device work (0: the context's)
+  // queries per wave for device work: 2 = target groups shared by two rows
+  // (KERNEL_LDS_F16, epsilon mode; qids holds the two rows of item k at 2k, 2k + 1)
+  int dev_split = 1;
 };
 
 

@@ -759,61 +759,89 @@ __global__ void band_counts_kernel(const int32_t *__restrict__ sorted, const lon
   kband[2 * i + 1] = first_at_most(t1);
 }
 
+// Full target groups of band t of a row holding k targets in it: k / 128
+// rounded up, except that a remainder of at most 64 targets of the fp16 band
+// (t = 2) goes to a half group shared with another row when split is set.
+__device__ __host__ inline bool band_half(int k, int t, int split) {
+  const int r = k % kGroupSize;
+  return split && t == 2 && r > 0 && r <= kGroupSize / 2;
+}
+__device__ __host__ inline int band_full_groups(int k, int t, int split) {
+  return band_half(k, t, split) ? k / kGroupSize : (k + kGroupSize - 1) / kGroupSize;
+}
+
 // One workgroup per row i: its ascending q (= descending target length) in
 // target groups of 128 per length band (band 0: k < kband[2i], band 1: up to
 // kband[2i + 1], band 2: the rest), so each group holds targets of similar
 // length and one lane format; the group's first target is its longest, its
-// last the shortest.  tlen[3i + band] = sum of the band's target lengths.
+// last the shortest.  A band-2 remainder of at most 64 targets (split set)
+// goes to the half group hgs[i] >> 1, segment hgs[i] & 1: segment s holds
+// its u-th target in slot (u / 32) * 64 + s * 32 + u % 32, the slots its
+// half-wave runs; the group's two rows set its length bounds by atomics
+// (gmax / gmin of half groups start at 0 / INT_MAX).  tlen[4i + band] =
+// sum of the band's target lengths in full groups, tlen[4i + 3] in the half
+// group.
 __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restrict__ sorted,
                                                          const long long *__restrict__ rs,
                                                          const int32_t *__restrict__ gbase,
                                                          const int32_t *__restrict__ kband,
+                                                         const int32_t *__restrict__ hgs, int split,
                                                          const int32_t *__restrict__ order,
                                                          const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
                                                          int32_t *__restrict__ gmax, int32_t *__restrict__ gmin,
                                                          unsigned long long *__restrict__ tlen) {
-  __shared__ unsigned long long red[3][256];
+  __shared__ unsigned long long red[4][256];
   const int i = blockIdx.x;
   const long long b = rs[i];
   const int cnt = (int)(rs[i + 1] - b);
   const int e0 = kband[2 * i], e1 = kband[2 * i + 1];
-  const int g0 = (e0 + kGroupSize - 1) / kGroupSize, g1 = (e1 - e0 + kGroupSize - 1) / kGroupSize;
-  unsigned long long sum[3] = {0, 0, 0};
+  const int g0 = band_full_groups(e0, 0, split), g1 = band_full_groups(e1 - e0, 1, split);
+  const int full2 = band_full_groups(cnt - e1, 2, split) * kGroupSize;  // band-2 targets in full groups
+  unsigned long long sum[4] = {0, 0, 0, 0};
   for (int k = threadIdx.x; k < cnt; k += 256) {
     const int band = k < e0 ? 0 : k < e1 ? 1 : 2;
     const int from = band == 0 ? 0 : band == 1 ? e0 : e1;
     const int to = band == 0 ? e0 : band == 1 ? e1 : cnt;
-    const int gfirst = gbase[i] + (band == 0 ? 0 : band == 1 ? g0 : g0 + g1);
     const int r = k - from;
-    const int g = gfirst + r / kGroupSize, slot = r % kGroupSize;
     const int j = order[sorted[b + k]];
     const int lj = len[j];
+    if (band == 2 && r >= full2) {  // the half group
+      const int u = r - full2, h = hgs[i];
+      const int g = h >> 1, slot = (u / 32) * (kGroupSize / 2) + (h & 1) * 32 + u % 32;
+      gtgt[(size_t)g * kGroupSize + slot] = j;
+      if (u == 0) atomicMax(&gmax[g], lj);
+      if (k == to - 1) atomicMin(&gmin[g], lj);
+      sum[3] += (unsigned long long)lj;
+      continue;
+    }
+    const int gfirst = gbase[i] + (band == 0 ? 0 : band == 1 ? g0 : g0 + g1);
+    const int g = gfirst + r / kGroupSize, slot = r % kGroupSize;
     gtgt[(size_t)g * kGroupSize + slot] = j;
     if (slot == 0) gmax[g] = lj;
-    if (slot == kGroupSize - 1 || k == to - 1) gmin[g] = lj;
+    if (slot == kGroupSize - 1 || k == to - 1 || (band == 2 && r == full2 - 1)) gmin[g] = lj;
     sum[band] += (unsigned long long)lj;
   }
-  for (int t = 0; t < 3; ++t) red[t][threadIdx.x] = sum[t];
+  for (int t = 0; t < 4; ++t) red[t][threadIdx.x] = sum[t];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
-      for (int t = 0; t < 3; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
+      for (int t = 0; t < 4; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x < 3) tlen[3 * i + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 4) tlen[4 * i + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// items of row i, one per target group: band t's groups at ibase[3i + t] ..
+// items of row i, one per full target group: band t's groups at ibase[3i + t] ..
 __global__ void cand_items_kernel(const int32_t *__restrict__ gbase, const int32_t *__restrict__ kband,
                                   const long long *__restrict__ rs, const int32_t *__restrict__ ibase, int n,
-                                  int4 *__restrict__ items) {
+                                  int split, int4 *__restrict__ items) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int cnt = (int)(rs[i + 1] - rs[i]);
   const int e[3] = {kband[2 * i], kband[2 * i + 1] - kband[2 * i], cnt - kband[2 * i + 1]};
   int g = gbase[i];
   for (int t = 0; t < 3; ++t) {
-    const int ng = (e[t] + kGroupSize - 1) / kGroupSize;
+    const int ng = band_full_groups(e[t], t, split);
     for (int q = 0; q < ng; ++q, ++g) items[ibase[3 * i + t] + q] = make_int4(g, i, i + 1, 0);
   }
 }
@@ -1327,10 +1355,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr,
-          *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr;
+          *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr, *hgs = nullptr, *qids2 = nullptr;
   unsigned long long *tlen = nullptr;
   long long *d_rs = nullptr;
-  int4 *items = nullptr;
+  int4 *items = nullptr, *sitems = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
     if (Xs) (void)hipFree(Xs);
@@ -1339,6 +1367,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
     dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs); dev_free(kband);
+    dev_free(hgs); dev_free(qids2); dev_free(sitems);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(ek0);
@@ -1517,20 +1546,50 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipMemcpyAsync(kb.data(), kband, 2 * (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    auto band_groups = [&](int i, int t) {
+    // Remainders of at most 64 fp16-band targets go to half groups shared by
+    // two rows (two queries per wave), paired longest first;
+    // MNHN_VERIFY_SPLIT=0 keeps one row per group.
+    const char *split_env = getenv("MNHN_VERIFY_SPLIT");
+    const int split = split_env && atoi(split_env) == 0 ? 0 : 1;
+    auto band_k = [&](int i, int t) {
       const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
-      const int k = t == 0 ? e0 : t == 1 ? e1 - e0 : cnt[i] - e1;
-      return (k + kGroupSize - 1) / kGroupSize;
+      return t == 0 ? e0 : t == 1 ? e1 - e0 : cnt[i] - e1;
     };
+    auto band_groups = [&](int i, int t) { return band_full_groups(band_k(i, t), t, split); };
     for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + band_groups(i, 0) + band_groups(i, 1) + band_groups(i, 2);
-    const int G = gb[n];
+    const int Gf = gb[n];
+    std::vector<int32_t> hrows;
+    for (int i = 0; i < n; ++i)
+      if (band_half(band_k(i, 2), 2, split)) hrows.push_back(i);
+    std::stable_sort(hrows.begin(), hrows.end(), [&](int x, int y) { return c->len[x] > c->len[y]; });
+    const int np = (int)((hrows.size() + 1) / 2);
+    const int G = Gf + np;
     // item order by class: fp16 groups of rows short enough for a 32-column
-    // strip, the other fp16 groups, the int16 groups, the 32-bit groups
+    // strip, the other fp16 groups, the int16 groups, the 32-bit groups; then
+    // the half-group pairs, those with a row longer than 128 (strip 64 when
+    // the context's strip is 64) before the others (strip 32)
     const int S = context_strip(c);
     auto band_class = [&](int i, int t) {
       if (t == 0) return 3;
       if (t == 1) return 2;
       return (S == 64 && c->len[i] <= 128) ? 0 : 1;
+    };
+    std::vector<int32_t> hg(n, -1), q2(2 * (size_t)np);
+    std::vector<int4> sit(np);
+    int split_begin[3] = {0, np, np};  // [5: strip 64, 4: strip 32)
+    for (int pi = 0; pi < np; ++pi) {
+      const int ra = hrows[2 * (size_t)pi];
+      const int rb = 2 * (size_t)pi + 1 < hrows.size() ? hrows[2 * (size_t)pi + 1] : ra;
+      hg[ra] = (Gf + pi) << 1;
+      if (rb != ra) hg[rb] = ((Gf + pi) << 1) | 1;
+      q2[2 * (size_t)pi] = ra;
+      q2[2 * (size_t)pi + 1] = rb;  // a lone row: the second half-wave has no targets
+      sit[pi] = make_int4(Gf + pi, 2 * pi, 2 * pi + 2, 0);
+      if (!(S == 64 && c->len[ra] > 128) && split_begin[1] == np) split_begin[1] = pi;
+    }
+    auto split_class = [&](int i) {
+      const int pi = (hg[i] >> 1) - Gf;
+      return (S == 64 && c->len[q2[2 * (size_t)pi]] > 128) ? 5 : 4;
     };
     std::vector<int32_t> ib(3 * (size_t)n, 0);
     int part_begin[5] = {0, 0, 0, 0, 0};
@@ -1548,34 +1607,49 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       part_begin[4] = o;
     }
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
-        (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)G)) ||
-        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 3 * (size_t)n)))
+        (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
+        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 4 * (size_t)n)) ||
+        (rc = dev_alloc(&hgs, (size_t)n)) || (rc = dev_alloc(&qids2, std::max<size_t>(2 * (size_t)np, 1))) ||
+        (rc = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))))
       return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(qids2, q2.data(), 2 * (size_t)np * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(sitems, sit.data(), (size_t)np * sizeof(int4), hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
-    hipLaunchKernelGGL(row_groups_kernel, dim3((unsigned)n), dim3(256), 0, st, sorted, d_rs, gbase, kband, d_order,
-                       c->d_len, gtgt, gmax, gmin, tlen);
+    if (np > 0) {
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, gmax + Gf, (long long)np, 0);
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, gmin + Gf, (long long)np,
+                         INT32_MAX);
+    }
+    hipLaunchKernelGGL(row_groups_kernel, dim3((unsigned)n), dim3(256), 0, st, sorted, d_rs, gbase, kband, hgs,
+                       split, d_order, c->d_len, gtgt, gmax, gmin, tlen);
     hipLaunchKernelGGL(cand_items_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, gbase, kband, d_rs, ibase, n,
-                       items);
+                       split, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
-    std::vector<unsigned long long> h_tlen(3 * (size_t)n);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 3 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
+    std::vector<unsigned long long> h_tlen(4 * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 4 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    double part_cells[4] = {0, 0, 0, 0};
-    for (int i = 0; i < n; ++i)
+    double part_cells[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
       for (int t = 0; t < 3; ++t) {
-        const double ci = (double)c->len[i] * (double)h_tlen[3 * (size_t)i + t];
+        const double ci = (double)c->len[i] * (double)h_tlen[4 * (size_t)i + t];
         stt.cells += ci;
         part_cells[band_class(i, t)] += ci;
       }
+      if (hg[i] >= 0) {
+        const double ci = (double)c->len[i] * (double)h_tlen[4 * (size_t)i + 3];
+        stt.cells += ci;
+        part_cells[split_class(i)] += ci;
+      }
+    }
     dev_free(sorted);
     Work w;
     w.dev = true;
-    w.d_qids = qids;
     w.d_gtgt = gtgt;
     w.d_gmax = gmax;
     w.d_gmin = gmin;
@@ -1590,12 +1664,26 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
       for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
-      for (int part = 0; part < 4 && rc == MNHN_OK; ++part) {
-        w.d_items = items + part_begin[part];
-        w.n_items = part_begin[part + 1] - part_begin[part];
-        w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
-        w.dev_f16_ok = part < 2;
-        w.dev_strip = part == 0 ? 32 : 0;
+      for (int part = 0; part < 6 && rc == MNHN_OK; ++part) {
+        if (part < 4) {
+          w.d_items = items + part_begin[part];
+          w.n_items = part_begin[part + 1] - part_begin[part];
+          w.d_qids = qids;
+          w.dev_split = 1;
+          w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
+          w.dev_f16_ok = part < 2;
+          w.dev_strip = part == 0 ? 32 : 0;
+        } else {  // half-group pairs: part 5 at strip 64, part 4 at strip 32
+          const int lo = part == 5 ? split_begin[0] : split_begin[1];
+          const int hi = part == 5 ? split_begin[1] : split_begin[2];
+          w.d_items = sitems + lo;
+          w.n_items = hi - lo;
+          w.d_qids = qids2;
+          w.dev_split = 2;
+          w.dev_kernel = KERNEL_LDS_F16;
+          w.dev_f16_ok = true;
+          w.dev_strip = part == 5 ? 64 : 32;
+        }
         if (w.n_items == 0) continue;
         w.cells = part_cells[part];
         rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);

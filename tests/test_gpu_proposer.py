@@ -126,6 +126,36 @@ def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, tuning):
     assert st["neighbours"] * 2 == len(col)
 
 
+@pytest.mark.parametrize("split", ["1", "0"])
+@pytest.mark.parametrize("shape", ["lone_half_row", "mixed_counts"])
+def test_prefilter_half_groups_equal_exact_lists(pkg, monkeypatch, split, shape):
+    """Verification with target groups shared by two rows (remainders of at
+    most 64 fp16-band targets, two queries per wave) and without them gives
+    the exact lists.  lone_half_row: 4 related sequences of distinct lengths
+    hold 3, 2, 1 and 0 candidates, so the third half row has no partner (its
+    group's second half-wave runs no targets); mixed_counts: rows with 0-200
+    candidates, i.e. full groups, half groups and both, with query lengths that
+    differ within a pair (the shorter query pre-padded)."""
+    monkeypatch.setenv("MNHN_VERIFY_SPLIT", split)
+    rng = np.random.default_rng(7)
+    seqs = []
+    if shape == "lone_half_row":
+        anc = synth.random_codes(rng, 200)
+        seqs = [synth.to_bytes(synth.mutate(rng, anc, 0.02, 0.0))[:200 - 7 * k] for k in range(4)]
+    else:
+        for L0, m in ((60, 40), (150, 200), (180, 70), (240, 110), (400, 35)):
+            anc = synth.random_codes(rng, L0)
+            seqs += [synth.to_bytes(synth.mutate(rng, anc, 0.03, 0.02)) for _ in range(m)]
+    eps = 300
+    _say(f"prefilter half groups split={split} shape={shape} n={len(seqs)}")
+    with pkg.Engine(seqs) as eng, pkg.Encoder(seed=3) as enc:
+        enc.margin = 1e4
+        rp, col, dist, st = enc.prefilter_csr(eng, eps)
+        erp, ecol, edist = eng.eps_csr(eps)
+    assert np.array_equal(rp, erp) and np.array_equal(col, ecol) and np.array_equal(dist, edist)
+    assert st["neighbours"] * 2 == len(col)
+
+
 def test_prefilter_trained_recall(pkg, oracle):
     seqs = synth.amplicon_families(seed=45, n_anc=20, per_anc=40, noise_frac=0.05)
     n = len(seqs)

@@ -251,8 +251,10 @@ def mix500k_workload(pkg, log, eps=200, pairs=2_000_000, epochs=3, train_draws=4
         labels.append((g, k))
     big = 1 << 40
     leaf = np.concatenate([g + k * big for g, k in labels])
-    child = np.concatenate([np.where(g < 2000, g // 5, g) + k * big for g, k in labels])
-    root = np.concatenate([np.where(g < 2000, g // 40, g) + k * big for g, k in labels])
+    n_leaf = synth.MIX_ROOTS * synth.MIX_PER_ROOT * synth.MIX_PER_CHILD
+    child = np.concatenate([np.where(g < n_leaf, g // synth.MIX_PER_CHILD, g) + k * big for g, k in labels])
+    root = np.concatenate([np.where(g < n_leaf, g // (synth.MIX_PER_ROOT * synth.MIX_PER_CHILD), g) + k * big
+                           for g, k in labels])
     enc, info = train_encoder(pkg, parts, [(leaf, 0.4), (child, 0.15), (root, 0.1)], root, pairs, eps, 5,
                               epochs, log, use_cosine=use_cosine, recall=recall, exact_below=exact_below)
     del parts

@@ -153,18 +153,25 @@ def pareto_lengths(rng: np.random.Generator, n: int, alpha: float = 1.5, xm: flo
     return np.minimum(np.rint(xm * (1.0 + rng.pareto(alpha, size=n))), hi).astype(np.int64)
 
 
-def heavy_tail_mix(seed: int = 5, n: int = 500_000, n_roots: int = 50, per_root: int = 8,
-            per_child: int = 5, tree_frac: float = 0.70, repeat_frac: float = 0.20,
+# tree shape of heavy_tail_mix: roots x children x grandchildren = 2,000 leaf clusters
+MIX_ROOTS, MIX_PER_ROOT, MIX_PER_CHILD = 250, 4, 2
+
+
+def heavy_tail_mix(seed: int = 5, n: int = 500_000, n_roots: int = MIX_ROOTS, per_root: int = MIX_PER_ROOT,
+            per_child: int = MIX_PER_CHILD, tree_frac: float = 0.70, repeat_frac: float = 0.20,
             n_monomer_families: int = 200, monomer: int = 171, with_groups: bool = False):
     """End-to-end learned-prefilter workload (heavy-tailed lengths 80-1500):
-      70% from a 4-level tree with 2,000 leaf clusters: 50 roots (Pareto
-         lengths, alpha 1.5, x_m 80, clipped to 1,500) -> 8 children at 8% ->
-         5 grandchildren at 4% (= 2,000 leaf-cluster ancestors) -> members at 2%;
+      70% from a 4-level tree with 2,000 leaf clusters: 250 roots (Pareto
+         lengths, alpha 1.5, x_m 80, clipped to 1,500; each root is its own
+         length draw) -> 4 children at 8% -> 2 grandchildren at 4% (= 2,000
+         leaf-cluster ancestors) -> members at 2%;
       20% tandem repeats: 2-8 copies of one of 200 random 171-bp monomers, every
          copy at 3% divergence;
       10% uniform random sequences with Pareto lengths.
     (mutation: sub:indel 4:1).  with_groups=True also returns one label per
-    sequence: its leaf cluster, its monomer family, or a unique label for noise."""
+    sequence: its leaf cluster, its monomer family, or a unique label for noise.
+    Leaf clusters are numbered root-major: leaf g descends from child
+    g // per_child and root g // (per_root * per_child)."""
     rng = _rng(seed)
     n_tree = int(round(tree_frac * n))
     n_rep = int(round(repeat_frac * n))

@@ -500,19 +500,23 @@ __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const 
   }
 }
 
-// 128 x 128 tiles over the length band: the host lists, per row tile ti, the
-// column tiles ti .. that the length bound can reach; tile_off[ti] is the
-// first block of row tile ti.  4 waves of 64 x 64 per tile.  Stage 1: Es Es^T
+// 128 x 128 tiles over the length band: the host lists the band tiles (ti, tj)
+// (row tile ti, column tiles ti .. that the length bound can reach) in
+// tile_list, in blocks of kTileBlock row tiles and, within a block, by column
+// tile: consecutive tiles share their column operand and cycle through the
+// block's row operands.  4 waves of 64 x 64 per tile.  Stage 1: Es Es^T
 // (K = 128) and the radius test.  Stage 2 (kCos, skipped for a tile with no
 // stage-1 survivor): Xs Xs^T over the 1024 raw 5-mer counts and the cosine
 // test dot >= cmin * |x_p| |x_q|, cmin = cmin of the longer sequence (< 0:
 // no test).  Hits are counted first and reserved with one atomic per wave,
 // then written (sorted positions p << 32 | q) at consecutive addresses.
+constexpr int kTileBlock = 8;  // row tiles per block of the band-tile order
+
 template <bool kCos, bool kI8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void pair_candidates_kernel(
     const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
     const void *__restrict__ Xv, const float *__restrict__ pnorm, const float *__restrict__ cmin,
-    const int32_t *__restrict__ lens, int n, const int32_t *__restrict__ tile_off, int tiles, int max_dl,
+    const int32_t *__restrict__ lens, int n, const int2 *__restrict__ tile_list, int max_dl,
     unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap,
     unsigned long long *__restrict__ stage2) {
   constexpr int BT = 128, LK = 64 + 8;
@@ -520,10 +524,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
   __shared__ __attribute__((aligned(16))) bf16 sB[2 * BT * LK];
   __shared__ int any_hit;
   // Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8),
-  // each XCD with its own L2.  Band tiles of one row tile are consecutive and
-  // share their row operand (and, with the next row tiles, most column
-  // operands), so each XCD takes a contiguous run of tiles: workgroup b
-  // computes tile (b % 8) * run + b / 8 (runs of per or per + 1 tiles).
+  // each XCD with its own L2.  Consecutive tiles of the list share operands
+  // (the workgroups resident on one XCD at a time span a few column tiles and
+  // at most kTileBlock row tiles, whose operands fit its L2), so each XCD
+  // takes a contiguous run of the list: workgroup b computes tile
+  // (b % 8) * run + b / 8 (runs of per or per + 1 tiles).
   int blk;
   {
     constexpr int kXcds = 8;
@@ -531,13 +536,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
     const int x = (int)blockIdx.x % kXcds, k = (int)blockIdx.x / kXcds;
     blk = x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
   }
-  int lo = 0, hi = tiles;  // tile_off[lo] <= blk < tile_off[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tile_off[mid] <= blk) lo = mid;
-    else hi = mid;
-  }
-  const int ti = lo, tj = ti + (blk - tile_off[ti]);
+  const int2 tt = tile_list[blk];
+  const int ti = __builtin_amdgcn_readfirstlane(tt.x), tj = __builtin_amdgcn_readfirstlane(tt.y);
   const int i0 = ti * BT, j0 = tj * BT;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -761,10 +761,11 @@ __global__ void band_counts_kernel(const int32_t *__restrict__ sorted, const lon
 
 // Full target groups of band t of a row holding k targets in it: k / 128
 // rounded up, except that a remainder of at most 64 targets of the fp16 band
-// (t = 2) goes to a half group shared with another row when split is set.
+// (t = 2) or the int16 band (t = 1) goes to a half group shared with another
+// row when split is set.
 __device__ __host__ inline bool band_half(int k, int t, int split) {
   const int r = k % kGroupSize;
-  return split && t == 2 && r > 0 && r <= kGroupSize / 2;
+  return split && (t == 2 || t == 1) && r > 0 && r <= kGroupSize / 2;
 }
 __device__ __host__ inline int band_full_groups(int k, int t, int split) {
   return band_half(k, t, split) ? k / kGroupSize : (k + kGroupSize - 1) / kGroupSize;
@@ -774,13 +775,13 @@ __device__ __host__ inline int band_full_groups(int k, int t, int split) {
 // target groups of 128 per length band (band 0: k < kband[2i], band 1: up to
 // kband[2i + 1], band 2: the rest), so each group holds targets of similar
 // length and one lane format; the group's first target is its longest, its
-// last the shortest.  A band-2 remainder of at most 64 targets (split set)
-// goes to the half group hgs[i] >> 1, segment hgs[i] & 1: segment s holds
-// its u-th target in slot (u / 32) * 64 + s * 32 + u % 32, the slots its
-// half-wave runs; the group's two rows set its length bounds by atomics
-// (gmax / gmin of half groups start at 0 / INT_MAX).  tlen[4i + band] =
-// sum of the band's target lengths in full groups, tlen[4i + 3] in the half
-// group.
+// last the shortest.  A band-2 (band-1) remainder of at most 64 targets
+// (split set) goes to the half group h >> 1, segment h & 1, with
+// h = hgs[2i] (hgs[2i + 1]): segment s holds its u-th target in slot
+// (u / 32) * 64 + s * 32 + u % 32, the slots its half-wave runs; the group's
+// two rows set its length bounds by atomics (gmax / gmin of half groups start
+// at 0 / INT_MAX).  tlen[5i + band] = sum of the band's target lengths in
+// full groups, tlen[5i + 3] (tlen[5i + 4]) in the band-2 (band-1) half group.
 __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restrict__ sorted,
                                                          const long long *__restrict__ rs,
                                                          const int32_t *__restrict__ gbase,
@@ -790,14 +791,15 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
                                                          const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
                                                          int32_t *__restrict__ gmax, int32_t *__restrict__ gmin,
                                                          unsigned long long *__restrict__ tlen) {
-  __shared__ unsigned long long red[4][256];
+  __shared__ unsigned long long red[5][256];
   const int i = blockIdx.x;
   const long long b = rs[i];
   const int cnt = (int)(rs[i + 1] - b);
   const int e0 = kband[2 * i], e1 = kband[2 * i + 1];
   const int g0 = band_full_groups(e0, 0, split), g1 = band_full_groups(e1 - e0, 1, split);
-  const int full2 = band_full_groups(cnt - e1, 2, split) * kGroupSize;  // band-2 targets in full groups
-  unsigned long long sum[4] = {0, 0, 0, 0};
+  // band-1 / band-2 targets in full groups
+  const int full1 = g1 * kGroupSize, full2 = band_full_groups(cnt - e1, 2, split) * kGroupSize;
+  unsigned long long sum[5] = {0, 0, 0, 0, 0};
   for (int k = threadIdx.x; k < cnt; k += 256) {
     const int band = k < e0 ? 0 : k < e1 ? 1 : 2;
     const int from = band == 0 ? 0 : band == 1 ? e0 : e1;
@@ -805,30 +807,31 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
     const int r = k - from;
     const int j = order[sorted[b + k]];
     const int lj = len[j];
-    if (band == 2 && r >= full2) {  // the half group
-      const int u = r - full2, h = hgs[i];
+    const int full = band == 1 ? full1 : full2;
+    if (band != 0 && r >= full) {  // the half group
+      const int u = r - full, h = hgs[2 * i + (band == 1 ? 1 : 0)];
       const int g = h >> 1, slot = (u / 32) * (kGroupSize / 2) + (h & 1) * 32 + u % 32;
       gtgt[(size_t)g * kGroupSize + slot] = j;
       if (u == 0) atomicMax(&gmax[g], lj);
       if (k == to - 1) atomicMin(&gmin[g], lj);
-      sum[3] += (unsigned long long)lj;
+      sum[band == 1 ? 4 : 3] += (unsigned long long)lj;
       continue;
     }
     const int gfirst = gbase[i] + (band == 0 ? 0 : band == 1 ? g0 : g0 + g1);
     const int g = gfirst + r / kGroupSize, slot = r % kGroupSize;
     gtgt[(size_t)g * kGroupSize + slot] = j;
     if (slot == 0) gmax[g] = lj;
-    if (slot == kGroupSize - 1 || k == to - 1 || (band == 2 && r == full2 - 1)) gmin[g] = lj;
+    if (slot == kGroupSize - 1 || k == to - 1 || (band != 0 && r == full - 1)) gmin[g] = lj;
     sum[band] += (unsigned long long)lj;
   }
-  for (int t = 0; t < 4; ++t) red[t][threadIdx.x] = sum[t];
+  for (int t = 0; t < 5; ++t) red[t][threadIdx.x] = sum[t];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
-      for (int t = 0; t < 4; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
+      for (int t = 0; t < 5; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x < 4) tlen[4 * i + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 5) tlen[5 * i + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // items of row i, one per full target group: band t's groups at ibase[3i + t] ..
@@ -1354,18 +1357,19 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   float *norm = nullptr, *srad = nullptr, *cmin = nullptr, *pnorm = nullptr;
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
-          *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr, *d_toff = nullptr,
+          *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr,
           *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr, *hgs = nullptr, *qids2 = nullptr;
   unsigned long long *tlen = nullptr;
   long long *d_rs = nullptr;
   int4 *items = nullptr, *sitems = nullptr;
+  int2 *d_tiles = nullptr;
   auto cleanup = [&]() {
     dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
     if (Xs) (void)hipFree(Xs);
     Xs = nullptr;
     dev_free(d_over); dev_free(cmin); dev_free(pnorm);
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
-    dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_toff); dev_free(items);
+    dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_tiles); dev_free(items);
     dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs); dev_free(kband);
     dev_free(hgs); dev_free(qids2); dev_free(sitems);
     (void)hipEventDestroy(e0);
@@ -1440,31 +1444,41 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   // ---- proposals over the length band: row tile ti reaches column tiles
   // ti .. tj_max(ti), the last whose longest length is >= shortest(ti) - max_dl
   const int tiles = (n + 127) / 128;
-  std::vector<int32_t> toff((size_t)tiles + 1, 0);
+  std::vector<int32_t> tjmax(tiles);
+  long long nblk_ll = 0;
   {
-    long long acc = 0;
     int tj = 0;
     for (int ti = 0; ti < tiles; ++ti) {
       const long long shortest = c->len[c->order[std::min((ti + 1) * 128, n) - 1]];
       tj = std::max(tj, ti);
       while (tj + 1 < tiles && (long long)c->len[c->order[(tj + 1) * 128]] >= shortest - max_dl) ++tj;
-      toff[ti] = (int32_t)acc;
-      acc += tj - ti + 1;
-      if (acc > 0x7fffffffLL) return fail(MNHN_ERR_RANGE);
+      tjmax[ti] = tj;  // nondecreasing in ti
+      nblk_ll += tj - ti + 1;
     }
-    toff[tiles] = (int32_t)acc;
   }
-  const unsigned nblk = (unsigned)toff[tiles];
+  if (nblk_ll > 0x7fffffffLL) return fail(MNHN_ERR_RANGE);
+  const unsigned nblk = (unsigned)nblk_ll;
   stt.tiles = nblk;
-  // d_cnt64[0]: candidates, d_cnt64[1]: band tiles that ran the cosine stage
-  if ((rc = dev_alloc(&d_toff, (size_t)tiles + 1)) || (rc = dev_alloc(&d_cnt64, 2))) return fail(rc);
-  if (hipMemcpyAsync(d_toff, toff.data(), ((size_t)tiles + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess)
-    return fail(MNHN_ERR_DEVICE);
+  // the band tiles in blocks of kTileBlock row tiles, by column tile within a block
+  {
+    std::vector<int2> tl((size_t)nblk);
+    size_t o = 0;
+    for (int b0 = 0; b0 < tiles; b0 += kTileBlock) {
+      const int b1 = std::min(tiles, b0 + kTileBlock);
+      for (int tj = b0; tj <= tjmax[b1 - 1]; ++tj)
+        for (int ti = b0; ti < b1 && ti <= tj; ++ti)
+          if (tj <= tjmax[ti]) tl[o++] = make_int2(ti, tj);
+    }
+    if (o != (size_t)nblk) return fail(MNHN_ERR_ARG);
+    // d_cnt64[0]: candidates, d_cnt64[1]: band tiles that ran the cosine stage
+    if ((rc = dev_alloc(&d_tiles, (size_t)nblk)) || (rc = dev_alloc(&d_cnt64, 2))) return fail(rc);
+    if (hipMemcpyAsync(d_tiles, tl.data(), (size_t)nblk * sizeof(int2), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return fail(MNHN_ERR_DEVICE);
+  }
   // candidate buffer: the band's pair count, at most MNHN_PREFILTER_CAND_GB
   // (default 8) GB; grown to the exact count and recomputed once on overflow
-  unsigned long long band_pairs = 0;
-  for (int ti = 0; ti < tiles; ++ti)
-    band_pairs += (unsigned long long)(toff[ti + 1] - toff[ti]) * 128ull * 128ull;
+  const unsigned long long band_pairs = (unsigned long long)nblk * 128ull * 128ull;
   const char *cg = getenv("MNHN_PREFILTER_CAND_GB");
   const double cand_gb = cg ? atof(cg) : 8.0;
   unsigned long long cap = std::max<unsigned long long>(
@@ -1477,13 +1491,13 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       (void)hipEventRecord(ek0, st);
       if (use_cos && xs_i8)
         hipLaunchKernelGGL((pair_candidates_kernel<true, true>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
-                           pnorm, cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+                           pnorm, cmin, lens, n, d_tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       else if (use_cos)
         hipLaunchKernelGGL((pair_candidates_kernel<true, false>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
-                           pnorm, cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+                           pnorm, cmin, lens, n, d_tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       else
         hipLaunchKernelGGL((pair_candidates_kernel<false, false>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
-                           pnorm, cmin, lens, n, d_toff, tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+                           pnorm, cmin, lens, n, d_tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
       e = hipGetLastError();
       (void)hipEventRecord(ek1, st);
     }
@@ -1546,9 +1560,9 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipMemcpyAsync(kb.data(), kband, 2 * (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    // Remainders of at most 64 fp16-band targets go to half groups shared by
-    // two rows (two queries per wave), paired longest first;
-    // MNHN_VERIFY_SPLIT=0 keeps one row per group.
+    // Remainders of at most 64 fp16-band or int16-band targets go to half
+    // groups shared by two rows of the same band (two queries per wave),
+    // paired longest first; MNHN_VERIFY_SPLIT=0 keeps one row per group.
     const char *split_env = getenv("MNHN_VERIFY_SPLIT");
     const int split = split_env && atoi(split_env) == 0 ? 0 : 1;
     auto band_k = [&](int i, int t) {
@@ -1558,37 +1572,48 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     auto band_groups = [&](int i, int t) { return band_full_groups(band_k(i, t), t, split); };
     for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + band_groups(i, 0) + band_groups(i, 1) + band_groups(i, 2);
     const int Gf = gb[n];
-    std::vector<int32_t> hrows;
-    for (int i = 0; i < n; ++i)
-      if (band_half(band_k(i, 2), 2, split)) hrows.push_back(i);
-    std::stable_sort(hrows.begin(), hrows.end(), [&](int x, int y) { return c->len[x] > c->len[y]; });
-    const int np = (int)((hrows.size() + 1) / 2);
+    std::vector<int32_t> hrows2, hrows1;
+    for (int i = 0; i < n; ++i) {
+      if (band_half(band_k(i, 2), 2, split)) hrows2.push_back(i);
+      if (band_half(band_k(i, 1), 1, split)) hrows1.push_back(i);
+    }
+    auto longer = [&](int x, int y) { return c->len[x] > c->len[y]; };
+    std::stable_sort(hrows2.begin(), hrows2.end(), longer);
+    std::stable_sort(hrows1.begin(), hrows1.end(), longer);
+    const int np2 = (int)((hrows2.size() + 1) / 2), np1 = (int)((hrows1.size() + 1) / 2);
+    const int np = np2 + np1;
     const int G = Gf + np;
     // item order by class: fp16 groups of rows short enough for a 32-column
     // strip, the other fp16 groups, the int16 groups, the 32-bit groups; then
-    // the half-group pairs, those with a row longer than 128 (strip 64 when
-    // the context's strip is 64) before the others (strip 32)
+    // the half-group pairs: fp16 pairs with a row longer than 128 (strip 64
+    // when the context's strip is 64) before the other fp16 pairs (strip 32),
+    // then the int16 pairs (rows longer than 512, the context's strip)
     const int S = context_strip(c);
     auto band_class = [&](int i, int t) {
       if (t == 0) return 3;
       if (t == 1) return 2;
       return (S == 64 && c->len[i] <= 128) ? 0 : 1;
     };
-    std::vector<int32_t> hg(n, -1), q2(2 * (size_t)np);
+    // hg[2i] / hg[2i + 1]: row i's fp16 / int16 half group (group << 1 | segment)
+    std::vector<int32_t> hg(2 * (size_t)n, -1), q2(2 * (size_t)np);
     std::vector<int4> sit(np);
-    int split_begin[3] = {0, np, np};  // [5: strip 64, 4: strip 32)
+    int split_begin[4] = {0, np2, np2, np};  // [5: fp16 strip 64, 4: fp16 strip 32, 6: int16)
     for (int pi = 0; pi < np; ++pi) {
-      const int ra = hrows[2 * (size_t)pi];
-      const int rb = 2 * (size_t)pi + 1 < hrows.size() ? hrows[2 * (size_t)pi + 1] : ra;
-      hg[ra] = (Gf + pi) << 1;
-      if (rb != ra) hg[rb] = ((Gf + pi) << 1) | 1;
+      const bool i16 = pi >= np2;
+      const std::vector<int32_t> &hr = i16 ? hrows1 : hrows2;
+      const size_t k = 2 * (size_t)(i16 ? pi - np2 : pi);
+      const int ra = hr[k];
+      const int rb = k + 1 < hr.size() ? hr[k + 1] : ra;
+      hg[2 * (size_t)ra + i16] = (Gf + pi) << 1;
+      if (rb != ra) hg[2 * (size_t)rb + i16] = ((Gf + pi) << 1) | 1;
       q2[2 * (size_t)pi] = ra;
       q2[2 * (size_t)pi + 1] = rb;  // a lone row: the second half-wave has no targets
       sit[pi] = make_int4(Gf + pi, 2 * pi, 2 * pi + 2, 0);
-      if (!(S == 64 && c->len[ra] > 128) && split_begin[1] == np) split_begin[1] = pi;
+      if (!i16 && !(S == 64 && c->len[ra] > 128) && split_begin[1] == np2) split_begin[1] = pi;
     }
-    auto split_class = [&](int i) {
-      const int pi = (hg[i] >> 1) - Gf;
+    auto split_class = [&](int i, int t) {  // the part of row i's band-t half group
+      if (t == 1) return 6;
+      const int pi = (hg[2 * (size_t)i] >> 1) - Gf;
       return (S == 64 && c->len[q2[2 * (size_t)pi]] > 128) ? 5 : 4;
     };
     std::vector<int32_t> ib(3 * (size_t)n, 0);
@@ -1608,13 +1633,13 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     }
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
-        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 4 * (size_t)n)) ||
-        (rc = dev_alloc(&hgs, (size_t)n)) || (rc = dev_alloc(&qids2, std::max<size_t>(2 * (size_t)np, 1))) ||
+        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 5 * (size_t)n)) ||
+        (rc = dev_alloc(&hgs, 2 * (size_t)n)) || (rc = dev_alloc(&qids2, std::max<size_t>(2 * (size_t)np, 1))) ||
         (rc = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))))
       return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 2 * (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && np > 0) e = hipMemcpyAsync(qids2, q2.data(), 2 * (size_t)np * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && np > 0) e = hipMemcpyAsync(sitems, sit.data(), (size_t)np * sizeof(int4), hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
@@ -1630,22 +1655,23 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
                        split, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
-    std::vector<unsigned long long> h_tlen(4 * (size_t)n);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 4 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
+    std::vector<unsigned long long> h_tlen(5 * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 5 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    double part_cells[6] = {0, 0, 0, 0, 0, 0};
+    double part_cells[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
       for (int t = 0; t < 3; ++t) {
-        const double ci = (double)c->len[i] * (double)h_tlen[4 * (size_t)i + t];
+        const double ci = (double)c->len[i] * (double)h_tlen[5 * (size_t)i + t];
         stt.cells += ci;
         part_cells[band_class(i, t)] += ci;
       }
-      if (hg[i] >= 0) {
-        const double ci = (double)c->len[i] * (double)h_tlen[4 * (size_t)i + 3];
-        stt.cells += ci;
-        part_cells[split_class(i)] += ci;
-      }
+      for (int t = 1; t <= 2; ++t)
+        if (hg[2 * (size_t)i + (t == 1)] >= 0) {
+          const double ci = (double)c->len[i] * (double)h_tlen[5 * (size_t)i + (t == 1 ? 4 : 3)];
+          stt.cells += ci;
+          part_cells[split_class(i, t)] += ci;
+        }
     }
     dev_free(sorted);
     Work w;
@@ -1664,7 +1690,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
       for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
-      for (int part = 0; part < 6 && rc == MNHN_OK; ++part) {
+      for (int part = 0; part < 7 && rc == MNHN_OK; ++part) {
         if (part < 4) {
           w.d_items = items + part_begin[part];
           w.n_items = part_begin[part + 1] - part_begin[part];
@@ -1673,16 +1699,16 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
           w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
           w.dev_f16_ok = part < 2;
           w.dev_strip = part == 0 ? 32 : 0;
-        } else {  // half-group pairs: part 5 at strip 64, part 4 at strip 32
-          const int lo = part == 5 ? split_begin[0] : split_begin[1];
-          const int hi = part == 5 ? split_begin[1] : split_begin[2];
+        } else {  // half-group pairs: fp16 part 5 at strip 64, part 4 at strip 32; int16 part 6
+          const int lo = part == 5 ? split_begin[0] : part == 4 ? split_begin[1] : split_begin[2];
+          const int hi = part == 5 ? split_begin[1] : part == 4 ? split_begin[2] : split_begin[3];
           w.d_items = sitems + lo;
           w.n_items = hi - lo;
           w.d_qids = qids2;
           w.dev_split = 2;
-          w.dev_kernel = KERNEL_LDS_F16;
-          w.dev_f16_ok = true;
-          w.dev_strip = part == 5 ? 64 : 32;
+          w.dev_kernel = part == 6 ? KERNEL_LDS : KERNEL_LDS_F16;
+          w.dev_f16_ok = part != 6;
+          w.dev_strip = part == 5 ? 64 : part == 4 ? 32 : (S == 64 ? 64 : 32);
         }
         if (w.n_items == 0) continue;
         w.cells = part_cells[part];

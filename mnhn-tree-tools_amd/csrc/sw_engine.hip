@@ -648,20 +648,28 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
 // (96).  Measured on MI355X, the extra resident waves hide the wait states
 // between one row's dependent packed ops better than interleaving two rows
 // in one wave (which needs registers for both rows' profiles).
-template <int S, bool kG = false>
+//
+// kQ = 2 (epsilon mode, item lists): two queries per wave, for target groups
+// shared by two rows, with the slot layout and left pre-padding of the fp16
+// kernel's kQ = 2 form below (lanes 0-31: query qids[qi] against slots 0-31
+// and 64-95; lanes 32-63: qids[qi + 1] against slots 32-63 and 96-127).
+template <int S, bool kG = false, int kQ = 1>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_kernel(KArgs a) {
+  static_assert(kQ == 1 || (kQ == 2 && !kG), "two queries per wave: item lists only");
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
+  constexpr int kTab = kPairs * RS;  // words of one query's table
   constexpr int kWaves = kBlock / kWave;
-  __shared__ __attribute__((aligned(16))) uint32_t tab_all[kWaves * kPairs * RS];
+  __shared__ __attribute__((aligned(16))) uint32_t tab_all[kWaves * kQ * kTab];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wave = blockIdx.x * kWaves + wv;
-  uint32_t *tab = tab_all + wv * kPairs * RS;
+  uint32_t *tab = tab_all + wv * kQ * kTab;
   uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
   const WaveRows ws(bnd, a.bound_stride, lane);
   static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
-  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab;
+  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's query
+  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab + (uint32_t)seg * (kTab * 4u);
 
   const s16x2 floor2 = {kFloor, kFloor};
   const s16x2 gapv = {(short)-a.gap, (short)-a.gap};
@@ -694,11 +702,17 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
     const int nrows = gm > 1 ? gm - 1 : 0;     // DP rows = inner residues of the longest target
     build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
 
-    for (int qi = qb; qi < qe; ++qi) {
-      const int q = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
-      const int la = __builtin_amdgcn_readfirstlane(a.len[q]);
-      const uint32_t qwo = __builtin_amdgcn_readfirstlane(a.woff[q]);
-      const int ncols = la > 1 ? la - 1 : 0;
+    for (int qi = qb; qi < qe; qi += kQ) {
+      const int q0 = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
+      const int q1 = kQ == 2 ? __builtin_amdgcn_readfirstlane(a.qids[qi + 1]) : q0;
+      const int la0 = __builtin_amdgcn_readfirstlane(a.len[q0]);
+      const int la1 = kQ == 2 ? __builtin_amdgcn_readfirstlane(a.len[q1]) : la0;
+      const uint32_t qwo0 = __builtin_amdgcn_readfirstlane(a.woff[q0]);
+      const uint32_t qwo1 = kQ == 2 ? __builtin_amdgcn_readfirstlane(a.woff[q1]) : qwo0;
+      const int nc0 = la0 > 1 ? la0 - 1 : 0, nc1 = la1 > 1 ? la1 - 1 : 0;
+      const int ncols = max(nc0, nc1);
+      const int sh0 = ncols - nc0, sh1 = ncols - nc1;  // left pre-padding of each query
+      const int q = seg ? q1 : q0, la = seg ? la1 : la0;  // the lane's query
       const int nstrips = (ncols + S - 1) / S;
       s16x2 best = floor2, best2 = floor2;
       s16x2 corner = floor2;
@@ -719,17 +733,19 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
         }
         // ---- score table of this strip
         __builtin_amdgcn_wave_barrier();
-        for (int e = lane; e < kPairs * S; e += kWave) {
-          const int p = e / S, jj = e - p * S;
-          const int k = j0 + jj;
+        for (int e = lane; e < kQ * kPairs * S; e += kWave) {
+          const int sg = kQ == 2 && e >= kPairs * S ? 1 : 0;
+          const int e1 = e - sg * (kPairs * S);
+          const int p = e1 / S, jj = e1 - p * S;
+          const int k = j0 + jj - (sg ? sh1 : sh0);
           uint32_t v = 0x80008000u;
           if (k >= 1) {
             const uint32_t r = (uint32_t)(k - 1);
-            const uint32_t qc = (a.words[qwo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            const uint32_t qc = (a.words[(sg ? qwo1 : qwo0) + (r >> 4)] >> (2u * (r & 15u))) & 3u;
             const uint32_t clo = (uint32_t)(p / 5), chi = (uint32_t)(p - 5 * (p / 5));
             v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
           }
-          tab[p * RS + jj] = v;
+          tab[sg * kTab + p * RS + jj] = v;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1192,14 +1208,19 @@ int wave_rows(int len, int k) {
 }
 
 // Kernel of variant k at strip S; group_items: the allocation-free
-// one-vs-all instantiation; split = 2: the fp16 kernel with two queries per
-// wave (strips 32 and 64).
+// one-vs-all instantiation; split = 2: the fp16 or int16 LDS kernel with two
+// queries per wave (strips 32 and 64).
 void *kernel_ptr(int k, int S, bool acgt, bool group_items = false, int split = 1) {
-  if (split == 2)
-    return k == KERNEL_LDS_F16 && !group_items
-               ? (S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 2>)
-                          : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 2>))
-               : nullptr;
+  if (split == 2) {
+    if (group_items) return nullptr;
+    if (k == KERNEL_LDS_F16)
+      return S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 2>)
+                     : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 2>);
+    if (k == KERNEL_LDS)
+      return S == 64 ? reinterpret_cast<void *>(&sw_lds_kernel<64, false, 2>)
+                     : reinterpret_cast<void *>(&sw_lds_kernel<32, false, 2>);
+    return nullptr;
+  }
   return group_items ? kernel_ptr_g<true>(k, S, acgt) : kernel_ptr_g<false>(k, S, acgt);
 }
 
@@ -1319,7 +1340,8 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   if (w.dev) {
     const int DK = (K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K;
     const int DS = DK == KERNEL_I32 ? kI32Strip : (w.dev_strip ? w.dev_strip : S);
-    if (w.dev_split == 2 && (DK != KERNEL_LDS_F16 || mode != MODE_EPS || (DS != 32 && DS != 64))) {
+    if (w.dev_split == 2 && ((DK != KERNEL_LDS_F16 && DK != KERNEL_LDS) || mode != MODE_EPS ||
+                             (DS != 32 && DS != 64))) {
       cleanup();
       return MNHN_ERR_ARG;
     }

@@ -127,33 +127,45 @@ def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, tuning):
 
 
 @pytest.mark.parametrize("split", ["1", "0"])
-@pytest.mark.parametrize("shape", ["lone_half_row", "mixed_counts"])
+@pytest.mark.parametrize("shape", ["lone_half_row", "mixed_counts", "long_rows"])
 def test_prefilter_half_groups_equal_exact_lists(pkg, monkeypatch, split, shape):
     """Verification with target groups shared by two rows (remainders of at
-    most 64 fp16-band targets, two queries per wave) and without them gives
-    the exact lists.  lone_half_row: 4 related sequences of distinct lengths
-    hold 3, 2, 1 and 0 candidates, so the third half row has no partner (its
-    group's second half-wave runs no targets); mixed_counts: rows with 0-200
-    candidates, i.e. full groups, half groups and both, with query lengths that
-    differ within a pair (the shorter query pre-padded)."""
+    most 64 fp16-band or int16-band targets, two queries per wave) and without
+    them gives the exact lists.  lone_half_row: 4 related sequences of
+    distinct lengths hold 3, 2, 1 and 0 candidates, so the third half row has
+    no partner (its group's second half-wave runs no targets); mixed_counts:
+    rows with 0-200 candidates, i.e. full groups, half groups and both, with
+    query lengths that differ within a pair (the shorter query pre-padded);
+    long_rows: the same over rows longer than 512 residues, whose targets
+    beyond 512 residues run the int16 form (rows near 515 residues hold
+    targets of both forms), with a lone int16 half row among them."""
     monkeypatch.setenv("MNHN_VERIFY_SPLIT", split)
     rng = np.random.default_rng(7)
     seqs = []
     if shape == "lone_half_row":
         anc = synth.random_codes(rng, 200)
         seqs = [synth.to_bytes(synth.mutate(rng, anc, 0.02, 0.0))[:200 - 7 * k] for k in range(4)]
+    elif shape == "long_rows":
+        for L0, m, ps, pi in ((515, 30, 0.03, 0.02), (600, 45, 0.01, 0.005), (700, 100, 0.005, 0.002),
+                              (900, 12, 0.03, 0.02), (1300, 70, 0.004, 0.002)):
+            anc = synth.random_codes(rng, L0)
+            seqs += [synth.to_bytes(synth.mutate(rng, anc, ps, pi)) for _ in range(m)]
+        anc = synth.random_codes(rng, 1100)
+        seqs += [synth.to_bytes(synth.mutate(rng, anc, 0.02, 0.0))[:1100 - 7 * k] for k in range(4)]
     else:
         for L0, m in ((60, 40), (150, 200), (180, 70), (240, 110), (400, 35)):
             anc = synth.random_codes(rng, L0)
             seqs += [synth.to_bytes(synth.mutate(rng, anc, 0.03, 0.02)) for _ in range(m)]
-    eps = 300
-    _say(f"prefilter half groups split={split} shape={shape} n={len(seqs)}")
+    eps = 1000 if shape == "long_rows" else 300
+    _say(f"prefilter half groups split={split} shape={shape} n={len(seqs)} eps={eps}")
     with pkg.Engine(seqs) as eng, pkg.Encoder(seed=3) as enc:
         enc.margin = 1e4
         rp, col, dist, st = enc.prefilter_csr(eng, eps)
         erp, ecol, edist = eng.eps_csr(eps)
     assert np.array_equal(rp, erp) and np.array_equal(col, ecol) and np.array_equal(dist, edist)
     assert st["neighbours"] * 2 == len(col)
+    if shape == "long_rows":
+        assert st["cells"] > 0 and len(col) > 2 * len(seqs)
 
 
 def test_prefilter_trained_recall(pkg, oracle):

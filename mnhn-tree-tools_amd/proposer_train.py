@@ -57,20 +57,34 @@ def profile_cosine(P, i, j, chunk=1 << 18):
     return out
 
 
-def bucket_edges(lmax, buckets=8):
+# bucket edges beyond the quantile edges: neighbour pairs crowd at the short
+# lengths, so the quantile buckets alone leave every longer pair in one bucket
+# whose thresholds are set by its shortest pairs
+LONG_EDGES = (150, 200, 300, 450, 700, 1000)
+
+
+def bucket_edges(lmax, buckets=8, long_edges=LONG_EDGES):
+    """Upper edges of the length buckets (longer sequence of a pair): the
+    quantiles of lmax in `buckets` equal parts, then long_edges above the last
+    quantile and below the longest pair."""
     qs = np.unique(np.quantile(lmax, np.linspace(0, 1, buckets + 1)[1:-1]).astype(np.int64))
-    return np.concatenate([qs, [2**31 - 1]]).astype(np.int64)
+    top = qs[-1] if len(qs) else 0
+    extra = [e for e in long_edges if top < e < lmax.max()]
+    return np.concatenate([qs, extra, [2**31 - 1]]).astype(np.int64)
 
 
-def per_bucket(values, lmax, upper, q, high):
+def per_bucket(values, lmax, upper, q, high, min_pairs=2000):
     """Per length bucket: the q-quantile of values (high=True) or the
-    (1-q)-quantile (high=False); the extreme when the bucket is too small."""
+    (1-q)-quantile (high=False), the extreme when the quantile rests on fewer
+    than 10 pairs.  A bucket of fewer than min_pairs pairs takes the value of
+    the bucket below it (the looser one: neighbour pairs of longer sequences
+    keep more of their k-mers and sit closer in the embedding), the first
+    bucket the extreme over all pairs."""
     out, lo = [], -1
-    allq = float(np.quantile(values, q if high else 1 - q))
     for u in upper:
         v = values[(lmax > lo) & (lmax <= u)]
-        if len(v) == 0:
-            out.append(allq)
+        if len(v) < min_pairs:
+            out.append(out[-1] if out else float(values.max() if high else values.min()))
         elif len(v) * (1 - q) < 10:
             out.append(float(v.max() if high else v.min()))
         else:

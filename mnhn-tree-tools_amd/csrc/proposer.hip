@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -318,11 +319,9 @@ __global__ __launch_bounds__(256) void sorted_rows_kernel(const bf16 *__restrict
   }
 }
 
-// One wave per sorted row p: raw 5-mer counts of sequence order[p] as bf16
-// (integers up to 256 are exact) and their squared norm, for the profile
-// cosine of the pairwise stage.
 // Raw 5-mer counts of every sequence in length order (row p = sequence
-// order[p]) and their squared norms.  T = bf16 (integers exact up to 256), or
+// order[p]) and their norms, for the count cosine of the pairwise stage.
+// T = bf16 (integers exact up to 256), or
 // int8 for the int8 MFMA stage: counts above 127 set *over (the host then
 // builds the bf16 rows instead).
 template <typename T>
@@ -372,7 +371,7 @@ __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t 
       if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
       __syncthreads();
     }
-    if (threadIdx.x == 0) pnorm[p] = red[0];
+    if (threadIdx.x == 0) pnorm[p] = sqrtf(red[0]);
     __syncthreads();
   }
 }
@@ -544,8 +543,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
   if (tid == 0) any_hit = 0;
   // Stage 1 is folded into two 64-bit masks per lane (bit 16 i + 4 r + j):
   // the radius test, and which side is the longer sequence; its accumulators
-  // and row / column terms are dead before stage 2 starts, which keeps the
-  // kernel at 4 waves per SIMD.
+  // and row / column terms are dead before stage 2 starts.  In an
+  // off-diagonal tile every row precedes every column in length order, so the
+  // row is the longer sequence of each pair (equal lengths have equal radii
+  // and cosine bounds): its radius and bound decide, and the length test
+  // 0 <= l_i - l_j <= max_dl is one unsigned compare.  Rows and columns past n
+  // get lengths that make l_i - l_j negative.
+  const bool diag = ti == tj;
   unsigned long long m1 = 0ull, mlong = 0ull;
   {
     f32x4 acc[4][4];
@@ -554,8 +558,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
     tile_dot<D3>(Es, Es, n, i0, j0, sA, sB, acc);
-    // row / column terms are loaded at clamped indices (no branches);
-    // out-of-range rows and columns get lengths that no |dl| <= max_dl matches
+    // row / column terms are loaded at clamped indices (no branches)
     float nj[4], sj2[4];
     int lj[4];
 #pragma unroll
@@ -567,29 +570,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
       sj2[j] = sj * sj;
       lj[j] = gj < n ? lens[cj] : (1 << 29);
     }
-    const bool diag = ti == tj;
+    if (!diag) {
+      // |e_i - e_j|^2 <= s_i^2  <=>  n_j - 2 e_i.e_j <= s_i^2 - n_i
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-        const int ci = min(gi, n - 1);
-        const float ni = norm[ci];
-        const float si = srad[ci];
-        const float si2 = si * si;
-        const int li = gi < n ? lens[ci] : -(1 << 28);
+        for (int r = 0; r < 4; ++r) {
+          const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+          const int ci = min(gi, n - 1);
+          const float si = srad[ci];
+          const float ri = si * si - norm[ci];
+          const int li = gi < n ? lens[ci] : -(1 << 28);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
-          const int dl = li - lj[j];
-          const float rad2 = dl >= 0 ? si2 : sj2[j];
-          const float d2 = ni + nj[j] - 2.0f * acc[i][j][r];
-          const bool h = (!diag || gi < gj) && abs(dl) <= max_dl && d2 <= rad2;
-          const int bit = 16 * i + 4 * r + j;
-          m1 |= h ? 1ull << bit : 0ull;
-          mlong |= dl >= 0 ? 1ull << bit : 0ull;
+          for (int j = 0; j < 4; ++j) {
+            const bool h = (unsigned)(li - lj[j]) <= (unsigned)max_dl && nj[j] - 2.0f * acc[i][j][r] <= ri;
+            m1 |= h ? 1ull << (16 * i + 4 * r + j) : 0ull;
+          }
         }
-      }
+      mlong = ~0ull;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+          const int ci = min(gi, n - 1);
+          const float ni = norm[ci];
+          const float si = srad[ci];
+          const float si2 = si * si;
+          const int li = gi < n ? lens[ci] : -(1 << 28);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
+            const int dl = li - lj[j];
+            const float rad2 = dl >= 0 ? si2 : sj2[j];
+            const float d2 = ni + nj[j] - 2.0f * acc[i][j][r];
+            const bool h = gi < gj && abs(dl) <= max_dl && d2 <= rad2;
+            const int bit = 16 * i + 4 * r + j;
+            m1 |= h ? 1ull << bit : 0ull;
+            mlong |= dl >= 0 ? 1ull << bit : 0ull;
+          }
+        }
+    }
   }
   unsigned long long m2 = m1;  // hits after both stages
   if constexpr (kCos) {
@@ -597,6 +619,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
     __syncthreads();
     if (any_hit == 0) return;  // uniform over the block
     if (tid == 0) atomicAdd(stage2, 1ull);
+    // cosine test dot >= c |x_i| |x_j| with c the longer sequence's bound
+    // (c < 0: no test); pnorm holds the norms |x| of the count rows, and the
+    // counts are nonnegative, so dot >= 0
     f32x4 accx[4][4];
     if constexpr (kI8) {
       i32x4 acci[4][4];
@@ -620,13 +645,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
       const bf16 *Xs = static_cast<const bf16 *>(Xv);
       tile_dot<D0>(Xs, Xs, n, i0, j0, sA, sB, accx);
     }
-    float pj[4], cj[4];
+    float vj[4], uj[4];  // |x_j| and the column's bound times it (-FLT_MAX: no test)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gj = j0 + 64 * wn + 16 * j + (lane & 15);
       const int cc = min(gj, n - 1);
-      pj[j] = pnorm[cc];
-      cj[j] = gj < n ? cmin[cc] : -1.f;
+      const float c = gj < n ? cmin[cc] : -1.f;
+      vj[j] = pnorm[cc];
+      uj[j] = c >= 0.0f ? c * vj[j] : -FLT_MAX;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -634,15 +660,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
       for (int r = 0; r < 4; ++r) {
         const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
         const int cc = min(gi, n - 1);
-        const float pi = pnorm[cc];
-        const float ci = gi < n ? cmin[cc] : -1.f;
+        const float vi = pnorm[cc];
+        const float c = gi < n ? cmin[cc] : -1.f;
+        const float ui = c >= 0.0f ? c * vi : -FLT_MAX;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int bit = 16 * i + 4 * r + j;
-          const float c = ((mlong >> bit) & 1ull) ? ci : cj[j];
-          const float dot = accx[i][j][r];
-          const bool fail = c >= 0.0f && !(dot >= 0.0f && dot * dot >= c * c * pi * pj[j]);
-          m2 &= fail ? ~(1ull << bit) : ~0ull;
+          // off-diagonal: the row's bound (mlong is all ones)
+          const float thr = diag && !((mlong >> bit) & 1ull) ? uj[j] * vi : ui * vj[j];
+          m2 &= accx[i][j][r] < thr ? ~(1ull << bit) : ~0ull;
         }
       }
   }

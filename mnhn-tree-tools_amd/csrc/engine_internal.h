@@ -4,11 +4,37 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <vector>
 
 #include "mnhn_sw.h"
+
+// MNHN_TIMING=2: wall time of each step of a list build on stderr.  Every
+// mark waits for the stream, which perturbs the timing it reports, so timed
+// runs leave the level at 0 or 1.
+struct mnhn_phase_clock {
+  bool on = false;
+  hipStream_t st = nullptr;
+  const char *who = "";
+  std::chrono::steady_clock::time_point t;
+  mnhn_phase_clock(hipStream_t s, const char *w) : st(s), who(w) {
+    const char *v = getenv("MNHN_TIMING");
+    on = v && atoi(v) >= 2;
+    t = std::chrono::steady_clock::now();
+  }
+  void mark(const char *what) {
+    if (!on) return;
+    (void)hipStreamSynchronize(st);
+    const auto now = std::chrono::steady_clock::now();
+    fprintf(stderr, "[mnhn] %s %-30s %9.3f ms\n", who, what,
+            std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  }
+};
 
 // Pinned staging buffers for large device-to-host copies (allocated on first
 // use, 64 MB each; released by stage_free).

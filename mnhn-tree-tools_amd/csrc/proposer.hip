@@ -377,9 +377,8 @@ __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t 
 }
 
 // acc += A[i0 .. i0+128) . B[j0 .. j0+128)^T over k in [0, K) (row stride K),
-// staged through LDS 64 k at a time; each wave owns a 64 x 64 sub-tile.  The
-// next chunk's global loads are issued before the current chunk's MFMAs, so
-// their latency hides behind the math.
+// staged through LDS 64 k at a time; each wave owns a 64 x 64 sub-tile.
+// Global loads run two chunks ahead of the MFMAs (below).
 template <int K>
 __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int n,
                                          int i0, int j0, bf16 *sA, bf16 *sB, f32x4 (&acc)[4][4]) {
@@ -393,8 +392,14 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
   // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread.  Rows past
   // n load row n - 1 instead (no branch): their products are never used, as
   // the callers' tests reject every row or column at or beyond n.
-  bf16x8 va[4], vb[4];
-  auto load = [&](int kh) {
+  // two register sets: chunk c + 2 is loaded while chunk c is multiplied
+  // and chunk c + 1 (loaded one chunk earlier) goes into the other LDS
+  // stage, so a load has two chunks of math and barriers to arrive.  Chunk k
+  // travels in register set k & 1 and LDS stage k & 1.
+  constexpr int NC = K / KH;
+  static_assert(K % KH == 0 && NC % 2 == 0, "whole pairs of chunks");
+  bf16x8 va0[4], vb0[4], va1[4], vb1[4];
+  auto load = [&](bf16x8 (&va)[4], bf16x8 (&vb)[4], int kh) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
@@ -403,7 +408,7 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
       vb[t] = *reinterpret_cast<const bf16x8 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
     }
   };
-  auto store = [&](int stage) {
+  auto store = [&](const bf16x8 (&va)[4], const bf16x8 (&vb)[4], int stage) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
@@ -412,15 +417,8 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
       *reinterpret_cast<bf16x8 *>(sB + stage * STG + r * LK + k) = vb[t];
     }
   };
-  __syncthreads();  // the previous user of both stages is done
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int c = 0; c * KH < K; ++c) {
-    const int cur = c & 1;
-    const bool more = (c + 1) * KH < K;
-    if (more) load((c + 1) * KH);
-    const bf16 *a = sA + cur * STG, *b = sB + cur * STG;
+  auto mfma = [&](int stage) {
+    const bf16 *a = sA + stage * STG, *b = sB + stage * STG;
 #pragma unroll
     for (int ks = 0; ks < KH; ks += 32) {
       bf16x8 af[4], bfr[4];
@@ -434,16 +432,32 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
         for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
-    if (more) store(cur ^ 1);
+  };
+  __syncthreads();  // the previous user of both stages is done
+  load(va0, vb0, 0);
+  load(va1, vb1, KH);
+  store(va0, vb0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; c += 2) {
+    if (c + 2 < NC) load(va0, vb0, (c + 2) * KH);  // set 0 held chunk c, already in stage 0
+    __builtin_amdgcn_sched_barrier(0);  // issue the loads here, ahead of the math
+    mfma(0);
+    store(va1, vb1, 1);  // chunk c + 1
+    __syncthreads();
+    if (c + 3 < NC) load(va1, vb1, (c + 3) * KH);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(1);
+    if (c + 2 < NC) store(va0, vb0, 0);  // chunk c + 2
     __syncthreads();
   }
 }
 
 // tile_dot over int8 operands (raw 5-mer counts <= 127) on the int8 MFMA
 // (v_mfma_i32_16x16x64_i8): int32 sums, exact.  Chunks of 128 bytes per row
-// (two MFMA k-steps of 64) in the same LDS footprint and double buffering as
-// the bf16 form, so a chunk moves the same bytes for twice the k.  Rows past
-// n load row n - 1, as in tile_dot.
+// (two MFMA k-steps of 64) in the same LDS footprint, double buffering and
+// two-chunk-ahead loads as the bf16 form, so a chunk moves the same bytes for
+// twice the k.  Rows past n load row n - 1, as in tile_dot.
 template <int K>
 __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const int8_t *__restrict__ B, int n,
                                             int i0, int j0, int8_t *sA, int8_t *sB, i32x4 (&acc)[4][4]) {
@@ -452,35 +466,29 @@ __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const 
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 16 * (lane >> 4);
   // 128 rows x 128 bytes = 1024 pieces of 16 bytes per operand; 4 per thread
-  uint4 va[4], vb[4];
-  auto load = [&](int kh) {
+  constexpr int NC = K / KH;  // two register sets, as in tile_dot
+  static_assert(K % KH == 0 && NC % 2 == 0, "whole pairs of chunks");
+  i32x4 va0[4], vb0[4], va1[4], vb1[4];  // native vectors: registers, not a stack object
+  auto load = [&](i32x4 (&va)[4], i32x4 (&vb)[4], int kh) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
       const int r = ch >> 3, k = (ch & 7) * 16;
-      va[t] = *reinterpret_cast<const uint4 *>(A + (size_t)min(i0 + r, n - 1) * K + kh + k);
-      vb[t] = *reinterpret_cast<const uint4 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
+      va[t] = *reinterpret_cast<const i32x4 *>(A + (size_t)min(i0 + r, n - 1) * K + kh + k);
+      vb[t] = *reinterpret_cast<const i32x4 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
     }
   };
-  auto store = [&](int stage) {
+  auto store = [&](const i32x4 (&va)[4], const i32x4 (&vb)[4], int stage) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int ch = tid + 256 * t;
       const int r = ch >> 3, k = (ch & 7) * 16;
-      *reinterpret_cast<uint4 *>(sA + stage * STG + r * LK + k) = va[t];
-      *reinterpret_cast<uint4 *>(sB + stage * STG + r * LK + k) = vb[t];
+      *reinterpret_cast<i32x4 *>(sA + stage * STG + r * LK + k) = va[t];
+      *reinterpret_cast<i32x4 *>(sB + stage * STG + r * LK + k) = vb[t];
     }
   };
-  __syncthreads();  // the previous user of both stages is done
-  load(0);
-  store(0);
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c * KH < K; ++c) {
-    const int cur = c & 1;
-    const bool more = (c + 1) * KH < K;
-    if (more) load((c + 1) * KH);
-    const int8_t *a = sA + cur * STG, *b = sB + cur * STG;
+  auto mfma = [&](int stage) {
+    const int8_t *a = sA + stage * STG, *b = sB + stage * STG;
 #pragma unroll
     for (int ks = 0; ks < KH; ks += 64) {
       i32x4 af[4], bfr[4];
@@ -494,21 +502,27 @@ __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const 
         for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
-    if (more) store(cur ^ 1);
+  };
+  __syncthreads();  // the previous user of both stages is done
+  load(va0, vb0, 0);
+  load(va1, vb1, KH);
+  store(va0, vb0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; c += 2) {
+    if (c + 2 < NC) load(va0, vb0, (c + 2) * KH);  // set 0 held chunk c, already in stage 0
+    __builtin_amdgcn_sched_barrier(0);  // issue the loads here, ahead of the math
+    mfma(0);
+    store(va1, vb1, 1);  // chunk c + 1
+    __syncthreads();
+    if (c + 3 < NC) load(va1, vb1, (c + 3) * KH);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(1);
+    if (c + 2 < NC) store(va0, vb0, 0);  // chunk c + 2
     __syncthreads();
   }
 }
 
-// 128 x 128 tiles over the length band: the host lists the band tiles (ti, tj)
-// (row tile ti, column tiles ti .. that the length bound can reach) in
-// tile_list, in blocks of kTileBlock row tiles and, within a block, by column
-// tile: consecutive tiles share their column operand and cycle through the
-// block's row operands.  4 waves of 64 x 64 per tile.  Stage 1: Es Es^T
-// (K = 128) and the radius test.  Stage 2 (kCos, skipped for a tile with no
-// stage-1 survivor): Xs Xs^T over the 1024 raw 5-mer counts and the cosine
-// test dot >= cmin * |x_p| |x_q|, cmin = cmin of the longer sequence (< 0:
-// no test).  Hits are counted first and reserved with one atomic per wave,
-// then written (sorted positions p << 32 | q) at consecutive addresses.
 constexpr int kTileBlock = 8;  // row tiles per block of the band-tile order
 
 template <bool kCos, bool kI8>
@@ -1415,6 +1429,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   }
   int rc = list_bound(c, eps_max, &eps_max);
   if (rc) return fail(rc);
+  mnhn_phase_clock clk(st, "prefilter");
   (void)hipEventRecord(e0, st);
   if ((rc = profiles_dev(c, &X, nullptr))) return fail(rc);
   stt.profile_ms = phase_ms();
@@ -1467,6 +1482,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   }
   dev_free(E16);
   stt.embed_ms = phase_ms();
+  clk.mark("profiles, embedding, count rows");
   // ---- proposals over the length band: row tile ti reaches column tiles
   // ti .. tj_max(ti), the last whose longest length is >= shortest(ti) - max_dl
   const int tiles = (n + 127) / 128;
@@ -1502,6 +1518,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
         hipStreamSynchronize(st) != hipSuccess)
       return fail(MNHN_ERR_DEVICE);
   }
+  clk.mark("band tile list");
   // candidate buffer: the band's pair count, at most MNHN_PREFILTER_CAND_GB
   // (default 8) GB; grown to the exact count and recomputed once on overflow
   const unsigned long long band_pairs = (unsigned long long)nblk * 128ull * 128ull;
@@ -1512,6 +1529,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   for (int attempt = 0; attempt < 2; ++attempt) {
     dev_free(cand);
     if ((rc = dev_alloc(&cand, cap))) break;
+    clk.mark("candidate buffer");
     hipError_t e = hipMemsetAsync(d_cnt64, 0, 16, st);
     if (e == hipSuccess) {
       (void)hipEventRecord(ek0, st);
@@ -1546,6 +1564,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   Xs = nullptr;
   stt.candidates = (int64_t)m;
   stt.propose_ms = phase_ms();
+  clk.mark("pairwise kernel, frees");
   // ---- exact verification: candidates grouped per row into targets of 128
   unsigned long long nk = 0;
   if (m > 0) {
@@ -1561,6 +1580,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     // row segments of the candidates, sorted by target position within rows
+    clk.mark("row counts");
     std::vector<long long> rsh((size_t)n + 1, 0);
     for (int i = 0; i < n; ++i) rsh[i + 1] = rsh[i] + cnt[i];
     if ((rc = dev_alloc(&d_rs, (size_t)n + 1)) || (rc = dev_alloc(&seg, (size_t)m))) return fail(rc);
@@ -1574,6 +1594,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if ((rc = dev_alloc(&sorted, (size_t)m))) return fail(rc);
     if ((rc = sort_row_segments(st, d_rs, seg, sorted, nullptr, nullptr, n, n))) return fail(rc);
     dev_free(seg);
+    clk.mark("scatter, row sort");
     // each row's targets in length bands by the lane format their pairs need
     // (the row's sequence is the longer of each pair, so the target's length
     // bounds the score): match * len beyond 65535 -> 32-bit form, beyond 2048
@@ -1586,6 +1607,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipMemcpyAsync(kb.data(), kband, 2 * (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    clk.mark("length bands");
     // Remainders of at most 64 fp16-band or int16-band targets go to half
     // groups shared by two rows of the same band (two queries per wave),
     // paired longest first; MNHN_VERIFY_SPLIT=0 keeps one row per group.
@@ -1657,6 +1679,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       }
       part_begin[4] = o;
     }
+    clk.mark("host group layout");
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
         (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 5 * (size_t)n)) ||
@@ -1685,6 +1708,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 5 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
+    clk.mark("group buffers, row groups");
     double part_cells[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
       for (int t = 0; t < 3; ++t) {
@@ -1711,8 +1735,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     // once more
     unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap));
     double kernel_ms = 0, form_cells[3] = {0, 0, 0}, form_ms[3] = {0, 0, 0};
+    clk.mark("cell counts");
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
+      clk.mark("key buffer");
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
       for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
@@ -1750,6 +1776,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if (nk <= c->keys_cap) break;
       kcap = nk;
     }
+    clk.mark("verification kernels");
     if (nk > c->keys_cap) return fail(MNHN_ERR_NOMEM);
     c->last.kernel_ms = kernel_ms;
     c->last.pairs = (double)m;
@@ -1764,7 +1791,9 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   stt.neighbours = (int64_t)(nk / 2);
   rc = csr_from_keys(c, nk, out);
   stt.csr_ms = phase_ms();
+  clk.mark("lists (csr)");
   cleanup();
+  clk.mark("release");
   if (stats) *stats = stt;
   return rc;
 }

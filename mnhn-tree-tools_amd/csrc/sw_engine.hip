@@ -1015,10 +1015,22 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
       };
       if (nstrips > 0 && nrows > 0) {
         // strips are right-aligned: the first holds the rem = 1..S leftmost
-        // columns and runs in the narrowest of S/4, S/2, 3S/4 and S that
-        // holds them (multiples of 8 columns), so it pads fewer than S/4
+        // columns and runs in the narrowest multiple of 8 columns that holds
+        // them (S = 64: 8, 16, ..., 64; S = 32: 8, 16, 24, 32), so it pads
+        // fewer than 8
         const int rem = ncols - (nstrips - 1) * S;
-        if constexpr (S >= 32) {
+        if constexpr (S == 64) {
+          switch ((rem + 7) >> 3) {
+            case 1: strip(Width<8>{}, rem - 8 + 1, std::true_type{}); break;
+            case 2: strip(Width<16>{}, rem - 16 + 1, std::true_type{}); break;
+            case 3: strip(Width<24>{}, rem - 24 + 1, std::true_type{}); break;
+            case 4: strip(Width<32>{}, rem - 32 + 1, std::true_type{}); break;
+            case 5: strip(Width<40>{}, rem - 40 + 1, std::true_type{}); break;
+            case 6: strip(Width<48>{}, rem - 48 + 1, std::true_type{}); break;
+            case 7: strip(Width<56>{}, rem - 56 + 1, std::true_type{}); break;
+            default: strip(Width<64>{}, rem - 64 + 1, std::true_type{}); break;
+          }
+        } else if constexpr (S == 32) {
           if (rem <= S / 4) strip(Width<S / 4>{}, rem - S / 4 + 1, std::true_type{});
           else if (rem <= S / 2) strip(Width<S / 2>{}, rem - S / 2 + 1, std::true_type{});
           else if (rem <= 3 * S / 4) strip(Width<3 * S / 4>{}, rem - 3 * S / 4 + 1, std::true_type{});
@@ -2423,6 +2435,7 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
       (rc = dev_alloc(&d_col, nk)) || (rc = dev_alloc(&d_dist, nk))) {
     cleanup(); mnhn_csr_free(out); return rc;
   }
+  mnhn_phase_clock clk(c->stream, "csr");
   const int tb = 256;
   const unsigned kblocks = stride_blocks((long long)nk, tb);
   std::vector<int32_t> cnt(n);
@@ -2459,15 +2472,19 @@ int mnhn_detail::csr_from_keys(mnhn_sw_ctx *c, unsigned long long nk, mnhn_csr *
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[3]);
   float post_ms = 0.f;
   if (e == hipSuccess) (void)hipEventElapsedTime(&post_ms, c->ev[2], c->ev[3]);
+  clk.mark("count, scatter, row sort");
   if (e == hipSuccess) {
     out->col = (int32_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(int32_t));
     out->dist = (uint16_t *)host_alloc(std::max<size_t>(nk, 1) * sizeof(uint16_t));
     if (!out->col || !out->dist) { cleanup(); mnhn_csr_free(out); return MNHN_ERR_NOMEM; }
     rc = copy_d2h(c, out->col, d_col, nk * sizeof(int32_t));
+    clk.mark("columns to host");
     if (rc == MNHN_OK) rc = copy_d2h(c, out->dist, d_dist, nk * sizeof(uint16_t));
+    clk.mark("distances to host");
     if (rc != MNHN_OK) e = hipErrorUnknown;
   }
   cleanup();
+  clk.mark("release");
   if (e != hipSuccess) { mnhn_csr_free(out); return MNHN_ERR_DEVICE; }
   out->nnz = (int64_t)nk;
   c->last.kernel_ms = kernel_ms;

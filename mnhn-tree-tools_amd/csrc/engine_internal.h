@@ -37,7 +37,7 @@ struct mnhn_phase_clock {
 };
 
 // Pinned staging buffers for large device-to-host copies (allocated on first
-// use, 64 MB each; released by stage_free).
+// use, 16 MB each; released by stage_free).
 struct mnhn_detail_stage {
   void *h[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};

@@ -1613,21 +1613,25 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     // paired longest first; MNHN_VERIFY_SPLIT=0 keeps one row per group.
     const char *split_env = getenv("MNHN_VERIFY_SPLIT");
     const int split = split_env && atoi(split_env) == 0 ? 0 : 1;
-    auto band_k = [&](int i, int t) {
-      const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
-      return t == 0 ? e0 : t == 1 ? e1 - e0 : cnt[i] - e1;
-    };
-    auto band_groups = [&](int i, int t) { return band_full_groups(band_k(i, t), t, split); };
-    for (int i = 0; i < n; ++i) gb[i + 1] = gb[i] + band_groups(i, 0) + band_groups(i, 1) + band_groups(i, 2);
-    const int Gf = gb[n];
-    std::vector<int32_t> hrows2, hrows1;
+    // ng[3i + t]: full target groups of row i in band t
+    std::vector<int32_t> ng(3 * (size_t)n);
     for (int i = 0; i < n; ++i) {
-      if (band_half(band_k(i, 2), 2, split)) hrows2.push_back(i);
-      if (band_half(band_k(i, 1), 1, split)) hrows1.push_back(i);
+      const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
+      const int k3[3] = {e0, e1 - e0, cnt[i] - e1};
+      int sum = 0;
+      for (int t = 0; t < 3; ++t) sum += (ng[3 * (size_t)i + t] = band_full_groups(k3[t], t, split));
+      gb[i + 1] = gb[i] + sum;
     }
-    auto longer = [&](int x, int y) { return c->len[x] > c->len[y]; };
-    std::stable_sort(hrows2.begin(), hrows2.end(), longer);
-    std::stable_sort(hrows1.begin(), hrows1.end(), longer);
+    const int Gf = gb[n];
+    // rows with a band-2 / band-1 half group, longest first: the length order
+    // breaks ties by id, as a stable sort of the ids by length would
+    std::vector<int32_t> hrows2, hrows1;
+    for (int p = 0; p < n; ++p) {
+      const int i = c->order[p];
+      const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
+      if (band_half(cnt[i] - e1, 2, split)) hrows2.push_back(i);
+      if (band_half(e1 - e0, 1, split)) hrows1.push_back(i);
+    }
     const int np2 = (int)((hrows2.size() + 1) / 2), np1 = (int)((hrows1.size() + 1) / 2);
     const int np = np2 + np1;
     const int G = Gf + np;
@@ -1664,20 +1668,20 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       const int pi = (hg[2 * (size_t)i] >> 1) - Gf;
       return (S == 64 && c->len[q2[2 * (size_t)pi]] > 128) ? 5 : 4;
     };
+    // first item of each (row, band): by class, then row, then band
     std::vector<int32_t> ib(3 * (size_t)n, 0);
     int part_begin[5] = {0, 0, 0, 0, 0};
     {
-      int o = 0;
-      for (int cls = 0; cls < 4; ++cls) {
-        part_begin[cls] = o;
-        for (int i = 0; i < n; ++i)
-          for (int t = 0; t < 3; ++t)
-            if (band_class(i, t) == cls) {
-              ib[3 * (size_t)i + t] = o;
-              o += band_groups(i, t);
-            }
-      }
-      part_begin[4] = o;
+      for (int i = 0; i < n; ++i)
+        for (int t = 0; t < 3; ++t) part_begin[band_class(i, t) + 1] += ng[3 * (size_t)i + t];
+      for (int cls = 0; cls < 4; ++cls) part_begin[cls + 1] += part_begin[cls];
+      int off[4] = {part_begin[0], part_begin[1], part_begin[2], part_begin[3]};
+      for (int i = 0; i < n; ++i)
+        for (int t = 0; t < 3; ++t) {
+          const int cls = band_class(i, t);
+          ib[3 * (size_t)i + t] = off[cls];
+          off[cls] += ng[3 * (size_t)i + t];
+        }
     }
     clk.mark("host group layout");
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||

@@ -146,11 +146,34 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_u4;
 
+// Profile-table row of a lane's target code pair (lo, hi) (codes 0-3 = A, C,
+// G, T; 4 = past the target's end).  The 16 pairs of residues come first
+// (lo * 4 + hi): with the table's row stride of S + 4 words (an odd number of
+// 16-byte bank groups mod 64 banks) the rows the lanes of one DP row read lie
+// in distinct bank groups, so profile reads of different rows do not
+// conflict; the 9 pairs with a past-the-end code follow: (4, hi) -> 16 + hi,
+// (lo < 4, 4) -> 21 + lo.
+__device__ __forceinline__ uint32_t tab_row(uint32_t lo, uint32_t hi) {
+  return ((lo | hi) & 4u) ? (lo == 4u ? 16u + hi : 21u + lo) : lo * 4u + hi;
+}
+__device__ __forceinline__ void tab_codes(int p, uint32_t &lo, uint32_t &hi) {
+  if (p < 16) {
+    lo = (uint32_t)p >> 2;
+    hi = (uint32_t)p & 3u;
+  } else if (p <= 20) {
+    lo = 4u;
+    hi = (uint32_t)(p - 16);
+  } else {
+    lo = (uint32_t)(p - 21);
+    hi = 4u;
+  }
+}
+
 // LDS kernels: the DP rows of a work item's target group, decoded once per
 // item instead of once per strip and query.  Word m of the row-address list
 // (workspace row list_row + m) holds, for DP rows j = 2m + 1 (low half) and
 // 2m + 2 (high half), the LDS byte address of the profile row that the
-// lane's two target residues select, (c_lo * 5 + c_hi) * row_bytes past the
+// lane's two target residues select, tab_row(c_lo, c_hi) * row_bytes past the
 // wave's table, with code 4 for rows past a target's end.  Rows are the
 // target residues 1..nrows; the words read are those of residues
 // 0..16 * ((npairs - 1) / 8) + 15, inside the words the row loop of the
@@ -172,8 +195,8 @@ __device__ __forceinline__ void build_row_list(const KArgs &a, const WaveRows &w
       c1l = j + 1 > rows_lo ? 4u : c1l;
       c0h = j > rows_hi ? 4u : c0h;
       c1h = j + 1 > rows_hi ? 4u : c1h;
-      const uint32_t a0 = tab_lds + (c0l * 5u + c0h) * row_bytes;
-      const uint32_t a1 = tab_lds + (c1l * 5u + c1h) * row_bytes;
+      const uint32_t a0 = tab_lds + tab_row(c0l, c0h) * row_bytes;
+      const uint32_t a1 = tab_lds + tab_row(c1l, c1h) * row_bytes;
       ws.store(a.list_row + m, a0 | (a1 << 16));
     }
   }
@@ -639,8 +662,8 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
 // last column is always strip slot S-1), and each lane streams its two targets
 // as DP rows.  Per strip the wave writes a table of packed substitution scores
 // into its own LDS slice: one row per (code_lo, code_hi) pair of the lane's two
-// target residues (codes A, C, G, T and 4 = past the target's end), S entries
-// per row.  A DP row then reads its S scores with S/4 ds_read_b128 -- no
+// target residues (codes A, C, G, T and 4 = past the target's end; row
+// tab_row(code_lo, code_hi)), S entries per row.  A DP row then reads its S scores with S/4 ds_read_b128 -- no
 // per-row selection, no register copies.  Columns left of the query's first
 // residue ("pre-padding" of the first strip) hold -32768, which keeps their
 // cells at the floor.
@@ -742,7 +765,8 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
           if (k >= 1) {
             const uint32_t r = (uint32_t)(k - 1);
             const uint32_t qc = (a.words[(sg ? qwo1 : qwo0) + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            const uint32_t clo = (uint32_t)(p / 5), chi = (uint32_t)(p - 5 * (p / 5));
+            uint32_t clo, chi;
+            tab_codes(p, clo, chi);
             v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
           }
           tab[sg * kTab + p * RS + jj] = v;
@@ -962,7 +986,8 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
           if (k >= 1) {
             const uint32_t r = (uint32_t)(k - 1);
             const uint32_t qc = (a.words[(sg ? qwo1 : qwo0) + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            const uint32_t clo = (uint32_t)(p / 5), chi = (uint32_t)(p - 5 * (p / 5));
+            uint32_t clo, chi;
+            tab_codes(p, clo, chi);
             v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
           }
           tab[sg * kTab + p * RS + jj] = v;

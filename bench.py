@@ -135,21 +135,26 @@ def cpu_baseline(seqs, sample_n, seed=7, threads=1, port=False):
                       f"host: {cpu_model()}, nproc {os.cpu_count()}"}
 
 
-def profiled_traffic(config, key="traffic_bytes_fp16_launches"):
+def profiled_traffic(config, strip, key="traffic_bytes_fp16_launches"):
     """HBM bytes of the SW kernels from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (calibrated against a plain copy
     kernel): `traffic_bytes_fp16_launches` = the fp16-form launches of one
     step (the roofline's kernel), `traffic_bytes_per_step` = every SW launch
-    of the step; None if there is no summary for it."""
+    of the step; None if there is no summary for it or it was taken at
+    another strip width than the step ran (`traffic_strip`)."""
     path = os.path.join(ROOT, "profiles", {"tree20k": "sw_lds_f16_kernel_tree20k.md"}.get(config, "-"))
+    vals = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith(key + ":"):
-                    return float(line.split(":", 1)[1])
+                k, sep, v = line.partition(":")
+                if sep and k in (key, "traffic_strip"):
+                    vals[k] = float(v)
     except OSError:
         pass
-    return None
+    if vals.get("traffic_strip") != strip:
+        return None
+    return vals.get(key)
 
 
 def run_dbscan(seqs, eps_start, eps_inc, minpts):
@@ -702,8 +707,8 @@ def main():
                      "unit": "GCUPS (cells of the pairs the kernel computed / its time, HIP events on the "
                              "engine's stream; peak = the form's instruction roofline)",
                      "frac": f16["gcups"] / f16_peak,
-                     "traffic": profiled_traffic(args.config),
-                     "step_traffic": profiled_traffic(args.config, "traffic_bytes_per_step"),
+                     "traffic": profiled_traffic(args.config, t["strip"]),
+                     "step_traffic": profiled_traffic(args.config, t["strip"], "traffic_bytes_per_step"),
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload, calibrated on a "
                                        "plain copy (profiles/); not measured in this run",
                      "step_kernel_ms": kernel_s * 1e3,

@@ -2783,7 +2783,14 @@ extern "C" int mnhn_sw_autotune(mnhn_sw_ctx *c, int use_cache, mnhn_sw_tuning *o
     if (tv && *tv && strcmp(tv, "0") != 0)
       fprintf(stderr, "[mnhn] autotune kernel %d strip %d waves/CU %d: %.0f GCUPS on %.3g sample cells\n",
               cd.kernel, cd.strip, cd.wpc, g, sample.cells);
-    if (g > best_gcups) {
+    // a narrower strip must win by 1 % per halving: within the sample's
+    // noise the wider strip moves half the workspace bytes per cell and
+    // carries half the per-row bookkeeping, and the choice stays stable
+    // from run to run
+    double need = best_gcups;
+    for (int sw = cd.strip; sw < best.strip; sw *= 2) need *= 1.01;
+    for (int sw = best.strip; best.kernel && sw < cd.strip; sw *= 2) need /= 1.01;
+    if (g > need) {
       best_gcups = g;
       best = cd;
     }

@@ -176,6 +176,17 @@ int list_bound(const mnhn_sw_ctx *c, int32_t eps_max, int32_t *eff);
 // Make sure c->d_keys holds at least cap keys.
 int ensure_keys(mnhn_sw_ctx *c, unsigned long long cap);
 
+// Epsilon-pass work over the unique pairs of the sequences at length-order
+// positions p0 .. n-1 (the shortest ones; p0 = 0: all pairs), with the
+// length-bound pruning of the all-pairs pass; items finalised for launch.
+Work suffix_work(const mnhn_sw_ctx *c, int32_t eps, int p0);
+
+// Keys an epsilon pass of work w will emit (both directions of every listed
+// pair), estimated from a systematic 0.5 % sample of its items run into
+// c->d_keys (overwritten), times 1.3; w's pair bound (2 keys per pair) when
+// it has fewer than 4,096 items.
+int eps_keys_estimate(mnhn_sw_ctx *c, const Work &w, int32_t eps, unsigned long long *est);
+
 enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3, KERNEL_I32 = 4 };
 enum { MODE_DIST = 0, MODE_EPS = 1 };
 constexpr int kGroupSize = 128;

@@ -523,13 +523,25 @@ __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const 
   }
 }
 
+// 128 x 128 tiles over the length band: the host lists the band tiles (ti, tj)
+// (row tile ti, column tiles ti .. that the length bound can reach) in
+// tile_list, in blocks of kTileBlock row tiles and, within a block, by column
+// tile: consecutive tiles share their column operand and cycle through the
+// block's row operands.  4 waves of 64 x 64 per tile.  Stage 1: Es Es^T
+// (K = 128) and the radius test.  Stage 2 (kCos, skipped for a tile with no
+// stage-1 survivor): Xs Xs^T over the 1024 raw 5-mer counts and the cosine
+// test dot >= cmin * |x_p| |x_q|, cmin = cmin of the longer sequence (< 0:
+// no test).  Pairs whose longer sequence is at most exact_len long are left
+// to the exact pass over the shortest sequences (exact_len < 0: none).  Hits
+// are counted first and reserved with one atomic per wave, then written
+// (sorted positions p << 32 | q) at consecutive addresses.
 constexpr int kTileBlock = 8;  // row tiles per block of the band-tile order
 
 template <bool kCos, bool kI8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void pair_candidates_kernel(
     const bf16 *__restrict__ Es, const float *__restrict__ norm, const float *__restrict__ srad,
     const void *__restrict__ Xv, const float *__restrict__ pnorm, const float *__restrict__ cmin,
-    const int32_t *__restrict__ lens, int n, const int2 *__restrict__ tile_list, int max_dl,
+    const int32_t *__restrict__ lens, int n, const int2 *__restrict__ tile_list, int max_dl, int exact_len,
     unsigned long long *__restrict__ out, unsigned long long *__restrict__ count, unsigned long long cap,
     unsigned long long *__restrict__ stage2) {
   constexpr int BT = 128, LK = 64 + 8;
@@ -593,8 +605,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
           const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
           const int ci = min(gi, n - 1);
           const float si = srad[ci];
-          const float ri = si * si - norm[ci];
           const int li = gi < n ? lens[ci] : -(1 << 28);
+          const float ri = li > exact_len ? si * si - norm[ci] : -FLT_MAX;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const bool h = (unsigned)(li - lj[j]) <= (unsigned)max_dl && nj[j] - 2.0f * acc[i][j][r] <= ri;
@@ -619,7 +631,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
             const int dl = li - lj[j];
             const float rad2 = dl >= 0 ? si2 : sj2[j];
             const float d2 = ni + nj[j] - 2.0f * acc[i][j][r];
-            const bool h = gi < gj && abs(dl) <= max_dl && d2 <= rad2;
+            const bool h = gi < gj && abs(dl) <= max_dl && d2 <= rad2 && max(li, lj[j]) > exact_len;
             const int bit = 16 * i + 4 * r + j;
             m1 |= h ? 1ull << bit : 0ull;
             mlong |= dl >= 0 ? 1ull << bit : 0ull;
@@ -1483,14 +1495,56 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   dev_free(E16);
   stt.embed_ms = phase_ms();
   clk.mark("profiles, embedding, count rows");
+  // ---- exact bucket: the shortest lengths, whose buckets switch both tests
+  // off (radius margin >= kMarginOff, no cosine bound), would have every band
+  // pair proposed; the all-pairs pass over those sequences lists them
+  // directly (positions p0 .. n-1 of the length order) instead of through
+  // the candidate lists.  Lists only gain pairs by it, never lose any.
+  // MNHN_PREFILTER_EXACT_BUCKET=0 sends those pairs through the candidate
+  // lists as well.
+  constexpr float kMarginOff = 100.f;
+  const char *xb_env = getenv("MNHN_PREFILTER_EXACT_BUCKET");
+  const bool exact_bucket = !(xb_env && atoi(xb_env) == 0);
+  auto host_bucket = [](const MarginTable &t, int L) {
+    float v = t.margin[t.n - 1];
+    for (int k = t.n - 1; k >= 0; --k)
+      if (L <= t.upper[k]) v = t.margin[k];
+    return v;
+  };
+  int exact_len = -1;
+  {
+    std::vector<int> edges;
+    for (int k = 0; k < enc->margins.n; ++k) edges.push_back(enc->margins.upper[k]);
+    for (int k = 0; k < enc->cosmin.n; ++k) edges.push_back(enc->cosmin.upper[k]);
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    for (int e : edges) {  // both tables are constant between consecutive edges
+      if (exact_bucket && host_bucket(enc->margins, e) >= kMarginOff && host_bucket(enc->cosmin, e) < 0.f)
+        exact_len = e;
+      else break;
+    }
+  }
+  int p0 = n;  // first length-order position with len <= exact_len
+  {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (c->len[c->order[mid]] > exact_len) lo = mid + 1;
+      else hi = mid;
+    }
+    p0 = lo;
+  }
   // ---- proposals over the length band: row tile ti reaches column tiles
-  // ti .. tj_max(ti), the last whose longest length is >= shortest(ti) - max_dl
+  // ti .. tj_max(ti), the last whose longest length is >= shortest(ti) - max_dl;
+  // row tiles whose longest row is in the exact bucket (ti >= t_short) have
+  // no pair to propose
   const int tiles = (n + 127) / 128;
+  const int t_short = (p0 + 127) / 128;
   std::vector<int32_t> tjmax(tiles);
   long long nblk_ll = 0;
   {
     int tj = 0;
-    for (int ti = 0; ti < tiles; ++ti) {
+    for (int ti = 0; ti < t_short; ++ti) {
       const long long shortest = c->len[c->order[std::min((ti + 1) * 128, n) - 1]];
       tj = std::max(tj, ti);
       while (tj + 1 < tiles && (long long)c->len[c->order[(tj + 1) * 128]] >= shortest - max_dl) ++tj;
@@ -1503,10 +1557,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   stt.tiles = nblk;
   // the band tiles in blocks of kTileBlock row tiles, by column tile within a block
   {
-    std::vector<int2> tl((size_t)nblk);
+    std::vector<int2> tl(std::max<size_t>((size_t)nblk, 1));
     size_t o = 0;
-    for (int b0 = 0; b0 < tiles; b0 += kTileBlock) {
-      const int b1 = std::min(tiles, b0 + kTileBlock);
+    for (int b0 = 0; b0 < t_short; b0 += kTileBlock) {
+      const int b1 = std::min(t_short, b0 + kTileBlock);
       for (int tj = b0; tj <= tjmax[b1 - 1]; ++tj)
         for (int ti = b0; ti < b1 && ti <= tj; ++ti)
           if (tj <= tjmax[ti]) tl[o++] = make_int2(ti, tj);
@@ -1520,10 +1574,16 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   }
   clk.mark("band tile list");
   // candidate buffer: the band's pair count, at most MNHN_PREFILTER_CAND_GB
-  // (default 8) GB; grown to the exact count and recomputed once on overflow
+  // GB (default: 8 GB or 40 % of the free device memory, the larger; an
+  // allocation of 100 GB took 0.4 ms on MI355X); grown to the exact count and
+  // recomputed once on overflow
   const unsigned long long band_pairs = (unsigned long long)nblk * 128ull * 128ull;
   const char *cg = getenv("MNHN_PREFILTER_CAND_GB");
-  const double cand_gb = cg ? atof(cg) : 8.0;
+  double cand_gb = cg ? atof(cg) : 8.0;
+  if (!cg) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) cand_gb = std::max(cand_gb, 0.4 * (double)free_b / 1e9);
+  }
   unsigned long long cap = std::max<unsigned long long>(
       1ull << 20, std::min<unsigned long long>(band_pairs, (unsigned long long)(cand_gb * 1e9 / 8.0))), m = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -1531,20 +1591,20 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if ((rc = dev_alloc(&cand, cap))) break;
     clk.mark("candidate buffer");
     hipError_t e = hipMemsetAsync(d_cnt64, 0, 16, st);
-    if (e == hipSuccess) {
-      (void)hipEventRecord(ek0, st);
+    (void)hipEventRecord(ek0, st);
+    if (e == hipSuccess && nblk > 0) {  // no band tile: every pair is the exact pass's
       if (use_cos && xs_i8)
         hipLaunchKernelGGL((pair_candidates_kernel<true, true>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
-                           pnorm, cmin, lens, n, d_tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+                           pnorm, cmin, lens, n, d_tiles, max_dl, exact_len, cand, d_cnt64, cap, d_cnt64 + 1);
       else if (use_cos)
         hipLaunchKernelGGL((pair_candidates_kernel<true, false>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
-                           pnorm, cmin, lens, n, d_tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+                           pnorm, cmin, lens, n, d_tiles, max_dl, exact_len, cand, d_cnt64, cap, d_cnt64 + 1);
       else
         hipLaunchKernelGGL((pair_candidates_kernel<false, false>), dim3(nblk), dim3(256), 0, st, Es, norm, srad, Xs,
-                           pnorm, cmin, lens, n, d_tiles, max_dl, cand, d_cnt64, cap, d_cnt64 + 1);
+                           pnorm, cmin, lens, n, d_tiles, max_dl, exact_len, cand, d_cnt64, cap, d_cnt64 + 1);
       e = hipGetLastError();
-      (void)hipEventRecord(ek1, st);
     }
+    (void)hipEventRecord(ek1, st);
     unsigned long long cnt2[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(cnt2, d_cnt64, 16, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1565,6 +1625,37 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   stt.candidates = (int64_t)m;
   stt.propose_ms = phase_ms();
   clk.mark("pairwise kernel, frees");
+  // ---- the exact bucket's pass over the shortest sequences: its keys follow
+  // the verification's in the key buffer (same attempts); its pairs count as
+  // verified candidates
+  Work wsub;
+  unsigned long long est_sub = 0;
+  if (p0 < n - 1) {
+    wsub = suffix_work(c, eps_max, p0);
+    if (!wsub.items.empty() && (rc = eps_keys_estimate(c, wsub, eps_max, &est_sub))) return fail(rc);
+    // the pairs the proposal would have made there: every pair within the
+    // length band (positions q > p with len >= len[p] - max_dl)
+    long long band = 0;
+    for (int p = p0, r = p0; p < n; ++p) {
+      r = std::max(r, p);
+      while (r + 1 < n && c->len[c->order[r + 1]] >= c->len[c->order[p]] - max_dl) ++r;
+      band += r - p;
+    }
+    stt.candidates += band;
+    stt.cells += wsub.cells;
+  }
+  double kernel_ms = 0, form_cells[3] = {0, 0, 0}, form_ms[3] = {0, 0, 0};
+  auto run_sub = [&]() -> int {
+    if (wsub.items.empty()) return MNHN_OK;
+    const int r = launch_tiles(c, wsub, MODE_EPS, 1, eps_max, nullptr, c->d_keys, c->keys_cap);
+    kernel_ms += c->last.kernel_ms;
+    for (int f = 0; f < 3; ++f) {
+      form_cells[f] += c->last.form_cells[f];
+      form_ms[f] += c->last.form_ms[f];
+    }
+    return r;
+  };
+  clk.mark("exact bucket work, key estimate");
   // ---- exact verification: candidates grouped per row into targets of 128
   unsigned long long nk = 0;
   if (m > 0) {
@@ -1737,8 +1828,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     // key buffer: both directions of every neighbour; start from at most 2^28
     // keys and, if the neighbours need more, grow to the exact count and verify
     // once more
-    unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap));
-    double kernel_ms = 0, form_cells[3] = {0, 0, 0}, form_ms[3] = {0, 0, 0};
+    unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap)) + est_sub;
     clk.mark("cell counts");
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
@@ -1775,6 +1865,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
           form_ms[f] += c->last.form_ms[f];
         }
       }
+      if (rc == MNHN_OK) rc = run_sub();
       if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
       if (rc) return fail(rc);
       if (nk <= c->keys_cap) break;
@@ -1782,8 +1873,25 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     }
     clk.mark("verification kernels");
     if (nk > c->keys_cap) return fail(MNHN_ERR_NOMEM);
+  } else if (!wsub.items.empty()) {  // no candidate: the exact bucket's pass alone
+    unsigned long long kcap = std::max<unsigned long long>(est_sub, 1024ull);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if ((rc = ensure_keys(c, kcap))) return fail(rc);
+      if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
+      kernel_ms = 0;
+      for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
+      rc = run_sub();
+      if (rc == MNHN_OK && hipMemcpy(&nk, c->d_nkeys, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = MNHN_ERR_DEVICE;
+      if (rc) return fail(rc);
+      if (nk <= c->keys_cap) break;
+      kcap = nk;
+    }
+    clk.mark("exact bucket kernels");
+    if (nk > c->keys_cap) return fail(MNHN_ERR_NOMEM);
+  }
+  if (m > 0 || !wsub.items.empty()) {
     c->last.kernel_ms = kernel_ms;
-    c->last.pairs = (double)m;
+    c->last.pairs = (double)m + wsub.pairs;
     c->last.cells = stt.cells;
     for (int f = 0; f < 3; ++f) {
       c->last.form_cells[f] = form_cells[f];

@@ -1639,10 +1639,13 @@ void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
 // local score is at most match * min(la, lb), so d >= match * |la - lb|.
 // Lengths fall along the order, so for target group g the queries that the
 // bound removes (match * (len_p - gmax[g]) > eps) are a prefix of positions.
-Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
+// With p0 > 0 only the pairs among positions p0 .. n-1 are covered: queries
+// start at p0, and a target group's slots before p0 are never paired with
+// them (the triangle keeps targets after the query).
+Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps, int p0 = 0) {
   Work w;
   w.qids = c->order;
-  std::vector<int> first(c->groups, 0);
+  std::vector<int> first(c->groups, p0);
   if (eps >= 0) {
     for (int g = 0; g < c->groups; ++g) {
       const long long gm = c->gmax[g];
@@ -1652,7 +1655,7 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
         if ((long long)c->match * ((long long)c->len[c->order[mid]] - gm) > (long long)eps) lo = mid + 1;
         else hi = mid;
       }
-      first[g] = lo;
+      first[g] = std::max(lo, p0);
     }
   }
   double total = 0;
@@ -1672,7 +1675,7 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps) {
   std::vector<double> prefix((size_t)c->n + 1, 0.0);
   for (int p = 0; p < c->n; ++p) prefix[p + 1] = prefix[p] + c->len[c->order[p]];
   double cells = 0, pairs = 0;
-  for (int t = 0; t < c->n; ++t) {
+  for (int t = p0; t < c->n; ++t) {
     const int f = std::min(first[t / kGroup], t);
     cells += (double)c->len[c->order[t]] * (prefix[t] - prefix[f]);
     pairs += (double)(t - f);
@@ -2293,6 +2296,44 @@ int mnhn_sw_eps_csr(mnhn_sw_ctx *c, int32_t eps_max, mnhn_csr *out) {
 }
 
 }  // extern "C"
+
+Work mnhn_detail::suffix_work(const mnhn_sw_ctx *c, int32_t eps, int p0) {
+  const int S = pick_strip(c);
+  Work w = triangle_work(c, S, length_prune() ? eps : -1, std::max(0, std::min(p0, c->n)));
+  finalize_items(c, w, S, false);
+  return w;
+}
+
+int mnhn_detail::eps_keys_estimate(mnhn_sw_ctx *c, const Work &w, int32_t eps, unsigned long long *est) {
+  *est = (unsigned long long)(2.0 * w.pairs);
+  if (w.items.size() < 4096) return MNHN_OK;
+  const int S = pick_strip(c);
+  Work smp;
+  smp.qids = w.qids;
+  smp.max_q_len = w.max_q_len;
+  const size_t stride = 200;
+  double smp_cost = 0, all_cost = 0;
+  for (size_t k = 0; k < w.items.size(); ++k) {
+    const double ck = item_cost(c, w, w.items[k], S);
+    all_cost += ck;
+    if (k % stride == 0) {
+      smp.items.push_back(w.items[k]);
+      smp_cost += ck;
+    }
+  }
+  // the sample's keys are at most twice its pairs
+  double smp_pairs = 0;
+  for (const int4 &it : smp.items) smp_pairs += (double)(it.z - it.y) * kGroup;
+  int rc = ensure_keys(c, std::max<unsigned long long>((unsigned long long)(2.0 * smp_pairs), 1024ull));
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(c->d_nkeys, 0, sizeof(unsigned long long), c->stream));
+  if ((rc = launch_tiles(c, smp, MODE_EPS, 1, eps, nullptr, c->d_keys, c->keys_cap))) return rc;
+  unsigned long long sk = 0;
+  HIP_TRY(hipMemcpy(&sk, c->d_nkeys, sizeof(sk), hipMemcpyDeviceToHost));
+  const double e = (double)sk * all_cost / std::max(smp_cost, 1.0) * 1.3;
+  *est = std::min(*est, (unsigned long long)e + 1024ull);
+  return MNHN_OK;
+}
 
 int mnhn_detail::ensure_keys(mnhn_sw_ctx *c, unsigned long long cap) {
   if (cap <= c->keys_cap && c->d_keys) return MNHN_OK;

@@ -98,12 +98,17 @@ def test_training_reduces_loss(pkg):
     assert de[fam[a] == fam[b]].mean() < 0.5 * de[fam[a] != fam[b]].mean()
 
 
+@pytest.mark.parametrize("exact_bucket", ["1", "0"])
 @pytest.mark.parametrize("tuning", [None, (3, 64, 8), (2, 16, 16)])
-def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, tuning):
+def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, monkeypatch, tuning, exact_bucket):
     """A huge margin proposes every pair that passes the exact length bound,
     so the verified lists must equal the exact all-pairs lists (verification
     in every kernel form and strip width, incl. the 32-column strip the short
-    rows take when the context runs at 64)."""
+    rows take when the context runs at 64).  With the exact bucket on, a
+    margin this large switches the proposal off for every length and the
+    all-pairs pass over the shortest sequences covers the whole set; off,
+    every pair goes through the candidate lists."""
+    monkeypatch.setenv("MNHN_PREFILTER_EXACT_BUCKET", exact_bucket)
     rng = np.random.default_rng(44)
     seqs = []
     for L0 in (70, 90, 110, 125, 140, 300, 520, 560):  # short, mid and int16-only rows
@@ -140,6 +145,7 @@ def test_prefilter_half_groups_equal_exact_lists(pkg, monkeypatch, split, shape)
     beyond 512 residues run the int16 form (rows near 515 residues hold
     targets of both forms), with a lone int16 half row among them."""
     monkeypatch.setenv("MNHN_VERIFY_SPLIT", split)
+    monkeypatch.setenv("MNHN_PREFILTER_EXACT_BUCKET", "0")  # every pair through the candidate lists
     rng = np.random.default_rng(7)
     seqs = []
     if shape == "lone_half_row":
@@ -166,6 +172,47 @@ def test_prefilter_half_groups_equal_exact_lists(pkg, monkeypatch, split, shape)
     assert st["neighbours"] * 2 == len(col)
     if shape == "long_rows":
         assert st["cells"] > 0 and len(col) > 2 * len(seqs)
+
+
+def test_prefilter_exact_bucket_lists(pkg, monkeypatch):
+    """Length buckets with both tests off (radius margin >= 100, no cosine
+    bound) are listed by the all-pairs pass over the shortest sequences
+    instead of the candidate lists: the pairs whose longer sequence is in
+    those buckets equal the exact lists' pairs, the others are a subset of
+    them, and the lists equal those of the same tables run through the
+    candidate lists (MNHN_PREFILTER_EXACT_BUCKET=0); the candidate count
+    includes the bucket's band pairs."""
+    rng = np.random.default_rng(48)
+    seqs = []
+    for L0, m in ((60, 50), (85, 60), (110, 40), (118, 30), (150, 40), (240, 30), (400, 20)):
+        anc = synth.random_codes(rng, L0)
+        seqs += [synth.to_bytes(synth.mutate(rng, anc, 0.03, 0.01)) for _ in range(m)]
+    eps, cut = 300, 120
+    L = np.array([len(x) for x in seqs])
+    n = len(seqs)
+    _say(f"prefilter exact bucket n={n} cut={cut}")
+    runs = {}
+    for xb in ("1", "0"):
+        monkeypatch.setenv("MNHN_PREFILTER_EXACT_BUCKET", xb)
+        with pkg.Engine(seqs) as eng, pkg.Encoder(seed=3) as enc:
+            enc.set_margin_table(np.array([cut, 2**31 - 1], np.int32), np.array([1e3, 0.05], np.float32))
+            enc.set_cosine_table(np.array([cut, 2**31 - 1], np.int32), np.array([-1.0, 0.5], np.float32))
+            runs[xb] = enc.prefilter_csr(eng, eps)
+            erp, ecol, edist = eng.eps_csr(eps)
+    (rp, col, dist, st), (rp0, col0, dist0, st0) = runs["1"], runs["0"]
+    assert np.array_equal(rp, rp0) and np.array_equal(col, col0) and np.array_equal(dist, dist0)
+    assert st["candidates"] == st0["candidates"] and st["neighbours"] == st0["neighbours"]
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    erows = np.repeat(np.arange(n), np.diff(erp))
+    got = set(zip(rows.tolist(), col.tolist(), dist.tolist()))
+    want = set(zip(erows.tolist(), ecol.tolist(), edist.tolist()))
+    assert got <= want
+    short = {t for t in want if max(L[t[0]], L[t[1]]) <= cut}
+    assert short and {t for t in got if max(L[t[0]], L[t[1]]) <= cut} == short
+    iu = np.triu_indices(n, 1)
+    lmax = np.maximum(L[iu[0]], L[iu[1]])
+    band_short = int(((4 * np.abs(L[iu[0]] - L[iu[1]]) <= eps) & (lmax <= cut)).sum())
+    assert st["candidates"] >= band_short
 
 
 def test_prefilter_trained_recall(pkg, oracle):
@@ -246,10 +293,12 @@ def test_proposal_rule_and_margin_table_match_numpy(pkg, tmp_path):
     assert not np.any(got & ~exact)
 
 
-def test_prefilter_positions_beyond_2e31(pkg):
+def test_prefilter_positions_beyond_2e31(pkg, monkeypatch):
     """80,000 amplicon-like sequences with an unbounded radius propose 2.35e9
     pairs, so candidate positions pass 2^31 (wave-uniform 64-bit offsets);
-    the verified lists must still equal the exact all-pairs lists."""
+    the verified lists must still equal the exact all-pairs lists.  (Every
+    pair goes through the candidate lists: the exact bucket is off.)"""
+    monkeypatch.setenv("MNHN_PREFILTER_EXACT_BUCKET", "0")
     full = synth.amplicon_families(seed=44)
     pick = np.sort(np.random.default_rng(44).choice(len(full), 80000, replace=False))
     seqs = [full[k] for k in pick]

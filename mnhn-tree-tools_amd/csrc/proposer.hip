@@ -1825,10 +1825,17 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     w.d_gmax = gmax;
     w.d_gmin = gmin;
     w.max_q_len = c->maxlen;
-    // key buffer: both directions of every neighbour; start from at most 2^28
-    // keys and, if the neighbours need more, grow to the exact count and verify
-    // once more
-    unsigned long long kcap = std::min(2ull * m, std::max(1ull << 28, (unsigned long long)c->keys_cap)) + est_sub;
+    // key buffer: both directions of every neighbour, at most two keys per
+    // candidate; start from 2m capped at 40 % of the free device memory (at
+    // least 2^28 keys) and, if the neighbours need more, grow to the exact
+    // count and verify once more
+    unsigned long long kroom = std::max(1ull << 28, (unsigned long long)c->keys_cap);
+    {
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        kroom = std::max<unsigned long long>(kroom, (unsigned long long)(0.4 * (double)free_b / 8.0));
+    }
+    unsigned long long kcap = std::min(2ull * m, kroom) + est_sub;
     clk.mark("cell counts");
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);

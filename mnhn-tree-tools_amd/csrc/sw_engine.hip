@@ -2307,20 +2307,13 @@ Work mnhn_detail::suffix_work(const mnhn_sw_ctx *c, int32_t eps, int p0) {
 int mnhn_detail::eps_keys_estimate(mnhn_sw_ctx *c, const Work &w, int32_t eps, unsigned long long *est) {
   *est = (unsigned long long)(2.0 * w.pairs);
   if (w.items.size() < 4096) return MNHN_OK;
-  const int S = pick_strip(c);
+  // items are cut to a common cost (add_group_items), so the sample stands
+  // for the work by its share of the items
   Work smp;
   smp.qids = w.qids;
   smp.max_q_len = w.max_q_len;
   const size_t stride = 200;
-  double smp_cost = 0, all_cost = 0;
-  for (size_t k = 0; k < w.items.size(); ++k) {
-    const double ck = item_cost(c, w, w.items[k], S);
-    all_cost += ck;
-    if (k % stride == 0) {
-      smp.items.push_back(w.items[k]);
-      smp_cost += ck;
-    }
-  }
+  for (size_t k = 0; k < w.items.size(); k += stride) smp.items.push_back(w.items[k]);
   // the sample's keys are at most twice its pairs
   double smp_pairs = 0;
   for (const int4 &it : smp.items) smp_pairs += (double)(it.z - it.y) * kGroup;
@@ -2330,7 +2323,7 @@ int mnhn_detail::eps_keys_estimate(mnhn_sw_ctx *c, const Work &w, int32_t eps, u
   if ((rc = launch_tiles(c, smp, MODE_EPS, 1, eps, nullptr, c->d_keys, c->keys_cap))) return rc;
   unsigned long long sk = 0;
   HIP_TRY(hipMemcpy(&sk, c->d_nkeys, sizeof(sk), hipMemcpyDeviceToHost));
-  const double e = (double)sk * all_cost / std::max(smp_cost, 1.0) * 1.3;
+  const double e = (double)sk * (double)w.items.size() / (double)smp.items.size() * 1.3;
   *est = std::min(*est, (unsigned long long)e + 1024ull);
   return MNHN_OK;
 }

@@ -811,29 +811,41 @@ __global__ void band_counts_kernel(const int32_t *__restrict__ sorted, const lon
   kband[2 * i + 1] = first_at_most(t1);
 }
 
-// Full target groups of band t of a row holding k targets in it: k / 128
-// rounded up, except that a remainder of at most 64 targets of the fp16 band
-// (t = 2) or the int16 band (t = 1) goes to a half group shared with another
-// row when split is set.
-__device__ __host__ inline bool band_half(int k, int t, int split) {
-  const int r = k % kGroupSize;
-  return split && (t == 2 || t == 1) && r > 0 && r <= kGroupSize / 2;
-}
-__device__ __host__ inline int band_full_groups(int k, int t, int split) {
-  return band_half(k, t, split) ? k / kGroupSize : (k + kGroupSize - 1) / kGroupSize;
+// A row's k targets of band t in target groups: `full` groups of 128, then,
+// when groups are shared (split >= 1), the remainder in pieces of shared
+// groups.  The int16 band (t = 1): a remainder of at most 64 goes to a half
+// piece (a group shared by two rows, one per half-wave).  The fp16 band
+// (t = 2): a half piece of up to 64 targets (two rows per group, one per half
+// of the lanes' packed pairs) and, with split = 2, a quarter piece of up to 32
+// (four rows per group), chosen so that fewer than 32 of the row's slots stay
+// empty: remainder r in (96, 128): one more full group; (64, 96]: a full half
+// piece and r - 64 in a quarter; (32, 64]: a half piece; (0, 32]: a quarter.
+struct Pieces {
+  int full, half, quarter;  // full groups; targets in the half / quarter piece
+};
+__device__ __host__ inline Pieces band_pieces(int k, int t, int split) {
+  const int r = k % kGroupSize, f = k / kGroupSize;
+  if (!split || t == 0 || r == 0) return {(k + kGroupSize - 1) / kGroupSize, 0, 0};
+  if (t == 1 || split == 1) return r <= kGroupSize / 2 ? Pieces{f, r, 0} : Pieces{f + 1, 0, 0};
+  if (r > 3 * kGroupSize / 4) return {f + 1, 0, 0};
+  if (r > kGroupSize / 2) return {f, kGroupSize / 2, r - kGroupSize / 2};
+  if (r > kGroupSize / 4) return {f, r, 0};
+  return {f, 0, r};
 }
 
 // One workgroup per row i: its ascending q (= descending target length) in
 // target groups of 128 per length band (band 0: k < kband[2i], band 1: up to
 // kband[2i + 1], band 2: the rest), so each group holds targets of similar
 // length and one lane format; the group's first target is its longest, its
-// last the shortest.  A band-2 (band-1) remainder of at most 64 targets
-// (split set) goes to the half group h >> 1, segment h & 1, with
-// h = hgs[2i] (hgs[2i + 1]): segment s holds its u-th target in slot
-// (u / 32) * 64 + s * 32 + u % 32, the slots its half-wave runs; the group's
-// two rows set its length bounds by atomics (gmax / gmin of half groups start
-// at 0 / INT_MAX).  tlen[5i + band] = sum of the band's target lengths in
-// full groups, tlen[5i + 3] (tlen[5i + 4]) in the band-2 (band-1) half group.
+// last the shortest.  The shared pieces (band_pieces) follow the band's full
+// groups: piece h = hgs[3i + x] (x = 0: fp16 half, 1: int16 half, 2: fp16
+// quarter) is sub-list h & 3 of the shared group h >> 2; its u-th target
+// goes to the slot the kernels read for it: fp16 half s: s * 64 + u;
+// fp16 quarter s: (s & 1) * 64 + (s >> 1) * 32 + u; int16 half s:
+// (u / 32) * 64 + s * 32 + u % 32.  The rows of a shared group set its length
+// bounds by atomics (gmax / gmin of shared groups start at 0 / INT_MAX).
+// tlen[6i + band] = sum of the band's target lengths in full groups,
+// tlen[6i + 3 / 4 / 5] in the fp16 half / int16 half / fp16 quarter piece.
 __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restrict__ sorted,
                                                          const long long *__restrict__ rs,
                                                          const int32_t *__restrict__ gbase,
@@ -843,15 +855,16 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
                                                          const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
                                                          int32_t *__restrict__ gmax, int32_t *__restrict__ gmin,
                                                          unsigned long long *__restrict__ tlen) {
-  __shared__ unsigned long long red[5][256];
+  __shared__ unsigned long long red[6][256];
   const int i = blockIdx.x;
   const long long b = rs[i];
   const int cnt = (int)(rs[i + 1] - b);
   const int e0 = kband[2 * i], e1 = kband[2 * i + 1];
-  const int g0 = band_full_groups(e0, 0, split), g1 = band_full_groups(e1 - e0, 1, split);
+  const Pieces p1 = band_pieces(e1 - e0, 1, split), p2 = band_pieces(cnt - e1, 2, split);
+  const int g0 = band_pieces(e0, 0, split).full, g1 = p1.full;
   // band-1 / band-2 targets in full groups
-  const int full1 = g1 * kGroupSize, full2 = band_full_groups(cnt - e1, 2, split) * kGroupSize;
-  unsigned long long sum[5] = {0, 0, 0, 0, 0};
+  const int full1 = g1 * kGroupSize, full2 = p2.full * kGroupSize;
+  unsigned long long sum[6] = {0, 0, 0, 0, 0, 0};
   for (int k = threadIdx.x; k < cnt; k += 256) {
     const int band = k < e0 ? 0 : k < e1 ? 1 : 2;
     const int from = band == 0 ? 0 : band == 1 ? e0 : e1;
@@ -860,13 +873,29 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
     const int j = order[sorted[b + k]];
     const int lj = len[j];
     const int full = band == 1 ? full1 : full2;
-    if (band != 0 && r >= full) {  // the half group
-      const int u = r - full, h = hgs[2 * i + (band == 1 ? 1 : 0)];
-      const int g = h >> 1, slot = (u / 32) * (kGroupSize / 2) + (h & 1) * 32 + u % 32;
+    if (band != 0 && r >= full) {  // a shared piece
+      int u = r - full, h, slot, x;
+      bool last = k == to - 1;
+      if (band == 1) {
+        x = 1;
+        h = hgs[3 * i + 1];
+        slot = (u / 32) * (kGroupSize / 2) + (h & 3) * 32 + u % 32;
+      } else if (u < p2.half) {
+        x = 0;
+        h = hgs[3 * i];
+        slot = (h & 3) * (kGroupSize / 2) + u;
+        last = u == p2.half - 1;
+      } else {
+        x = 2;
+        u -= p2.half;
+        h = hgs[3 * i + 2];
+        slot = (h & 1) * (kGroupSize / 2) + ((h & 3) >> 1) * (kGroupSize / 4) + u;
+      }
+      const int g = h >> 2;
       gtgt[(size_t)g * kGroupSize + slot] = j;
       if (u == 0) atomicMax(&gmax[g], lj);
-      if (k == to - 1) atomicMin(&gmin[g], lj);
-      sum[band == 1 ? 4 : 3] += (unsigned long long)lj;
+      if (last) atomicMin(&gmin[g], lj);
+      sum[x == 0 ? 3 : x == 1 ? 4 : 5] += (unsigned long long)lj;
       continue;
     }
     const int gfirst = gbase[i] + (band == 0 ? 0 : band == 1 ? g0 : g0 + g1);
@@ -876,14 +905,14 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
     if (slot == kGroupSize - 1 || k == to - 1 || (band != 0 && r == full - 1)) gmin[g] = lj;
     sum[band] += (unsigned long long)lj;
   }
-  for (int t = 0; t < 5; ++t) red[t][threadIdx.x] = sum[t];
+  for (int t = 0; t < 6; ++t) red[t][threadIdx.x] = sum[t];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
-      for (int t = 0; t < 5; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
+      for (int t = 0; t < 6; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x < 5) tlen[5 * i + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 6) tlen[6 * i + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // items of row i, one per full target group: band t's groups at ibase[3i + t] ..
@@ -896,9 +925,50 @@ __global__ void cand_items_kernel(const int32_t *__restrict__ gbase, const int32
   const int e[3] = {kband[2 * i], kband[2 * i + 1] - kband[2 * i], cnt - kband[2 * i + 1]};
   int g = gbase[i];
   for (int t = 0; t < 3; ++t) {
-    const int ng = band_full_groups(e[t], t, split);
+    const int ng = band_pieces(e[t], t, split).full;
     for (int q = 0; q < ng; ++q, ++g) items[ibase[3 * i + t] + q] = make_int4(g, i, i + 1, 0);
   }
+}
+
+// Cells (la * lb) of each verification part, from the rows' target-length
+// sums tlen (row_groups_kernel): full groups of band t go to part
+// 3 (t = 0), 2 (t = 1), or for t = 2 part 0 when the row is at most
+// short_len long, else 1; shared pieces to their group's part gpart[g - Gf].
+// cells[9] must start at zero.
+__global__ __launch_bounds__(256) void part_cells_kernel(const unsigned long long *__restrict__ tlen,
+                                                         const int32_t *__restrict__ len,
+                                                         const int32_t *__restrict__ hgs,
+                                                         const int32_t *__restrict__ gpart, int Gf, int n,
+                                                         int short_len, unsigned long long *__restrict__ cells) {
+  constexpr int kP = 9;
+  __shared__ unsigned long long red[kP];
+  if (threadIdx.x < kP) red[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long acc[kP] = {};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned long long li = (unsigned long long)len[i];
+    acc[3] += li * tlen[6 * (size_t)i];
+    acc[2] += li * tlen[6 * (size_t)i + 1];
+    acc[len[i] <= short_len ? 0 : 1] += li * tlen[6 * (size_t)i + 2];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const int h = hgs[3 * (size_t)i + x];
+      if (h >= 0) {
+        const int part = gpart[(h >> 2) - Gf];
+#pragma unroll
+        for (int p = 4; p < kP; ++p)
+          if (p == part) acc[p] += li * tlen[6 * (size_t)i + 3 + x];
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kP; ++p) {
+    unsigned long long v = acc[p];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&red[p], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < kP && red[threadIdx.x]) atomicAdd(&cells[threadIdx.x], red[threadIdx.x]);
 }
 
 __global__ void iota_kernel(int32_t *a, int n) {
@@ -1411,7 +1481,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr,
           *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr, *hgs = nullptr, *qids2 = nullptr;
-  unsigned long long *tlen = nullptr;
+  unsigned long long *tlen = nullptr, *pcells = nullptr;
+  int32_t *gparts = nullptr;
   long long *d_rs = nullptr;
   int4 *items = nullptr, *sitems = nullptr;
   int2 *d_tiles = nullptr;
@@ -1423,7 +1494,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
     dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_tiles); dev_free(items);
     dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs); dev_free(kband);
-    dev_free(hgs); dev_free(qids2); dev_free(sitems);
+    dev_free(hgs); dev_free(qids2); dev_free(sitems); dev_free(gparts); dev_free(pcells);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(ek0);
@@ -1699,67 +1770,86 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     clk.mark("length bands");
-    // Remainders of at most 64 fp16-band or int16-band targets go to half
-    // groups shared by two rows of the same band (two queries per wave),
-    // paired longest first; MNHN_VERIFY_SPLIT=0 keeps one row per group.
+    // Remainders go to target groups shared by several rows of the same band
+    // (band_pieces), grouped longest first: MNHN_VERIFY_SPLIT=0 keeps one row
+    // per group, 1 shares half groups only, 2 (default) also quarter groups.
     const char *split_env = getenv("MNHN_VERIFY_SPLIT");
-    const int split = split_env && atoi(split_env) == 0 ? 0 : 1;
+    const int split = split_env ? std::min(std::max(atoi(split_env), 0), 2) : 2;
     // ng[3i + t]: full target groups of row i in band t
     std::vector<int32_t> ng(3 * (size_t)n);
     for (int i = 0; i < n; ++i) {
       const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
       const int k3[3] = {e0, e1 - e0, cnt[i] - e1};
       int sum = 0;
-      for (int t = 0; t < 3; ++t) sum += (ng[3 * (size_t)i + t] = band_full_groups(k3[t], t, split));
+      for (int t = 0; t < 3; ++t) sum += (ng[3 * (size_t)i + t] = band_pieces(k3[t], t, split).full);
       gb[i + 1] = gb[i] + sum;
     }
     const int Gf = gb[n];
-    // rows with a band-2 / band-1 half group, longest first: the length order
-    // breaks ties by id, as a stable sort of the ids by length would
-    std::vector<int32_t> hrows2, hrows1;
+    // rows with an fp16 half / int16 half / fp16 quarter piece, longest first:
+    // the length order breaks ties by id, as a stable sort of the ids by
+    // length would
+    std::vector<int32_t> hrows[3];
     for (int p = 0; p < n; ++p) {
       const int i = c->order[p];
       const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
-      if (band_half(cnt[i] - e1, 2, split)) hrows2.push_back(i);
-      if (band_half(e1 - e0, 1, split)) hrows1.push_back(i);
+      const Pieces p2 = band_pieces(cnt[i] - e1, 2, split), p1 = band_pieces(e1 - e0, 1, split);
+      if (p2.half) hrows[0].push_back(i);
+      if (p1.half) hrows[1].push_back(i);
+      if (p2.quarter) hrows[2].push_back(i);
     }
-    const int np2 = (int)((hrows2.size() + 1) / 2), np1 = (int)((hrows1.size() + 1) / 2);
-    const int np = np2 + np1;
-    const int G = Gf + np;
     // item order by class: fp16 groups of rows short enough for a 32-column
-    // strip, the other fp16 groups, the int16 groups, the 32-bit groups; then
-    // the half-group pairs: fp16 pairs with a row longer than 128 (strip 64
-    // when the context's strip is 64) before the other fp16 pairs (strip 32),
-    // then the int16 pairs (rows longer than 512, the context's strip)
+    // strip, the other fp16 groups, the int16 groups, the 32-bit groups
+    // (parts 0-3); then the shared groups, by kind (fp16 half, fp16 quarter,
+    // int16 half) and within a kind longest first: fp16 groups whose longest
+    // row is longer than 128 run at strip 64 when the context's strip is 64
+    // (parts 5, 8), the other fp16 groups at strip 32 (parts 4, 7), the int16
+    // groups (rows longer than 512) at the context's strip (part 6)
     const int S = context_strip(c);
     auto band_class = [&](int i, int t) {
       if (t == 0) return 3;
       if (t == 1) return 2;
       return (S == 64 && c->len[i] <= 128) ? 0 : 1;
     };
-    // hg[2i] / hg[2i + 1]: row i's fp16 / int16 half group (group << 1 | segment)
-    std::vector<int32_t> hg(2 * (size_t)n, -1), q2(2 * (size_t)np);
+    constexpr int kParts = 9;
+    constexpr int kRowsPer[3] = {2, 2, 4};  // rows per shared group of each kind
+    const int kind_order[3] = {0, 2, 1};
+    int nshared[3];
+    for (int x = 0; x < 3; ++x) nshared[x] = (int)((hrows[x].size() + kRowsPer[x] - 1) / kRowsPer[x]);
+    const int np = nshared[0] + nshared[1] + nshared[2];
+    const int G = Gf + np;
+    // hg[3i + x]: row i's piece of kind x (shared group << 2 | sub-list)
+    std::vector<int32_t> hg(3 * (size_t)n, -1), q2, gpart(np);
     std::vector<int4> sit(np);
-    int split_begin[4] = {0, np2, np2, np};  // [5: fp16 strip 64, 4: fp16 strip 32, 6: int16)
-    for (int pi = 0; pi < np; ++pi) {
-      const bool i16 = pi >= np2;
-      const std::vector<int32_t> &hr = i16 ? hrows1 : hrows2;
-      const size_t k = 2 * (size_t)(i16 ? pi - np2 : pi);
-      const int ra = hr[k];
-      const int rb = k + 1 < hr.size() ? hr[k + 1] : ra;
-      hg[2 * (size_t)ra + i16] = (Gf + pi) << 1;
-      if (rb != ra) hg[2 * (size_t)rb + i16] = ((Gf + pi) << 1) | 1;
-      q2[2 * (size_t)pi] = ra;
-      q2[2 * (size_t)pi + 1] = rb;  // a lone row: the second half-wave has no targets
-      sit[pi] = make_int4(Gf + pi, 2 * pi, 2 * pi + 2, 0);
-      if (!i16 && !(S == 64 && c->len[ra] > 128) && split_begin[1] == np2) split_begin[1] = pi;
+    int rng[kParts][2] = {};  // shared parts: their range of sit
+    {
+      int pi = 0;
+      for (int xo = 0; xo < 3; ++xo) {
+        const int x = kind_order[xo];
+        const int per = kRowsPer[x];
+        const std::vector<int32_t> &hr = hrows[x];
+        const int p_long = x == 0 ? 5 : x == 2 ? 8 : 6, p_short = x == 0 ? 4 : x == 2 ? 7 : 6;
+        rng[p_long][0] = rng[p_long][1] = pi;
+        for (int gi = 0; gi < nshared[x]; ++gi, ++pi) {
+          const int qb = (int)q2.size();
+          const size_t k0 = (size_t)gi * per;
+          for (int u = 0; u < per; ++u) {
+            const size_t k = k0 + u;
+            // a short last group repeats its first row, with no targets
+            const int r = k < hr.size() ? hr[k] : hr[k0];
+            if (k < hr.size()) hg[3 * (size_t)r + x] = ((Gf + pi) << 2) | u;
+            q2.push_back(r);
+          }
+          sit[pi] = make_int4(Gf + pi, qb, qb + per, 0);
+          const int part = (x == 1 || (S == 64 && c->len[hr[k0]] > 128)) ? p_long : p_short;
+          gpart[pi] = part;
+          if (part == p_short && p_short != p_long && rng[p_short][1] == 0) rng[p_short][0] = pi;
+          rng[part][1] = pi + 1;
+        }
+        if (rng[p_short][1] == 0) rng[p_short][0] = rng[p_short][1] = pi;
+      }
     }
-    auto split_class = [&](int i, int t) {  // the part of row i's band-t half group
-      if (t == 1) return 6;
-      const int pi = (hg[2 * (size_t)i] >> 1) - Gf;
-      return (S == 64 && c->len[q2[2 * (size_t)pi]] > 128) ? 5 : 4;
-    };
-    // first item of each (row, band): by class, then row, then band
+    // first item of each (row, band): by class, then row, longest row first
+    // (the persistent queue then ends on the cheapest items), then band
     std::vector<int32_t> ib(3 * (size_t)n, 0);
     int part_begin[5] = {0, 0, 0, 0, 0};
     {
@@ -1767,8 +1857,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
         for (int t = 0; t < 3; ++t) part_begin[band_class(i, t) + 1] += ng[3 * (size_t)i + t];
       for (int cls = 0; cls < 4; ++cls) part_begin[cls + 1] += part_begin[cls];
       int off[4] = {part_begin[0], part_begin[1], part_begin[2], part_begin[3]};
-      for (int i = 0; i < n; ++i)
-        for (int t = 0; t < 3; ++t) {
+      for (int p = 0; p < n; ++p)
+        for (int t = 0, i = c->order[p]; t < 3; ++t) {
           const int cls = band_class(i, t);
           ib[3 * (size_t)i + t] = off[cls];
           off[cls] += ng[3 * (size_t)i + t];
@@ -1777,14 +1867,15 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     clk.mark("host group layout");
     if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
         (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
-        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 5 * (size_t)n)) ||
-        (rc = dev_alloc(&hgs, 2 * (size_t)n)) || (rc = dev_alloc(&qids2, std::max<size_t>(2 * (size_t)np, 1))) ||
-        (rc = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))))
+        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 6 * (size_t)n)) ||
+        (rc = dev_alloc(&hgs, 3 * (size_t)n)) || (rc = dev_alloc(&qids2, std::max<size_t>(q2.size(), 1))) ||
+        (rc = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))) ||
+        (rc = dev_alloc(&gparts, std::max<size_t>((size_t)np, 1))) || (rc = dev_alloc(&pcells, (size_t)kParts)))
       return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 2 * (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(qids2, q2.data(), 2 * (size_t)np * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(qids2, q2.data(), q2.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && np > 0) e = hipMemcpyAsync(sitems, sit.data(), (size_t)np * sizeof(int4), hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
@@ -1799,24 +1890,21 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
                        split, items);
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, qids, n);
     if (e == hipSuccess) e = hipGetLastError();
-    std::vector<unsigned long long> h_tlen(5 * (size_t)n);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_tlen.data(), tlen, 5 * (size_t)n * 8, hipMemcpyDeviceToHost, st);
+    // cells of each part (band_class; shared pieces by their group's part)
+    unsigned long long h_pc[kParts] = {};
+    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(gparts, gpart.data(), (size_t)np * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(pcells, 0, kParts * 8, st);
+    hipLaunchKernelGGL(part_cells_kernel, dim3(std::min(blocks_for(n, 256), 1024u)), dim3(256), 0, st, tlen,
+                       c->d_len, hgs, gparts, Gf, n, S == 64 ? 128 : -1, pcells);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h_pc, pcells, kParts * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MNHN_ERR_DEVICE);
     clk.mark("group buffers, row groups");
-    double part_cells[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < n; ++i) {
-      for (int t = 0; t < 3; ++t) {
-        const double ci = (double)c->len[i] * (double)h_tlen[5 * (size_t)i + t];
-        stt.cells += ci;
-        part_cells[band_class(i, t)] += ci;
-      }
-      for (int t = 1; t <= 2; ++t)
-        if (hg[2 * (size_t)i + (t == 1)] >= 0) {
-          const double ci = (double)c->len[i] * (double)h_tlen[5 * (size_t)i + (t == 1 ? 4 : 3)];
-          stt.cells += ci;
-          part_cells[split_class(i, t)] += ci;
-        }
+    double part_cells[kParts] = {};
+    for (int p = 0; p < kParts; ++p) {
+      part_cells[p] = (double)h_pc[p];
+      stt.cells += part_cells[p];
     }
     dev_free(sorted);
     Work w;
@@ -1837,13 +1925,40 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     }
     unsigned long long kcap = std::min(2ull * m, kroom) + est_sub;
     clk.mark("cell counts");
+    if (clk.on) {  // lane fill of each verification part: cells / (128 lanes x rows x columns)
+      std::vector<int32_t> h_gmax((size_t)G);
+      if (hipMemcpy(h_gmax.data(), gmax, (size_t)G * 4, hipMemcpyDeviceToHost) == hipSuccess) {
+        double pad[kParts] = {};
+        long long nit[kParts] = {};
+        for (int i = 0; i < n; ++i) {
+          int g = gb[i];
+          const double cols = (double)((c->len[i] + 7) / 8 * 8);
+          for (int t = 0; t < 3; ++t)
+            for (int q = 0; q < ng[3 * (size_t)i + t]; ++q, ++g) {
+              pad[band_class(i, t)] += (double)kGroupSize * h_gmax[g] * cols;
+              ++nit[band_class(i, t)];
+            }
+        }
+        for (int pi = 0; pi < np; ++pi) {
+          int lmax = 0;
+          for (int qi = sit[pi].y; qi < sit[pi].z; ++qi) lmax = std::max(lmax, c->len[q2[qi]]);
+          const double cols = (double)((lmax + 7) / 8 * 8);
+          pad[gpart[pi]] += (double)kGroupSize * h_gmax[Gf + pi] * cols;
+          ++nit[gpart[pi]];
+        }
+        for (int p = 0; p < kParts; ++p)
+          if (nit[p])
+            fprintf(stderr, "[mnhn] prefilter part %d: %lld items, %.4e cells, lane fill %.3f\n", p, nit[p],
+                    part_cells[p], pad[p] > 0 ? part_cells[p] / pad[p] : 0.0);
+      }
+    }
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
       clk.mark("key buffer");
       if (hipMemsetAsync(c->d_nkeys, 0, 8, st) != hipSuccess) return fail(MNHN_ERR_DEVICE);
       kernel_ms = 0;
       for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
-      for (int part = 0; part < 7 && rc == MNHN_OK; ++part) {
+      for (int part = 0; part < kParts && rc == MNHN_OK; ++part) {
         if (part < 4) {
           w.d_items = items + part_begin[part];
           w.n_items = part_begin[part + 1] - part_begin[part];
@@ -1852,21 +1967,22 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
           w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
           w.dev_f16_ok = part < 2;
           w.dev_strip = part == 0 ? 32 : 0;
-        } else {  // half-group pairs: fp16 part 5 at strip 64, part 4 at strip 32; int16 part 6
-          const int lo = part == 5 ? split_begin[0] : part == 4 ? split_begin[1] : split_begin[2];
-          const int hi = part == 5 ? split_begin[1] : part == 4 ? split_begin[2] : split_begin[3];
-          w.d_items = sitems + lo;
-          w.n_items = hi - lo;
+        } else {  // shared groups: fp16 halves (parts 4, 5), int16 halves (6), fp16 quarters (7, 8)
+          w.d_items = sitems + rng[part][0];
+          w.n_items = rng[part][1] - rng[part][0];
           w.d_qids = qids2;
-          w.dev_split = 2;
+          w.dev_split = part == 6 ? 2 : part <= 5 ? 3 : 4;
           w.dev_kernel = part == 6 ? KERNEL_LDS : KERNEL_LDS_F16;
           w.dev_f16_ok = part != 6;
-          w.dev_strip = part == 5 ? 64 : part == 4 ? 32 : (S == 64 ? 64 : 32);
+          w.dev_strip = (part == 5 || part == 8) ? 64 : (part == 4 || part == 7) ? 32 : (S == 64 ? 64 : 32);
         }
         if (w.n_items == 0) continue;
         w.cells = part_cells[part];
         rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
         kernel_ms += c->last.kernel_ms;
+        if (clk.on)
+          fprintf(stderr, "[mnhn] prefilter part %d: %.3f ms, %.0f GCUPS\n", part, c->last.kernel_ms,
+                  c->last.kernel_ms > 0 ? part_cells[part] / (c->last.kernel_ms * 1e6) : 0.0);
         for (int f = 0; f < 3; ++f) {
           form_cells[f] += c->last.form_cells[f];
           form_ms[f] += c->last.form_ms[f];

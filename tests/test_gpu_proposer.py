@@ -131,19 +131,22 @@ def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, monkeypatch, tu
     assert st["neighbours"] * 2 == len(col)
 
 
-@pytest.mark.parametrize("split", ["1", "0"])
+@pytest.mark.parametrize("split", ["2", "1", "0"])
 @pytest.mark.parametrize("shape", ["lone_half_row", "mixed_counts", "long_rows"])
 def test_prefilter_half_groups_equal_exact_lists(pkg, monkeypatch, split, shape):
-    """Verification with target groups shared by two rows (remainders of at
-    most 64 fp16-band or int16-band targets, two queries per wave) and without
-    them gives the exact lists.  lone_half_row: 4 related sequences of
-    distinct lengths hold 3, 2, 1 and 0 candidates, so the third half row has
-    no partner (its group's second half-wave runs no targets); mixed_counts:
-    rows with 0-200 candidates, i.e. full groups, half groups and both, with
-    query lengths that differ within a pair (the shorter query pre-padded);
-    long_rows: the same over rows longer than 512 residues, whose targets
-    beyond 512 residues run the int16 form (rows near 515 residues hold
-    targets of both forms), with a lone int16 half row among them."""
+    """Verification with target groups shared by several rows and without
+    them gives the exact lists.  split 2: fp16-band remainders in half pieces
+    (two rows per group, one per half of the lanes' packed pairs) and quarter
+    pieces (four rows per group); split 1: half pieces only; both: int16-band
+    remainders of at most 64 in groups shared by two rows, one per half-wave.
+    lone_half_row: 4 related sequences of distinct lengths hold 3, 2, 1 and 0
+    candidates, so the last shared group is short of rows (its missing rows
+    run no targets); mixed_counts: rows with 0-200 candidates, i.e. full
+    groups, half and quarter pieces and their mixes, with query lengths that
+    differ within a group (the shorter queries pre-padded) at both strip
+    widths; long_rows: the same over rows longer than 512 residues, whose
+    targets beyond 512 residues run the int16 form (rows near 515 residues
+    hold targets of both forms), with a lone int16 half row among them."""
     monkeypatch.setenv("MNHN_VERIFY_SPLIT", split)
     monkeypatch.setenv("MNHN_PREFILTER_EXACT_BUCKET", "0")  # every pair through the candidate lists
     rng = np.random.default_rng(7)

@@ -153,14 +153,18 @@ int launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int triangle, int eps_
 int sort_row_segments(hipStream_t st, const long long *d_rp, const int32_t *seg, int32_t *out,
                       const uint32_t *pay_in, uint32_t *pay_out, int n_rows, int n_cols);
 
-// Host allocation for large result arrays (free() releases it).
+// Host allocation for large result arrays, released with host_free: pinned
+// (copy_d2h then copies into it directly) within a budget of pinned bytes
+// outstanding, else 2 MB-aligned pages marked for transparent huge pages.
 void *host_alloc(size_t bytes);
+void host_free(void *p);
 
 // Register-strip width the context's launches use (tuning, environment or default).
 int context_strip(const mnhn_sw_ctx *c);
 
-// Device-to-host copy through two pinned staging buffers: chunk k+1 is in
-// flight on stream st while chunk k is copied to dst (pageable).
+// Device-to-host copy on stream st: straight into a pinned host_alloc
+// buffer, otherwise through two pinned staging buffers (chunk k+1 in flight
+// while chunk k is copied to dst).
 int copy_d2h(hipStream_t st, mnhn_detail_stage &s, void *dst, const void *src, size_t bytes);
 inline int copy_d2h(mnhn_sw_ctx *c, void *dst, const void *src, size_t bytes) {
   return copy_d2h(c->stream, c->stage, dst, src, bytes);

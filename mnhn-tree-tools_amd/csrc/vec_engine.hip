@@ -505,9 +505,9 @@ int mnhn_vec_eps_csr(mnhn_vec_ctx *c, int norm, float thr, mnhn_vcsr *out) {
 
 void mnhn_vcsr_free(mnhn_vcsr *csr) {
   if (!csr) return;
-  free(csr->row_ptr);
-  free(csr->col);
-  free(csr->val);
+  host_free(csr->row_ptr);
+  host_free(csr->col);
+  host_free(csr->val);
   csr->row_ptr = nullptr;
   csr->col = nullptr;
   csr->val = nullptr;

@@ -1478,6 +1478,9 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   int *d_over = nullptr;
   float *norm = nullptr, *srad = nullptr, *cmin = nullptr, *pnorm = nullptr;
   unsigned long long *cand = nullptr, *d_cnt64 = nullptr;
+  // the candidate buffer, kept after the candidates are grouped to become
+  // the verification's key buffer (spare_cap keys)
+  unsigned long long *spare = nullptr, spare_cap = 0;
   int32_t *rowcnt = nullptr, *gbase = nullptr, *cursor = nullptr, *gtgt = nullptr, *gmax = nullptr,
           *gmin = nullptr, *qids = nullptr, *d_order = nullptr, *lens = nullptr,
           *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr, *hgs = nullptr, *qids2 = nullptr;
@@ -1487,7 +1490,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
   int4 *items = nullptr, *sitems = nullptr;
   int2 *d_tiles = nullptr;
   auto cleanup = [&]() {
-    dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand);
+    dev_free(X); dev_free(E16); dev_free(Es); dev_free(norm); dev_free(srad); dev_free(cand); dev_free(spare);
     if (Xs) (void)hipFree(Xs);
     Xs = nullptr;
     dev_free(d_over); dev_free(cmin); dev_free(pnorm);
@@ -1752,8 +1755,22 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     hipLaunchKernelGGL(cand_scatter_kernel, dim3(stride_blocks((long long)m, 256)), dim3(256), 0, st, cand,
                        (long long)m, d_order, d_rs, cursor, seg);
     if (hipGetLastError() != hipSuccess) return fail(MNHN_ERR_DEVICE);
-    dev_free(cand);
-    if ((rc = dev_alloc(&sorted, (size_t)m))) return fail(rc);
+    // The candidate buffer is kept as the verification's key buffer (below)
+    // when it is larger than the context's: on the 500,000-sequence set that
+    // saves freeing ~100 GB and allocating ~100 GB again (150 ms).  Should
+    // the next allocations not fit beside it, it is freed first.
+    if (cap > c->keys_cap) {
+      spare = cand;
+      spare_cap = cap;
+      cand = nullptr;
+    } else {
+      dev_free(cand);
+    }
+    if ((rc = dev_alloc(&sorted, (size_t)m)) && spare) {
+      dev_free(spare);
+      rc = dev_alloc(&sorted, (size_t)m);
+    }
+    if (rc) return fail(rc);
     if ((rc = sort_row_segments(st, d_rs, seg, sorted, nullptr, nullptr, n, n))) return fail(rc);
     dev_free(seg);
     clk.mark("scatter, row sort");
@@ -1865,13 +1882,25 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
         }
     }
     clk.mark("host group layout");
-    if ((rc = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (rc = dev_alloc(&gmax, (size_t)G)) ||
-        (rc = dev_alloc(&gmin, (size_t)G)) || (rc = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
-        (rc = dev_alloc(&ibase, 3 * (size_t)n)) || (rc = dev_alloc(&tlen, 6 * (size_t)n)) ||
-        (rc = dev_alloc(&hgs, 3 * (size_t)n)) || (rc = dev_alloc(&qids2, std::max<size_t>(q2.size(), 1))) ||
-        (rc = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))) ||
-        (rc = dev_alloc(&gparts, std::max<size_t>((size_t)np, 1))) || (rc = dev_alloc(&pcells, (size_t)kParts)))
-      return fail(rc);
+    auto group_buffers = [&]() -> int {
+      int r;
+      if ((r = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (r = dev_alloc(&gmax, (size_t)G)) ||
+          (r = dev_alloc(&gmin, (size_t)G)) || (r = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
+          (r = dev_alloc(&ibase, 3 * (size_t)n)) || (r = dev_alloc(&tlen, 6 * (size_t)n)) ||
+          (r = dev_alloc(&hgs, 3 * (size_t)n)) || (r = dev_alloc(&qids2, std::max<size_t>(q2.size(), 1))) ||
+          (r = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))) ||
+          (r = dev_alloc(&gparts, std::max<size_t>((size_t)np, 1))) || (r = dev_alloc(&pcells, (size_t)kParts))) {
+        dev_free(gtgt); dev_free(gmax); dev_free(gmin); dev_free(items); dev_free(ibase); dev_free(tlen);
+        dev_free(hgs); dev_free(qids2); dev_free(sitems); dev_free(gparts); dev_free(pcells);
+        return r;
+      }
+      return MNHN_OK;
+    };
+    if ((rc = group_buffers()) && spare) {
+      dev_free(spare);
+      rc = group_buffers();
+    }
+    if (rc) return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
@@ -1951,7 +1980,36 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
             fprintf(stderr, "[mnhn] prefilter part %d: %lld items, %.4e cells, lane fill %.3f\n", p, nit[p],
                     part_cells[p], pad[p] > 0 ? part_cells[p] / pad[p] : 0.0);
       }
+      clk.mark("lane fill statistics");
     }
+    // the work of one verification part (its items, kernel form, strip and queries per group)
+    auto part_work = [&](int part) {
+      if (part < 4) {
+        w.d_items = items + part_begin[part];
+        w.n_items = part_begin[part + 1] - part_begin[part];
+        w.d_qids = qids;
+        w.dev_split = 1;
+        w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
+        w.dev_f16_ok = part < 2;
+        w.dev_strip = part == 0 ? 32 : 0;
+      } else {  // shared groups: fp16 halves (parts 4, 5), int16 halves (6), fp16 quarters (7, 8)
+        w.d_items = sitems + rng[part][0];
+        w.n_items = rng[part][1] - rng[part][0];
+        w.d_qids = qids2;
+        w.dev_split = part == 6 ? 2 : part <= 5 ? 3 : 4;
+        w.dev_kernel = part == 6 ? KERNEL_LDS : KERNEL_LDS_F16;
+        w.dev_f16_ok = part != 6;
+        w.dev_strip = (part == 5 || part == 8) ? 64 : (part == 4 || part == 7) ? 32 : (S == 64 ? 64 : 32);
+      }
+      w.cells = part_cells[part];
+    };
+    if (spare && spare_cap >= kcap && spare_cap > c->keys_cap) {  // the candidate buffer takes the keys
+      dev_free(c->d_keys);
+      c->d_keys = spare;
+      c->keys_cap = spare_cap;
+      spare = nullptr;
+    }
+    dev_free(spare);
     for (int attempt = 0; attempt < 2; ++attempt) {
       if ((rc = ensure_keys(c, kcap))) return fail(rc);
       clk.mark("key buffer");
@@ -1959,25 +2017,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       kernel_ms = 0;
       for (int f = 0; f < 3; ++f) form_cells[f] = form_ms[f] = 0;
       for (int part = 0; part < kParts && rc == MNHN_OK; ++part) {
-        if (part < 4) {
-          w.d_items = items + part_begin[part];
-          w.n_items = part_begin[part + 1] - part_begin[part];
-          w.d_qids = qids;
-          w.dev_split = 1;
-          w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
-          w.dev_f16_ok = part < 2;
-          w.dev_strip = part == 0 ? 32 : 0;
-        } else {  // shared groups: fp16 halves (parts 4, 5), int16 halves (6), fp16 quarters (7, 8)
-          w.d_items = sitems + rng[part][0];
-          w.n_items = rng[part][1] - rng[part][0];
-          w.d_qids = qids2;
-          w.dev_split = part == 6 ? 2 : part <= 5 ? 3 : 4;
-          w.dev_kernel = part == 6 ? KERNEL_LDS : KERNEL_LDS_F16;
-          w.dev_f16_ok = part != 6;
-          w.dev_strip = (part == 5 || part == 8) ? 64 : (part == 4 || part == 7) ? 32 : (S == 64 ? 64 : 32);
-        }
+        part_work(part);
         if (w.n_items == 0) continue;
-        w.cells = part_cells[part];
         rc = launch_tiles(c, w, MODE_EPS, 0, eps_max, nullptr, c->d_keys, c->keys_cap);
         kernel_ms += c->last.kernel_ms;
         if (clk.on)

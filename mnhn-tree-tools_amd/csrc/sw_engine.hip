@@ -1614,31 +1614,44 @@ double items_per_cu() {
   return v > 0 ? v : 4096.0;
 }
 
-// Split the query ranges [0, nq_g) of each group into items of similar cost.
-void add_group_items(const mnhn_sw_ctx *c, Work &w, int g, int q_begin, int q_end,
+// Query costs of a work list as prefix sums over its qids: P[k] = sum over
+// u < k of max(len[qids[u]] - 1, 0), so that the cost of the contiguous query
+// range [qb, qe) of a target group with `strips` strips of S columns is
+// strips * S * (P[qe] - P[qb]) + 64 * (qe - qb) (DP columns, plus a fixed
+// share per query) in O(1), instead of a walk over the range.
+std::vector<double> query_prefix(const mnhn_sw_ctx *c, const Work &w) {
+  std::vector<double> P(w.qids.size() + 1, 0.0);
+  for (size_t u = 0; u < w.qids.size(); ++u) P[u + 1] = P[u] + (double)std::max(c->len[w.qids[u]] - 1, 0);
+  return P;
+}
+inline double range_cost(const std::vector<double> &P, int qb, int qe, int strips, int S) {
+  return (double)strips * S * (P[qe] - P[qb]) + 64.0 * (qe - qb);
+}
+
+// Split the query range [q_begin, q_end) of group g into items of similar
+// cost: each item ends at the first query where its cost reaches
+// target_cost (binary search on the prefix sums), the last at q_end.
+void add_group_items(const mnhn_sw_ctx *c, Work &w, const std::vector<double> &P, int g, int q_begin, int q_end,
                      double target_cost, int S) {
   const int gm = c->gmax[g];
   const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
   int qb = q_begin;
-  double acc = 0;
-  for (int qi = q_begin; qi < q_end; ++qi) {
-    const int la = c->len[w.qids[qi]];
-    acc += (double)std::max(la - 1, 0) * strips * S + 64.0;
-    if (acc >= target_cost || qi + 1 == q_end) {
-      w.items.push_back(make_int4(g, qb, qi + 1, 0));
-      qb = qi + 1;
-      acc = 0;
+  while (qb < q_end) {
+    int lo = qb, hi = q_end - 1;  // first qi with cost(qb .. qi) >= target, else q_end - 1
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if (range_cost(P, qb, mid + 1, strips, S) >= target_cost) hi = mid;
+      else lo = mid + 1;
     }
+    w.items.push_back(make_int4(g, qb, lo + 1, 0));
+    qb = lo + 1;
   }
 }
 
-double item_cost(const mnhn_sw_ctx *c, const Work &w, const int4 &it, int S) {
+double item_cost(const mnhn_sw_ctx *c, const Work &w, const std::vector<double> &P, const int4 &it, int S) {
   const int gm = w.gmax.empty() ? c->gmax[it.x] : w.gmax[it.x];
   const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
-  double acc = 0;
-  for (int qi = it.y; qi < it.z; ++qi)
-    acc += (double)std::max(c->len[w.qids[qi]] - 1, 0) * strips * S + 64.0;
-  return acc;
+  return range_cost(P, it.y, it.z, strips, S);
 }
 
 // Items longest first, so the queue's last items are its smallest: a
@@ -1650,8 +1663,9 @@ void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
   const size_t m = w.items.size();
   std::vector<uint16_t> cls(m);
   std::vector<size_t> start(kClasses + 1, 0);
+  const std::vector<double> P = query_prefix(c, w);
   for (size_t k = 0; k < m; ++k) {
-    const double cost = item_cost(c, w, w.items[k], S);
+    const double cost = item_cost(c, w, P, w.items[k], S);
     int q = cost > 1.0 ? (int)(16.0 * std::log2(cost)) : 0;
     q = std::min(std::max(q, 0), kClasses - 1);
     cls[k] = (uint16_t)(kClasses - 1 - q);  // larger cost, smaller class index
@@ -1699,18 +1713,19 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps, int p0 = 0) {
       first[g] = std::max(lo, p0);
     }
   }
+  const std::vector<double> P = query_prefix(c, w);  // qids = positions in length order
   double total = 0;
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     const int gm = c->gmax[g];
     const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
-    for (int p = first[g]; p < last; ++p) total += (double)std::max(c->len[c->order[p]] - 1, 0) * strips * S;
+    if (last > first[g]) total += (double)strips * S * (P[last] - P[first[g]]);
   }
   const double target = std::max(total / ((double)std::max(c->num_cu, 1) * items_per_cu()), 2.0e5);
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     if (last <= first[g]) continue;
-    add_group_items(c, w, g, first[g], last, target, S);
+    add_group_items(c, w, P, g, first[g], last, target, S);
   }
   // algorithmic cells and pairs of the unique pairs the items cover
   std::vector<double> prefix((size_t)c->n + 1, 0.0);
@@ -2275,8 +2290,9 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_req, unsigned long long *nkeys_out) {
       for (size_t k = 0; k < w.items.size(); k += stride) smp.items.push_back(w.items[k]);
       smp.max_q_len = w.max_q_len;
       double smp_cost = 0, all_cost = 0;
+      const std::vector<double> P = query_prefix(c, w);
       for (size_t k = 0; k < w.items.size(); ++k) {
-        const double ck = item_cost(c, w, w.items[k], S);
+        const double ck = item_cost(c, w, P, w.items[k], S);
         all_cost += ck;
         if (k % stride == 0) smp_cost += ck;
       }

@@ -133,11 +133,10 @@ struct Work {
   // match * min(la, lb) <= 2048 (otherwise the int16 LDS kernel runs it)
   bool dev_f16_ok = false;
   int dev_strip = 0;  // strip width for device work (0: the context's)
-  // target groups shared by several rows (epsilon mode; the item's qids range
-  // holds its rows): 2 = two rows, one per half-wave (KERNEL_LDS or
-  // KERNEL_LDS_F16); 3 = two rows, one per half of the lanes' packed pairs
-  // (slots 0-63 and 64-127), and 4 = four rows, (half-wave, half) (slots
-  // 0-31, 64-95, 32-63, 96-127 in qids order), KERNEL_LDS_F16 only
+  // target groups shared by several rows (epsilon mode, KERNEL_LDS or
+  // KERNEL_LDS_F16; the item's qids range holds its rows): 3 = two rows, one
+  // per half of the lanes' packed pairs (slots 0-63 and 64-127), 4 = four
+  // rows, (half-wave, half) (slots 0-31, 64-95, 32-63, 96-127 in qids order)
   int dev_split = 1;
 };
 

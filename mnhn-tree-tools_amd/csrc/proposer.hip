@@ -812,21 +812,21 @@ __global__ void band_counts_kernel(const int32_t *__restrict__ sorted, const lon
 }
 
 // A row's k targets of band t in target groups: `full` groups of 128, then,
-// when groups are shared (split >= 1), the remainder in pieces of shared
-// groups.  The int16 band (t = 1): a remainder of at most 64 goes to a half
-// piece (a group shared by two rows, one per half-wave).  The fp16 band
-// (t = 2): a half piece of up to 64 targets (two rows per group, one per half
-// of the lanes' packed pairs) and, with split = 2, a quarter piece of up to 32
-// (four rows per group), chosen so that fewer than 32 of the row's slots stay
-// empty: remainder r in (96, 128): one more full group; (64, 96]: a full half
-// piece and r - 64 in a quarter; (32, 64]: a half piece; (0, 32]: a quarter.
+// when groups are shared (split >= 1) and the band is the int16 (t = 1) or
+// the fp16 one (t = 2), the remainder in pieces of shared groups: a half
+// piece of up to 64 targets (two rows per group, one per half of the lanes'
+// packed pairs) and, with split = 2, a quarter piece of up to 32 (four rows
+// per group), chosen so that fewer than 32 of the row's slots stay empty:
+// remainder r in (96, 128): one more full group; (64, 96]: a full half piece
+// and r - 64 in a quarter; (32, 64]: a half piece; (0, 32]: a quarter.
+// split = 1: half pieces only (r <= 64).
 struct Pieces {
   int full, half, quarter;  // full groups; targets in the half / quarter piece
 };
 __device__ __host__ inline Pieces band_pieces(int k, int t, int split) {
   const int r = k % kGroupSize, f = k / kGroupSize;
   if (!split || t == 0 || r == 0) return {(k + kGroupSize - 1) / kGroupSize, 0, 0};
-  if (t == 1 || split == 1) return r <= kGroupSize / 2 ? Pieces{f, r, 0} : Pieces{f + 1, 0, 0};
+  if (split == 1) return r <= kGroupSize / 2 ? Pieces{f, r, 0} : Pieces{f + 1, 0, 0};
   if (r > 3 * kGroupSize / 4) return {f + 1, 0, 0};
   if (r > kGroupSize / 2) return {f, kGroupSize / 2, r - kGroupSize / 2};
   if (r > kGroupSize / 4) return {f, r, 0};
@@ -838,14 +838,13 @@ __device__ __host__ inline Pieces band_pieces(int k, int t, int split) {
 // kband[2i + 1], band 2: the rest), so each group holds targets of similar
 // length and one lane format; the group's first target is its longest, its
 // last the shortest.  The shared pieces (band_pieces) follow the band's full
-// groups: piece h = hgs[3i + x] (x = 0: fp16 half, 1: int16 half, 2: fp16
-// quarter) is sub-list h & 3 of the shared group h >> 2; its u-th target
-// goes to the slot the kernels read for it: fp16 half s: s * 64 + u;
-// fp16 quarter s: (s & 1) * 64 + (s >> 1) * 32 + u; int16 half s:
-// (u / 32) * 64 + s * 32 + u % 32.  The rows of a shared group set its length
-// bounds by atomics (gmax / gmin of shared groups start at 0 / INT_MAX).
-// tlen[6i + band] = sum of the band's target lengths in full groups,
-// tlen[6i + 3 / 4 / 5] in the fp16 half / int16 half / fp16 quarter piece.
+// groups: piece h = hgs[4i + x] (x = 0: fp16 half, 1: int16 half, 2: fp16
+// quarter, 3: int16 quarter) is sub-list h & 3 of the shared group h >> 2;
+// its u-th target goes to the slot the kernels read for it: half s:
+// s * 64 + u; quarter s: (s & 1) * 64 + (s >> 1) * 32 + u.  The rows of a
+// shared group set its length bounds by atomics (gmax / gmin of shared
+// groups start at 0 / INT_MAX).  tlen[7i + band] = sum of the band's target
+// lengths in full groups, tlen[7i + 3 + x] in its piece of kind x.
 __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restrict__ sorted,
                                                          const long long *__restrict__ rs,
                                                          const int32_t *__restrict__ gbase,
@@ -855,7 +854,7 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
                                                          const int32_t *__restrict__ len, int32_t *__restrict__ gtgt,
                                                          int32_t *__restrict__ gmax, int32_t *__restrict__ gmin,
                                                          unsigned long long *__restrict__ tlen) {
-  __shared__ unsigned long long red[6][256];
+  __shared__ unsigned long long red[7][256];
   const int i = blockIdx.x;
   const long long b = rs[i];
   const int cnt = (int)(rs[i + 1] - b);
@@ -864,7 +863,7 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
   const int g0 = band_pieces(e0, 0, split).full, g1 = p1.full;
   // band-1 / band-2 targets in full groups
   const int full1 = g1 * kGroupSize, full2 = p2.full * kGroupSize;
-  unsigned long long sum[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long sum[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int k = threadIdx.x; k < cnt; k += 256) {
     const int band = k < e0 ? 0 : k < e1 ? 1 : 2;
     const int from = band == 0 ? 0 : band == 1 ? e0 : e1;
@@ -874,28 +873,25 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
     const int lj = len[j];
     const int full = band == 1 ? full1 : full2;
     if (band != 0 && r >= full) {  // a shared piece
+      const Pieces &pp = band == 1 ? p1 : p2;
       int u = r - full, h, slot, x;
       bool last = k == to - 1;
-      if (band == 1) {
-        x = 1;
-        h = hgs[3 * i + 1];
-        slot = (u / 32) * (kGroupSize / 2) + (h & 3) * 32 + u % 32;
-      } else if (u < p2.half) {
-        x = 0;
-        h = hgs[3 * i];
+      if (u < pp.half) {
+        x = band == 1 ? 1 : 0;
+        h = hgs[4 * i + x];
         slot = (h & 3) * (kGroupSize / 2) + u;
-        last = u == p2.half - 1;
+        last = u == pp.half - 1;
       } else {
-        x = 2;
-        u -= p2.half;
-        h = hgs[3 * i + 2];
+        x = band == 1 ? 3 : 2;
+        u -= pp.half;
+        h = hgs[4 * i + x];
         slot = (h & 1) * (kGroupSize / 2) + ((h & 3) >> 1) * (kGroupSize / 4) + u;
       }
       const int g = h >> 2;
       gtgt[(size_t)g * kGroupSize + slot] = j;
       if (u == 0) atomicMax(&gmax[g], lj);
       if (last) atomicMin(&gmin[g], lj);
-      sum[x == 0 ? 3 : x == 1 ? 4 : 5] += (unsigned long long)lj;
+      sum[3 + x] += (unsigned long long)lj;
       continue;
     }
     const int gfirst = gbase[i] + (band == 0 ? 0 : band == 1 ? g0 : g0 + g1);
@@ -905,14 +901,14 @@ __global__ __launch_bounds__(256) void row_groups_kernel(const int32_t *__restri
     if (slot == kGroupSize - 1 || k == to - 1 || (band != 0 && r == full - 1)) gmin[g] = lj;
     sum[band] += (unsigned long long)lj;
   }
-  for (int t = 0; t < 6; ++t) red[t][threadIdx.x] = sum[t];
+  for (int t = 0; t < 7; ++t) red[t][threadIdx.x] = sum[t];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
-      for (int t = 0; t < 6; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
+      for (int t = 0; t < 7; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x < 6) tlen[6 * i + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 7) tlen[7 * i + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // items of row i, one per full target group: band t's groups at ibase[3i + t] ..
@@ -934,30 +930,30 @@ __global__ void cand_items_kernel(const int32_t *__restrict__ gbase, const int32
 // sums tlen (row_groups_kernel): full groups of band t go to part
 // 3 (t = 0), 2 (t = 1), or for t = 2 part 0 when the row is at most
 // short_len long, else 1; shared pieces to their group's part gpart[g - Gf].
-// cells[9] must start at zero.
+// cells[10] must start at zero.
 __global__ __launch_bounds__(256) void part_cells_kernel(const unsigned long long *__restrict__ tlen,
                                                          const int32_t *__restrict__ len,
                                                          const int32_t *__restrict__ hgs,
                                                          const int32_t *__restrict__ gpart, int Gf, int n,
                                                          int short_len, unsigned long long *__restrict__ cells) {
-  constexpr int kP = 9;
+  constexpr int kP = 10;
   __shared__ unsigned long long red[kP];
   if (threadIdx.x < kP) red[threadIdx.x] = 0;
   __syncthreads();
   unsigned long long acc[kP] = {};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const unsigned long long li = (unsigned long long)len[i];
-    acc[3] += li * tlen[6 * (size_t)i];
-    acc[2] += li * tlen[6 * (size_t)i + 1];
-    acc[len[i] <= short_len ? 0 : 1] += li * tlen[6 * (size_t)i + 2];
+    acc[3] += li * tlen[7 * (size_t)i];
+    acc[2] += li * tlen[7 * (size_t)i + 1];
+    acc[len[i] <= short_len ? 0 : 1] += li * tlen[7 * (size_t)i + 2];
 #pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      const int h = hgs[3 * (size_t)i + x];
+    for (int x = 0; x < 4; ++x) {
+      const int h = hgs[4 * (size_t)i + x];
       if (h >= 0) {
         const int part = gpart[(h >> 2) - Gf];
 #pragma unroll
         for (int p = 4; p < kP; ++p)
-          if (p == part) acc[p] += li * tlen[6 * (size_t)i + 3 + x];
+          if (p == part) acc[p] += li * tlen[7 * (size_t)i + 3 + x];
       }
     }
   }
@@ -1802,10 +1798,10 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       gb[i + 1] = gb[i] + sum;
     }
     const int Gf = gb[n];
-    // rows with an fp16 half / int16 half / fp16 quarter piece, longest first:
-    // the length order breaks ties by id, as a stable sort of the ids by
-    // length would
-    std::vector<int32_t> hrows[3];
+    // rows with an fp16 half / int16 half / fp16 quarter / int16 quarter
+    // piece, longest first: the length order breaks ties by id, as a stable
+    // sort of the ids by length would
+    std::vector<int32_t> hrows[4];
     for (int p = 0; p < n; ++p) {
       const int i = c->order[p];
       const int e0 = kb[2 * (size_t)i], e1 = kb[2 * (size_t)i + 1];
@@ -1813,38 +1809,41 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       if (p2.half) hrows[0].push_back(i);
       if (p1.half) hrows[1].push_back(i);
       if (p2.quarter) hrows[2].push_back(i);
+      if (p1.quarter) hrows[3].push_back(i);
     }
     // item order by class: fp16 groups of rows short enough for a 32-column
     // strip, the other fp16 groups, the int16 groups, the 32-bit groups
     // (parts 0-3); then the shared groups, by kind (fp16 half, fp16 quarter,
-    // int16 half) and within a kind longest first: fp16 groups whose longest
-    // row is longer than 128 run at strip 64 when the context's strip is 64
-    // (parts 5, 8), the other fp16 groups at strip 32 (parts 4, 7), the int16
-    // groups (rows longer than 512) at the context's strip (part 6)
+    // int16 half, int16 quarter) and within a kind longest first: fp16 groups
+    // whose longest row is longer than 128 run at strip 64 when the context's
+    // strip is 64 (parts 5, 8), the other fp16 groups at strip 32 (parts 4,
+    // 7), the int16 groups (rows longer than 512) at the context's strip
+    // (parts 6, 9)
     const int S = context_strip(c);
     auto band_class = [&](int i, int t) {
       if (t == 0) return 3;
       if (t == 1) return 2;
       return (S == 64 && c->len[i] <= 128) ? 0 : 1;
     };
-    constexpr int kParts = 9;
-    constexpr int kRowsPer[3] = {2, 2, 4};  // rows per shared group of each kind
-    const int kind_order[3] = {0, 2, 1};
-    int nshared[3];
-    for (int x = 0; x < 3; ++x) nshared[x] = (int)((hrows[x].size() + kRowsPer[x] - 1) / kRowsPer[x]);
-    const int np = nshared[0] + nshared[1] + nshared[2];
+    constexpr int kParts = 10;
+    constexpr int kRowsPer[4] = {2, 2, 4, 4};  // rows per shared group of each kind
+    const int kind_order[4] = {0, 2, 1, 3};
+    int nshared[4];
+    for (int x = 0; x < 4; ++x) nshared[x] = (int)((hrows[x].size() + kRowsPer[x] - 1) / kRowsPer[x]);
+    const int np = nshared[0] + nshared[1] + nshared[2] + nshared[3];
     const int G = Gf + np;
-    // hg[3i + x]: row i's piece of kind x (shared group << 2 | sub-list)
-    std::vector<int32_t> hg(3 * (size_t)n, -1), q2, gpart(np);
+    // hg[4i + x]: row i's piece of kind x (shared group << 2 | sub-list)
+    std::vector<int32_t> hg(4 * (size_t)n, -1), q2, gpart(np);
     std::vector<int4> sit(np);
     int rng[kParts][2] = {};  // shared parts: their range of sit
     {
       int pi = 0;
-      for (int xo = 0; xo < 3; ++xo) {
+      for (int xo = 0; xo < 4; ++xo) {
         const int x = kind_order[xo];
         const int per = kRowsPer[x];
         const std::vector<int32_t> &hr = hrows[x];
-        const int p_long = x == 0 ? 5 : x == 2 ? 8 : 6, p_short = x == 0 ? 4 : x == 2 ? 7 : 6;
+        const bool i16 = x == 1 || x == 3;
+        const int p_long = x == 0 ? 5 : x == 2 ? 8 : x == 1 ? 6 : 9, p_short = x == 0 ? 4 : x == 2 ? 7 : p_long;
         rng[p_long][0] = rng[p_long][1] = pi;
         for (int gi = 0; gi < nshared[x]; ++gi, ++pi) {
           const int qb = (int)q2.size();
@@ -1853,11 +1852,11 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
             const size_t k = k0 + u;
             // a short last group repeats its first row, with no targets
             const int r = k < hr.size() ? hr[k] : hr[k0];
-            if (k < hr.size()) hg[3 * (size_t)r + x] = ((Gf + pi) << 2) | u;
+            if (k < hr.size()) hg[4 * (size_t)r + x] = ((Gf + pi) << 2) | u;
             q2.push_back(r);
           }
           sit[pi] = make_int4(Gf + pi, qb, qb + per, 0);
-          const int part = (x == 1 || (S == 64 && c->len[hr[k0]] > 128)) ? p_long : p_short;
+          const int part = (i16 || (S == 64 && c->len[hr[k0]] > 128)) ? p_long : p_short;
           gpart[pi] = part;
           if (part == p_short && p_short != p_long && rng[p_short][1] == 0) rng[p_short][0] = pi;
           rng[part][1] = pi + 1;
@@ -1886,8 +1885,8 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       int r;
       if ((r = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (r = dev_alloc(&gmax, (size_t)G)) ||
           (r = dev_alloc(&gmin, (size_t)G)) || (r = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
-          (r = dev_alloc(&ibase, 3 * (size_t)n)) || (r = dev_alloc(&tlen, 6 * (size_t)n)) ||
-          (r = dev_alloc(&hgs, 3 * (size_t)n)) || (r = dev_alloc(&qids2, std::max<size_t>(q2.size(), 1))) ||
+          (r = dev_alloc(&ibase, 3 * (size_t)n)) || (r = dev_alloc(&tlen, 7 * (size_t)n)) ||
+          (r = dev_alloc(&hgs, 4 * (size_t)n)) || (r = dev_alloc(&qids2, std::max<size_t>(q2.size(), 1))) ||
           (r = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))) ||
           (r = dev_alloc(&gparts, std::max<size_t>((size_t)np, 1))) || (r = dev_alloc(&pcells, (size_t)kParts))) {
         dev_free(gtgt); dev_free(gmax); dev_free(gmin); dev_free(items); dev_free(ibase); dev_free(tlen);
@@ -1903,7 +1902,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (rc) return fail(rc);
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 4 * (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && np > 0) e = hipMemcpyAsync(qids2, q2.data(), q2.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && np > 0) e = hipMemcpyAsync(sitems, sit.data(), (size_t)np * sizeof(int4), hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
@@ -1992,13 +1991,14 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
         w.dev_kernel = part < 2 ? KERNEL_LDS_F16 : part == 2 ? KERNEL_LDS : KERNEL_I32;
         w.dev_f16_ok = part < 2;
         w.dev_strip = part == 0 ? 32 : 0;
-      } else {  // shared groups: fp16 halves (parts 4, 5), int16 halves (6), fp16 quarters (7, 8)
+      } else {  // shared groups: halves (parts 4, 5 fp16, 6 int16), quarters (7, 8 fp16, 9 int16)
+        const bool i16 = part == 6 || part == 9;
         w.d_items = sitems + rng[part][0];
         w.n_items = rng[part][1] - rng[part][0];
         w.d_qids = qids2;
-        w.dev_split = part == 6 ? 2 : part <= 5 ? 3 : 4;
-        w.dev_kernel = part == 6 ? KERNEL_LDS : KERNEL_LDS_F16;
-        w.dev_f16_ok = part != 6;
+        w.dev_split = part <= 6 ? 3 : 4;
+        w.dev_kernel = i16 ? KERNEL_LDS : KERNEL_LDS_F16;
+        w.dev_f16_ok = !i16;
         w.dev_strip = (part == 5 || part == 8) ? 64 : (part == 4 || part == 7) ? 32 : (S == 64 ? 64 : 32);
       }
       w.cells = part_cells[part];

@@ -343,10 +343,11 @@ __device__ __forceinline__ void finish_query_int(const KArgs &a, int lane, int q
 template <bool kG>
 __device__ __forceinline__ void finish_query(const KArgs &a, int lane, int q, int qrel, int ob,
                                              int la, int t_lo, int t_hi, int lb_lo, int lb_hi,
-                                             int rk_lo, int rk_hi, s16x2 best, s16x2 corner) {
+                                             int rk_lo, int rk_hi, s16x2 best, s16x2 corner,
+                                             int q_hi, int la_hi) {
   finish_query_int<kG>(a, lane, q, qrel, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
                    (int)best.x + kBias, (int)best.y + kBias, (int)corner.x + kBias,
-                   (int)corner.y + kBias, q, la);
+                   (int)corner.y + kBias, q_hi, la_hi);
 }
 
 template <int S, bool kAcgt, bool kG = false>
@@ -496,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void sw_tiles_kernel(KArgs a) {
       }
 
       finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
-                   best, corner);
+                   best, corner, q, la);
     }
   }
 }
@@ -680,9 +681,13 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
 // shared by two rows, with the slot layout and left pre-padding of the fp16
 // kernel's kQ = 2 form below (lanes 0-31: query qids[qi] against slots 0-31
 // and 64-95; lanes 32-63: qids[qi + 1] against slots 32-63 and 96-127).
-template <int S, bool kG = false, int kQ = 1>
+// kQ / kLH: several queries per wave, as in sw_lds_f16_kernel (below).
+template <int S, bool kG = false, int kQ = 1, bool kLH = false>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_kernel(KArgs a) {
-  static_assert(kQ == 1 || (kQ == 2 && !kG), "two queries per wave: item lists only");
+  static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
+  static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
+  constexpr int kPer = kLH ? 2 : 1;  // queries per table
+  constexpr int NQ = kQ * kPer;      // queries per wave
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
   constexpr int kTab = kPairs * RS;  // words of one query's table
@@ -695,7 +700,7 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
   uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
   const WaveRows ws(bnd, a.bound_stride, lane);
   static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
-  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's query
+  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's table
   const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab + (uint32_t)seg * (kTab * 4u);
 
   const s16x2 floor2 = {kFloor, kFloor};
@@ -729,17 +734,25 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
     const int nrows = gm > 1 ? gm - 1 : 0;     // DP rows = inner residues of the longest target
     build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
 
-    for (int qi = qb; qi < qe; qi += kQ) {
-      const int q0 = __builtin_amdgcn_readfirstlane(load_qid<kG>(a, qi));
-      const int q1 = kQ == 2 ? __builtin_amdgcn_readfirstlane(a.qids[qi + 1]) : q0;
-      const int la0 = __builtin_amdgcn_readfirstlane(a.len[q0]);
-      const int la1 = kQ == 2 ? __builtin_amdgcn_readfirstlane(a.len[q1]) : la0;
-      const uint32_t qwo0 = __builtin_amdgcn_readfirstlane(a.woff[q0]);
-      const uint32_t qwo1 = kQ == 2 ? __builtin_amdgcn_readfirstlane(a.woff[q1]) : qwo0;
-      const int nc0 = la0 > 1 ? la0 - 1 : 0, nc1 = la1 > 1 ? la1 - 1 : 0;
-      const int ncols = max(nc0, nc1);
-      const int sh0 = ncols - nc0, sh1 = ncols - nc1;  // left pre-padding of each query
-      const int q = seg ? q1 : q0, la = seg ? la1 : la0;  // the lane's query
+    for (int qi = qb; qi < qe; qi += NQ) {
+      // the wave's queries: qv[u], its length, first word and left pre-padding
+      int qv[NQ], lav[NQ], shv[NQ];
+      uint32_t qwov[NQ];
+      int ncols = 0;
+#pragma unroll
+      for (int u = 0; u < NQ; ++u) {
+        qv[u] = __builtin_amdgcn_readfirstlane(u == 0 ? load_qid<kG>(a, qi) : a.qids[qi + u]);
+        lav[u] = __builtin_amdgcn_readfirstlane(a.len[qv[u]]);
+        qwov[u] = __builtin_amdgcn_readfirstlane(a.woff[qv[u]]);
+        ncols = max(ncols, lav[u] > 1 ? lav[u] - 1 : 0);
+      }
+#pragma unroll
+      for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
+      // the lane's queries: table seg's, low halves q, high halves q_hi
+      const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
+      const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
+      const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
+      const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
       const int nstrips = (ncols + S - 1) / S;
       s16x2 best = floor2, best2 = floor2;
       s16x2 corner = floor2;
@@ -764,14 +777,23 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
           const int sg = kQ == 2 && e >= kPairs * S ? 1 : 0;
           const int e1 = e - sg * (kPairs * S);
           const int p = e1 / S, jj = e1 - p * S;
-          const int k = j0 + jj - (sg ? sh1 : sh0);
+          uint32_t clo, chi;
+          tab_codes(p, clo, chi);
+          // table sg: low halves score query sg * kPer, high halves sg * kPer + kPer - 1
+          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
+          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
           uint32_t v = 0x80008000u;
-          if (k >= 1) {
-            const uint32_t r = (uint32_t)(k - 1);
-            const uint32_t qc = (a.words[(sg ? qwo1 : qwo0) + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            uint32_t clo, chi;
-            tab_codes(p, clo, chi);
-            v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
+          if (kl >= 1) {
+            const uint32_t r = (uint32_t)(kl - 1);
+            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
+          }
+          if (kh >= 1) {
+            const uint32_t r = (uint32_t)(kh - 1);
+            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
           }
           tab[sg * kTab + p * RS + jj] = v;
         }
@@ -833,7 +855,7 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
         corner = as_s16x2(cu);
       }
       finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
-                   best, corner);
+                   best, corner, q_hi, la_hi);
     }
   }
 }
@@ -890,17 +912,15 @@ __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f
 
 // Register budgets as for sw_lds_kernel.
 //
-// Several queries per wave (epsilon mode, item lists), for target groups
-// shared by several rows:
-// - kQ = 2: lanes 0-31 run query qids[qi] against the targets in group slots
-//   0-31 and 64-95, lanes 32-63 query qids[qi + 1] against slots 32-63 and
-//   96-127; each half-wave reads its own profile table.
-// - kLH: the low and high halves of the lanes run different queries, low
-//   halves qids[qi + 2s] against slots s * 32 .. (all 64 lanes: slots 0-63
-//   when kQ = 1), high halves qids[qi + 2s + 1] against slots 64 + s * 32 ..;
-//   a table word holds the low query's score in its low half and the high
-//   query's in its high half, so one table serves both (kQ = 1: two queries
-//   per wave, 64 targets each, at one query's LDS; kQ = 2: four, 32 each).
+// Several queries per wave (kLH; epsilon mode, item lists), for target
+// groups shared by several rows: the low and high halves of the lanes run
+// different queries.  A table word holds the low query's score in its low
+// half and the high query's in its high half, so one table serves both.
+// - kQ = 1: two queries per wave at one query's LDS, qids[qi] against group
+//   slots 0-63 (low halves), qids[qi + 1] against slots 64-127.
+// - kQ = 2: four, one table per half-wave s (lanes 32 s .. 32 s + 31):
+//   qids[qi + 2s] against slots 32 s .., qids[qi + 2s + 1] against slots
+//   64 + 32 s ...
 // All queries of a wave share the strips, the shorter ones pre-padded on the
 // left (padding columns score below every cell they could feed, so a
 // query's maximum and its right-most column are unchanged); the rows are
@@ -908,7 +928,7 @@ __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f
 template <int S, bool kG = false, int kQ = 1, bool kLH = false>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_f16_kernel(KArgs a) {
   static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
-  static_assert(kQ == 1 || kQ == 2, "one or two tables per wave");
+  static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
   constexpr int kPer = kLH ? 2 : 1;  // queries per table
   constexpr int NQ = kQ * kPer;      // queries per wave
   constexpr int kPairs = 25;
@@ -1276,29 +1296,26 @@ int wave_rows(int len, int k) {
 }
 
 // Kernel of variant k at strip S; group_items: the allocation-free
-// one-vs-all instantiation; split = 2: the fp16 or int16 LDS kernel with two
-// queries per wave, one per half-wave (strips 32 and 64); split = 3 / 4: the
-// fp16 kernel with two / four queries per wave, the lanes' low and high
-// halves on different queries.
+// one-vs-all instantiation; split = 3 / 4: the fp16 or int16 LDS kernel with
+// two / four queries per wave (the lanes' low and high halves on different
+// queries, four with one table per half-wave; strips 32 and 64).
 void *kernel_ptr(int k, int S, bool acgt, bool group_items = false, int split = 1) {
-  if (split == 3 || split == 4) {  // fp16 form, low and high halves on different queries
-    if (group_items || k != KERNEL_LDS_F16) return nullptr;
+  if (split == 3 || split == 4) {
+    if (group_items || (k != KERNEL_LDS_F16 && k != KERNEL_LDS)) return nullptr;
+    if (k == KERNEL_LDS_F16) {
+      if (split == 3)
+        return S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 1, true>)
+                       : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 1, true>);
+      return S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 2, true>)
+                     : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 2, true>);
+    }
     if (split == 3)
-      return S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 1, true>)
-                     : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 1, true>);
-    return S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 2, true>)
-                   : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 2, true>);
+      return S == 64 ? reinterpret_cast<void *>(&sw_lds_kernel<64, false, 1, true>)
+                     : reinterpret_cast<void *>(&sw_lds_kernel<32, false, 1, true>);
+    return S == 64 ? reinterpret_cast<void *>(&sw_lds_kernel<64, false, 2, true>)
+                   : reinterpret_cast<void *>(&sw_lds_kernel<32, false, 2, true>);
   }
-  if (split == 2) {
-    if (group_items) return nullptr;
-    if (k == KERNEL_LDS_F16)
-      return S == 64 ? reinterpret_cast<void *>(&sw_lds_f16_kernel<64, false, 2>)
-                     : reinterpret_cast<void *>(&sw_lds_f16_kernel<32, false, 2>);
-    if (k == KERNEL_LDS)
-      return S == 64 ? reinterpret_cast<void *>(&sw_lds_kernel<64, false, 2>)
-                     : reinterpret_cast<void *>(&sw_lds_kernel<32, false, 2>);
-    return nullptr;
-  }
+  if (split != 1) return nullptr;
   return group_items ? kernel_ptr_g<true>(k, S, acgt) : kernel_ptr_g<false>(k, S, acgt);
 }
 
@@ -1418,7 +1435,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   if (w.dev) {
     const int DK = (K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K;
     const int DS = DK == KERNEL_I32 ? kI32Strip : (w.dev_strip ? w.dev_strip : S);
-    if (w.dev_split >= 2 && (w.dev_split > 4 || (DK != KERNEL_LDS_F16 && (DK != KERNEL_LDS || w.dev_split != 2)) ||
+    if (w.dev_split != 1 && (w.dev_split < 3 || w.dev_split > 4 || (DK != KERNEL_LDS_F16 && DK != KERNEL_LDS) ||
                              mode != MODE_EPS || (DS != 32 && DS != 64))) {
       cleanup();
       return MNHN_ERR_ARG;

@@ -135,10 +135,10 @@ def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, monkeypatch, tu
 @pytest.mark.parametrize("shape", ["lone_half_row", "mixed_counts", "long_rows"])
 def test_prefilter_half_groups_equal_exact_lists(pkg, monkeypatch, split, shape):
     """Verification with target groups shared by several rows and without
-    them gives the exact lists.  split 2: fp16-band remainders in half pieces
-    (two rows per group, one per half of the lanes' packed pairs) and quarter
-    pieces (four rows per group); split 1: half pieces only; both: int16-band
-    remainders of at most 64 in groups shared by two rows, one per half-wave.
+    them gives the exact lists.  split 2: fp16-band and int16-band remainders
+    in half pieces (two rows per group, one per half of the lanes' packed
+    pairs) and quarter pieces (four rows per group); split 1: half pieces
+    only.
     lone_half_row: 4 related sequences of distinct lengths hold 3, 2, 1 and 0
     candidates, so the last shared group is short of rows (its missing rows
     run no targets); mixed_counts: rows with 0-200 candidates, i.e. full

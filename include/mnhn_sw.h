@@ -47,7 +47,8 @@ typedef struct {
 typedef struct mnhn_sw_ctx mnhn_sw_ctx;
 
 /* epsilon-neighbour lists in CSR form (library-owned; free with
- * mnhn_csr_free).  Row i lists every j != i with d(i, j) <= eps_max, in
+ * mnhn_csr_free only: the arrays of large lists are pinned host memory, which
+ * free() must not release).  Row i lists every j != i with d(i, j) <= eps_max, in
  * ascending j, with its distance (unsigned 16-bit: listed distances are at
  * most 65535, see mnhn_sw_eps_csr). */
 typedef struct {

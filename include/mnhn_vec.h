@@ -51,6 +51,8 @@ void mnhn_vec_destroy(mnhn_vec_ctx *ctx);
 /* All unordered pairs once, both directions listed.  thr is compared with
  * the accumulator as is: pass eps * eps (float) for L2, eps for L1. */
 int mnhn_vec_eps_csr(mnhn_vec_ctx *ctx, int norm, float thr, mnhn_vcsr *out);
+/* The only way to release a mnhn_vcsr: the arrays of large lists are pinned
+ * host memory, which free() must not release. */
 void mnhn_vcsr_free(mnhn_vcsr *csr);
 
 /* Kernel milliseconds and pair count of the last mnhn_vec_eps_csr. */

@@ -779,21 +779,30 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
           const int p = e1 / S, jj = e1 - p * S;
           uint32_t clo, chi;
           tab_codes(p, clo, chi);
-          // table sg: low halves score query sg * kPer, high halves sg * kPer + kPer - 1
-          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
-          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
           uint32_t v = 0x80008000u;
-          if (kl >= 1) {
-            const uint32_t r = (uint32_t)(kl - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
-            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
-          }
-          if (kh >= 1) {
-            const uint32_t r = (uint32_t)(kh - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
-            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
+          if constexpr (!kLH) {  // one query: one code for both halves
+            const int k = j0 + jj - shv[0];
+            if (k >= 1) {
+              const uint32_t r = (uint32_t)(k - 1);
+              const uint32_t qc = (a.words[qwov[0] + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+              v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
+            }
+          } else {
+            // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
+            const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
+            const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
+            if (kl >= 1) {
+              const uint32_t r = (uint32_t)(kl - 1);
+              const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+              v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
+            }
+            if (kh >= 1) {
+              const uint32_t r = (uint32_t)(kh - 1);
+              const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+              v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
+            }
           }
           tab[sg * kTab + p * RS + jj] = v;
         }
@@ -1025,21 +1034,30 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
           const int p = e1 / W, jj = e1 - p * W;
           uint32_t clo, chi;
           tab_codes(p, clo, chi);
-          // table sg: low halves score query sg * kPer, high halves sg * kPer + kPer - 1
-          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
-          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
           uint32_t v = spre | (spre << 16);
-          if (kl >= 1) {
-            const uint32_t r = (uint32_t)(kl - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
-            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
-          }
-          if (kh >= 1) {
-            const uint32_t r = (uint32_t)(kh - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
-            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-            v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
+          if constexpr (!kLH) {  // one query: one code for both halves
+            const int k = j0 + jj - shv[0];
+            if (k >= 1) {
+              const uint32_t r = (uint32_t)(k - 1);
+              const uint32_t qc = (a.words[qwov[0] + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+              v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
+            }
+          } else {
+            // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
+            const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
+            const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
+            if (kl >= 1) {
+              const uint32_t r = (uint32_t)(kl - 1);
+              const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+              v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
+            }
+            if (kh >= 1) {
+              const uint32_t r = (uint32_t)(kh - 1);
+              const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+              v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
+            }
           }
           tab[sg * kTab + p * RS + jj] = v;
         }

@@ -74,6 +74,10 @@ struct mnhn_sw_ctx {
   uint64_t data_hash = 0;  // scores, count, lengths and bytes of the sequences
   void *scratch = nullptr;  // this device's static scratch (mnhn_sw_one_vs_all_noalloc)
   mnhn_detail_stage stage;
+  // pinned staging of host work lists for their copy to the device (grown on
+  // demand, released with the context)
+  void *h_items = nullptr;
+  size_t h_items_bytes = 0;
 };
 
 namespace mnhn_detail {
@@ -120,6 +124,10 @@ void dev_free(T *&p) {
 struct Work {
   std::vector<int4> items;
   std::vector<int32_t> qids;
+  // qids is the context's length order (qids[k] = order[k], so rank[qids[k]]
+  // = k and lengths fall along it): launch_tiles then takes an item's longest
+  // query and its cells from prefix sums instead of walking its queries
+  bool qids_order = false;
   std::vector<int32_t> gtgt, gmax, gmin;  // empty = use the context's length groups
   double cells = 0, pairs = 0;
   int max_q_len = 0;

@@ -1400,6 +1400,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
                  unsigned long long key_cap) {
   const size_t n_items_total = w.dev ? (size_t)w.n_items : w.items.size();
   if (n_items_total == 0) return MNHN_OK;
+  mnhn_phase_clock clk(c->stream, "launch_tiles");
   int rc;
   int4 *d_items = nullptr;
   int32_t *d_qids = nullptr, *d_gt = nullptr, *d_gm = nullptr, *d_gn = nullptr;
@@ -1450,6 +1451,19 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     int split = 1;
   };
   std::vector<Pass> passes;
+  // an item's longest query; with the length order, its first
+  auto item_max_query = [&](const int4 &it) {
+    if (w.qids_order) return it.z > it.y ? c->len[w.qids[it.y]] : 0;
+    int m = 0;
+    for (int qi = it.y; qi < it.z; ++qi) m = std::max(m, c->len[w.qids[qi]]);
+    return m;
+  };
+  // prefix sums of the queries' lengths along qids (length order only)
+  std::vector<double> qlen_prefix;
+  if (!w.dev && w.qids_order) {
+    qlen_prefix.assign(w.qids.size() + 1, 0.0);
+    for (size_t k = 0; k < w.qids.size(); ++k) qlen_prefix[k + 1] = qlen_prefix[k] + c->len[w.qids[k]];
+  }
   if (w.dev) {
     const int DK = (K == KERNEL_LDS_F16 && !w.dev_f16_ok) ? KERNEL_LDS : K;
     const int DS = DK == KERNEL_I32 ? kI32Strip : (w.dev_strip ? w.dev_strip : S);
@@ -1465,10 +1479,10 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     // run the 32-bit kernel; with the fp16 form, items whose bound is at most
     // 2048 run exactly in fp16 and the rest in packed int16
     std::vector<int4> f16, i16, i32;
+    f16.reserve(w.items.size());
     for (const int4 &it : w.items) {
       const int gm = own_groups ? w.gmax[it.x] : c->gmax[it.x];
-      int maxla = 0;
-      for (int qi = it.y; qi < it.z; ++qi) maxla = std::max(maxla, c->len[w.qids[qi]]);
+      const int maxla = item_max_query(it);
       const long long bound = (long long)c->match * std::min(gm, maxla);
       if (bound > kI16ScoreMax) i32.push_back(it);
       else if (K == KERNEL_LDS_F16 && bound <= 2048) f16.push_back(it);
@@ -1493,6 +1507,18 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   }
   auto item_cells = [&](const int4 &it) {
     double s = 0;
+    if (w.qids_order && !own_groups) {
+      // queries before the group's first member pair with all its slots
+      // (suffix from 0); with the triangle, the group's own members pair
+      // with the slots after them
+      const double *gs = &c->gsuf[(size_t)it.x * (kGroup + 1)];
+      const int g0 = it.x * kGroup;
+      const int mid = triangle ? std::min(std::max(g0, it.y), it.z) : it.z;
+      s = gs[0] * (qlen_prefix[mid] - qlen_prefix[it.y]);
+      for (int k = mid; k < it.z; ++k)
+        s += (double)c->len[c->order[k]] * gs[std::max(0, std::min(kGroup, k - g0 + 1))];
+      return s;
+    }
     for (int qi = it.y; qi < it.z; ++qi) {
       const int q = w.qids[qi];
       double tg;
@@ -1519,11 +1545,8 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
         continue;
       }
       std::vector<int4> shorter, rest;
-      for (const int4 &it : ps.items) {
-        int maxla = 0;
-        for (int qi = it.y; qi < it.z; ++qi) maxla = std::max(maxla, c->len[w.qids[qi]]);
-        (maxla <= kShortQuery ? shorter : rest).push_back(it);
-      }
+      rest.reserve(ps.items.size());
+      for (const int4 &it : ps.items) (item_max_query(it) <= kShortQuery ? shorter : rest).push_back(it);
       if (!rest.empty()) split.push_back({ps.kernel, S, std::move(rest)});
       if (!shorter.empty()) split.push_back({ps.kernel, 32, std::move(shorter)});
     }
@@ -1553,6 +1576,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   const int ws_len = std::max(w.max_q_len, c->maxlen);
   int waves_used = 0;
   bool any_i32 = false;
+  clk.mark("passes and cells");
   (void)hipEventRecord(c->ev[0], c->stream);
   for (size_t pi = 0; pi < passes.size(); ++pi) {
     Pass &pass = passes[pi];
@@ -1585,9 +1609,27 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
       }
       c->bound_bytes = need;
     }
-    if (!w.dev)
-      e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
-                         c->stream);
+    if (!w.dev) {
+      // lists of 1 MB and more go through the context's pinned staging (an
+      // 11 MB list: 3 ms from pageable memory); the previous pass has
+      // finished with it (the stream is synchronized after every pass)
+      const size_t ib = items.size() * sizeof(int4);
+      const void *src = items.data();
+      if (ib >= (1u << 20)) {
+        if (c->h_items_bytes < ib) {
+          if (c->h_items) (void)hipHostFree(c->h_items);
+          c->h_items = nullptr;
+          c->h_items_bytes = 0;
+          if (hipHostMalloc(&c->h_items, ib, hipHostMallocDefault) == hipSuccess) c->h_items_bytes = ib;
+          else c->h_items = nullptr;
+        }
+        if (c->h_items) {
+          memcpy(c->h_items, items.data(), ib);
+          src = c->h_items;
+        }
+      }
+      e = hipMemcpyAsync(d_items, src, ib, hipMemcpyHostToDevice, c->stream);
+    }
     if (e == hipSuccess) e = hipMemsetAsync(c->d_counter, 0, sizeof(int), c->stream);
     KArgs a{};
     a.words = c->d_words; a.woff = c->d_woff; a.len = c->d_len; a.rank = c->d_rank;
@@ -1610,6 +1652,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     // the next pass reuses d_items: wait until this one has read them
     if (passes.size() > 1) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) break;
+    clk.mark(PK == KERNEL_LDS_F16 ? "pass (fp16 form)" : PK == KERNEL_LDS ? "pass (int16 form)" : "pass");
   }
   (void)hipEventRecord(c->ev[1], c->stream);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
@@ -1672,7 +1715,14 @@ void add_group_items(const mnhn_sw_ctx *c, Work &w, const std::vector<double> &P
   const int strips = gm > 1 ? (gm - 1 + S - 1) / S : 0;
   int qb = q_begin;
   while (qb < q_end) {
-    int lo = qb, hi = q_end - 1;  // first qi with cost(qb .. qi) >= target, else q_end - 1
+    // first qi with cost(qb .. qi) >= target, else q_end - 1: a galloping
+    // search from qb (items hold a few queries), then a binary one
+    int lo = qb, hi = qb, step = 1;
+    while (hi < q_end - 1 && range_cost(P, qb, hi + 1, strips, S) < target_cost) {
+      lo = hi + 1;
+      hi = std::min(q_end - 1, hi + step);
+      step *= 2;
+    }
     while (lo < hi) {
       const int mid = lo + (hi - lo) / 2;
       if (range_cost(P, qb, mid + 1, strips, S) >= target_cost) hi = mid;
@@ -1699,9 +1749,22 @@ void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
   std::vector<uint16_t> cls(m);
   std::vector<size_t> start(kClasses + 1, 0);
   const std::vector<double> P = query_prefix(c, w);
+  // class floor(16 log2(cost)) from the exponent and a table of 2^(j / 16)
+  static const double kSteps[15] = {1.0442737824274138, 1.0905077326652577, 1.1387886347566916,
+                                    1.1892071150027210, 1.2418578120734840, 1.2968395546510096,
+                                    1.3542555469368927, 1.4142135623730951, 1.4768261459394993,
+                                    1.5422108254079407, 1.6104903319492543, 1.6817928305074290,
+                                    1.7562521603732995, 1.8340080864093424, 1.9152065613971474};
   for (size_t k = 0; k < m; ++k) {
     const double cost = item_cost(c, w, P, w.items[k], S);
-    int q = cost > 1.0 ? (int)(16.0 * std::log2(cost)) : 0;
+    int q = 0;
+    if (cost > 1.0) {
+      int e = 0;
+      const double f = 2.0 * std::frexp(cost, &e);  // cost = f 2^(e - 1), f in [1, 2)
+      int j = 0;
+      while (j < 15 && f >= kSteps[j]) ++j;
+      q = 16 * (e - 1) + j;
+    }
     q = std::min(std::max(q, 0), kClasses - 1);
     cls[k] = (uint16_t)(kClasses - 1 - q);  // larger cost, smaller class index
     ++start[cls[k] + 1];
@@ -1735,6 +1798,7 @@ void finalize_items(const mnhn_sw_ctx *c, Work &w, int S, bool dense_out) {
 Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps, int p0 = 0) {
   Work w;
   w.qids = c->order;
+  w.qids_order = true;
   std::vector<int> first(c->groups, p0);
   if (eps >= 0) {
     for (int g = 0; g < c->groups; ++g) {
@@ -1757,6 +1821,7 @@ Work triangle_work(const mnhn_sw_ctx *c, int S, int32_t eps, int p0 = 0) {
     if (last > first[g]) total += (double)strips * S * (P[last] - P[first[g]]);
   }
   const double target = std::max(total / ((double)std::max(c->num_cu, 1) * items_per_cu()), 2.0e5);
+  w.items.reserve((size_t)(total / target * 1.5) + 2 * (size_t)c->groups + 16);
   for (int g = 0; g < c->groups; ++g) {
     const int last = std::min((g + 1) * kGroup, c->n) - 1;
     if (last <= first[g]) continue;
@@ -1964,6 +2029,7 @@ void mnhn_sw_destroy(mnhn_sw_ctx *c) {
   for (auto &e : c->pev)
     if (e) (void)hipEventDestroy(e);
   stage_free(c->stage);
+  if (c->h_items) (void)hipHostFree(c->h_items);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2304,9 +2370,11 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_req, unsigned long long *nkeys_out) {
   int32_t eps_max = eps_req;
   int rc = list_bound(c, eps_req, &eps_max);
   if (rc) return rc;
+  mnhn_phase_clock clk(c->stream, "eps pass");
   const int S = pick_strip(c);
   Work w = triangle_work(c, S, length_prune() ? eps_max : -1);
   finalize_items(c, w, S, false);
+  clk.mark("work list");
   const unsigned long long all = (unsigned long long)c->n * (unsigned long long)(c->n > 0 ? c->n - 1 : 0);
   unsigned long long cap = c->keys_cap;
   if (cap == 0) {
@@ -2346,8 +2414,10 @@ int eps_pass(mnhn_sw_ctx *c, int32_t eps_req, unsigned long long *nkeys_out) {
     cap = std::max<unsigned long long>(std::min(cap, max_entries), 1);
     if ((rc = ensure_keys(c, cap))) return rc;
     HIP_TRY(hipMemsetAsync(c->d_nkeys, 0, sizeof(unsigned long long), c->stream));
+    clk.mark("key buffer");
     rc = launch_tiles(c, w, MODE_EPS, 1, eps_max, nullptr, c->d_keys, c->keys_cap);
     if (rc) return rc;
+    clk.mark("launch_tiles");
     unsigned long long nk = 0;
     HIP_TRY(hipMemcpy(&nk, c->d_nkeys, sizeof(nk), hipMemcpyDeviceToHost));
     if (nk <= c->keys_cap) {

@@ -135,13 +135,13 @@ def cpu_baseline(seqs, sample_n, seed=7, threads=1, port=False):
                       f"host: {cpu_model()}, nproc {os.cpu_count()}"}
 
 
-def profiled_traffic(config, strip, key="traffic_bytes_fp16_launches"):
+def profiled_traffic(config, strip, key="traffic_bytes_launch"):
     """HBM bytes of the SW kernels from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (calibrated against a plain copy
-    kernel): `traffic_bytes_fp16_launches` = the fp16-form launches of one
-    step (the roofline's kernel), `traffic_bytes_per_step` = every SW launch
-    of the step; None if there is no summary for it or it was taken at
-    another strip width than the step ran (`traffic_strip`)."""
+    kernel): `traffic_bytes_launch` = the roofline's kernel in one step (the
+    launch that runs the fp16 and int16 forms' items), `traffic_bytes_per_step`
+    = every SW launch of the step; None if there is no summary for it or it
+    was taken at another strip width than the step ran (`traffic_strip`)."""
     path = os.path.join(ROOT, "profiles", {"tree20k": "sw_lds_f16_kernel_tree20k.md"}.get(config, "-"))
     vals = {}
     try:
@@ -679,14 +679,32 @@ def main():
     gcups = comp_cells / step_s / 1e9
     peaks = [FP16_FORM_ROOFLINE_GCUPS, INT16_FORM_ROOFLINE_GCUPS, None]
     forms = {}
-    for f, name in enumerate(("fp16", "int16", "int32")):
-        if form_cells[f] > 0:
-            g = form_cells[f] / (form_ms[f] / 1e3) / 1e9
-            forms[name] = {"cells": form_cells[f], "kernel_ms": form_ms[f], "gcups": g,
-                           "instruction_roofline_gcups": peaks[f],
-                           "frac": g / peaks[f] if peaks[f] else None}
-    # dominant kernel: the exact fp16 form (sw_lds_f16_kernel, strips 64 and 32)
-    f16 = forms.get("fp16") or forms.get("int16")
+    mixed = bool(t.get("mixed_launch"))
+    if mixed:
+        # one launch ran the fp16 form's items and the int16 form's (its time is
+        # form_ms[0]): its roofline is the time both forms' cells take at their
+        # own instruction rooflines
+        c0, c1 = form_cells[0], form_cells[1]
+        at_roof = c0 / (peaks[0] * 1e9) + c1 / (peaks[1] * 1e9)
+        ms = form_ms[0]
+        forms["fp16+int16 (one launch)"] = {
+            "cells": c0 + c1, "fp16_cells": c0, "int16_cells": c1, "kernel_ms": ms,
+            "gcups": (c0 + c1) / (ms / 1e3) / 1e9,
+            "instruction_roofline_gcups": (c0 + c1) / at_roof / 1e9, "frac": at_roof / (ms / 1e3)}
+        if form_cells[2] > 0:
+            forms["int32"] = {"cells": form_cells[2], "kernel_ms": form_ms[2],
+                              "gcups": form_cells[2] / (form_ms[2] / 1e3) / 1e9,
+                              "instruction_roofline_gcups": None, "frac": None}
+    else:
+        for f, name in enumerate(("fp16", "int16", "int32")):
+            if form_cells[f] > 0:
+                g = form_cells[f] / (form_ms[f] / 1e3) / 1e9
+                forms[name] = {"cells": form_cells[f], "kernel_ms": form_ms[f], "gcups": g,
+                               "instruction_roofline_gcups": peaks[f],
+                               "frac": g / peaks[f] if peaks[f] else None}
+    # dominant kernel: the one launch of both forms (sw_lds_mixed_kernel), or
+    # the exact fp16 form's launches
+    f16 = forms.get("fp16+int16 (one launch)") or forms.get("fp16") or forms.get("int16")
     f16_peak = f16["instruction_roofline_gcups"]
     # the step against each form's own roofline: time at roofline / kernel time
     at_roof_s = sum(forms[k]["cells"] / (forms[k]["instruction_roofline_gcups"] * 1e9)
@@ -708,10 +726,13 @@ def main():
                    "strip": t["strip"], "waves": t["waves"], "items": t["items"],
                    "autotune": tuning},
         "roofline": {"bound": "valu",
-                     "kernel": "sw_lds_f16_kernel (strips 64 and 32)" if "fp16" in forms else "sw_lds_kernel",
+                     "kernel": ("sw_lds_mixed_kernel (one launch: exact fp16 form, int16 form for pairs "
+                                "beyond it)" if mixed else
+                                "sw_lds_f16_kernel (strips 64 and 32)" if "fp16" in forms else "sw_lds_kernel"),
                      "achieved": f16["gcups"], "peak": f16_peak,
                      "unit": "GCUPS (cells of the pairs the kernel computed / its time, HIP events on the "
-                             "engine's stream; peak = the form's instruction roofline)",
+                             "engine's stream; peak = the instruction roofline of its forms, each form's "
+                             "cells at its own)",
                      "frac": f16["gcups"] / f16_peak,
                      "traffic": profiled_traffic(args.config, t["strip"]),
                      "step_traffic": profiled_traffic(args.config, t["strip"], "traffic_bytes_per_step"),

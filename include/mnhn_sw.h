@@ -72,7 +72,8 @@ typedef struct {
   int32_t strip;      /* register strip width used */
   int32_t waves;      /* persistent waves launched */
   int32_t items;      /* work items in the queue */
-  int32_t reserved;
+  int32_t mixed_launch; /* 1: forms [0] and [1] ran as one launch, whose time
+                           is form_ms[0] (form_ms[1] is 0) */
   /* per lane format: [0] exact fp16 form, [1] packed int16 forms, [2] 32-bit
    * form -- the cells each computed (sum of la*lb over its pairs) and its
    * kernel time (HIP events around its launch) */

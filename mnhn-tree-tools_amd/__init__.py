@@ -96,7 +96,7 @@ class Timing(ctypes.Structure):
                 ("cells", ctypes.c_double), ("pairs", ctypes.c_double),
                 ("lane_bits", ctypes.c_int32), ("kernel", ctypes.c_int32),
                 ("strip", ctypes.c_int32),
-                ("waves", ctypes.c_int32), ("items", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("waves", ctypes.c_int32), ("items", ctypes.c_int32), ("mixed_launch", ctypes.c_int32),
                 ("form_cells", ctypes.c_double * 3), ("form_ms", ctypes.c_double * 3)]
 
 

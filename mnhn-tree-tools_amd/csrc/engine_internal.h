@@ -201,7 +201,9 @@ Work suffix_work(const mnhn_sw_ctx *c, int32_t eps, int p0);
 // it has fewer than 4,096 items.
 int eps_keys_estimate(mnhn_sw_ctx *c, const Work &w, int32_t eps, unsigned long long *est);
 
-enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3, KERNEL_I32 = 4 };
+// KERNEL_LDS_MIXED: one launch for the exact fp16 form's and the int16 form's
+// items (epsilon mode, host work lists), not a tuning choice of its own
+enum { KERNEL_AUTO = 0, KERNEL_REG = 1, KERNEL_LDS = 2, KERNEL_LDS_F16 = 3, KERNEL_I32 = 4, KERNEL_LDS_MIXED = 5 };
 enum { MODE_DIST = 0, MODE_EPS = 1 };
 constexpr int kGroupSize = 128;
 

@@ -677,11 +677,183 @@ __global__ __launch_bounds__(kBlock) void sw_i32_kernel(KArgs a) {
 // between one row's dependent packed ops better than interleaving two rows
 // in one wave (which needs registers for both rows' profiles).
 //
-// kQ = 2 (epsilon mode, item lists): two queries per wave, for target groups
-// shared by two rows, with the slot layout and left pre-padding of the fp16
-// kernel's kQ = 2 form below (lanes 0-31: query qids[qi] against slots 0-31
-// and 64-95; lanes 32-63: qids[qi + 1] against slots 32-63 and 96-127).
 // kQ / kLH: several queries per wave, as in sw_lds_f16_kernel (below).
+// One work item of sw_lds_kernel: target group it.x against the queries
+// qids[it.y .. it.z) (it.w: output base), with the wave's profile table(s)
+// at tab and its workspace ws.
+template <int S, bool kG, int kQ, bool kLH>
+__device__ __forceinline__ void sw_lds_item(const KArgs &a, uint32_t *tab, const WaveRows &ws, int lane,
+                                        const int4 it) {
+  static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
+  static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
+  constexpr int kPer = kLH ? 2 : 1;  // queries per table
+  constexpr int NQ = kQ * kPer;      // queries per wave
+  constexpr int kPairs = 25;
+  constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
+  constexpr int kTab = kPairs * RS;  // words of one query's table
+  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's table
+  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab + (uint32_t)seg * (kTab * 4u);
+
+  const s16x2 floor2 = {kFloor, kFloor};
+  const s16x2 gapv = {(short)-a.gap, (short)-a.gap};
+  const uint32_t smat = (uint32_t)(uint16_t)(short)a.match;
+  const uint32_t smis = (uint32_t)(uint16_t)(short)a.mismatch;
+
+  const int g = __builtin_amdgcn_readfirstlane(it.x);
+  const int qb = __builtin_amdgcn_readfirstlane(it.y);
+  const int qe = __builtin_amdgcn_readfirstlane(it.z);
+  const int ob = __builtin_amdgcn_readfirstlane(it.w);
+
+  const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
+  const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
+  const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
+  const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
+  const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
+  // empty slots are treated as full length (their results are discarded)
+  const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
+  const int rows_hi = t_hi >= 0 ? lb_hi - 1 : gm;
+  const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
+  const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
+  const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
+  const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
+  const int nrows = gm > 1 ? gm - 1 : 0;     // DP rows = inner residues of the longest target
+  build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
+
+  for (int qi = qb; qi < qe; qi += NQ) {
+    // the wave's queries: qv[u], its length, first word and left pre-padding
+    int qv[NQ], lav[NQ], shv[NQ];
+    uint32_t qwov[NQ];
+    int ncols = 0;
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      qv[u] = __builtin_amdgcn_readfirstlane(u == 0 ? load_qid<kG>(a, qi) : a.qids[qi + u]);
+      lav[u] = __builtin_amdgcn_readfirstlane(a.len[qv[u]]);
+      qwov[u] = __builtin_amdgcn_readfirstlane(a.woff[qv[u]]);
+      ncols = max(ncols, lav[u] > 1 ? lav[u] - 1 : 0);
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
+    // the lane's queries: table seg's, low halves q, high halves q_hi
+    const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
+    const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
+    const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
+    const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
+    const int nstrips = (ncols + S - 1) / S;
+    s16x2 best = floor2, best2 = floor2;
+    s16x2 corner = floor2;
+
+    // strip s of the query; first (compile-time): s == 0, no strip to its left
+    auto strip = [&](int s, auto firstc) {
+      constexpr bool first = decltype(firstc)::value;
+      const int j0 = ncols - (nstrips - s) * S + 1;  // query column of slot 0
+      // The first list word and left boundaries are loaded before the
+      // table build, whose own loads are waited for, so nothing is pending
+      // when the row loop starts: the loop's waits then count only the
+      // loads and stores of one iteration.
+      uint32_t ra = ws.load(a.list_row);
+      uint32_t bA = as_u32(floor2), bB = as_u32(floor2);
+      if (!first) {
+        bA = ws.load(1);
+        if (nrows >= 2) bB = ws.load(2);
+      }
+      // ---- score table of this strip
+      __builtin_amdgcn_wave_barrier();
+      for (int e = lane; e < kQ * kPairs * S; e += kWave) {
+        const int sg = kQ == 2 && e >= kPairs * S ? 1 : 0;
+        const int e1 = e - sg * (kPairs * S);
+        const int p = e1 / S, jj = e1 - p * S;
+        uint32_t clo, chi;
+        tab_codes(p, clo, chi);
+        uint32_t v = 0x80008000u;
+        if constexpr (!kLH) {  // one query: one code for both halves
+          const int k = j0 + jj - shv[0];
+          if (k >= 1) {
+            const uint32_t r = (uint32_t)(k - 1);
+            const uint32_t qc = (a.words[qwov[0] + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
+          }
+        } else {
+          // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
+          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
+          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
+          if (kl >= 1) {
+            const uint32_t r = (uint32_t)(kl - 1);
+            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
+          }
+          if (kh >= 1) {
+            const uint32_t r = (uint32_t)(kh - 1);
+            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
+          }
+        }
+        tab[sg * kTab + p * RS + jj] = v;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+
+      s16x2 H[S];
+#pragma unroll
+      for (int jj = 0; jj < S; ++jj) H[jj] = floor2;
+      s16x2 diag_b = floor2;
+      // one DP row; its left boundary becomes the next row's diagonal
+      auto do_row = [&](int j, uint32_t left_u, uint32_t paddr) {
+        const lds_u4 *prow = (const lds_u4 *)(uintptr_t)paddr;
+        s16x2 P[S];
+#pragma unroll
+        for (int w = 0; w < S / 4; ++w) {
+          const u32x4 v = prow[w];
+          P[4 * w + 0] = as_s16x2(v.x);
+          P[4 * w + 1] = as_s16x2(v.y);
+          P[4 * w + 2] = as_s16x2(v.z);
+          P[4 * w + 3] = as_s16x2(v.w);
+        }
+        const s16x2 left_b = as_s16x2(left_u);
+        dp_row2<S>(H, P, left_b, diag_b, gapv, best, best2);
+        ws.store(j, as_u32(H[S - 1]));
+        diag_b = left_b;
+      };
+      // Two rows per iteration; their profile-row addresses and left
+      // boundaries are loaded one iteration ahead, each into the register
+      // whose previous value has just been used: a load that had to be
+      // copied into a loop-carried register would have to complete first.
+      // The loads are unconditional, keeping the iteration one basic block:
+      // boundary rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
+      // wave's maxlen + 2 boundary rows, and list word (j + 1) / 2 <=
+      // nrows / 2 inside its maxlen / 2 + 1 list rows; what is loaded past
+      // nrows is never used.
+      int j = 1;
+      for (; j + 1 <= nrows; j += 2) {
+        const uint32_t p0 = ra & 0xffffu, p1 = ra >> 16;
+        ra = ws.load(a.list_row + ((j + 1) >> 1));
+        __builtin_amdgcn_sched_barrier(0);  // issue the load here, a full iteration ahead
+        do_row(j, bA, p0);
+        if (!first) bA = ws.load(j + 2);
+        do_row(j + 1, bB, p1);
+        if (!first) bB = ws.load(j + 3);
+      }
+      if (j <= nrows) do_row(j, bA, ra & 0xffffu);  // odd row count: the strip's last row
+    };
+    if (nstrips > 0 && nrows > 0) {
+      strip(0, std::true_type{});
+      for (int s = 1; s < nstrips; ++s) strip(s, std::false_type{});
+    }
+    best = __builtin_elementwise_max(best, best2);
+    if (nstrips > 0 && nrows > 0) {
+      // corner H(la-1, lb-1): the last strip's right-most column at row lb-1
+      uint32_t cu = as_u32(floor2);
+      if (lb_lo >= 2) cu = (cu & 0xffff0000u) | (ws.load_lane_row(lb_lo - 1) & 0xffffu);
+      if (lb_hi >= 2) cu = (cu & 0x0000ffffu) | (ws.load_lane_row(lb_hi - 1) & 0xffff0000u);
+      corner = as_s16x2(cu);
+    }
+    finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+                 best, corner, q_hi, la_hi);
+  }
+}
+
+// The persistent kernel: each wave takes work items from the queue.
 template <int S, bool kG = false, int kQ = 1, bool kLH = false>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_kernel(KArgs a) {
   static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
@@ -700,172 +872,12 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
   uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
   const WaveRows ws(bnd, a.bound_stride, lane);
   static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
-  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's table
-  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab + (uint32_t)seg * (kTab * 4u);
-
-  const s16x2 floor2 = {kFloor, kFloor};
-  const s16x2 gapv = {(short)-a.gap, (short)-a.gap};
-  const uint32_t smat = (uint32_t)(uint16_t)(short)a.match;
-  const uint32_t smis = (uint32_t)(uint16_t)(short)a.mismatch;
-
   for (;;) {
     int item_idx = 0;
     if (lane == 0) item_idx = atomicAdd(a.counter, 1);
     item_idx = __builtin_amdgcn_readfirstlane(item_idx);
     if (item_idx >= a.n_items) break;
-    const int4 it = load_item<kG>(a, item_idx);
-    const int g = __builtin_amdgcn_readfirstlane(it.x);
-    const int qb = __builtin_amdgcn_readfirstlane(it.y);
-    const int qe = __builtin_amdgcn_readfirstlane(it.z);
-    const int ob = __builtin_amdgcn_readfirstlane(it.w);
-
-    const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
-    const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
-    const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
-    const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
-    const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
-    // empty slots are treated as full length (their results are discarded)
-    const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
-    const int rows_hi = t_hi >= 0 ? lb_hi - 1 : gm;
-    const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
-    const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
-    const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
-    const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
-    const int nrows = gm > 1 ? gm - 1 : 0;     // DP rows = inner residues of the longest target
-    build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
-
-    for (int qi = qb; qi < qe; qi += NQ) {
-      // the wave's queries: qv[u], its length, first word and left pre-padding
-      int qv[NQ], lav[NQ], shv[NQ];
-      uint32_t qwov[NQ];
-      int ncols = 0;
-#pragma unroll
-      for (int u = 0; u < NQ; ++u) {
-        qv[u] = __builtin_amdgcn_readfirstlane(u == 0 ? load_qid<kG>(a, qi) : a.qids[qi + u]);
-        lav[u] = __builtin_amdgcn_readfirstlane(a.len[qv[u]]);
-        qwov[u] = __builtin_amdgcn_readfirstlane(a.woff[qv[u]]);
-        ncols = max(ncols, lav[u] > 1 ? lav[u] - 1 : 0);
-      }
-#pragma unroll
-      for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
-      // the lane's queries: table seg's, low halves q, high halves q_hi
-      const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
-      const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
-      const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
-      const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
-      const int nstrips = (ncols + S - 1) / S;
-      s16x2 best = floor2, best2 = floor2;
-      s16x2 corner = floor2;
-
-      // strip s of the query; first (compile-time): s == 0, no strip to its left
-      auto strip = [&](int s, auto firstc) {
-        constexpr bool first = decltype(firstc)::value;
-        const int j0 = ncols - (nstrips - s) * S + 1;  // query column of slot 0
-        // The first list word and left boundaries are loaded before the
-        // table build, whose own loads are waited for, so nothing is pending
-        // when the row loop starts: the loop's waits then count only the
-        // loads and stores of one iteration.
-        uint32_t ra = ws.load(a.list_row);
-        uint32_t bA = as_u32(floor2), bB = as_u32(floor2);
-        if (!first) {
-          bA = ws.load(1);
-          if (nrows >= 2) bB = ws.load(2);
-        }
-        // ---- score table of this strip
-        __builtin_amdgcn_wave_barrier();
-        for (int e = lane; e < kQ * kPairs * S; e += kWave) {
-          const int sg = kQ == 2 && e >= kPairs * S ? 1 : 0;
-          const int e1 = e - sg * (kPairs * S);
-          const int p = e1 / S, jj = e1 - p * S;
-          uint32_t clo, chi;
-          tab_codes(p, clo, chi);
-          uint32_t v = 0x80008000u;
-          if constexpr (!kLH) {  // one query: one code for both halves
-            const int k = j0 + jj - shv[0];
-            if (k >= 1) {
-              const uint32_t r = (uint32_t)(k - 1);
-              const uint32_t qc = (a.words[qwov[0] + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-              v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
-            }
-          } else {
-            // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
-            const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
-            const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
-            if (kl >= 1) {
-              const uint32_t r = (uint32_t)(kl - 1);
-              const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
-              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-              v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
-            }
-            if (kh >= 1) {
-              const uint32_t r = (uint32_t)(kh - 1);
-              const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
-              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-              v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
-            }
-          }
-          tab[sg * kTab + p * RS + jj] = v;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-
-        s16x2 H[S];
-#pragma unroll
-        for (int jj = 0; jj < S; ++jj) H[jj] = floor2;
-        s16x2 diag_b = floor2;
-        // one DP row; its left boundary becomes the next row's diagonal
-        auto do_row = [&](int j, uint32_t left_u, uint32_t paddr) {
-          const lds_u4 *prow = (const lds_u4 *)(uintptr_t)paddr;
-          s16x2 P[S];
-#pragma unroll
-          for (int w = 0; w < S / 4; ++w) {
-            const u32x4 v = prow[w];
-            P[4 * w + 0] = as_s16x2(v.x);
-            P[4 * w + 1] = as_s16x2(v.y);
-            P[4 * w + 2] = as_s16x2(v.z);
-            P[4 * w + 3] = as_s16x2(v.w);
-          }
-          const s16x2 left_b = as_s16x2(left_u);
-          dp_row2<S>(H, P, left_b, diag_b, gapv, best, best2);
-          ws.store(j, as_u32(H[S - 1]));
-          diag_b = left_b;
-        };
-        // Two rows per iteration; their profile-row addresses and left
-        // boundaries are loaded one iteration ahead, each into the register
-        // whose previous value has just been used: a load that had to be
-        // copied into a loop-carried register would have to complete first.
-        // The loads are unconditional, keeping the iteration one basic block:
-        // boundary rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
-        // wave's maxlen + 2 boundary rows, and list word (j + 1) / 2 <=
-        // nrows / 2 inside its maxlen / 2 + 1 list rows; what is loaded past
-        // nrows is never used.
-        int j = 1;
-        for (; j + 1 <= nrows; j += 2) {
-          const uint32_t p0 = ra & 0xffffu, p1 = ra >> 16;
-          ra = ws.load(a.list_row + ((j + 1) >> 1));
-          __builtin_amdgcn_sched_barrier(0);  // issue the load here, a full iteration ahead
-          do_row(j, bA, p0);
-          if (!first) bA = ws.load(j + 2);
-          do_row(j + 1, bB, p1);
-          if (!first) bB = ws.load(j + 3);
-        }
-        if (j <= nrows) do_row(j, bA, ra & 0xffffu);  // odd row count: the strip's last row
-      };
-      if (nstrips > 0 && nrows > 0) {
-        strip(0, std::true_type{});
-        for (int s = 1; s < nstrips; ++s) strip(s, std::false_type{});
-      }
-      best = __builtin_elementwise_max(best, best2);
-      if (nstrips > 0 && nrows > 0) {
-        // corner H(la-1, lb-1): the last strip's right-most column at row lb-1
-        uint32_t cu = as_u32(floor2);
-        if (lb_lo >= 2) cu = (cu & 0xffff0000u) | (ws.load_lane_row(lb_lo - 1) & 0xffffu);
-        if (lb_hi >= 2) cu = (cu & 0x0000ffffu) | (ws.load_lane_row(lb_hi - 1) & 0xffff0000u);
-        corner = as_s16x2(cu);
-      }
-      finish_query<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
-                   best, corner, q_hi, la_hi);
-    }
+    sw_lds_item<S, kG, kQ, kLH>(a, tab, ws, lane, load_item<kG>(a, item_idx));
   }
 }
 
@@ -934,6 +946,217 @@ __device__ __forceinline__ void dp_row_f16(f16x2 (&G)[S], const f16x2 (&P)[S], f
 // left (padding columns score below every cell they could feed, so a
 // query's maximum and its right-most column are unchanged); the rows are
 // the group's, as for one query.
+// One work item of sw_lds_f16_kernel: target group it.x against the queries
+// qids[it.y .. it.z) (it.w: output base), with the wave's profile table(s)
+// at tab and its workspace ws.
+template <int S, bool kG, int kQ, bool kLH>
+__device__ __forceinline__ void sw_lds_f16_item(const KArgs &a, uint32_t *tab, const WaveRows &ws, int lane,
+                                        const int4 it) {
+  static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
+  static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
+  constexpr int kPer = kLH ? 2 : 1;  // queries per table
+  constexpr int NQ = kQ * kPer;      // queries per wave
+  constexpr int kPairs = 25;
+  constexpr int RS = S + 4;
+  constexpr int kTab = kPairs * RS;  // words of one table
+  static_assert(S % 2 == 0, "pairs of columns");
+  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's table
+  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab + (uint32_t)seg * (kTab * 4u);
+
+  const _Float16 gap_h = (_Float16)((float)a.gap * kF16Scale);
+  const f16x2 gapv = {(_Float16)-gap_h, (_Float16)-gap_h};
+  const f16x2 g0 = gapv;  // G of an H = 0 cell
+  const f16x2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
+  // substitution scores plus the gap (G stores H - gap), scaled
+  const uint32_t smat = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)((float)(a.match + a.gap) * kF16Scale));
+  const uint32_t smis = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)((float)(a.mismatch + a.gap) * kF16Scale));
+  const uint32_t spre = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)-60000.0f);
+
+  const int g = __builtin_amdgcn_readfirstlane(it.x);
+  const int qb = __builtin_amdgcn_readfirstlane(it.y);
+  const int qe = __builtin_amdgcn_readfirstlane(it.z);
+  const int ob = __builtin_amdgcn_readfirstlane(it.w);
+
+  const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
+  const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
+  const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
+  const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
+  const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
+  const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
+  const int rows_hi = t_hi >= 0 ? lb_hi - 1 : gm;
+  const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
+  const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
+  const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
+  const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
+  const int nrows = gm > 1 ? gm - 1 : 0;
+  build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
+
+  for (int qi = qb; qi < qe; qi += NQ) {
+    // the wave's queries: qv[u], its length, first word and left pre-padding
+    int qv[NQ], lav[NQ], shv[NQ];
+    uint32_t qwov[NQ];
+    int ncols = 0;
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      qv[u] = __builtin_amdgcn_readfirstlane(u == 0 ? load_qid<kG>(a, qi) : a.qids[qi + u]);
+      lav[u] = __builtin_amdgcn_readfirstlane(a.len[qv[u]]);
+      qwov[u] = __builtin_amdgcn_readfirstlane(a.woff[qv[u]]);
+      ncols = max(ncols, lav[u] > 1 ? lav[u] - 1 : 0);
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
+    // the lane's queries: table seg's, low halves q, high halves q_hi
+    const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
+    const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
+    const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
+    const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
+    const int nstrips = (ncols + S - 1) / S;
+    f16x2 best = zero2, best2 = zero2;
+
+    // One strip of W query columns whose slot 0 is column j0 (columns < 1
+    // are pre-padding); first (compile-time): no strip to its left.
+    auto strip = [&](auto wc, int j0, auto firstc) {
+      constexpr int W = decltype(wc)::value;
+      constexpr bool first = decltype(firstc)::value;
+      // The first list word and left boundaries are loaded before the
+      // table build, whose own loads are waited for, so nothing is pending
+      // when the row loop starts: the loop's waits then count only the
+      // loads and stores of one iteration.
+      uint32_t ra = ws.load(a.list_row);
+      uint32_t bA = f16_u32(g0), bB = f16_u32(g0);
+      if (!first) {
+        bA = ws.load(1);
+        if (nrows >= 2) bB = ws.load(2);
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (int e = lane; e < kQ * kPairs * W; e += kWave) {
+        const int sg = kQ == 2 && e >= kPairs * W ? 1 : 0;
+        const int e1 = e - sg * (kPairs * W);
+        const int p = e1 / W, jj = e1 - p * W;
+        uint32_t clo, chi;
+        tab_codes(p, clo, chi);
+        uint32_t v = spre | (spre << 16);
+        if constexpr (!kLH) {  // one query: one code for both halves
+          const int k = j0 + jj - shv[0];
+          if (k >= 1) {
+            const uint32_t r = (uint32_t)(k - 1);
+            const uint32_t qc = (a.words[qwov[0] + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
+          }
+        } else {
+          // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
+          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
+          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
+          if (kl >= 1) {
+            const uint32_t r = (uint32_t)(kl - 1);
+            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
+          }
+          if (kh >= 1) {
+            const uint32_t r = (uint32_t)(kh - 1);
+            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+            const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
+            v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
+          }
+        }
+        tab[sg * kTab + p * RS + jj] = v;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+
+      f16x2 G[W];
+#pragma unroll
+      for (int jj = 0; jj < W; ++jj) G[jj] = g0;
+      f16x2 gdiag = g0;
+      auto do_row = [&](int j, uint32_t left_u, uint32_t paddr) {
+        const lds_u4 *prow = (const lds_u4 *)(uintptr_t)paddr;
+        f16x2 P[W];
+#pragma unroll
+        for (int w = 0; w < W / 4; ++w) {
+          const u32x4 v = prow[w];
+          P[4 * w + 0] = as_f16x2(v.x);
+          P[4 * w + 1] = as_f16x2(v.y);
+          P[4 * w + 2] = as_f16x2(v.z);
+          P[4 * w + 3] = as_f16x2(v.w);
+        }
+        const f16x2 gleft = as_f16x2(left_u);
+        dp_row_f16<W>(G, P, gleft, gdiag, gapv, best, best2);
+        ws.store(j, f16_u32(G[W - 1]));
+        gdiag = gleft;
+      };
+      // Two rows per iteration; their profile-row addresses and left
+      // boundaries are loaded one iteration ahead, each into the register
+      // whose previous value has just been used: a load that had to be
+      // copied into a loop-carried register would have to complete first.
+      // The loads are unconditional, keeping the iteration one basic block:
+      // boundary rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
+      // wave's maxlen + 2 boundary rows, and list word (j + 1) / 2 <=
+      // nrows / 2 inside its maxlen / 2 + 1 list rows; what is loaded past
+      // nrows is never used.
+      int j = 1;
+      for (; j + 1 <= nrows; j += 2) {
+        const uint32_t p0 = ra & 0xffffu, p1 = ra >> 16;
+        ra = ws.load(a.list_row + ((j + 1) >> 1));
+        __builtin_amdgcn_sched_barrier(0);  // issue the load here, a full iteration ahead
+        // the second row's profile reads after the first row's first
+        // column adds, not in one burst with the first row's
+        do_row(j, bA, p0);
+        if (!first) bA = ws.load(j + 2);
+        do_row(j + 1, bB, p1);
+        if (!first) bB = ws.load(j + 3);
+      }
+      if (j <= nrows) do_row(j, bA, ra & 0xffffu);
+    };
+    if (nstrips > 0 && nrows > 0) {
+      // strips are right-aligned: the first holds the rem = 1..S leftmost
+      // columns and runs in the narrowest multiple of 8 columns that holds
+      // them (S = 64: 8, 16, ..., 64; S = 32: 8, 16, 24, 32), so it pads
+      // fewer than 8
+      const int rem = ncols - (nstrips - 1) * S;
+      if constexpr (S == 64) {
+        switch ((rem + 7) >> 3) {
+          case 1: strip(Width<8>{}, rem - 8 + 1, std::true_type{}); break;
+          case 2: strip(Width<16>{}, rem - 16 + 1, std::true_type{}); break;
+          case 3: strip(Width<24>{}, rem - 24 + 1, std::true_type{}); break;
+          case 4: strip(Width<32>{}, rem - 32 + 1, std::true_type{}); break;
+          case 5: strip(Width<40>{}, rem - 40 + 1, std::true_type{}); break;
+          case 6: strip(Width<48>{}, rem - 48 + 1, std::true_type{}); break;
+          case 7: strip(Width<56>{}, rem - 56 + 1, std::true_type{}); break;
+          default: strip(Width<64>{}, rem - 64 + 1, std::true_type{}); break;
+        }
+      } else if constexpr (S == 32) {
+        if (rem <= S / 4) strip(Width<S / 4>{}, rem - S / 4 + 1, std::true_type{});
+        else if (rem <= S / 2) strip(Width<S / 2>{}, rem - S / 2 + 1, std::true_type{});
+        else if (rem <= 3 * S / 4) strip(Width<3 * S / 4>{}, rem - 3 * S / 4 + 1, std::true_type{});
+        else strip(Width<S>{}, rem - S + 1, std::true_type{});
+      } else {
+        strip(Width<S>{}, rem - S + 1, std::true_type{});
+      }
+      for (int s = 1; s < nstrips; ++s) strip(Width<S>{}, rem + (s - 1) * S + 1, std::false_type{});
+    }
+    best = __builtin_elementwise_maximum(best, best2);
+    // back to integers: H = 2048 * value; corner H = G + gap at row lb-1
+    const float sc = 2048.0f;
+    int best_lo = (int)__builtin_rintf((float)best.x * sc);
+    int best_hi = (int)__builtin_rintf((float)best.y * sc);
+    int cor_lo = 0, cor_hi = 0;
+    if (nstrips > 0 && nrows > 0) {
+      if (lb_lo >= 2) {
+        const f16x2 gv = as_f16x2(ws.load_lane_row(lb_lo - 1));
+        cor_lo = (int)__builtin_rintf((float)gv.x * sc) + a.gap;
+      }
+      if (lb_hi >= 2) {
+        const f16x2 gv = as_f16x2(ws.load_lane_row(lb_hi - 1));
+        cor_hi = (int)__builtin_rintf((float)gv.y * sc) + a.gap;
+      }
+    }
+    finish_query_int<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
+                     best_lo, best_hi, cor_lo, cor_hi, q_hi, la_hi);
+  }
+}
+
+// The persistent kernel: each wave takes work items from the queue.
 template <int S, bool kG = false, int kQ = 1, bool kLH = false>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_f16_kernel(KArgs a) {
   static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
@@ -953,206 +1176,42 @@ __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_
   uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
   const WaveRows ws(bnd, a.bound_stride, lane);
   static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
-  const int seg = kQ == 2 ? lane >> 5 : 0;  // the lane's table
-  const uint32_t tab_lds = (uint32_t)(uintptr_t)(lds_u32 *)tab + (uint32_t)seg * (kTab * 4u);
-
-  const _Float16 gap_h = (_Float16)((float)a.gap * kF16Scale);
-  const f16x2 gapv = {(_Float16)-gap_h, (_Float16)-gap_h};
-  const f16x2 g0 = gapv;  // G of an H = 0 cell
-  const f16x2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
-  // substitution scores plus the gap (G stores H - gap), scaled
-  const uint32_t smat = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)((float)(a.match + a.gap) * kF16Scale));
-  const uint32_t smis = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)((float)(a.mismatch + a.gap) * kF16Scale));
-  const uint32_t spre = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)-60000.0f);
-
   for (;;) {
     int item_idx = 0;
     if (lane == 0) item_idx = atomicAdd(a.counter, 1);
     item_idx = __builtin_amdgcn_readfirstlane(item_idx);
     if (item_idx >= a.n_items) break;
-    const int4 it = load_item<kG>(a, item_idx);
-    const int g = __builtin_amdgcn_readfirstlane(it.x);
-    const int qb = __builtin_amdgcn_readfirstlane(it.y);
-    const int qe = __builtin_amdgcn_readfirstlane(it.z);
-    const int ob = __builtin_amdgcn_readfirstlane(it.w);
+    sw_lds_f16_item<S, kG, kQ, kLH>(a, tab, ws, lane, load_item<kG>(a, item_idx));
+  }
+}
 
-    const int gm = __builtin_amdgcn_readfirstlane(a.gmax[g]);
-    const int t_lo = a.gtgt[(size_t)g * kGroup + lane];
-    const int t_hi = a.gtgt[(size_t)g * kGroup + kWave + lane];
-    const int lb_lo = t_lo >= 0 ? a.len[t_lo] : 0;
-    const int lb_hi = t_hi >= 0 ? a.len[t_hi] : 0;
-    const int rows_lo = t_lo >= 0 ? lb_lo - 1 : gm;
-    const int rows_hi = t_hi >= 0 ? lb_hi - 1 : gm;
-    const uint32_t wo_lo = t_lo >= 0 ? a.woff[t_lo] : 0u;
-    const uint32_t wo_hi = t_hi >= 0 ? a.woff[t_hi] : 0u;
-    const int rk_lo = t_lo >= 0 ? a.rank[t_lo] : 0;
-    const int rk_hi = t_hi >= 0 ? a.rank[t_hi] : 0;
-    const int nrows = gm > 1 ? gm - 1 : 0;
-    build_row_list(a, ws, nrows, wo_lo, wo_hi, rows_lo, rows_hi, tab_lds, RS * 4u);
-
-    for (int qi = qb; qi < qe; qi += NQ) {
-      // the wave's queries: qv[u], its length, first word and left pre-padding
-      int qv[NQ], lav[NQ], shv[NQ];
-      uint32_t qwov[NQ];
-      int ncols = 0;
-#pragma unroll
-      for (int u = 0; u < NQ; ++u) {
-        qv[u] = __builtin_amdgcn_readfirstlane(u == 0 ? load_qid<kG>(a, qi) : a.qids[qi + u]);
-        lav[u] = __builtin_amdgcn_readfirstlane(a.len[qv[u]]);
-        qwov[u] = __builtin_amdgcn_readfirstlane(a.woff[qv[u]]);
-        ncols = max(ncols, lav[u] > 1 ? lav[u] - 1 : 0);
-      }
-#pragma unroll
-      for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
-      // the lane's queries: table seg's, low halves q, high halves q_hi
-      const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
-      const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
-      const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
-      const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
-      const int nstrips = (ncols + S - 1) / S;
-      f16x2 best = zero2, best2 = zero2;
-
-      // One strip of W query columns whose slot 0 is column j0 (columns < 1
-      // are pre-padding); first (compile-time): no strip to its left.
-      auto strip = [&](auto wc, int j0, auto firstc) {
-        constexpr int W = decltype(wc)::value;
-        constexpr bool first = decltype(firstc)::value;
-        // The first list word and left boundaries are loaded before the
-        // table build, whose own loads are waited for, so nothing is pending
-        // when the row loop starts: the loop's waits then count only the
-        // loads and stores of one iteration.
-        uint32_t ra = ws.load(a.list_row);
-        uint32_t bA = f16_u32(g0), bB = f16_u32(g0);
-        if (!first) {
-          bA = ws.load(1);
-          if (nrows >= 2) bB = ws.load(2);
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (int e = lane; e < kQ * kPairs * W; e += kWave) {
-          const int sg = kQ == 2 && e >= kPairs * W ? 1 : 0;
-          const int e1 = e - sg * (kPairs * W);
-          const int p = e1 / W, jj = e1 - p * W;
-          uint32_t clo, chi;
-          tab_codes(p, clo, chi);
-          uint32_t v = spre | (spre << 16);
-          if constexpr (!kLH) {  // one query: one code for both halves
-            const int k = j0 + jj - shv[0];
-            if (k >= 1) {
-              const uint32_t r = (uint32_t)(k - 1);
-              const uint32_t qc = (a.words[qwov[0] + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-              v = (clo == qc ? smat : smis) | ((chi == qc ? smat : smis) << 16);
-            }
-          } else {
-            // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
-            const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
-            const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
-            if (kl >= 1) {
-              const uint32_t r = (uint32_t)(kl - 1);
-              const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
-              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-              v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
-            }
-            if (kh >= 1) {
-              const uint32_t r = (uint32_t)(kh - 1);
-              const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
-              const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
-              v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
-            }
-          }
-          tab[sg * kTab + p * RS + jj] = v;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-
-        f16x2 G[W];
-#pragma unroll
-        for (int jj = 0; jj < W; ++jj) G[jj] = g0;
-        f16x2 gdiag = g0;
-        auto do_row = [&](int j, uint32_t left_u, uint32_t paddr) {
-          const lds_u4 *prow = (const lds_u4 *)(uintptr_t)paddr;
-          f16x2 P[W];
-#pragma unroll
-          for (int w = 0; w < W / 4; ++w) {
-            const u32x4 v = prow[w];
-            P[4 * w + 0] = as_f16x2(v.x);
-            P[4 * w + 1] = as_f16x2(v.y);
-            P[4 * w + 2] = as_f16x2(v.z);
-            P[4 * w + 3] = as_f16x2(v.w);
-          }
-          const f16x2 gleft = as_f16x2(left_u);
-          dp_row_f16<W>(G, P, gleft, gdiag, gapv, best, best2);
-          ws.store(j, f16_u32(G[W - 1]));
-          gdiag = gleft;
-        };
-        // Two rows per iteration; their profile-row addresses and left
-        // boundaries are loaded one iteration ahead, each into the register
-        // whose previous value has just been used: a load that had to be
-        // copied into a loop-carried register would have to complete first.
-        // The loads are unconditional, keeping the iteration one basic block:
-        // boundary rows j + 2, j + 3 <= nrows + 2 <= maxlen + 1 stay inside the
-        // wave's maxlen + 2 boundary rows, and list word (j + 1) / 2 <=
-        // nrows / 2 inside its maxlen / 2 + 1 list rows; what is loaded past
-        // nrows is never used.
-        int j = 1;
-        for (; j + 1 <= nrows; j += 2) {
-          const uint32_t p0 = ra & 0xffffu, p1 = ra >> 16;
-          ra = ws.load(a.list_row + ((j + 1) >> 1));
-          __builtin_amdgcn_sched_barrier(0);  // issue the load here, a full iteration ahead
-          // the second row's profile reads after the first row's first
-          // column adds, not in one burst with the first row's
-          do_row(j, bA, p0);
-          if (!first) bA = ws.load(j + 2);
-          do_row(j + 1, bB, p1);
-          if (!first) bB = ws.load(j + 3);
-        }
-        if (j <= nrows) do_row(j, bA, ra & 0xffffu);
-      };
-      if (nstrips > 0 && nrows > 0) {
-        // strips are right-aligned: the first holds the rem = 1..S leftmost
-        // columns and runs in the narrowest multiple of 8 columns that holds
-        // them (S = 64: 8, 16, ..., 64; S = 32: 8, 16, 24, 32), so it pads
-        // fewer than 8
-        const int rem = ncols - (nstrips - 1) * S;
-        if constexpr (S == 64) {
-          switch ((rem + 7) >> 3) {
-            case 1: strip(Width<8>{}, rem - 8 + 1, std::true_type{}); break;
-            case 2: strip(Width<16>{}, rem - 16 + 1, std::true_type{}); break;
-            case 3: strip(Width<24>{}, rem - 24 + 1, std::true_type{}); break;
-            case 4: strip(Width<32>{}, rem - 32 + 1, std::true_type{}); break;
-            case 5: strip(Width<40>{}, rem - 40 + 1, std::true_type{}); break;
-            case 6: strip(Width<48>{}, rem - 48 + 1, std::true_type{}); break;
-            case 7: strip(Width<56>{}, rem - 56 + 1, std::true_type{}); break;
-            default: strip(Width<64>{}, rem - 64 + 1, std::true_type{}); break;
-          }
-        } else if constexpr (S == 32) {
-          if (rem <= S / 4) strip(Width<S / 4>{}, rem - S / 4 + 1, std::true_type{});
-          else if (rem <= S / 2) strip(Width<S / 2>{}, rem - S / 2 + 1, std::true_type{});
-          else if (rem <= 3 * S / 4) strip(Width<3 * S / 4>{}, rem - 3 * S / 4 + 1, std::true_type{});
-          else strip(Width<S>{}, rem - S + 1, std::true_type{});
-        } else {
-          strip(Width<S>{}, rem - S + 1, std::true_type{});
-        }
-        for (int s = 1; s < nstrips; ++s) strip(Width<S>{}, rem + (s - 1) * S + 1, std::false_type{});
-      }
-      best = __builtin_elementwise_maximum(best, best2);
-      // back to integers: H = 2048 * value; corner H = G + gap at row lb-1
-      const float sc = 2048.0f;
-      int best_lo = (int)__builtin_rintf((float)best.x * sc);
-      int best_hi = (int)__builtin_rintf((float)best.y * sc);
-      int cor_lo = 0, cor_hi = 0;
-      if (nstrips > 0 && nrows > 0) {
-        if (lb_lo >= 2) {
-          const f16x2 gv = as_f16x2(ws.load_lane_row(lb_lo - 1));
-          cor_lo = (int)__builtin_rintf((float)gv.x * sc) + a.gap;
-        }
-        if (lb_hi >= 2) {
-          const f16x2 gv = as_f16x2(ws.load_lane_row(lb_hi - 1));
-          cor_hi = (int)__builtin_rintf((float)gv.y * sc) + a.gap;
-        }
-      }
-      finish_query_int<kG>(a, lane, q, qi - qb, ob, la, t_lo, t_hi, lb_lo, lb_hi, rk_lo, rk_hi,
-                       best_lo, best_hi, cor_lo, cor_hi, q_hi, la_hi);
-    }
+// One launch for the items of both 16-bit lane formats (epsilon mode, item
+// lists, one query per wave): it.w holds the item's form, so the items that
+// need the int16 lanes and those the exact fp16 form covers share one
+// persistent queue in one cost order, with no tail between two launches.
+// Both item bodies use the same table layout (25 rows of S + 4 words).
+constexpr int kFormF16 = 0, kFormI16 = 1;
+template <int S>
+__global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_mixed_kernel(KArgs a) {
+  constexpr int kTab = 25 * (S + 4);
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ __attribute__((aligned(16))) uint32_t tab_all[kWaves * kTab];
+  static_assert(sizeof(tab_all) <= 65536, "profile-row addresses are 16-bit");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = blockIdx.x * kWaves + wv;
+  uint32_t *tab = tab_all + wv * kTab;
+  uint32_t *bnd = a.bound + (size_t)wave * a.bound_stride;
+  const WaveRows ws(bnd, a.bound_stride, lane);
+  for (;;) {
+    int item_idx = 0;
+    if (lane == 0) item_idx = atomicAdd(a.counter, 1);
+    item_idx = __builtin_amdgcn_readfirstlane(item_idx);
+    if (item_idx >= a.n_items) break;
+    const int4 it = a.items[item_idx];
+    const int4 it0 = make_int4(it.x, it.y, it.z, 0);
+    if (__builtin_amdgcn_readfirstlane(it.w) == kFormI16) sw_lds_item<S, false, 1, false>(a, tab, ws, lane, it0);
+    else sw_lds_f16_item<S, false, 1, false>(a, tab, ws, lane, it0);
   }
 }
 
@@ -1310,7 +1369,7 @@ void *kernel_ptr_g(int k, int S, bool acgt) {
 // their target rows once per item, len / 2 + 1 rows of the row-address list
 // after them (first list row: len + 2).
 int wave_rows(int len, int k) {
-  return len + 2 + (k == KERNEL_LDS_F16 || k == KERNEL_LDS ? len / 2 + 1 : 0);
+  return len + 2 + (k == KERNEL_LDS_F16 || k == KERNEL_LDS || k == KERNEL_LDS_MIXED ? len / 2 + 1 : 0);
 }
 
 // Kernel of variant k at strip S; group_items: the allocation-free
@@ -1318,6 +1377,12 @@ int wave_rows(int len, int k) {
 // two / four queries per wave (the lanes' low and high halves on different
 // queries, four with one table per half-wave; strips 32 and 64).
 void *kernel_ptr(int k, int S, bool acgt, bool group_items = false, int split = 1) {
+  if (k == KERNEL_LDS_MIXED) {
+    if (group_items || split != 1) return nullptr;
+    if (S == 16) return reinterpret_cast<void *>(&sw_lds_mixed_kernel<16>);
+    return S == 64 ? reinterpret_cast<void *>(&sw_lds_mixed_kernel<64>)
+                   : reinterpret_cast<void *>(&sw_lds_mixed_kernel<32>);
+  }
   if (split == 3 || split == 4) {
     if (group_items || (k != KERNEL_LDS_F16 && k != KERNEL_LDS)) return nullptr;
     if (k == KERNEL_LDS_F16) {
@@ -1449,6 +1514,7 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     std::vector<int4> items;
     double cells = 0;
     int split = 1;
+    double form_cells[2] = {0, 0};  // KERNEL_LDS_MIXED: cells of its fp16 / int16 items
   };
   std::vector<Pass> passes;
   // an item's longest query; with the length order, its first
@@ -1478,6 +1544,11 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     // most match * min(la, lb), so items whose bound exceeds the 16-bit range
     // run the 32-bit kernel; with the fp16 form, items whose bound is at most
     // 2048 run exactly in fp16 and the rest in packed int16
+    // With the fp16 form in epsilon mode, its items and the int16 ones run
+    // in one launch (sw_lds_mixed_kernel, the form in it.w) in the work
+    // list's cost order; MNHN_SW_MIXED=0 gives each form a launch of its own.
+    const char *mx = getenv("MNHN_SW_MIXED");
+    const bool mixed = K == KERNEL_LDS_F16 && mode == MODE_EPS && !(mx && atoi(mx) == 0);
     std::vector<int4> f16, i16, i32;
     f16.reserve(w.items.size());
     for (const int4 &it : w.items) {
@@ -1485,10 +1556,11 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
       const int maxla = item_max_query(it);
       const long long bound = (long long)c->match * std::min(gm, maxla);
       if (bound > kI16ScoreMax) i32.push_back(it);
+      else if (mixed) f16.push_back(make_int4(it.x, it.y, it.z, bound <= 2048 ? kFormF16 : kFormI16));
       else if (K == KERNEL_LDS_F16 && bound <= 2048) f16.push_back(it);
       else i16.push_back(it);
     }
-    if (!f16.empty()) passes.push_back({KERNEL_LDS_F16, S, std::move(f16)});
+    if (!f16.empty()) passes.push_back({mixed ? KERNEL_LDS_MIXED : KERNEL_LDS_F16, S, std::move(f16)});
     if (!i16.empty()) passes.push_back({K == KERNEL_LDS_F16 ? KERNEL_LDS : K, S, std::move(i16)});
     if (!i32.empty()) passes.push_back({KERNEL_I32, kI32Strip, std::move(i32)});
   }
@@ -1556,7 +1628,11 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     if (w.dev) {
       ps.cells = w.cells;
     } else {
-      for (const int4 &it : ps.items) ps.cells += item_cells(it);
+      for (const int4 &it : ps.items) {
+        const double ci = item_cells(it);
+        ps.cells += ci;
+        if (ps.kernel == KERNEL_LDS_MIXED) ps.form_cells[it.w == kFormI16 ? 1 : 0] += ci;
+      }
     }
   }
   if ((int)passes.size() * 2 > (int)(sizeof(c->pev) / sizeof(c->pev[0]))) {
@@ -1652,7 +1728,10 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
     // the next pass reuses d_items: wait until this one has read them
     if (passes.size() > 1) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) break;
-    clk.mark(PK == KERNEL_LDS_F16 ? "pass (fp16 form)" : PK == KERNEL_LDS ? "pass (int16 form)" : "pass");
+    clk.mark(PK == KERNEL_LDS_F16    ? "pass (fp16 form)"
+             : PK == KERNEL_LDS       ? "pass (int16 form)"
+             : PK == KERNEL_LDS_MIXED ? "pass (fp16 and int16 forms)"
+                                      : "pass");
   }
   (void)hipEventRecord(c->ev[1], c->stream);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
@@ -1661,10 +1740,18 @@ int mnhn_detail::launch_tiles(mnhn_sw_ctx *c, const Work &w, int mode, int trian
   cleanup();
   if (e != hipSuccess) return MNHN_ERR_DEVICE;
   for (int f = 0; f < 3; ++f) c->last.form_cells[f] = c->last.form_ms[f] = 0;
+  c->last.mixed_launch = 0;
   for (size_t pi = 0; pi < passes.size(); ++pi) {
-    const int f = passes[pi].kernel == KERNEL_LDS_F16 ? 0 : passes[pi].kernel == KERNEL_I32 ? 2 : 1;
     float pms = 0.f;
     (void)hipEventElapsedTime(&pms, c->pev[2 * pi], c->pev[2 * pi + 1]);
+    if (passes[pi].kernel == KERNEL_LDS_MIXED) {  // one launch: its time goes to form 0
+      c->last.form_cells[0] += passes[pi].form_cells[0];
+      c->last.form_cells[1] += passes[pi].form_cells[1];
+      c->last.form_ms[0] += pms;
+      c->last.mixed_launch = 1;
+      continue;
+    }
+    const int f = passes[pi].kernel == KERNEL_LDS_F16 ? 0 : passes[pi].kernel == KERNEL_I32 ? 2 : 1;
     c->last.form_cells[f] += passes[pi].cells;
     c->last.form_ms[f] += pms;
   }

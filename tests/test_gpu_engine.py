@@ -154,6 +154,39 @@ def test_eps_csr_length_pruning_is_exact(pkg, oracle, monkeypatch, eps):
     assert np.array_equal(got["1"][1], expect)
 
 
+@pytest.mark.parametrize("strip", [None, 32])
+def test_eps_csr_one_launch_for_both_forms(pkg, oracle, monkeypatch, strip):
+    """Lengths on both sides of 512 residues put items of the exact fp16 form
+    and of the int16 form in one epsilon pass: by default they run as one
+    launch (the form per item), with MNHN_SW_MIXED=0 as a launch per form;
+    both give the oracle's lists, and the timing reports the one launch with
+    the cells of each form."""
+    rng = np.random.default_rng(77)
+    seqs = _rand_set(rng, 120, 300, 800)
+    base = _rand_set(rng, 4, 400, 750, b"ACGT")
+    for b in base:
+        for _ in range(15):
+            seqs.append(synth.to_bytes(synth.mutate(
+                rng, np.frombuffer(b, np.uint8).copy() % 4, 0.03, 0.01)))
+    n, eps = len(seqs), 1200
+    _say(f"eps csr one launch n={n} eps={eps} strip={strip}")
+    want = oracle.sw_matrix(seqs, nthreads=16)
+    expect = np.where(want <= eps, want, -1)
+    np.fill_diagonal(expect, -1)
+    with pkg.Engine(seqs) as eng:
+        if strip:
+            eng.set_tuning(3, strip, 0)
+        for mixed in ("1", "0"):
+            monkeypatch.setenv("MNHN_SW_MIXED", mixed)
+            rp, col, dist = eng.eps_csr(eps)
+            t = eng.last_timing()
+            assert np.array_equal(_csr_to_dense(rp, col, dist, n), expect)
+            assert t["mixed_launch"] == (1 if mixed == "1" else 0)
+            assert t["form_cells"][0] > 0 and t["form_cells"][1] > 0
+            if mixed == "1":
+                assert t["form_ms"][1] == 0 and t["form_ms"][0] > 0
+
+
 def test_families_200bp_full_matrix_and_eps(pkg, oracle):
     seqs = synth.families_200bp(seed=1)
     n = len(seqs)

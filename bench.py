@@ -142,7 +142,7 @@ def profiled_traffic(config, strip, key="traffic_bytes_launch"):
     launch that runs the fp16 and int16 forms' items), `traffic_bytes_per_step`
     = every SW launch of the step; None if there is no summary for it or it
     was taken at another strip width than the step ran (`traffic_strip`)."""
-    path = os.path.join(ROOT, "profiles", {"tree20k": "sw_lds_f16_kernel_tree20k.md"}.get(config, "-"))
+    path = os.path.join(ROOT, "profiles", {"tree20k": "sw_lds_mixed_kernel_tree20k.md"}.get(config, "-"))
     vals = {}
     try:
         with open(path) as f:

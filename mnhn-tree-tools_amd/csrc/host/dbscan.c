@@ -960,6 +960,10 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   int *nb = (int *)malloc(sizeof(int) * (size_t)n);
   int *queue = (int *)malloc(sizeof(int) * (size_t)n);
   int *rest = (int *)malloc(sizeof(int) * (size_t)n); /* points outside the union, ascending */
+  /* the members of the cluster being expanded; copied out at its exact size
+   * when it is done (one n-int buffer per cluster would cost a page mapping
+   * and a shrinking realloc each) */
+  int *grow = (int *)malloc(sizeof(int) * (size_t)n);
   int *tmp = ix ? (int *)malloc(sizeof(int) * (size_t)n) : NULL;
   triggers tg = {NULL, NULL, NULL, 0};
   const int use_triggers = ix && ix->ix->symmetric;
@@ -991,7 +995,7 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
     const int mark = id + 1;
     cluster *cl = &ret.clusters[id];
     cl->id = id;
-    cl->members = (int *)malloc(sizeof(int) * (size_t)n);
+    cl->members = grow;
     cl->n_members = 0;
     cl->members[cl->n_members++] = i;
     member[i] = 1;
@@ -1103,7 +1107,13 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
         member[q] = 1;
       }
     }
-    cl->members = (int *)realloc(cl->members, sizeof(int) * (size_t)(cl->n_members ? cl->n_members : 1));
+    cl->members = (int *)malloc(sizeof(int) * (size_t)(cl->n_members ? cl->n_members : 1));
+    if (!cl->members) {
+      printf("out of memory storing a cluster\n");
+      fflush(stdout);
+      _exit(1);
+    }
+    memcpy(cl->members, grow, sizeof(int) * (size_t)cl->n_members);
     ret.n_clusters++;
   }
   free(visited);
@@ -1112,6 +1122,7 @@ static split_set dbscan_core(int n, float epsilon, int minpts, region_fn rq, voi
   free(nb);
   free(queue);
   free(rest);
+  free(grow);
   free(tmp);
   free(tg.px);
   free(tg.rp);

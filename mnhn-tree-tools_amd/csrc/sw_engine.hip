@@ -688,6 +688,8 @@ __device__ __forceinline__ void sw_lds_item(const KArgs &a, uint32_t *tab, const
   static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
   constexpr int kPer = kLH ? 2 : 1;  // queries per table
   constexpr int NQ = kQ * kPer;      // queries per wave
+  // queries of the second half-wave's table (kQ = 2): low halves, high halves
+  constexpr int kSegLo = kQ == 2 ? kPer : 0, kSegHi = kQ == 2 ? NQ - 1 : kPer - 1;
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
   constexpr int kTab = kPairs * RS;  // words of one query's table
@@ -734,10 +736,10 @@ __device__ __forceinline__ void sw_lds_item(const KArgs &a, uint32_t *tab, const
 #pragma unroll
     for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
     // the lane's queries: table seg's, low halves q, high halves q_hi
-    const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
-    const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
-    const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
-    const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
+    const int q = (kQ == 2 && seg) ? qv[kSegLo] : qv[0];
+    const int q_hi = (kQ == 2 && seg) ? qv[kSegHi] : qv[kPer - 1];
+    const int la = (kQ == 2 && seg) ? lav[kSegLo] : lav[0];
+    const int la_hi = (kQ == 2 && seg) ? lav[kSegHi] : lav[kPer - 1];
     const int nstrips = (ncols + S - 1) / S;
     s16x2 best = floor2, best2 = floor2;
     s16x2 corner = floor2;
@@ -774,17 +776,17 @@ __device__ __forceinline__ void sw_lds_item(const KArgs &a, uint32_t *tab, const
           }
         } else {
           // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
-          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
-          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
+          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kSegLo] : shv[0]);
+          const int kh = j0 + jj - (kQ == 2 && sg ? shv[kSegHi] : shv[kPer - 1]);
           if (kl >= 1) {
             const uint32_t r = (uint32_t)(kl - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+            const uint32_t wo = kQ == 2 && sg ? qwov[kSegLo] : qwov[0];
             const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
             v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
           }
           if (kh >= 1) {
             const uint32_t r = (uint32_t)(kh - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+            const uint32_t wo = kQ == 2 && sg ? qwov[kSegHi] : qwov[kPer - 1];
             const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
             v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
           }
@@ -858,8 +860,6 @@ template <int S, bool kG = false, int kQ = 1, bool kLH = false>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_kernel(KArgs a) {
   static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
   static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
-  constexpr int kPer = kLH ? 2 : 1;  // queries per table
-  constexpr int NQ = kQ * kPer;      // queries per wave
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;  // table row stride in words; the +4 shifts banks per row
   constexpr int kTab = kPairs * RS;  // words of one query's table
@@ -956,6 +956,8 @@ __device__ __forceinline__ void sw_lds_f16_item(const KArgs &a, uint32_t *tab, c
   static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
   constexpr int kPer = kLH ? 2 : 1;  // queries per table
   constexpr int NQ = kQ * kPer;      // queries per wave
+  // queries of the second half-wave's table (kQ = 2): low halves, high halves
+  constexpr int kSegLo = kQ == 2 ? kPer : 0, kSegHi = kQ == 2 ? NQ - 1 : kPer - 1;
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;
   constexpr int kTab = kPairs * RS;  // words of one table
@@ -1006,10 +1008,10 @@ __device__ __forceinline__ void sw_lds_f16_item(const KArgs &a, uint32_t *tab, c
 #pragma unroll
     for (int u = 0; u < NQ; ++u) shv[u] = ncols - (lav[u] > 1 ? lav[u] - 1 : 0);
     // the lane's queries: table seg's, low halves q, high halves q_hi
-    const int q = (kQ == 2 && seg) ? qv[kPer] : qv[0];
-    const int q_hi = (kQ == 2 && seg) ? qv[NQ - 1] : qv[kPer - 1];
-    const int la = (kQ == 2 && seg) ? lav[kPer] : lav[0];
-    const int la_hi = (kQ == 2 && seg) ? lav[NQ - 1] : lav[kPer - 1];
+    const int q = (kQ == 2 && seg) ? qv[kSegLo] : qv[0];
+    const int q_hi = (kQ == 2 && seg) ? qv[kSegHi] : qv[kPer - 1];
+    const int la = (kQ == 2 && seg) ? lav[kSegLo] : lav[0];
+    const int la_hi = (kQ == 2 && seg) ? lav[kSegHi] : lav[kPer - 1];
     const int nstrips = (ncols + S - 1) / S;
     f16x2 best = zero2, best2 = zero2;
 
@@ -1045,17 +1047,17 @@ __device__ __forceinline__ void sw_lds_f16_item(const KArgs &a, uint32_t *tab, c
           }
         } else {
           // table sg: low halves score query sg * kPer, high halves sg * kPer + 1
-          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kPer] : shv[0]);
-          const int kh = j0 + jj - (kQ == 2 && sg ? shv[NQ - 1] : shv[kPer - 1]);
+          const int kl = j0 + jj - (kQ == 2 && sg ? shv[kSegLo] : shv[0]);
+          const int kh = j0 + jj - (kQ == 2 && sg ? shv[kSegHi] : shv[kPer - 1]);
           if (kl >= 1) {
             const uint32_t r = (uint32_t)(kl - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[kPer] : qwov[0];
+            const uint32_t wo = kQ == 2 && sg ? qwov[kSegLo] : qwov[0];
             const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
             v = (v & 0xffff0000u) | (clo == qc ? smat : smis);
           }
           if (kh >= 1) {
             const uint32_t r = (uint32_t)(kh - 1);
-            const uint32_t wo = kQ == 2 && sg ? qwov[NQ - 1] : qwov[kPer - 1];
+            const uint32_t wo = kQ == 2 && sg ? qwov[kSegHi] : qwov[kPer - 1];
             const uint32_t qc = (a.words[wo + (r >> 4)] >> (2u * (r & 15u))) & 3u;
             v = (v & 0xffffu) | ((chi == qc ? smat : smis) << 16);
           }
@@ -1161,8 +1163,6 @@ template <int S, bool kG = false, int kQ = 1, bool kLH = false>
 __global__ __launch_bounds__(kBlock, S == 64 ? 3 : S == 32 ? 5 : 1) void sw_lds_f16_kernel(KArgs a) {
   static_assert((kQ == 1 && !kLH) || !kG, "several queries per wave: item lists only");
   static_assert(kQ == 1 || (kQ == 2 && kLH), "two tables per wave: four queries");
-  constexpr int kPer = kLH ? 2 : 1;  // queries per table
-  constexpr int NQ = kQ * kPer;      // queries per wave
   constexpr int kPairs = 25;
   constexpr int RS = S + 4;
   constexpr int kTab = kPairs * RS;  // words of one table

@@ -1482,6 +1482,7 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
           *ibase = nullptr, *seg = nullptr, *sorted = nullptr, *kband = nullptr, *hgs = nullptr, *qids2 = nullptr;
   unsigned long long *tlen = nullptr, *pcells = nullptr;
   int32_t *gparts = nullptr;
+  unsigned char *garena = nullptr;  // one allocation for the group tables beside gtgt
   long long *d_rs = nullptr;
   int4 *items = nullptr, *sitems = nullptr;
   int2 *d_tiles = nullptr;
@@ -1490,10 +1491,13 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (Xs) (void)hipFree(Xs);
     Xs = nullptr;
     dev_free(d_over); dev_free(cmin); dev_free(pnorm);
-    dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt); dev_free(gmax);
-    dev_free(gmin); dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_tiles); dev_free(items);
-    dev_free(ibase); dev_free(tlen); dev_free(seg); dev_free(sorted); dev_free(d_rs); dev_free(kband);
-    dev_free(hgs); dev_free(qids2); dev_free(sitems); dev_free(gparts); dev_free(pcells);
+    dev_free(d_cnt64); dev_free(rowcnt); dev_free(gbase); dev_free(cursor); dev_free(gtgt);
+    dev_free(qids); dev_free(d_order); dev_free(lens); dev_free(d_tiles);
+    dev_free(seg); dev_free(sorted); dev_free(d_rs); dev_free(kband);
+    dev_free(garena);  // gmax, gmin, items, ibase, tlen, hgs, qids2, sitems, gparts, pcells
+    gmax = gmin = ibase = hgs = qids2 = gparts = nullptr;
+    items = sitems = nullptr;
+    tlen = pcells = nullptr;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(ek0);
@@ -1881,18 +1885,39 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
         }
     }
     clk.mark("host group layout");
+    // gtgt on its own; the other group tables are slices of one allocation,
+    // the five that come from the host first (one copy through the
+    // context's pinned staging)
+    size_t off_arena = 0;
+    auto slice = [&](size_t bytes) {
+      const size_t o = off_arena;
+      off_arena += (bytes + 255) / 256 * 256;
+      return o;
+    };
+    const size_t o_ibase = slice(3 * (size_t)n * 4), o_hgs = slice(4 * (size_t)n * 4),
+                 o_qids2 = slice(std::max<size_t>(q2.size(), 1) * 4), o_gparts = slice(std::max(np, 1) * 4ull),
+                 o_sitems = slice(std::max(np, 1) * sizeof(int4));
+    const size_t up_bytes = off_arena;  // the uploaded part
+    const size_t o_gmax = slice((size_t)G * 4), o_gmin = slice((size_t)G * 4),
+                 o_items = slice((size_t)std::max(Gf, 1) * sizeof(int4)), o_tlen = slice(7 * (size_t)n * 8),
+                 o_pcells = slice((size_t)kParts * 8);
     auto group_buffers = [&]() -> int {
       int r;
-      if ((r = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (r = dev_alloc(&gmax, (size_t)G)) ||
-          (r = dev_alloc(&gmin, (size_t)G)) || (r = dev_alloc(&items, (size_t)std::max(Gf, 1))) ||
-          (r = dev_alloc(&ibase, 3 * (size_t)n)) || (r = dev_alloc(&tlen, 7 * (size_t)n)) ||
-          (r = dev_alloc(&hgs, 4 * (size_t)n)) || (r = dev_alloc(&qids2, std::max<size_t>(q2.size(), 1))) ||
-          (r = dev_alloc(&sitems, std::max<size_t>((size_t)np, 1))) ||
-          (r = dev_alloc(&gparts, std::max<size_t>((size_t)np, 1))) || (r = dev_alloc(&pcells, (size_t)kParts))) {
-        dev_free(gtgt); dev_free(gmax); dev_free(gmin); dev_free(items); dev_free(ibase); dev_free(tlen);
-        dev_free(hgs); dev_free(qids2); dev_free(sitems); dev_free(gparts); dev_free(pcells);
+      if ((r = dev_alloc(&gtgt, (size_t)G * kGroupSize)) || (r = dev_alloc(&garena, off_arena))) {
+        dev_free(gtgt);
+        dev_free(garena);
         return r;
       }
+      ibase = reinterpret_cast<int32_t *>(garena + o_ibase);
+      hgs = reinterpret_cast<int32_t *>(garena + o_hgs);
+      qids2 = reinterpret_cast<int32_t *>(garena + o_qids2);
+      gparts = reinterpret_cast<int32_t *>(garena + o_gparts);
+      sitems = reinterpret_cast<int4 *>(garena + o_sitems);
+      gmax = reinterpret_cast<int32_t *>(garena + o_gmax);
+      gmin = reinterpret_cast<int32_t *>(garena + o_gmin);
+      items = reinterpret_cast<int4 *>(garena + o_items);
+      tlen = reinterpret_cast<unsigned long long *>(garena + o_tlen);
+      pcells = reinterpret_cast<unsigned long long *>(garena + o_pcells);
       return MNHN_OK;
     };
     if ((rc = group_buffers()) && spare) {
@@ -1900,11 +1925,27 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
       rc = group_buffers();
     }
     if (rc) return fail(rc);
+    // the uploaded tables, packed as in the arena
+    if (c->h_items_bytes < up_bytes) {
+      if (c->h_items) (void)hipHostFree(c->h_items);
+      c->h_items = nullptr;
+      c->h_items_bytes = 0;
+      if (hipHostMalloc(&c->h_items, up_bytes, hipHostMallocDefault) == hipSuccess) c->h_items_bytes = up_bytes;
+      else c->h_items = nullptr;
+    }
+    std::vector<unsigned char> up_pageable;
+    unsigned char *up = static_cast<unsigned char *>(c->h_items);
+    if (!up) {
+      up_pageable.resize(up_bytes);
+      up = up_pageable.data();
+    }
+    memcpy(up + o_ibase, ib.data(), 3 * (size_t)n * 4);
+    memcpy(up + o_hgs, hg.data(), 4 * (size_t)n * 4);
+    if (!q2.empty()) memcpy(up + o_qids2, q2.data(), q2.size() * 4);
+    if (np > 0) memcpy(up + o_gparts, gpart.data(), (size_t)np * 4);
+    if (np > 0) memcpy(up + o_sitems, sit.data(), (size_t)np * sizeof(int4));
     e = hipMemcpyAsync(gbase, gb.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(ibase, ib.data(), 3 * (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hgs, hg.data(), 4 * (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(qids2, q2.data(), q2.size() * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(sitems, sit.data(), (size_t)np * sizeof(int4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(garena, up, up_bytes, hipMemcpyHostToDevice, st);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(stride_blocks((long long)G * kGroupSize, 256)), dim3(256), 0, st, gtgt,
                        (long long)G * kGroupSize, -1);
     if (np > 0) {
@@ -1920,7 +1961,6 @@ int mnhn_prefilter_eps_csr(mnhn_encoder *enc, mnhn_sw_ctx *c, int32_t eps_max, m
     if (e == hipSuccess) e = hipGetLastError();
     // cells of each part (band_class; shared pieces by their group's part)
     unsigned long long h_pc[kParts] = {};
-    if (e == hipSuccess && np > 0) e = hipMemcpyAsync(gparts, gpart.data(), (size_t)np * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(pcells, 0, kParts * 8, st);
     hipLaunchKernelGGL(part_cells_kernel, dim3(std::min(blocks_for(n, 256), 1024u)), dim3(256), 0, st, tlen,
                        c->d_len, hgs, gparts, Gf, n, S == 64 ? 128 : -1, pcells);

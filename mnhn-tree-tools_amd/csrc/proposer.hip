@@ -376,6 +376,16 @@ __global__ __launch_bounds__(256) void kmer_counts_sorted_kernel(const uint32_t 
   }
 }
 
+// Buffer resource over rows [r0, r0 + 128) of a row-major matrix with n rows
+// of row_bytes each.  Its record count ends at row n, so loads of rows past n
+// return zeros (the hardware's range check) and need no clamp; the 32-bit
+// offsets stay below 128 rows whatever n is.  0x00020000: the third word of a
+// raw buffer on gfx9-family parts.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rows_rsrc(const void *base, int r0, int n, int row_bytes) {
+  const char *p = static_cast<const char *>(base) + (size_t)r0 * row_bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(p), (short)0, min(n - r0, 128) * row_bytes, 0x00020000);
+}
+
 // acc += A[i0 .. i0+128) . B[j0 .. j0+128)^T over k in [0, K) (row stride K),
 // staged through LDS 64 k at a time; each wave owns a 64 x 64 sub-tile.
 // Global loads run two chunks ahead of the MFMAs (below).
@@ -389,23 +399,25 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 8 * (lane >> 4);
-  // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread.  Rows past
-  // n load row n - 1 instead (no branch): their products are never used, as
-  // the callers' tests reject every row or column at or beyond n.
+  // 128 rows x 64 k = 1024 chunks of 8 per operand; 4 per thread (thread
+  // tid takes rows tid / 8 + 32 t, k (tid % 8) * 8).  Rows past n read zeros
+  // (tile_rows_rsrc): their products are never used, as the callers' tests
+  // reject every row or column at or beyond n.  One 32-bit offset per thread;
+  // the row step and the chunk's k go into the scalar offset.
   // two register sets: chunk c + 2 is loaded while chunk c is multiplied
   // and chunk c + 1 (loaded one chunk earlier) goes into the other LDS
   // stage, so a load has two chunks of math and barriers to arrive.  Chunk k
   // travels in register set k & 1 and LDS stage k & 1.
-  constexpr int NC = K / KH;
+  constexpr int NC = K / KH, RB = K * 2;
   static_assert(K % KH == 0 && NC % 2 == 0, "whole pairs of chunks");
+  const __amdgpu_buffer_rsrc_t ra = tile_rows_rsrc(A, i0, n, RB), rb = tile_rows_rsrc(B, j0, n, RB);
+  const int voff = (tid >> 3) * RB + (tid & 7) * 16;
   bf16x8 va0[4], vb0[4], va1[4], vb1[4];
   auto load = [&](bf16x8 (&va)[4], bf16x8 (&vb)[4], int kh) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int ch = tid + 256 * t;
-      const int r = ch >> 3, k = (ch & 7) * 8;
-      va[t] = *reinterpret_cast<const bf16x8 *>(A + (size_t)min(i0 + r, n - 1) * K + kh + k);
-      vb[t] = *reinterpret_cast<const bf16x8 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
+      va[t] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(ra, voff, 32 * t * RB + 2 * kh, 0));
+      vb[t] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rb, voff, 32 * t * RB + 2 * kh, 0));
     }
   };
   auto store = [&](const bf16x8 (&va)[4], const bf16x8 (&vb)[4], int stage) {
@@ -457,7 +469,7 @@ __device__ __forceinline__ void tile_dot(const bf16 *__restrict__ A, const bf16 
 // (v_mfma_i32_16x16x64_i8): int32 sums, exact.  Chunks of 128 bytes per row
 // (two MFMA k-steps of 64) in the same LDS footprint, double buffering and
 // two-chunk-ahead loads as the bf16 form, so a chunk moves the same bytes for
-// twice the k.  Rows past n load row n - 1, as in tile_dot.
+// twice the k.  Rows past n read zeros, as in tile_dot.
 template <int K>
 __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const int8_t *__restrict__ B, int n,
                                             int i0, int j0, int8_t *sA, int8_t *sB, i32x4 (&acc)[4][4]) {
@@ -466,16 +478,17 @@ __device__ __forceinline__ void tile_dot_i8(const int8_t *__restrict__ A, const 
   const int wm = w >> 1, wn = w & 1;
   const int fr = lane & 15, fk = 16 * (lane >> 4);
   // 128 rows x 128 bytes = 1024 pieces of 16 bytes per operand; 4 per thread
+  // (rows tid / 8 + 32 t, bytes (tid % 8) * 16), addressed as in tile_dot
   constexpr int NC = K / KH;  // two register sets, as in tile_dot
   static_assert(K % KH == 0 && NC % 2 == 0, "whole pairs of chunks");
+  const __amdgpu_buffer_rsrc_t ra = tile_rows_rsrc(A, i0, n, K), rb = tile_rows_rsrc(B, j0, n, K);
+  const int voff = (tid >> 3) * K + (tid & 7) * 16;
   i32x4 va0[4], vb0[4], va1[4], vb1[4];  // native vectors: registers, not a stack object
   auto load = [&](i32x4 (&va)[4], i32x4 (&vb)[4], int kh) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int ch = tid + 256 * t;
-      const int r = ch >> 3, k = (ch & 7) * 16;
-      va[t] = *reinterpret_cast<const i32x4 *>(A + (size_t)min(i0 + r, n - 1) * K + kh + k);
-      vb[t] = *reinterpret_cast<const i32x4 *>(B + (size_t)min(j0 + r, n - 1) * K + kh + k);
+      va[t] = __builtin_bit_cast(i32x4, __builtin_amdgcn_raw_buffer_load_b128(ra, voff, 32 * t * K + kh, 0));
+      vb[t] = __builtin_bit_cast(i32x4, __builtin_amdgcn_raw_buffer_load_b128(rb, voff, 32 * t * K + kh, 0));
     }
   };
   auto store = [&](const i32x4 (&va)[4], const i32x4 (&vb)[4], int stage) {
@@ -680,23 +693,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
       vj[j] = pnorm[cc];
       uj[j] = c >= 0.0f ? c * vj[j] : -FLT_MAX;
     }
+    // off-diagonal tiles (kDiag false) take the row's bound throughout; the
+    // branch is uniform and taken once per tile
+    auto cos_test = [&](auto kDiag) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
-        const int cc = min(gi, n - 1);
-        const float vi = pnorm[cc];
-        const float c = gi < n ? cmin[cc] : -1.f;
-        const float ui = c >= 0.0f ? c * vi : -FLT_MAX;
+        for (int r = 0; r < 4; ++r) {
+          const int gi = i0 + 64 * wm + 16 * i + 4 * (lane >> 4) + r;
+          const int cc = min(gi, n - 1);
+          const float vi = pnorm[cc];
+          const float c = gi < n ? cmin[cc] : -1.f;
+          const float ui = c >= 0.0f ? c * vi : -FLT_MAX;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int bit = 16 * i + 4 * r + j;
-          // off-diagonal: the row's bound (mlong is all ones)
-          const float thr = diag && !((mlong >> bit) & 1ull) ? uj[j] * vi : ui * vj[j];
-          m2 &= accx[i][j][r] < thr ? ~(1ull << bit) : ~0ull;
+          for (int j = 0; j < 4; ++j) {
+            const int bit = 16 * i + 4 * r + j;
+            const float thr = decltype(kDiag)::value && !((mlong >> bit) & 1ull) ? uj[j] * vi : ui * vj[j];
+            m2 &= accx[i][j][r] < thr ? ~(1ull << bit) : ~0ull;
+          }
         }
-      }
+    };
+    if (diag) cos_test(std::integral_constant<bool, true>{});
+    else cos_test(std::integral_constant<bool, false>{});
   }
   // Emission: each lane's hit count, a wave-wide prefix sum of the counts,
   // one reservation per wave, then each lane writes its own hits (ascending

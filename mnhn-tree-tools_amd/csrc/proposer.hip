@@ -226,11 +226,12 @@ __global__ __launch_bounds__(256) void pair_loss_kernel(const float *__restrict_
   if (lane == 0) atomicAdd(loss, L);
 }
 
-// out[c] += sum_r dY[r][c] (out zeroed by the caller).  256 threads = 64
-// columns x 4 row lanes per block, 128 rows per block (grid.y), one atomic per
-// column per block.
+// part[y][c] = sum over the rows of block row y (128 rows) of dY[r][c].
+// 256 threads = 64 columns x 4 row lanes per block; each sum runs in a fixed
+// order, and col_reduce_kernel adds the blocks' partials in block order, so
+// the column sums (the bias gradients) are the same on every run.
 __global__ __launch_bounds__(256) void col_sum_kernel(const bf16 *__restrict__ dY, int R, int C,
-                                                      float *__restrict__ out) {
+                                                      float *__restrict__ part_out) {
   __shared__ float part[4][64];
   const int cl = threadIdx.x & 63, rl = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + cl;
@@ -240,7 +241,16 @@ __global__ __launch_bounds__(256) void col_sum_kernel(const bf16 *__restrict__ d
     for (int r = r0 + rl; r < r1; r += 4) s += (float)dY[(size_t)r * C + c];
   part[rl][cl] = s;
   __syncthreads();
-  if (rl == 0 && c < C) atomicAdd(&out[c], part[0][cl] + part[1][cl] + part[2][cl] + part[3][cl]);
+  if (rl == 0 && c < C) part_out[(size_t)blockIdx.y * C + c] = part[0][cl] + part[1][cl] + part[2][cl] + part[3][cl];
+}
+
+// out[c] = sum over y < nb of part[y][c], in order of y
+__global__ void col_reduce_kernel(const float *__restrict__ part, int nb, int C, float *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int y = 0; y < nb; ++y) s += part[(size_t)y * C + c];
+  out[c] = s;
 }
 
 // Adam with decoupled weight decay wd (0 for biases)
@@ -1241,12 +1251,13 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
   bf16 *Xb = nullptr, *XbT = nullptr, *demb = nullptr, *dembT = nullptr, *a1T = nullptr, *a2T = nullptr,
        *dp1 = nullptr, *dp1T = nullptr, *dp2 = nullptr, *dp2T = nullptr;
   int32_t *d_idx = nullptr;
-  float *d_tgt = nullptr, *d_rad = nullptr, *d_loss = nullptr;
+  float *d_tgt = nullptr, *d_rad = nullptr, *d_loss = nullptr, *d_colpart = nullptr;
+  const int col_blocks = (R + 127) / 128;  // row blocks of col_sum_kernel
   auto cleanup = [&]() {
     dev_free(X); dev_free(a.pre1); dev_free(a.pre2); dev_free(a.emb); dev_free(a.a1); dev_free(a.a2);
     dev_free(Xb); dev_free(XbT); dev_free(demb); dev_free(dembT); dev_free(a1T); dev_free(a2T);
     dev_free(dp1); dev_free(dp1T); dev_free(dp2); dev_free(dp2T); dev_free(d_idx); dev_free(d_tgt);
-    dev_free(d_rad); dev_free(d_loss);
+    dev_free(d_rad); dev_free(d_loss); dev_free(d_colpart);
   };
   if ((rc = dev_alloc(&a.pre1, (size_t)R * D1)) || (rc = dev_alloc(&a.pre2, (size_t)R * D2)) ||
       (rc = dev_alloc(&a.emb, (size_t)R * D3)) || (rc = dev_alloc(&a.a1, (size_t)R * D1)) ||
@@ -1257,10 +1268,18 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       (rc = dev_alloc(&dp1T, (size_t)R * D1)) || (rc = dev_alloc(&dp2, (size_t)R * D2)) ||
       (rc = dev_alloc(&dp2T, (size_t)R * D2)) || (rc = dev_alloc(&d_idx, (size_t)R)) ||
       (rc = dev_alloc(&d_tgt, (size_t)B)) || (rc = dev_alloc(&d_rad, (size_t)B)) ||
-      (rc = dev_alloc(&d_loss, 1))) {
+      (rc = dev_alloc(&d_loss, 1)) ||
+      (rc = dev_alloc(&d_colpart, (size_t)col_blocks * std::max(D1, std::max(D2, D3))))) {
     cleanup();
     return rc;
   }
+  // bias gradient: column sums of dY [R x C] into gb, in a fixed order
+  auto col_sums = [&](const bf16 *dY, int C, float *gb) {
+    hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((C + 63) / 64), (unsigned)col_blocks), dim3(256), 0, st,
+                       dY, R, C, d_colpart);
+    hipLaunchKernelGGL(col_reduce_kernel, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, d_colpart,
+                       col_blocks, C, gb);
+  };
   std::vector<int64_t> order(m);
   for (int64_t k = 0; k < m; ++k) order[k] = k;
   std::mt19937_64 rng(p->seed);
@@ -1301,27 +1320,21 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       if ((rc = transpose(st, demb, R, D3, dembT)) || (rc = transpose(st, a.a2, R, D2, a2T))) break;
       Epi g3; g3.c32 = enc->L[2].gw; g3.ldc = D2;
       if ((rc = gemm(st, EPI_F32, dembT, R, a2T, R, D3, D2, R, g3))) break;
-      if (hipMemsetAsync(enc->L[2].gb, 0, D3 * sizeof(float), st) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
-      hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((D3 + 63) / 64), (unsigned)((R + 127) / 128)), dim3(256), 0, st,
-                         demb, R, D3, enc->L[2].gb);
+      col_sums(demb, D3, enc->L[2].gb);
       Epi d2; d2.c16 = dp2; d2.ldc16 = D2; d2.pre = a.pre2; d2.ldp = D2;
       if ((rc = gemm(st, EPI_GELU_BWD, demb, D3, enc->L[2].w16, D3, R, D2, D3, d2))) break;
       // layer 2
       if ((rc = transpose(st, dp2, R, D2, dp2T)) || (rc = transpose(st, a.a1, R, D1, a1T))) break;
       Epi g2; g2.c32 = enc->L[1].gw; g2.ldc = D1;
       if ((rc = gemm(st, EPI_F32, dp2T, R, a1T, R, D2, D1, R, g2))) break;
-      if (hipMemsetAsync(enc->L[1].gb, 0, D2 * sizeof(float), st) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
-      hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((D2 + 63) / 64), (unsigned)((R + 127) / 128)), dim3(256), 0, st,
-                         dp2, R, D2, enc->L[1].gb);
+      col_sums(dp2, D2, enc->L[1].gb);
       Epi d1; d1.c16 = dp1; d1.ldc16 = D1; d1.pre = a.pre1; d1.ldp = D1;
       if ((rc = gemm(st, EPI_GELU_BWD, dp2, D2, enc->L[1].w16, D2, R, D1, D2, d1))) break;
       // layer 1
       if ((rc = transpose(st, dp1, R, D1, dp1T)) || (rc = transpose(st, Xb, R, D0, XbT))) break;
       Epi g1; g1.c32 = enc->L[0].gw; g1.ldc = D0;
       if ((rc = gemm(st, EPI_F32, dp1T, R, XbT, R, D1, D0, R, g1))) break;
-      if (hipMemsetAsync(enc->L[0].gb, 0, D1 * sizeof(float), st) != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
-      hipLaunchKernelGGL(col_sum_kernel, dim3((unsigned)((D1 + 63) / 64), (unsigned)((R + 127) / 128)), dim3(256), 0, st,
-                         dp1, R, D1, enc->L[0].gb);
+      col_sums(dp1, D1, enc->L[0].gb);
       // Adam
       enc->adam_t++;
       const float b1 = 0.9f, b2 = 0.999f;

@@ -98,6 +98,27 @@ def test_training_reduces_loss(pkg):
     assert de[fam[a] == fam[b]].mean() < 0.5 * de[fam[a] != fam[b]].mean()
 
 
+def test_training_is_deterministic(pkg, tmp_path):
+    """Two trainings from the same seed on the same pairs give byte-identical
+    weights (every gradient sum, the bias column sums included, runs in a
+    fixed order), so the prefilter's candidates repeat from run to run."""
+    seqs = synth.families_200bp(seed=43, n_anc=10, per_anc=40, length=160)
+    rng = np.random.default_rng(1)
+    n = len(seqs)
+    ij = rng.integers(0, n, size=(6000, 2)).astype(np.int32)
+    saved = []
+    with pkg.Engine(seqs) as eng:
+        d = eng.pairs(ij)
+        for k in range(2):
+            with pkg.Encoder(seed=5) as enc:
+                enc.train(eng, ij, d, epochs=3, batch=512, lr=2e-3, eps_train=300)
+                path = tmp_path / f"enc{k}.bin"
+                enc.save(str(path))
+                saved.append(path.read_bytes())
+    _say(f"weights {len(saved[0])} bytes")
+    assert saved[0] == saved[1]
+
+
 @pytest.mark.parametrize("exact_bucket", ["1", "0"])
 @pytest.mark.parametrize("tuning", [None, (3, 64, 8), (2, 16, 16)])
 def test_prefilter_with_unbounded_radius_equals_exact_lists(pkg, monkeypatch, tuning, exact_bucket):

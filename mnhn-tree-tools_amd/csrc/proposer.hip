@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void pair_loss_kernel(const float *__restrict_
   da[lane + 64] = (bf16)(s * e1);
   db[lane] = (bf16)(-s * e0);
   db[lane + 64] = (bf16)(-s * e1);
-  if (lane == 0) atomicAdd(loss, L);
+  if (lane == 0) loss[k] = L;  // per pair; the host adds them in order
 }
 
 // part[y][c] = sum over the rows of block row y (128 rows) of dY[r][c].
@@ -1268,7 +1268,7 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       (rc = dev_alloc(&dp1T, (size_t)R * D1)) || (rc = dev_alloc(&dp2, (size_t)R * D2)) ||
       (rc = dev_alloc(&dp2T, (size_t)R * D2)) || (rc = dev_alloc(&d_idx, (size_t)R)) ||
       (rc = dev_alloc(&d_tgt, (size_t)B)) || (rc = dev_alloc(&d_rad, (size_t)B)) ||
-      (rc = dev_alloc(&d_loss, 1)) ||
+      (rc = dev_alloc(&d_loss, (size_t)B)) ||
       (rc = dev_alloc(&d_colpart, (size_t)col_blocks * std::max(D1, std::max(D2, D3))))) {
     cleanup();
     return rc;
@@ -1284,7 +1284,7 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
   for (int64_t k = 0; k < m; ++k) order[k] = k;
   std::mt19937_64 rng(p->seed);
   std::vector<int32_t> h_idx(R);
-  std::vector<float> h_tgt(B), h_rad(B);
+  std::vector<float> h_tgt(B), h_rad(B), h_loss(B);
   double first_loss = 0, last_loss = 0;
   int64_t steps = 0;
   const int64_t steps_per_epoch = (m + B - 1) / B;
@@ -1308,7 +1308,6 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
       hipError_t e = hipMemcpyAsync(d_idx, h_idx.data(), R * 4, hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipMemcpyAsync(d_tgt, h_tgt.data(), B * 4, hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipMemcpyAsync(d_rad, h_rad.data(), B * 4, hipMemcpyHostToDevice, st);
-      if (e == hipSuccess) e = hipMemsetAsync(d_loss, 0, 4, st);
       if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
       hipLaunchKernelGGL(gather_rows_kernel, dim3(R), dim3(128), 0, st, X, d_idx, R, Xb);
       Acts b = a;
@@ -1351,12 +1350,11 @@ int mnhn_encoder_train(mnhn_encoder *enc, mnhn_sw_ctx *c, const int32_t *ij, con
         hipLaunchKernelGGL(refresh_weights_kernel, dim3(blocks_for(nw, 256)), dim3(256), 0, st, L.wt, L.out,
                            L.in, L.wt16, L.w16);
       }
-      float h_loss = 0;
       e = hipGetLastError();
-      if (e == hipSuccess) e = hipMemcpyAsync(&h_loss, d_loss, 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(h_loss.data(), d_loss, (size_t)B * 4, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) { rc = MNHN_ERR_DEVICE; break; }
-      ep_loss += h_loss;
+      for (int k = 0; k < B; ++k) ep_loss += h_loss[k];
       ep_pairs += B;
       ++steps;
     }

@@ -100,23 +100,26 @@ def test_training_reduces_loss(pkg):
 
 def test_training_is_deterministic(pkg, tmp_path):
     """Two trainings from the same seed on the same pairs give byte-identical
-    weights (every gradient sum, the bias column sums included, runs in a
-    fixed order), so the prefilter's candidates repeat from run to run."""
+    weights and the same reported losses (every sum, the bias column sums
+    included, runs in a fixed order), so the prefilter's candidates repeat
+    from run to run."""
     seqs = synth.families_200bp(seed=43, n_anc=10, per_anc=40, length=160)
     rng = np.random.default_rng(1)
     n = len(seqs)
     ij = rng.integers(0, n, size=(6000, 2)).astype(np.int32)
-    saved = []
+    saved, losses = [], []
     with pkg.Engine(seqs) as eng:
         d = eng.pairs(ij)
         for k in range(2):
             with pkg.Encoder(seed=5) as enc:
-                enc.train(eng, ij, d, epochs=3, batch=512, lr=2e-3, eps_train=300)
+                st = enc.train(eng, ij, d, epochs=3, batch=512, lr=2e-3, eps_train=300)
+                losses.append((st["first_loss"], st["final_loss"]))
                 path = tmp_path / f"enc{k}.bin"
                 enc.save(str(path))
                 saved.append(path.read_bytes())
-    _say(f"weights {len(saved[0])} bytes")
+    _say(f"weights {len(saved[0])} bytes, losses {losses}")
     assert saved[0] == saved[1]
+    assert losses[0] == losses[1]  # per-pair losses, added in order on the host
 
 
 @pytest.mark.parametrize("exact_bucket", ["1", "0"])
